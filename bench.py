@@ -1,0 +1,154 @@
+"""VBD benchmark: vertex updates per second of the plain VBD step on one MI355X.
+
+Workload: 64^3 cube grid, 6 tets per cell (274,625 vertices, 1,572,864 tets),
+interior vertices jittered +-0.1 cell, per-tet Young's modulus log-uniform in
+[1e5, 1e7], nu = 0.45, rho = 1000, gravity -y, bottom face fixed, h = 1/60,
+20 VBD iterations per step, colouring on the host. Inputs are synthetic and
+generated from a fixed seed. One "step" = one Integrator.step(h, 20, 1).
+
+value = vertex updates / s = (#non-Dirichlet vertices x iterations x steps) / time,
+with the mesh resident in HBM before the timed region starts.
+
+The JSON line also carries
+  roofline: algorithmic bytes of the sweep kernel per launch (60 B per updated
+            vertex + 120 B per vertex-tet incidence, FP32/int32) divided by the
+            kernel's average launch time measured with HIP events on the
+            integrator's stream, against 8 TB/s;
+  cpu_baseline: the FP64 CPU oracle (same mesh, colouring and iterations) on a
+            bounded sample (one step) using the host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_VERTEX = 60
+BYTES_PER_INCIDENCE = 120
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--iterations", type=int, default=20)
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the VBD path has no multi-GPU form in this build "
+              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from physicsbasedanimationtoolkit_amd import meshes, pbat
+
+    torch.cuda.set_device(0)
+    s = meshes.plain_gpu_scene(args.grid, seed=0)
+    dt, iters = s.dt, args.iterations
+    log(f"[bench] scene {s.name}: {s.n_vertices} vertices, {s.n_tets} tets")
+    t0 = time.time()
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+            .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).construct())
+    vbd = pbat.gpu.vbd.Integrator(data)
+    if args.lanes:
+        vbd.lanes_per_vertex = args.lanes
+    nColors, nUpd, nInc = vbd.counts()
+    log(f"[bench] construct+upload {time.time() - t0:.2f}s: {nColors} colours, {nUpd} updated vertices, "
+        f"{nInc} incidences")
+
+    for _ in range(args.warmup):
+        vbd.step(dt, iters, 1)
+    vbd.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        vbd.step(dt, iters, 1)
+    vbd.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    vu = nUpd * iters * args.steps
+    value = vu / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} vertex-updates/s")
+
+    # Sweep-kernel timing with HIP events on the integrator's stream (same steps, eager launches)
+    vbd.set_timing(True)
+    vbd.sweep_timing(reset=True)
+    tsteps = max(1, min(args.steps, 10))
+    for _ in range(tsteps):
+        vbd.step(dt, iters, 1)
+    ms, launches = vbd.sweep_timing(reset=True)
+    vbd.set_timing(False)
+    avg_launch_s = ms * 1e-3 / launches
+    bytes_per_sweep = BYTES_PER_VERTEX * nUpd + BYTES_PER_INCIDENCE * nInc
+    bytes_per_launch = bytes_per_sweep / nColors
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "vbd_sweep_64cube_traffic.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    log(f"[bench] sweep kernel: {launches} launches, avg {avg_launch_s * 1e6:.2f} us, "
+        f"{bytes_per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        ref = O.OracleVbd(s.X, s.E, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
+                          nthreads=threads)
+        t0 = time.perf_counter()
+        ref.step(dt, iters, 1)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": nUpd * iters / cpu_s, "unit": "vertex-updates/s", "cores": threads, "kind": "port",
+               "sample": f"1 step x {iters} iterations of the same 64^3 workload, FP64 oracle, "
+                         f"{threads} threads, {cpu_s:.2f}s"}
+        log(f"[bench] cpu baseline {cpu['value']:.4e} vu/s on {threads} threads")
+
+    out = {
+        "metric": "VBD vertex-updates/s (plain, 1 GPU, <=1e-5 rel err)",
+        "value": value,
+        "unit": "vertex-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain VBD, {args.grid}^3 jittered cube grid ({s.n_vertices} verts, "
+                               f"{s.n_tets} tets), SNH per-tet log-uniform Y, h=1/60, {iters} iterations, "
+                               f"bottom fixed", "colours": nColors, "updated_vertices": nUpd,
+                   "incidences": nInc, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "VbdSweep", "avg_launch_us": avg_launch_s * 1e6,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,187 @@
+/*
+ * pbat_vbd.h -- C ABI of the MI355X-native Vertex Block Descent library
+ * (libpbat_vbd.so).
+ *
+ * Drop-in for PBAT's GPU VBD path: pbat::gpu::vbd::Integrator
+ * (source/pbat/gpu/vbd/Integrator.h:33-150) constructed from a
+ * pbat::sim::vbd::Data (source/pbat/sim/vbd/Data.h:14-247). Each entry point
+ * names the reference member it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Host arrays are caller-owned and are copied
+ *    in or out during the call; nothing is retained across the boundary.
+ *  - Per-vertex 3-vectors are passed as 3 x n column-major matrices, i.e. the
+ *    layout of Eigen's Matrix<Scalar, 3, Dynamic>: value (d, i) at [3*i + d].
+ *    Elements are 4 x nT column-major: vertex k of element e at [4*e + k].
+ *  - Every function returns PBAT_VBD_OK (0) or a negative status; the message
+ *    of the last failure on the calling thread is pbat_vbd_last_error().
+ *    The reference raises std::invalid_argument / std::runtime_error in the
+ *    same situations (sim/vbd/Data.cpp:245-305, gpu/impl/common/Eigen.cuh).
+ *  - An integrator handle is not thread-safe. Its work runs on one HIP stream
+ *    owned by the handle; setters and getters synchronise that stream, so the
+ *    observable behaviour is host-synchronous as in the reference.
+ */
+#ifndef PBAT_VBD_H
+#define PBAT_VBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBAT_VBD_OK 0
+#define PBAT_VBD_ERROR_INVALID_ARGUMENT (-1) /* std::invalid_argument in the reference */
+#define PBAT_VBD_ERROR_HIP (-2)              /* HIP runtime failure */
+#define PBAT_VBD_ERROR_OUT_OF_MEMORY (-3)
+#define PBAT_VBD_ERROR_SINGULAR_ELEMENT (-4) /* fem/Jacobian.h:74-77 "Inverted or singular jacobian" */
+#define PBAT_VBD_ERROR_PALETTE (-5)          /* more than 128 colours (graph/Color.h NC) */
+#define PBAT_VBD_ERROR_NONFINITE (-6)        /* NaN/Inf positions detected on the device */
+
+/* sim/vbd/Enums.h:12-18 */
+#define PBAT_VBD_INIT_POSITION 0
+#define PBAT_VBD_INIT_INERTIA 1
+#define PBAT_VBD_INIT_KINETIC_ENERGY_MINIMUM 2
+#define PBAT_VBD_INIT_ADAPTIVE_VBD 3
+#define PBAT_VBD_INIT_ADAPTIVE_PBAT 4
+
+/* graph/Enums.h: EGreedyColorOrderingStrategy / EGreedyColorSelectionStrategy */
+#define PBAT_VBD_ORDER_NATURAL 0
+#define PBAT_VBD_ORDER_SMALLEST_DEGREE 1
+#define PBAT_VBD_ORDER_LARGEST_DEGREE 2
+#define PBAT_VBD_SELECT_LEAST_USED 0
+#define PBAT_VBD_SELECT_FIRST_AVAILABLE 1
+
+#define PBAT_VBD_MAX_COLORS 128
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char* pbat_vbd_last_error(void);
+
+/* Library version string. */
+const char* pbat_vbd_version(void);
+
+/* ------------------------------------------------------------------------- */
+/* Data construction                                                         */
+/* ------------------------------------------------------------------------- */
+
+/*
+ * Replaces pbat::sim::vbd::Data::Construct (sim/vbd/Data.cpp:179-244) for the
+ * element, adjacency and partition data; host-only, touches no GPU.
+ *
+ * Inputs: rest positions X (3 x nV), tetrahedra E (4 x nT), per-element mass
+ * density rhoe, Lame coefficients mue, lambdae (each nT; pass NULL for the
+ * reference defaults rho = 1e3, LameCoefficients(1e6, 0.45)), Dirichlet
+ * vertices dbc (nDbc, any order; NULL if none) and the colouring strategy.
+ *
+ * Outputs (caller-allocated, sizes in brackets):
+ *   m [nV] lumped masses; GP [12 nT] shape function gradients as the 4 x 3nT
+ *   matrix of the reference; wg [nT] quadrature weights; lame [2 nT];
+ *   GVGp [nV+1], GVGe [4 nT], GVGilocal [4 nT] vertex -> (element, local index)
+ *   CSR; colors [nV]; Pptr [PBAT_VBD_MAX_COLORS + 1]; Padj [nV].
+ *   *nPartitions receives the number of colours, *nPadj the number of
+ *   non-Dirichlet vertices listed in Padj.
+ */
+int pbat_vbd_data_construct(int64_t nV, int64_t nT, const double* X, const int64_t* E, const double* rhoe,
+                            const double* mue, const double* lambdae, const int64_t* dbc, int64_t nDbc,
+                            int ordering, int selection, double* m, double* GP, double* wg, double* lame,
+                            int64_t* GVGp, int64_t* GVGe, int64_t* GVGilocal, int64_t* colors, int64_t* Pptr,
+                            int64_t* Padj, int64_t* nPartitions, int64_t* nPadj);
+
+/* ------------------------------------------------------------------------- */
+/* GPU integrator                                                            */
+/* ------------------------------------------------------------------------- */
+
+/* Fields of pbat::sim::vbd::Data consumed by gpu::impl::vbd::Integrator's
+ * constructor (gpu/impl/vbd/Integrator.cu:22-80). All FP64/int64 host arrays,
+ * converted to the device's FP32/int32 layout on upload. */
+typedef struct pbat_vbd_data {
+    int64_t nV;
+    int64_t nT;
+    const double* x;         /* 3 x nV positions */
+    const double* v;         /* 3 x nV velocities */
+    const double* aext;      /* 3 x nV external accelerations */
+    const double* m;         /* nV lumped masses */
+    const int64_t* E;        /* 4 x nT */
+    const double* GP;        /* 4 x 3nT */
+    const double* wg;        /* nT */
+    const double* lame;      /* 2 x nT */
+    const int64_t* GVGp;     /* nV + 1 */
+    const int64_t* GVGe;     /* GVGp[nV] */
+    const int64_t* GVGilocal;/* GVGp[nV] */
+    int64_t nPartitions;
+    const int64_t* Pptr;     /* nPartitions + 1 */
+    const int64_t* Padj;     /* Pptr[nPartitions] */
+    int32_t strategy;        /* PBAT_VBD_INIT_* */
+    double kD;               /* Rayleigh damping coefficient */
+    double detHZero;         /* numerical zero of |det H| */
+} pbat_vbd_data;
+
+typedef struct pbat_vbd_integrator pbat_vbd_integrator;
+
+/* gpu::vbd::Integrator::Integrator(Data const&) (gpu/vbd/Integrator.cu:15-33).
+ * Uses the current HIP device. */
+int pbat_vbd_integrator_create(const pbat_vbd_data* data, pbat_vbd_integrator** out);
+/* gpu::vbd::Integrator::~Integrator (gpu/vbd/Integrator.cu:52-56) */
+int pbat_vbd_integrator_destroy(pbat_vbd_integrator* h);
+
+/* gpu::vbd::Integrator::Step (gpu/vbd/Integrator.h:72; gpu/impl/vbd/Integrator.cu:82-103) */
+int pbat_vbd_integrator_step(pbat_vbd_integrator* h, float dt, int32_t iterations, int32_t substeps);
+/* gpu::vbd::Integrator::TracedStep (gpu/vbd/Integrator.h:85-90; gpu/impl/vbd/Integrator.cu:105-148).
+ * Writes MatrixMarket files named {variable}.t.{t}.s.{s}[.k.{k}].mtx into dir. */
+int pbat_vbd_integrator_traced_step(pbat_vbd_integrator* h, float dt, int32_t iterations, int32_t substeps,
+                                    int32_t t, const char* dir);
+
+/* gpu::vbd::Integrator::SetPositions / SetVelocities / SetExternalAcceleration
+ * (gpu/vbd/Integrator.cu:73-91): 3 x n column-major FP32, n must equal nV. */
+int pbat_vbd_integrator_set_positions(pbat_vbd_integrator* h, const float* x, int64_t n);
+int pbat_vbd_integrator_set_velocities(pbat_vbd_integrator* h, const float* v, int64_t n);
+int pbat_vbd_integrator_set_external_acceleration(pbat_vbd_integrator* h, const float* a, int64_t n);
+/* gpu::vbd::Integrator::GetPositions / GetVelocities (gpu/vbd/Integrator.cu:117-125) */
+int pbat_vbd_integrator_get_positions(pbat_vbd_integrator* h, float* x, int64_t n);
+int pbat_vbd_integrator_get_velocities(pbat_vbd_integrator* h, float* v, int64_t n);
+
+/* gpu::vbd::Integrator::SetNumericalZeroForHessianDeterminant (gpu/vbd/Integrator.cu:93-96) */
+int pbat_vbd_integrator_set_detH_zero(pbat_vbd_integrator* h, float zero);
+/* gpu::vbd::Integrator::SetRayleighDampingCoefficient (gpu/vbd/Integrator.cu:98-101) */
+int pbat_vbd_integrator_set_rayleigh_damping(pbat_vbd_integrator* h, float kD);
+/* gpu::vbd::Integrator::SetInitializationStrategy (gpu/vbd/Integrator.cu:103-106) */
+int pbat_vbd_integrator_set_initialization_strategy(pbat_vbd_integrator* h, int32_t strategy);
+/* gpu::vbd::Integrator::SetBlockSize (gpu/vbd/Integrator.cu:108-111; clamp as gpu/impl/vbd/Integrator.cu:158-161).
+ * Threads per workgroup of the sweep kernel; clamped to [64, 256] and rounded down to a multiple of 64. */
+int pbat_vbd_integrator_set_block_size(pbat_vbd_integrator* h, int32_t block_size);
+/* gpu::vbd::Integrator::SetSceneBoundingBox (gpu/vbd/Integrator.cu:113-116). Stored only: the
+ * bounding box feeds the contact detector, which single-body VBD does not use. */
+int pbat_vbd_integrator_set_scene_bounding_box(pbat_vbd_integrator* h, const float min[3], const float max[3]);
+
+/* Lanes of a wavefront that cooperate on one vertex (1, 2, 4, 8, 16, 32 or 64;
+ * 0 = choose from the mesh's maximum vertex degree). No reference counterpart:
+ * the reference always uses one thread block per vertex. */
+int pbat_vbd_integrator_set_lanes_per_vertex(pbat_vbd_integrator* h, int32_t lanes);
+/* Capture each Step's launch sequence in a hipGraph and replay it (default on). */
+int pbat_vbd_integrator_set_graph_capture(pbat_vbd_integrator* h, int32_t enabled);
+
+/* Total mechanical energy terms reduced in FP64 on the device:
+ * out = {1/2 sum m |v|^2, sum_e w_e Psi(F_e), -sum m aext . x}.
+ * Counterpart of the FP64 objective reductions of
+ * gpu/impl/vbd/TrustRegionIntegrator.cu:265-364 and sim/vbd/Integrator.cpp:138-166. */
+int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3]);
+
+/* Wait for all work queued on the handle's stream. */
+int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);
+
+/* Number of colours (vertex partitions) and the number of vertices updated per sweep. */
+int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPartitions, int64_t* nUpdated,
+                               int64_t* nIncidences);
+
+/* Kernel-time instrumentation: when enabled, every sweep launch is bracketed by
+ * HIP events on the handle's stream and the summed event time is kept.
+ * pbat_vbd_integrator_sweep_timing returns (total ms, number of sweep launches)
+ * since the last reset. No reference counterpart. */
+int pbat_vbd_integrator_set_timing(pbat_vbd_integrator* h, int32_t enabled);
+int pbat_vbd_integrator_sweep_timing(pbat_vbd_integrator* h, double* total_ms, int64_t* launches, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBAT_VBD_H */

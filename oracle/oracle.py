@@ -1,0 +1,327 @@
+"""FP64 CPU oracle for the VBD path -- TEST INFRASTRUCTURE ONLY.
+
+Python front end of ``libvbd_oracle.so`` (``vbd_oracle.c``), the checker the
+MI355X product is compared against. Only ``tests/``, ``__graft_entry__.smoke``
+and the CPU-baseline leg of ``bench.py`` may import this module; the product
+package never does.
+
+``OracleVbd`` restates ``pbat::sim::vbd::Data::Construct``
+(sim/vbd/Data.cpp:179-244) and ``pbat::sim::vbd::Integrator``
+(sim/vbd/Integrator.cpp:22-200). Arrays use the reference's shapes at the
+interface (3 x nV positions, 4 x nT elements) and are stored vertex-major
+(nV x 3, C order) internally, which is the memory layout of an Eigen 3 x nV
+column-major matrix.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libvbd_oracle.so")
+
+# sim/vbd/Enums.h:12-18
+INIT_POSITION = 0
+INIT_INERTIA = 1
+INIT_KINETIC_ENERGY_MINIMUM = 2
+INIT_ADAPTIVE_VBD = 3
+INIT_ADAPTIVE_PBAT = 4
+# graph/Enums.h
+SELECT_LEAST_USED = 0
+SELECT_FIRST_AVAILABLE = 1
+ORDER_NATURAL = 0
+ORDER_SMALLEST_DEGREE = 1
+ORDER_LARGEST_DEGREE = 2
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle shared library with gcc (see oracle/Makefile)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "all", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        i64 = ctypes.c_int64
+        L.ora_lame.argtypes = [ctypes.c_double, ctypes.c_double, dp, dp]
+        L.ora_snh_eval.argtypes = [dp, ctypes.c_double, ctypes.c_double]
+        L.ora_snh_eval.restype = ctypes.c_double
+        L.ora_snh_grad.argtypes = [dp, ctypes.c_double, ctypes.c_double, dp]
+        L.ora_snh_hessian.argtypes = [dp, ctypes.c_double, ctypes.c_double, dp]
+        for f in ("ora_shape_function_gradients", "ora_quadrature_weights"):
+            getattr(L, f).argtypes = [i64, i64, dp, ip, dp]
+            getattr(L, f).restype = ctypes.c_int
+        L.ora_lumped_mass.argtypes = [i64, i64, dp, ip, dp, dp]
+        L.ora_lumped_mass.restype = ctypes.c_int
+        L.ora_vertex_tet_adjacency.argtypes = [i64, i64, ip, ip, ip, ip]
+        L.ora_greedy_color.argtypes = [i64, i64, ip, ctypes.c_int, ctypes.c_int, ip]
+        L.ora_greedy_color.restype = i64
+        L.ora_map_to_adjacency.argtypes = [i64, ip, ip, ip]
+        L.ora_map_to_adjacency.restype = i64
+        L.ora_remove_vertices.argtypes = [i64, ip, ip, i64, i64, ip]
+        L.ora_remove_vertices.restype = i64
+        L.ora_step.argtypes = [ctypes.c_void_p, ctypes.c_double, i64, i64]
+        L.ora_step.restype = ctypes.c_int
+        L.ora_vertex_elastic_block.argtypes = [ctypes.c_void_p, i64, dp, dp]
+        L.ora_objective.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double]
+        L.ora_objective.restype = ctypes.c_double
+        L.ora_objective_gradient.argtypes = [ctypes.c_void_p, dp, dp, ctypes.c_double, dp]
+        L.ora_energy.argtypes = [ctypes.c_void_p, dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _ip(a: np.ndarray):
+    assert a.dtype == np.int64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+class _State(ctypes.Structure):
+    _fields_ = [
+        ("nV", ctypes.c_int64), ("nT", ctypes.c_int64),
+        ("x", ctypes.c_void_p), ("v", ctypes.c_void_p), ("aext", ctypes.c_void_p),
+        ("m", ctypes.c_void_p), ("xt", ctypes.c_void_p), ("vt", ctypes.c_void_p),
+        ("xtilde", ctypes.c_void_p),
+        ("E", ctypes.c_void_p), ("GP", ctypes.c_void_p), ("wg", ctypes.c_void_p),
+        ("lame", ctypes.c_void_p),
+        ("GVGp", ctypes.c_void_p), ("GVGe", ctypes.c_void_p), ("GVGilocal", ctypes.c_void_p),
+        ("nPartitions", ctypes.c_int64), ("Pptr", ctypes.c_void_p), ("Padj", ctypes.c_void_p),
+        ("strategy", ctypes.c_int), ("kD", ctypes.c_double), ("detHZero", ctypes.c_double),
+        ("nthreads", ctypes.c_int), ("gpu_velocity_semantics", ctypes.c_int),
+    ]
+
+
+# ---- constitutive model ---------------------------------------------------
+
+def lame(Y: float, nu: float):
+    mu, la = ctypes.c_double(), ctypes.c_double()
+    lib().ora_lame(Y, nu, ctypes.byref(mu), ctypes.byref(la))
+    return mu.value, la.value
+
+
+def snh_eval(F: np.ndarray, mu: float, lam: float) -> float:
+    """F is 3x3; flattened column-major as the reference does."""
+    f = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T.reshape(9))
+    return lib().ora_snh_eval(_dp(f), mu, lam)
+
+
+def snh_grad(F, mu, lam) -> np.ndarray:
+    f = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T.reshape(9))
+    g = np.zeros(9)
+    lib().ora_snh_grad(_dp(f), mu, lam, _dp(g))
+    return g
+
+
+def snh_hessian(F, mu, lam) -> np.ndarray:
+    f = np.ascontiguousarray(np.asarray(F, dtype=np.float64).T.reshape(9))
+    H = np.zeros(81)
+    lib().ora_snh_hessian(_dp(f), mu, lam, _dp(H))
+    return H.reshape(9, 9).T.copy()
+
+
+# ---- precompute pieces ----------------------------------------------------
+
+class SingularJacobian(ValueError):
+    pass
+
+
+def _prep(X, E):
+    Xv = np.ascontiguousarray(np.asarray(X, dtype=np.float64).T)  # nV x 3
+    Ev = np.ascontiguousarray(np.asarray(E, dtype=np.int64).T)    # nT x 4
+    return Xv, Ev
+
+
+def shape_function_gradients(X, E) -> np.ndarray:
+    """Returns GP as the reference's 4 x 3nT matrix."""
+    Xv, Ev = _prep(X, E)
+    nV, nT = Xv.shape[0], Ev.shape[0]
+    GP = np.zeros(12 * nT)
+    if lib().ora_shape_function_gradients(nV, nT, _dp(Xv), _ip(Ev), _dp(GP)) != 0:
+        raise SingularJacobian("Inverted or singular jacobian")
+    return GP.reshape(3 * nT, 4).T.copy()
+
+
+def quadrature_weights(X, E) -> np.ndarray:
+    Xv, Ev = _prep(X, E)
+    wg = np.zeros(Ev.shape[0])
+    if lib().ora_quadrature_weights(Xv.shape[0], Ev.shape[0], _dp(Xv), _ip(Ev), _dp(wg)) != 0:
+        raise SingularJacobian("Inverted or singular jacobian")
+    return wg
+
+
+def lumped_mass(X, E, rhoe) -> np.ndarray:
+    Xv, Ev = _prep(X, E)
+    rho = np.ascontiguousarray(np.asarray(rhoe, dtype=np.float64))
+    m = np.zeros(Xv.shape[0])
+    if lib().ora_lumped_mass(Xv.shape[0], Ev.shape[0], _dp(Xv), _ip(Ev), _dp(rho), _dp(m)) != 0:
+        raise SingularJacobian("Inverted or singular jacobian")
+    return m
+
+
+def vertex_tet_adjacency(E, nV):
+    Ev = np.ascontiguousarray(np.asarray(E, dtype=np.int64).T)
+    nT = Ev.shape[0]
+    p = np.zeros(nV + 1, dtype=np.int64)
+    e = np.zeros(4 * nT, dtype=np.int64)
+    il = np.zeros(4 * nT, dtype=np.int64)
+    lib().ora_vertex_tet_adjacency(nV, nT, _ip(Ev), _ip(p), _ip(e), _ip(il))
+    return p, e, il
+
+
+def greedy_color(E, nV, ordering=ORDER_LARGEST_DEGREE, selection=SELECT_LEAST_USED):
+    Ev = np.ascontiguousarray(np.asarray(E, dtype=np.int64).T)
+    C = np.zeros(nV, dtype=np.int64)
+    nc = lib().ora_greedy_color(nV, Ev.shape[0], _ip(Ev), int(ordering), int(selection), _ip(C))
+    if nc < 0:
+        raise RuntimeError(f"greedy colouring failed ({nc})")
+    return C
+
+
+def map_to_adjacency(colors):
+    c = np.ascontiguousarray(np.asarray(colors, dtype=np.int64))
+    nc = int(c.max()) + 1 if c.size else 0
+    Pptr = np.zeros(nc + 1, dtype=np.int64)
+    Padj = np.zeros(c.size, dtype=np.int64)
+    lib().ora_map_to_adjacency(c.size, _ip(c), _ip(Pptr), _ip(Padj))
+    return Pptr, Padj
+
+
+def remove_vertices(Pptr, Padj, nV, remove):
+    Pptr = np.ascontiguousarray(Pptr, dtype=np.int64).copy()
+    Padj = np.ascontiguousarray(Padj, dtype=np.int64).copy()
+    r = np.ascontiguousarray(np.asarray(remove, dtype=np.int64))
+    n = lib().ora_remove_vertices(Pptr.size - 1, _ip(Pptr), _ip(Padj), nV, r.size, _ip(r))
+    return Pptr, Padj[:n].copy()
+
+
+# ---- the integrator ---------------------------------------------------------
+
+class OracleVbd:
+    """``Data().with_*(...).construct()`` followed by ``Integrator(data)``.
+
+    Restates sim/vbd/Data.cpp:179-244 (defaults: aext = (0,0,-9.81),
+    Y = 1e6 / nu = 0.45 Lame, rho = 1e3, AdaptivePbat initialisation,
+    detHZero = 1e-7, kD = 0; sim/vbd/Data.h:205-246) and
+    sim/vbd/Integrator.cpp:22-200.
+    """
+
+    def __init__(self, X, E, *, v=None, aext=None, rhoe=None, mue=None, lambdae=None,
+                 dbc=None, ordering=ORDER_LARGEST_DEGREE, selection=SELECT_LEAST_USED,
+                 strategy=INIT_ADAPTIVE_PBAT, kD=0.0, detHZero=1e-7, nthreads=1,
+                 gpu_velocity_semantics=False, colors=None):
+        Xv, Ev = _prep(X, E)
+        nV, nT = Xv.shape[0], Ev.shape[0]
+        self.nV, self.nT = nV, nT
+        self.X = Xv.copy()
+        self.E = Ev
+        self.x = Xv.copy()
+        self.xt = Xv.copy()
+        self.v = np.zeros((nV, 3)) if v is None else np.ascontiguousarray(np.asarray(v, np.float64).T).copy()
+        if aext is None:
+            self.aext = np.tile(np.array([0.0, 0.0, -9.81]), (nV, 1))
+        else:
+            self.aext = np.ascontiguousarray(np.asarray(aext, np.float64).T).copy()
+        self.xtilde = np.zeros((nV, 3))
+        self.vt = np.zeros((nV, 3))
+        if mue is None or lambdae is None:
+            mu, la = lame(1e6, 0.45)
+            mue, lambdae = np.full(nT, mu), np.full(nT, la)
+        self.lame = np.ascontiguousarray(np.stack([np.asarray(mue, np.float64),
+                                                   np.asarray(lambdae, np.float64)], axis=1))
+        self.rhoe = np.full(nT, 1e3) if rhoe is None else np.asarray(rhoe, np.float64).copy()
+        self.m = lumped_mass(X, E, self.rhoe)
+        self.GP = np.ascontiguousarray(shape_function_gradients(X, E).T.reshape(-1))  # 12 per tet
+        self.wg = quadrature_weights(X, E)
+        self.GVGp, self.GVGe, self.GVGilocal = vertex_tet_adjacency(E, nV)
+        if colors is None:
+            colors = greedy_color(E, nV, ordering, selection)
+        self.colors = np.asarray(colors, dtype=np.int64)
+        Pptr, Padj = map_to_adjacency(self.colors)
+        self.dbc = np.sort(np.asarray([] if dbc is None else dbc, dtype=np.int64))
+        self.v[self.dbc] = 0.0
+        self.aext[self.dbc] = 0.0
+        self.Pptr, self.Padj = remove_vertices(Pptr, Padj, nV, self.dbc)
+        self.strategy = int(strategy)
+        self.kD = float(kD)
+        self.detHZero = float(detHZero)
+        self.nthreads = int(nthreads)
+        self.gpu_velocity_semantics = bool(gpu_velocity_semantics)
+
+    def _state(self) -> _State:
+        s = _State()
+        s.nV, s.nT = self.nV, self.nT
+        for name in ("x", "v", "aext", "m", "xt", "vt", "xtilde", "GP", "wg", "lame"):
+            a = getattr(self, name)
+            assert a.dtype == np.float64 and a.flags.c_contiguous, name
+            setattr(s, name, a.ctypes.data)
+        for name in ("E", "GVGp", "GVGe", "GVGilocal", "Pptr", "Padj"):
+            a = getattr(self, name)
+            assert a.dtype == np.int64 and a.flags.c_contiguous, name
+            setattr(s, name, a.ctypes.data)
+        s.nPartitions = self.Pptr.size - 1
+        s.strategy, s.kD, s.detHZero = self.strategy, self.kD, self.detHZero
+        s.nthreads = self.nthreads
+        s.gpu_velocity_semantics = int(self.gpu_velocity_semantics)
+        return s
+
+    # sim/vbd/Integrator.cpp:22-45
+    def step(self, dt: float, iterations: int, substeps: int = 1) -> None:
+        s = self._state()
+        rc = lib().ora_step(ctypes.byref(s), float(dt), int(iterations), int(substeps))
+        if rc != 0:
+            raise ValueError(f"oracle step failed ({rc})")
+
+    def objective(self, x, xtilde, dt) -> float:
+        s = self._state()
+        xv = np.ascontiguousarray(np.asarray(x, np.float64).T)
+        xl = np.ascontiguousarray(np.asarray(xtilde, np.float64).T)
+        return lib().ora_objective(ctypes.byref(s), _dp(xv), _dp(xl), float(dt))
+
+    def objective_gradient(self, x, xtilde, dt) -> np.ndarray:
+        """Gradient in the reference's reshaped (3*nV,) vertex-major order."""
+        s = self._state()
+        xv = np.ascontiguousarray(np.asarray(x, np.float64).T)
+        xl = np.ascontiguousarray(np.asarray(xtilde, np.float64).T)
+        g = np.zeros(3 * self.nV)
+        lib().ora_objective_gradient(ctypes.byref(s), _dp(xv), _dp(xl), float(dt), _dp(g))
+        return g
+
+    def vertex_elastic_block(self, i: int):
+        s = self._state()
+        H = np.zeros(9)
+        g = np.zeros(3)
+        lib().ora_vertex_elastic_block(ctypes.byref(s), int(i), _dp(H), _dp(g))
+        return H.reshape(3, 3).T.copy(), g
+
+    def energy(self) -> np.ndarray:
+        """(kinetic, elastic, external potential) in FP64."""
+        s = self._state()
+        out = np.zeros(3)
+        lib().ora_energy(ctypes.byref(s), _dp(out))
+        return out
+
+    # reference-shaped views (3 x nV)
+    @property
+    def positions(self) -> np.ndarray:
+        return self.x.T
+
+    @property
+    def velocities(self) -> np.ndarray:
+        return self.v.T

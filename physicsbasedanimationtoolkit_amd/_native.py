@@ -1,0 +1,118 @@
+"""ctypes binding of ``libpbat_vbd.so``, the C ABI declared in ``include/pbat_vbd.h``.
+
+The library holds the HIP kernels for gfx950 and the host code around them.
+There is no fallback: if the library is missing or cannot be loaded, every
+entry point of this package raises ``ImportError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libpbat_vbd.so")
+
+OK = 0
+ERROR_INVALID_ARGUMENT = -1
+ERROR_HIP = -2
+ERROR_OUT_OF_MEMORY = -3
+ERROR_SINGULAR_ELEMENT = -4
+ERROR_PALETTE = -5
+ERROR_NONFINITE = -6
+
+MAX_COLORS = 128
+
+_lib = None
+
+c_i64 = ctypes.c_int64
+c_i32 = ctypes.c_int32
+c_f32 = ctypes.c_float
+c_f64 = ctypes.c_double
+p_f64 = ctypes.POINTER(ctypes.c_double)
+p_f32 = ctypes.POINTER(ctypes.c_float)
+p_i64 = ctypes.POINTER(ctypes.c_int64)
+
+
+class Data(ctypes.Structure):
+    """``pbat_vbd_data`` of include/pbat_vbd.h."""
+
+    _fields_ = [
+        ("nV", c_i64), ("nT", c_i64),
+        ("x", p_f64), ("v", p_f64), ("aext", p_f64), ("m", p_f64),
+        ("E", p_i64), ("GP", p_f64), ("wg", p_f64), ("lame", p_f64),
+        ("GVGp", p_i64), ("GVGe", p_i64), ("GVGilocal", p_i64),
+        ("nPartitions", c_i64), ("Pptr", p_i64), ("Padj", p_i64),
+        ("strategy", c_i32), ("kD", c_f64), ("detHZero", c_f64),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point
+SIGNATURES = [
+    ("pbat_vbd_last_error", ctypes.c_char_p, []),
+    ("pbat_vbd_version", ctypes.c_char_p, []),
+    ("pbat_vbd_data_construct", ctypes.c_int,
+     [c_i64, c_i64, p_f64, p_i64, p_f64, p_f64, p_f64, p_i64, c_i64, ctypes.c_int, ctypes.c_int,
+      p_f64, p_f64, p_f64, p_f64, p_i64, p_i64, p_i64, p_i64, p_i64, p_i64, p_i64, p_i64]),
+    ("pbat_vbd_integrator_create", ctypes.c_int, [ctypes.POINTER(Data), ctypes.POINTER(ctypes.c_void_p)]),
+    ("pbat_vbd_integrator_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("pbat_vbd_integrator_step", ctypes.c_int, [ctypes.c_void_p, c_f32, c_i32, c_i32]),
+    ("pbat_vbd_integrator_traced_step", ctypes.c_int,
+     [ctypes.c_void_p, c_f32, c_i32, c_i32, c_i32, ctypes.c_char_p]),
+    ("pbat_vbd_integrator_set_positions", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_integrator_set_velocities", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_integrator_set_external_acceleration", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_integrator_get_positions", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_integrator_get_velocities", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_integrator_set_detH_zero", ctypes.c_int, [ctypes.c_void_p, c_f32]),
+    ("pbat_vbd_integrator_set_rayleigh_damping", ctypes.c_int, [ctypes.c_void_p, c_f32]),
+    ("pbat_vbd_integrator_set_initialization_strategy", ctypes.c_int, [ctypes.c_void_p, c_i32]),
+    ("pbat_vbd_integrator_set_block_size", ctypes.c_int, [ctypes.c_void_p, c_i32]),
+    ("pbat_vbd_integrator_set_scene_bounding_box", ctypes.c_int, [ctypes.c_void_p, p_f32, p_f32]),
+    ("pbat_vbd_integrator_set_lanes_per_vertex", ctypes.c_int, [ctypes.c_void_p, c_i32]),
+    ("pbat_vbd_integrator_set_graph_capture", ctypes.c_int, [ctypes.c_void_p, c_i32]),
+    ("pbat_vbd_integrator_energy", ctypes.c_int, [ctypes.c_void_p, p_f64]),
+    ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("pbat_vbd_integrator_counts", ctypes.c_int, [ctypes.c_void_p, p_i64, p_i64, p_i64]),
+    ("pbat_vbd_integrator_set_timing", ctypes.c_int, [ctypes.c_void_p, c_i32]),
+    ("pbat_vbd_integrator_sweep_timing", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(c_f64), p_i64, c_i32]),
+]
+
+
+def lib():
+    """Load libpbat_vbd.so (once) and declare its signatures."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise ImportError(
+                f"{LIBRARY_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "or `make -C physicsbasedanimationtoolkit_amd/csrc`")
+        L = ctypes.CDLL(LIBRARY_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def check(rc: int) -> None:
+    """Raise the Python counterpart of a C ABI status code."""
+    if rc == OK:
+        return
+    msg = lib().pbat_vbd_last_error().decode(errors="replace")
+    if rc in (ERROR_INVALID_ARGUMENT, ERROR_SINGULAR_ELEMENT):
+        raise ValueError(msg)
+    if rc == ERROR_OUT_OF_MEMORY:
+        raise MemoryError(msg)
+    if rc == ERROR_HIP:
+        raise HipError(msg)
+    raise RuntimeError(f"{msg} (status {rc})")
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,204 @@
+"""``pbat.gpu.vbd`` counterpart: the VBD integrator on an MI355X.
+
+Mirrors ``pbat.gpu.vbd.Integrator`` (bindings/pypbat/gpu/vbd/Integrator.cpp:19-105)
+over ``pbat::gpu::vbd::Integrator`` (gpu/vbd/Integrator.h:33-150): same
+constructor, ``step``/``traced_step`` signatures and defaults, and the same
+properties (``x``, ``v`` read-write; ``a``, ``detH_residual``, ``kD``,
+``strategy``, ``gpu_block_size``, ``scene_bounding_box`` write-only). All work
+runs in hand-written HIP kernels of libpbat_vbd.so, called through its C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .. import _native
+from ..sim.vbd import AccelerationStrategy, Data, InitializationStrategy
+
+
+def _vertex_matrix_f32(a, n: int, name: str) -> np.ndarray:
+    a = np.asarray(a)
+    if a.ndim != 2 or a.shape[0] != 3:
+        raise ValueError(f"Expected input dimensions 3x{n}, but got {'x'.join(map(str, a.shape))} for {name}")
+    # column-major 3 x n == row-major n x 3
+    return np.ascontiguousarray(a.T, dtype=np.float32)
+
+
+def _write_only(name):
+    def getter(self):
+        raise AttributeError(f"'{name}' is write-only")
+    return getter
+
+
+class Integrator:
+    """VBD time integrator running on the current HIP device."""
+
+    def __init__(self, data: Data):
+        if not isinstance(data, Data):
+            raise TypeError("Integrator expects a pbat.sim.vbd.Data")
+        if data.accelerator != AccelerationStrategy.Base:
+            raise ValueError(
+                f"acceleration strategy {data.accelerator.name} is not available on this device path; "
+                "only the base VBD integrator is implemented")
+        if data.m.size == 0:
+            raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
+        self._lib = _native.lib()
+        nV, nT = data.x.shape[1], data.E.shape[1]
+        self._nV = nV
+        keep = {
+            "x": np.ascontiguousarray(data.x.T, dtype=np.float64),
+            "v": np.ascontiguousarray(data.v.T, dtype=np.float64),
+            "aext": np.ascontiguousarray(data.aext.T, dtype=np.float64),
+            "m": np.ascontiguousarray(data.m, dtype=np.float64),
+            "E": np.ascontiguousarray(data.E.T, dtype=np.int64),
+            "GP": np.ascontiguousarray(data.GP.T, dtype=np.float64),
+            "wg": np.ascontiguousarray(data.wg, dtype=np.float64),
+            "lame": np.ascontiguousarray(data.lame.T, dtype=np.float64),
+            "GVGp": np.ascontiguousarray(data.GVGp, dtype=np.int64),
+            "GVGe": np.ascontiguousarray(data.GVGe, dtype=np.int64),
+            "GVGilocal": np.ascontiguousarray(data.GVGilocal, dtype=np.int64),
+            "Pptr": np.ascontiguousarray(data.Pptr, dtype=np.int64),
+            "Padj": np.ascontiguousarray(data.Padj, dtype=np.int64),
+        }
+        for k in ("x", "v", "aext"):
+            if keep[k].shape != (nV, 3):
+                raise ValueError(f"data.{k} must be 3x{nV}")
+        d = _native.Data()
+        d.nV, d.nT = nV, nT
+        f64, i64 = ctypes.c_double, ctypes.c_int64
+        for k in ("x", "v", "aext", "m", "GP", "wg", "lame"):
+            setattr(d, k, _native.ptr(keep[k], f64))
+        for k in ("E", "GVGp", "GVGe", "GVGilocal", "Pptr", "Padj"):
+            setattr(d, k, _native.ptr(keep[k], i64))
+        d.nPartitions = max(keep["Pptr"].size - 1, 0)
+        d.strategy = int(data.strategy)
+        d.kD = float(data.kD)
+        d.detHZero = float(data.detH_zero)
+        handle = ctypes.c_void_p()
+        _native.check(self._lib.pbat_vbd_integrator_create(ctypes.byref(d), ctypes.byref(handle)))
+        self._h = handle
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.pbat_vbd_integrator_destroy(h)
+            self._h = None
+
+    # ---- stepping -------------------------------------------------------------
+    def step(self, dt: float = 0.01, iterations: int = 20, substeps: int = 1) -> None:
+        """Integrate 1 time step (gpu/vbd/Integrator.cu:58-61)."""
+        _native.check(self._lib.pbat_vbd_integrator_step(self._h, float(dt), int(iterations), int(substeps)))
+
+    def traced_step(self, dt: float = 0.01, iterations: int = 20, substeps: int = 1, t: int = 0,
+                    dir: str = ".") -> None:
+        """Integrate 1 time step and write {variable}.t.{t}.s.{s}[.k.{k}].mtx files into dir."""
+        os.makedirs(dir, exist_ok=True)
+        _native.check(self._lib.pbat_vbd_integrator_traced_step(
+            self._h, float(dt), int(iterations), int(substeps), int(t), str(dir).encode()))
+
+    # ---- state ----------------------------------------------------------------
+    def _get(self, fn) -> np.ndarray:
+        out = np.empty((self._nV, 3), dtype=np.float32)
+        _native.check(fn(self._h, _native.ptr(out, ctypes.c_float), self._nV))
+        return out.T
+
+    def _set(self, fn, value, name):
+        a = _vertex_matrix_f32(value, self._nV, name)
+        _native.check(fn(self._h, _native.ptr(a, ctypes.c_float), a.shape[0]))
+
+    @property
+    def x(self) -> np.ndarray:
+        """|#dims|x|#vertices| vertex positions"""
+        return self._get(self._lib.pbat_vbd_integrator_get_positions)
+
+    @x.setter
+    def x(self, value):
+        self._set(self._lib.pbat_vbd_integrator_set_positions, value, "positions")
+
+    @property
+    def v(self) -> np.ndarray:
+        """|#dims|x|#vertices| vertex velocities"""
+        return self._get(self._lib.pbat_vbd_integrator_get_velocities)
+
+    @v.setter
+    def v(self, value):
+        self._set(self._lib.pbat_vbd_integrator_set_velocities, value, "velocities")
+
+    def _set_a(self, value):
+        self._set(self._lib.pbat_vbd_integrator_set_external_acceleration, value, "external acceleration")
+
+    a = property(_write_only("a"), _set_a, doc="|#dims|x|#vertices| vertex external accelerations")
+
+    def _set_detH(self, value):
+        _native.check(self._lib.pbat_vbd_integrator_set_detH_zero(self._h, float(value)))
+
+    detH_residual = property(_write_only("detH_residual"), _set_detH,
+                             doc="Numerical zero used in the Hessian determinant check")
+
+    def _set_kD(self, value):
+        _native.check(self._lib.pbat_vbd_integrator_set_rayleigh_damping(self._h, float(value)))
+
+    kD = property(_write_only("kD"), _set_kD, doc="Uniform Rayleigh damping coefficient on the mesh")
+
+    def _set_strategy(self, value):
+        _native.check(self._lib.pbat_vbd_integrator_set_initialization_strategy(
+            self._h, int(InitializationStrategy(int(value)))))
+
+    strategy = property(_write_only("strategy"), _set_strategy,
+                        doc="VBD's time step minimization initialization strategy")
+
+    def _set_block(self, value):
+        _native.check(self._lib.pbat_vbd_integrator_set_block_size(self._h, int(value)))
+
+    gpu_block_size = property(_write_only("gpu_block_size"), _set_block,
+                              doc="Number of threads per GPU workgroup of the sweep kernel")
+
+    def _set_box(self, box):
+        lo = np.ascontiguousarray(np.asarray(box[0], dtype=np.float32).reshape(3))
+        hi = np.ascontiguousarray(np.asarray(box[1], dtype=np.float32).reshape(3))
+        _native.check(self._lib.pbat_vbd_integrator_set_scene_bounding_box(
+            self._h, _native.ptr(lo, ctypes.c_float), _native.ptr(hi, ctypes.c_float)))
+
+    scene_bounding_box = property(_write_only("scene_bounding_box"), _set_box,
+                                  doc="Tuple of (min,max) scene bounding box extremities")
+
+    # ---- extensions (no reference counterpart) ----------------------------------
+    def _set_lanes(self, value):
+        _native.check(self._lib.pbat_vbd_integrator_set_lanes_per_vertex(self._h, int(value)))
+
+    lanes_per_vertex = property(_write_only("lanes_per_vertex"), _set_lanes,
+                                doc="Lanes cooperating on one vertex in the sweep kernel (0 = automatic)")
+
+    def _set_graph(self, value):
+        _native.check(self._lib.pbat_vbd_integrator_set_graph_capture(self._h, int(bool(value))))
+
+    graph_capture = property(_write_only("graph_capture"), _set_graph,
+                             doc="Replay each step's launches from a captured hipGraph")
+
+    def energy(self) -> np.ndarray:
+        """(kinetic, elastic, external potential) energy terms reduced in FP64 on the device."""
+        out = np.zeros(3)
+        _native.check(self._lib.pbat_vbd_integrator_energy(self._h, _native.ptr(out, ctypes.c_double)))
+        return out
+
+    def counts(self):
+        """(#colours, #updated vertices, #vertex-tet incidences of updated vertices)"""
+        p, u, i = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _native.check(self._lib.pbat_vbd_integrator_counts(self._h, ctypes.byref(p), ctypes.byref(u),
+                                                           ctypes.byref(i)))
+        return p.value, u.value, i.value
+
+    def synchronize(self) -> None:
+        _native.check(self._lib.pbat_vbd_integrator_synchronize(self._h))
+
+    def set_timing(self, enabled: bool) -> None:
+        _native.check(self._lib.pbat_vbd_integrator_set_timing(self._h, int(bool(enabled))))
+
+    def sweep_timing(self, reset: bool = False):
+        """(summed HIP-event milliseconds of sweep launches, number of launches)"""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        _native.check(self._lib.pbat_vbd_integrator_sweep_timing(self._h, ctypes.byref(ms), ctypes.byref(n),
+                                                                 int(bool(reset))))
+        return ms.value, n.value
