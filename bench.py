@@ -45,9 +45,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=64)
     ap.add_argument("--iterations", type=int, default=20)
-    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=0, help="lanes per vertex (0: autotune)")
+    ap.add_argument("--per-lane", type=int, default=0, help="incidences prefetched per lane (0: auto)")
+    ap.add_argument("--no-xcd", action="store_true", help="disable the XCD-aware workgroup order")
+    ap.add_argument("--nontemporal", action="store_true", help="non-temporal loads of the incidence stream")
+    ap.add_argument("--stride", choices=["auto", "fixed", "csr"], default="auto",
+                    help="incidence rows: fixed-stride (ELL) or CSR (with --lanes)")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MB (counter calibration) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -61,6 +69,15 @@ def main():
     from physicsbasedanimationtoolkit_amd import meshes, pbat
 
     torch.cuda.set_device(0)
+    if args.copy_probe_mb:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}), flush=True)
+        return
     s = meshes.plain_gpu_scene(args.grid, seed=0)
     dt, iters = s.dt, args.iterations
     log(f"[bench] scene {s.name}: {s.n_vertices} vertices, {s.n_tets} tets")
@@ -68,35 +85,42 @@ def main():
     data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
             .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).construct())
     vbd = pbat.gpu.vbd.Integrator(data)
-    if args.lanes:
-        vbd.lanes_per_vertex = args.lanes
     nColors, nUpd, nInc = vbd.counts()
     log(f"[bench] construct+upload {time.time() - t0:.2f}s: {nColors} colours, {nUpd} updated vertices, "
         f"{nInc} incidences")
+    if args.lanes:
+        vbd.set_sweep_config(args.lanes, args.per_lane, not args.no_xcd, args.nontemporal,
+                             None if args.stride == "auto" else args.stride == "fixed")
+    else:
+        for L, K, X, N, S, us in vbd.autotune(dt, sweeps=10):
+            log(f"[bench] autotune lanes={int(L)} per_lane={int(K)} xcd={int(X)} nt={int(N)} "
+                f"fixed_stride={int(S)}: {us:.2f} us/sweep")
+    if args.no_graph:
+        vbd.graph_capture = False
+    shape = vbd.sweep_config()
+    log(f"[bench] sweep shape: {shape}")
 
-    for _ in range(args.warmup):
+    # HIP events recorded on the integrator's stream around each replayed step
+    # graph time the timed region itself; per sweep launch this is an upper
+    # bound (the bracket also holds the two elementwise substep kernels).
+    vbd.set_timing(True)
+    for _ in range(max(1, args.warmup)):
         vbd.step(dt, iters, 1)
     vbd.synchronize()
     torch.cuda.synchronize()
+    vbd.sweep_timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         vbd.step(dt, iters, 1)
     vbd.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    ms, launches = vbd.sweep_timing(reset=True)
+    vbd.set_timing(False)
     vu = nUpd * iters * args.steps
     value = vu / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"[bench] {args.steps} steps in {elapsed:.4f}s -> {value:.4e} vertex-updates/s")
-
-    # Sweep-kernel timing with HIP events on the integrator's stream (same steps, eager launches)
-    vbd.set_timing(True)
-    vbd.sweep_timing(reset=True)
-    tsteps = max(1, min(args.steps, 10))
-    for _ in range(tsteps):
-        vbd.step(dt, iters, 1)
-    ms, launches = vbd.sweep_timing(reset=True)
-    vbd.set_timing(False)
     avg_launch_s = ms * 1e-3 / launches
     bytes_per_sweep = BYTES_PER_VERTEX * nUpd + BYTES_PER_INCIDENCE * nInc
     bytes_per_launch = bytes_per_sweep / nColors
@@ -140,7 +164,8 @@ def main():
         "config": {"workload": f"plain VBD, {args.grid}^3 jittered cube grid ({s.n_vertices} verts, "
                                f"{s.n_tets} tets), SNH per-tet log-uniform Y, h=1/60, {iters} iterations, "
                                f"bottom fixed", "colours": nColors, "updated_vertices": nUpd,
-                   "incidences": nInc, "parallelism": "single GPU"},
+                   "incidences": nInc, "parallelism": "single GPU",
+                   "sweep_shape": dict(shape, hip_graph=not args.no_graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "VbdSweep", "avg_launch_us": avg_launch_s * 1e6,

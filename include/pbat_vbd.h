@@ -157,6 +157,28 @@ int pbat_vbd_integrator_set_scene_bounding_box(pbat_vbd_integrator* h, const flo
  * 0 = choose from the mesh's maximum vertex degree). No reference counterpart:
  * the reference always uses one thread block per vertex. */
 int pbat_vbd_integrator_set_lanes_per_vertex(pbat_vbd_integrator* h, int32_t lanes);
+/* Shape of the sweep kernel. No reference counterpart. */
+typedef struct pbat_vbd_sweep_config {
+    int32_t lanes_per_vertex;    /* power of two in [1, 64]; 0 = automatic */
+    int32_t incidences_per_lane; /* incidences a lane prefetches before accumulating, 1..4; 0 = automatic */
+    int32_t xcd_remap;           /* reorder workgroups so each XCD's L2 serves one range of vertices */
+    int32_t nontemporal;         /* non-temporal loads of the incidence stream and tet constants */
+    int32_t fixed_stride;        /* incidences in fixed-stride (ELL) rows instead of CSR rows */
+} pbat_vbd_sweep_config;
+
+/* Set / read the sweep shape. get_ reports the values in effect (automatic
+ * choices resolved). fixed_stride = 1 fails if the mesh's degree spread made
+ * the library skip the fixed-stride copy at construction. */
+int pbat_vbd_integrator_set_sweep_config(pbat_vbd_integrator* h, const pbat_vbd_sweep_config* config);
+int pbat_vbd_integrator_get_sweep_config(pbat_vbd_integrator* h, pbat_vbd_sweep_config* config);
+/* Autotuner: times `sweeps` full colour sweeps at substep dt for every
+ * candidate sweep shape (lanes with their natural prefetch depth, XCD order
+ * on/off, fixed-stride/CSR rows) and keeps the fastest; the simulation state
+ * is restored afterwards. timings (nullable, >= 192 doubles) receives up to 32
+ * rows of (lanes, incidences per lane, xcd remap, non-temporal, fixed stride,
+ * microseconds per sweep); *n_candidates the number of candidates timed. */
+int pbat_vbd_integrator_autotune(pbat_vbd_integrator* h, float dt, int32_t sweeps, double* timings,
+                                 int32_t* n_candidates);
 /* Capture each Step's launch sequence in a hipGraph and replay it (default on). */
 int pbat_vbd_integrator_set_graph_capture(pbat_vbd_integrator* h, int32_t enabled);
 
@@ -173,10 +195,11 @@ int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);
 int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPartitions, int64_t* nUpdated,
                                int64_t* nIncidences);
 
-/* Kernel-time instrumentation: when enabled, every sweep launch is bracketed by
- * HIP events on the handle's stream and the summed event time is kept.
- * pbat_vbd_integrator_sweep_timing returns (total ms, number of sweep launches)
- * since the last reset. No reference counterpart. */
+/* Kernel-time instrumentation: when enabled, Step launches eagerly and each
+ * substep's back-to-back sweep launches are bracketed by one pair of HIP
+ * events on the handle's stream. pbat_vbd_integrator_sweep_timing returns
+ * (summed event ms, number of sweep launches inside the brackets) since the
+ * last reset. No reference counterpart. */
 int pbat_vbd_integrator_set_timing(pbat_vbd_integrator* h, int32_t enabled);
 int pbat_vbd_integrator_sweep_timing(pbat_vbd_integrator* h, double* total_ms, int64_t* launches, int32_t reset);
 

@@ -46,6 +46,15 @@ class Data(ctypes.Structure):
     ]
 
 
+class SweepConfig(ctypes.Structure):
+    """``pbat_vbd_sweep_config`` of include/pbat_vbd.h."""
+
+    _fields_ = [
+        ("lanes_per_vertex", c_i32), ("incidences_per_lane", c_i32), ("xcd_remap", c_i32),
+        ("nontemporal", c_i32), ("fixed_stride", c_i32),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("pbat_vbd_last_error", ctypes.c_char_p, []),
@@ -69,6 +78,10 @@ SIGNATURES = [
     ("pbat_vbd_integrator_set_block_size", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_set_scene_bounding_box", ctypes.c_int, [ctypes.c_void_p, p_f32, p_f32]),
     ("pbat_vbd_integrator_set_lanes_per_vertex", ctypes.c_int, [ctypes.c_void_p, c_i32]),
+    ("pbat_vbd_integrator_set_sweep_config", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SweepConfig)]),
+    ("pbat_vbd_integrator_get_sweep_config", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(SweepConfig)]),
+    ("pbat_vbd_integrator_autotune", ctypes.c_int,
+     [ctypes.c_void_p, c_f32, c_i32, p_f64, ctypes.POINTER(c_i32)]),
     ("pbat_vbd_integrator_set_graph_capture", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_energy", ctypes.c_int, [ctypes.c_void_p, p_f64]),
     ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),

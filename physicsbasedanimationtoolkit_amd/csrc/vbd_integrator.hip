@@ -5,11 +5,16 @@
 //  - vertices are renumbered colour-major, so each colour is one contiguous
 //    range and a sweep over it reads per-vertex data coalesced; within a colour
 //    vertices are sorted by degree (stable) so lanes of a wave do equal work;
-//  - per-tet data is one 64-byte record (vertex ids, 9 shape-function
-//    gradients, w*mu, w*lambda, alpha), gathered through a 4-byte per-incidence
-//    CSR reference (tet << 2 | ilocal);
+//  - each vertex-tet incidence is a 64-byte record stored in sweep order: the
+//    tet's other three vertex ids, their three shape-function-gradient rows
+//    and the tet's w*mu, w*lambda, alpha. A colour pass streams its records
+//    (instead of gathering per-tet data that straddles cache lines), and only
+//    the neighbour positions are gathered, one dependent load after the record;
 //  - positions carry the lumped mass in their 4th component, so one 16-byte
 //    gather serves both;
+//  - workgroups are reordered so each XCD's L2 serves one contiguous range of
+//    vertices; the sweep shape (lanes per vertex, incidences prefetched per
+//    lane, XCD order, non-temporal stream loads) is chosen by an autotuner;
 //  - updates are written in place (no xb write buffer + copy-back kernel):
 //    vertices of one colour share no tet, so no vertex reads what another
 //    vertex of its colour writes;
@@ -83,15 +88,45 @@ struct DeviceArray
     }
 };
 
+// Incidence records of the sweep (structure of arrays, see SweepArgs).
+struct IncidenceRecords
+{
+    DeviceArray<int4> ids;
+    DeviceArray<float4> g0, g1, g2;
+    void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(); }
+};
+
+struct HostIncidenceRecords
+{
+    std::vector<int4> ids;
+    std::vector<float4> g0, g1, g2;
+    void resize(size_t n)
+    {
+        ids.assign(n, make_int4(0, 0, 0, -1));
+        g0.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
+        g1 = g0;
+        g2 = g0;
+    }
+    void upload(IncidenceRecords& d, hipStream_t s) const
+    {
+        d.ids.upload(ids, s);
+        d.g0.upload(g0, s);
+        d.g1.upload(g1, s);
+        d.g2.upload(g2, s);
+    }
+};
+
 struct GraphKey
 {
     float dt = 0.f;
-    int iterations = -1, substeps = -1, strategy = -1, lanes = -1, block = -1;
+    int iterations = -1, substeps = -1, strategy = -1, lanes = -1, block = -1, perLane = -1;
+    bool xcd = false, nt = false, ell = false;
     float kD = 0.f, detHZero = 0.f;
     bool operator==(GraphKey const& o) const
     {
         return dt == o.dt && iterations == o.iterations && substeps == o.substeps && strategy == o.strategy &&
-               lanes == o.lanes && block == o.block && kD == o.kD && detHZero == o.detHZero;
+               lanes == o.lanes && block == o.block && perLane == o.perLane && xcd == o.xcd && nt == o.nt &&
+               ell == o.ell && kD == o.kD && detHZero == o.detHZero;
     }
 };
 
@@ -107,17 +142,27 @@ struct pbat_vbd_integrator
     std::vector<int> perm;       // internal -> original vertex index
     std::vector<int> iperm;      // original -> internal
     std::vector<float> massOrig; // lumped masses in original order (FP32)
+    std::vector<float> wgHost;   // quadrature weights (traces only)
+    std::vector<float> lameHost; // (mu, lambda) per element (traces only)
 
     pbat_vbd::DeviceArray<float4> xm, xtil, xt, vel, vprev, aext;
-    pbat_vbd::DeviceArray<pbat_vbd::device::TetRecord> tets;
-    pbat_vbd::DeviceArray<int> vtp, vtref;
+    pbat_vbd::DeviceArray<float4> tgp; // 3 per tet
+    pbat_vbd::DeviceArray<int4> tetv;
+    pbat_vbd::IncidenceRecords rows, ell; // CSR rows, fixed-stride rows
+    int ellStride = 0;   // > 0 when the fixed-stride rows exist
+    bool useEll = false; // sweep reads the fixed-stride rows
+    pbat_vbd::DeviceArray<int> vtp;
     pbat_vbd::DeviceArray<double> energy;
 
     int strategy = PBAT_VBD_INIT_ADAPTIVE_PBAT;
     float kD = 0.f;
     float detHZero = 1e-7f;
     int blockSize = 256;
-    int lanes = 0; // 0 = automatic
+    int lanes = 0;         // 0 = automatic
+    int perLane = 0;       // incidences prefetched per lane, 0 = automatic
+    bool xcdRemap = true;  // XCD-aware block order
+    bool ntStream = false; // non-temporal loads of the incidence stream
+    bool tuneNontemporal = true; // autotune also tries non-temporal stream loads
     bool useGraph = true;
     float worldMin[3] = {0.f, 0.f, 0.f}, worldMax[3] = {0.f, 0.f, 0.f};
 
@@ -126,7 +171,9 @@ struct pbat_vbd_integrator
     pbat_vbd::GraphKey graphKey;
 
     bool timing = false;
+    int graphSweeps = 0; // sweep launches in the captured graph
     std::vector<hipEvent_t> events; // pairs (start, stop)
+    std::vector<int> pendingLaunches; // sweep launches between each pair
     size_t eventsUsed = 0;
     double timedMs = 0.;
     int64_t timedLaunches = 0;
@@ -142,7 +189,7 @@ struct pbat_vbd_integrator
         if (stream)
             (void)hipStreamSynchronize(stream);
         xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
-        tets.reset(), vtp.reset(), vtref.reset(), energy.reset();
+        tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -163,17 +210,46 @@ int EffectiveLanes(pbat_vbd_integrator const& h)
     return L;
 }
 
-template <int L>
-void LaunchSweepL(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts)
+// Incidences each lane prefetches before accumulating (1..4): enough to cover
+// the busiest updated vertex, capped to bound register use.
+int EffectivePerLane(pbat_vbd_integrator const& h)
+{
+    if (h.perLane > 0)
+        return h.perLane;
+    int const L = EffectiveLanes(h);
+    int const d = static_cast<int>(std::max<int64_t>(h.maxDegree, 1));
+    return std::clamp((d + L - 1) / L, 1, 4);
+}
+
+template <int L, int K>
+void LaunchSweepLK(pbat_vbd_integrator& h, device::SweepArgs a, int nVerts)
 {
     int const block = h.blockSize;
     int64_t const threads = static_cast<int64_t>(nVerts) * L;
-    int64_t const grid64 = (threads + block - 1) / block;
+    int64_t grid64 = (threads + block - 1) / block;
+    a.xcdChunk = 0;
+    if (h.xcdRemap && grid64 >= 16)
+    {
+        // pad to a multiple of the 8 XCDs; padded blocks hold no vertex
+        grid64 = (grid64 + 7) / 8 * 8;
+        a.xcdChunk = static_cast<int>(grid64 / 8);
+    }
     // the kernel indexes threads with 32-bit ints
     if (grid64 * block >= (int64_t{1} << 31))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "colour too large for the sweep launch: reduce lanes per vertex");
-    int const grid = static_cast<int>(grid64);
-    hipLaunchKernelGGL(device::VbdSweep<L>, dim3(grid), dim3(block), 0, h.stream, a);
+    hipLaunchKernelGGL((device::VbdSweep<L, K>), dim3(static_cast<int>(grid64)), dim3(block), 0, h.stream, a);
+}
+
+template <int L>
+void LaunchSweepL(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts)
+{
+    switch (EffectivePerLane(h))
+    {
+        case 1: LaunchSweepLK<L, 1>(h, a, nVerts); break;
+        case 2: LaunchSweepLK<L, 2>(h, a, nVerts); break;
+        case 3: LaunchSweepLK<L, 3>(h, a, nVerts); break;
+        default: LaunchSweepLK<L, 4>(h, a, nVerts); break;
+    }
 }
 
 void LaunchSweep(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts)
@@ -190,6 +266,28 @@ void LaunchSweep(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts)
     }
 }
 
+device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2)
+{
+    device::SweepArgs a{};
+    pbat_vbd::IncidenceRecords const& r = (h.useEll && h.ellStride > 0) ? h.ell : h.rows;
+    a.vtp = h.vtp.ptr;
+    a.ids = r.ids.ptr;
+    a.g0 = r.g0.ptr;
+    a.g1 = r.g1.ptr;
+    a.g2 = r.g2.ptr;
+    a.xm = h.xm.ptr;
+    a.xtil = h.xtil.ptr;
+    a.xt = h.xt.ptr;
+    a.c.dt = sdt;
+    a.c.dt2 = sdt2;
+    a.c.kD = h.kD;
+    a.c.detHZero = h.detHZero;
+    a.ntStream = h.ntStream ? 1 : 0;
+    if (h.useEll && h.ellStride > 0)
+        a.ellStride = h.ellStride;
+    return a;
+}
+
 hipEvent_t NextEvent(pbat_vbd_integrator& h)
 {
     if (h.eventsUsed == h.events.size())
@@ -201,6 +299,7 @@ hipEvent_t NextEvent(pbat_vbd_integrator& h)
     return h.events[h.eventsUsed++];
 }
 
+// Sum the event pairs recorded by the last Step and release them.
 void CollectTiming(pbat_vbd_integrator& h)
 {
     if (h.eventsUsed == 0)
@@ -211,34 +310,33 @@ void CollectTiming(pbat_vbd_integrator& h)
         float ms = 0.f;
         PBAT_HIP(hipEventElapsedTime(&ms, h.events[k], h.events[k + 1]));
         h.timedMs += ms;
-        ++h.timedLaunches;
+        h.timedLaunches += h.pendingLaunches[k / 2];
     }
     h.eventsUsed = 0;
+    h.pendingLaunches.clear();
 }
 
-// Enqueue the launches of one Step (gpu/impl/vbd/Integrator.cu:82-103).
-void EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
+// Enqueue the launches of one Step (gpu/impl/vbd/Integrator.cu:82-103) and
+// return the number of sweep launches. With eventBrackets, one event pair
+// brackets each substep's back-to-back sweep launches.
+int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool eventBrackets)
 {
     float const sdt = dt / static_cast<float>(substeps);
     float const sdt2 = sdt * sdt;
     int const n = static_cast<int>(h.nV);
     int const eb = 256, eg = (n + eb - 1) / eb;
-    device::SweepArgs a{};
-    a.vtp = h.vtp.ptr;
-    a.vtref = h.vtref.ptr;
-    a.tets = h.tets.ptr;
-    a.xm = h.xm.ptr;
-    a.xtil = h.xtil.ptr;
-    a.xt = h.xt.ptr;
-    a.dt = sdt;
-    a.dt2 = sdt2;
-    a.kD = h.kD;
-    a.detHZero = h.detHZero;
+    device::SweepArgs a = MakeSweepArgs(h, sdt, sdt2);
     int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
+    int total = 0;
     for (int s = 0; s < substeps; ++s)
     {
         hipLaunchKernelGGL(device::SubstepBegin, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xtil.ptr, h.xt.ptr,
                            h.vel.ptr, h.vprev.ptr, h.aext.ptr, n, sdt, sdt2, h.strategy);
+        // Per-launch events would add their own overhead to every kernel;
+        // the gaps between queued launches are small.
+        int launched = 0;
+        if (eventBrackets)
+            PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
         for (int k = 0; k < iterations; ++k)
             for (int c = 0; c < nColors; ++c)
             {
@@ -247,16 +345,20 @@ void EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
                 int const nc = a.vend - a.vbeg;
                 if (nc <= 0)
                     continue;
-                if (h.timing)
-                    PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
                 LaunchSweep(h, a, nc);
-                if (h.timing)
-                    PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
+                ++launched;
             }
+        if (eventBrackets)
+        {
+            PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
+            h.pendingLaunches.push_back(launched);
+        }
+        total += launched;
         hipLaunchKernelGGL(device::SubstepEnd, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xt.ptr, h.vel.ptr,
                            h.vprev.ptr, n, sdt);
     }
     PBAT_HIP(hipGetLastError());
+    return total;
 }
 
 void InvalidateGraph(pbat_vbd_integrator& h)
@@ -268,6 +370,7 @@ void InvalidateGraph(pbat_vbd_integrator& h)
     h.graphExec = nullptr;
     h.graph = nullptr;
     h.graphKey = GraphKey{};
+    h.graphSweeps = 0;
 }
 
 void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
@@ -277,9 +380,10 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     if (iterations < 0 || substeps < 1)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "expected iterations >= 0 and substeps >= 1");
     PBAT_HIP(hipSetDevice(h.device));
-    if (h.timing || !h.useGraph)
+    if (!h.useGraph)
     {
-        EnqueueStep(h, dt, iterations, substeps);
+        InvalidateGraph(h);
+        EnqueueStep(h, dt, iterations, substeps, h.timing);
         if (h.timing)
             CollectTiming(h);
         PBAT_HIP(hipStreamSynchronize(h.stream));
@@ -288,28 +392,45 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     GraphKey key;
     key.dt = dt, key.iterations = iterations, key.substeps = substeps, key.strategy = h.strategy;
     key.lanes = EffectiveLanes(h), key.block = h.blockSize, key.kD = h.kD, key.detHZero = h.detHZero;
+    key.perLane = EffectivePerLane(h), key.xcd = h.xcdRemap, key.nt = h.ntStream, key.ell = h.useEll;
     if (!(h.graphExec && h.graphKey == key))
     {
         InvalidateGraph(h);
         PBAT_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
         try
         {
-            EnqueueStep(h, dt, iterations, substeps);
+            h.graphSweeps = EnqueueStep(h, dt, iterations, substeps, false);
+            PBAT_HIP(hipStreamEndCapture(h.stream, &h.graph));
+            PBAT_HIP(hipGraphInstantiate(&h.graphExec, h.graph, nullptr, nullptr, 0));
         }
         catch (...)
         {
-            hipGraph_t g = nullptr;
-            (void)hipStreamEndCapture(h.stream, &g);
-            if (g)
-                (void)hipGraphDestroy(g);
+            hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+            if (hipStreamIsCapturing(h.stream, &st) == hipSuccess && st != hipStreamCaptureStatusNone)
+            {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(h.stream, &g);
+                if (g)
+                    (void)hipGraphDestroy(g);
+            }
+            InvalidateGraph(h);
             throw;
         }
-        PBAT_HIP(hipStreamEndCapture(h.stream, &h.graph));
-        PBAT_HIP(hipGraphInstantiate(&h.graphExec, h.graph, nullptr, nullptr, 0));
         h.graphKey = key;
     }
+    // Timing a replay: one event pair around the graph launch. The bracket also
+    // holds the substep begin/end kernels (two elementwise passes per substep).
+    if (h.timing)
+        PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
     PBAT_HIP(hipGraphLaunch(h.graphExec, h.stream));
+    if (h.timing)
+    {
+        PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
+        h.pendingLaunches.push_back(h.graphSweeps);
+    }
     PBAT_HIP(hipStreamSynchronize(h.stream));
+    if (h.timing)
+        CollectTiming(h);
 }
 
 void CheckVertexArray(pbat_vbd_integrator const& h, void const* p, int64_t n, char const* what)
@@ -444,40 +565,108 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
                 }
     }
 
-    // Tet records
-    std::vector<device::TetRecord> T(static_cast<size_t>(nT));
+    // Per-tet constants (3 float4) and vertex ids
+    std::vector<float4> G4(static_cast<size_t>(3 * nT));
+    std::vector<int4> TV(static_cast<size_t>(nT));
     for (int64_t e = 0; e < nT; ++e)
     {
         double const* G = d->GP + 12 * e; // G(r, c) = G[4c + r]
         auto gp = [&](int r, int c) { return static_cast<float>(G[4 * c + r]); };
         double const w = d->wg[e], mu = d->lame[2 * e], lam = d->lame[2 * e + 1];
-        device::TetRecord& R = T[e];
-        R.v = make_int4(h->iperm[d->E[4 * e]], h->iperm[d->E[4 * e + 1]], h->iperm[d->E[4 * e + 2]],
-                        h->iperm[d->E[4 * e + 3]]);
-        R.q0 = make_float4(gp(1, 0), gp(1, 1), gp(1, 2), gp(2, 0));
-        R.q1 = make_float4(gp(2, 1), gp(2, 2), gp(3, 0), gp(3, 1));
-        R.q2 = make_float4(gp(3, 2), static_cast<float>(w * mu), static_cast<float>(w * lam),
-                           static_cast<float>(1. + mu / lam));
+        if (!(lam != 0.))
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "lambda must be non-zero (element " + std::to_string(e) + ")");
+        TV[e] = make_int4(h->iperm[d->E[4 * e]], h->iperm[d->E[4 * e + 1]], h->iperm[d->E[4 * e + 2]],
+                          h->iperm[d->E[4 * e + 3]]);
+        G4[3 * e] = make_float4(gp(1, 0), gp(1, 1), gp(1, 2), gp(2, 0));
+        G4[3 * e + 1] = make_float4(gp(2, 1), gp(2, 2), gp(3, 0), gp(3, 1));
+        G4[3 * e + 2] = make_float4(gp(3, 2), static_cast<float>(w * mu), static_cast<float>(w * lam),
+                                    static_cast<float>(1. + mu / lam));
     }
-    // CSR in internal numbering
-    std::vector<int> P(static_cast<size_t>(nV) + 1, 0), R;
-    R.reserve(static_cast<size_t>(nInc));
+    // Incidence records in internal numbering and sweep order: for each
+    // incidence the tet's other three vertices (local order) with their GP
+    // rows, and the tet's w*mu, w*lambda, alpha. Only updated vertices are
+    // swept, so only their rows are stored.
+    std::vector<int> P(static_cast<size_t>(nV) + 1, 0);
     for (int64_t v = 0; v < nV; ++v)
     {
         int64_t const i = h->perm[v];
-        h->maxDegree = std::max(h->maxDegree, d->GVGp[i + 1] - d->GVGp[i]);
         for (int64_t k = d->GVGp[i]; k < d->GVGp[i + 1]; ++k)
-            R.push_back(static_cast<int>((d->GVGe[k] << 2) | d->GVGilocal[k]));
-        P[v + 1] = static_cast<int>(R.size());
+        {
+            int64_t const e = d->GVGe[k];
+            if (d->E[4 * e + d->GVGilocal[k]] != i)
+                Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "GVGe/GVGilocal inconsistent with E at vertex " +
+                                                          std::to_string(i));
+        }
+        P[v + 1] = P[v] + static_cast<int>(d->GVGp[i + 1] - d->GVGp[i]);
     }
-    // maximum degree over updated vertices decides the default lanes per vertex
     h->maxDegree = 0;
     for (int64_t v = 0; v < nUpd; ++v)
         h->maxDegree = std::max<int64_t>(h->maxDegree, P[v + 1] - P[v]);
+    int64_t const nIncUpd = P[nUpd];
+    auto fill = [&](HostIncidenceRecords& R, int64_t slot, int64_t k) {
+        int64_t const e = d->GVGe[k];
+        int const il = static_cast<int>(d->GVGilocal[k]);
+        double const* G = d->GP + 12 * e; // G(r, c) = G[4c + r]
+        int ids[3];
+        float g[3][3];
+        int n = 0;
+        for (int j = 0; j < 4; ++j)
+        {
+            if (j == il)
+                continue;
+            ids[n] = h->iperm[d->E[4 * e + j]];
+            for (int c = 0; c < 3; ++c)
+                g[n][c] = static_cast<float>(G[4 * c + j]);
+            ++n;
+        }
+        float4 const m = G4[3 * e + 2]; // (., w*mu, w*lambda, alpha)
+        R.ids[slot] = make_int4(ids[0], ids[1], ids[2], static_cast<int>(e));
+        R.g0[slot] = make_float4(g[0][0], g[0][1], g[0][2], g[1][0]);
+        R.g1[slot] = make_float4(g[1][1], g[1][2], g[2][0], g[2][1]);
+        R.g2[slot] = make_float4(g[2][2], m.y, m.z, m.w);
+    };
+    {
+        HostIncidenceRecords R;
+        R.resize(static_cast<size_t>(nIncUpd));
+        for (int64_t v = 0; v < nUpd; ++v)
+        {
+            int64_t const i = h->perm[v];
+            for (int64_t k = d->GVGp[i]; k < d->GVGp[i + 1]; ++k)
+                fill(R, P[v] + (k - d->GVGp[i]), k);
+        }
+        R.upload(h->rows, h->stream);
+        PBAT_HIP(hipStreamSynchronize(h->stream));
+    }
+    // Fixed-stride copy when padding every row to the maximum degree costs at
+    // most 35% more records than the CSR rows.
+    if (nUpd > 0 && h->maxDegree > 0 && h->maxDegree * nUpd * 100 <= 135 * nIncUpd &&
+        h->maxDegree * nUpd < (int64_t{1} << 31))
+    {
+        int64_t const S = h->maxDegree;
+        HostIncidenceRecords R;
+        R.resize(static_cast<size_t>(nUpd * S));
+        for (int64_t v = 0; v < nUpd; ++v)
+        {
+            int64_t const i = h->perm[v];
+            for (int64_t k = d->GVGp[i]; k < d->GVGp[i + 1]; ++k)
+                fill(R, v * S + (k - d->GVGp[i]), k);
+        }
+        R.upload(h->ell, h->stream);
+        PBAT_HIP(hipStreamSynchronize(h->stream));
+        h->ellStride = static_cast<int>(S);
+    }
 
-    h->tets.upload(T, h->stream);
+    h->wgHost.resize(static_cast<size_t>(nT));
+    h->lameHost.resize(static_cast<size_t>(2 * nT));
+    for (int64_t e = 0; e < nT; ++e)
+    {
+        h->wgHost[e] = static_cast<float>(d->wg[e]);
+        h->lameHost[2 * e] = static_cast<float>(d->lame[2 * e]);
+        h->lameHost[2 * e + 1] = static_cast<float>(d->lame[2 * e + 1]);
+    }
+    h->tgp.upload(G4, h->stream);
+    h->tetv.upload(TV, h->stream);
     h->vtp.upload(P, h->stream);
-    h->vtref.upload(R, h->stream);
     for (auto* b : {&h->xm, &h->xtil, &h->xt, &h->vel, &h->vprev, &h->aext})
         b->allocate(static_cast<size_t>(nV));
     h->energy.allocate(3);
@@ -509,7 +698,7 @@ void Energy(pbat_vbd_integrator& h, double out[3])
     int const n = static_cast<int>(std::max(h.nV, h.nT));
     int const grid = std::min(4096, (n + 255) / 256);
     hipLaunchKernelGGL(device::EnergyTerms, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr, h.vel.ptr, h.aext.ptr,
-                       h.tets.ptr, static_cast<int>(h.nV), static_cast<int>(h.nT), h.energy.ptr);
+                       h.tetv.ptr, h.tgp.ptr, static_cast<int>(h.nV), static_cast<int>(h.nT), h.energy.ptr);
     PBAT_HIP(hipGetLastError());
     PBAT_HIP(hipMemcpyAsync(out, h.energy.ptr, 3 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
     PBAT_HIP(hipStreamSynchronize(h.stream));
@@ -561,26 +750,45 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
     if (t == 0)
     {
         // Element and vertex data of the first frame, in original numbering
-        std::vector<device::TetRecord> T(static_cast<size_t>(h.nT));
-        PBAT_HIP(hipMemcpy(T.data(), h.tets.ptr, T.size() * sizeof(device::TetRecord), hipMemcpyDeviceToHost));
-        std::vector<double> Tm(static_cast<size_t>(4 * h.nT)), M(h.massOrig.begin(), h.massOrig.end());
-        for (int64_t e = 0; e < h.nT; ++e)
+        // (T |#elements|x4, M |#vertices|x1, wg |#elements|x1, GP 4x3|#elements|, lame |#elements|x2)
+        int64_t const nT = h.nT;
+        std::vector<int4> TV(static_cast<size_t>(nT));
+        std::vector<float4> G4(static_cast<size_t>(3 * nT));
+        PBAT_HIP(hipMemcpy(TV.data(), h.tetv.ptr, TV.size() * sizeof(int4), hipMemcpyDeviceToHost));
+        PBAT_HIP(hipMemcpy(G4.data(), h.tgp.ptr, G4.size() * sizeof(float4), hipMemcpyDeviceToHost));
+        std::vector<double> Tm(static_cast<size_t>(4 * nT)), M(h.massOrig.begin(), h.massOrig.end());
+        std::vector<double> W(static_cast<size_t>(nT)), GP(static_cast<size_t>(12 * nT)), La(2 * W.size());
+        for (int64_t e = 0; e < nT; ++e)
         {
-            int const ids[4] = {T[e].v.x, T[e].v.y, T[e].v.z, T[e].v.w};
+            int const ids[4] = {TV[e].x, TV[e].y, TV[e].z, TV[e].w};
             for (int k = 0; k < 4; ++k)
-                Tm[k * h.nT + e] = h.perm[ids[k]];
+                Tm[k * nT + e] = h.perm[ids[k]];
+            W[e] = h.wgHost[e];
+            La[e] = h.lameHost[2 * e];
+            La[nT + e] = h.lameHost[2 * e + 1];
+            float4 const q0 = G4[3 * e], q1 = G4[3 * e + 1], q2 = G4[3 * e + 2];
+            float const rows[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+            for (int c = 0; c < 3; ++c)
+            {
+                // column 3e + c of the 4 x 3nT matrix
+                double* col = GP.data() + 4 * (3 * e + c);
+                col[0] = -(rows[0][c] + rows[1][c] + rows[2][c]);
+                for (int r = 0; r < 3; ++r)
+                    col[r + 1] = rows[r][c];
+            }
         }
-        WriteDense(Join(dir, "T.mtx"), h.nT, 4, Tm);
+        WriteDense(Join(dir, "T.mtx"), nT, 4, Tm);
         WriteDense(Join(dir, "M.mtx"), h.nV, 1, M);
+        WriteDense(Join(dir, "wg.mtx"), nT, 1, W);
+        WriteDense(Join(dir, "GP.mtx"), 4, 3 * nT, GP);
+        WriteDense(Join(dir, "lame.mtx"), nT, 2, La);
     }
     float const sdt = dt / static_cast<float>(substeps);
     float const sdt2 = sdt * sdt;
     int const n = static_cast<int>(h.nV);
     int const eb = 256, eg = (n + eb - 1) / eb;
     int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
-    device::SweepArgs a{};
-    a.vtp = h.vtp.ptr, a.vtref = h.vtref.ptr, a.tets = h.tets.ptr, a.xm = h.xm.ptr, a.xtil = h.xtil.ptr;
-    a.xt = h.xt.ptr, a.dt = sdt, a.dt2 = sdt2, a.kD = h.kD, a.detHZero = h.detHZero;
+    device::SweepArgs a = MakeSweepArgs(h, sdt, sdt2);
     for (int s = 0; s < substeps; ++s)
     {
         hipLaunchKernelGGL(device::SubstepBegin, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xtil.ptr, h.xt.ptr,
@@ -603,6 +811,110 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
     }
     PBAT_HIP(hipGetLastError());
     PBAT_HIP(hipStreamSynchronize(h.stream));
+}
+
+// Time full sweeps (all colours) for each candidate sweep shape and keep the
+// fastest. Device state is saved before and restored after, so the simulation
+// is unchanged. Candidates: lanes L in {2..64} with the natural prefetch depth
+// K = ceil(maxDegree / L) (when K <= 4), each with and without the XCD order
+// and with and without non-temporal loads of the incidence stream.
+constexpr int kMaxTuneRows = 32;
+
+void Autotune(pbat_vbd_integrator& h, float dt, int sweeps, double* timingsUs, int32_t* nCandidates)
+{
+    if (!(dt > 0.f) || sweeps < 1)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "autotune needs dt > 0 and sweeps >= 1");
+    PBAT_HIP(hipSetDevice(h.device));
+    size_t const bytes = sizeof(float4) * static_cast<size_t>(h.nV);
+    DeviceArray<float4> saved;
+    saved.allocate(5 * static_cast<size_t>(h.nV));
+    float4* bufs[5] = {h.xm.ptr, h.xtil.ptr, h.xt.ptr, h.vel.ptr, h.vprev.ptr};
+    for (int b = 0; b < 5; ++b)
+        PBAT_HIP(hipMemcpyAsync(saved.ptr + b * h.nV, bufs[b], bytes, hipMemcpyDeviceToDevice, h.stream));
+    int const n = static_cast<int>(h.nV);
+    hipLaunchKernelGGL(device::SubstepBegin, dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr, h.xtil.ptr,
+                       h.xt.ptr, h.vel.ptr, h.vprev.ptr, h.aext.ptr, n, dt, dt * dt, h.strategy);
+    int const oldL = h.lanes, oldK = h.perLane;
+    bool const oldX = h.xcdRemap, oldN = h.ntStream, oldTiming = h.timing;
+    h.timing = false;
+    hipEvent_t e0, e1;
+    PBAT_HIP(hipEventCreate(&e0));
+    PBAT_HIP(hipEventCreate(&e1));
+    device::SweepArgs a = MakeSweepArgs(h, dt, dt * dt);
+    int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
+    auto sweep = [&] {
+        for (int c = 0; c < nColors; ++c)
+        {
+            a.vbeg = h.colorBegin[c];
+            a.vend = h.colorBegin[c + 1];
+            if (a.vend > a.vbeg)
+                LaunchSweep(h, a, a.vend - a.vbeg);
+        }
+    };
+    struct Candidate
+    {
+        int L, K;
+        bool xcd, nt, ell;
+    };
+    std::vector<Candidate> cands;
+    int const d = static_cast<int>(std::max<int64_t>(h.maxDegree, 1));
+    for (int L = 2; L <= 64; L <<= 1)
+    {
+        int const K = (d + L - 1) / L;
+        if (K > 4)
+            continue;
+        for (int ell = (h.ellStride > 0 ? 1 : 0); ell >= 0; --ell)
+            for (int x = 1; x >= 0; --x)
+                for (int nt = 0; nt <= (h.tuneNontemporal ? 1 : 0); ++nt)
+                    cands.push_back({L, K, x != 0, nt != 0, ell != 0});
+    }
+    bool const oldE = h.useEll;
+    double best = 1e300;
+    Candidate bestC{EffectiveLanes(h), EffectivePerLane(h), h.xcdRemap, h.ntStream, h.useEll};
+    int count = 0;
+    try
+    {
+        for (Candidate const& c : cands)
+        {
+            h.lanes = c.L, h.perLane = c.K, h.xcdRemap = c.xcd, h.ntStream = c.nt, h.useEll = c.ell;
+            a = MakeSweepArgs(h, dt, dt * dt);
+            sweep(); // warm-up
+            PBAT_HIP(hipEventRecord(e0, h.stream));
+            for (int s = 0; s < sweeps; ++s)
+                sweep();
+            PBAT_HIP(hipEventRecord(e1, h.stream));
+            PBAT_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            PBAT_HIP(hipEventElapsedTime(&ms, e0, e1));
+            double const us = 1e3 * ms / sweeps;
+            if (timingsUs && count < kMaxTuneRows)
+            {
+                double* row = timingsUs + 6 * count;
+                row[0] = c.L, row[1] = c.K, row[2] = c.xcd, row[3] = c.nt, row[4] = c.ell, row[5] = us;
+            }
+            ++count;
+            if (us < best)
+                best = us, bestC = c;
+        }
+    }
+    catch (...)
+    {
+        h.lanes = oldL, h.perLane = oldK, h.xcdRemap = oldX, h.ntStream = oldN, h.useEll = oldE;
+        h.timing = oldTiming;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    h.lanes = bestC.L, h.perLane = bestC.K, h.xcdRemap = bestC.xcd, h.ntStream = bestC.nt, h.useEll = bestC.ell;
+    h.timing = oldTiming;
+    for (int b = 0; b < 5; ++b)
+        PBAT_HIP(hipMemcpyAsync(bufs[b], saved.ptr + b * h.nV, bytes, hipMemcpyDeviceToDevice, h.stream));
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    InvalidateGraph(h);
+    if (nCandidates)
+        *nCandidates = count;
 }
 
 pbat_vbd_integrator& Handle(pbat_vbd_integrator* h)
@@ -742,6 +1054,48 @@ extern "C" int pbat_vbd_integrator_set_lanes_per_vertex(pbat_vbd_integrator* h, 
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "lanes per vertex must be 0 or a power of two in [1, 64]");
         Handle(h).lanes = lanes;
     });
+}
+
+extern "C" int pbat_vbd_integrator_set_sweep_config(pbat_vbd_integrator* h, const pbat_vbd_sweep_config* config)
+{
+    return Guard([&] {
+        auto& H = Handle(h);
+        if (!config)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null sweep config");
+        int const lanes = config->lanes_per_vertex, perLane = config->incidences_per_lane;
+        if (lanes != 0 && (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0))
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "lanes per vertex must be 0 or a power of two in [1, 64]");
+        if (perLane < 0 || perLane > 4)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "incidences per lane must be in [0, 4]");
+        if (config->fixed_stride && H.ellStride == 0)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
+                 "fixed-stride incidence rows are not available for this mesh (degree spread too large)");
+        H.lanes = lanes;
+        H.perLane = perLane;
+        H.xcdRemap = config->xcd_remap != 0;
+        H.ntStream = config->nontemporal != 0;
+        H.useEll = config->fixed_stride != 0;
+    });
+}
+
+extern "C" int pbat_vbd_integrator_get_sweep_config(pbat_vbd_integrator* h, pbat_vbd_sweep_config* config)
+{
+    return Guard([&] {
+        auto& H = Handle(h);
+        if (!config)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null sweep config");
+        config->lanes_per_vertex = EffectiveLanes(H);
+        config->incidences_per_lane = EffectivePerLane(H);
+        config->xcd_remap = H.xcdRemap ? 1 : 0;
+        config->nontemporal = H.ntStream ? 1 : 0;
+        config->fixed_stride = H.useEll ? 1 : 0;
+    });
+}
+
+extern "C" int pbat_vbd_integrator_autotune(pbat_vbd_integrator* h, float dt, int32_t sweeps, double* timings,
+                                            int32_t* n_candidates)
+{
+    return Guard([&] { Autotune(Handle(h), dt, sweeps, timings, n_candidates); });
 }
 
 extern "C" int pbat_vbd_integrator_set_graph_capture(pbat_vbd_integrator* h, int32_t enabled)

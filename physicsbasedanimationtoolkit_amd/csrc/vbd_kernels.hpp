@@ -6,10 +6,14 @@
 //   xtil[v]  float4 (x~, 0)           inertial target
 //   xt[v]    float4 (x_t, 0)          positions at the start of the substep
 //   vel[v], vprev[v], aext[v]  float4 velocities and external acceleration
-//   vtp[v]   int32 CSR prefix into vtref; vtref[k] = (tet << 2) | ilocal
-//   tets[e]  TetRecord, 64 B = one aligned half cache line:
-//            4 vertex ids, GP rows 1..3 (row 0 = -(row1+row2+row3)),
-//            w*mu, w*lambda, alpha = 1 + mu/lambda.
+//   vtp[v]   int32 CSR prefix over vertex-tet incidences
+//   incidence records (64 B, structure of arrays, in sweep order): the other
+//            three vertex ids of the tet with their GP rows and the tet's
+//            w*mu, w*lambda, alpha = 1 + mu/lambda (see SweepArgs)
+//   tgp[3e..3e+2]  float4 x3 (48 B) per tet: rows 1..3 of the shape function
+//            gradients GP (row 0 = -(row1+row2+row3)), w*mu, w*lambda, alpha
+//   tetv[e]  int4 vertex ids of tet e
+//   (tgp and tetv serve the energy reduction and traces, not the sweep)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -17,31 +21,6 @@
 #include <cstdint>
 
 namespace pbat_vbd::device {
-
-struct alignas(64) TetRecord
-{
-    int4 v;    // vertex indices
-    float4 q0; // GP(1,0) GP(1,1) GP(1,2) GP(2,0)
-    float4 q1; // GP(2,1) GP(2,2) GP(3,0) GP(3,1)
-    float4 q2; // GP(3,2) w*mu    w*lam   alpha
-};
-static_assert(sizeof(TetRecord) == 64, "TetRecord must be 64 bytes");
-
-struct SweepArgs
-{
-    int const* __restrict__ vtp;
-    int const* __restrict__ vtref;
-    TetRecord const* __restrict__ tets;
-    float4* __restrict__ xm;
-    float4 const* __restrict__ xtil;
-    float4 const* __restrict__ xt;
-    int vbeg;
-    int vend;
-    float dt;       // substep h
-    float dt2;      // h^2
-    float kD;       // Rayleigh damping coefficient
-    float detHZero; // skip threshold on |det H|
-};
 
 // ---- cross-lane sums over groups of L consecutive lanes (L | 64) ----------
 template <int kCtrl>
@@ -73,64 +52,62 @@ __device__ __forceinline__ float GroupSum(float x)
     return x;
 }
 
-// Elastic contribution of one incident tet to vertex i's 3x3 block.
-// F = Ds Dm^{-1} with Ds = [x1-x0, x2-x0, x3-x0]; the own row g of GP gives
-//   grad_i  += w mu F g + w lambda (J - alpha) c,   c = cof(F) g
-//   Hess_i  += w mu |g|^2 I + w lambda c c^T
-// (the d2J/dF2 term contracts to zero because det(F0 + x_i g^T) is affine in x_i).
+// 3x3 symmetric block and gradient of one vertex.
 struct Block
 {
     float h00, h01, h02, h11, h12, h22;
     float g0, g1, g2;
 };
 
-__device__ __forceinline__ void AccumulateTet(TetRecord const& T, int il, float4 const& x0, float4 const& x1,
-                                              float4 const& x2, float4 const& x3, Block& B)
+template <int L>
+__device__ __forceinline__ void GroupSum(Block& B)
 {
-    // rows of GP: r1, r2, r3 stored; r0 = -(r1 + r2 + r3)
-    float const r1x = T.q0.x, r1y = T.q0.y, r1z = T.q0.z;
-    float const r2x = T.q0.w, r2y = T.q1.x, r2z = T.q1.y;
-    float const r3x = T.q1.z, r3y = T.q1.w, r3z = T.q2.x;
-    float const wmu = T.q2.y, wlam = T.q2.z, alpha = T.q2.w;
-    // edge vectors
-    float const d1x = x1.x - x0.x, d1y = x1.y - x0.y, d1z = x1.z - x0.z;
-    float const d2x = x2.x - x0.x, d2y = x2.y - x0.y, d2z = x2.z - x0.z;
-    float const d3x = x3.x - x0.x, d3y = x3.y - x0.y, d3z = x3.z - x0.z;
-    // columns of F
-    float const f0x = d1x * r1x + d2x * r2x + d3x * r3x;
-    float const f0y = d1y * r1x + d2y * r2x + d3y * r3x;
-    float const f0z = d1z * r1x + d2z * r2x + d3z * r3x;
-    float const f1x = d1x * r1y + d2x * r2y + d3x * r3y;
-    float const f1y = d1y * r1y + d2y * r2y + d3y * r3y;
-    float const f1z = d1z * r1y + d2z * r2y + d3z * r3y;
-    float const f2x = d1x * r1z + d2x * r2z + d3x * r3z;
-    float const f2y = d1y * r1z + d2y * r2z + d3y * r3z;
-    float const f2z = d1z * r1z + d2z * r2z + d3z * r3z;
-    // cofactor columns
-    float const ax = f1y * f2z - f1z * f2y, ay = f1z * f2x - f1x * f2z, az = f1x * f2y - f1y * f2x; // f1 x f2
-    float const bx = f2y * f0z - f2z * f0y, by = f2z * f0x - f2x * f0z, bz = f2x * f0y - f2y * f0x; // f2 x f0
-    float const cx = f0y * f1z - f0z * f1y, cy = f0z * f1x - f0x * f1z, cz = f0x * f1y - f0y * f1x; // f0 x f1
+    B.h00 = GroupSum<L>(B.h00);
+    B.h01 = GroupSum<L>(B.h01);
+    B.h02 = GroupSum<L>(B.h02);
+    B.h11 = GroupSum<L>(B.h11);
+    B.h12 = GroupSum<L>(B.h12);
+    B.h22 = GroupSum<L>(B.h22);
+    B.g0 = GroupSum<L>(B.g0);
+    B.g1 = GroupSum<L>(B.g1);
+    B.g2 = GroupSum<L>(B.g2);
+}
+
+// Elastic contribution of one incident tet to vertex i's block (Stable
+// Neo-Hookean, physics/StableNeoHookeanEnergy.h), in the own-vertex frame.
+// With a, b, c the tet's other vertices and ga, gb, gc their rows of the
+// shape function gradients GP (so the own row is g = -(ga + gb + gc)),
+//   F = (xa - x) ga^T + (xb - x) gb^T + (xc - x) gc^T,
+// which equals x_e GP because the rows of GP sum to zero. With c = cof(F) g,
+//   grad_i += w mu F g + w lambda (J - alpha) c
+//   Hess_i += w mu |g|^2 I + w lambda c c^T
+// The d2J/dF2 term of the SNH Hessian contracts to zero for one vertex:
+// det(F0 + x_i g^T) is affine in x_i.
+__device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 const& q1, float4 const& q2,
+                                                   float4 const& own, float4 const& pa, float4 const& pb,
+                                                   float4 const& pc, Block& B)
+{
+    float const gax = q0.x, gay = q0.y, gaz = q0.z;
+    float const gbx = q0.w, gby = q1.x, gbz = q1.y;
+    float const gcx = q1.z, gcy = q1.w, gcz = q2.x;
+    float const wmu = q2.y, wlam = q2.z, alpha = q2.w;
+    float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
+    float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
+    float const dcx = pc.x - own.x, dcy = pc.y - own.y, dcz = pc.z - own.z;
+    float const f0x = dax * gax + dbx * gbx + dcx * gcx;
+    float const f0y = day * gax + dby * gbx + dcy * gcx;
+    float const f0z = daz * gax + dbz * gbx + dcz * gcx;
+    float const f1x = dax * gay + dbx * gby + dcx * gcy;
+    float const f1y = day * gay + dby * gby + dcy * gcy;
+    float const f1z = daz * gay + dbz * gby + dcz * gcy;
+    float const f2x = dax * gaz + dbx * gbz + dcx * gcz;
+    float const f2y = day * gaz + dby * gbz + dcy * gcz;
+    float const f2z = daz * gaz + dbz * gbz + dcz * gcz;
+    float const ax = f1y * f2z - f1z * f2y, ay = f1z * f2x - f1x * f2z, az = f1x * f2y - f1y * f2x;
+    float const bx = f2y * f0z - f2z * f0y, by = f2z * f0x - f2x * f0z, bz = f2x * f0y - f2y * f0x;
+    float const cx = f0y * f1z - f0z * f1y, cy = f0z * f1x - f0x * f1z, cz = f0x * f1y - f0y * f1x;
     float const J = f0x * ax + f0y * ay + f0z * az;
-    // own GP row
-    float gx, gy, gz;
-    if (il == 0)
-    {
-        gx = -(r1x + r2x + r3x);
-        gy = -(r1y + r2y + r3y);
-        gz = -(r1z + r2z + r3z);
-    }
-    else if (il == 1)
-    {
-        gx = r1x, gy = r1y, gz = r1z;
-    }
-    else if (il == 2)
-    {
-        gx = r2x, gy = r2y, gz = r2z;
-    }
-    else
-    {
-        gx = r3x, gy = r3y, gz = r3z;
-    }
+    float const gx = -(gax + gbx + gcx), gy = -(gay + gby + gcy), gz = -(gaz + gbz + gcz);
     float const kx = gx * ax + gy * bx + gz * cx;
     float const ky = gx * ay + gy * by + gz * cy;
     float const kz = gx * az + gy * bz + gz * cz;
@@ -151,14 +128,22 @@ __device__ __forceinline__ void AccumulateTet(TetRecord const& T, int il, float4
     B.h12 += lky * kz;
 }
 
+struct StepConstants
+{
+    float dt;       // substep h
+    float dt2;      // h^2
+    float kD;       // Rayleigh damping coefficient
+    float detHZero; // skip threshold on |det H|
+};
+
 // Damping, inertia and the 3x3 Newton step of one vertex
 // (sim/vbd/Kernels.h:173-191, 304-339). Returns the displacement to subtract.
 __device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 const& xl, float4 const& xtv,
-                                             SweepArgs const& a)
+                                             StepConstants const& c)
 {
-    if (a.kD != 0.f)
+    if (c.kD != 0.f)
     {
-        float const D = a.kD / a.dt;
+        float const D = c.kD / c.dt;
         float const ux = xv.x - xtv.x, uy = xv.y - xtv.y, uz = xv.z - xtv.z;
         B.g0 += D * (B.h00 * ux + B.h01 * uy + B.h02 * uz);
         B.g1 += D * (B.h01 * ux + B.h11 * uy + B.h12 * uz);
@@ -166,7 +151,7 @@ __device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 c
         float const s = 1.f + D;
         B.h00 *= s, B.h01 *= s, B.h02 *= s, B.h11 *= s, B.h12 *= s, B.h22 *= s;
     }
-    float const K = xv.w / a.dt2;
+    float const K = xv.w / c.dt2;
     B.h00 += K, B.h11 += K, B.h22 += K;
     B.g0 += K * (xv.x - xl.x);
     B.g1 += K * (xv.y - xl.y);
@@ -176,7 +161,7 @@ __device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 c
     float const a01 = B.h02 * B.h12 - B.h01 * B.h22;
     float const a02 = B.h01 * B.h12 - B.h02 * B.h11;
     float const det = B.h00 * a00 + B.h01 * a01 + B.h02 * a02;
-    if (!(fabsf(det) > a.detHZero))
+    if (!(fabsf(det) > c.detHZero))
         return make_float3(0.f, 0.f, 0.f);
     float const dscale = fabsf(B.h00 * B.h11 * B.h22);
     if (fabsf(det) < 1e-4f * dscale)
@@ -199,50 +184,150 @@ __device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 c
                        (a02 * B.g0 + a12 * B.g1 + a22 * B.g2) * inv);
 }
 
-// One colour of one VBD sweep. L lanes cooperate on each vertex: lane l of
-// the group handles incidences l, l+L, ... of the vertex's CSR row, the group
-// reduces its 9 partial sums with DPP, and the group's first lane applies the
-// Newton update in place (same-colour vertices share no tet).
-template <int L>
+// Incidence records, structure of arrays, row k of vertex v's incidences
+// (CSR rows through vtp, or fixed-stride rows of ellStride entries):
+//   ids[k] int4   (a, b, c, tet): the tet's other three vertices in local
+//                 order; tet = -1 marks an unused fixed-stride slot
+//   g0[k], g1[k], g2[k] float4: (ga, gb.x), (gb.yz, gc.xy), (gc.z, w*mu,
+//                 w*lambda, alpha) with ga, gb, gc the GP rows of a, b, c
+// Records are stored in sweep order, so a colour pass streams them.
+struct SweepArgs
+{
+    int const* __restrict__ vtp;
+    int4 const* __restrict__ ids;
+    float4 const* __restrict__ g0;
+    float4 const* __restrict__ g1;
+    float4 const* __restrict__ g2;
+    float4* __restrict__ xm;
+    float4 const* __restrict__ xtil;
+    float4 const* __restrict__ xt;
+    int vbeg;
+    int vend;
+    int xcdChunk;  // > 0: blocks per XCD for the XCD-aware block order
+    int ntStream;  // != 0: read the incidence records with non-temporal loads
+    int ellStride; // > 0: fixed-stride rows of this many records
+    StepConstants c;
+};
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Loads of the incidence stream, optionally with the non-temporal hint.
+__device__ __forceinline__ int4 LoadIds(int4 const* p, bool nt)
+{
+    if (nt)
+    {
+        v4i const r = __builtin_nontemporal_load(reinterpret_cast<v4i const*>(p));
+        return make_int4(r.x, r.y, r.z, r.w);
+    }
+    return *p;
+}
+
+__device__ __forceinline__ float4 LoadConst(float4 const* p, bool nt)
+{
+    if (nt)
+    {
+        v4f const r = __builtin_nontemporal_load(reinterpret_cast<v4f const*>(p));
+        return make_float4(r.x, r.y, r.z, r.w);
+    }
+    return *p;
+}
+
+// One colour of one VBD sweep (sim/vbd/Integrator.cpp:73-136,
+// gpu/impl/vbd/Kernels.cuh:148-233). L lanes cooperate on each vertex: lane l
+// of the group takes incidences l, l+L, ... of the vertex's row, first
+// issuing the loads of its first K incidences together, then accumulating.
+// The group reduces its 9 partial sums with DPP and its first lane applies the
+// Newton update in place: vertices of one colour share no tet, so none reads
+// what another vertex of the colour writes.
+template <int L, int K>
 __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 {
-    int const tid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    int block = static_cast<int>(blockIdx.x);
+    if (a.xcdChunk > 0)
+    {
+        // Blocks are dealt to the 8 XCDs round-robin; give each XCD a
+        // contiguous range of vertices so neighbouring vertices share its L2.
+        // The host pads the grid to 8 * xcdChunk blocks, which makes this a
+        // permutation of [0, gridDim.x).
+        block = (block & 7) * a.xcdChunk + (block >> 3);
+    }
+    int const tid = block * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
     int const lane = tid & (L - 1);
     int const v = a.vbeg + tid / L;
     bool const active = v < a.vend;
     int kb = 0, ke = 0;
+    float4 own = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 xl = own, xtv = own;
     if (active)
     {
-        kb = a.vtp[v];
-        ke = a.vtp[v + 1];
+        if (a.ellStride > 0)
+        {
+            // fixed stride: the incidence address needs no prefix load
+            kb = v * a.ellStride;
+            ke = kb + a.ellStride;
+        }
+        else
+        {
+            kb = a.vtp[v];
+            ke = a.vtp[v + 1];
+        }
+        own = a.xm[v];
+        if (lane == 0)
+        {
+            // issued now, used after the reduction
+            xl = a.xtil[v];
+            if (a.c.kD != 0.f)
+                xtv = a.xt[v];
+        }
     }
     Block B{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int k = kb + lane; k < ke; k += L)
+    bool const nt = a.ntStream != 0;
+    int4 I[K];
+    float4 q0[K], q1[K], q2[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
     {
-        int const ref = a.vtref[k];
-        TetRecord const T = a.tets[ref >> 2];
-        float4 const x0 = a.xm[T.v.x];
-        float4 const x1 = a.xm[T.v.y];
-        float4 const x2 = a.xm[T.v.z];
-        float4 const x3 = a.xm[T.v.w];
-        AccumulateTet(T, ref & 3, x0, x1, x2, x3, B);
+        int const k = kb + lane + j * L;
+        if (k < ke)
+        {
+            I[j] = LoadIds(a.ids + k, nt);
+            q0[j] = LoadConst(a.g0 + k, nt);
+            q1[j] = LoadConst(a.g1 + k, nt);
+            q2[j] = LoadConst(a.g2 + k, nt);
+        }
+        else
+            I[j] = make_int4(0, 0, 0, -1);
     }
-    B.h00 = GroupSum<L>(B.h00);
-    B.h01 = GroupSum<L>(B.h01);
-    B.h02 = GroupSum<L>(B.h02);
-    B.h11 = GroupSum<L>(B.h11);
-    B.h12 = GroupSum<L>(B.h12);
-    B.h22 = GroupSum<L>(B.h22);
-    B.g0 = GroupSum<L>(B.g0);
-    B.g1 = GroupSum<L>(B.g1);
-    B.g2 = GroupSum<L>(B.g2);
+    float4 pa[K], pb[K], pc[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+    {
+        if (I[j].w >= 0)
+        {
+            pa[j] = a.xm[I[j].x];
+            pb[j] = a.xm[I[j].y];
+            pc[j] = a.xm[I[j].z];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if (I[j].w >= 0)
+            AccumulateOwnFrame(q0[j], q1[j], q2[j], own, pa[j], pb[j], pc[j], B);
+    // incidences beyond K per lane (vertices of unusually high degree)
+    for (int k = kb + lane + K * L; k < ke; k += L)
+    {
+        int4 const J = LoadIds(a.ids + k, nt);
+        if (J.w < 0)
+            continue;
+        AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt), own,
+                           a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
+    }
+    GroupSum<L>(B);
     if (!active || lane != 0)
         return;
-    float4 const xv = a.xm[v];
-    float4 const xl = a.xtil[v];
-    float4 const xtv = (a.kD != 0.f) ? a.xt[v] : xv;
-    float3 const dx = NewtonStep(B, xv, xl, xtv, a);
-    a.xm[v] = make_float4(xv.x - dx.x, xv.y - dx.y, xv.z - dx.z, xv.w);
+    float3 const dx = NewtonStep(B, own, xl, (a.c.kD != 0.f) ? xtv : own, a.c);
+    a.xm[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
 }
 
 // Substep begin (gpu/impl/vbd/Integrator.cu:190-248; sim/vbd/Kernels.h:24-94):
@@ -308,13 +393,13 @@ __global__ __launch_bounds__(256) void SubstepEnd(float4 const* __restrict__ xm,
 }
 
 // FP64 energy reduction: out[0] += 1/2 m |v|^2, out[2] += -m a.x over vertices,
-// out[1] += w Psi(F) over tets. One atomic per block and term.
+// out[1] += w Psi(F) over tets. One atomic per block and term. blockDim = 256.
 __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm, float4 const* __restrict__ vel,
-                                                   float4 const* __restrict__ aext,
-                                                   TetRecord const* __restrict__ tets, int nV, int nT,
+                                                   float4 const* __restrict__ aext, int4 const* __restrict__ tetv,
+                                                   float4 const* __restrict__ tgp, int nV, int nT,
                                                    double* __restrict__ out)
 {
-    __shared__ double red[3][256 / 64];
+    __shared__ double red[3][4];
     double K = 0., U = 0., P = 0.;
     int const stride = static_cast<int>(gridDim.x * blockDim.x);
     for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < nV; i += stride)
@@ -326,13 +411,14 @@ __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm
     }
     for (int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); e < nT; e += stride)
     {
-        TetRecord const T = tets[e];
-        float4 const p0 = xm[T.v.x], p1 = xm[T.v.y], p2 = xm[T.v.z], p3 = xm[T.v.w];
-        double const r[3][3] = {{T.q0.x, T.q0.y, T.q0.z}, {T.q0.w, T.q1.x, T.q1.y}, {T.q1.z, T.q1.w, T.q2.x}};
+        int4 const t = tetv[e];
+        float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
+        float4 const p0 = xm[t.x], p1 = xm[t.y], p2 = xm[t.z], p3 = xm[t.w];
+        double const r[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
         double const d[3][3] = {{double(p1.x) - p0.x, double(p1.y) - p0.y, double(p1.z) - p0.z},
                                 {double(p2.x) - p0.x, double(p2.y) - p0.y, double(p2.z) - p0.z},
                                 {double(p3.x) - p0.x, double(p3.y) - p0.y, double(p3.z) - p0.z}};
-        double F[3][3]; // F[row][col]
+        double F[3][3]; // F[row][col] = sum_k d[k][row] r[k][col]
         double I2 = 0.;
         for (int rr = 0; rr < 3; ++rr)
             for (int c = 0; c < 3; ++c)
@@ -343,17 +429,16 @@ __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm
         double const J = F[0][0] * (F[1][1] * F[2][2] - F[2][1] * F[1][2]) -
                          F[0][1] * (F[1][0] * F[2][2] - F[2][0] * F[1][2]) +
                          F[0][2] * (F[1][0] * F[2][1] - F[2][0] * F[1][1]);
-        double const q = J - double(T.q2.w);
-        U += 0.5 * double(T.q2.y) * (I2 - 3.) + 0.5 * double(T.q2.z) * q * q;
+        double const q = J - double(q2.w);
+        U += 0.5 * double(q2.y) * (I2 - 3.) + 0.5 * double(q2.z) * q * q;
     }
-    // wave reduction then block reduction
     for (int off = 32; off > 0; off >>= 1)
     {
         K += __shfl_xor(K, off);
         U += __shfl_xor(U, off);
         P += __shfl_xor(P, off);
     }
-    int const w = threadIdx.x / 64;
+    int const w = static_cast<int>(threadIdx.x) / 64;
     if ((threadIdx.x & 63) == 0)
     {
         red[0][w] = K;
@@ -364,7 +449,7 @@ __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm
     if (threadIdx.x == 0)
     {
         double s0 = 0., s1 = 0., s2 = 0.;
-        for (int k = 0; k < static_cast<int>(blockDim.x / 64); ++k)
+        for (int k = 0; k < 4; ++k)
         {
             s0 += red[0][k];
             s1 += red[1][k];

@@ -171,6 +171,34 @@ class Integrator:
     lanes_per_vertex = property(_write_only("lanes_per_vertex"), _set_lanes,
                                 doc="Lanes cooperating on one vertex in the sweep kernel (0 = automatic)")
 
+    def set_sweep_config(self, lanes: int = 0, incidences_per_lane: int = 0, xcd_remap: bool = True,
+                         nontemporal: bool = False, fixed_stride: bool | None = None) -> None:
+        """Sweep-kernel shape: lanes per vertex (0 = auto), incidences prefetched per
+        lane (0 = auto, else 1..4), XCD-aware workgroup order, non-temporal stream
+        loads, fixed-stride (ELL) incidence rows (None = keep the current layout)."""
+        if fixed_stride is None:
+            fixed_stride = self.sweep_config()["fixed_stride"]
+        c = _native.SweepConfig(int(lanes), int(incidences_per_lane), int(bool(xcd_remap)), int(bool(nontemporal)),
+                                int(bool(fixed_stride)))
+        _native.check(self._lib.pbat_vbd_integrator_set_sweep_config(self._h, ctypes.byref(c)))
+
+    def sweep_config(self) -> dict:
+        """Sweep shape in effect (automatic choices resolved)."""
+        c = _native.SweepConfig()
+        _native.check(self._lib.pbat_vbd_integrator_get_sweep_config(self._h, ctypes.byref(c)))
+        return {"lanes_per_vertex": c.lanes_per_vertex, "incidences_per_lane": c.incidences_per_lane,
+                "xcd_remap": bool(c.xcd_remap), "nontemporal": bool(c.nontemporal),
+                "fixed_stride": bool(c.fixed_stride)}
+
+    def autotune(self, dt: float = 0.01, sweeps: int = 10):
+        """Pick the fastest sweep shape by timing full sweeps; state is unchanged.
+        Returns rows (lanes, incidences per lane, xcd remap, non-temporal, fixed stride, us per sweep)."""
+        t = np.zeros(192)
+        n = ctypes.c_int32()
+        _native.check(self._lib.pbat_vbd_integrator_autotune(self._h, float(dt), int(sweeps),
+                                                             _native.ptr(t, ctypes.c_double), ctypes.byref(n)))
+        return [tuple(r) for r in t[: 6 * min(n.value, 32)].reshape(-1, 6)]
+
     def _set_graph(self, value):
         _native.check(self._lib.pbat_vbd_integrator_set_graph_capture(self._h, int(bool(value))))
 

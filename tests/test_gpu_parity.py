@@ -91,11 +91,35 @@ def test_one_step_jittered_64_moving(gpu_device):
     check_step(gpu, ref, s.X, s.dt, s.iterations)
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
-def test_lanes_per_vertex(gpu_device, lanes):
+SHAPES = [(1, 0), (2, 0), (4, 0), (8, 0), (16, 0), (32, 0), (64, 0),
+          # prefetch depth below ceil(degree / lanes): the remaining incidences take the tail loop
+          (8, 1), (4, 2), (2, 4), (1, 4)]
+
+
+@pytest.mark.parametrize("fixed_stride", [False, True])
+@pytest.mark.parametrize("lanes,per_lane", SHAPES)
+def test_sweep_shapes(gpu_device, lanes, per_lane, fixed_stride):
     s = meshes.plain_gpu_scene(12, seed=3)
     _, gpu, ref = build_pair(s)
-    gpu.lanes_per_vertex = lanes
+    gpu.set_sweep_config(lanes, per_lane, xcd_remap=True, fixed_stride=fixed_stride)
+    cfg = gpu.sweep_config()
+    assert cfg["lanes_per_vertex"] == lanes and cfg["fixed_stride"] == fixed_stride
+    check_step(gpu, ref, s.X, s.dt, s.iterations)
+
+
+def test_nontemporal_and_no_xcd(gpu_device):
+    s = meshes.plain_gpu_scene(20, seed=8)
+    _, gpu, ref = build_pair(s)
+    gpu.set_sweep_config(32, 0, xcd_remap=False, nontemporal=True)
+    check_step(gpu, ref, s.X, s.dt, s.iterations)
+
+
+def test_autotune_keeps_state(gpu_device):
+    s = meshes.plain_gpu_scene(16, seed=6)
+    _, gpu, ref = build_pair(s)
+    rows = gpu.autotune(s.dt, sweeps=2)
+    assert len(rows) >= 4 and all(r[-1] > 0 for r in rows)
+    assert np.array_equal(gpu.x, s.X.astype(np.float32))
     check_step(gpu, ref, s.X, s.dt, s.iterations)
 
 
