@@ -182,6 +182,22 @@ int pbat_vbd_integrator_autotune(pbat_vbd_integrator* h, float dt, int32_t sweep
 /* Capture each Step's launch sequence in a hipGraph and replay it (default on). */
 int pbat_vbd_integrator_set_graph_capture(pbat_vbd_integrator* h, int32_t enabled);
 
+/* ------------------------------------------------------------------------- */
+/* Batched integrator (no reference counterpart)                             */
+/* ------------------------------------------------------------------------- */
+
+/* One integrator for `count` independent meshes (each a constructed Data, all
+ * sharing strategy, kD and detHZero). The meshes become one mesh with
+ * disconnected components: global colour c is the union of every mesh's
+ * colour c, so each mesh is swept in its own order and one launch per colour
+ * serves the batch. Every pbat_vbd_integrator_* entry point applies to the
+ * returned handle; positions and velocities are the meshes' 3 x nV_b matrices
+ * concatenated in batch order. */
+int pbat_vbd_batch_create(const pbat_vbd_data* meshes, int32_t count, pbat_vbd_integrator** out);
+/* First vertex of each mesh in the concatenated arrays: offsets[count + 1]
+ * (nullable to query *count first). A plain integrator reports one mesh. */
+int pbat_vbd_batch_vertex_offsets(pbat_vbd_integrator* h, int64_t* offsets, int32_t* count);
+
 /* Total mechanical energy terms reduced in FP64 on the device:
  * out = {1/2 sum m |v|^2, sum_e w_e Psi(F_e), -sum m aext . x}.
  * Counterpart of the FP64 objective reductions of

@@ -64,6 +64,8 @@ SIGNATURES = [
       p_f64, p_f64, p_f64, p_f64, p_i64, p_i64, p_i64, p_i64, p_i64, p_i64, p_i64, p_i64]),
     ("pbat_vbd_integrator_create", ctypes.c_int, [ctypes.POINTER(Data), ctypes.POINTER(ctypes.c_void_p)]),
     ("pbat_vbd_integrator_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("pbat_vbd_batch_create", ctypes.c_int, [ctypes.POINTER(Data), c_i32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("pbat_vbd_batch_vertex_offsets", ctypes.c_int, [ctypes.c_void_p, p_i64, ctypes.POINTER(c_i32)]),
     ("pbat_vbd_integrator_step", ctypes.c_int, [ctypes.c_void_p, c_f32, c_i32, c_i32]),
     ("pbat_vbd_integrator_traced_step", ctypes.c_int,
      [ctypes.c_void_p, c_f32, c_i32, c_i32, c_i32, ctypes.c_char_p]),

@@ -142,6 +142,7 @@ struct pbat_vbd_integrator
     std::vector<int> perm;       // internal -> original vertex index
     std::vector<int> iperm;      // original -> internal
     std::vector<float> massOrig; // lumped masses in original order (FP32)
+    std::vector<int64_t> meshOffsets; // batch: first vertex of each mesh (count + 1), empty otherwise
     std::vector<float> wgHost;   // quadrature weights (traces only)
     std::vector<float> lameHost; // (mu, lambda) per element (traces only)
 
@@ -917,6 +918,98 @@ void Autotune(pbat_vbd_integrator& h, float dt, int sweeps, double* timingsUs, i
         *nCandidates = count;
 }
 
+// A batch of independent meshes is one mesh with disconnected components:
+// vertex, element and incidence indices are offset per mesh, and global
+// colour c is the union of every mesh's colour c. Each mesh is therefore
+// swept in exactly the order its own integrator would use, and one launch
+// per colour serves the whole batch.
+struct UnionData
+{
+    std::vector<double> x, v, aext, m, GP, wg, lame;
+    std::vector<int64_t> E, GVGp, GVGe, GVGilocal, Pptr, Padj;
+    std::vector<int64_t> vertexOffsets;
+    pbat_vbd_data d{};
+};
+
+void BuildUnion(pbat_vbd_data const* meshes, int count, UnionData& U)
+{
+    if (!meshes || count < 1)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "a batch needs at least one mesh");
+    int64_t nV = 0, nT = 0, nInc = 0, nColors = 0;
+    for (int b = 0; b < count; ++b)
+    {
+        pbat_vbd_data const& m = meshes[b];
+        if (m.nV <= 0 || m.nT <= 0 || !m.GVGp || !m.Pptr || m.nPartitions < 0 ||
+            m.nPartitions > PBAT_VBD_MAX_COLORS)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid mesh " + std::to_string(b) + " in batch");
+        if (m.strategy != meshes[0].strategy || m.kD != meshes[0].kD || m.detHZero != meshes[0].detHZero)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
+                 "meshes of a batch must share the initialization strategy, kD and detHZero");
+        nV += m.nV;
+        nT += m.nT;
+        nInc += m.GVGp[m.nV];
+        nColors = std::max(nColors, m.nPartitions);
+    }
+    U.vertexOffsets.assign(static_cast<size_t>(count) + 1, 0);
+    U.GVGp.assign(1, 0);
+    U.x.reserve(3 * nV), U.v.reserve(3 * nV), U.aext.reserve(3 * nV), U.m.reserve(nV);
+    U.E.reserve(4 * nT), U.GP.reserve(12 * nT), U.wg.reserve(nT), U.lame.reserve(2 * nT);
+    U.GVGp.reserve(nV + 1), U.GVGe.reserve(nInc), U.GVGilocal.reserve(nInc);
+    int64_t vOff = 0, tOff = 0;
+    for (int b = 0; b < count; ++b)
+    {
+        pbat_vbd_data const& m = meshes[b];
+        if (!m.x || !m.v || !m.aext || !m.m || !m.E || !m.GP || !m.wg || !m.lame || !m.GVGe || !m.GVGilocal ||
+            (m.nPartitions > 0 && !m.Padj))
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null array in mesh " + std::to_string(b) + " of batch");
+        U.x.insert(U.x.end(), m.x, m.x + 3 * m.nV);
+        U.v.insert(U.v.end(), m.v, m.v + 3 * m.nV);
+        U.aext.insert(U.aext.end(), m.aext, m.aext + 3 * m.nV);
+        U.m.insert(U.m.end(), m.m, m.m + m.nV);
+        for (int64_t k = 0; k < 4 * m.nT; ++k)
+            U.E.push_back(m.E[k] + vOff);
+        U.GP.insert(U.GP.end(), m.GP, m.GP + 12 * m.nT);
+        U.wg.insert(U.wg.end(), m.wg, m.wg + m.nT);
+        U.lame.insert(U.lame.end(), m.lame, m.lame + 2 * m.nT);
+        int64_t const incBase = U.GVGp.back();
+        for (int64_t i = 0; i < m.nV; ++i)
+            U.GVGp.push_back(incBase + m.GVGp[i + 1]);
+        for (int64_t k = 0; k < m.GVGp[m.nV]; ++k)
+        {
+            U.GVGe.push_back(m.GVGe[k] + tOff);
+            U.GVGilocal.push_back(m.GVGilocal[k]);
+        }
+        vOff += m.nV;
+        tOff += m.nT;
+        U.vertexOffsets[b + 1] = vOff;
+    }
+    U.Pptr.assign(1, 0);
+    for (int64_t c = 0; c < nColors; ++c)
+    {
+        for (int b = 0; b < count; ++b)
+        {
+            pbat_vbd_data const& m = meshes[b];
+            if (c >= m.nPartitions)
+                continue;
+            for (int64_t k = m.Pptr[c]; k < m.Pptr[c + 1]; ++k)
+            {
+                if (m.Padj[k] < 0 || m.Padj[k] >= m.nV)
+                    Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "Padj entry out of range in mesh " + std::to_string(b));
+                U.Padj.push_back(m.Padj[k] + U.vertexOffsets[b]);
+            }
+        }
+        U.Pptr.push_back(static_cast<int64_t>(U.Padj.size()));
+    }
+    pbat_vbd_data& d = U.d;
+    d.nV = nV, d.nT = nT;
+    d.x = U.x.data(), d.v = U.v.data(), d.aext = U.aext.data(), d.m = U.m.data();
+    d.E = U.E.data(), d.GP = U.GP.data(), d.wg = U.wg.data(), d.lame = U.lame.data();
+    d.GVGp = U.GVGp.data(), d.GVGe = U.GVGe.data(), d.GVGilocal = U.GVGilocal.data();
+    // Padj may be empty (every vertex fixed); any non-null pointer serves then
+    d.nPartitions = nColors, d.Pptr = U.Pptr.data(), d.Padj = U.Padj.empty() ? U.Pptr.data() : U.Padj.data();
+    d.strategy = meshes[0].strategy, d.kD = meshes[0].kD, d.detHZero = meshes[0].detHZero;
+}
+
 pbat_vbd_integrator& Handle(pbat_vbd_integrator* h)
 {
     if (!h)
@@ -932,6 +1025,35 @@ using namespace pbat_vbd;
 extern "C" int pbat_vbd_integrator_create(const pbat_vbd_data* data, pbat_vbd_integrator** out)
 {
     return Guard([&] { Create(data, out); });
+}
+
+extern "C" int pbat_vbd_batch_create(const pbat_vbd_data* meshes, int32_t count, pbat_vbd_integrator** out)
+{
+    return Guard([&] {
+        if (!out)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null output handle");
+        UnionData U;
+        BuildUnion(meshes, count, U);
+        Create(&U.d, out);
+        (*out)->meshOffsets = U.vertexOffsets;
+    });
+}
+
+extern "C" int pbat_vbd_batch_vertex_offsets(pbat_vbd_integrator* h, int64_t* offsets, int32_t* count)
+{
+    return Guard([&] {
+        auto& H = Handle(h);
+        int const n = H.meshOffsets.empty() ? 1 : static_cast<int>(H.meshOffsets.size()) - 1;
+        if (count)
+            *count = n;
+        if (offsets)
+        {
+            if (H.meshOffsets.empty())
+                offsets[0] = 0, offsets[1] = H.nV;
+            else
+                std::copy(H.meshOffsets.begin(), H.meshOffsets.end(), offsets);
+        }
+    });
 }
 
 extern "C" int pbat_vbd_integrator_destroy(pbat_vbd_integrator* h)

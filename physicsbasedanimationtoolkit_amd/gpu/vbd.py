@@ -32,52 +32,61 @@ def _write_only(name):
     return getter
 
 
+def _marshal(data: Data):
+    """pbat_vbd_data view of a constructed Data, plus the arrays it points into."""
+    if not isinstance(data, Data):
+        raise TypeError("expected a pbat.sim.vbd.Data")
+    if data.accelerator != AccelerationStrategy.Base:
+        raise ValueError(
+            f"acceleration strategy {data.accelerator.name} is not available on this device path; "
+            "only the base VBD integrator is implemented")
+    if data.m.size == 0:
+        raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
+    nV, nT = data.x.shape[1], data.E.shape[1]
+    keep = {
+        "x": np.ascontiguousarray(data.x.T, dtype=np.float64),
+        "v": np.ascontiguousarray(data.v.T, dtype=np.float64),
+        "aext": np.ascontiguousarray(data.aext.T, dtype=np.float64),
+        "m": np.ascontiguousarray(data.m, dtype=np.float64),
+        "E": np.ascontiguousarray(data.E.T, dtype=np.int64),
+        "GP": np.ascontiguousarray(data.GP.T, dtype=np.float64),
+        "wg": np.ascontiguousarray(data.wg, dtype=np.float64),
+        "lame": np.ascontiguousarray(data.lame.T, dtype=np.float64),
+        "GVGp": np.ascontiguousarray(data.GVGp, dtype=np.int64),
+        "GVGe": np.ascontiguousarray(data.GVGe, dtype=np.int64),
+        "GVGilocal": np.ascontiguousarray(data.GVGilocal, dtype=np.int64),
+        "Pptr": np.ascontiguousarray(data.Pptr, dtype=np.int64),
+        "Padj": np.ascontiguousarray(data.Padj, dtype=np.int64),
+    }
+    for k in ("x", "v", "aext"):
+        if keep[k].shape != (nV, 3):
+            raise ValueError(f"data.{k} must be 3x{nV}")
+    if keep["Padj"].size == 0:
+        keep["Padj"] = np.zeros(1, dtype=np.int64)  # non-null pointer; no entries are read
+    d = _native.Data()
+    d.nV, d.nT = nV, nT
+    f64, i64 = ctypes.c_double, ctypes.c_int64
+    for k in ("x", "v", "aext", "m", "GP", "wg", "lame"):
+        setattr(d, k, _native.ptr(keep[k], f64))
+    for k in ("E", "GVGp", "GVGe", "GVGilocal", "Pptr", "Padj"):
+        setattr(d, k, _native.ptr(keep[k], i64))
+    d.nPartitions = max(keep["Pptr"].size - 1, 0)
+    d.strategy = int(data.strategy)
+    d.kD = float(data.kD)
+    d.detHZero = float(data.detH_zero)
+    return d, keep
+
+
 class Integrator:
     """VBD time integrator running on the current HIP device."""
 
     def __init__(self, data: Data):
-        if not isinstance(data, Data):
-            raise TypeError("Integrator expects a pbat.sim.vbd.Data")
-        if data.accelerator != AccelerationStrategy.Base:
-            raise ValueError(
-                f"acceleration strategy {data.accelerator.name} is not available on this device path; "
-                "only the base VBD integrator is implemented")
-        if data.m.size == 0:
-            raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
         self._lib = _native.lib()
-        nV, nT = data.x.shape[1], data.E.shape[1]
-        self._nV = nV
-        keep = {
-            "x": np.ascontiguousarray(data.x.T, dtype=np.float64),
-            "v": np.ascontiguousarray(data.v.T, dtype=np.float64),
-            "aext": np.ascontiguousarray(data.aext.T, dtype=np.float64),
-            "m": np.ascontiguousarray(data.m, dtype=np.float64),
-            "E": np.ascontiguousarray(data.E.T, dtype=np.int64),
-            "GP": np.ascontiguousarray(data.GP.T, dtype=np.float64),
-            "wg": np.ascontiguousarray(data.wg, dtype=np.float64),
-            "lame": np.ascontiguousarray(data.lame.T, dtype=np.float64),
-            "GVGp": np.ascontiguousarray(data.GVGp, dtype=np.int64),
-            "GVGe": np.ascontiguousarray(data.GVGe, dtype=np.int64),
-            "GVGilocal": np.ascontiguousarray(data.GVGilocal, dtype=np.int64),
-            "Pptr": np.ascontiguousarray(data.Pptr, dtype=np.int64),
-            "Padj": np.ascontiguousarray(data.Padj, dtype=np.int64),
-        }
-        for k in ("x", "v", "aext"):
-            if keep[k].shape != (nV, 3):
-                raise ValueError(f"data.{k} must be 3x{nV}")
-        d = _native.Data()
-        d.nV, d.nT = nV, nT
-        f64, i64 = ctypes.c_double, ctypes.c_int64
-        for k in ("x", "v", "aext", "m", "GP", "wg", "lame"):
-            setattr(d, k, _native.ptr(keep[k], f64))
-        for k in ("E", "GVGp", "GVGe", "GVGilocal", "Pptr", "Padj"):
-            setattr(d, k, _native.ptr(keep[k], i64))
-        d.nPartitions = max(keep["Pptr"].size - 1, 0)
-        d.strategy = int(data.strategy)
-        d.kD = float(data.kD)
-        d.detHZero = float(data.detH_zero)
+        d, keep = _marshal(data)
+        self._nV = d.nV
         handle = ctypes.c_void_p()
         _native.check(self._lib.pbat_vbd_integrator_create(ctypes.byref(d), ctypes.byref(handle)))
+        del keep
         self._h = handle
 
     def __del__(self):
@@ -230,3 +239,44 @@ class Integrator:
         _native.check(self._lib.pbat_vbd_integrator_sweep_timing(self._h, ctypes.byref(ms), ctypes.byref(n),
                                                                  int(bool(reset))))
         return ms.value, n.value
+
+
+class BatchIntegrator(Integrator):
+    """One integrator for many independent meshes (no reference counterpart).
+
+    The meshes are stepped together: one launch per colour covers the whole
+    batch, and each mesh is swept in the same order as its own ``Integrator``
+    would sweep it. ``x``/``v`` are the meshes' ``3 x nV_b`` matrices
+    concatenated in batch order; ``mesh_positions()`` splits them.
+    """
+
+    def __init__(self, datas):
+        self._lib = _native.lib()
+        datas = list(datas)
+        if not datas:
+            raise ValueError("BatchIntegrator needs at least one Data")
+        views = [_marshal(d) for d in datas]
+        arr = (_native.Data * len(views))(*[v[0] for v in views])
+        handle = ctypes.c_void_p()
+        _native.check(self._lib.pbat_vbd_batch_create(arr, len(views), ctypes.byref(handle)))
+        del views
+        self._h = handle
+        count = ctypes.c_int32()
+        _native.check(self._lib.pbat_vbd_batch_vertex_offsets(self._h, None, ctypes.byref(count)))
+        offs = np.zeros(count.value + 1, dtype=np.int64)
+        _native.check(self._lib.pbat_vbd_batch_vertex_offsets(self._h, _native.ptr(offs, ctypes.c_int64),
+                                                              ctypes.byref(count)))
+        self.offsets = offs
+        self._nV = int(offs[-1])
+
+    @property
+    def mesh_count(self) -> int:
+        return self.offsets.size - 1
+
+    def mesh_positions(self):
+        x = self.x
+        return [x[:, self.offsets[b]:self.offsets[b + 1]] for b in range(self.mesh_count)]
+
+    def mesh_velocities(self):
+        v = self.v
+        return [v[:, self.offsets[b]:self.offsets[b + 1]] for b in range(self.mesh_count)]
