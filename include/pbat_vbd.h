@@ -204,6 +204,63 @@ int pbat_vbd_batch_vertex_offsets(pbat_vbd_integrator* h, int64_t* offsets, int3
  * gpu/impl/vbd/TrustRegionIntegrator.cu:265-364 and sim/vbd/Integrator.cpp:138-166. */
 int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3]);
 
+/* ------------------------------------------------------------------------- */
+/* Differentiable rollouts (no reference counterpart)                        */
+/* ------------------------------------------------------------------------- */
+/* The reference has no gradients of its VBD step. These entry points give
+ * d L / d (per-tet log Young's modulus, v_0, x_0) of a loss L(x_T, v_T) of a
+ * rollout by the implicit-function theorem on each step's incremental
+ * potential (sim/vbd/Integrator.cpp:138-166: its gradient is zero at the
+ * step's minimiser). Per step, backwards:
+ *   solve (M / h^2 + d2U/dx2(x_{t+1})) y = xbar + vbar / h   (free vertices)
+ *   dL/dlogY_e -= y . (w_e dPsi_e/dx)
+ *   xbar <- M y / h^2 - vbar / h,  vbar <- M y / h
+ * Assumes one substep, no Rayleigh damping and a converged forward: with a
+ * fixed iteration count the adjoint is that of the exact minimiser. */
+
+enum pbat_vbd_adjoint_solver {
+    PBAT_VBD_ADJOINT_PCG = 0,          /* CG preconditioned by symmetric coloured block Gauss-Seidel */
+    PBAT_VBD_ADJOINT_GAUSS_SEIDEL = 1  /* coloured block Gauss-Seidel sweeps */
+};
+
+typedef struct pbat_vbd_adjoint_config {
+    int32_t solver;         /* pbat_vbd_adjoint_solver */
+    int32_t max_iterations; /* per linear solve */
+    float tolerance;        /* on |b - H y| / |b| */
+    int32_t check_every;    /* iterations between residual checks (host syncs) */
+} pbat_vbd_adjoint_config;
+
+typedef struct pbat_vbd_adjoint_stats {
+    int32_t solves;
+    int32_t total_iterations;
+    int32_t max_iterations;
+    float max_relative_residual;
+} pbat_vbd_adjoint_stats;
+
+/* Per-tet Young's modulus Y_e = exp(logY[e]) with each tet's Poisson ratio
+ * kept: (mu_e, lambda_e) are rescaled by Y_e / Y_e(construction). nT values. */
+int pbat_vbd_integrator_set_log_young_modulus(pbat_vbd_integrator* h, const float* logY, int64_t nT);
+int pbat_vbd_integrator_get_log_young_modulus(pbat_vbd_integrator* h, float* logY, int64_t nT);
+/* Step `steps` times (one substep each) from the current state, keeping the
+ * state every `checkpoint_every` steps for the backward pass. */
+int pbat_vbd_rollout_forward(pbat_vbd_integrator* h, float dt, int32_t iterations, int32_t steps,
+                             int32_t checkpoint_every);
+/* Gradients of L given dL/dx_T (3 x nV, required) and dL/dv_T (3 x nV,
+ * nullable = 0), original vertex order. Outputs (each nullable): dL/dlogY
+ * (nT), dL/dv_0 and dL/dx_0 (3 x nV; zero on Dirichlet vertices). Segments
+ * between checkpoints are recomputed; the integrator's state at the end of
+ * the rollout is restored afterwards. config nullable (PCG, 200, 1e-6, 5). */
+int pbat_vbd_rollout_backward(pbat_vbd_integrator* h, const float* dL_dx, const float* dL_dv, float* dL_dlogY,
+                              float* dL_dv0, float* dL_dx0, const pbat_vbd_adjoint_config* config,
+                              pbat_vbd_adjoint_stats* stats);
+/* out = (M / h^2 + d2U/dx2(x)) p on free vertices (0 on Dirichlet ones);
+ * x, p, out: 3 x nV. */
+int pbat_vbd_hessian_vector(pbat_vbd_integrator* h, const float* x, float dt, const float* p, float* out);
+/* Solve (M / h^2 + d2U/dx2(x)) y = b on free vertices with the adjoint's
+ * solver (b's Dirichlet rows ignored). x, b, y: 3 x nV. */
+int pbat_vbd_adjoint_solve(pbat_vbd_integrator* h, const float* x, float dt, const float* b, float* y,
+                           const pbat_vbd_adjoint_config* config, pbat_vbd_adjoint_stats* stats);
+
 /* Wait for all work queued on the handle's stream. */
 int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);
 

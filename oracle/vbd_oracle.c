@@ -794,3 +794,46 @@ void ora_energy(const ora_vbd* s, double out[3])
     out[1] = U;
     out[2] = P;
 }
+
+void ora_element_derivatives(const ora_vbd* s, const double* x, double* ge, double* He, double* psi)
+{
+    for (int64_t e = 0; e < s->nT; ++e)
+    {
+        const double* G = s->GP + 12 * e;
+        double const w = s->wg[e], mu = s->lame[2 * e], lambda = s->lame[2 * e + 1];
+        double F[9];
+        element_F(x, s->E + 4 * e, G, F);
+        if (psi)
+            psi[e] = w * ora_snh_eval(F, mu, lambda);
+        if (ge)
+        {
+            double gF[9];
+            ora_snh_grad(F, mu, lambda, gF);
+            /* dF(r,c)/dx_k[d] = delta_rd GP(k,c) */
+            for (int k = 0; k < 4; ++k)
+                for (int d = 0; d < 3; ++d)
+                {
+                    double acc = 0.;
+                    for (int c = 0; c < 3; ++c)
+                        acc += G[4 * c + k] * gF[3 * c + d];
+                    ge[12 * e + 3 * k + d] = w * acc;
+                }
+        }
+        if (He)
+        {
+            double HF[81];
+            ora_snh_hessian(F, mu, lambda, HF);
+            for (int k = 0; k < 4; ++k)
+                for (int a = 0; a < 3; ++a)
+                    for (int l = 0; l < 4; ++l)
+                        for (int b = 0; b < 3; ++b)
+                        {
+                            double acc = 0.;
+                            for (int c = 0; c < 3; ++c)
+                                for (int dd = 0; dd < 3; ++dd)
+                                    acc += G[4 * c + k] * G[4 * dd + l] * HF[9 * (3 * dd + b) + 3 * c + a];
+                            He[144 * e + 12 * (3 * k + a) + 3 * l + b] = w * acc;
+                        }
+        }
+    }
+}

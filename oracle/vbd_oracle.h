@@ -119,6 +119,12 @@ double ora_objective(const ora_vbd* s, const double* x, const double* xtilde, do
 void ora_objective_gradient(const ora_vbd* s, const double* x, const double* xtilde, double dt, double* grad);
 /* Total mechanical energy terms: out = {kinetic 1/2 sum m|v|^2, elastic sum w Psi(F), external -sum m aext.x} */
 void ora_energy(const ora_vbd* s, double out[3]);
+/* Per-element derivatives of w_e Psi_e at positions x (3*nV):
+ *   ge[12 e + 3 k + d]            = w_e dPsi_e / dx_k[d]   (fem/DeformationGradient.h:126-141)
+ *   He[144 e + 12 (3k+a) + 3l+b]  = w_e d2Psi_e / dx_k[a] dx_l[b]  (fem/DeformationGradient.h:231-253)
+ *   psi[e]                        = w_e Psi_e
+ * Any output may be NULL. */
+void ora_element_derivatives(const ora_vbd* s, const double* x, double* ge, double* He, double* psi);
 
 #ifdef __cplusplus
 }

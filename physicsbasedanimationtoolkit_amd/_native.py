@@ -55,6 +55,23 @@ class SweepConfig(ctypes.Structure):
     ]
 
 
+ADJOINT_PCG = 0
+ADJOINT_GAUSS_SEIDEL = 1
+
+
+class AdjointConfig(ctypes.Structure):
+    """``pbat_vbd_adjoint_config`` of include/pbat_vbd.h."""
+
+    _fields_ = [("solver", c_i32), ("max_iterations", c_i32), ("tolerance", c_f32), ("check_every", c_i32)]
+
+
+class AdjointStats(ctypes.Structure):
+    """``pbat_vbd_adjoint_stats`` of include/pbat_vbd.h."""
+
+    _fields_ = [("solves", c_i32), ("total_iterations", c_i32), ("max_iterations", c_i32),
+                ("max_relative_residual", c_f32)]
+
+
 # (name, restype, argtypes) of every exported entry point
 SIGNATURES = [
     ("pbat_vbd_last_error", ctypes.c_char_p, []),
@@ -91,6 +108,15 @@ SIGNATURES = [
     ("pbat_vbd_integrator_set_timing", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_sweep_timing", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(c_f64), p_i64, c_i32]),
+    ("pbat_vbd_integrator_set_log_young_modulus", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_integrator_get_log_young_modulus", ctypes.c_int, [ctypes.c_void_p, p_f32, c_i64]),
+    ("pbat_vbd_rollout_forward", ctypes.c_int, [ctypes.c_void_p, c_f32, c_i32, c_i32, c_i32]),
+    ("pbat_vbd_rollout_backward", ctypes.c_int,
+     [ctypes.c_void_p, p_f32, p_f32, p_f32, p_f32, p_f32, ctypes.POINTER(AdjointConfig),
+      ctypes.POINTER(AdjointStats)]),
+    ("pbat_vbd_hessian_vector", ctypes.c_int, [ctypes.c_void_p, p_f32, c_f32, p_f32, p_f32]),
+    ("pbat_vbd_adjoint_solve", ctypes.c_int,
+     [ctypes.c_void_p, p_f32, c_f32, p_f32, p_f32, ctypes.POINTER(AdjointConfig), ctypes.POINTER(AdjointStats)]),
 ]
 
 

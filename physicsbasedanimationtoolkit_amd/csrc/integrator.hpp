@@ -1,0 +1,222 @@
+// Internal declarations shared by the translation units of libpbat_vbd.so:
+// device arrays, the integrator handle and the host helpers that the
+// forward integrator (vbd_integrator.hip) and the adjoint (vbd_adjoint.hip)
+// both use. Not part of the C ABI.
+#pragma once
+
+#include "pbat_vbd.h"
+#include "status.hpp"
+#include "vbd_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pbat_vbd {
+
+#define PBAT_HIP(call)                                                                                            \
+    do                                                                                                            \
+    {                                                                                                             \
+        hipError_t const _e = (call);                                                                             \
+        if (_e != hipSuccess)                                                                                     \
+            Fail(PBAT_VBD_ERROR_HIP, std::string(#call) + ": " + hipGetErrorString(_e));                          \
+    } while (0)
+
+template <class T>
+struct DeviceArray
+{
+    T* ptr = nullptr;
+    size_t n = 0;
+    DeviceArray() = default;
+    DeviceArray(DeviceArray const&) = delete;
+    DeviceArray& operator=(DeviceArray const&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept : ptr(o.ptr), n(o.n) { o.ptr = nullptr, o.n = 0; }
+    DeviceArray& operator=(DeviceArray&& o) noexcept
+    {
+        if (this != &o)
+        {
+            reset();
+            ptr = o.ptr, n = o.n;
+            o.ptr = nullptr, o.n = 0;
+        }
+        return *this;
+    }
+    ~DeviceArray() { reset(); }
+    void reset()
+    {
+        if (ptr)
+            (void)hipFree(ptr);
+        ptr = nullptr;
+        n = 0;
+    }
+    void allocate(size_t count)
+    {
+        reset();
+        if (count == 0)
+            return;
+        hipError_t const e = hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T));
+        if (e != hipSuccess)
+        {
+            ptr = nullptr;
+            Fail(PBAT_VBD_ERROR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    void upload(std::vector<T> const& h, hipStream_t s)
+    {
+        allocate(h.size());
+        if (!h.empty())
+            PBAT_HIP(hipMemcpyAsync(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// Incidence records of the sweep (structure of arrays, see SweepArgs).
+struct IncidenceRecords
+{
+    DeviceArray<int4> ids;
+    DeviceArray<float4> g0, g1, g2;
+    void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(); }
+};
+
+struct HostIncidenceRecords
+{
+    std::vector<int4> ids;
+    std::vector<float4> g0, g1, g2;
+    void resize(size_t n)
+    {
+        ids.assign(n, make_int4(0, 0, 0, -1));
+        g0.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
+        g1 = g0;
+        g2 = g0;
+    }
+    void upload(IncidenceRecords& d, hipStream_t s) const
+    {
+        d.ids.upload(ids, s);
+        d.g0.upload(g0, s);
+        d.g1.upload(g1, s);
+        d.g2.upload(g2, s);
+    }
+};
+
+// Rollout checkpoints and adjoint work vectors (vbd_adjoint.hip).
+struct RolloutState
+{
+    bool valid = false;
+    float dt = 0.f;
+    int iterations = 0, steps = 0, every = 1;
+    std::vector<DeviceArray<float4>> ckptX, ckptV, ckptVprev; // every `every` steps
+    std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
+    DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
+    DeviceArray<double> scal;                                 // device scalars of the solvers
+    DeviceArray<double> grad;                                 // d L / d log Y per tet
+    DeviceArray<float> logY;                                  // parameter upload buffer
+};
+
+struct GraphKey
+{
+    float dt = 0.f;
+    int iterations = -1, substeps = -1, strategy = -1, lanes = -1, block = -1, perLane = -1;
+    bool xcd = false, nt = false, ell = false;
+    float kD = 0.f, detHZero = 0.f;
+    bool operator==(GraphKey const& o) const
+    {
+        return dt == o.dt && iterations == o.iterations && substeps == o.substeps && strategy == o.strategy &&
+               lanes == o.lanes && block == o.block && perLane == o.perLane && xcd == o.xcd && nt == o.nt &&
+               ell == o.ell && kD == o.kD && detHZero == o.detHZero;
+    }
+};
+
+} // namespace pbat_vbd
+
+struct pbat_vbd_integrator
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t nV = 0, nT = 0, nIncidences = 0, nUpdated = 0, maxDegree = 0;
+    std::vector<int> colorBegin; // internal vertex ranges per colour (size nColours + 1)
+    std::vector<int> perm;       // internal -> original vertex index
+    std::vector<int> iperm;      // original -> internal
+    std::vector<float> massOrig; // lumped masses in original order (FP32)
+    std::vector<int64_t> meshOffsets; // batch: first vertex of each mesh (count + 1), empty otherwise
+    std::vector<float> wgHost;   // quadrature weights (traces only)
+    std::vector<float> lameHost; // (mu, lambda) per element (traces only)
+
+    pbat_vbd::DeviceArray<float4> xm, xtil, xt, vel, vprev, aext;
+    pbat_vbd::DeviceArray<float4> tgp; // 3 per tet
+    pbat_vbd::DeviceArray<int4> tetv;
+    pbat_vbd::IncidenceRecords rows, ell; // CSR rows, fixed-stride rows
+    int ellStride = 0;   // > 0 when the fixed-stride rows exist
+    bool useEll = false; // sweep reads the fixed-stride rows
+    pbat_vbd::DeviceArray<int> vtp;
+    pbat_vbd::DeviceArray<double> energy;
+    // record slot of (tet, local vertex) in the CSR and fixed-stride rows (-1: not swept)
+    pbat_vbd::DeviceArray<int4> slotsRows, slotsEll;
+    pbat_vbd::DeviceArray<float2> matBase; // (w mu / Y, w lambda / Y) per tet
+    std::vector<float> logY0;              // log Young's modulus per tet at construction
+    std::vector<float> logYCurrent;        // last value set (log Young's modulus per tet)
+    pbat_vbd::RolloutState rollout;
+
+    int strategy = PBAT_VBD_INIT_ADAPTIVE_PBAT;
+    float kD = 0.f;
+    float detHZero = 1e-7f;
+    int blockSize = 256;
+    int lanes = 0;         // 0 = automatic
+    int perLane = 0;       // incidences prefetched per lane, 0 = automatic
+    bool xcdRemap = true;  // XCD-aware block order
+    bool ntStream = false; // non-temporal loads of the incidence stream
+    bool tuneNontemporal = true; // autotune also tries non-temporal stream loads
+    bool useGraph = true;
+    float worldMin[3] = {0.f, 0.f, 0.f}, worldMax[3] = {0.f, 0.f, 0.f};
+
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    pbat_vbd::GraphKey graphKey;
+
+    bool timing = false;
+    int graphSweeps = 0; // sweep launches in the captured graph
+    std::vector<hipEvent_t> events; // pairs (start, stop)
+    std::vector<int> pendingLaunches; // sweep launches between each pair
+    size_t eventsUsed = 0;
+    double timedMs = 0.;
+    int64_t timedLaunches = 0;
+
+    ~pbat_vbd_integrator()
+    {
+        if (graphExec)
+            (void)hipGraphExecDestroy(graphExec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : events)
+            (void)hipEventDestroy(e);
+        if (stream)
+            (void)hipStreamSynchronize(stream);
+        xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
+        tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
+        slotsRows.reset(), slotsEll.reset(), matBase.reset();
+        rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
+        if (stream)
+            (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace pbat_vbd {
+
+// Sweep shape in effect (automatic choices resolved).
+int EffectiveLanes(pbat_vbd_integrator const& h);
+int EffectivePerLane(pbat_vbd_integrator const& h);
+// Launch one colour's sweep over nVerts vertices with the current shape.
+void LaunchSweep(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts);
+device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2);
+// One forward Step (gpu/impl/vbd/Integrator.cu:82-103): graph replay or eager launches.
+void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps);
+void InvalidateGraph(pbat_vbd_integrator& h);
+// Host <-> device copies of 3 x nV column-major matrices (original vertex order)
+// into / out of float4 arrays (internal order); w components from wsrc (or 0).
+void UploadVec(pbat_vbd_integrator& h, float4* dst, float const* src, float const* wsrc);
+void DownloadVec(pbat_vbd_integrator& h, float4 const* src, float* dst);
+void CheckVertexArray(pbat_vbd_integrator const& h, void const* p, int64_t n, char const* what);
+pbat_vbd_integrator& Handle(pbat_vbd_integrator* h);
+
+} // namespace pbat_vbd

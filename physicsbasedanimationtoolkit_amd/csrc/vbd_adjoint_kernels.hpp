@@ -1,0 +1,399 @@
+// Device kernels of the VBD adjoint (implicit-function-theorem gradient of a
+// rollout). The linear systems are H y = b with
+//   H = M / h^2 + sum_e w_e d2Psi_e/dx2   (the forward's scaling)
+// evaluated at a step's minimiser x*, over the updated (non-Dirichlet)
+// vertices, which are the internal range [0, nUpdated). Vectors are float4
+// arrays in internal vertex order with zeros on Dirichlet vertices.
+#pragma once
+
+#include "vbd_kernels.hpp"
+
+namespace pbat_vbd::device {
+
+// Directional derivative of one incidence's elastic gradient: with
+// dF = sum_k (p_k - p_i) g_k^T (the own frame, as AccumulateOwnFrame),
+//   (H_e p)_i = [w mu dF + w lambda (cof F : dF) cof F + w lambda (J - alpha) dcof(F)[dF]] g
+// where dcof's columns are df1 x f2 + f1 x df2, df2 x f0 + f2 x df0,
+// df0 x f1 + f0 x df1. Adds the block-diagonal part w (mu |g|^2 I + lambda c c^T)
+// into D when kDiag.
+template <bool kDiag>
+__device__ __forceinline__ void AccumulateHessVec(float4 const& q0, float4 const& q1, float4 const& q2,
+                                                  float4 const& own, float4 const& pa, float4 const& pb,
+                                                  float4 const& pc, float4 const& yo, float4 const& ya,
+                                                  float4 const& yb, float4 const& yc, float3& hv, Block& D)
+{
+    float const gax = q0.x, gay = q0.y, gaz = q0.z;
+    float const gbx = q0.w, gby = q1.x, gbz = q1.y;
+    float const gcx = q1.z, gcy = q1.w, gcz = q2.x;
+    float const wmu = q2.y, wlam = q2.z, alpha = q2.w;
+    // F
+    float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
+    float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
+    float const dcx = pc.x - own.x, dcy = pc.y - own.y, dcz = pc.z - own.z;
+    float const f0x = dax * gax + dbx * gbx + dcx * gcx, f0y = day * gax + dby * gbx + dcy * gcx,
+                f0z = daz * gax + dbz * gbx + dcz * gcx;
+    float const f1x = dax * gay + dbx * gby + dcx * gcy, f1y = day * gay + dby * gby + dcy * gcy,
+                f1z = daz * gay + dbz * gby + dcz * gcy;
+    float const f2x = dax * gaz + dbx * gbz + dcx * gcz, f2y = day * gaz + dby * gbz + dcy * gcz,
+                f2z = daz * gaz + dbz * gbz + dcz * gcz;
+    // dF
+    float const eax = ya.x - yo.x, eay = ya.y - yo.y, eaz = ya.z - yo.z;
+    float const ebx = yb.x - yo.x, eby = yb.y - yo.y, ebz = yb.z - yo.z;
+    float const ecx = yc.x - yo.x, ecy = yc.y - yo.y, ecz = yc.z - yo.z;
+    float const h0x = eax * gax + ebx * gbx + ecx * gcx, h0y = eay * gax + eby * gbx + ecy * gcx,
+                h0z = eaz * gax + ebz * gbx + ecz * gcx;
+    float const h1x = eax * gay + ebx * gby + ecx * gcy, h1y = eay * gay + eby * gby + ecy * gcy,
+                h1z = eaz * gay + ebz * gby + ecz * gcy;
+    float const h2x = eax * gaz + ebx * gbz + ecx * gcz, h2y = eay * gaz + eby * gbz + ecy * gcz,
+                h2z = eaz * gaz + ebz * gbz + ecz * gcz;
+    // cofactor columns A = f1 x f2, B = f2 x f0, C = f0 x f1
+    float const Ax = f1y * f2z - f1z * f2y, Ay = f1z * f2x - f1x * f2z, Az = f1x * f2y - f1y * f2x;
+    float const Bx = f2y * f0z - f2z * f0y, By = f2z * f0x - f2x * f0z, Bz = f2x * f0y - f2y * f0x;
+    float const Cx = f0y * f1z - f0z * f1y, Cy = f0z * f1x - f0x * f1z, Cz = f0x * f1y - f0y * f1x;
+    float const J = f0x * Ax + f0y * Ay + f0z * Az;
+    float const dJ = Ax * h0x + Ay * h0y + Az * h0z + Bx * h1x + By * h1y + Bz * h1z + Cx * h2x + Cy * h2y +
+                     Cz * h2z;
+    // d(cof) columns
+    float const dAx = (h1y * f2z - h1z * f2y) + (f1y * h2z - f1z * h2y);
+    float const dAy = (h1z * f2x - h1x * f2z) + (f1z * h2x - f1x * h2z);
+    float const dAz = (h1x * f2y - h1y * f2x) + (f1x * h2y - f1y * h2x);
+    float const dBx = (h2y * f0z - h2z * f0y) + (f2y * h0z - f2z * h0y);
+    float const dBy = (h2z * f0x - h2x * f0z) + (f2z * h0x - f2x * h0z);
+    float const dBz = (h2x * f0y - h2y * f0x) + (f2x * h0y - f2y * h0x);
+    float const dCx = (h0y * f1z - h0z * f1y) + (f0y * h1z - f0z * h1y);
+    float const dCy = (h0z * f1x - h0x * f1z) + (f0z * h1x - f0x * h1z);
+    float const dCz = (h0x * f1y - h0y * f1x) + (f0x * h1y - f0y * h1x);
+    float const gx = -(gax + gbx + gcx), gy = -(gay + gby + gcy), gz = -(gaz + gbz + gcz);
+    // cof(F) g, dcof g, dF g
+    float const kx = gx * Ax + gy * Bx + gz * Cx, ky = gx * Ay + gy * By + gz * Cy, kz = gx * Az + gy * Bz + gz * Cz;
+    float const lx = gx * dAx + gy * dBx + gz * dCx, ly = gx * dAy + gy * dBy + gz * dCy,
+                lz = gx * dAz + gy * dBz + gz * dCz;
+    float const mx = h0x * gx + h1x * gy + h2x * gz, my = h0y * gx + h1y * gy + h2y * gz,
+                mz = h0z * gx + h1z * gy + h2z * gz;
+    float const s1 = wlam * dJ, s2 = wlam * (J - alpha);
+    hv.x += wmu * mx + s1 * kx + s2 * lx;
+    hv.y += wmu * my + s1 * ky + s2 * ly;
+    hv.z += wmu * mz + s1 * kz + s2 * lz;
+    if constexpr (kDiag)
+    {
+        float const dg = wmu * (gx * gx + gy * gy + gz * gz);
+        float const lkx = wlam * kx, lky = wlam * ky, lkz = wlam * kz;
+        D.h00 += dg + lkx * kx;
+        D.h11 += dg + lky * ky;
+        D.h22 += dg + lkz * kz;
+        D.h01 += lkx * ky;
+        D.h02 += lkx * kz;
+        D.h12 += lky * kz;
+    }
+}
+
+struct AdjointArgs
+{
+    int const* __restrict__ vtp;
+    int4 const* __restrict__ ids;
+    float4 const* __restrict__ g0;
+    float4 const* __restrict__ g1;
+    float4 const* __restrict__ g2;
+    float4 const* __restrict__ xs; // minimiser positions (with lumped mass in .w)
+    float4* __restrict__ y;        // unknowns (Gauss-Seidel: updated in place) / vector multiplied
+    float4 const* __restrict__ b;  // right-hand side (Gauss-Seidel)
+    float4* __restrict__ out;      // H y (Hessian-vector product)
+    int vbeg;
+    int vend;
+    int xcdChunk;
+    int ellStride;
+    float invDt2; // 1 / h^2
+};
+
+// kGaussSeidel: one colour of block Gauss-Seidel on H y = b,
+//   y_i += D_i^{-1} (b_i - (H y)_i),  D_i = m_i / h^2 I + sum_e w_e (mu |g|^2 I + lambda c c^T).
+// Otherwise out_i = (H y)_i for the vertices [vbeg, vend).
+template <int L, int K, bool kGaussSeidel>
+__global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
+{
+    int block = static_cast<int>(blockIdx.x);
+    if (a.xcdChunk > 0)
+        block = (block & 7) * a.xcdChunk + (block >> 3);
+    int const tid = block * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    int const lane = tid & (L - 1);
+    int const v = a.vbeg + tid / L;
+    bool const active = v < a.vend;
+    int kb = 0, ke = 0;
+    float4 own = make_float4(0.f, 0.f, 0.f, 0.f), yo = own, bo = own;
+    if (active)
+    {
+        if (a.ellStride > 0)
+        {
+            kb = v * a.ellStride;
+            ke = kb + a.ellStride;
+        }
+        else
+        {
+            kb = a.vtp[v];
+            ke = a.vtp[v + 1];
+        }
+        own = a.xs[v];
+        yo = a.y[v];
+        if (kGaussSeidel && lane == 0)
+            bo = a.b[v];
+    }
+    float3 hv = make_float3(0.f, 0.f, 0.f);
+    Block D{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int4 I[K];
+    float4 q0[K], q1[K], q2[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+    {
+        int const k = kb + lane + j * L;
+        if (k < ke)
+        {
+            I[j] = a.ids[k];
+            q0[j] = a.g0[k];
+            q1[j] = a.g1[k];
+            q2[j] = a.g2[k];
+        }
+        else
+            I[j] = make_int4(0, 0, 0, -1);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+    {
+        if (I[j].w >= 0)
+            AccumulateHessVec<kGaussSeidel>(q0[j], q1[j], q2[j], own, a.xs[I[j].x], a.xs[I[j].y], a.xs[I[j].z],
+                                            yo, a.y[I[j].x], a.y[I[j].y], a.y[I[j].z], hv, D);
+    }
+    for (int k = kb + lane + K * L; k < ke; k += L)
+    {
+        int4 const J = a.ids[k];
+        if (J.w < 0)
+            continue;
+        AccumulateHessVec<kGaussSeidel>(a.g0[k], a.g1[k], a.g2[k], own, a.xs[J.x], a.xs[J.y], a.xs[J.z], yo,
+                                        a.y[J.x], a.y[J.y], a.y[J.z], hv, D);
+    }
+    hv.x = GroupSum<L>(hv.x);
+    hv.y = GroupSum<L>(hv.y);
+    hv.z = GroupSum<L>(hv.z);
+    if constexpr (kGaussSeidel)
+        GroupSum<L>(D);
+    if (!active || lane != 0)
+        return;
+    float const K0 = own.w * a.invDt2;
+    float const Hx = hv.x + K0 * yo.x, Hy = hv.y + K0 * yo.y, Hz = hv.z + K0 * yo.z;
+    if constexpr (!kGaussSeidel)
+    {
+        a.out[v] = make_float4(Hx, Hy, Hz, 0.f);
+    }
+    else
+    {
+        float const rx = bo.x - Hx, ry = bo.y - Hy, rz = bo.z - Hz;
+        double const d00 = D.h00 + K0, d11 = D.h11 + K0, d22 = D.h22 + K0;
+        double const d01 = D.h01, d02 = D.h02, d12 = D.h12;
+        double const c00 = d11 * d22 - d12 * d12, c01 = d02 * d12 - d01 * d22, c02 = d01 * d12 - d02 * d11;
+        double const c11 = d00 * d22 - d02 * d02, c12 = d01 * d02 - d00 * d12, c22 = d00 * d11 - d01 * d01;
+        double const det = d00 * c00 + d01 * c01 + d02 * c02;
+        if (!(fabs(det) > 0.))
+            return;
+        double const inv = 1. / det;
+        a.y[v] = make_float4(yo.x + static_cast<float>((c00 * rx + c01 * ry + c02 * rz) * inv),
+                             yo.y + static_cast<float>((c01 * rx + c11 * ry + c12 * rz) * inv),
+                             yo.z + static_cast<float>((c02 * rx + c12 * ry + c22 * rz) * inv), 0.f);
+    }
+}
+
+// d L / d log Y_e -= (w_e P_e) : dF_e(y) at the minimiser, for every tet
+// (mu, lambda proportional to Y, so d(grad U)/d log Y_e = w_e dPsi_e/dx).
+// Evaluated in FP64 from the FP32 positions: near rest the stress
+// P = mu F + lambda (J - alpha) cof F is a small difference of O(Y) terms
+// (P(I) = 0), which FP32 would resolve only to ~1e-7 / strain.
+__global__ __launch_bounds__(256) void LogYoungGradient(float4 const* __restrict__ xs, float4 const* __restrict__ y,
+                                                        int4 const* __restrict__ tetv,
+                                                        float4 const* __restrict__ tgp, int nT,
+                                                        double* __restrict__ grad)
+{
+    int const e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= nT)
+        return;
+    int4 const t = tetv[e];
+    float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
+    float4 const x0 = xs[t.x], x1 = xs[t.y], x2 = xs[t.z], x3 = xs[t.w];
+    float4 const y0 = y[t.x], y1 = y[t.y], y2 = y[t.z], y3 = y[t.w];
+    double const r[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+    double const d[3][3] = {{double(x1.x) - x0.x, double(x1.y) - x0.y, double(x1.z) - x0.z},
+                            {double(x2.x) - x0.x, double(x2.y) - x0.y, double(x2.z) - x0.z},
+                            {double(x3.x) - x0.x, double(x3.y) - x0.y, double(x3.z) - x0.z}};
+    double const u[3][3] = {{double(y1.x) - y0.x, double(y1.y) - y0.y, double(y1.z) - y0.z},
+                            {double(y2.x) - y0.x, double(y2.y) - y0.y, double(y2.z) - y0.z},
+                            {double(y3.x) - y0.x, double(y3.y) - y0.y, double(y3.z) - y0.z}};
+    double F[3][3], dF[3][3]; // [row][col]
+    for (int rr = 0; rr < 3; ++rr)
+        for (int c = 0; c < 3; ++c)
+        {
+            F[rr][c] = d[0][rr] * r[0][c] + d[1][rr] * r[1][c] + d[2][rr] * r[2][c];
+            dF[rr][c] = u[0][rr] * r[0][c] + u[1][rr] * r[1][c] + u[2][rr] * r[2][c];
+        }
+    // cofactor matrix (column c = f_{c+1} x f_{c+2})
+    double cof[3][3];
+    for (int c = 0; c < 3; ++c)
+    {
+        int const c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        cof[0][c] = F[1][c1] * F[2][c2] - F[2][c1] * F[1][c2];
+        cof[1][c] = F[2][c1] * F[0][c2] - F[0][c1] * F[2][c2];
+        cof[2][c] = F[0][c1] * F[1][c2] - F[1][c1] * F[0][c2];
+    }
+    double const J = F[0][0] * cof[0][0] + F[1][0] * cof[1][0] + F[2][0] * cof[2][0];
+    double const wmu = q2.y, s = double(q2.z) * (J - double(q2.w));
+    double acc = 0.;
+    for (int rr = 0; rr < 3; ++rr)
+        for (int c = 0; c < 3; ++c)
+            acc += (wmu * F[rr][c] + s * cof[rr][c]) * dF[rr][c];
+    grad[e] -= acc;
+}
+
+// rhs = xbar + vbar / h on updated vertices, 0 elsewhere
+__global__ __launch_bounds__(256) void AdjointRhs(float4 const* __restrict__ xbar, float4 const* __restrict__ vbar,
+                                                  float4* __restrict__ rhs, int nUpd, int nV, float invDt)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= nV)
+        return;
+    float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < nUpd)
+    {
+        float4 const x = xbar[i], v = vbar[i];
+        r = make_float4(x.x + v.x * invDt, x.y + v.y * invDt, x.z + v.z * invDt, 0.f);
+    }
+    rhs[i] = r;
+}
+
+// xt~bar = m y / h^2; xbar <- xt~bar - vbar / h; vbar <- h xt~bar (updated vertices; 0 elsewhere)
+__global__ __launch_bounds__(256) void AdjointStepBack(float4 const* __restrict__ xs, float4 const* __restrict__ y,
+                                                       float4* __restrict__ xbar, float4* __restrict__ vbar, int nUpd,
+                                                       int nV, float dt)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= nV)
+        return;
+    float4 xb = make_float4(0.f, 0.f, 0.f, 0.f), vb = xb;
+    if (i < nUpd)
+    {
+        float const s = xs[i].w / (dt * dt);
+        float4 const yi = y[i], vo = vbar[i];
+        float const tx = s * yi.x, ty = s * yi.y, tz = s * yi.z;
+        xb = make_float4(tx - vo.x / dt, ty - vo.y / dt, tz - vo.z / dt, 0.f);
+        vb = make_float4(dt * tx, dt * ty, dt * tz, 0.f);
+    }
+    xbar[i] = xb;
+    vbar[i] = vb;
+}
+
+// ---- conjugate-gradient vector kernels with scalars kept on the device ----
+
+// acc[slot] += sum_i a_i . b_i over i < n (xyz), FP64; blockDim = 256
+__global__ __launch_bounds__(256) void Dot3(float4 const* __restrict__ a, float4 const* __restrict__ b, int n,
+                                            double* __restrict__ acc)
+{
+    __shared__ double red[4];
+    double s = 0.;
+    int const stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride)
+    {
+        float4 const u = a[i], w = b[i];
+        s += double(u.x) * w.x + double(u.y) * w.y + double(u.z) * w.z;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(acc, red[0] + red[1] + red[2] + red[3]);
+}
+
+// y += alpha p; r -= alpha q with alpha = num / den (device scalars)
+__global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* __restrict__ r,
+                                                float4 const* __restrict__ p, float4 const* __restrict__ q, int n,
+                                                double const* __restrict__ num, double const* __restrict__ den)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    double const d = *den;
+    float const alpha = (d != 0.) ? static_cast<float>(*num / d) : 0.f;
+    float4 const pi = p[i], qi = q[i];
+    float4 yi = y[i], ri = r[i];
+    yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
+    ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
+    y[i] = yi;
+    r[i] = ri;
+}
+
+// p = z + beta p with beta = num / den (device scalars)
+__global__ __launch_bounds__(256) void CgDirection(float4* __restrict__ p, float4 const* __restrict__ z, int n,
+                                                   double const* __restrict__ num, double const* __restrict__ den)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    double const d = *den;
+    float const beta = (d != 0.) ? static_cast<float>(*num / d) : 0.f;
+    float4 const zi = z[i];
+    float4 pi = p[i];
+    p[i] = make_float4(zi.x + beta * pi.x, zi.y + beta * pi.y, zi.z + beta * pi.z, 0.f);
+}
+
+// dst = a - b (xyz), n entries
+__global__ __launch_bounds__(256) void Sub4(float4* __restrict__ dst, float4 const* __restrict__ a,
+                                            float4 const* __restrict__ b, int n)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 const u = a[i], w = b[i];
+    dst[i] = make_float4(u.x - w.x, u.y - w.y, u.z - w.z, 0.f);
+}
+
+__global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        dst[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Rescale per-tet Lame coefficients to Young's moduli exp(logY) (nu fixed):
+// w*mu and w*lambda of the tet constants and of its incidence records.
+__global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ logY, float2 const* __restrict__ base,
+                                                   float4* __restrict__ tgp, int4 const* __restrict__ slots,
+                                                   float4* __restrict__ g2, int4 const* __restrict__ slotsEll,
+                                                   float4* __restrict__ g2Ell, int nT)
+{
+    int const e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= nT)
+        return;
+    float const Y = expf(logY[e]);
+    float2 const b = base[e]; // (w mu / Y, w lambda / Y)
+    float const wmu = Y * b.x, wlam = Y * b.y;
+    float4 t = tgp[3 * e + 2];
+    t.y = wmu, t.z = wlam;
+    tgp[3 * e + 2] = t;
+    int const s[4] = {slots[e].x, slots[e].y, slots[e].z, slots[e].w};
+    for (int k = 0; k < 4; ++k)
+        if (s[k] >= 0)
+        {
+            float4 r = g2[s[k]];
+            r.y = wmu, r.z = wlam;
+            g2[s[k]] = r;
+        }
+    if (g2Ell)
+    {
+        int const t4[4] = {slotsEll[e].x, slotsEll[e].y, slotsEll[e].z, slotsEll[e].w};
+        for (int k = 0; k < 4; ++k)
+            if (t4[k] >= 0)
+            {
+                float4 r = g2Ell[t4[k]];
+                r.y = wmu, r.z = wlam;
+                g2Ell[t4[k]] = r;
+            }
+    }
+}
+
+} // namespace pbat_vbd::device

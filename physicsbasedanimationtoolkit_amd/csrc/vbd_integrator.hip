@@ -39,165 +39,9 @@
 #include <string>
 #include <vector>
 
-namespace pbat_vbd {
-
-namespace {
-
-#define PBAT_HIP(call)                                                                                            \
-    do                                                                                                            \
-    {                                                                                                             \
-        hipError_t const _e = (call);                                                                             \
-        if (_e != hipSuccess)                                                                                     \
-            Fail(PBAT_VBD_ERROR_HIP, std::string(#call) + ": " + hipGetErrorString(_e));                          \
-    } while (0)
-
-template <class T>
-struct DeviceArray
-{
-    T* ptr = nullptr;
-    size_t n = 0;
-    DeviceArray() = default;
-    DeviceArray(DeviceArray const&) = delete;
-    DeviceArray& operator=(DeviceArray const&) = delete;
-    ~DeviceArray() { reset(); }
-    void reset()
-    {
-        if (ptr)
-            (void)hipFree(ptr);
-        ptr = nullptr;
-        n = 0;
-    }
-    void allocate(size_t count)
-    {
-        reset();
-        if (count == 0)
-            return;
-        hipError_t const e = hipMalloc(reinterpret_cast<void**>(&ptr), count * sizeof(T));
-        if (e != hipSuccess)
-        {
-            ptr = nullptr;
-            Fail(PBAT_VBD_ERROR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-        }
-        n = count;
-    }
-    void upload(std::vector<T> const& h, hipStream_t s)
-    {
-        allocate(h.size());
-        if (!h.empty())
-            PBAT_HIP(hipMemcpyAsync(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
-    }
-};
-
-// Incidence records of the sweep (structure of arrays, see SweepArgs).
-struct IncidenceRecords
-{
-    DeviceArray<int4> ids;
-    DeviceArray<float4> g0, g1, g2;
-    void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(); }
-};
-
-struct HostIncidenceRecords
-{
-    std::vector<int4> ids;
-    std::vector<float4> g0, g1, g2;
-    void resize(size_t n)
-    {
-        ids.assign(n, make_int4(0, 0, 0, -1));
-        g0.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
-        g1 = g0;
-        g2 = g0;
-    }
-    void upload(IncidenceRecords& d, hipStream_t s) const
-    {
-        d.ids.upload(ids, s);
-        d.g0.upload(g0, s);
-        d.g1.upload(g1, s);
-        d.g2.upload(g2, s);
-    }
-};
-
-struct GraphKey
-{
-    float dt = 0.f;
-    int iterations = -1, substeps = -1, strategy = -1, lanes = -1, block = -1, perLane = -1;
-    bool xcd = false, nt = false, ell = false;
-    float kD = 0.f, detHZero = 0.f;
-    bool operator==(GraphKey const& o) const
-    {
-        return dt == o.dt && iterations == o.iterations && substeps == o.substeps && strategy == o.strategy &&
-               lanes == o.lanes && block == o.block && perLane == o.perLane && xcd == o.xcd && nt == o.nt &&
-               ell == o.ell && kD == o.kD && detHZero == o.detHZero;
-    }
-};
-
-} // namespace
-} // namespace pbat_vbd
-
-struct pbat_vbd_integrator
-{
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int64_t nV = 0, nT = 0, nIncidences = 0, nUpdated = 0, maxDegree = 0;
-    std::vector<int> colorBegin; // internal vertex ranges per colour (size nColours + 1)
-    std::vector<int> perm;       // internal -> original vertex index
-    std::vector<int> iperm;      // original -> internal
-    std::vector<float> massOrig; // lumped masses in original order (FP32)
-    std::vector<int64_t> meshOffsets; // batch: first vertex of each mesh (count + 1), empty otherwise
-    std::vector<float> wgHost;   // quadrature weights (traces only)
-    std::vector<float> lameHost; // (mu, lambda) per element (traces only)
-
-    pbat_vbd::DeviceArray<float4> xm, xtil, xt, vel, vprev, aext;
-    pbat_vbd::DeviceArray<float4> tgp; // 3 per tet
-    pbat_vbd::DeviceArray<int4> tetv;
-    pbat_vbd::IncidenceRecords rows, ell; // CSR rows, fixed-stride rows
-    int ellStride = 0;   // > 0 when the fixed-stride rows exist
-    bool useEll = false; // sweep reads the fixed-stride rows
-    pbat_vbd::DeviceArray<int> vtp;
-    pbat_vbd::DeviceArray<double> energy;
-
-    int strategy = PBAT_VBD_INIT_ADAPTIVE_PBAT;
-    float kD = 0.f;
-    float detHZero = 1e-7f;
-    int blockSize = 256;
-    int lanes = 0;         // 0 = automatic
-    int perLane = 0;       // incidences prefetched per lane, 0 = automatic
-    bool xcdRemap = true;  // XCD-aware block order
-    bool ntStream = false; // non-temporal loads of the incidence stream
-    bool tuneNontemporal = true; // autotune also tries non-temporal stream loads
-    bool useGraph = true;
-    float worldMin[3] = {0.f, 0.f, 0.f}, worldMax[3] = {0.f, 0.f, 0.f};
-
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    pbat_vbd::GraphKey graphKey;
-
-    bool timing = false;
-    int graphSweeps = 0; // sweep launches in the captured graph
-    std::vector<hipEvent_t> events; // pairs (start, stop)
-    std::vector<int> pendingLaunches; // sweep launches between each pair
-    size_t eventsUsed = 0;
-    double timedMs = 0.;
-    int64_t timedLaunches = 0;
-
-    ~pbat_vbd_integrator()
-    {
-        if (graphExec)
-            (void)hipGraphExecDestroy(graphExec);
-        if (graph)
-            (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : events)
-            (void)hipEventDestroy(e);
-        if (stream)
-            (void)hipStreamSynchronize(stream);
-        xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
-        tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
-        if (stream)
-            (void)hipStreamDestroy(stream);
-    }
-};
+#include "integrator.hpp"
 
 namespace pbat_vbd {
-namespace {
 
 int EffectiveLanes(pbat_vbd_integrator const& h)
 {
@@ -604,9 +448,14 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
     for (int64_t v = 0; v < nUpd; ++v)
         h->maxDegree = std::max<int64_t>(h->maxDegree, P[v + 1] - P[v]);
     int64_t const nIncUpd = P[nUpd];
+    std::vector<int4> slots(static_cast<size_t>(nT), make_int4(-1, -1, -1, -1));
+    auto setSlot = [](int4& s, int il, int slot) {
+        (il == 0 ? s.x : il == 1 ? s.y : il == 2 ? s.z : s.w) = slot;
+    };
     auto fill = [&](HostIncidenceRecords& R, int64_t slot, int64_t k) {
         int64_t const e = d->GVGe[k];
         int const il = static_cast<int>(d->GVGilocal[k]);
+        setSlot(slots[e], il, static_cast<int>(slot));
         double const* G = d->GP + 12 * e; // G(r, c) = G[4c + r]
         int ids[3];
         float g[3][3];
@@ -636,6 +485,7 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
                 fill(R, P[v] + (k - d->GVGp[i]), k);
         }
         R.upload(h->rows, h->stream);
+        h->slotsRows.upload(slots, h->stream);
         PBAT_HIP(hipStreamSynchronize(h->stream));
     }
     // Fixed-stride copy when padding every row to the maximum degree costs at
@@ -646,6 +496,7 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
         int64_t const S = h->maxDegree;
         HostIncidenceRecords R;
         R.resize(static_cast<size_t>(nUpd * S));
+        std::fill(slots.begin(), slots.end(), make_int4(-1, -1, -1, -1));
         for (int64_t v = 0; v < nUpd; ++v)
         {
             int64_t const i = h->perm[v];
@@ -653,18 +504,28 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
                 fill(R, v * S + (k - d->GVGp[i]), k);
         }
         R.upload(h->ell, h->stream);
+        h->slotsEll.upload(slots, h->stream);
         PBAT_HIP(hipStreamSynchronize(h->stream));
         h->ellStride = static_cast<int>(S);
     }
 
     h->wgHost.resize(static_cast<size_t>(nT));
     h->lameHost.resize(static_cast<size_t>(2 * nT));
+    h->logY0.resize(static_cast<size_t>(nT));
+    std::vector<float2> base(static_cast<size_t>(nT));
     for (int64_t e = 0; e < nT; ++e)
     {
-        h->wgHost[e] = static_cast<float>(d->wg[e]);
-        h->lameHost[2 * e] = static_cast<float>(d->lame[2 * e]);
-        h->lameHost[2 * e + 1] = static_cast<float>(d->lame[2 * e + 1]);
+        double const w = d->wg[e], mu = d->lame[2 * e], lam = d->lame[2 * e + 1];
+        h->wgHost[e] = static_cast<float>(w);
+        h->lameHost[2 * e] = static_cast<float>(mu);
+        h->lameHost[2 * e + 1] = static_cast<float>(lam);
+        // Young's modulus of the tet: Y = mu (3 lambda + 2 mu) / (lambda + mu)
+        double const Y = mu * (3. * lam + 2. * mu) / (lam + mu);
+        h->logY0[e] = static_cast<float>(std::log(Y));
+        base[e] = make_float2(static_cast<float>(w * mu / Y), static_cast<float>(w * lam / Y));
     }
+    h->logYCurrent = h->logY0;
+    h->matBase.upload(base, h->stream);
     h->tgp.upload(G4, h->stream);
     h->tetv.upload(TV, h->stream);
     h->vtp.upload(P, h->stream);
@@ -1017,7 +878,6 @@ pbat_vbd_integrator& Handle(pbat_vbd_integrator* h)
     return *h;
 }
 
-} // namespace
 } // namespace pbat_vbd
 
 using namespace pbat_vbd;

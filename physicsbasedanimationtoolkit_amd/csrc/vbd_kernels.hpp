@@ -332,7 +332,7 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 
 // Substep begin (gpu/impl/vbd/Integrator.cu:190-248; sim/vbd/Kernels.h:24-94):
 // inertial target x~ = xt + h v + h^2 a and the initial iterate.
-__global__ __launch_bounds__(256) void SubstepBegin(float4* __restrict__ xm, float4* __restrict__ xtil,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void SubstepBegin(float4* __restrict__ xm, float4* __restrict__ xtil,
                                                     float4 const* __restrict__ xt, float4 const* __restrict__ vel,
                                                     float4 const* __restrict__ vprev, float4 const* __restrict__ aext,
                                                     int n, float dt, float dt2, int strategy)
@@ -378,7 +378,7 @@ __global__ __launch_bounds__(256) void SubstepBegin(float4* __restrict__ xm, flo
 }
 
 // Substep end (gpu/impl/vbd/Integrator.cu:329-347): vprev <- v, v = (x - xt)/h, xt <- x.
-__global__ __launch_bounds__(256) void SubstepEnd(float4 const* __restrict__ xm, float4* __restrict__ xt,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void SubstepEnd(float4 const* __restrict__ xm, float4* __restrict__ xt,
                                                   float4* __restrict__ vel, float4* __restrict__ vprev, int n,
                                                   float dt)
 {
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(256) void SubstepEnd(float4 const* __restrict__ xm,
 
 // FP64 energy reduction: out[0] += 1/2 m |v|^2, out[2] += -m a.x over vertices,
 // out[1] += w Psi(F) over tets. One atomic per block and term. blockDim = 256.
-__global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm, float4 const* __restrict__ vel,
+[[maybe_unused]] static __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm, float4 const* __restrict__ vel,
                                                    float4 const* __restrict__ aext, int4 const* __restrict__ tetv,
                                                    float4 const* __restrict__ tgp, int nV, int nT,
                                                    double* __restrict__ out)
@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm
     }
 }
 
-__global__ void ZeroDoubles(double* p, int n)
+[[maybe_unused]] static __global__ void ZeroDoubles(double* p, int n)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n)

@@ -84,6 +84,7 @@ class Integrator:
         self._lib = _native.lib()
         d, keep = _marshal(data)
         self._nV = d.nV
+        self._nT = d.nT
         handle = ctypes.c_void_p()
         _native.check(self._lib.pbat_vbd_integrator_create(ctypes.byref(d), ctypes.byref(handle)))
         del keep
@@ -240,6 +241,81 @@ class Integrator:
                                                                  int(bool(reset))))
         return ms.value, n.value
 
+    # ---- differentiable rollouts (no reference counterpart) ---------------------
+    @property
+    def element_count(self) -> int:
+        return self._nT
+
+    def set_log_young_modulus(self, log_y) -> None:
+        """Per-tet Young's modulus exp(log_y) (|#elements| values), Poisson ratios kept."""
+        a = np.ascontiguousarray(np.asarray(log_y, dtype=np.float32).reshape(-1))
+        _native.check(self._lib.pbat_vbd_integrator_set_log_young_modulus(
+            self._h, _native.ptr(a, ctypes.c_float), a.size))
+
+    def log_young_modulus(self) -> np.ndarray:
+        out = np.empty(self._nT, dtype=np.float32)
+        _native.check(self._lib.pbat_vbd_integrator_get_log_young_modulus(
+            self._h, _native.ptr(out, ctypes.c_float), self._nT))
+        return out
+
+    def rollout_forward(self, dt: float = 0.01, iterations: int = 20, steps: int = 1,
+                        checkpoint_every: int = 10) -> None:
+        """Step `steps` times (1 substep each), keeping the state every
+        `checkpoint_every` steps for rollout_backward."""
+        _native.check(self._lib.pbat_vbd_rollout_forward(self._h, float(dt), int(iterations), int(steps),
+                                                         int(checkpoint_every)))
+
+    @staticmethod
+    def adjoint_config(solver: str = "pcg", max_iterations: int = 200, tolerance: float = 1e-6,
+                       check_every: int = 5) -> _native.AdjointConfig:
+        kinds = {"pcg": _native.ADJOINT_PCG, "gauss_seidel": _native.ADJOINT_GAUSS_SEIDEL}
+        if solver not in kinds:
+            raise ValueError(f"solver must be one of {sorted(kinds)}")
+        return _native.AdjointConfig(kinds[solver], int(max_iterations), float(tolerance), int(check_every))
+
+    def rollout_backward(self, dL_dx, dL_dv=None, **solver):
+        """Gradients of a loss L(x_T, v_T) of the last rollout_forward.
+
+        dL_dx, dL_dv: 3 x nV. Returns dict(log_young_modulus=[nT], v0=3 x nV,
+        x0=3 x nV, stats=dict). The end-of-rollout state is kept."""
+        gx = _vertex_matrix_f32(dL_dx, self._nV, "dL/dx")
+        gv = None if dL_dv is None else _vertex_matrix_f32(dL_dv, self._nV, "dL/dv")
+        dlogy = np.zeros(self._nT, dtype=np.float32)
+        dv0 = np.zeros((self._nV, 3), dtype=np.float32)
+        dx0 = np.zeros((self._nV, 3), dtype=np.float32)
+        cfg = self.adjoint_config(**solver)
+        st = _native.AdjointStats()
+        f = ctypes.c_float
+        _native.check(self._lib.pbat_vbd_rollout_backward(
+            self._h, _native.ptr(gx, f), None if gv is None else _native.ptr(gv, f), _native.ptr(dlogy, f),
+            _native.ptr(dv0, f), _native.ptr(dx0, f), ctypes.byref(cfg), ctypes.byref(st)))
+        return {"log_young_modulus": dlogy, "v0": dv0.T, "x0": dx0.T,
+                "stats": {"solves": st.solves, "total_iterations": st.total_iterations,
+                          "max_iterations": st.max_iterations,
+                          "max_relative_residual": st.max_relative_residual}}
+
+    def hessian_vector(self, x, dt: float, p) -> np.ndarray:
+        """(M / dt^2 + d2U/dx2(x)) p on free vertices (3 x nV)."""
+        xa = _vertex_matrix_f32(x, self._nV, "x")
+        pa = _vertex_matrix_f32(p, self._nV, "p")
+        out = np.zeros((self._nV, 3), dtype=np.float32)
+        f = ctypes.c_float
+        _native.check(self._lib.pbat_vbd_hessian_vector(self._h, _native.ptr(xa, f), float(dt), _native.ptr(pa, f),
+                                                        _native.ptr(out, f)))
+        return out.T
+
+    def adjoint_solve(self, x, dt: float, b, **solver):
+        """Solve (M / dt^2 + d2U/dx2(x)) y = b on free vertices; returns (y, stats)."""
+        xa = _vertex_matrix_f32(x, self._nV, "x")
+        ba = _vertex_matrix_f32(b, self._nV, "b")
+        y = np.zeros((self._nV, 3), dtype=np.float32)
+        cfg = self.adjoint_config(**solver)
+        st = _native.AdjointStats()
+        f = ctypes.c_float
+        _native.check(self._lib.pbat_vbd_adjoint_solve(self._h, _native.ptr(xa, f), float(dt), _native.ptr(ba, f),
+                                                       _native.ptr(y, f), ctypes.byref(cfg), ctypes.byref(st)))
+        return y.T, {"iterations": st.total_iterations, "relative_residual": st.max_relative_residual}
+
 
 class BatchIntegrator(Integrator):
     """One integrator for many independent meshes (no reference counterpart).
@@ -256,6 +332,7 @@ class BatchIntegrator(Integrator):
         if not datas:
             raise ValueError("BatchIntegrator needs at least one Data")
         views = [_marshal(d) for d in datas]
+        self._nT = int(sum(v[0].nT for v in views))
         arr = (_native.Data * len(views))(*[v[0] for v in views])
         handle = ctypes.c_void_p()
         _native.check(self._lib.pbat_vbd_batch_create(arr, len(views), ctypes.byref(handle)))
