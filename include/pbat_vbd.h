@@ -220,21 +220,24 @@ int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3]);
 
 enum pbat_vbd_adjoint_solver {
     PBAT_VBD_ADJOINT_PCG = 0,          /* CG preconditioned by symmetric coloured block Gauss-Seidel */
-    PBAT_VBD_ADJOINT_GAUSS_SEIDEL = 1  /* coloured block Gauss-Seidel sweeps */
+    PBAT_VBD_ADJOINT_GAUSS_SEIDEL = 1, /* coloured block Gauss-Seidel sweeps */
+    PBAT_VBD_ADJOINT_PCG_JACOBI = 2    /* CG preconditioned by the inverse 3x3 diagonal blocks */
 };
 
+/* Each mesh of a batch is solved with its own CG scalars and stopping test
+ * (a converged mesh is frozen), as if it were solved alone. */
 typedef struct pbat_vbd_adjoint_config {
     int32_t solver;         /* pbat_vbd_adjoint_solver */
     int32_t max_iterations; /* per linear solve */
-    float tolerance;        /* on |b - H y| / |b| */
-    int32_t check_every;    /* iterations between residual checks (host syncs) */
+    float tolerance;        /* on |b - H y| / |b| per mesh */
+    int32_t check_every;    /* iterations between host polls of the live-mesh count */
 } pbat_vbd_adjoint_config;
 
 typedef struct pbat_vbd_adjoint_stats {
     int32_t solves;
-    int32_t total_iterations;
+    int32_t total_iterations; /* sum over solves of the largest per-mesh iteration count */
     int32_t max_iterations;
-    float max_relative_residual;
+    float max_relative_residual; /* largest per-mesh |b - H y| / |b| */
 } pbat_vbd_adjoint_stats;
 
 /* Per-tet Young's modulus Y_e = exp(logY[e]) with each tet's Poisson ratio
@@ -260,6 +263,11 @@ int pbat_vbd_hessian_vector(pbat_vbd_integrator* h, const float* x, float dt, co
  * solver (b's Dirichlet rows ignored). x, b, y: 3 x nV. */
 int pbat_vbd_adjoint_solve(pbat_vbd_integrator* h, const float* x, float dt, const float* b, float* y,
                            const pbat_vbd_adjoint_config* config, pbat_vbd_adjoint_stats* stats);
+
+/* Device memory held by the library's arrays in this process (all handles),
+ * in bytes, and its peak since load or the last reset_peak != 0. No reference
+ * counterpart. */
+int pbat_vbd_device_memory(int64_t* current, int64_t* peak, int32_t reset_peak);
 
 /* Wait for all work queued on the handle's stream. */
 int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);

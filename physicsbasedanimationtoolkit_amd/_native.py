@@ -57,6 +57,7 @@ class SweepConfig(ctypes.Structure):
 
 ADJOINT_PCG = 0
 ADJOINT_GAUSS_SEIDEL = 1
+ADJOINT_PCG_JACOBI = 2
 
 
 class AdjointConfig(ctypes.Structure):
@@ -104,6 +105,7 @@ SIGNATURES = [
     ("pbat_vbd_integrator_set_graph_capture", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_energy", ctypes.c_int, [ctypes.c_void_p, p_f64]),
     ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("pbat_vbd_device_memory", ctypes.c_int, [p_i64, p_i64, c_i32]),
     ("pbat_vbd_integrator_counts", ctypes.c_int, [ctypes.c_void_p, p_i64, p_i64, p_i64]),
     ("pbat_vbd_integrator_set_timing", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_sweep_timing", ctypes.c_int,
@@ -153,6 +155,13 @@ def check(rc: int) -> None:
     if rc == ERROR_HIP:
         raise HipError(msg)
     raise RuntimeError(f"{msg} (status {rc})")
+
+
+def device_memory(reset_peak: bool = False):
+    """(bytes held now, peak bytes) of the library's device arrays in this process."""
+    cur, peak = ctypes.c_int64(), ctypes.c_int64()
+    check(lib().pbat_vbd_device_memory(ctypes.byref(cur), ctypes.byref(peak), int(bool(reset_peak))))
+    return cur.value, peak.value
 
 
 def ptr(a, ctype):

@@ -10,11 +10,25 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
 
 namespace pbat_vbd {
+
+// Device bytes held by the library's arrays, process-wide, and their peak.
+inline std::atomic<int64_t> gDeviceBytes{0};
+inline std::atomic<int64_t> gDevicePeak{0};
+
+inline void CountDeviceBytes(int64_t delta)
+{
+    int64_t const now = gDeviceBytes.fetch_add(delta) + delta;
+    int64_t peak = gDevicePeak.load();
+    while (now > peak && !gDevicePeak.compare_exchange_weak(peak, now))
+    {
+    }
+}
 
 #define PBAT_HIP(call)                                                                                            \
     do                                                                                                            \
@@ -47,7 +61,10 @@ struct DeviceArray
     void reset()
     {
         if (ptr)
+        {
             (void)hipFree(ptr);
+            CountDeviceBytes(-static_cast<int64_t>(n * sizeof(T)));
+        }
         ptr = nullptr;
         n = 0;
     }
@@ -63,6 +80,7 @@ struct DeviceArray
             Fail(PBAT_VBD_ERROR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
         }
         n = count;
+        CountDeviceBytes(static_cast<int64_t>(count * sizeof(T)));
     }
     void upload(std::vector<T> const& h, hipStream_t s)
     {
@@ -109,7 +127,9 @@ struct RolloutState
     std::vector<DeviceArray<float4>> ckptX, ckptV, ckptVprev; // every `every` steps
     std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
     DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
-    DeviceArray<double> scal;                                 // device scalars of the solvers
+    DeviceArray<float4> dinv;                                 // inverse diagonal blocks, 2 per vertex
+    DeviceArray<double> scal;                                 // per-mesh solver scalars (kScalarCount x nMeshes)
+    DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count
     DeviceArray<double> grad;                                 // d L / d log Y per tet
     DeviceArray<float> logY;                                  // parameter upload buffer
 };
@@ -140,6 +160,8 @@ struct pbat_vbd_integrator
     std::vector<int> iperm;      // original -> internal
     std::vector<float> massOrig; // lumped masses in original order (FP32)
     std::vector<int64_t> meshOffsets; // batch: first vertex of each mesh (count + 1), empty otherwise
+    int nMeshes = 1;
+    pbat_vbd::DeviceArray<int> meshOf; // mesh of each internal vertex (0 for a plain integrator)
     std::vector<float> wgHost;   // quadrature weights (traces only)
     std::vector<float> lameHost; // (mu, lambda) per element (traces only)
 
@@ -194,7 +216,7 @@ struct pbat_vbd_integrator
             (void)hipStreamSynchronize(stream);
         xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
         tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
-        slotsRows.reset(), slotsEll.reset(), matBase.reset();
+        slotsRows.reset(), slotsEll.reset(), matBase.reset(), meshOf.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)
             (void)hipStreamDestroy(stream);

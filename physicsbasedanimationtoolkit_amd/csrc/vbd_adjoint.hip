@@ -25,9 +25,9 @@
 namespace pbat_vbd {
 namespace {
 
-enum ScalarSlot { kRz0 = 0, kRz1 = 1, kPq = 2, kRr = 3, kBb = 4 };
+int Blocks(int64_t n, int block = 256) { return static_cast<int>((std::max<int64_t>(n, 1) + block - 1) / block); }
 
-int Blocks(int64_t n, int block = 256) { return static_cast<int>((n + block - 1) / block); }
+using device::AdjMode;
 
 device::AdjointArgs MakeAdjointArgs(pbat_vbd_integrator& h, float4 const* xs, float dt)
 {
@@ -39,12 +39,13 @@ device::AdjointArgs MakeAdjointArgs(pbat_vbd_integrator& h, float4 const* xs, fl
     a.g1 = r.g1.ptr;
     a.g2 = r.g2.ptr;
     a.xs = xs;
+    a.meshOf = h.meshOf.ptr;
     a.ellStride = (h.useEll && h.ellStride > 0) ? h.ellStride : 0;
     a.invDt2 = 1.f / (dt * dt);
     return a;
 }
 
-template <int L, int K, bool kGS>
+template <int L, int K, AdjMode M>
 void LaunchAdjointLK(pbat_vbd_integrator& h, device::AdjointArgs a)
 {
     int const block = h.blockSize;
@@ -58,38 +59,45 @@ void LaunchAdjointLK(pbat_vbd_integrator& h, device::AdjointArgs a)
     }
     if (grid64 * block >= (int64_t{1} << 31))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "vertex range too large for the adjoint launch");
-    hipLaunchKernelGGL((device::AdjointVertexKernel<L, K, kGS>), dim3(static_cast<int>(grid64)), dim3(block), 0,
+    hipLaunchKernelGGL((device::AdjointVertexKernel<L, K, M>), dim3(static_cast<int>(grid64)), dim3(block), 0,
                        h.stream, a);
 }
 
-template <int L, bool kGS>
+template <int L, AdjMode M>
 void LaunchAdjointL(pbat_vbd_integrator& h, device::AdjointArgs const& a)
 {
     switch (EffectivePerLane(h))
     {
-        case 1: LaunchAdjointLK<L, 1, kGS>(h, a); break;
-        case 2: LaunchAdjointLK<L, 2, kGS>(h, a); break;
-        case 3: LaunchAdjointLK<L, 3, kGS>(h, a); break;
-        default: LaunchAdjointLK<L, 4, kGS>(h, a); break;
+        case 1: LaunchAdjointLK<L, 1, M>(h, a); break;
+        case 2: LaunchAdjointLK<L, 2, M>(h, a); break;
+        case 3: LaunchAdjointLK<L, 3, M>(h, a); break;
+        default: LaunchAdjointLK<L, 4, M>(h, a); break;
     }
 }
 
-template <bool kGS>
+template <AdjMode M>
 void LaunchAdjoint(pbat_vbd_integrator& h, device::AdjointArgs const& a)
 {
     if (a.vend <= a.vbeg)
         return;
     switch (EffectiveLanes(h))
     {
-        case 1: LaunchAdjointL<1, kGS>(h, a); break;
-        case 2: LaunchAdjointL<2, kGS>(h, a); break;
-        case 4: LaunchAdjointL<4, kGS>(h, a); break;
-        case 8: LaunchAdjointL<8, kGS>(h, a); break;
-        case 16: LaunchAdjointL<16, kGS>(h, a); break;
-        case 32: LaunchAdjointL<32, kGS>(h, a); break;
-        default: LaunchAdjointL<64, kGS>(h, a); break;
+        case 1: LaunchAdjointL<1, M>(h, a); break;
+        case 2: LaunchAdjointL<2, M>(h, a); break;
+        case 4: LaunchAdjointL<4, M>(h, a); break;
+        case 8: LaunchAdjointL<8, M>(h, a); break;
+        case 16: LaunchAdjointL<16, M>(h, a); break;
+        case 32: LaunchAdjointL<32, M>(h, a); break;
+        default: LaunchAdjointL<64, M>(h, a); break;
     }
 }
+
+// Solver scalars: plain per-mesh values (padded to 128 B), then the accumulators.
+size_t PlainDoubles(pbat_vbd_integrator const& h)
+{
+    return (static_cast<size_t>(device::kPlainCount) * h.nMeshes + 15) / 16 * 16;
+}
+size_t ScalarDoubles(pbat_vbd_integrator const& h) { return PlainDoubles(h) + device::AccDoubles(h.nMeshes); }
 
 void EnsureBuffers(pbat_vbd_integrator& h)
 {
@@ -98,8 +106,13 @@ void EnsureBuffers(pbat_vbd_integrator& h)
     for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
-    if (R.scal.n != 8)
-        R.scal.allocate(8);
+    if (R.dinv.n != 2 * n)
+        R.dinv.allocate(2 * n);
+    size_t const nM = static_cast<size_t>(h.nMeshes);
+    if (R.scal.n != ScalarDoubles(h))
+        R.scal.allocate(ScalarDoubles(h));
+    if (R.flags.n != 2 * nM + 1)
+        R.flags.allocate(2 * nM + 1);
     if (R.grad.n != static_cast<size_t>(h.nT))
         R.grad.allocate(static_cast<size_t>(h.nT));
 }
@@ -110,133 +123,169 @@ void Zero4(pbat_vbd_integrator& h, float4* p, int n)
         hipLaunchKernelGGL(device::Fill4, dim3(Blocks(n)), dim3(256), 0, h.stream, p, n);
 }
 
-// acc = a . b over the updated vertices (FP64, device scalar)
-void Dot(pbat_vbd_integrator& h, float4 const* a, float4 const* b, double* acc)
-{
-    int const n = static_cast<int>(h.nUpdated);
-    hipLaunchKernelGGL(device::ZeroDoubles, dim3(1), dim3(64), 0, h.stream, acc, 1);
-    if (n > 0)
-        hipLaunchKernelGGL(device::Dot3, dim3(std::min(1024, Blocks(n))), dim3(256), 0, h.stream, a, b, n, acc);
-}
-
-double ReadScalar(pbat_vbd_integrator& h, double const* p)
-{
-    double v = 0.;
-    PBAT_HIP(hipMemcpyAsync(&v, p, sizeof(double), hipMemcpyDeviceToHost, h.stream));
-    PBAT_HIP(hipStreamSynchronize(h.stream));
-    return v;
-}
-
-// out = H p over the updated vertices
-void HessVec(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* p, float4* out)
-{
-    device::AdjointArgs a = MakeAdjointArgs(h, xs, dt);
-    a.y = p;
-    a.out = out;
-    a.vbeg = 0;
-    a.vend = static_cast<int>(h.nUpdated);
-    LaunchAdjoint<false>(h, a);
-}
-
-// One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b.
-void GaussSeidelPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b, bool reverse)
+// Vertex-kernel pass of mode M over [vbeg, vend): y, b, out, acc as the mode uses them.
+template <AdjMode M>
+void VertexPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b, float4* out,
+                double* acc, int accArray, int vbeg, int vend)
 {
     device::AdjointArgs a = MakeAdjointArgs(h, xs, dt);
     a.y = y;
     a.b = b;
+    a.out = out;
+    a.acc = acc;
+    a.accArray = accArray;
+    a.nM = h.nMeshes;
+    a.vbeg = vbeg;
+    a.vend = vend;
+    LaunchAdjoint<M>(h, a);
+}
+
+// One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b;
+// with acc, the accumulator r.z of each mesh receives b . y of the final
+// values (a reverse pass after a forward pass updates each vertex last).
+void GaussSeidelPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b, bool reverse,
+                     double* acc)
+{
     int const nC = static_cast<int>(h.colorBegin.size()) - 1;
     for (int k = 0; k < nC; ++k)
     {
         int const c = reverse ? nC - 1 - k : k;
-        a.vbeg = h.colorBegin[c];
-        a.vend = h.colorBegin[c + 1];
-        LaunchAdjoint<true>(h, a);
+        if (acc)
+            VertexPass<AdjMode::kGaussSeidelDot>(h, xs, dt, y, b, nullptr, acc, device::kAccRzNew, h.colorBegin[c],
+                                                 h.colorBegin[c + 1]);
+        else
+            VertexPass<AdjMode::kGaussSeidel>(h, xs, dt, y, b, nullptr, nullptr, 0, h.colorBegin[c],
+                                              h.colorBegin[c + 1]);
     }
 }
 
 struct SolveResult
 {
-    int iterations = 0;
-    double relResidual = 0.;
+    int iterations = 0;         // largest per-mesh iteration count
+    double relResidual = 0.;    // largest per-mesh |r| / |b| at its last iteration
 };
 
-// Solve H y = b (y holds the initial guess). Residuals are checked on the host
-// every cfg.check_every iterations.
+int ReadLive(pbat_vbd_integrator& h)
+{
+    int live = 0;
+    int const nM = h.nMeshes;
+    PBAT_HIP(hipMemcpyAsync(&live, h.rollout.flags.ptr + 2 * nM, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    return live;
+}
+
+SolveResult ReadResult(pbat_vbd_integrator& h)
+{
+    int const nM = h.nMeshes;
+    RolloutState& R = h.rollout;
+    std::vector<double> S(static_cast<size_t>(device::kPlainCount * nM));
+    std::vector<int> F(static_cast<size_t>(2 * nM + 1));
+    PBAT_HIP(hipMemcpyAsync(S.data(), R.scal.ptr, S.size() * sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    PBAT_HIP(hipMemcpyAsync(F.data(), R.flags.ptr, F.size() * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    SolveResult out;
+    for (int m = 0; m < nM; ++m)
+    {
+        out.iterations = std::max(out.iterations, F[device::kIters * nM + m]);
+        double const bb = S[device::kBb * nM + m];
+        if (bb > 0.)
+            out.relResidual = std::max(out.relResidual, std::sqrt(S[device::kRrLast * nM + m] / bb));
+    }
+    return out;
+}
+
+// Solve H y = b (y holds the initial guess) for every mesh of the handle, each
+// with its own CG scalars and stopping test |b - H y| <= tol |b|; the host
+// polls the number of live meshes every cfg.check_every iterations.
 SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b,
                   pbat_vbd_adjoint_config const& cfg)
 {
     RolloutState& R = h.rollout;
-    int const n = static_cast<int>(h.nUpdated);
-    double* S = R.scal.ptr;
-    Dot(h, b, b, S + kBb);
-    double const bb = ReadScalar(h, S + kBb);
-    SolveResult out;
-    if (!(bb > 0.))
-    {
-        Zero4(h, y, static_cast<int>(h.nV));
-        return out;
-    }
+    int const n = static_cast<int>(h.nUpdated), nV = static_cast<int>(h.nV), nM = h.nMeshes;
+    double* S = R.scal.ptr;                  // plain per-mesh values
+    double* A = R.scal.ptr + PlainDoubles(h); // accumulators
+    int* F = R.flags.ptr;
+    double const tol2 = static_cast<double>(cfg.tolerance) * cfg.tolerance;
     int const check = std::max(1, cfg.check_every);
-    auto residual = [&]() {
-        HessVec(h, xs, dt, y, R.q.ptr);
-        hipLaunchKernelGGL(device::Sub4, dim3(Blocks(std::max(n, 1))), dim3(256), 0, h.stream, R.r.ptr, b, R.q.ptr,
-                           n);
-        Dot(h, R.r.ptr, R.r.ptr, S + kRr);
-        return std::sqrt(ReadScalar(h, S + kRr) / bb);
+    int const gridN = Blocks(n);
+    bool const jacobi = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI;
+    float4 const* dinv = jacobi ? R.dinv.ptr : nullptr;
+    auto zeroScalars = [&] {
+        int const nd = static_cast<int>(ScalarDoubles(h));
+        hipLaunchKernelGGL(device::ZeroScalars, dim3(std::min(1024, Blocks(nd + 2 * nM + 1))), dim3(256), 0,
+                           h.stream, S, nd, F, 2 * nM + 1);
     };
+    auto scalars = [&](int first) {
+        hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, nM, tol2, first, S, A, F);
+    };
+    // residual r = b - H y with per-mesh |b|^2, |r|^2 (and r . D^-1 r with dinv)
+    auto residual = [&](float4 const* pre) {
+        VertexPass<AdjMode::kHessVec>(h, xs, dt, y, nullptr, R.q.ptr, nullptr, 0, 0, n);
+        hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr,
+                           R.z.ptr, pre, h.meshOf.ptr, n, nM, A);
+    };
+    zeroScalars();
+    Zero4(h, R.z.ptr, nV);
     if (cfg.solver == PBAT_VBD_ADJOINT_GAUSS_SEIDEL)
     {
-        for (int it = 1; it <= cfg.max_iterations; ++it)
+        int it = 0;
+        while (it < cfg.max_iterations)
         {
-            GaussSeidelPass(h, xs, dt, y, b, false);
-            out.iterations = it;
-            if (it % check == 0 || it == cfg.max_iterations)
-            {
-                out.relResidual = residual();
-                if (out.relResidual <= cfg.tolerance)
-                    break;
-            }
+            int const chunk = std::min(check, cfg.max_iterations - it);
+            for (int k = 0; k < chunk; ++k)
+                GaussSeidelPass(h, xs, dt, y, b, false, nullptr);
+            it += chunk;
+            zeroScalars();
+            residual(nullptr);
+            scalars(1);
+            if (ReadLive(h) == 0)
+                break;
         }
+        SolveResult out = ReadResult(h);
+        out.iterations = it;
         PBAT_HIP(hipGetLastError());
         return out;
     }
-    // Conjugate gradients preconditioned by a symmetric Gauss-Seidel pass
-    // (forward then reverse colour order, from zero), all scalars on the device.
-    out.relResidual = residual(); // r = b - H y
-    if (out.relResidual <= cfg.tolerance)
-        return out;
-    auto precondition = [&]() {
-        Zero4(h, R.z.ptr, static_cast<int>(h.nV));
-        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
-        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true);
-    };
-    precondition();
-    PBAT_HIP(hipMemcpyAsync(R.p.ptr, R.z.ptr, sizeof(float4) * h.nV, hipMemcpyDeviceToDevice, h.stream));
-    Dot(h, R.r.ptr, R.z.ptr, S + kRz0);
-    int cur = kRz0;
-    for (int it = 1; it <= cfg.max_iterations; ++it)
+    if (jacobi)
+        VertexPass<AdjMode::kDiagInverse>(h, xs, dt, y, nullptr, R.dinv.ptr, nullptr, 0, 0, n);
+    residual(dinv);
+    if (!jacobi)
     {
-        HessVec(h, xs, dt, R.p.ptr, R.q.ptr);
-        Dot(h, R.p.ptr, R.q.ptr, S + kPq);
-        hipLaunchKernelGGL(device::CgUpdate, dim3(Blocks(std::max(n, 1))), dim3(256), 0, h.stream, y, R.r.ptr,
-                           R.p.ptr, R.q.ptr, n, S + cur, S + kPq);
-        out.iterations = it;
-        if (it % check == 0 || it == cfg.max_iterations)
+        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false, nullptr);
+        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, A);
+    }
+    scalars(1);
+    hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr, nV);
+    if (ReadLive(h) > 0)
+    {
+        for (int it = 1; it <= cfg.max_iterations; ++it)
         {
-            Dot(h, R.r.ptr, R.r.ptr, S + kRr);
-            out.relResidual = std::sqrt(ReadScalar(h, S + kRr) / bb);
-            if (out.relResidual <= cfg.tolerance)
+            VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, A, device::kAccPq, 0, n);
+            hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr, R.p.ptr,
+                               R.q.ptr, dinv, h.meshOf.ptr, n, nM, static_cast<double const*>(S), A,
+                               static_cast<int const*>(F));
+            if (!jacobi)
+            {
+                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false, nullptr);
+                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, A);
+            }
+            scalars(0);
+            hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr,
+                               h.meshOf.ptr, n, nM, static_cast<double const*>(S), static_cast<int const*>(F));
+            if ((it % check == 0 || it == cfg.max_iterations) && ReadLive(h) == 0)
                 break;
         }
-        precondition();
-        int const next = (cur == kRz0) ? kRz1 : kRz0;
-        Dot(h, R.r.ptr, R.z.ptr, S + next);
-        hipLaunchKernelGGL(device::CgDirection, dim3(Blocks(std::max(n, 1))), dim3(256), 0, h.stream, R.p.ptr,
-                           R.z.ptr, n, S + next, S + cur);
-        cur = next;
     }
     PBAT_HIP(hipGetLastError());
-    return out;
+    return ReadResult(h);
+}
+
+void CheckConfig(pbat_vbd_adjoint_config const& cfg)
+{
+    if (cfg.max_iterations < 1 || !(cfg.tolerance >= 0.f) ||
+        (cfg.solver != PBAT_VBD_ADJOINT_PCG && cfg.solver != PBAT_VBD_ADJOINT_GAUSS_SEIDEL &&
+         cfg.solver != PBAT_VBD_ADJOINT_PCG_JACOBI))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid adjoint solver configuration");
 }
 
 void SaveState(pbat_vbd_integrator& h, float4* x, float4* v, float4* vp)
@@ -300,8 +349,7 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
     pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5};
     if (cfgIn)
         cfg = *cfgIn;
-    if (cfg.max_iterations < 1 || (cfg.solver != PBAT_VBD_ADJOINT_PCG && cfg.solver != PBAT_VBD_ADJOINT_GAUSS_SEIDEL))
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid adjoint solver configuration");
+    CheckConfig(cfg);
     PBAT_HIP(hipSetDevice(h.device));
     EnsureBuffers(h);
     int const nV = static_cast<int>(h.nV), nUpd = static_cast<int>(h.nUpdated), nT = static_cast<int>(h.nT);
@@ -409,7 +457,8 @@ void HessianVectorProduct(pbat_vbd_integrator& h, float const* x, float dt, floa
     if (nV > nUpd)
         Zero4(h, R.p.ptr + nUpd, nV - nUpd);
     Zero4(h, R.q.ptr, nV);
-    HessVec(h, R.hold.ptr, dt, R.p.ptr, R.q.ptr);
+    VertexPass<AdjMode::kHessVec>(h, R.hold.ptr, dt, R.p.ptr, nullptr, R.q.ptr, nullptr, 0, 0,
+                                  static_cast<int>(h.nUpdated));
     PBAT_HIP(hipGetLastError());
     DownloadVec(h, R.q.ptr, out);
 }
@@ -422,6 +471,7 @@ void AdjointSolve(pbat_vbd_integrator& h, float const* x, float dt, float const*
     pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5};
     if (cfgIn)
         cfg = *cfgIn;
+    CheckConfig(cfg);
     PBAT_HIP(hipSetDevice(h.device));
     EnsureBuffers(h);
     RolloutState& R = h.rollout;

@@ -87,6 +87,122 @@ __device__ __forceinline__ void AccumulateHessVec(float4 const& q0, float4 const
     }
 }
 
+// Per-mesh solver scalars. Plain values: kPlainCount arrays of nM doubles.
+enum PlainArray : int
+{
+    kRz = 0,     // r . z of the current iteration
+    kBeta = 1,   // CG direction coefficient
+    kRrLast = 2, // r . r at the last completed iteration
+    kBb = 3,     // b . b
+    kPlainCount = 4
+};
+// Accumulated dot products: each (array, mesh) is spread over kSpread atomic
+// targets kSlotStride doubles (128 B) apart, so that thousands of workgroups
+// adding into one mesh's sum do not serialise on one address.
+enum AccArray : int
+{
+    kAccRzNew = 0, // r . z being accumulated
+    kAccPq = 1,    // p . H p
+    kAccRr = 2,    // r . r
+    kAccBb = 3,    // b . b
+    kAccCount = 4
+};
+constexpr int kSpread = 8;
+constexpr int kSlotStride = 16;
+
+__host__ __device__ constexpr size_t AccDoubles(int nM)
+{
+    return static_cast<size_t>(kAccCount) * nM * kSpread * kSlotStride;
+}
+
+__device__ __forceinline__ double* AccSlot(double* A, int arr, int nM, int m, int slot)
+{
+    return A + ((static_cast<size_t>(arr) * nM + m) * kSpread + slot) * kSlotStride;
+}
+
+__device__ __forceinline__ double AccSum(double const* A, int arr, int nM, int m)
+{
+    double s = 0.;
+#pragma unroll
+    for (int k = 0; k < kSpread; ++k)
+        s += A[((static_cast<size_t>(arr) * nM + m) * kSpread + k) * kSlotStride];
+    return s;
+}
+
+// Per-mesh flags: done[nM], iterations[nM], then one count of live meshes.
+enum FlagArray : int
+{
+    kDone = 0,
+    kIters = 1
+};
+
+// Sum val into accumulated array arr of mesh m over the calling workgroup;
+// every thread of the workgroup must call it (threads without a contribution
+// pass m < 0); blockDim <= 256. Waves whose contributors share one mesh (the
+// vertices of a colour are grouped by mesh) reduce in registers, the
+// workgroup's waves then combine in LDS, and one FP64 atomic per mesh per
+// workgroup lands on one of the mesh's kSpread targets.
+__device__ __forceinline__ void BlockAccumulate(double val, int m, double* __restrict__ A, int arr, int nM)
+{
+    __shared__ double sv[4];
+    __shared__ int sm[4];
+    int const lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
+    int const slot = static_cast<int>(blockIdx.x) & (kSpread - 1);
+    unsigned long long const live = __ballot(m >= 0);
+    int mref = -1;
+    double v = 0.;
+    if (live != 0ull)
+    {
+        int const first = __ffsll(static_cast<long long>(live)) - 1;
+        int const mf = __shfl(m, first);
+        if (__all(m < 0 || m == mf))
+        {
+            v = (m >= 0) ? val : 0.;
+            for (int off = 32; off > 0; off >>= 1)
+                v += __shfl_xor(v, off);
+            mref = mf;
+        }
+        else if (m >= 0)
+        {
+            atomicAdd(AccSlot(A, arr, nM, m, slot), val);
+        }
+    }
+    if (lane == 0)
+    {
+        sv[wave] = v;
+        sm[wave] = mref;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int const nw = static_cast<int>((blockDim.x + 63) / 64);
+        for (int w = 0; w < nw; ++w)
+        {
+            int const mw = sm[w];
+            if (mw < 0)
+                continue;
+            double sum = sv[w];
+            for (int u = w + 1; u < nw; ++u)
+                if (sm[u] == mw)
+                {
+                    sum += sv[u];
+                    sm[u] = -1;
+                }
+            atomicAdd(AccSlot(A, arr, nM, mw, slot), sum);
+        }
+    }
+    __syncthreads();
+}
+
+enum class AdjMode : int
+{
+    kGaussSeidel = 0,    // one colour of block Gauss-Seidel on H y = b
+    kGaussSeidelDot = 1, // same, then acc[mesh] += b . y (final pass of symmetric GS)
+    kHessVec = 2,        // out = H y
+    kHessVecDot = 3,     // out = H y, acc[mesh] += y . out
+    kDiagInverse = 4     // dinv = inverse of the 3x3 diagonal block of H
+};
+
 struct AdjointArgs
 {
     int const* __restrict__ vtp;
@@ -94,10 +210,14 @@ struct AdjointArgs
     float4 const* __restrict__ g0;
     float4 const* __restrict__ g1;
     float4 const* __restrict__ g2;
-    float4 const* __restrict__ xs; // minimiser positions (with lumped mass in .w)
-    float4* __restrict__ y;        // unknowns (Gauss-Seidel: updated in place) / vector multiplied
-    float4 const* __restrict__ b;  // right-hand side (Gauss-Seidel)
-    float4* __restrict__ out;      // H y (Hessian-vector product)
+    float4 const* __restrict__ xs;  // minimiser positions (with lumped mass in .w)
+    float4* __restrict__ y;         // unknowns (Gauss-Seidel: updated in place) / vector multiplied
+    float4 const* __restrict__ b;   // right-hand side (Gauss-Seidel)
+    float4* __restrict__ out;       // H y (Hessian-vector product) / inverse blocks (2 per vertex)
+    int const* __restrict__ meshOf; // mesh of each vertex
+    double* __restrict__ acc;       // accumulators (Dot modes): array accArray of nM meshes
+    int accArray;
+    int nM;
     int vbeg;
     int vend;
     int xcdChunk;
@@ -105,12 +225,30 @@ struct AdjointArgs
     float invDt2; // 1 / h^2
 };
 
-// kGaussSeidel: one colour of block Gauss-Seidel on H y = b,
-//   y_i += D_i^{-1} (b_i - (H y)_i),  D_i = m_i / h^2 I + sum_e w_e (mu |g|^2 I + lambda c c^T).
-// Otherwise out_i = (H y)_i for the vertices [vbeg, vend).
-template <int L, int K, bool kGaussSeidel>
+// The 3x3 symmetric block solve / inverse in FP64 (adjugate / determinant).
+struct Inverse3
+{
+    double c00, c01, c02, c11, c12, c22, inv;
+    __device__ __forceinline__ Inverse3(double d00, double d01, double d02, double d11, double d12, double d22)
+    {
+        c00 = d11 * d22 - d12 * d12, c01 = d02 * d12 - d01 * d22, c02 = d01 * d12 - d02 * d11;
+        c11 = d00 * d22 - d02 * d02, c12 = d01 * d02 - d00 * d12, c22 = d00 * d11 - d01 * d01;
+        double const det = d00 * c00 + d01 * c01 + d02 * c02;
+        inv = (fabs(det) > 0.) ? 1. / det : 0.;
+    }
+};
+
+// Per vertex i of [vbeg, vend), with D_i = m_i / h^2 I + sum_e w_e (mu |g|^2 I + lambda c c^T)
+// and (H y)_i = m_i / h^2 y_i + sum_e (H_e y)_i:
+//   kGaussSeidel:  y_i += D_i^{-1} (b_i - (H y)_i)
+//   kHessVec:      out_i = (H y)_i
+//   kDiagInverse:  out[2i], out[2i+1] = D_i^{-1} as (00, 01, 02, 11), (12, 22, ., .)
+template <int L, int K, AdjMode kMode>
 __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
 {
+    constexpr bool kGS = kMode == AdjMode::kGaussSeidel || kMode == AdjMode::kGaussSeidelDot;
+    constexpr bool kDiag = kGS || kMode == AdjMode::kDiagInverse;
+    constexpr bool kDot = kMode == AdjMode::kGaussSeidelDot || kMode == AdjMode::kHessVecDot;
     int block = static_cast<int>(blockIdx.x);
     if (a.xcdChunk > 0)
         block = (block & 7) * a.xcdChunk + (block >> 3);
@@ -134,7 +272,7 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
         }
         own = a.xs[v];
         yo = a.y[v];
-        if (kGaussSeidel && lane == 0)
+        if (kGS && lane == 0)
             bo = a.b[v];
     }
     float3 hv = make_float3(0.f, 0.f, 0.f);
@@ -159,45 +297,55 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
     for (int j = 0; j < K; ++j)
     {
         if (I[j].w >= 0)
-            AccumulateHessVec<kGaussSeidel>(q0[j], q1[j], q2[j], own, a.xs[I[j].x], a.xs[I[j].y], a.xs[I[j].z],
-                                            yo, a.y[I[j].x], a.y[I[j].y], a.y[I[j].z], hv, D);
+            AccumulateHessVec<kDiag>(q0[j], q1[j], q2[j], own, a.xs[I[j].x], a.xs[I[j].y], a.xs[I[j].z], yo,
+                                     a.y[I[j].x], a.y[I[j].y], a.y[I[j].z], hv, D);
     }
     for (int k = kb + lane + K * L; k < ke; k += L)
     {
         int4 const J = a.ids[k];
         if (J.w < 0)
             continue;
-        AccumulateHessVec<kGaussSeidel>(a.g0[k], a.g1[k], a.g2[k], own, a.xs[J.x], a.xs[J.y], a.xs[J.z], yo,
-                                        a.y[J.x], a.y[J.y], a.y[J.z], hv, D);
+        AccumulateHessVec<kDiag>(a.g0[k], a.g1[k], a.g2[k], own, a.xs[J.x], a.xs[J.y], a.xs[J.z], yo, a.y[J.x],
+                                 a.y[J.y], a.y[J.z], hv, D);
     }
     hv.x = GroupSum<L>(hv.x);
     hv.y = GroupSum<L>(hv.y);
     hv.z = GroupSum<L>(hv.z);
-    if constexpr (kGaussSeidel)
+    if constexpr (kDiag)
         GroupSum<L>(D);
-    if (!active || lane != 0)
-        return;
+    bool const leader = active && lane == 0;
     float const K0 = own.w * a.invDt2;
     float const Hx = hv.x + K0 * yo.x, Hy = hv.y + K0 * yo.y, Hz = hv.z + K0 * yo.z;
-    if constexpr (!kGaussSeidel)
+    double dot = 0.;
+    if (leader)
     {
-        a.out[v] = make_float4(Hx, Hy, Hz, 0.f);
+        if constexpr (kMode == AdjMode::kHessVec || kMode == AdjMode::kHessVecDot)
+        {
+            a.out[v] = make_float4(Hx, Hy, Hz, 0.f);
+            dot = double(yo.x) * Hx + double(yo.y) * Hy + double(yo.z) * Hz;
+        }
+        else if constexpr (kMode == AdjMode::kDiagInverse)
+        {
+            Inverse3 const C(double(D.h00) + K0, D.h01, D.h02, double(D.h11) + K0, D.h12, double(D.h22) + K0);
+            a.out[2 * v] = make_float4(static_cast<float>(C.c00 * C.inv), static_cast<float>(C.c01 * C.inv),
+                                       static_cast<float>(C.c02 * C.inv), static_cast<float>(C.c11 * C.inv));
+            a.out[2 * v + 1] =
+                make_float4(static_cast<float>(C.c12 * C.inv), static_cast<float>(C.c22 * C.inv), 0.f, 0.f);
+        }
+        else
+        {
+            float const rx = bo.x - Hx, ry = bo.y - Hy, rz = bo.z - Hz;
+            Inverse3 const C(double(D.h00) + K0, D.h01, D.h02, double(D.h11) + K0, D.h12, double(D.h22) + K0);
+            float4 const yn =
+                make_float4(yo.x + static_cast<float>((C.c00 * rx + C.c01 * ry + C.c02 * rz) * C.inv),
+                            yo.y + static_cast<float>((C.c01 * rx + C.c11 * ry + C.c12 * rz) * C.inv),
+                            yo.z + static_cast<float>((C.c02 * rx + C.c12 * ry + C.c22 * rz) * C.inv), 0.f);
+            a.y[v] = yn;
+            dot = double(bo.x) * yn.x + double(bo.y) * yn.y + double(bo.z) * yn.z;
+        }
     }
-    else
-    {
-        float const rx = bo.x - Hx, ry = bo.y - Hy, rz = bo.z - Hz;
-        double const d00 = D.h00 + K0, d11 = D.h11 + K0, d22 = D.h22 + K0;
-        double const d01 = D.h01, d02 = D.h02, d12 = D.h12;
-        double const c00 = d11 * d22 - d12 * d12, c01 = d02 * d12 - d01 * d22, c02 = d01 * d12 - d02 * d11;
-        double const c11 = d00 * d22 - d02 * d02, c12 = d01 * d02 - d00 * d12, c22 = d00 * d11 - d01 * d01;
-        double const det = d00 * c00 + d01 * c01 + d02 * c02;
-        if (!(fabs(det) > 0.))
-            return;
-        double const inv = 1. / det;
-        a.y[v] = make_float4(yo.x + static_cast<float>((c00 * rx + c01 * ry + c02 * rz) * inv),
-                             yo.y + static_cast<float>((c01 * rx + c11 * ry + c12 * rz) * inv),
-                             yo.z + static_cast<float>((c02 * rx + c12 * ry + c22 * rz) * inv), 0.f);
-    }
+    if constexpr (kDot)
+        BlockAccumulate(dot, leader ? a.meshOf[v] : -1, a.acc, a.accArray, a.nM);
 }
 
 // d L / d log Y_e -= (w_e P_e) : dF_e(y) at the minimiser, for every tet
@@ -286,70 +434,175 @@ __global__ __launch_bounds__(256) void AdjointStepBack(float4 const* __restrict_
     vbar[i] = vb;
 }
 
-// ---- conjugate-gradient vector kernels with scalars kept on the device ----
+// ---- conjugate-gradient kernels with per-mesh scalars kept on the device ----
+// Every mesh of a batch runs its own CG (its own alpha, beta and stopping
+// test); a mesh that has converged is frozen. Vertices [0, n) are the updated
+// ones; P points at kPlainCount x nM doubles, A at the accumulators
+// (AccDoubles(nM)), F at 2 nM + 1 ints.
 
-// acc[slot] += sum_i a_i . b_i over i < n (xyz), FP64; blockDim = 256
-__global__ __launch_bounds__(256) void Dot3(float4 const* __restrict__ a, float4 const* __restrict__ b, int n,
-                                            double* __restrict__ acc)
+__device__ __forceinline__ double Dot3(float4 const& u, float4 const& w)
 {
-    __shared__ double red[4];
-    double s = 0.;
-    int const stride = static_cast<int>(gridDim.x * blockDim.x);
-    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride)
+    return double(u.x) * w.x + double(u.y) * w.y + double(u.z) * w.z;
+}
+
+__device__ __forceinline__ float4 ApplyInverse(float4 const* __restrict__ dinv, int i, float4 const& r)
+{
+    float4 const A = dinv[2 * i], B = dinv[2 * i + 1]; // (00, 01, 02, 11), (12, 22)
+    return make_float4(A.x * r.x + A.y * r.y + A.z * r.z, A.y * r.x + A.w * r.y + B.x * r.z,
+                       A.z * r.x + B.x * r.y + B.y * r.z, 0.f);
+}
+
+// r = b - q (q = H y); z = D^-1 r when dinv; accumulates b.b, r.r (and r.z)
+__global__ __launch_bounds__(256) void CgInitResidual(float4 const* __restrict__ b, float4 const* __restrict__ q,
+                                                      float4* __restrict__ r, float4* __restrict__ z,
+                                                      float4 const* __restrict__ dinv, int const* __restrict__ meshOf,
+                                                      int n, int nM, double* __restrict__ A)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    double bb = 0., rr = 0., rz = 0.;
+    int m = -1;
+    if (i < n)
     {
-        float4 const u = a[i], w = b[i];
-        s += double(u.x) * w.x + double(u.y) * w.y + double(u.z) * w.z;
+        m = meshOf[i];
+        float4 const bi = b[i], qi = q[i];
+        float4 const ri = make_float4(bi.x - qi.x, bi.y - qi.y, bi.z - qi.z, 0.f);
+        r[i] = ri;
+        bb = Dot3(bi, bi);
+        rr = Dot3(ri, ri);
+        if (dinv)
+        {
+            float4 const zi = ApplyInverse(dinv, i, ri);
+            z[i] = zi;
+            rz = Dot3(ri, zi);
+        }
     }
-    for (int off = 32; off > 0; off >>= 1)
-        s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0)
-        red[threadIdx.x / 64] = s;
+    BlockAccumulate(bb, m, A, kAccBb, nM);
+    BlockAccumulate(rr, m, A, kAccRr, nM);
+    if (dinv)
+        BlockAccumulate(rz, m, A, kAccRzNew, nM);
+}
+
+// y += alpha p, r -= alpha q (alpha = rz / pq of the vertex's mesh, 0 when done);
+// with dinv: z = D^-1 r and r.z accumulated; without: z = 0 (a Gauss-Seidel
+// preconditioner pass follows). r.r accumulated.
+__global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* __restrict__ r,
+                                                float4* __restrict__ z, float4 const* __restrict__ p,
+                                                float4 const* __restrict__ q, float4 const* __restrict__ dinv,
+                                                int const* __restrict__ meshOf, int n, int nM,
+                                                double const* __restrict__ P, double* __restrict__ A,
+                                                int const* __restrict__ F)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    double rr = 0., rz = 0.;
+    int m = -1;
+    if (i < n)
+    {
+        m = meshOf[i];
+        float4 ri = r[i];
+        if (!F[kDone * nM + m])
+        {
+            double const den = AccSum(A, kAccPq, nM, m);
+            float const alpha = (den != 0.) ? static_cast<float>(P[kRz * nM + m] / den) : 0.f;
+            float4 const pi = p[i], qi = q[i];
+            float4 yi = y[i];
+            yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
+            ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
+            y[i] = yi;
+            r[i] = ri;
+        }
+        rr = Dot3(ri, ri);
+        if (dinv)
+        {
+            float4 const zi = ApplyInverse(dinv, i, ri);
+            z[i] = zi;
+            rz = Dot3(ri, zi);
+        }
+        else
+        {
+            z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    BlockAccumulate(rr, m, A, kAccRr, nM);
+    if (dinv)
+        BlockAccumulate(rz, m, A, kAccRzNew, nM);
+}
+
+// Per mesh, after an iteration's accumulations: stopping test |r| <= tol |b|,
+// beta = rz_new / rz, rotate rz, clear the accumulators, count live meshes.
+// first: the initialisation (takes b.b, counts no iteration).
+__global__ __launch_bounds__(256) void CgScalars(int nM, double tol2, int first, double* __restrict__ P,
+                                                 double* __restrict__ A, int* __restrict__ F)
+{
+    __shared__ int live;
+    if (threadIdx.x == 0)
+        live = 0;
+    __syncthreads();
+    int count = 0;
+    for (int m = static_cast<int>(threadIdx.x); m < nM; m += static_cast<int>(blockDim.x))
+    {
+        if (first)
+            P[kBb * nM + m] = AccSum(A, kAccBb, nM, m);
+        int done = F[kDone * nM + m];
+        double const rr = AccSum(A, kAccRr, nM, m), bb = P[kBb * nM + m];
+        if (!done)
+        {
+            if (!first)
+                F[kIters * nM + m] += 1;
+            P[kRrLast * nM + m] = rr;
+            if (!(bb > 0.) || rr <= tol2 * bb)
+                done = 1;
+        }
+        double const rzNew = AccSum(A, kAccRzNew, nM, m), rz = P[kRz * nM + m];
+        P[kBeta * nM + m] = (!done && rz != 0.) ? rzNew / rz : 0.;
+        P[kRz * nM + m] = rzNew;
+        F[kDone * nM + m] = done;
+        for (int arr = 0; arr < kAccCount; ++arr)
+            for (int k = 0; k < kSpread; ++k)
+                *AccSlot(A, arr, nM, m, k) = 0.;
+        count += done ? 0 : 1;
+    }
+    if (count)
+        atomicAdd(&live, count);
     __syncthreads();
     if (threadIdx.x == 0)
-        atomicAdd(acc, red[0] + red[1] + red[2] + red[3]);
+        F[2 * nM] = live;
 }
 
-// y += alpha p; r -= alpha q with alpha = num / den (device scalars)
-__global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* __restrict__ r,
-                                                float4 const* __restrict__ p, float4 const* __restrict__ q, int n,
-                                                double const* __restrict__ num, double const* __restrict__ den)
+// p = z + beta p (beta of the vertex's mesh; frozen meshes keep p)
+__global__ __launch_bounds__(256) void CgDirection(float4* __restrict__ p, float4 const* __restrict__ z,
+                                                   int const* __restrict__ meshOf, int n, int nM,
+                                                   double const* __restrict__ P, int const* __restrict__ F)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n)
         return;
-    double const d = *den;
-    float const alpha = (d != 0.) ? static_cast<float>(*num / d) : 0.f;
-    float4 const pi = p[i], qi = q[i];
-    float4 yi = y[i], ri = r[i];
-    yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
-    ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
-    y[i] = yi;
-    r[i] = ri;
-}
-
-// p = z + beta p with beta = num / den (device scalars)
-__global__ __launch_bounds__(256) void CgDirection(float4* __restrict__ p, float4 const* __restrict__ z, int n,
-                                                   double const* __restrict__ num, double const* __restrict__ den)
-{
-    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n)
+    int const m = meshOf[i];
+    if (F[kDone * nM + m])
         return;
-    double const d = *den;
-    float const beta = (d != 0.) ? static_cast<float>(*num / d) : 0.f;
+    float const beta = static_cast<float>(P[kBeta * nM + m]);
     float4 const zi = z[i];
-    float4 pi = p[i];
+    float4 const pi = p[i];
     p[i] = make_float4(zi.x + beta * pi.x, zi.y + beta * pi.y, zi.z + beta * pi.z, 0.f);
 }
 
-// dst = a - b (xyz), n entries
-__global__ __launch_bounds__(256) void Sub4(float4* __restrict__ dst, float4 const* __restrict__ a,
-                                            float4 const* __restrict__ b, int n)
+// Clear n doubles and m ints
+__global__ __launch_bounds__(256) void ZeroScalars(double* __restrict__ S, int n, int* __restrict__ F, int m)
+{
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n + m;
+         i += static_cast<int>(gridDim.x * blockDim.x))
+    {
+        if (i < n)
+            S[i] = 0.;
+        else
+            F[i - n] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void Copy4(float4* __restrict__ dst, float4 const* __restrict__ src, int n)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n)
-        return;
-    float4 const u = a[i], w = b[i];
-    dst[i] = make_float4(u.x - w.x, u.y - w.y, u.z - w.z, 0.f);
+    if (i < n)
+        dst[i] = src[i];
 }
 
 __global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)

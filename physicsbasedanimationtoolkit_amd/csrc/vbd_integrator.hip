@@ -318,7 +318,9 @@ void DownloadVec(pbat_vbd_integrator& h, float4 const* src, float* dst)
     }
 }
 
-void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
+// vertexOffsets (batches): first vertex of each mesh in d's numbering, count + 1
+// entries; vertices of one colour are then grouped by mesh.
+void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64_t> const* vertexOffsets = nullptr)
 {
     if (!d || !out)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null argument");
@@ -364,7 +366,22 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
     h->kD = static_cast<float>(d->kD);
     h->detHZero = static_cast<float>(d->detHZero);
 
-    // Colour-major internal numbering; degree-sorted (stable) within a colour.
+    // Mesh of each vertex (batches; one mesh otherwise)
+    std::vector<int> meshOfOrig(static_cast<size_t>(nV), 0);
+    if (vertexOffsets)
+    {
+        auto const& o = *vertexOffsets;
+        if (o.size() < 2 || o.front() != 0 || o.back() != nV)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid mesh vertex offsets");
+        for (size_t b = 0; b + 1 < o.size(); ++b)
+            for (int64_t i = o[b]; i < o[b + 1]; ++i)
+                meshOfOrig[i] = static_cast<int>(b);
+        h->meshOffsets = o;
+    }
+    h->nMeshes = vertexOffsets ? static_cast<int>(vertexOffsets->size()) - 1 : 1;
+
+    // Colour-major internal numbering; within a colour grouped by mesh, then
+    // sorted by degree (stable).
     std::vector<char> seen(static_cast<size_t>(nV), 0);
     h->perm.reserve(static_cast<size_t>(nV));
     h->colorBegin.push_back(0);
@@ -380,6 +397,8 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
             part.push_back(static_cast<int>(i));
         }
         std::stable_sort(part.begin(), part.end(), [&](int a, int b) {
+            if (meshOfOrig[a] != meshOfOrig[b])
+                return meshOfOrig[a] < meshOfOrig[b];
             return (d->GVGp[a + 1] - d->GVGp[a]) > (d->GVGp[b + 1] - d->GVGp[b]);
         });
         h->perm.insert(h->perm.end(), part.begin(), part.end());
@@ -391,6 +410,13 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out)
     h->iperm.assign(static_cast<size_t>(nV), 0);
     for (int64_t v = 0; v < nV; ++v)
         h->iperm[h->perm[v]] = static_cast<int>(v);
+    {
+        std::vector<int> meshOf(static_cast<size_t>(nV));
+        for (int64_t v = 0; v < nV; ++v)
+            meshOf[v] = meshOfOrig[h->perm[v]];
+        h->meshOf.upload(meshOf, h->stream);
+        PBAT_HIP(hipStreamSynchronize(h->stream));
+    }
 
     // Validate the colouring: no element may hold two vertices of one colour.
     {
@@ -894,8 +920,7 @@ extern "C" int pbat_vbd_batch_create(const pbat_vbd_data* meshes, int32_t count,
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null output handle");
         UnionData U;
         BuildUnion(meshes, count, U);
-        Create(&U.d, out);
-        (*out)->meshOffsets = U.vertexOffsets;
+        Create(&U.d, out, &U.vertexOffsets);
     });
 }
 
@@ -1102,6 +1127,18 @@ extern "C" int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3])
 extern "C" int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h)
 {
     return Guard([&] { PBAT_HIP(hipStreamSynchronize(Handle(h).stream)); });
+}
+
+extern "C" int pbat_vbd_device_memory(int64_t* current, int64_t* peak, int32_t reset_peak)
+{
+    return Guard([&] {
+        if (current)
+            *current = gDeviceBytes.load();
+        if (peak)
+            *peak = gDevicePeak.load();
+        if (reset_peak)
+            gDevicePeak.store(gDeviceBytes.load());
+    });
 }
 
 extern "C" int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPartitions, int64_t* nUpdated,

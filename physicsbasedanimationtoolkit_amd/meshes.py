@@ -178,3 +178,53 @@ def batched_scenes(count: int = 256, lo: int = 4, hi: int = 24, seed: int = 0, *
         scenes.append(Scene(X, E, mu, lam, np.full(nT, 1e3), bottom_vertices(X),
                             gravity(X.shape[1]), dt, iterations, steps, name=f"batch{b}"))
     return scenes
+
+
+class DiffScene(Scene):
+    """A differentiable-rollout workload: a Scene plus hidden per-tet log Y (the
+    parameters the target rollout used), the initial guess of log Y and v0."""
+
+    def __init__(self, scene: Scene, log_y_hidden, log_y_guess, v0):
+        super().__init__(scene.X, scene.E, scene.mue, scene.lambdae, scene.rhoe, scene.dbc, scene.aext,
+                         scene.dt, scene.iterations, scene.steps, scene.name)
+        self.log_y_hidden = log_y_hidden
+        self.log_y_guess = log_y_guess
+        self.v0 = v0
+
+
+def _random_unit_vectors(rng, n: int) -> np.ndarray:
+    u = rng.normal(size=(3, n))
+    return u / np.linalg.norm(u, axis=0, keepdims=True)
+
+
+def differentiable_scene(n: int = 32, seed: int = 0, *, nx=None, ny=None, nz=None, dt: float = 1.0 / 60.0,
+                         iterations: int = 20, steps: int = 20) -> DiffScene:
+    """n^3 jittered grid, hidden per-tet log Y log-uniform in [1e5, 1e7] (nu = 0.45,
+    rho = 1000), bottom face fixed, gravity; v0 = 0.5 x a random unit vector per
+    vertex (0 on Dirichlet vertices); the material the scene is built with is the
+    guess Y = 1e6 everywhere. 32^3: 35,937 vertices, 196,608 tets."""
+    nx = n if nx is None else nx
+    X, E = cube_tet_grid(nx, ny, nz, jitter=0.1, seed=seed)
+    rng = np.random.default_rng(seed + 7919)
+    nT = E.shape[1]
+    log_y_hidden = np.log(log_uniform(rng, 1e5, 1e7, nT))
+    mu, lam = lame_coefficients(np.full(nT, 1e6), 0.45)
+    dbc = bottom_vertices(X)
+    v0 = 0.5 * _random_unit_vectors(rng, X.shape[1])
+    v0[:, dbc] = 0.0
+    s = Scene(X, E, mu, lam, np.full(nT, 1e3), dbc, gravity(X.shape[1]), dt, iterations, steps,
+              name=f"diff_{nx}x{ny or nx}x{nz or nx}")
+    return DiffScene(s, log_y_hidden, np.full(nT, np.log(1e6)), v0)
+
+
+def batched_differentiable_scenes(count: int = 64, lo: int = 8, hi: int = 20, seed: int = 0, *,
+                                  dt: float = 1.0 / 120.0, iterations: int = 10, steps: int = 200):
+    """``count`` differentiable scenes with grid dims U{lo..hi} per axis (h = 1/120,
+    10 iterations, 200 steps by default)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(count):
+        d = rng.integers(lo, hi + 1, size=3)
+        out.append(differentiable_scene(int(d[0]), seed=seed * 100003 + b, nx=int(d[0]), ny=int(d[1]),
+                                        nz=int(d[2]), dt=dt, iterations=iterations, steps=steps))
+    return out

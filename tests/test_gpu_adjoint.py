@@ -77,7 +77,7 @@ def test_hessian_vector_matches_fp64(gpu_device):
     assert np.all(hv[:, ~free] == 0)
 
 
-@pytest.mark.parametrize("solver,iters", [("pcg", 300), ("gauss_seidel", 20000)])
+@pytest.mark.parametrize("solver,iters", [("pcg", 300), ("pcg_jacobi", 600), ("gauss_seidel", 20000)])
 def test_adjoint_solve_matches_fp64(gpu_device, solver, iters):
     X, E, mu, lam, v0 = diff_scene(6)
     g = make_gpu(X, E, mu, lam, v0)
@@ -109,7 +109,7 @@ def gpu_trajectory(g, x0, v0, steps, iterations):
     return xs
 
 
-@pytest.mark.parametrize("solver", ["pcg", "gauss_seidel"])
+@pytest.mark.parametrize("solver", ["pcg", "pcg_jacobi", "gauss_seidel"])
 def test_rollout_gradient_matches_fp64_ift(gpu_device, solver):
     X, E, mu, lam, v0 = diff_scene(8, seed=2)
     steps, iters = 5, 20
@@ -125,8 +125,8 @@ def test_rollout_gradient_matches_fp64_ift(gpu_device, solver):
     g.v = v0f
     g.rollout_forward(DT, iters, steps, checkpoint_every=2)
     np.testing.assert_array_equal(g.x, xs[-1].T.astype(np.float32))
-    kw = dict(solver=solver, tolerance=1e-6, max_iterations=300 if solver == "pcg" else 20000,
-              check_every=10 if solver == "pcg" else 100)
+    gs = solver == "gauss_seidel"
+    kw = dict(solver=solver, tolerance=1e-6, max_iterations=20000 if gs else 600, check_every=100 if gs else 10)
     r = g.rollout_backward(w, u, **kw)
     print(solver, r["stats"])
     for key, refk in (("log_young_modulus", ref[0]), ("v0", ref[1].T), ("x0", ref[2].T)):
@@ -178,9 +178,11 @@ def test_checkpoint_interval_does_not_change_gradients(gpu_device):
         g.v = v0
         g.rollout_forward(DT, 10, 7, checkpoint_every=every)
         out.append(g.rollout_backward(w, None, tolerance=1e-6))
+    # The recomputed segments are bitwise equal to the forward; the solves'
+    # FP64 atomic dot products may round differently from run to run.
     for r in out[1:]:
         for k in ("log_young_modulus", "v0", "x0"):
-            np.testing.assert_array_equal(r[k], out[0][k])
+            assert rel(r[k], out[0][k]) <= 1e-5, k
 
 
 def test_backward_restores_end_state(gpu_device):
@@ -243,7 +245,10 @@ def test_torch_autograd_function(gpu_device):
     np.testing.assert_allclose(x0.grad.numpy(), r["x0"], rtol=1e-6, atol=1e-9)
 
 
-def test_batched_gradients_match_per_mesh(gpu_device):
+@pytest.mark.parametrize("solver", ["pcg", "pcg_jacobi"])
+def test_batched_gradients_match_per_mesh(gpu_device, solver):
+    """Each mesh of a batch runs its own CG, so the batch reproduces the per-mesh
+    loop up to rounding."""
     scenes = meshes.batched_scenes(3, 3, 6, seed=4, per_tet_log_y=True)
     datas, singles, ws = [], [], []
     rng = np.random.default_rng(0)
@@ -255,16 +260,17 @@ def test_batched_gradients_match_per_mesh(gpu_device):
         ws.append(rng.normal(size=s.X.shape))
     batch = pbat.gpu.vbd.BatchIntegrator(datas)
     batch.rollout_forward(DT, 10, 4, checkpoint_every=2)
-    rb = batch.rollout_backward(np.concatenate(ws, axis=1), None, tolerance=1e-6)
+    rb = batch.rollout_backward(np.concatenate(ws, axis=1), None, solver=solver, tolerance=1e-6, check_every=1)
     t0 = 0
     for b, (g, w, s) in enumerate(zip(singles, ws, scenes)):
         g.rollout_forward(DT, 10, 4, checkpoint_every=2)
-        r = g.rollout_backward(w, None, tolerance=1e-6)
+        r = g.rollout_backward(w, None, solver=solver, tolerance=1e-6, check_every=1)
+        assert r["stats"]["max_relative_residual"] <= 1e-6
         sl = slice(batch.offsets[b], batch.offsets[b + 1])
         nT = s.E.shape[1]
-        assert rel(rb["v0"][:, sl], r["v0"]) <= GRAD_TOL
-        assert rel(rb["x0"][:, sl], r["x0"]) <= GRAD_TOL
-        assert rel(rb["log_young_modulus"][t0:t0 + nT], r["log_young_modulus"]) <= GRAD_TOL
+        assert rel(rb["v0"][:, sl], r["v0"]) <= 1e-5
+        assert rel(rb["x0"][:, sl], r["x0"]) <= 1e-5
+        assert rel(rb["log_young_modulus"][t0:t0 + nT], r["log_young_modulus"]) <= 1e-5
         t0 += nT
 
 
