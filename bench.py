@@ -15,7 +15,11 @@ The JSON line also carries
             kernel's average launch time measured with HIP events on the
             integrator's stream, against 8 TB/s;
   cpu_baseline: the FP64 CPU oracle (same mesh, colouring and iterations) on a
-            bounded sample (one step) using the host's cores.
+            bounded sample (one step) using the host's cores;
+  batched: 256 meshes stepped together vs the plain per-mesh loop;
+  adjoint: fwd+adjoint steps/s of a differentiable 20-step rollout on a 32^3 mesh;
+  batched_adjoint: fwd+adjoint steps/s of 64 meshes over 200 steps vs the
+            plain per-mesh loop, with the peak device memory.
 """
 from __future__ import annotations
 
@@ -88,6 +92,171 @@ def bench_batched(args, torch):
             "sweep_shape": batch.sweep_config()}
 
 
+def _diff_data(pbat, s):
+    return (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+            .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).with_velocity(s.v0).construct())
+
+
+def _rel(a, b):
+    import numpy as np
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def bench_adjoint(args, torch):
+    """Differentiable VBD on one mesh: 32^3 grid (35,937 vertices, 196,608 tets),
+    hidden per-tet log Y log-uniform in [1e5, 1e7], target = 20-step rollout
+    (h = 1/60, 20 iterations) from v0 = 0.5 x random unit vectors; loss
+    L = sum |x_T - x_T^target|^2 from log Y = ln 1e6. One fwd+adjoint pass = the
+    20-step forward (state kept every 10 steps) + the IFT backward (one
+    block-Jacobi PCG solve per step, tolerance 1e-6 per mesh) giving dL/dlogY,
+    dL/dv0, dL/dx0. Also: central differences of the device forward on 64
+    random log Y entries (the forward is truncated at 20 iterations, so these
+    measure truncation as much as the adjoint)."""
+    import numpy as np
+    from physicsbasedanimationtoolkit_amd import meshes, pbat, _native
+
+    s = meshes.differentiable_scene(args.adjoint_grid, seed=0)
+    g = pbat.gpu.vbd.Integrator(_diff_data(pbat, s))
+    g.autotune(s.dt, sweeps=5)
+    v0 = g.v.copy()
+    dt, it, T, every = s.dt, s.iterations, s.steps, 10
+    solver = dict(solver="pcg_jacobi", tolerance=1e-6, max_iterations=2000, check_every=10)
+
+    def forward(logy):
+        g.set_log_young_modulus(logy)
+        g.x = s.X
+        g.v = v0
+        g.rollout_forward(dt, it, T, every)
+
+    forward(s.log_y_hidden)
+    target = g.x.astype(np.float64)
+    forward(s.log_y_guess)
+    g.rollout_backward(2 * (g.x - target), None, **solver)  # warm-up
+    g.synchronize()
+    reps = args.adjoint_reps
+    tf = tb = 0.0
+    stats = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        forward(s.log_y_guess)
+        g.synchronize()
+        t1 = time.perf_counter()
+        xT = g.x.astype(np.float64)
+        r = g.rollout_backward(2 * (xT - target), None, **solver)
+        tb += time.perf_counter() - t1
+        tf += t1 - t0
+        stats = r["stats"]
+    loss = float(((xT - target) ** 2).sum())
+    fd = None
+    if args.adjoint_fd:
+        rng = np.random.default_rng(1)
+        idx = rng.choice(s.n_tets, 64, replace=False)
+        eps = 1e-2
+        pairs = []
+        for e in idx:
+            lv = []
+            for sgn in (1.0, -1.0):
+                ly = s.log_y_guess.copy()
+                ly[e] += sgn * eps
+                forward(ly)
+                lv.append(float(((g.x.astype(np.float64) - target) ** 2).sum()))
+            pairs.append(((lv[0] - lv[1]) / (2 * eps), float(r["log_young_modulus"][e])))
+        pairs = np.array(pairs)
+        scale = np.abs(pairs[:, 0]).max()
+        err = np.abs(pairs[:, 0] - pairs[:, 1]) / scale
+        fd = {"params": 64, "eps": eps, "median_err_rel_to_max": float(np.median(err)),
+              "max_err_rel_to_max": float(err.max()),
+              "note": "FD of the 20-iteration (unconverged) forward vs the IFT gradient of the converged step"}
+    out = {"workload": f"differentiable VBD, {args.adjoint_grid}^3 jittered grid ({s.n_vertices} verts, "
+                       f"{s.n_tets} tets), hidden per-tet log Y, {T} steps h=1/60 x {it} iterations, "
+                       f"v0 = 0.5 random unit vectors, L = |x_T - target|^2 from log Y = ln 1e6",
+           "fwd_adjoint_steps_per_s": T * reps / (tf + tb), "forward_steps_per_s": T * reps / tf,
+           "forward_ms": 1e3 * tf / reps, "adjoint_ms": 1e3 * tb / reps, "loss": loss,
+           "solver": solver, "checkpoint_every": every, "adjoint_stats": stats, "fd_check": fd,
+           "device_bytes": _native.device_memory()[0]}
+    log(f"[bench] adjoint 32^3: fwd {out['forward_ms']:.1f} ms + adjoint {out['adjoint_ms']:.1f} ms per "
+        f"{T}-step rollout -> {out['fwd_adjoint_steps_per_s']:.1f} fwd+adj steps/s; {stats}")
+    del g
+    return out
+
+
+def bench_batched_adjoint(args, torch):
+    """Batched + differentiable VBD: 64 meshes with grid dims U{8..20} per axis
+    (236,010 vertices, 1,158,930 tets), hidden per-tet log Y, 200 steps at
+    h = 1/120 x 10 iterations, state kept every 10 steps and recomputed in the
+    backward pass, L = sum |x_T - x_T^target|^2 over the batch, gradient wrt
+    per-tet log Y (and v0, x0). Compared with the plain per-mesh loop: one
+    Integrator per mesh doing the same fwd+adjoint rollout, same sweep shape."""
+    import numpy as np
+    from physicsbasedanimationtoolkit_amd import meshes, pbat, _native
+
+    scenes = meshes.batched_differentiable_scenes(args.batch_adjoint_meshes, 8, 20, seed=0)
+    datas = [_diff_data(pbat, s) for s in scenes]
+    b = pbat.gpu.vbd.BatchIntegrator(datas)
+    b.autotune(scenes[0].dt, sweeps=3)
+    shape = b.sweep_config()
+    dt, it, T, every = scenes[0].dt, scenes[0].iterations, args.batch_adjoint_steps, 10
+    solver = dict(solver="pcg_jacobi", tolerance=1e-6, max_iterations=2000, check_every=10)
+    hidden = np.concatenate([s.log_y_hidden for s in scenes])
+    guess = np.concatenate([s.log_y_guess for s in scenes])
+    Xb = np.concatenate([s.X for s in scenes], axis=1)
+    vb = b.v.copy()
+
+    def forward(logy):
+        b.set_log_young_modulus(logy)
+        b.x = Xb
+        b.v = vb
+        b.rollout_forward(dt, it, T, every)
+
+    forward(hidden)
+    target = b.x.astype(np.float64)
+    _native.device_memory(reset_peak=True)
+    t0 = time.perf_counter()
+    forward(guess)
+    b.synchronize()
+    t1 = time.perf_counter()
+    rb = b.rollout_backward(2 * (b.x - target), None, **solver)
+    tb = time.perf_counter() - t1
+    tf = t1 - t0
+    mem_now, mem_peak = _native.device_memory()
+    log(f"[bench] batched adjoint: {len(scenes)} meshes, fwd {tf:.2f}s + adjoint {tb:.2f}s per {T}-step rollout "
+        f"-> {T / (tf + tb):.2f} steps/s; peak device memory {mem_peak / 2**20:.0f} MiB; {rb['stats']}")
+    # plain per-mesh loop, same sweep shape so that each forward is bitwise the batch's
+    loop_t = 0.0
+    errs = []
+    t0e = 0
+    for k, (s, d) in enumerate(zip(scenes, datas)):
+        gi = pbat.gpu.vbd.Integrator(d)
+        gi.set_sweep_config(shape["lanes_per_vertex"], shape["incidences_per_lane"], shape["xcd_remap"],
+                            shape["nontemporal"], shape["fixed_stride"])
+        sl = slice(b.offsets[k], b.offsets[k + 1])
+        nT = s.n_tets
+        tk = target[:, sl]
+        gi.rollout_forward(dt, 2, 2, every)  # build the step graph outside the timed region
+        gi.synchronize()
+        t0 = time.perf_counter()
+        gi.set_log_young_modulus(s.log_y_guess)
+        gi.x = s.X
+        gi.v = vb[:, sl]
+        gi.rollout_forward(dt, it, T, every)
+        r = gi.rollout_backward(2 * (gi.x - tk), None, **solver)
+        loop_t += time.perf_counter() - t0
+        errs.append(_rel(rb["log_young_modulus"][t0e:t0e + nT], r["log_young_modulus"]))
+        t0e += nT
+        del gi
+    log(f"[bench] per-mesh loop: {loop_t:.2f}s -> {T / loop_t:.2f} steps/s; batched is {loop_t / (tf + tb):.1f}x; "
+        f"max rel grad diff batch vs loop {max(errs):.2e}")
+    return {"workload": f"{len(scenes)} jittered meshes, grid dims U{{8..20}} per axis ({int(b.offsets[-1])} verts, "
+                        f"{sum(s.n_tets for s in scenes)} tets), hidden per-tet log Y, {T} steps h=1/120 x {it} "
+                        f"iterations, checkpoint every {every}, L = |x_T - target|^2, grad wrt per-tet log Y",
+            "fwd_adjoint_steps_per_s": T / (tf + tb), "per_mesh_loop_steps_per_s": T / loop_t,
+            "speedup_vs_loop": loop_t / (tf + tb), "forward_s": tf, "adjoint_s": tb, "loop_s": loop_t,
+            "max_rel_grad_diff_vs_loop": max(errs), "peak_device_bytes": mem_peak,
+            "solver": solver, "adjoint_stats": rb["stats"], "sweep_shape": shape}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +276,12 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout measurements")
+    ap.add_argument("--adjoint-grid", type=int, default=32)
+    ap.add_argument("--adjoint-reps", type=int, default=3)
+    ap.add_argument("--adjoint-fd", action="store_true", help="central differences on 64 log Y entries")
+    ap.add_argument("--batch-adjoint-meshes", type=int, default=64)
+    ap.add_argument("--batch-adjoint-steps", type=int, default=200)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MB (counter calibration) and exit")
     args = ap.parse_args()
@@ -187,6 +362,10 @@ def main():
         f"{bytes_per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
 
     batched = None if args.no_batched else bench_batched(args, torch)
+    adjoint = batched_adjoint = None
+    if not args.no_adjoint:
+        adjoint = bench_adjoint(args, torch)
+        batched_adjoint = bench_batched_adjoint(args, torch)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -204,7 +383,7 @@ def main():
         log(f"[bench] cpu baseline {cpu['value']:.4e} vu/s on {threads} threads")
 
     out = {
-        "metric": "VBD vertex-updates/s (plain, 1 GPU, <=1e-5 rel err)",
+        "metric": "VBD vertex-updates/s + fwd+adjoint steps/s vs plain, 1 GPU, \u22641e-5 rel err",
         "value": value,
         "unit": "vertex-updates/s",
         "n_gpus": 1,
@@ -227,6 +406,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
+        "batched_adjoint": batched_adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -135,6 +135,38 @@ def test_rollout_gradient_matches_fp64_ift(gpu_device, solver):
         assert e <= GRAD_TOL, key
 
 
+def test_differentiable_workload_gradient_matches_fp64(gpu_device):
+    """The 32^3 differentiable workload's settings on a 12^3 grid (the FP64 direct
+    solve of the 32^3 step Hessian takes minutes): hidden per-tet log Y, v0 = 0.5 x
+    random unit vectors, 20 steps x 20 iterations, L = |x_T - target|^2 from
+    log Y = ln 1e6; gradients wrt log Y and v0 vs the FP64 IFT gradient along the
+    device trajectory."""
+    s = meshes.differentiable_scene(12, seed=3)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+            .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).with_velocity(s.v0).construct())
+    g = pbat.gpu.vbd.Integrator(data)
+    v0 = g.v.copy()
+    g.set_log_young_modulus(s.log_y_hidden)
+    g.rollout_forward(s.dt, s.iterations, s.steps, 10)
+    target = g.x.astype(np.float64)
+    g.set_log_young_modulus(s.log_y_guess)
+    xs = gpu_trajectory(g, s.X, v0, s.steps, s.iterations)
+    dLdx = 2 * (xs[-1].T - target)
+    g.x = s.X
+    g.v = v0
+    g.rollout_forward(s.dt, s.iterations, s.steps, 10)
+    r = g.rollout_backward(dLdx, None, solver="pcg_jacobi", tolerance=1e-6, max_iterations=3000)
+    o = O.OracleVbd(s.X, s.E, v=s.v0, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
+                    nthreads=8)
+    A.set_log_young_modulus(o, s.log_y_guess)
+    ref = A.ift_gradient(o, xs, s.dt, dLdx.T)
+    e_y = rel(r["log_young_modulus"], ref[0])
+    e_v = rel(r["v0"], ref[1].T)
+    print(f"12^3 workload: {r['stats']} rel err log Y {e_y:.3e} v0 {e_v:.3e}")
+    assert e_y <= GRAD_TOL
+    assert e_v <= GRAD_TOL
+
+
 def test_gradient_matches_fd_of_device_forward(gpu_device):
     """Directional central differences of the FP32 device rollout on a soft
     material where 200 iterations converge every step."""
