@@ -54,6 +54,12 @@ extern "C" {
 
 #define PBAT_VBD_MAX_COLORS 128
 
+/* sim/vbd/Enums.h:21-28 EAccelerationStrategy; gpu::vbd::Integrator's factory
+ * (gpu/vbd/Integrator.cu:15-33) picks the integrator from it. Only None and
+ * Chebyshev (gpu/impl/vbd/ChebyshevIntegrator.cu) are available here. */
+#define PBAT_VBD_ACCELERATION_NONE 0
+#define PBAT_VBD_ACCELERATION_CHEBYSHEV 1
+
 /* Message of the last failure on the calling thread ("" if none). */
 const char* pbat_vbd_last_error(void);
 
@@ -114,6 +120,8 @@ typedef struct pbat_vbd_data {
     int32_t strategy;        /* PBAT_VBD_INIT_* */
     double kD;               /* Rayleigh damping coefficient */
     double detHZero;         /* numerical zero of |det H| */
+    int32_t acceleration;    /* PBAT_VBD_ACCELERATION_* (sim/vbd/Enums.h:21-28) */
+    double rho;              /* Chebyshev estimated spectral radius, 0 < rho < 1 (sim/vbd/Data.h:233-234) */
 } pbat_vbd_data;
 
 typedef struct pbat_vbd_integrator pbat_vbd_integrator;

@@ -79,6 +79,8 @@ struct Data
     EInitializationStrategy strategy = EInitializationStrategy::AdaptivePbat;
     double kD = 0.;
     double detHZero = 1e-7;
+    int32_t acceleration = PBAT_VBD_ACCELERATION_NONE; // sim/vbd/Data.h:231
+    double rho = 1.;                                   // sim/vbd/Data.h:234
     std::vector<double> mue, lambdae;
 
     template <class TX, class TE>
@@ -128,6 +130,13 @@ struct Data
     Data& WithRayleighDamping(double kDIn)
     {
         kD = kDIn;
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:141-146 */
+    Data& WithChebyshevAcceleration(double rhoIn)
+    {
+        rho = rhoIn;
+        acceleration = PBAT_VBD_ACCELERATION_CHEBYSHEV;
         return *this;
     }
     Data& WithHessianDeterminantZeroUnder(double zero)
@@ -215,6 +224,8 @@ class Integrator
         d.strategy = static_cast<int32_t>(data.strategy);
         d.kD = data.kD;
         d.detHZero = data.detHZero;
+        d.acceleration = data.acceleration;
+        d.rho = data.rho;
         Check(pbat_vbd_integrator_create(&d, &mImpl));
         mVertices = d.nV;
     }

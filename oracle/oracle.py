@@ -105,7 +105,15 @@ class _State(ctypes.Structure):
         ("nPartitions", ctypes.c_int64), ("Pptr", ctypes.c_void_p), ("Padj", ctypes.c_void_p),
         ("strategy", ctypes.c_int), ("kD", ctypes.c_double), ("detHZero", ctypes.c_double),
         ("nthreads", ctypes.c_int), ("gpu_velocity_semantics", ctypes.c_int),
+        ("chebyshev", ctypes.c_int), ("rho", ctypes.c_double), ("xkm1", ctypes.c_void_p),
+        ("xkm2", ctypes.c_void_p),
     ]
+
+
+def chebyshev_omega(k: int, rho2: float, omega: float) -> float:
+    lib().ora_chebyshev_omega.restype = ctypes.c_double
+    lib().ora_chebyshev_omega.argtypes = [ctypes.c_int64, ctypes.c_double, ctypes.c_double]
+    return lib().ora_chebyshev_omega(int(k), float(rho2), float(omega))
 
 
 # ---- constitutive model ---------------------------------------------------
@@ -225,7 +233,7 @@ class OracleVbd:
     def __init__(self, X, E, *, v=None, aext=None, rhoe=None, mue=None, lambdae=None,
                  dbc=None, ordering=ORDER_LARGEST_DEGREE, selection=SELECT_LEAST_USED,
                  strategy=INIT_ADAPTIVE_PBAT, kD=0.0, detHZero=1e-7, nthreads=1,
-                 gpu_velocity_semantics=False, colors=None):
+                 gpu_velocity_semantics=False, colors=None, chebyshev_rho=None):
         Xv, Ev = _prep(X, E)
         nV, nT = Xv.shape[0], Ev.shape[0]
         self.nV, self.nT = nV, nT
@@ -263,6 +271,11 @@ class OracleVbd:
         self.detHZero = float(detHZero)
         self.nthreads = int(nthreads)
         self.gpu_velocity_semantics = bool(gpu_velocity_semantics)
+        # ChebyshevIntegrator (sim/vbd/ChebyshevIntegrator.cpp:10-13): previous iterates start at x
+        self.chebyshev = chebyshev_rho is not None
+        self.rho = 1.0 if chebyshev_rho is None else float(chebyshev_rho)
+        self.xkm1 = self.x.copy()
+        self.xkm2 = self.x.copy()
 
     def _state(self) -> _State:
         s = _State()
@@ -279,6 +292,8 @@ class OracleVbd:
         s.strategy, s.kD, s.detHZero = self.strategy, self.kD, self.detHZero
         s.nthreads = self.nthreads
         s.gpu_velocity_semantics = int(self.gpu_velocity_semantics)
+        s.chebyshev, s.rho = int(self.chebyshev), self.rho
+        s.xkm1, s.xkm2 = self.xkm1.ctypes.data, self.xkm2.ctypes.data
         return s
 
     # sim/vbd/Integrator.cpp:22-45

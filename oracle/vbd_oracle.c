@@ -649,6 +649,28 @@ static void substep_end(ora_vbd* s, int64_t vb, int64_t ve, double sdt)
             s->v[3 * i + d] = (s->x[3 * i + d] - s->xt[3 * i + d]) / sdt;
 }
 
+double ora_chebyshev_omega(int64_t k, double rho2, double omega)
+{
+    if (k == 0)
+        return 1.;
+    if (k == 1)
+        return 2. / (2. - rho2);
+    return 4. / 4. - rho2 * omega;
+}
+
+/* sim/vbd/Kernels.h:104-118 over vertices [vb, ve): x = omega (x - xkm2) + xkm2
+ * for k > 1, then xkm2 = xkm1, xkm1 = x. */
+static void chebyshev_update(ora_vbd* s, int64_t vb, int64_t ve, int64_t k, double omega)
+{
+    for (int64_t i = 3 * vb; i < 3 * ve; ++i)
+    {
+        if (k > 1)
+            s->x[i] = omega * (s->x[i] - s->xkm2[i]) + s->xkm2[i];
+        s->xkm2[i] = s->xkm1[i];
+        s->xkm1[i] = s->x[i];
+    }
+}
+
 static void* worker(void* arg)
 {
     worker_ctx* w = (worker_ctx*)arg;
@@ -659,7 +681,10 @@ static void* worker(void* arg)
     {
         substep_begin(s, vb, ve, w->sdt, w->sdt2);
         pthread_barrier_wait(w->bar);
+        double const rho2 = s->rho * s->rho;
+        double omega = 0.;
         for (int64_t k = 0; k < w->iterations; ++k)
+        {
             for (int64_t p = 0; p < s->nPartitions; ++p)
             {
                 int64_t pb, pe;
@@ -668,6 +693,13 @@ static void* worker(void* arg)
                     solve_vertex(s, s->Padj[q], w->sdt, w->sdt2);
                 pthread_barrier_wait(w->bar);
             }
+            if (s->chebyshev)
+            {
+                omega = ora_chebyshev_omega(k, rho2, omega);
+                chebyshev_update(s, vb, ve, k, omega);
+                pthread_barrier_wait(w->bar);
+            }
+        }
         substep_end(s, vb, ve, w->sdt);
         pthread_barrier_wait(w->bar);
     }
@@ -678,6 +710,8 @@ int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps)
 {
     if (substeps < 1 || iterations < 0)
         return ORA_ERR_ARG;
+    if (s->chebyshev && (!s->xkm1 || !s->xkm2))
+        return ORA_ERR_ARG;
     double const sdt = dt / (double)substeps;
     double const sdt2 = sdt * sdt;
     int nt = s->nthreads > 0 ? s->nthreads : 1;
@@ -686,10 +720,20 @@ int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps)
         for (int64_t st = 0; st < substeps; ++st)
         {
             substep_begin(s, 0, s->nV, sdt, sdt2);
+            double const rho2 = s->rho * s->rho;
+            double omega = 0.;
             for (int64_t k = 0; k < iterations; ++k)
+            {
                 for (int64_t p = 0; p < s->nPartitions; ++p)
                     for (int64_t q = s->Pptr[p]; q < s->Pptr[p + 1]; ++q)
                         solve_vertex(s, s->Padj[q], sdt, sdt2);
+                if (s->chebyshev)
+                {
+                    /* ChebyshevIntegrator::Solve (sim/vbd/ChebyshevIntegrator.cpp:15-32) */
+                    omega = ora_chebyshev_omega(k, rho2, omega);
+                    chebyshev_update(s, 0, s->nV, k, omega);
+                }
+            }
             substep_end(s, 0, s->nV, sdt);
         }
         return ORA_OK;

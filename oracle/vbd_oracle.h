@@ -106,7 +106,18 @@ typedef struct ora_vbd {
      * 1: GPU integrator semantics (vt keeps the previous substep's velocity,
      *    gpu/impl/vbd/Integrator.cu:329-347). Only AdaptiveVbd initialisation sees the difference. */
     int gpu_velocity_semantics;
+    /* Chebyshev semi-iterative acceleration (sim/vbd/ChebyshevIntegrator.cpp:15-32):
+     * on when chebyshev != 0, with estimated spectral radius rho and the previous
+     * two iterates xkm1, xkm2 (3*nV each, caller-owned). */
+    int chebyshev;
+    double rho;
+    double* xkm1;
+    double* xkm2;
 } ora_vbd;
+
+/* sim/vbd/Kernels.h:97-102, including its operator precedence: for k >= 2 the
+ * result is 4/4 - rho2*omega = 1 - rho2*omega. */
+double ora_chebyshev_omega(int64_t k, double rho2, double omega);
 
 /* sim/vbd/Integrator.cpp:22-45 */
 int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps);

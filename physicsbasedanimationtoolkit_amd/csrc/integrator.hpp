@@ -140,11 +140,14 @@ struct GraphKey
     int iterations = -1, substeps = -1, strategy = -1, lanes = -1, block = -1, perLane = -1;
     bool xcd = false, nt = false, ell = false;
     float kD = 0.f, detHZero = 0.f;
+    int acceleration = -1;
+    float rho = 0.f;
     bool operator==(GraphKey const& o) const
     {
         return dt == o.dt && iterations == o.iterations && substeps == o.substeps && strategy == o.strategy &&
                lanes == o.lanes && block == o.block && perLane == o.perLane && xcd == o.xcd && nt == o.nt &&
-               ell == o.ell && kD == o.kD && detHZero == o.detHZero;
+               ell == o.ell && kD == o.kD && detHZero == o.detHZero && acceleration == o.acceleration &&
+               rho == o.rho;
     }
 };
 
@@ -181,6 +184,9 @@ struct pbat_vbd_integrator
     pbat_vbd::RolloutState rollout;
 
     int strategy = PBAT_VBD_INIT_ADAPTIVE_PBAT;
+    int acceleration = PBAT_VBD_ACCELERATION_NONE;
+    float rho = 1.f;                            // Chebyshev estimated spectral radius
+    pbat_vbd::DeviceArray<float4> xkm1, xkm2;   // Chebyshev: previous two iterates
     float kD = 0.f;
     float detHZero = 1e-7f;
     int blockSize = 256;
@@ -216,7 +222,7 @@ struct pbat_vbd_integrator
             (void)hipStreamSynchronize(stream);
         xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
         tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
-        slotsRows.reset(), slotsEll.reset(), matBase.reset(), meshOf.reset();
+        slotsRows.reset(), slotsEll.reset(), matBase.reset(), meshOf.reset(), xkm1.reset(), xkm2.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)
             (void)hipStreamDestroy(stream);

@@ -164,6 +164,24 @@ void CollectTiming(pbat_vbd_integrator& h)
 // Enqueue the launches of one Step (gpu/impl/vbd/Integrator.cu:82-103) and
 // return the number of sweep launches. With eventBrackets, one event pair
 // brackets each substep's back-to-back sweep launches.
+// sim/vbd/Kernels.h:97-102 in FP32 as gpu/impl/vbd/ChebyshevIntegrator.cu:28-33
+// evaluates it, operator precedence included: for k >= 2, 4/4 - rho2 * omega.
+float ChebyshevOmega(int k, float rho2, float omega)
+{
+    return (k == 0) ? 1.f : (k == 1) ? 2.f / (2.f - rho2) : 4.f / 4.f - rho2 * omega;
+}
+
+// After iteration k of a substep: the Chebyshev update (no-op without acceleration).
+void LaunchAcceleration(pbat_vbd_integrator& h, int k, float& omega)
+{
+    if (h.acceleration != PBAT_VBD_ACCELERATION_CHEBYSHEV)
+        return;
+    omega = ChebyshevOmega(k, h.rho * h.rho, omega);
+    int const n = static_cast<int>(h.nV);
+    hipLaunchKernelGGL(device::ChebyshevUpdate, dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr,
+                       h.xkm1.ptr, h.xkm2.ptr, n, k, omega);
+}
+
 int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool eventBrackets)
 {
     float const sdt = dt / static_cast<float>(substeps);
@@ -182,7 +200,9 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         int launched = 0;
         if (eventBrackets)
             PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
+        float omega = 0.f;
         for (int k = 0; k < iterations; ++k)
+        {
             for (int c = 0; c < nColors; ++c)
             {
                 a.vbeg = h.colorBegin[c];
@@ -193,6 +213,8 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
                 LaunchSweep(h, a, nc);
                 ++launched;
             }
+            LaunchAcceleration(h, k, omega);
+        }
         if (eventBrackets)
         {
             PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
@@ -238,6 +260,7 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     key.dt = dt, key.iterations = iterations, key.substeps = substeps, key.strategy = h.strategy;
     key.lanes = EffectiveLanes(h), key.block = h.blockSize, key.kD = h.kD, key.detHZero = h.detHZero;
     key.perLane = EffectivePerLane(h), key.xcd = h.xcdRemap, key.nt = h.ntStream, key.ell = h.useEll;
+    key.acceleration = h.acceleration, key.rho = h.rho;
     if (!(h.graphExec && h.graphKey == key))
     {
         InvalidateGraph(h);
@@ -336,6 +359,10 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid number of partitions");
     if (d->strategy < 0 || d->strategy > 4)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid initialization strategy");
+    if (d->acceleration != PBAT_VBD_ACCELERATION_NONE && d->acceleration != PBAT_VBD_ACCELERATION_CHEBYSHEV)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "only the base and Chebyshev-accelerated VBD integrators are available");
+    if (d->acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV && !(d->rho > 0. && d->rho < 1.))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "Expected 0 < rho < 1");
     int64_t const nV = d->nV, nT = d->nT;
     if (d->GVGp[0] != 0)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "GVGp must start at 0");
@@ -363,6 +390,8 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     PBAT_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     h->nV = nV, h->nT = nT, h->nIncidences = nInc, h->nUpdated = nUpd;
     h->strategy = d->strategy;
+    h->acceleration = d->acceleration;
+    h->rho = static_cast<float>(d->rho);
     h->kD = static_cast<float>(d->kD);
     h->detHZero = static_cast<float>(d->detHZero);
 
@@ -557,6 +586,11 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     h->vtp.upload(P, h->stream);
     for (auto* b : {&h->xm, &h->xtil, &h->xt, &h->vel, &h->vprev, &h->aext})
         b->allocate(static_cast<size_t>(nV));
+    if (h->acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV)
+    {
+        h->xkm1.allocate(static_cast<size_t>(nV));
+        h->xkm2.allocate(static_cast<size_t>(nV));
+    }
     h->energy.allocate(3);
     PBAT_HIP(hipMemsetAsync(h->xtil.ptr, 0, sizeof(float4) * nV, h->stream));
     PBAT_HIP(hipMemsetAsync(h->vprev.ptr, 0, sizeof(float4) * nV, h->stream));
@@ -682,6 +716,7 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         hipLaunchKernelGGL(device::SubstepBegin, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xtil.ptr, h.xt.ptr,
                            h.vel.ptr, h.vprev.ptr, h.aext.ptr, n, sdt, sdt2, h.strategy);
         WriteVertexMatrix(h, h.xtil.ptr, Join(dir, name("xtilde", t, s, -1)));
+        float omega = 0.f;
         for (int k = 0; k < iterations; ++k)
         {
             WriteVertexMatrix(h, h.xm.ptr, Join(dir, name("x", t, s, k)));
@@ -692,6 +727,7 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
                 if (a.vend > a.vbeg)
                     LaunchSweep(h, a, a.vend - a.vbeg);
             }
+            LaunchAcceleration(h, k, omega);
         }
         WriteVertexMatrix(h, h.xm.ptr, Join(dir, name("x", t, s, iterations)));
         hipLaunchKernelGGL(device::SubstepEnd, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xt.ptr, h.vel.ptr,
@@ -829,9 +865,11 @@ void BuildUnion(pbat_vbd_data const* meshes, int count, UnionData& U)
         if (m.nV <= 0 || m.nT <= 0 || !m.GVGp || !m.Pptr || m.nPartitions < 0 ||
             m.nPartitions > PBAT_VBD_MAX_COLORS)
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid mesh " + std::to_string(b) + " in batch");
-        if (m.strategy != meshes[0].strategy || m.kD != meshes[0].kD || m.detHZero != meshes[0].detHZero)
+        if (m.strategy != meshes[0].strategy || m.kD != meshes[0].kD || m.detHZero != meshes[0].detHZero ||
+            m.acceleration != meshes[0].acceleration ||
+            (m.acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV && m.rho != meshes[0].rho))
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
-                 "meshes of a batch must share the initialization strategy, kD and detHZero");
+                 "meshes of a batch must share the initialization strategy, kD, detHZero and acceleration");
         nV += m.nV;
         nT += m.nT;
         nInc += m.GVGp[m.nV];
@@ -895,6 +933,7 @@ void BuildUnion(pbat_vbd_data const* meshes, int count, UnionData& U)
     // Padj may be empty (every vertex fixed); any non-null pointer serves then
     d.nPartitions = nColors, d.Pptr = U.Pptr.data(), d.Padj = U.Padj.empty() ? U.Pptr.data() : U.Padj.data();
     d.strategy = meshes[0].strategy, d.kD = meshes[0].kD, d.detHZero = meshes[0].detHZero;
+    d.acceleration = meshes[0].acceleration, d.rho = meshes[0].rho;
 }
 
 pbat_vbd_integrator& Handle(pbat_vbd_integrator* h)

@@ -392,6 +392,30 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     xt[i] = make_float4(x.x, x.y, x.z, 0.f);
 }
 
+// Chebyshev semi-iterative update after iteration k (sim/vbd/Kernels.h:104-118,
+// gpu/impl/vbd/ChebyshevIntegrator.cu:38-59): x = omega (x - x_{k-2}) + x_{k-2}
+// for k > 1, then x_{k-2} = x_{k-1}, x_{k-1} = x.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void ChebyshevUpdate(float4* __restrict__ xm,
+                                                                               float4* __restrict__ xkm1,
+                                                                               float4* __restrict__ xkm2, int n,
+                                                                               int k, float omega)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 x = xm[i];
+    if (k > 1)
+    {
+        float4 const a = xkm2[i];
+        x.x = omega * (x.x - a.x) + a.x;
+        x.y = omega * (x.y - a.y) + a.y;
+        x.z = omega * (x.z - a.z) + a.z;
+        xm[i] = x;
+    }
+    xkm2[i] = xkm1[i];
+    xkm1[i] = x;
+}
+
 // FP64 energy reduction: out[0] += 1/2 m |v|^2, out[2] += -m a.x over vertices,
 // out[1] += w Psi(F) over tets. One atomic per block and term. blockDim = 256.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm, float4 const* __restrict__ vel,

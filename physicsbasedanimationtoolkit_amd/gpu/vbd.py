@@ -36,10 +36,10 @@ def _marshal(data: Data):
     """pbat_vbd_data view of a constructed Data, plus the arrays it points into."""
     if not isinstance(data, Data):
         raise TypeError("expected a pbat.sim.vbd.Data")
-    if data.accelerator != AccelerationStrategy.Base:
+    if data.accelerator not in (AccelerationStrategy.Base, AccelerationStrategy.Chebyshev):
         raise ValueError(
             f"acceleration strategy {data.accelerator.name} is not available on this device path; "
-            "only the base VBD integrator is implemented")
+            "only the base and Chebyshev-accelerated VBD integrators are implemented")
     if data.m.size == 0:
         raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
     nV, nT = data.x.shape[1], data.E.shape[1]
@@ -74,6 +74,8 @@ def _marshal(data: Data):
     d.strategy = int(data.strategy)
     d.kD = float(data.kD)
     d.detHZero = float(data.detH_zero)
+    d.acceleration = 1 if data.accelerator == AccelerationStrategy.Chebyshev else 0
+    d.rho = float(data.rho)
     return d, keep
 
 

@@ -359,3 +359,72 @@ def test_rayleigh_damping_dissipates_energy():
         plain.step(s.dt, 20, 1)
         damped.step(s.dt, 20, 1)
     assert damped.energy().sum() < plain.energy().sum() < e0
+
+
+# ---- Chebyshev-accelerated VBD (sim/vbd/ChebyshevIntegrator.cpp) ------------
+
+def test_chebyshev_omega_sequence_keeps_reference_precedence():
+    """sim/vbd/Kernels.h:97-102: 1, 2/(2 - rho^2), then 4/4 - rho^2 omega (= 1 - rho^2 omega)."""
+    rho2 = 0.81
+    w0 = O.chebyshev_omega(0, rho2, 0.0)
+    w1 = O.chebyshev_omega(1, rho2, w0)
+    w2 = O.chebyshev_omega(2, rho2, w1)
+    w3 = O.chebyshev_omega(3, rho2, w2)
+    assert w0 == 1.0
+    assert w1 == pytest.approx(2.0 / 1.19, rel=1e-15)
+    assert w2 == pytest.approx(1.0 - 0.81 * (2.0 / 1.19), rel=1e-15)  # -0.36134...
+    assert w3 == pytest.approx(1.0 - 0.81 * w2, rel=1e-15)
+    assert w2 < 0.0  # not 4 / (4 - rho^2 omega), which would stay above 1
+
+
+def test_chebyshev_reference_cube_kat():
+    """sim/vbd/ChebyshevIntegrator.cpp:37-85: rho 0.9, dt 1e-2, 10 iterations, the
+    8-vertex cube falls along z only."""
+    o = O.OracleVbd(CUBE_P, CUBE_T, chebyshev_rho=0.9)
+    o.step(1e-2, 10, 1)
+    dx = o.positions - CUBE_P
+    assert np.all(dx[2] < 0)
+    assert np.all(np.abs(dx[:2]) < 1e-4)
+
+
+def test_chebyshev_update_composes_plain_sweeps():
+    """Three iterations: no update at k = 0, 1 and x = w2 (S^3 x0 - S x0) + S x0
+    at k = 2, where S^n x0 is the plain VBD result after n iterations."""
+    X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=2)
+    dbc = meshes.bottom_vertices(X)
+    a = meshes.gravity(X.shape[1])
+    v = np.random.default_rng(0).normal(size=X.shape) * 0.3
+
+    def run(iters, rho=None):
+        o = O.OracleVbd(X, E, v=v, aext=a, dbc=dbc, chebyshev_rho=rho)
+        o.step(1.0 / 60.0, iters, 1)
+        return o.positions.copy()
+
+    x1, x3 = run(1), run(3)
+    rho = 0.7
+    w0 = O.chebyshev_omega(0, rho * rho, 0.0)
+    w1 = O.chebyshev_omega(1, rho * rho, w0)
+    w2 = O.chebyshev_omega(2, rho * rho, w1)
+    np.testing.assert_allclose(run(2, rho), run(2), rtol=0, atol=0)
+    np.testing.assert_allclose(run(3, rho), w2 * (x3 - x1) + x1, rtol=0, atol=1e-14)
+
+
+def test_chebyshev_with_vanishing_rho_is_plain_vbd():
+    X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=5)
+    kw = dict(aext=meshes.gravity(X.shape[1]), dbc=meshes.bottom_vertices(X))
+    a = O.OracleVbd(X, E, chebyshev_rho=1e-12, **kw)
+    b = O.OracleVbd(X, E, **kw)
+    for _ in range(3):
+        a.step(1.0 / 60.0, 8, 1)
+        b.step(1.0 / 60.0, 8, 1)
+    np.testing.assert_allclose(a.positions, b.positions, rtol=0, atol=1e-13)
+
+
+def test_chebyshev_multithreaded_matches_single_thread():
+    X, E = meshes.cube_tet_grid(5, jitter=0.1, seed=1)
+    kw = dict(aext=meshes.gravity(X.shape[1]), dbc=meshes.bottom_vertices(X), chebyshev_rho=0.8)
+    a = O.OracleVbd(X, E, nthreads=1, **kw)
+    b = O.OracleVbd(X, E, nthreads=4, **kw)
+    a.step(1.0 / 60.0, 12, 2)
+    b.step(1.0 / 60.0, 12, 2)
+    np.testing.assert_array_equal(a.positions, b.positions)
