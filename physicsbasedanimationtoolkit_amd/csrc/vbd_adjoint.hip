@@ -106,6 +106,7 @@ void EnsureBuffers(pbat_vbd_integrator& h)
     for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
+
     if (R.dinv.n != 2 * n)
         R.dinv.allocate(2 * n);
     size_t const nM = static_cast<size_t>(h.nMeshes);
@@ -140,11 +141,11 @@ void VertexPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, f
     LaunchAdjoint<M>(h, a);
 }
 
-// One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b;
-// with acc, the accumulator r.z of each mesh receives b . y of the final
-// values (a reverse pass after a forward pass updates each vertex last).
+// One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b.
+// With acc (a reverse pass after a forward pass, which updates each vertex
+// last), the accumulator r.z of each mesh receives b . y of the final values.
 void GaussSeidelPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b, bool reverse,
-                     double* acc)
+                     double* acc = nullptr)
 {
     int const nC = static_cast<int>(h.colorBegin.size()) - 1;
     for (int k = 0; k < nC; ++k)
@@ -197,15 +198,25 @@ SolveResult ReadResult(pbat_vbd_integrator& h)
 // Solve H y = b (y holds the initial guess) for every mesh of the handle, each
 // with its own CG scalars and stopping test |b - H y| <= tol |b|; the host
 // polls the number of live meshes every cfg.check_every iterations.
+//
+// One CG iteration is four launches with the block-Jacobi preconditioner:
+//   (1) q = H p, p.q;
+//   (2) y += alpha p, r -= alpha q (alpha = rz / pq), z = D^-1 r, r.r, r.z;
+//   (3) one workgroup: stopping test, beta, rotation, cleared accumulators;
+//   (4) p = z + beta p (finished meshes keep p);
+// the symmetric Gauss-Seidel preconditioner replaces the z of (2) by one
+// forward and one reverse coloured pass.
 SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b,
                   pbat_vbd_adjoint_config const& cfg)
 {
     RolloutState& R = h.rollout;
     int const n = static_cast<int>(h.nUpdated), nV = static_cast<int>(h.nV), nM = h.nMeshes;
-    double* S = R.scal.ptr;                  // plain per-mesh values
-    double* A = R.scal.ptr + PlainDoubles(h); // accumulators
-    int* F = R.flags.ptr;
-    double const tol2 = static_cast<double>(cfg.tolerance) * cfg.tolerance;
+    device::CgState cg{};
+    cg.P = R.scal.ptr;
+    cg.A = R.scal.ptr + PlainDoubles(h);
+    cg.F = R.flags.ptr;
+    cg.nM = nM;
+    cg.tol2 = static_cast<double>(cfg.tolerance) * cfg.tolerance;
     int const check = std::max(1, cfg.check_every);
     int const gridN = Blocks(n);
     bool const jacobi = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI;
@@ -213,16 +224,13 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     auto zeroScalars = [&] {
         int const nd = static_cast<int>(ScalarDoubles(h));
         hipLaunchKernelGGL(device::ZeroScalars, dim3(std::min(1024, Blocks(nd + 2 * nM + 1))), dim3(256), 0,
-                           h.stream, S, nd, F, 2 * nM + 1);
+                           h.stream, cg.P, nd, cg.F, 2 * nM + 1);
     };
-    auto scalars = [&](int first) {
-        hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, nM, tol2, first, S, A, F);
-    };
-    // residual r = b - H y with per-mesh |b|^2, |r|^2 (and r . D^-1 r with dinv)
+    // r = b - H y with per-mesh |b|^2, |r|^2 (and r . D^-1 r with dinv)
     auto residual = [&](float4 const* pre) {
         VertexPass<AdjMode::kHessVec>(h, xs, dt, y, nullptr, R.q.ptr, nullptr, 0, 0, n);
         hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr,
-                           R.z.ptr, pre, h.meshOf.ptr, n, nM, A);
+                           R.z.ptr, pre, h.meshOf.ptr, n, nM, cg.A);
     };
     zeroScalars();
     Zero4(h, R.z.ptr, nV);
@@ -233,11 +241,11 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         {
             int const chunk = std::min(check, cfg.max_iterations - it);
             for (int k = 0; k < chunk; ++k)
-                GaussSeidelPass(h, xs, dt, y, b, false, nullptr);
+                GaussSeidelPass(h, xs, dt, y, b, false);
             it += chunk;
             zeroScalars();
             residual(nullptr);
-            scalars(1);
+            hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
             if (ReadLive(h) == 0)
                 break;
         }
@@ -251,27 +259,27 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     residual(dinv);
     if (!jacobi)
     {
-        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false, nullptr);
-        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, A);
+        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
+        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, cg.A);
     }
-    scalars(1);
+    hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
     hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr, nV);
     if (ReadLive(h) > 0)
     {
         for (int it = 1; it <= cfg.max_iterations; ++it)
         {
-            VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, A, device::kAccPq, 0, n);
-            hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr, R.p.ptr,
-                               R.q.ptr, dinv, h.meshOf.ptr, n, nM, static_cast<double const*>(S), A,
-                               static_cast<int const*>(F));
+            VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, cg.A, device::kAccPq, 0, n);
+            hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
+                               static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr), dinv,
+                               h.meshOf.ptr, n, cg);
             if (!jacobi)
             {
-                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false, nullptr);
-                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, A);
+                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
+                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, cg.A);
             }
-            scalars(0);
-            hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr,
-                               h.meshOf.ptr, n, nM, static_cast<double const*>(S), static_cast<int const*>(F));
+            hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 0);
+            hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr,
+                               static_cast<float4 const*>(R.z.ptr), h.meshOf.ptr, n, cg);
             if ((it % check == 0 || it == cfg.max_iterations) && ReadLive(h) == 0)
                 break;
         }
@@ -440,6 +448,13 @@ void SetLogYoung(pbat_vbd_integrator& h, float const* logY, int64_t nT)
                        h.ellStride > 0 ? h.ell.g2.ptr : nullptr, static_cast<int>(nT));
     PBAT_HIP(hipGetLastError());
     PBAT_HIP(hipStreamSynchronize(h.stream));
+    for (int64_t e = 0; e < nT; ++e)
+    {
+        // (mu, lambda) of the traces follow the new moduli
+        float const scale = std::exp(v[e] - h.logYCurrent[e]);
+        h.lameHost[2 * e] *= scale;
+        h.lameHost[2 * e + 1] *= scale;
+    }
     h.logYCurrent = v;
     R.valid = false;
 }

@@ -98,7 +98,8 @@ enum PlainArray : int
 };
 // Accumulated dot products: each (array, mesh) is spread over kSpread atomic
 // targets kSlotStride doubles (128 B) apart, so that thousands of workgroups
-// adding into one mesh's sum do not serialise on one address.
+// adding into one mesh's sum do not serialise on one address (same-address
+// FP64 atomics from many workgroups cost ~20 ns each, serialised).
 enum AccArray : int
 {
     kAccRzNew = 0, // r . z being accumulated
@@ -107,7 +108,7 @@ enum AccArray : int
     kAccBb = 3,    // b . b
     kAccCount = 4
 };
-constexpr int kSpread = 8;
+constexpr int kSpread = 16;
 constexpr int kSlotStride = 16;
 
 __host__ __device__ constexpr size_t AccDoubles(int nM)
@@ -120,6 +121,7 @@ __device__ __forceinline__ double* AccSlot(double* A, int arr, int nM, int m, in
     return A + ((static_cast<size_t>(arr) * nM + m) * kSpread + slot) * kSlotStride;
 }
 
+// Sum of a mesh's targets (read by a later launch than the one that added).
 __device__ __forceinline__ double AccSum(double const* A, int arr, int nM, int m)
 {
     double s = 0.;
@@ -135,6 +137,77 @@ enum FlagArray : int
     kDone = 0,
     kIters = 1
 };
+
+// Device state of the per-mesh conjugate-gradient solver.
+struct CgState
+{
+    double* P; // plain values (kPlainCount x nM)
+    double* A; // accumulators (AccDoubles(nM))
+    int* F;    // flags (2 nM + 1)
+    int nM;
+    double tol2; // squared relative tolerance
+};
+
+// Per mesh, after an iteration's accumulations (one workgroup, blockDim a
+// multiple of 64): each mesh's kSpread targets are summed by one half-wave in
+// a fixed order; then the stopping test |r| <= tol |b|, beta = rz_new / rz,
+// rz rotated, every accumulator of the mesh cleared, live meshes counted.
+// first: the initialisation (takes b.b, counts no iteration).
+__device__ __forceinline__ void FinishIteration(CgState const& c, int first)
+{
+    static_assert(kSpread <= 32, "one half-wave per mesh");
+    __shared__ int live;
+    if (threadIdx.x == 0)
+        live = 0;
+    __syncthreads();
+    int const nM = c.nM;
+    int const half = static_cast<int>(threadIdx.x >> 5), hl = static_cast<int>(threadIdx.x & 31);
+    int const nHalves = static_cast<int>(blockDim.x >> 5);
+    int count = 0;
+    for (int m = half; m < nM; m += nHalves)
+    {
+        double rr = 0., rz = 0., bb = 0.;
+        if (hl < kSpread)
+        {
+            rr = *AccSlot(c.A, kAccRr, nM, m, hl);
+            rz = *AccSlot(c.A, kAccRzNew, nM, m, hl);
+            bb = *AccSlot(c.A, kAccBb, nM, m, hl);
+            for (int arr = 0; arr < kAccCount; ++arr)
+                *AccSlot(c.A, arr, nM, m, hl) = 0.;
+        }
+        for (int off = 16; off > 0; off >>= 1)
+        {
+            rr += __shfl_xor(rr, off);
+            rz += __shfl_xor(rz, off);
+            bb += __shfl_xor(bb, off);
+        }
+        if (hl == 0)
+        {
+            if (first)
+                c.P[kBb * nM + m] = bb;
+            int done = c.F[kDone * nM + m];
+            double const b2 = c.P[kBb * nM + m];
+            if (!done)
+            {
+                if (!first)
+                    c.F[kIters * nM + m] += 1;
+                c.P[kRrLast * nM + m] = rr;
+                if (!(b2 > 0.) || rr <= c.tol2 * b2)
+                    done = 1;
+            }
+            double const rzOld = c.P[kRz * nM + m];
+            c.P[kBeta * nM + m] = (!done && rzOld != 0.) ? rz / rzOld : 0.;
+            c.P[kRz * nM + m] = rz;
+            c.F[kDone * nM + m] = done;
+            count += done ? 0 : 1;
+        }
+    }
+    if (count)
+        atomicAdd(&live, count);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        c.F[2 * nM] = live;
+}
 
 // Sum val into accumulated array arr of mesh m over the calling workgroup;
 // every thread of the workgroup must call it (threads without a contribution
@@ -197,7 +270,7 @@ __device__ __forceinline__ void BlockAccumulate(double val, int m, double* __res
 enum class AdjMode : int
 {
     kGaussSeidel = 0,    // one colour of block Gauss-Seidel on H y = b
-    kGaussSeidelDot = 1, // same, then acc[mesh] += b . y (final pass of symmetric GS)
+    kGaussSeidelDot = 1, // same, then r.z += b . y (final pass of symmetric GS)
     kHessVec = 2,        // out = H y
     kHessVecDot = 3,     // out = H y, acc[mesh] += y . out
     kDiagInverse = 4     // dinv = inverse of the 3x3 diagonal block of H
@@ -249,6 +322,7 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
     constexpr bool kGS = kMode == AdjMode::kGaussSeidel || kMode == AdjMode::kGaussSeidelDot;
     constexpr bool kDiag = kGS || kMode == AdjMode::kDiagInverse;
     constexpr bool kDot = kMode == AdjMode::kGaussSeidelDot || kMode == AdjMode::kHessVecDot;
+    constexpr bool kProduct = kMode == AdjMode::kHessVec || kMode == AdjMode::kHessVecDot;
     int block = static_cast<int>(blockIdx.x);
     if (a.xcdChunk > 0)
         block = (block & 7) * a.xcdChunk + (block >> 3);
@@ -258,6 +332,7 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
     bool const active = v < a.vend;
     int kb = 0, ke = 0;
     float4 own = make_float4(0.f, 0.f, 0.f, 0.f), yo = own, bo = own;
+    auto vec = [&](int k) { return a.y[k]; };
     if (active)
     {
         if (a.ellStride > 0)
@@ -271,7 +346,7 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
             ke = a.vtp[v + 1];
         }
         own = a.xs[v];
-        yo = a.y[v];
+        yo = vec(v);
         if (kGS && lane == 0)
             bo = a.b[v];
     }
@@ -298,15 +373,15 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
     {
         if (I[j].w >= 0)
             AccumulateHessVec<kDiag>(q0[j], q1[j], q2[j], own, a.xs[I[j].x], a.xs[I[j].y], a.xs[I[j].z], yo,
-                                     a.y[I[j].x], a.y[I[j].y], a.y[I[j].z], hv, D);
+                                     vec(I[j].x), vec(I[j].y), vec(I[j].z), hv, D);
     }
     for (int k = kb + lane + K * L; k < ke; k += L)
     {
         int4 const J = a.ids[k];
         if (J.w < 0)
             continue;
-        AccumulateHessVec<kDiag>(a.g0[k], a.g1[k], a.g2[k], own, a.xs[J.x], a.xs[J.y], a.xs[J.z], yo, a.y[J.x],
-                                 a.y[J.y], a.y[J.z], hv, D);
+        AccumulateHessVec<kDiag>(a.g0[k], a.g1[k], a.g2[k], own, a.xs[J.x], a.xs[J.y], a.xs[J.z], yo, vec(J.x),
+                                 vec(J.y), vec(J.z), hv, D);
     }
     hv.x = GroupSum<L>(hv.x);
     hv.y = GroupSum<L>(hv.y);
@@ -319,7 +394,7 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
     double dot = 0.;
     if (leader)
     {
-        if constexpr (kMode == AdjMode::kHessVec || kMode == AdjMode::kHessVecDot)
+        if constexpr (kProduct)
         {
             a.out[v] = make_float4(Hx, Hy, Hz, 0.f);
             dot = double(yo.x) * Hx + double(yo.y) * Hy + double(yo.z) * Hz;
@@ -482,34 +557,30 @@ __global__ __launch_bounds__(256) void CgInitResidual(float4 const* __restrict__
         BlockAccumulate(rz, m, A, kAccRzNew, nM);
 }
 
-// y += alpha p, r -= alpha q (alpha = rz / pq of the vertex's mesh, 0 when done);
-// with dinv: z = D^-1 r and r.z accumulated; without: z = 0 (a Gauss-Seidel
-// preconditioner pass follows). r.r accumulated.
+// y += alpha p, r -= alpha q with alpha = rz / pq of the vertex's mesh (0 when
+// it is done); r.r accumulated; with dinv: z = D^-1 r and r.z accumulated;
+// without: z = 0 (a Gauss-Seidel preconditioner pass follows).
 __global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* __restrict__ r,
                                                 float4* __restrict__ z, float4 const* __restrict__ p,
                                                 float4 const* __restrict__ q, float4 const* __restrict__ dinv,
-                                                int const* __restrict__ meshOf, int n, int nM,
-                                                double const* __restrict__ P, double* __restrict__ A,
-                                                int const* __restrict__ F)
+                                                int const* __restrict__ meshOf, int n, CgState c)
 {
+    int const nM = c.nM;
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     double rr = 0., rz = 0.;
     int m = -1;
     if (i < n)
     {
         m = meshOf[i];
-        float4 ri = r[i];
-        if (!F[kDone * nM + m])
-        {
-            double const den = AccSum(A, kAccPq, nM, m);
-            float const alpha = (den != 0.) ? static_cast<float>(P[kRz * nM + m] / den) : 0.f;
-            float4 const pi = p[i], qi = q[i];
-            float4 yi = y[i];
-            yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
-            ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
-            y[i] = yi;
-            r[i] = ri;
-        }
+        double const pq = AccSum(c.A, kAccPq, nM, m);
+        bool const done = c.F[kDone * nM + m] != 0;
+        float const alpha = (!done && pq != 0.) ? static_cast<float>(c.P[kRz * nM + m] / pq) : 0.f;
+        float4 const pi = p[i], qi = q[i];
+        float4 yi = y[i], ri = r[i];
+        yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
+        ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
+        y[i] = yi;
+        r[i] = ri;
         rr = Dot3(ri, ri);
         if (dinv)
         {
@@ -522,64 +593,28 @@ __global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* 
             z[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
-    BlockAccumulate(rr, m, A, kAccRr, nM);
+    BlockAccumulate(rr, m, c.A, kAccRr, nM);
     if (dinv)
-        BlockAccumulate(rz, m, A, kAccRzNew, nM);
+        BlockAccumulate(rz, m, c.A, kAccRzNew, nM);
 }
 
-// Per mesh, after an iteration's accumulations: stopping test |r| <= tol |b|,
-// beta = rz_new / rz, rotate rz, clear the accumulators, count live meshes.
-// first: the initialisation (takes b.b, counts no iteration).
-__global__ __launch_bounds__(256) void CgScalars(int nM, double tol2, int first, double* __restrict__ P,
-                                                 double* __restrict__ A, int* __restrict__ F)
+// End of a CG iteration (or of the initialisation when first): one workgroup.
+__global__ __launch_bounds__(256) void CgScalars(CgState c, int first)
 {
-    __shared__ int live;
-    if (threadIdx.x == 0)
-        live = 0;
-    __syncthreads();
-    int count = 0;
-    for (int m = static_cast<int>(threadIdx.x); m < nM; m += static_cast<int>(blockDim.x))
-    {
-        if (first)
-            P[kBb * nM + m] = AccSum(A, kAccBb, nM, m);
-        int done = F[kDone * nM + m];
-        double const rr = AccSum(A, kAccRr, nM, m), bb = P[kBb * nM + m];
-        if (!done)
-        {
-            if (!first)
-                F[kIters * nM + m] += 1;
-            P[kRrLast * nM + m] = rr;
-            if (!(bb > 0.) || rr <= tol2 * bb)
-                done = 1;
-        }
-        double const rzNew = AccSum(A, kAccRzNew, nM, m), rz = P[kRz * nM + m];
-        P[kBeta * nM + m] = (!done && rz != 0.) ? rzNew / rz : 0.;
-        P[kRz * nM + m] = rzNew;
-        F[kDone * nM + m] = done;
-        for (int arr = 0; arr < kAccCount; ++arr)
-            for (int k = 0; k < kSpread; ++k)
-                *AccSlot(A, arr, nM, m, k) = 0.;
-        count += done ? 0 : 1;
-    }
-    if (count)
-        atomicAdd(&live, count);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        F[2 * nM] = live;
+    FinishIteration(c, first);
 }
 
-// p = z + beta p (beta of the vertex's mesh; frozen meshes keep p)
+// p = z + beta p (beta of the vertex's mesh; finished meshes keep p)
 __global__ __launch_bounds__(256) void CgDirection(float4* __restrict__ p, float4 const* __restrict__ z,
-                                                   int const* __restrict__ meshOf, int n, int nM,
-                                                   double const* __restrict__ P, int const* __restrict__ F)
+                                                   int const* __restrict__ meshOf, int n, CgState c)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n)
         return;
     int const m = meshOf[i];
-    if (F[kDone * nM + m])
+    if (c.F[kDone * c.nM + m])
         return;
-    float const beta = static_cast<float>(P[kBeta * nM + m]);
+    float const beta = static_cast<float>(c.P[kBeta * c.nM + m]);
     float4 const zi = z[i];
     float4 const pi = p[i];
     p[i] = make_float4(zi.x + beta * pi.x, zi.y + beta * pi.y, zi.z + beta * pi.z, 0.f);

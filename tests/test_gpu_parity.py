@@ -258,3 +258,27 @@ def test_unavailable_accelerators_raise(gpu_device):
     data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_anderson_acceleration(5).construct())
     with pytest.raises(ValueError):
         pbat.gpu.vbd.Integrator(data)
+
+
+def test_traced_step_writes_reference_files(gpu_device, tmp_path):
+    """gpu/impl/vbd/Integrator.cu:105-148: MatrixMarket dumps of one traced step;
+    the last iterate equals the untraced step's result."""
+    import scipy.io
+
+    s = meshes.plain_gpu_scene(4, seed=2)
+    data, g1, _ = build_pair(s)
+    _, g2, _ = build_pair(s)
+    g1.traced_step(s.dt, 3, 2, t=0, dir=str(tmp_path))
+    g2.graph_capture = False
+    g2.step(s.dt, 3, 2)
+    read = lambda name: np.asarray(scipy.io.mmread(str(tmp_path / name)))
+    for sub in range(2):
+        assert (tmp_path / f"xtilde.t.0.s.{sub}.mtx").exists()
+        for k in range(4):
+            assert read(f"x.t.0.s.{sub}.k.{k}.mtx").shape == (s.n_vertices, 3)
+    np.testing.assert_array_equal(read("x.t.0.s.1.k.3.mtx").astype(np.float32), g2.x.T)
+    np.testing.assert_array_equal(g1.x, g2.x)
+    np.testing.assert_array_equal(read("T.mtx").astype(np.int64), data.E.T)
+    np.testing.assert_allclose(read("M.mtx").reshape(-1), data.m, rtol=1e-6)
+    np.testing.assert_allclose(read("lame.mtx"), data.lame.T, rtol=1e-6)
+    np.testing.assert_allclose(read("wg.mtx").reshape(-1), data.wg, rtol=1e-6)
