@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--adjoint-fd", action="store_true", help="central differences on 64 log Y entries")
     ap.add_argument("--batch-adjoint-meshes", type=int, default=64)
     ap.add_argument("--batch-adjoint-steps", type=int, default=200)
+    ap.add_argument("--only", choices=["batched", "adjoint", "batched_adjoint"], default=None,
+                    help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MB (counter calibration) and exit")
     args = ap.parse_args()
@@ -305,6 +307,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}), flush=True)
+        return
+    if args.only:
+        run = {"batched": bench_batched, "adjoint": bench_adjoint, "batched_adjoint": bench_batched_adjoint}
+        print(json.dumps({args.only: run[args.only](args, torch)}), flush=True)
         return
     s = meshes.plain_gpu_scene(args.grid, seed=0)
     dt, iters = s.dt, args.iterations

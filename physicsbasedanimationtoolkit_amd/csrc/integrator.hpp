@@ -94,20 +94,22 @@ struct DeviceArray
 struct IncidenceRecords
 {
     DeviceArray<int4> ids;
-    DeviceArray<float4> g0, g1, g2;
+    DeviceArray<float4> g0, g1;
+    DeviceArray<float2> g2;
     void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(); }
 };
 
 struct HostIncidenceRecords
 {
     std::vector<int4> ids;
-    std::vector<float4> g0, g1, g2;
+    std::vector<float4> g0, g1;
+    std::vector<float2> g2;
     void resize(size_t n)
     {
-        ids.assign(n, make_int4(0, 0, 0, -1));
+        ids.assign(n, make_int4(-1, 0, 0, 0)); // a = -1: unused fixed-stride slot
         g0.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
         g1 = g0;
-        g2 = g0;
+        g2.assign(n, make_float2(0.f, 0.f));
     }
     void upload(IncidenceRecords& d, hipStream_t s) const
     {

@@ -17,7 +17,7 @@ namespace pbat_vbd::device {
 // df0 x f1 + f0 x df1. Adds the block-diagonal part w (mu |g|^2 I + lambda c c^T)
 // into D when kDiag.
 template <bool kDiag>
-__device__ __forceinline__ void AccumulateHessVec(float4 const& q0, float4 const& q1, float4 const& q2,
+__device__ __forceinline__ void AccumulateHessVec(float4 const& q0, float4 const& q1, float2 const& q2, float wmu,
                                                   float4 const& own, float4 const& pa, float4 const& pb,
                                                   float4 const& pc, float4 const& yo, float4 const& ya,
                                                   float4 const& yb, float4 const& yc, float3& hv, Block& D)
@@ -25,7 +25,7 @@ __device__ __forceinline__ void AccumulateHessVec(float4 const& q0, float4 const
     float const gax = q0.x, gay = q0.y, gaz = q0.z;
     float const gbx = q0.w, gby = q1.x, gbz = q1.y;
     float const gcx = q1.z, gcy = q1.w, gcz = q2.x;
-    float const wmu = q2.y, wlam = q2.z, alpha = q2.w;
+    float const wlam = q2.y, alpha = 1.f + wmu / wlam;
     // F
     float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
     float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
@@ -282,7 +282,7 @@ struct AdjointArgs
     int4 const* __restrict__ ids;
     float4 const* __restrict__ g0;
     float4 const* __restrict__ g1;
-    float4 const* __restrict__ g2;
+    float2 const* __restrict__ g2;
     float4 const* __restrict__ xs;  // minimiser positions (with lumped mass in .w)
     float4* __restrict__ y;         // unknowns (Gauss-Seidel: updated in place) / vector multiplied
     float4 const* __restrict__ b;   // right-hand side (Gauss-Seidel)
@@ -353,7 +353,8 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
     float3 hv = make_float3(0.f, 0.f, 0.f);
     Block D{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     int4 I[K];
-    float4 q0[K], q1[K], q2[K];
+    float4 q0[K], q1[K];
+    float2 q2[K];
 #pragma unroll
     for (int j = 0; j < K; ++j)
     {
@@ -366,22 +367,22 @@ __global__ __launch_bounds__(256) void AdjointVertexKernel(AdjointArgs a)
             q2[j] = a.g2[k];
         }
         else
-            I[j] = make_int4(0, 0, 0, -1);
+            I[j] = make_int4(-1, 0, 0, 0);
     }
 #pragma unroll
     for (int j = 0; j < K; ++j)
     {
-        if (I[j].w >= 0)
-            AccumulateHessVec<kDiag>(q0[j], q1[j], q2[j], own, a.xs[I[j].x], a.xs[I[j].y], a.xs[I[j].z], yo,
-                                     vec(I[j].x), vec(I[j].y), vec(I[j].z), hv, D);
+        if (I[j].x >= 0)
+            AccumulateHessVec<kDiag>(q0[j], q1[j], q2[j], __int_as_float(I[j].w), own, a.xs[I[j].x], a.xs[I[j].y],
+                                     a.xs[I[j].z], yo, vec(I[j].x), vec(I[j].y), vec(I[j].z), hv, D);
     }
     for (int k = kb + lane + K * L; k < ke; k += L)
     {
         int4 const J = a.ids[k];
-        if (J.w < 0)
+        if (J.x < 0)
             continue;
-        AccumulateHessVec<kDiag>(a.g0[k], a.g1[k], a.g2[k], own, a.xs[J.x], a.xs[J.y], a.xs[J.z], yo, vec(J.x),
-                                 vec(J.y), vec(J.z), hv, D);
+        AccumulateHessVec<kDiag>(a.g0[k], a.g1[k], a.g2[k], __int_as_float(J.w), own, a.xs[J.x], a.xs[J.y],
+                                 a.xs[J.z], yo, vec(J.x), vec(J.y), vec(J.z), hv, D);
     }
     hv.x = GroupSum<L>(hv.x);
     hv.y = GroupSum<L>(hv.y);
@@ -648,11 +649,13 @@ __global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)
 }
 
 // Rescale per-tet Lame coefficients to Young's moduli exp(logY) (nu fixed):
-// w*mu and w*lambda of the tet constants and of its incidence records.
+// w*mu and w*lambda of the tet constants and of its incidence records
+// (w*mu in the records' id words, w*lambda in their last float pair).
 __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ logY, float2 const* __restrict__ base,
                                                    float4* __restrict__ tgp, int4 const* __restrict__ slots,
-                                                   float4* __restrict__ g2, int4 const* __restrict__ slotsEll,
-                                                   float4* __restrict__ g2Ell, int nT)
+                                                   int4* __restrict__ ids, float2* __restrict__ g2,
+                                                   int4 const* __restrict__ slotsEll, int4* __restrict__ idsEll,
+                                                   float2* __restrict__ g2Ell, int nT)
 {
     int const e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= nT)
@@ -663,25 +666,20 @@ __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ log
     float4 t = tgp[3 * e + 2];
     t.y = wmu, t.z = wlam;
     tgp[3 * e + 2] = t;
-    int const s[4] = {slots[e].x, slots[e].y, slots[e].z, slots[e].w};
-    for (int k = 0; k < 4; ++k)
-        if (s[k] >= 0)
-        {
-            float4 r = g2[s[k]];
-            r.y = wmu, r.z = wlam;
-            g2[s[k]] = r;
-        }
-    if (g2Ell)
-    {
-        int const t4[4] = {slotsEll[e].x, slotsEll[e].y, slotsEll[e].z, slotsEll[e].w};
+    auto update = [&](int4 const& sl, int4* I, float2* G) {
+        int const s4[4] = {sl.x, sl.y, sl.z, sl.w};
         for (int k = 0; k < 4; ++k)
-            if (t4[k] >= 0)
+            if (s4[k] >= 0)
             {
-                float4 r = g2Ell[t4[k]];
-                r.y = wmu, r.z = wlam;
-                g2Ell[t4[k]] = r;
+                int4 r = I[s4[k]];
+                r.w = __float_as_int(wmu);
+                I[s4[k]] = r;
+                G[s4[k]].y = wlam;
             }
-    }
+    };
+    update(slots[e], ids, g2);
+    if (g2Ell)
+        update(slotsEll[e], idsEll, g2Ell);
 }
 
 } // namespace pbat_vbd::device

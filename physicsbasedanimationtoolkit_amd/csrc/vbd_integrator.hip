@@ -525,10 +525,12 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
             ++n;
         }
         float4 const m = G4[3 * e + 2]; // (., w*mu, w*lambda, alpha)
-        R.ids[slot] = make_int4(ids[0], ids[1], ids[2], static_cast<int>(e));
+        int wmuBits;
+        std::memcpy(&wmuBits, &m.y, sizeof(int));
+        R.ids[slot] = make_int4(ids[0], ids[1], ids[2], wmuBits);
         R.g0[slot] = make_float4(g[0][0], g[0][1], g[0][2], g[1][0]);
         R.g1[slot] = make_float4(g[1][1], g[1][2], g[2][0], g[2][1]);
-        R.g2[slot] = make_float4(g[2][2], m.y, m.z, m.w);
+        R.g2[slot] = make_float2(g[2][2], m.z);
     };
     {
         HostIncidenceRecords R;

@@ -7,9 +7,9 @@
 //   xt[v]    float4 (x_t, 0)          positions at the start of the substep
 //   vel[v], vprev[v], aext[v]  float4 velocities and external acceleration
 //   vtp[v]   int32 CSR prefix over vertex-tet incidences
-//   incidence records (64 B, structure of arrays, in sweep order): the other
+//   incidence records (56 B, structure of arrays, in sweep order): the other
 //            three vertex ids of the tet with their GP rows and the tet's
-//            w*mu, w*lambda, alpha = 1 + mu/lambda (see SweepArgs)
+//            w*mu, w*lambda (see SweepArgs)
 //   tgp[3e..3e+2]  float4 x3 (48 B) per tet: rows 1..3 of the shape function
 //            gradients GP (row 0 = -(row1+row2+row3)), w*mu, w*lambda, alpha
 //   tetv[e]  int4 vertex ids of tet e
@@ -83,14 +83,15 @@ __device__ __forceinline__ void GroupSum(Block& B)
 //   Hess_i += w mu |g|^2 I + w lambda c c^T
 // The d2J/dF2 term of the SNH Hessian contracts to zero for one vertex:
 // det(F0 + x_i g^T) is affine in x_i.
-__device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 const& q1, float4 const& q2,
+// alpha = 1 + mu/lambda = 1 + (w mu)/(w lambda).
+__device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 const& q1, float2 const& q2, float wmu,
                                                    float4 const& own, float4 const& pa, float4 const& pb,
                                                    float4 const& pc, Block& B)
 {
     float const gax = q0.x, gay = q0.y, gaz = q0.z;
     float const gbx = q0.w, gby = q1.x, gbz = q1.y;
     float const gcx = q1.z, gcy = q1.w, gcz = q2.x;
-    float const wmu = q2.y, wlam = q2.z, alpha = q2.w;
+    float const wlam = q2.y, alpha = 1.f + wmu / wlam;
     float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
     float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
     float const dcx = pc.x - own.x, dcy = pc.y - own.y, dcz = pc.z - own.z;
@@ -185,11 +186,11 @@ __device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 c
 }
 
 // Incidence records, structure of arrays, row k of vertex v's incidences
-// (CSR rows through vtp, or fixed-stride rows of ellStride entries):
-//   ids[k] int4   (a, b, c, tet): the tet's other three vertices in local
-//                 order; tet = -1 marks an unused fixed-stride slot
-//   g0[k], g1[k], g2[k] float4: (ga, gb.x), (gb.yz, gc.xy), (gc.z, w*mu,
-//                 w*lambda, alpha) with ga, gb, gc the GP rows of a, b, c
+// (CSR rows through vtp, or fixed-stride rows of ellStride entries), 56 B:
+//   ids[k] int4   (a, b, c, w*mu as bits): the tet's other three vertices in
+//                 local order; a = -1 marks an unused fixed-stride slot
+//   g0[k], g1[k] float4, g2[k] float2: (ga, gb.x), (gb.yz, gc.xy),
+//                 (gc.z, w*lambda) with ga, gb, gc the GP rows of a, b, c
 // Records are stored in sweep order, so a colour pass streams them.
 struct SweepArgs
 {
@@ -197,7 +198,7 @@ struct SweepArgs
     int4 const* __restrict__ ids;
     float4 const* __restrict__ g0;
     float4 const* __restrict__ g1;
-    float4 const* __restrict__ g2;
+    float2 const* __restrict__ g2;
     float4* __restrict__ xm;
     float4 const* __restrict__ xtil;
     float4 const* __restrict__ xt;
@@ -211,6 +212,7 @@ struct SweepArgs
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
 
 // Loads of the incidence stream, optionally with the non-temporal hint.
 __device__ __forceinline__ int4 LoadIds(int4 const* p, bool nt)
@@ -229,6 +231,16 @@ __device__ __forceinline__ float4 LoadConst(float4 const* p, bool nt)
     {
         v4f const r = __builtin_nontemporal_load(reinterpret_cast<v4f const*>(p));
         return make_float4(r.x, r.y, r.z, r.w);
+    }
+    return *p;
+}
+
+__device__ __forceinline__ float2 LoadConst(float2 const* p, bool nt)
+{
+    if (nt)
+    {
+        v2f const r = __builtin_nontemporal_load(reinterpret_cast<v2f const*>(p));
+        return make_float2(r.x, r.y);
     }
     return *p;
 }
@@ -284,7 +296,8 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     Block B{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool const nt = a.ntStream != 0;
     int4 I[K];
-    float4 q0[K], q1[K], q2[K];
+    float4 q0[K], q1[K];
+    float2 q2[K];
 #pragma unroll
     for (int j = 0; j < K; ++j)
     {
@@ -297,13 +310,13 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
             q2[j] = LoadConst(a.g2 + k, nt);
         }
         else
-            I[j] = make_int4(0, 0, 0, -1);
+            I[j] = make_int4(-1, 0, 0, 0);
     }
     float4 pa[K], pb[K], pc[K];
 #pragma unroll
     for (int j = 0; j < K; ++j)
     {
-        if (I[j].w >= 0)
+        if (I[j].x >= 0)
         {
             pa[j] = a.xm[I[j].x];
             pb[j] = a.xm[I[j].y];
@@ -312,16 +325,16 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     }
 #pragma unroll
     for (int j = 0; j < K; ++j)
-        if (I[j].w >= 0)
-            AccumulateOwnFrame(q0[j], q1[j], q2[j], own, pa[j], pb[j], pc[j], B);
+        if (I[j].x >= 0)
+            AccumulateOwnFrame(q0[j], q1[j], q2[j], __int_as_float(I[j].w), own, pa[j], pb[j], pc[j], B);
     // incidences beyond K per lane (vertices of unusually high degree)
     for (int k = kb + lane + K * L; k < ke; k += L)
     {
         int4 const J = LoadIds(a.ids + k, nt);
-        if (J.w < 0)
+        if (J.x < 0)
             continue;
-        AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt), own,
-                           a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
+        AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
+                           __int_as_float(J.w), own, a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
     }
     GroupSum<L>(B);
     if (!active || lane != 0)
