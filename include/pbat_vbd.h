@@ -55,10 +55,13 @@ extern "C" {
 #define PBAT_VBD_MAX_COLORS 128
 
 /* sim/vbd/Enums.h:21-28 EAccelerationStrategy; gpu::vbd::Integrator's factory
- * (gpu/vbd/Integrator.cu:15-33) picks the integrator from it. Only None and
- * Chebyshev (gpu/impl/vbd/ChebyshevIntegrator.cu) are available here. */
+ * (gpu/vbd/Integrator.cu:15-33) picks the integrator from it. None, Chebyshev
+ * (gpu/impl/vbd/ChebyshevIntegrator.cu) and Anderson
+ * (gpu/impl/vbd/AndersonIntegrator.cu; single meshes, not batches) are available. */
 #define PBAT_VBD_ACCELERATION_NONE 0
 #define PBAT_VBD_ACCELERATION_CHEBYSHEV 1
+#define PBAT_VBD_ACCELERATION_ANDERSON 2
+#define PBAT_VBD_MAX_ANDERSON_WINDOW 16
 
 /* Message of the last failure on the calling thread ("" if none). */
 const char* pbat_vbd_last_error(void);
@@ -122,6 +125,7 @@ typedef struct pbat_vbd_data {
     double detHZero;         /* numerical zero of |det H| */
     int32_t acceleration;    /* PBAT_VBD_ACCELERATION_* (sim/vbd/Enums.h:21-28) */
     double rho;              /* Chebyshev estimated spectral radius, 0 < rho < 1 (sim/vbd/Data.h:233-234) */
+    int32_t window;          /* Anderson window size, 1..PBAT_VBD_MAX_ANDERSON_WINDOW (sim/vbd/Data.h:236-237) */
 } pbat_vbd_data;
 
 typedef struct pbat_vbd_integrator pbat_vbd_integrator;

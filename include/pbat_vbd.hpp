@@ -81,6 +81,7 @@ struct Data
     double detHZero = 1e-7;
     int32_t acceleration = PBAT_VBD_ACCELERATION_NONE; // sim/vbd/Data.h:231
     double rho = 1.;                                   // sim/vbd/Data.h:234
+    int32_t window = 5;                                // sim/vbd/Data.h:237
     std::vector<double> mue, lambdae;
 
     template <class TX, class TE>
@@ -137,6 +138,13 @@ struct Data
     {
         rho = rhoIn;
         acceleration = PBAT_VBD_ACCELERATION_CHEBYSHEV;
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:148-153 */
+    Data& WithAndersonAcceleration(int32_t windowIn)
+    {
+        window = windowIn;
+        acceleration = PBAT_VBD_ACCELERATION_ANDERSON;
         return *this;
     }
     Data& WithHessianDeterminantZeroUnder(double zero)
@@ -226,6 +234,7 @@ class Integrator
         d.detHZero = data.detHZero;
         d.acceleration = data.acceleration;
         d.rho = data.rho;
+        d.window = data.window;
         Check(pbat_vbd_integrator_create(&d, &mImpl));
         mVertices = d.nV;
     }

@@ -303,6 +303,46 @@ class OracleVbd:
         if rc != 0:
             raise ValueError(f"oracle step failed ({rc})")
 
+    def step_anderson(self, dt: float, iterations: int, substeps: int = 1, window: int = 5,
+                      rcond: float = 1e-10) -> None:
+        """AndersonIntegrator::Solve (sim/vbd/AndersonIntegrator.cpp:24-56) around the
+        same substep pieces as step(): after each VBD iteration k >= 1, with
+        G_k = x, F_k = G_k - x_{k-1} and the last min(k, m) differences DG, DF,
+        alpha = argmin |DF alpha - F_k| and x = G_k - DG alpha. The least-squares
+        solve is numpy's SVD-based lstsq with singular values below rcond x the
+        largest dropped (the reference uses Eigen's complete orthogonal
+        decomposition with threshold 1e-10)."""
+        L = lib()
+        for f in ("ora_substep_begin", "ora_sweep", "ora_substep_end"):
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_double]
+            getattr(L, f).restype = None
+        sdt = float(dt) / substeps
+        m = int(window)
+        n = 3 * self.nV
+        for _ in range(substeps):
+            s = self._state()
+            L.ora_substep_begin(ctypes.byref(s), sdt)
+            DGK = np.zeros((n, m))
+            DFK = np.zeros((n, m))
+            xkm1 = self.x.reshape(-1).copy()
+            if iterations > 0:
+                L.ora_sweep(ctypes.byref(s), sdt)
+                Gkm1 = self.x.reshape(-1).copy()
+                Fkm1 = Gkm1 - xkm1
+            for k in range(1, iterations):
+                xkm1 = self.x.reshape(-1).copy()
+                L.ora_sweep(ctypes.byref(s), sdt)
+                Gk = self.x.reshape(-1).copy()
+                Fk = Gk - xkm1
+                col = (k - 1) % m
+                DGK[:, col] = Gk - Gkm1
+                DFK[:, col] = Fk - Fkm1
+                Gkm1, Fkm1 = Gk, Fk
+                mk = min(m, k)
+                alpha = np.linalg.lstsq(DFK[:, :mk], Fk, rcond=rcond)[0]
+                self.x.reshape(-1)[:] = Gk - DGK[:, :mk] @ alpha
+            L.ora_substep_end(ctypes.byref(s), sdt)
+
     def objective(self, x, xtilde, dt) -> float:
         s = self._state()
         xv = np.ascontiguousarray(np.asarray(x, np.float64).T)

@@ -706,6 +706,26 @@ static void* worker(void* arg)
     return NULL;
 }
 
+/* The pieces of one substep, for integrators that act between iterations
+ * (sim/vbd/AndersonIntegrator.cpp:24-56): begin (x_t, v_t, x~, initial guess),
+ * one VBD iteration over all colours, end (velocity). Single-threaded. */
+void ora_substep_begin(ora_vbd* s, double sdt)
+{
+    substep_begin(s, 0, s->nV, sdt, sdt * sdt);
+}
+
+void ora_sweep(ora_vbd* s, double sdt)
+{
+    for (int64_t p = 0; p < s->nPartitions; ++p)
+        for (int64_t q = s->Pptr[p]; q < s->Pptr[p + 1]; ++q)
+            solve_vertex(s, s->Padj[q], sdt, sdt * sdt);
+}
+
+void ora_substep_end(ora_vbd* s, double sdt)
+{
+    substep_end(s, 0, s->nV, sdt);
+}
+
 int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps)
 {
     if (substeps < 1 || iterations < 0)

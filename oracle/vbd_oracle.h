@@ -121,6 +121,12 @@ double ora_chebyshev_omega(int64_t k, double rho2, double omega);
 
 /* sim/vbd/Integrator.cpp:22-45 */
 int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps);
+/* One substep in pieces (single-threaded): Integrator::Step's preamble
+ * (sim/vbd/Integrator.cpp:31-35, 54-71), one RunVbdIteration (:73-87), and the
+ * velocity update (:39). sdt is the substep length. */
+void ora_substep_begin(ora_vbd* s, double sdt);
+void ora_sweep(ora_vbd* s, double sdt);
+void ora_substep_end(ora_vbd* s, double sdt);
 /* sim/vbd/Integrator.cpp:98-136, elastic part only (before damping and inertia):
  * H (3x3 column-major) and g accumulated through the full 9x9 SNH Hessian. */
 void ora_vertex_elastic_block(const ora_vbd* s, int64_t i, double H[9], double g[3]);

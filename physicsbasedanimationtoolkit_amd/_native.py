@@ -43,7 +43,7 @@ class Data(ctypes.Structure):
         ("GVGp", p_i64), ("GVGe", p_i64), ("GVGilocal", p_i64),
         ("nPartitions", c_i64), ("Pptr", p_i64), ("Padj", p_i64),
         ("strategy", c_i32), ("kD", c_f64), ("detHZero", c_f64),
-        ("acceleration", c_i32), ("rho", c_f64),
+        ("acceleration", c_i32), ("rho", c_f64), ("window", c_i32),
     ]
 
 

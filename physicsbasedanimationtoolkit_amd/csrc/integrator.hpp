@@ -142,14 +142,14 @@ struct GraphKey
     int iterations = -1, substeps = -1, strategy = -1, lanes = -1, block = -1, perLane = -1;
     bool xcd = false, nt = false, ell = false;
     float kD = 0.f, detHZero = 0.f;
-    int acceleration = -1;
+    int acceleration = -1, window = 0;
     float rho = 0.f;
     bool operator==(GraphKey const& o) const
     {
         return dt == o.dt && iterations == o.iterations && substeps == o.substeps && strategy == o.strategy &&
                lanes == o.lanes && block == o.block && perLane == o.perLane && xcd == o.xcd && nt == o.nt &&
                ell == o.ell && kD == o.kD && detHZero == o.detHZero && acceleration == o.acceleration &&
-               rho == o.rho;
+               rho == o.rho && window == o.window;
     }
 };
 
@@ -188,7 +188,10 @@ struct pbat_vbd_integrator
     int strategy = PBAT_VBD_INIT_ADAPTIVE_PBAT;
     int acceleration = PBAT_VBD_ACCELERATION_NONE;
     float rho = 1.f;                            // Chebyshev estimated spectral radius
-    pbat_vbd::DeviceArray<float4> xkm1, xkm2;   // Chebyshev: previous two iterates
+    int window = 5;                             // Anderson window size
+    pbat_vbd::DeviceArray<float4> xkm1, xkm2;   // Chebyshev: previous two iterates; Anderson: x_{k-1}
+    pbat_vbd::DeviceArray<float4> gkm1, fkm1, dgk, dfk; // Anderson: G_{k-1}, F_{k-1}, window columns
+    pbat_vbd::DeviceArray<double> andersonAcc, andersonGram; // Anderson: dot products, Gram matrices
     float kD = 0.f;
     float detHZero = 1e-7f;
     int blockSize = 256;
@@ -225,6 +228,7 @@ struct pbat_vbd_integrator
         xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
         tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
         slotsRows.reset(), slotsEll.reset(), matBase.reset(), meshOf.reset(), xkm1.reset(), xkm2.reset();
+        gkm1.reset(), fkm1.reset(), dgk.reset(), dfk.reset(), andersonAcc.reset(), andersonGram.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)
             (void)hipStreamDestroy(stream);

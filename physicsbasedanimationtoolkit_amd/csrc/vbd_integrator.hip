@@ -171,15 +171,65 @@ float ChebyshevOmega(int k, float rho2, float omega)
     return (k == 0) ? 1.f : (k == 1) ? 2.f / (2.f - rho2) : 4.f / 4.f - rho2 * omega;
 }
 
-// After iteration k of a substep: the Chebyshev update (no-op without acceleration).
+// Before iteration 0 of a substep (Anderson: remember x).
+void BeginAcceleration(pbat_vbd_integrator& h)
+{
+    if (h.acceleration != PBAT_VBD_ACCELERATION_ANDERSON)
+        return;
+    int const n = static_cast<int>(h.nV);
+    hipLaunchKernelGGL(device::AndersonStart, dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr, h.xkm1.ptr,
+                       n, h.andersonAcc.ptr);
+}
+
+template <int M>
+void LaunchAndersonWindow(pbat_vbd_integrator& h, int n, int col, int mk, double* acc)
+{
+    hipLaunchKernelGGL((device::AndersonWindow<M>), dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr,
+                       h.xkm1.ptr, h.gkm1.ptr, h.fkm1.ptr, h.dgk.ptr, h.dfk.ptr, n, col, mk, acc);
+}
+
+// After iteration k of a substep: the Chebyshev update
+// (gpu/impl/vbd/ChebyshevIntegrator.cu:24-35) or the Anderson window update
+// and accelerated step (gpu/impl/vbd/AndersonIntegrator.cu:36-52); no-op
+// without acceleration.
 void LaunchAcceleration(pbat_vbd_integrator& h, int k, float& omega)
 {
-    if (h.acceleration != PBAT_VBD_ACCELERATION_CHEBYSHEV)
-        return;
-    omega = ChebyshevOmega(k, h.rho * h.rho, omega);
     int const n = static_cast<int>(h.nV);
-    hipLaunchKernelGGL(device::ChebyshevUpdate, dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr,
-                       h.xkm1.ptr, h.xkm2.ptr, n, k, omega);
+    int const grid = (n + 255) / 256;
+    if (h.acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV)
+    {
+        omega = ChebyshevOmega(k, h.rho * h.rho, omega);
+        hipLaunchKernelGGL(device::ChebyshevUpdate, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr, h.xkm1.ptr,
+                           h.xkm2.ptr, n, k, omega);
+        return;
+    }
+    if (h.acceleration != PBAT_VBD_ACCELERATION_ANDERSON)
+        return;
+    if (k == 0)
+    {
+        hipLaunchKernelGGL(device::AndersonFirst, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr, h.xkm1.ptr,
+                           h.gkm1.ptr, h.fkm1.ptr, n);
+        return;
+    }
+    int const m = h.window;
+    int const col = (k - 1) % m, mk = std::min(m, k);
+    size_t const setDoubles = static_cast<size_t>(device::AndersonAccDoubles());
+    double* acc = h.andersonAcc.ptr + (k & 1) * setDoubles;
+    double* accNext = h.andersonAcc.ptr + ((k + 1) & 1) * setDoubles;
+    if (m <= 2)
+        LaunchAndersonWindow<2>(h, n, col, mk, acc);
+    else if (m <= 4)
+        LaunchAndersonWindow<4>(h, n, col, mk, acc);
+    else if (m <= 8)
+        LaunchAndersonWindow<8>(h, n, col, mk, acc);
+    else
+        LaunchAndersonWindow<16>(h, n, col, mk, acc);
+    size_t const gram = static_cast<size_t>(m) * m;
+    hipLaunchKernelGGL(device::AndersonApply, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr, h.xkm1.ptr,
+                       static_cast<float4 const*>(h.gkm1.ptr), static_cast<float4 const*>(h.dgk.ptr), n, m, col, mk,
+                       static_cast<double const*>(acc), accNext,
+                       static_cast<double const*>(h.andersonGram.ptr + ((k - 1) & 1) * gram),
+                       h.andersonGram.ptr + (k & 1) * gram);
 }
 
 int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool eventBrackets)
@@ -201,6 +251,7 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         if (eventBrackets)
             PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
         float omega = 0.f;
+        BeginAcceleration(h);
         for (int k = 0; k < iterations; ++k)
         {
             for (int c = 0; c < nColors; ++c)
@@ -260,7 +311,7 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     key.dt = dt, key.iterations = iterations, key.substeps = substeps, key.strategy = h.strategy;
     key.lanes = EffectiveLanes(h), key.block = h.blockSize, key.kD = h.kD, key.detHZero = h.detHZero;
     key.perLane = EffectivePerLane(h), key.xcd = h.xcdRemap, key.nt = h.ntStream, key.ell = h.useEll;
-    key.acceleration = h.acceleration, key.rho = h.rho;
+    key.acceleration = h.acceleration, key.rho = h.rho, key.window = h.window;
     if (!(h.graphExec && h.graphKey == key))
     {
         InvalidateGraph(h);
@@ -359,10 +410,16 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid number of partitions");
     if (d->strategy < 0 || d->strategy > 4)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid initialization strategy");
-    if (d->acceleration != PBAT_VBD_ACCELERATION_NONE && d->acceleration != PBAT_VBD_ACCELERATION_CHEBYSHEV)
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "only the base and Chebyshev-accelerated VBD integrators are available");
+    if (d->acceleration != PBAT_VBD_ACCELERATION_NONE && d->acceleration != PBAT_VBD_ACCELERATION_CHEBYSHEV &&
+        d->acceleration != PBAT_VBD_ACCELERATION_ANDERSON)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
+             "only the base, Chebyshev- and Anderson-accelerated VBD integrators are available");
     if (d->acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV && !(d->rho > 0. && d->rho < 1.))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "Expected 0 < rho < 1");
+    if (d->acceleration == PBAT_VBD_ACCELERATION_ANDERSON &&
+        (d->window < 1 || d->window > PBAT_VBD_MAX_ANDERSON_WINDOW))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
+             "Anderson window size must be in [1, " + std::to_string(PBAT_VBD_MAX_ANDERSON_WINDOW) + "]");
     int64_t const nV = d->nV, nT = d->nT;
     if (d->GVGp[0] != 0)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "GVGp must start at 0");
@@ -392,6 +449,7 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     h->strategy = d->strategy;
     h->acceleration = d->acceleration;
     h->rho = static_cast<float>(d->rho);
+    h->window = d->acceleration == PBAT_VBD_ACCELERATION_ANDERSON ? d->window : 0;
     h->kD = static_cast<float>(d->kD);
     h->detHZero = static_cast<float>(d->detHZero);
 
@@ -593,6 +651,14 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         h->xkm1.allocate(static_cast<size_t>(nV));
         h->xkm2.allocate(static_cast<size_t>(nV));
     }
+    if (h->acceleration == PBAT_VBD_ACCELERATION_ANDERSON)
+    {
+        size_t const n = static_cast<size_t>(nV), m = static_cast<size_t>(h->window);
+        h->xkm1.allocate(n), h->gkm1.allocate(n), h->fkm1.allocate(n);
+        h->dgk.allocate(m * n), h->dfk.allocate(m * n);
+        h->andersonAcc.allocate(2 * static_cast<size_t>(device::AndersonAccDoubles()));
+        h->andersonGram.allocate(2 * m * m);
+    }
     h->energy.allocate(3);
     PBAT_HIP(hipMemsetAsync(h->xtil.ptr, 0, sizeof(float4) * nV, h->stream));
     PBAT_HIP(hipMemsetAsync(h->vprev.ptr, 0, sizeof(float4) * nV, h->stream));
@@ -719,6 +785,7 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
                            h.vel.ptr, h.vprev.ptr, h.aext.ptr, n, sdt, sdt2, h.strategy);
         WriteVertexMatrix(h, h.xtil.ptr, Join(dir, name("xtilde", t, s, -1)));
         float omega = 0.f;
+        BeginAcceleration(h);
         for (int k = 0; k < iterations; ++k)
         {
             WriteVertexMatrix(h, h.xm.ptr, Join(dir, name("x", t, s, k)));
@@ -872,6 +939,9 @@ void BuildUnion(pbat_vbd_data const* meshes, int count, UnionData& U)
             (m.acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV && m.rho != meshes[0].rho))
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
                  "meshes of a batch must share the initialization strategy, kD, detHZero and acceleration");
+        if (m.acceleration == PBAT_VBD_ACCELERATION_ANDERSON)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
+                 "Anderson acceleration couples every vertex of a handle; batches do not support it");
         nV += m.nV;
         nT += m.nT;
         nInc += m.GVGp[m.nV];
@@ -935,7 +1005,7 @@ void BuildUnion(pbat_vbd_data const* meshes, int count, UnionData& U)
     // Padj may be empty (every vertex fixed); any non-null pointer serves then
     d.nPartitions = nColors, d.Pptr = U.Pptr.data(), d.Padj = U.Padj.empty() ? U.Pptr.data() : U.Padj.data();
     d.strategy = meshes[0].strategy, d.kD = meshes[0].kD, d.detHZero = meshes[0].detHZero;
-    d.acceleration = meshes[0].acceleration, d.rho = meshes[0].rho;
+    d.acceleration = meshes[0].acceleration, d.rho = meshes[0].rho, d.window = meshes[0].window;
 }
 
 pbat_vbd_integrator& Handle(pbat_vbd_integrator* h)

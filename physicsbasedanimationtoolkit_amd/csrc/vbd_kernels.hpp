@@ -429,6 +429,215 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     xkm1[i] = x;
 }
 
+// ---- Anderson acceleration (gpu/impl/vbd/AndersonIntegrator.cu:36-105) ------
+// After VBD iteration k >= 1 of a substep, with G = x, F = G - x_{k-1} and the
+// window of the last min(k, m) differences DG, DF (column (k-1) mod m is the
+// newest): alpha = argmin |DF alpha - F| and x = G - DG alpha. The least
+// squares is solved from the FP64 Gram matrix DF^T DF and DF^T F; only the
+// newest column's row of the Gram matrix is recomputed each iteration.
+
+constexpr int kMaxAndersonWindow = 16;
+constexpr int kAndersonSpread = 16; // atomic targets per accumulated value
+constexpr int kAndersonStride = 16; // doubles between targets (128 B)
+
+// Accumulated values of one iteration: 2 kMaxAndersonWindow values
+// (DF_j . DF_new, then DF_j . F), each over kAndersonSpread targets.
+__host__ __device__ constexpr int AndersonAccDoubles()
+{
+    return 2 * kMaxAndersonWindow * kAndersonSpread * kAndersonStride;
+}
+
+// Before iteration 0 of a substep: x_{k-1} = x; clear both accumulator sets.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void AndersonStart(float4 const* __restrict__ xm,
+                                                                             float4* __restrict__ xkm1, int n,
+                                                                             double* __restrict__ acc)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        xkm1[i] = xm[i];
+    if (blockIdx.x == 0)
+        for (int j = static_cast<int>(threadIdx.x); j < 2 * AndersonAccDoubles(); j += static_cast<int>(blockDim.x))
+            acc[j] = 0.;
+}
+
+// After iteration 0: G_{k-1} = x, F_{k-1} = x - x_{k-1}, then x_{k-1} = x.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void AndersonFirst(float4 const* __restrict__ xm,
+                                                                             float4* __restrict__ xkm1,
+                                                                             float4* __restrict__ gkm1,
+                                                                             float4* __restrict__ fkm1, int n)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 const g = xm[i], p = xkm1[i];
+    gkm1[i] = make_float4(g.x, g.y, g.z, 0.f);
+    fkm1[i] = make_float4(g.x - p.x, g.y - p.y, g.z - p.z, 0.f);
+    xkm1[i] = make_float4(g.x, g.y, g.z, 0.f);
+}
+
+__device__ __forceinline__ double Dot3d(float4 const& u, float4 const& w)
+{
+    return double(u.x) * w.x + double(u.y) * w.y + double(u.z) * w.z;
+}
+
+// After iteration k >= 1: update the window (column col) and accumulate
+// DF_j . DF_col and DF_j . F for j < mk into acc. blockDim = 256.
+template <int M>
+__global__ __launch_bounds__(256) void AndersonWindow(float4 const* __restrict__ xm, float4 const* __restrict__ xkm1,
+                                                      float4* __restrict__ gkm1, float4* __restrict__ fkm1,
+                                                      float4* __restrict__ dgk, float4* __restrict__ dfk, int n,
+                                                      int col, int mk, double* __restrict__ acc)
+{
+    __shared__ double red[4][2 * M];
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    double vals[2 * M];
+#pragma unroll
+    for (int j = 0; j < 2 * M; ++j)
+        vals[j] = 0.;
+    if (i < n)
+    {
+        float4 const g = xm[i], p = xkm1[i], g1 = gkm1[i], f1 = fkm1[i];
+        float4 const f = make_float4(g.x - p.x, g.y - p.y, g.z - p.z, 0.f);
+        float4 const dg = make_float4(g.x - g1.x, g.y - g1.y, g.z - g1.z, 0.f);
+        float4 const df = make_float4(f.x - f1.x, f.y - f1.y, f.z - f1.z, 0.f);
+        dgk[static_cast<size_t>(col) * n + i] = dg;
+        dfk[static_cast<size_t>(col) * n + i] = df;
+        gkm1[i] = make_float4(g.x, g.y, g.z, 0.f);
+        fkm1[i] = f;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < mk)
+            {
+                float4 const dj = (j == col) ? df : dfk[static_cast<size_t>(j) * n + i];
+                vals[j] = Dot3d(dj, df);
+                vals[M + j] = Dot3d(dj, f);
+            }
+    }
+    int const lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
+#pragma unroll
+    for (int j = 0; j < 2 * M; ++j)
+    {
+        double v = vals[j];
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off);
+        if (lane == 0)
+            red[wave][j] = v;
+    }
+    __syncthreads();
+    int const nw = static_cast<int>((blockDim.x + 63) / 64);
+    int const t = static_cast<int>(threadIdx.x);
+    if (t < 2 * M && ((t < M) ? t : t - M) < mk)
+    {
+        double sum = 0.;
+        for (int w = 0; w < nw; ++w)
+            sum += red[w][t];
+        int const slot = static_cast<int>(blockIdx.x) & (kAndersonSpread - 1);
+        int const idx = (t < M) ? t : kMaxAndersonWindow + (t - M);
+        atomicAdd(acc + (static_cast<size_t>(idx) * kAndersonSpread + slot) * kAndersonStride, sum);
+    }
+}
+
+// After AndersonWindow: every workgroup forms the mk x mk Gram matrix (the
+// previous one, gramOld, with row and column col replaced) and solves for
+// alpha by an FP64 Cholesky factorisation that drops columns whose pivot is
+// below 1e-13 of the largest diagonal (a zero or dependent difference); the
+// first workgroup stores the Gram matrix (gramNew) and clears accNext. Then
+// x = G - DG alpha and x_{k-1} = x.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void AndersonApply(float4* __restrict__ xm, float4* __restrict__ xkm1,
+                                                     float4 const* __restrict__ gkm1,
+                                                     float4 const* __restrict__ dgk, int n, int m, int col, int mk,
+                                                     double const* __restrict__ acc, double* __restrict__ accNext,
+                                                     double const* __restrict__ gramOld, double* __restrict__ gramNew)
+{
+    constexpr int W = kMaxAndersonWindow;
+    __shared__ double sG[W * W], sL[W * W];
+    __shared__ double sb[W], sy[W], sa[W];
+    __shared__ int sKeep[W];
+    __shared__ float sAlpha[W];
+    int const t = static_cast<int>(threadIdx.x);
+    auto sum = [&](int idx) {
+        double s = 0.;
+        for (int k = 0; k < kAndersonSpread; ++k)
+            s += acc[(static_cast<size_t>(idx) * kAndersonSpread + k) * kAndersonStride];
+        return s;
+    };
+    if (t < mk * mk)
+    {
+        int const r = t / mk, c = t % mk;
+        double g;
+        if (r == col)
+            g = sum(c);
+        else if (c == col)
+            g = sum(r);
+        else
+            g = gramOld[r * m + c];
+        sG[r * W + c] = g;
+        if (blockIdx.x == 0)
+            gramNew[r * m + c] = g;
+    }
+    if (t < mk)
+        sb[t] = sum(kMaxAndersonWindow + t);
+    __syncthreads();
+    if (t == 0)
+    {
+        // Cholesky factor L (column j of L in sL[. * W + j]) with dropped columns
+        double dmax = 0.;
+        for (int j = 0; j < mk; ++j)
+            dmax = fmax(dmax, sG[j * W + j]);
+        for (int j = 0; j < mk; ++j)
+        {
+            double d = sG[j * W + j];
+            for (int p = 0; p < j; ++p)
+                d -= sL[j * W + p] * sL[j * W + p];
+            int const keep = (d > 1e-13 * dmax && d > 0.) ? 1 : 0;
+            sKeep[j] = keep;
+            double const ljj = keep ? sqrt(d) : 0.;
+            sL[j * W + j] = ljj;
+            for (int r = j + 1; r < mk; ++r)
+            {
+                double v = sG[r * W + j];
+                for (int p = 0; p < j; ++p)
+                    v -= sL[r * W + p] * sL[j * W + p];
+                sL[r * W + j] = keep ? v / ljj : 0.;
+            }
+        }
+        for (int j = 0; j < mk; ++j)
+        {
+            double v = sb[j];
+            for (int p = 0; p < j; ++p)
+                v -= sL[j * W + p] * sy[p];
+            sy[j] = sKeep[j] ? v / sL[j * W + j] : 0.;
+        }
+        for (int j = mk - 1; j >= 0; --j)
+        {
+            double v = sy[j];
+            for (int p = j + 1; p < mk; ++p)
+                v -= sL[p * W + j] * sa[p];
+            sa[j] = sKeep[j] ? v / sL[j * W + j] : 0.;
+        }
+        for (int j = 0; j < mk; ++j)
+            sAlpha[j] = static_cast<float>(sa[j]);
+    }
+    if (blockIdx.x == 0)
+        for (int j = t; j < AndersonAccDoubles(); j += static_cast<int>(blockDim.x))
+            accNext[j] = 0.;
+    __syncthreads();
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 const g = gkm1[i];
+    float x0 = g.x, x1 = g.y, x2 = g.z;
+    for (int j = 0; j < mk; ++j)
+    {
+        float4 const d = dgk[static_cast<size_t>(j) * n + i];
+        float const aj = sAlpha[j];
+        x0 -= aj * d.x, x1 -= aj * d.y, x2 -= aj * d.z;
+    }
+    float const mass = xm[i].w;
+    xm[i] = make_float4(x0, x1, x2, mass);
+    xkm1[i] = make_float4(x0, x1, x2, 0.f);
+}
+
 // FP64 energy reduction: out[0] += 1/2 m |v|^2, out[2] += -m a.x over vertices,
 // out[1] += w Psi(F) over tets. One atomic per block and term. blockDim = 256.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm, float4 const* __restrict__ vel,

@@ -36,10 +36,11 @@ def _marshal(data: Data):
     """pbat_vbd_data view of a constructed Data, plus the arrays it points into."""
     if not isinstance(data, Data):
         raise TypeError("expected a pbat.sim.vbd.Data")
-    if data.accelerator not in (AccelerationStrategy.Base, AccelerationStrategy.Chebyshev):
+    if data.accelerator not in (AccelerationStrategy.Base, AccelerationStrategy.Chebyshev,
+                                AccelerationStrategy.Anderson):
         raise ValueError(
             f"acceleration strategy {data.accelerator.name} is not available on this device path; "
-            "only the base and Chebyshev-accelerated VBD integrators are implemented")
+            "only the base, Chebyshev- and Anderson-accelerated VBD integrators are implemented")
     if data.m.size == 0:
         raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
     nV, nT = data.x.shape[1], data.E.shape[1]
@@ -74,8 +75,9 @@ def _marshal(data: Data):
     d.strategy = int(data.strategy)
     d.kD = float(data.kD)
     d.detHZero = float(data.detH_zero)
-    d.acceleration = 1 if data.accelerator == AccelerationStrategy.Chebyshev else 0
+    d.acceleration = {AccelerationStrategy.Chebyshev: 1, AccelerationStrategy.Anderson: 2}.get(data.accelerator, 0)
     d.rho = float(data.rho)
+    d.window = int(data.manderson)
     return d, keep
 
 

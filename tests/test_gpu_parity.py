@@ -255,9 +255,86 @@ def test_chebyshev_trajectory_and_graph_replay(gpu_device):
 
 def test_unavailable_accelerators_raise(gpu_device):
     s = meshes.plain_cpu_scene(3)
-    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_anderson_acceleration(5).construct())
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_trust_region_acceleration(0.2, 2.0).construct())
     with pytest.raises(ValueError):
         pbat.gpu.vbd.Integrator(data)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_anderson_acceleration(17).construct())
+    with pytest.raises(ValueError):
+        pbat.gpu.vbd.Integrator(data)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_anderson_acceleration(3).construct())
+    with pytest.raises(ValueError):
+        pbat.gpu.vbd.BatchIntegrator([data, data])
+
+
+# ---- Anderson-accelerated VBD (gpu/impl/vbd/AndersonIntegrator.cu) -----------
+# Anderson's least-squares coefficients are computed from differences of
+# successive iterates, which FP32 and FP64 trajectories resolve differently
+# (the iterates differ by ~1e-7 while the differences are ~1e-5), and the
+# extrapolation x = G - DG alpha carries that into the positions: one step
+# lands ~0.5% of the displacement (~1e-5 of the positions) from the FP64
+# oracle. The reference's GPU path computes in FP32 as well.
+ANDERSON_POS_TOL = 5e-5
+
+def build_anderson_pair(s, window, v=None):
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+            .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).with_anderson_acceleration(window))
+    if v is not None:
+        data = data.with_velocity(v)
+    data = data.construct()
+    gpu = pbat.gpu.vbd.Integrator(data)
+    ref = O.OracleVbd(s.X, s.E, v=v, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc)
+    return gpu, ref
+
+
+def test_anderson_reference_cube(gpu_device):
+    """gpu/impl/vbd/AndersonIntegrator.cu test: m = 5, dt 1e-2, 10 iterations on the
+    8-vertex cube: it falls along z only and the objective gradient vanishes."""
+    P = np.array([[0., 1., 0., 1., 0., 1., 0., 1.], [0., 0., 1., 1., 0., 0., 1., 1.],
+                  [0., 0., 0., 0., 1., 1., 1., 1.]])
+    T = np.array([[0, 3, 5, 6, 0], [1, 2, 4, 7, 5], [3, 0, 6, 5, 3], [5, 6, 0, 3, 6]])
+    data = pbat.sim.vbd.Data().with_volume_mesh(P, T).with_anderson_acceleration(5).construct()
+    vbd = pbat.gpu.vbd.Integrator(data)
+    vbd.step(1e-2, 10, 1)
+    dx = vbd.x - P.astype(np.float32)
+    assert (dx[2] < 0).all()
+    assert (np.abs(dx[:2]) < 1e-4).all()
+    ref = O.OracleVbd(P, T)
+    xtilde = (ref.x + 1e-2 * ref.v + 1e-4 * ref.aext).T
+    assert np.linalg.norm(ref.objective_gradient(vbd.x.astype(np.float64), xtilde, 1e-2)) < 1e-4
+
+
+@pytest.mark.parametrize("window", [1, 3, 5])
+def test_anderson_one_step_plain_10(gpu_device, window):
+    s = meshes.plain_cpu_scene(10)
+    gpu, ref = build_anderson_pair(s, window)
+    gpu.step(s.dt, s.iterations, 1)
+    ref.step_anderson(s.dt, s.iterations, 1, window=window)
+    e = rel(gpu.x, ref.positions)
+    ed = rel(gpu.x - s.X, ref.positions - s.X)
+    print(f"window {window}: rel pos err {e:.3e} rel disp err {ed:.3e}")
+    assert e <= ANDERSON_POS_TOL
+    assert ed <= 2e-2
+
+
+def test_anderson_moving_and_graph_replay(gpu_device):
+    """A jittered 16^3 mesh with an initial velocity field, 3 steps with 2 substeps:
+    graph replay bitwise equal to eager launches; each step within ANDERSON_POS_TOL of
+    the oracle stepping from the device's state."""
+    s = meshes.plain_gpu_scene(16, seed=3)
+    v0 = np.random.default_rng(5).normal(size=(3, s.n_vertices)) * 0.3
+    g1, ref = build_anderson_pair(s, 5, v=v0)
+    g2, _ = build_anderson_pair(s, 5, v=v0)
+    g2.graph_capture = False
+    for _ in range(3):
+        ref.x[:] = g1.x.T
+        ref.v[:] = g1.v.T
+        g1.step(s.dt, 12, 2)
+        g2.step(s.dt, 12, 2)
+        ref.step_anderson(s.dt, 12, 2, window=5)
+        assert np.array_equal(g1.x, g2.x)
+        e = rel(g1.x, ref.positions)
+        print(f"rel pos err {e:.3e}")
+        assert e <= ANDERSON_POS_TOL
 
 
 def test_traced_step_writes_reference_files(gpu_device, tmp_path):
