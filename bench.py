@@ -84,8 +84,15 @@ def bench_batched(args, torch):
     tl = (time.perf_counter() - t0) / loop_steps
     log(f"[bench] batched {1 / tb:.1f} steps/s ({1e3 * tb:.2f} ms) vs per-mesh loop {1 / tl:.2f} steps/s "
         f"({1e3 * tl:.1f} ms): {tl / tb:.1f}x")
+    # algorithmic bytes of one sweep over the batch (all colours), per step time
+    achieved = (BYTES_PER_VERTEX * nUpd + BYTES_PER_INCIDENCE * nInc) * iters / tb / 1e9
+    log(f"[bench] batched sweeps: {achieved:.0f} GB/s algorithmic, step-averaged "
+        f"({achieved / HBM_PEAK_GBS:.3f} of 8 TB/s)")
     return {"workload": f"{count} jittered cube meshes, grid dims U{{4..24}} per axis, per-mesh Y, nu; "
                         f"h=1/60, {iters} iterations",
+            "roofline_step_averaged": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": achieved / HBM_PEAK_GBS,
+                                       "note": "algorithmic bytes of the step's sweeps / whole step time"},
             "meshes": count, "vertices": int(batch.offsets[-1]), "updated_vertices": nUpd,
             "steps_per_s": 1 / tb, "per_mesh_loop_steps_per_s": 1 / tl, "speedup_vs_loop": tl / tb,
             "vertex_updates_per_s": nUpd * iters / tb, "steps": steps, "loop_steps": loop_steps,

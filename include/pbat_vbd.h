@@ -257,7 +257,9 @@ typedef struct pbat_vbd_adjoint_stats {
 int pbat_vbd_integrator_set_log_young_modulus(pbat_vbd_integrator* h, const float* logY, int64_t nT);
 int pbat_vbd_integrator_get_log_young_modulus(pbat_vbd_integrator* h, float* logY, int64_t nT);
 /* Step `steps` times (one substep each) from the current state, keeping the
- * state every `checkpoint_every` steps for the backward pass. */
+ * state every `checkpoint_every` steps for the backward pass. Setting
+ * positions, velocities, external accelerations, moduli, solver settings or
+ * the sweep shape (or autotuning) afterwards discards the rollout. */
 int pbat_vbd_rollout_forward(pbat_vbd_integrator* h, float dt, int32_t iterations, int32_t steps,
                              int32_t checkpoint_every);
 /* Gradients of L given dL/dx_T (3 x nV, required) and dL/dv_T (3 x nV,

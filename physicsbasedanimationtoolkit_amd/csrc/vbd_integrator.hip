@@ -1080,6 +1080,7 @@ extern "C" int pbat_vbd_integrator_set_positions(pbat_vbd_integrator* h, const f
         auto& H = Handle(h);
         CheckVertexArray(H, x, n, "positions");
         PBAT_HIP(hipSetDevice(H.device));
+        H.rollout.valid = false;
         UploadVec(H, H.xm.ptr, x, H.massOrig.data());
         UploadVec(H, H.xt.ptr, x, nullptr);
     });
@@ -1091,6 +1092,7 @@ extern "C" int pbat_vbd_integrator_set_velocities(pbat_vbd_integrator* h, const 
         auto& H = Handle(h);
         CheckVertexArray(H, v, n, "velocities");
         PBAT_HIP(hipSetDevice(H.device));
+        H.rollout.valid = false;
         UploadVec(H, H.vel.ptr, v, nullptr);
         UploadVec(H, H.vprev.ptr, v, nullptr);
     });
@@ -1102,6 +1104,7 @@ extern "C" int pbat_vbd_integrator_set_external_acceleration(pbat_vbd_integrator
         auto& H = Handle(h);
         CheckVertexArray(H, a, n, "external acceleration");
         PBAT_HIP(hipSetDevice(H.device));
+        H.rollout.valid = false;
         UploadVec(H, H.aext.ptr, a, nullptr);
     });
 }
@@ -1128,12 +1131,18 @@ extern "C" int pbat_vbd_integrator_get_velocities(pbat_vbd_integrator* h, float*
 
 extern "C" int pbat_vbd_integrator_set_detH_zero(pbat_vbd_integrator* h, float zero)
 {
-    return Guard([&] { Handle(h).detHZero = zero; });
+    return Guard([&] {
+        Handle(h).detHZero = zero;
+        Handle(h).rollout.valid = false;
+    });
 }
 
 extern "C" int pbat_vbd_integrator_set_rayleigh_damping(pbat_vbd_integrator* h, float kD)
 {
-    return Guard([&] { Handle(h).kD = kD; });
+    return Guard([&] {
+        Handle(h).kD = kD;
+        Handle(h).rollout.valid = false;
+    });
 }
 
 extern "C" int pbat_vbd_integrator_set_initialization_strategy(pbat_vbd_integrator* h, int32_t strategy)
@@ -1142,6 +1151,7 @@ extern "C" int pbat_vbd_integrator_set_initialization_strategy(pbat_vbd_integrat
         if (strategy < 0 || strategy > 4)
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid initialization strategy");
         Handle(h).strategy = strategy;
+        Handle(h).rollout.valid = false;
     });
 }
 
@@ -1171,6 +1181,7 @@ extern "C" int pbat_vbd_integrator_set_lanes_per_vertex(pbat_vbd_integrator* h, 
         if (lanes != 0 && (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0))
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "lanes per vertex must be 0 or a power of two in [1, 64]");
         Handle(h).lanes = lanes;
+        Handle(h).rollout.valid = false;
     });
 }
 
@@ -1193,6 +1204,7 @@ extern "C" int pbat_vbd_integrator_set_sweep_config(pbat_vbd_integrator* h, cons
         H.xcdRemap = config->xcd_remap != 0;
         H.ntStream = config->nontemporal != 0;
         H.useEll = config->fixed_stride != 0;
+        H.rollout.valid = false; // the recomputed steps must repeat the forward bit for bit
     });
 }
 
@@ -1213,7 +1225,10 @@ extern "C" int pbat_vbd_integrator_get_sweep_config(pbat_vbd_integrator* h, pbat
 extern "C" int pbat_vbd_integrator_autotune(pbat_vbd_integrator* h, float dt, int32_t sweeps, double* timings,
                                             int32_t* n_candidates)
 {
-    return Guard([&] { Autotune(Handle(h), dt, sweeps, timings, n_candidates); });
+    return Guard([&] {
+        Autotune(Handle(h), dt, sweeps, timings, n_candidates);
+        Handle(h).rollout.valid = false;
+    });
 }
 
 extern "C" int pbat_vbd_integrator_set_graph_capture(pbat_vbd_integrator* h, int32_t enabled)

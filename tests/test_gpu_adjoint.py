@@ -315,6 +315,15 @@ def test_adjoint_errors(gpu_device):
     g.kD = 1e-3
     with pytest.raises(ValueError):
         g.rollout_backward(np.ones_like(X))
+    g.kD = 0.0
+    g.rollout_forward(DT, 5, 2)
+    g.a = meshes.gravity(X.shape[1]) * 2.0  # changes the steps the backward pass would recompute
+    with pytest.raises(ValueError):
+        g.rollout_backward(np.ones_like(X))
+    g.rollout_forward(DT, 5, 2)
+    g.set_sweep_config(8, 0, True, False)
+    with pytest.raises(ValueError):
+        g.rollout_backward(np.ones_like(X))
     with pytest.raises(ValueError):
         g.set_log_young_modulus(np.zeros(3))
     with pytest.raises(ValueError):
