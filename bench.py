@@ -297,7 +297,8 @@ def main():
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the VBD path has no multi-GPU form in this build "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "(a batch of meshes would shard by mesh, a single mesh by spatial subdomain with a halo "
+              "exchange per colour); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
