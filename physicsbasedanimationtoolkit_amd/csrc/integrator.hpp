@@ -134,6 +134,20 @@ struct RolloutState
     DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count
     DeviceArray<double> grad;                                 // d L / d log Y per tet
     DeviceArray<float> logY;                                  // parameter upload buffer
+    // captured chunk of CG iterations and the settings it was captured for
+    hipGraph_t cgGraph = nullptr;
+    hipGraphExec_t cgExec = nullptr;
+    std::vector<double> cgKey;
+    void ResetGraph()
+    {
+        if (cgExec)
+            (void)hipGraphExecDestroy(cgExec);
+        if (cgGraph)
+            (void)hipGraphDestroy(cgGraph);
+        cgExec = nullptr;
+        cgGraph = nullptr;
+        cgKey.clear();
+    }
 };
 
 struct GraphKey
@@ -221,6 +235,7 @@ struct pbat_vbd_integrator
             (void)hipGraphExecDestroy(graphExec);
         if (graph)
             (void)hipGraphDestroy(graph);
+        rollout.ResetGraph();
         for (hipEvent_t e : events)
             (void)hipEventDestroy(e);
         if (stream)

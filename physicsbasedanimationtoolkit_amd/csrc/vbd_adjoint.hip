@@ -206,6 +206,11 @@ SolveResult ReadResult(pbat_vbd_integrator& h)
 //   (4) p = z + beta p (finished meshes keep p);
 // the symmetric Gauss-Seidel preconditioner replaces the z of (2) by one
 // forward and one reverse coloured pass.
+//
+// With graph capture on (the integrator's setting), each chunk of check_every
+// iterations is captured once into a hipGraph and replayed; the graph's
+// kernels take the handle's fixed work vectors (x = hold, y, b = rhs), so it
+// stays valid across solves with the same settings.
 SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b,
                   pbat_vbd_adjoint_config const& cfg)
 {
@@ -264,23 +269,52 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     }
     hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
     hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr, nV);
+    auto iteration = [&] {
+        VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, cg.A, device::kAccPq, 0, n);
+        hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
+                           static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr), dinv,
+                           h.meshOf.ptr, n, cg);
+        if (!jacobi)
+        {
+            GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
+            GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, cg.A);
+        }
+        hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 0);
+        hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr,
+                           static_cast<float4 const*>(R.z.ptr), h.meshOf.ptr, n, cg);
+    };
+    bool const graph = h.useGraph && xs == R.hold.ptr && y == R.y.ptr && b == R.rhs.ptr;
+    if (graph)
+    {
+        std::vector<double> const key{double(dt),          double(cfg.solver),    double(check),
+                                      cg.tol2,             double(EffectiveLanes(h)), double(EffectivePerLane(h)),
+                                      double(h.xcdRemap),  double(h.useEll),      double(h.blockSize)};
+        if (!R.cgExec || R.cgKey != key)
+        {
+            R.ResetGraph();
+            PBAT_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < check; ++k)
+                iteration();
+            hipGraph_t g = nullptr;
+            PBAT_HIP(hipStreamEndCapture(h.stream, &g));
+            R.cgGraph = g;
+            PBAT_HIP(hipGraphInstantiate(&R.cgExec, g, nullptr, nullptr, 0));
+            R.cgKey = key;
+        }
+    }
     if (ReadLive(h) > 0)
     {
-        for (int it = 1; it <= cfg.max_iterations; ++it)
+        int it = 0;
+        while (it < cfg.max_iterations)
         {
-            VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, cg.A, device::kAccPq, 0, n);
-            hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
-                               static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr), dinv,
-                               h.meshOf.ptr, n, cg);
-            if (!jacobi)
-            {
-                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
-                GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, cg.A);
-            }
-            hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 0);
-            hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr,
-                               static_cast<float4 const*>(R.z.ptr), h.meshOf.ptr, n, cg);
-            if ((it % check == 0 || it == cfg.max_iterations) && ReadLive(h) == 0)
+            int const chunk = std::min(check, cfg.max_iterations - it);
+            if (graph && chunk == check)
+                PBAT_HIP(hipGraphLaunch(R.cgExec, h.stream));
+            else
+                for (int k = 0; k < chunk; ++k)
+                    iteration();
+            it += chunk;
+            if (ReadLive(h) == 0)
                 break;
         }
     }
@@ -399,7 +433,10 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
         }
         for (int t = t1 - 1; t >= t0; --t)
         {
-            float4 const* xs = R.segX[t - t0].ptr;
+            // the step's minimiser, in the solver's fixed buffer
+            PBAT_HIP(hipMemcpyAsync(R.hold.ptr, R.segX[t - t0].ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice,
+                                    h.stream));
+            float4 const* xs = R.hold.ptr;
             hipLaunchKernelGGL(device::AdjointRhs, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.xbar.ptr, R.vbar.ptr,
                                R.rhs.ptr, nUpd, nV, 1.f / dt);
             SolveResult const sr = Solve(h, xs, dt, R.y.ptr, R.rhs.ptr, cfg);
