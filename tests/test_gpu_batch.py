@@ -72,3 +72,26 @@ def test_batch_rejects_mixed_solver_settings(gpu_device):
     datas[1].kD = 1e-3
     with pytest.raises(ValueError):
         pbat.gpu.vbd.BatchIntegrator(datas)
+
+
+def test_chebyshev_batch_equals_per_mesh_integrators_bitwise(gpu_device):
+    """The Chebyshev update is per vertex, so a Chebyshev batch is also each mesh's own
+    Chebyshev integrator bit for bit."""
+    scenes, _ = small_batch(6, seed=5)
+    datas = [(pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+              .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).with_chebyshev_acceleration(0.8)
+              .construct()) for s in scenes]
+    batch = pbat.gpu.vbd.BatchIntegrator(datas)
+    batch.set_sweep_config(16, 1, xcd_remap=True, fixed_stride=False)
+    singles = []
+    for d in datas:
+        g = pbat.gpu.vbd.Integrator(d)
+        g.set_sweep_config(16, 1, xcd_remap=False, fixed_stride=False)
+        singles.append(g)
+    dt, it = scenes[0].dt, scenes[0].iterations
+    for _ in range(3):
+        batch.step(dt, it, 1)
+        for g in singles:
+            g.step(dt, it, 1)
+    for xb, g in zip(batch.mesh_positions(), singles):
+        assert np.array_equal(xb, g.x)
