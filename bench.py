@@ -117,7 +117,7 @@ def bench_adjoint(args, torch):
     (h = 1/60, 20 iterations) from v0 = 0.5 x random unit vectors; loss
     L = sum |x_T - x_T^target|^2 from log Y = ln 1e6. One fwd+adjoint pass = the
     20-step forward (state kept every 10 steps) + the IFT backward (one
-    block-Jacobi PCG solve per step, tolerance 1e-6 per mesh) giving dL/dlogY,
+    two-level PCG solve per step, tolerance 1e-6 per mesh) giving dL/dlogY,
     dL/dv0, dL/dx0. Also: central differences of the device forward on 64
     random log Y entries (the forward is truncated at 20 iterations, so these
     measure truncation as much as the adjoint)."""
@@ -129,7 +129,7 @@ def bench_adjoint(args, torch):
     g.autotune(s.dt, sweeps=5)
     v0 = g.v.copy()
     dt, it, T, every = s.dt, s.iterations, s.steps, 10
-    solver = dict(solver="pcg_jacobi", tolerance=1e-6, max_iterations=2000, check_every=10)
+    solver = dict(solver="pcg_two_level", tolerance=1e-6, max_iterations=2000, check_every=10)
 
     def forward(logy):
         g.set_log_young_modulus(logy)
@@ -205,7 +205,7 @@ def bench_batched_adjoint(args, torch):
     b.autotune(scenes[0].dt, sweeps=3)
     shape = b.sweep_config()
     dt, it, T, every = scenes[0].dt, scenes[0].iterations, args.batch_adjoint_steps, 10
-    solver = dict(solver="pcg_jacobi", tolerance=1e-6, max_iterations=2000, check_every=10)
+    solver = dict(solver="pcg_two_level", tolerance=1e-6, max_iterations=2000, check_every=10)
     hidden = np.concatenate([s.log_y_hidden for s in scenes])
     guess = np.concatenate([s.log_y_guess for s in scenes])
     Xb = np.concatenate([s.X for s in scenes], axis=1)

@@ -59,6 +59,7 @@ class SweepConfig(ctypes.Structure):
 ADJOINT_PCG = 0
 ADJOINT_GAUSS_SEIDEL = 1
 ADJOINT_PCG_JACOBI = 2
+ADJOINT_PCG_TWO_LEVEL = 3
 
 
 class AdjointConfig(ctypes.Structure):

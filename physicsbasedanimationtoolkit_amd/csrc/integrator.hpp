@@ -130,6 +130,9 @@ struct RolloutState
     std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
     DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
     DeviceArray<float4> dinv;                                 // inverse diagonal blocks, 2 per vertex
+    DeviceArray<double> coarse[2];                            // coarse operator / its inverse (ping-pong)
+    DeviceArray<int> coarseFinal;                             // per mesh: buffer holding its inverse
+    DeviceArray<double> rc, zc;                               // coarse residual and correction
     DeviceArray<double> scal;                                 // per-mesh solver scalars (kScalarCount x nMeshes)
     DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count
     DeviceArray<double> grad;                                 // d L / d log Y per tet
@@ -181,6 +184,17 @@ struct pbat_vbd_integrator
     std::vector<int64_t> meshOffsets; // batch: first vertex of each mesh (count + 1), empty otherwise
     int nMeshes = 1;
     pbat_vbd::DeviceArray<int> meshOf; // mesh of each internal vertex (0 for a plain integrator)
+    // Aggregates of updated vertices for the adjoint's two-level preconditioner:
+    // ids contiguous per mesh; each mesh's coarse operator is a dense block of
+    // (3 x its aggregates)^2 doubles.
+    int nAggregates = 0;
+    std::vector<int> aggBegin;          // per mesh: first aggregate id (nMeshes + 1)
+    std::vector<int64_t> coarseOffset;  // per mesh: first double of its dense block (nMeshes + 1)
+    pbat_vbd::DeviceArray<int> aggOf;   // aggregate of each internal vertex (-1: not updated)
+    pbat_vbd::DeviceArray<int> aggMesh; // mesh of each aggregate
+    pbat_vbd::DeviceArray<int> aggVertPtr, aggVerts; // vertices of each aggregate (CSR)
+    pbat_vbd::DeviceArray<int> meshAggBegin;          // device copy of aggBegin
+    pbat_vbd::DeviceArray<long long> meshCoarseOffset; // device copy of coarseOffset
     std::vector<float> wgHost;   // quadrature weights (traces only)
     std::vector<float> lameHost; // (mu, lambda) per element (traces only)
 
@@ -243,6 +257,8 @@ struct pbat_vbd_integrator
         xm.reset(), xtil.reset(), xt.reset(), vel.reset(), vprev.reset(), aext.reset();
         tgp.reset(), tetv.reset(), rows.reset(), ell.reset(), vtp.reset(), energy.reset();
         slotsRows.reset(), slotsEll.reset(), matBase.reset(), meshOf.reset(), xkm1.reset(), xkm2.reset();
+        aggOf.reset(), aggMesh.reset(), meshAggBegin.reset(), meshCoarseOffset.reset();
+        aggVertPtr.reset(), aggVerts.reset();
         gkm1.reset(), fkm1.reset(), dgk.reset(), dfk.reset(), andersonAcc.reset(), andersonGram.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)

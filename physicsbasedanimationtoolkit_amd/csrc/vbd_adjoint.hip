@@ -118,6 +118,63 @@ void EnsureBuffers(pbat_vbd_integrator& h)
         R.grad.allocate(static_cast<size_t>(h.nT));
 }
 
+device::CoarseArgs MakeCoarseArgs(pbat_vbd_integrator& h)
+{
+    device::CoarseArgs c{};
+    c.aggOf = h.aggOf.ptr;
+    c.aggMesh = h.aggMesh.ptr;
+    c.meshAggBegin = h.meshAggBegin.ptr;
+    c.meshOffset = h.meshCoarseOffset.ptr;
+    c.nAgg = h.nAggregates;
+    return c;
+}
+
+// Assemble the coarse operator Z^T H Z at xs (positions with masses) and
+// invert each mesh's block in place.
+void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
+{
+    RolloutState& R = h.rollout;
+    size_t const total = static_cast<size_t>(h.coarseOffset.back());
+    int const nM = h.nMeshes;
+    if (R.coarse[0].n != total)
+    {
+        R.coarse[0].allocate(total);
+        R.coarse[1].allocate(total);
+    }
+    if (R.rc.n != 3 * static_cast<size_t>(h.nAggregates))
+    {
+        R.rc.allocate(3 * static_cast<size_t>(h.nAggregates));
+        R.zc.allocate(3 * static_cast<size_t>(h.nAggregates));
+    }
+    if (total == 0 || h.nAggregates == 0)
+        return;
+    device::CoarseArgs const c = MakeCoarseArgs(h);
+    hipLaunchKernelGGL(device::ZeroDoublesLong, dim3(1024), dim3(256), 0, h.stream, R.coarse[0].ptr,
+                       static_cast<long long>(total));
+    hipLaunchKernelGGL(device::ZeroDoublesLong, dim3(64), dim3(256), 0, h.stream, R.rc.ptr,
+                       static_cast<long long>(R.rc.n));
+    int const nT = static_cast<int>(h.nT), nUpd = static_cast<int>(h.nUpdated);
+    hipLaunchKernelGGL(device::CoarseAssembleElastic, dim3(Blocks(nT)), dim3(256), 0, h.stream, xs, h.tetv.ptr,
+                       h.tgp.ptr, nT, c, R.coarse[0].ptr);
+    hipLaunchKernelGGL(device::CoarseAssembleMass, dim3(Blocks(nUpd)), dim3(256), 0, h.stream, xs, nUpd, c,
+                       1. / (double(dt) * double(dt)), R.coarse[0].ptr);
+    int maxN = 0;
+    std::vector<int> finalBuf(static_cast<size_t>(nM));
+    for (int m = 0; m < nM; ++m)
+    {
+        int const n = 3 * (h.aggBegin[m + 1] - h.aggBegin[m]);
+        maxN = std::max(maxN, n);
+        finalBuf[m] = n % 2;
+    }
+    int64_t const maxEntries = int64_t(maxN) * maxN;
+    dim3 const grid(static_cast<unsigned>((maxEntries + 255) / 256), static_cast<unsigned>(nM));
+    for (int k = 0; k < maxN; ++k)
+        hipLaunchKernelGGL(device::CoarseInvertStep, grid, dim3(256), 0, h.stream,
+                           static_cast<double const*>(R.coarse[k & 1].ptr), R.coarse[(k + 1) & 1].ptr, k, c);
+    R.coarseFinal.upload(finalBuf, h.stream);
+    PBAT_HIP(hipGetLastError());
+}
+
 void Zero4(pbat_vbd_integrator& h, float4* p, int n)
 {
     if (n > 0)
@@ -224,18 +281,35 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     cg.tol2 = static_cast<double>(cfg.tolerance) * cfg.tolerance;
     int const check = std::max(1, cfg.check_every);
     int const gridN = Blocks(n);
-    bool const jacobi = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI;
+    bool const twoLevel = cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL && h.nAggregates > 0;
+    bool const jacobi = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI || cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL;
     float4 const* dinv = jacobi ? R.dinv.ptr : nullptr;
+    device::CoarseArgs const cargs = MakeCoarseArgs(h);
+    double* const rc = twoLevel ? R.rc.ptr : nullptr;
+    // two-level: rc = Z^T r, zc = (Z^T H Z)^-1 rc, z += Z zc with r.z
+    auto coarse = [&] {
+        int const rows = 3 * h.nAggregates;
+        hipLaunchKernelGGL(device::CoarseRestrict, dim3((h.nAggregates * 64 + 255) / 256), dim3(256), 0, h.stream,
+                           static_cast<float4 const*>(R.r.ptr), static_cast<int const*>(h.aggVertPtr.ptr),
+                           static_cast<int const*>(h.aggVerts.ptr), h.nAggregates, R.rc.ptr);
+        hipLaunchKernelGGL(device::CoarseApply, dim3((rows * 64 + 255) / 256), dim3(256), 0, h.stream,
+                           static_cast<double const*>(R.coarse[0].ptr), static_cast<double const*>(R.coarse[1].ptr),
+                           static_cast<int const*>(R.coarseFinal.ptr), static_cast<double const*>(R.rc.ptr),
+                           R.zc.ptr, cargs);
+        hipLaunchKernelGGL(device::CoarseProlong, dim3(gridN), dim3(256), 0, h.stream, R.z.ptr,
+                           static_cast<float4 const*>(R.r.ptr), n, cargs, static_cast<double const*>(R.zc.ptr),
+                           static_cast<int const*>(h.meshOf.ptr), nM, cg.A);
+    };
     auto zeroScalars = [&] {
         int const nd = static_cast<int>(ScalarDoubles(h));
         hipLaunchKernelGGL(device::ZeroScalars, dim3(std::min(1024, Blocks(nd + 2 * nM + 1))), dim3(256), 0,
                            h.stream, cg.P, nd, cg.F, 2 * nM + 1);
     };
     // r = b - H y with per-mesh |b|^2, |r|^2 (and r . D^-1 r with dinv)
-    auto residual = [&](float4 const* pre) {
+    auto residual = [&](float4 const* pre, double* restrictTo) {
         VertexPass<AdjMode::kHessVec>(h, xs, dt, y, nullptr, R.q.ptr, nullptr, 0, 0, n);
         hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr,
-                           R.z.ptr, pre, h.meshOf.ptr, n, nM, cg.A);
+                           R.z.ptr, pre, h.meshOf.ptr, n, nM, cg.A, static_cast<int const*>(h.aggOf.ptr), restrictTo);
     };
     zeroScalars();
     Zero4(h, R.z.ptr, nV);
@@ -249,7 +323,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
                 GaussSeidelPass(h, xs, dt, y, b, false);
             it += chunk;
             zeroScalars();
-            residual(nullptr);
+            residual(nullptr, nullptr);
             hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
             if (ReadLive(h) == 0)
                 break;
@@ -261,7 +335,9 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     }
     if (jacobi)
         VertexPass<AdjMode::kDiagInverse>(h, xs, dt, y, nullptr, R.dinv.ptr, nullptr, 0, 0, n);
-    residual(dinv);
+    residual(dinv, rc);
+    if (twoLevel)
+        coarse();
     if (!jacobi)
     {
         GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
@@ -273,7 +349,9 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, cg.A, device::kAccPq, 0, n);
         hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
                            static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr), dinv,
-                           h.meshOf.ptr, n, cg);
+                           h.meshOf.ptr, n, cg, static_cast<int const*>(h.aggOf.ptr), rc);
+        if (twoLevel)
+            coarse();
         if (!jacobi)
         {
             GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
@@ -326,7 +404,7 @@ void CheckConfig(pbat_vbd_adjoint_config const& cfg)
 {
     if (cfg.max_iterations < 1 || !(cfg.tolerance >= 0.f) ||
         (cfg.solver != PBAT_VBD_ADJOINT_PCG && cfg.solver != PBAT_VBD_ADJOINT_GAUSS_SEIDEL &&
-         cfg.solver != PBAT_VBD_ADJOINT_PCG_JACOBI))
+         cfg.solver != PBAT_VBD_ADJOINT_PCG_JACOBI && cfg.solver != PBAT_VBD_ADJOINT_PCG_TWO_LEVEL))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid adjoint solver configuration");
 }
 
@@ -419,6 +497,8 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
     for (auto& s : R.segX)
         if (s.n != n)
             s.allocate(n);
+    if (cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL)
+        BuildCoarse(h, h.xm.ptr, dt); // at x_T; reused by every step's solve
     pbat_vbd_adjoint_stats st{0, 0, 0, 0.f};
     int const nSeg = static_cast<int>(R.ckptX.size());
     for (int sgm = nSeg - 1; sgm >= 0; --sgm)
@@ -534,6 +614,8 @@ void AdjointSolve(pbat_vbd_integrator& h, float const* x, float dt, float const*
     if (nV > nUpd)
         Zero4(h, R.rhs.ptr + nUpd, nV - nUpd);
     Zero4(h, R.y.ptr, nV);
+    if (cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL)
+        BuildCoarse(h, R.hold.ptr, dt);
     SolveResult const sr = Solve(h, R.hold.ptr, dt, R.y.ptr, R.rhs.ptr, cfg);
     DownloadVec(h, R.y.ptr, y);
     if (stats)

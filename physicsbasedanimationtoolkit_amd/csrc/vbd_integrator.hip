@@ -392,6 +392,103 @@ void DownloadVec(pbat_vbd_integrator& h, float4 const* src, float* dst)
     }
 }
 
+// Aggregates of the adjoint's two-level preconditioner: each mesh's updated
+// vertices binned by rest position into cubes of four mean edge lengths
+// (enlarged so that a mesh has at most kMaxAggregatesPerMesh of them).
+constexpr int kMaxAggregatesPerMesh = 1024;
+constexpr double kAggregateEdges = 4.;
+
+void BuildAggregates(pbat_vbd_integrator& h, pbat_vbd_data const* d, std::vector<int> const& meshOfOrig)
+{
+    int const nM = h.nMeshes;
+    int64_t const nV = h.nV, nT = h.nT, nUpd = h.nUpdated;
+    std::vector<double> lo(3 * nM, 1e300), hi(3 * nM, -1e300), edgeSum(nM, 0.);
+    std::vector<int64_t> edgeCount(nM, 0), count(nM, 0);
+    for (int64_t i = 0; i < nV; ++i)
+    {
+        int const m = meshOfOrig[i];
+        for (int k = 0; k < 3; ++k)
+        {
+            lo[3 * m + k] = std::min(lo[3 * m + k], d->x[3 * i + k]);
+            hi[3 * m + k] = std::max(hi[3 * m + k], d->x[3 * i + k]);
+        }
+    }
+    for (int64_t e = 0; e < nT; ++e)
+    {
+        int const m = meshOfOrig[d->E[4 * e]];
+        for (int a = 0; a < 4; ++a)
+            for (int b = a + 1; b < 4; ++b)
+            {
+                double const* p = d->x + 3 * d->E[4 * e + a];
+                double const* q = d->x + 3 * d->E[4 * e + b];
+                edgeSum[m] += std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
+                                        (p[2] - q[2]) * (p[2] - q[2]));
+                ++edgeCount[m];
+            }
+    }
+    std::vector<double> cell(nM);
+    for (int m = 0; m < nM; ++m)
+    {
+        double const mean = edgeCount[m] ? edgeSum[m] / double(edgeCount[m]) : 1.;
+        double vol = 1.;
+        for (int k = 0; k < 3; ++k)
+            vol *= std::max(hi[3 * m + k] - lo[3 * m + k], mean);
+        cell[m] = std::max(kAggregateEdges * mean, std::cbrt(vol / kMaxAggregatesPerMesh));
+    }
+    // bin updated vertices (internal order), ids contiguous per mesh
+    std::vector<std::vector<std::pair<std::array<int64_t, 3>, int>>> keys(nM);
+    for (int64_t v = 0; v < nUpd; ++v)
+    {
+        int64_t const i = h.perm[v];
+        int const m = meshOfOrig[i];
+        std::array<int64_t, 3> k;
+        for (int a = 0; a < 3; ++a)
+            k[a] = static_cast<int64_t>(std::floor((d->x[3 * i + a] - lo[3 * m + a]) / cell[m]));
+        keys[m].push_back({k, static_cast<int>(v)});
+    }
+    std::vector<int> aggOf(static_cast<size_t>(nV), -1), aggMesh;
+    h.aggBegin.assign(static_cast<size_t>(nM) + 1, 0);
+    h.coarseOffset.assign(static_cast<size_t>(nM) + 1, 0);
+    int next = 0;
+    for (int m = 0; m < nM; ++m)
+    {
+        auto& km = keys[m];
+        std::sort(km.begin(), km.end());
+        h.aggBegin[m] = next;
+        for (size_t j = 0; j < km.size(); ++j)
+        {
+            if (j == 0 || km[j].first != km[j - 1].first)
+            {
+                aggMesh.push_back(m);
+                ++next;
+            }
+            aggOf[km[j].second] = next - 1;
+        }
+        int64_t const nc = 3 * int64_t(next - h.aggBegin[m]);
+        h.coarseOffset[m + 1] = h.coarseOffset[m] + nc * nc;
+    }
+    h.aggBegin[nM] = next;
+    h.nAggregates = next;
+    std::vector<int> ptr(static_cast<size_t>(next) + 1, 0), verts(static_cast<size_t>(nUpd));
+    for (int64_t v = 0; v < nUpd; ++v)
+        ++ptr[aggOf[v] + 1];
+    for (int g = 0; g < next; ++g)
+        ptr[g + 1] += ptr[g];
+    {
+        std::vector<int> fillp(ptr.begin(), ptr.end() - 1);
+        for (int64_t v = 0; v < nUpd; ++v)
+            verts[fillp[aggOf[v]]++] = static_cast<int>(v);
+    }
+    h.aggVertPtr.upload(ptr, h.stream);
+    h.aggVerts.upload(verts, h.stream);
+    std::vector<long long> off(h.coarseOffset.begin(), h.coarseOffset.end());
+    h.aggOf.upload(aggOf, h.stream);
+    h.aggMesh.upload(aggMesh, h.stream);
+    h.meshAggBegin.upload(h.aggBegin, h.stream);
+    h.meshCoarseOffset.upload(off, h.stream);
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+}
+
 // vertexOffsets (batches): first vertex of each mesh in d's numbering, count + 1
 // entries; vertices of one colour are then grouped by mesh.
 void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64_t> const* vertexOffsets = nullptr)
@@ -504,6 +601,8 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         h->meshOf.upload(meshOf, h->stream);
         PBAT_HIP(hipStreamSynchronize(h->stream));
     }
+
+    BuildAggregates(*h, d, meshOfOrig);
 
     // Validate the colouring: no element may hold two vertices of one colour.
     {

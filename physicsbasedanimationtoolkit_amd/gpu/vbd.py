@@ -273,7 +273,7 @@ class Integrator:
     def adjoint_config(solver: str = "pcg", max_iterations: int = 200, tolerance: float = 1e-6,
                        check_every: int = 5) -> _native.AdjointConfig:
         kinds = {"pcg": _native.ADJOINT_PCG, "gauss_seidel": _native.ADJOINT_GAUSS_SEIDEL,
-                 "pcg_jacobi": _native.ADJOINT_PCG_JACOBI}
+                 "pcg_jacobi": _native.ADJOINT_PCG_JACOBI, "pcg_two_level": _native.ADJOINT_PCG_TWO_LEVEL}
         if solver not in kinds:
             raise ValueError(f"solver must be one of {sorted(kinds)}")
         return _native.AdjointConfig(kinds[solver], int(max_iterations), float(tolerance), int(check_every))

@@ -77,7 +77,8 @@ def test_hessian_vector_matches_fp64(gpu_device):
     assert np.all(hv[:, ~free] == 0)
 
 
-@pytest.mark.parametrize("solver,iters", [("pcg", 300), ("pcg_jacobi", 600), ("gauss_seidel", 20000)])
+@pytest.mark.parametrize("solver,iters", [("pcg", 300), ("pcg_jacobi", 600), ("pcg_two_level", 600),
+                                          ("gauss_seidel", 20000)])
 def test_adjoint_solve_matches_fp64(gpu_device, solver, iters):
     X, E, mu, lam, v0 = diff_scene(6)
     g = make_gpu(X, E, mu, lam, v0)
@@ -109,7 +110,7 @@ def gpu_trajectory(g, x0, v0, steps, iterations):
     return xs
 
 
-@pytest.mark.parametrize("solver", ["pcg", "pcg_jacobi", "gauss_seidel"])
+@pytest.mark.parametrize("solver", ["pcg", "pcg_jacobi", "pcg_two_level", "gauss_seidel"])
 def test_rollout_gradient_matches_fp64_ift(gpu_device, solver):
     X, E, mu, lam, v0 = diff_scene(8, seed=2)
     steps, iters = 5, 20
@@ -155,7 +156,7 @@ def test_differentiable_workload_gradient_matches_fp64(gpu_device):
     g.x = s.X
     g.v = v0
     g.rollout_forward(s.dt, s.iterations, s.steps, 10)
-    r = g.rollout_backward(dLdx, None, solver="pcg_jacobi", tolerance=1e-6, max_iterations=3000)
+    r = g.rollout_backward(dLdx, None, solver="pcg_two_level", tolerance=1e-6, max_iterations=3000)
     o = O.OracleVbd(s.X, s.E, v=s.v0, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
                     nthreads=8)
     A.set_log_young_modulus(o, s.log_y_guess)
@@ -277,7 +278,7 @@ def test_torch_autograd_function(gpu_device):
     np.testing.assert_allclose(x0.grad.numpy(), r["x0"], rtol=1e-6, atol=1e-9)
 
 
-@pytest.mark.parametrize("solver", ["pcg", "pcg_jacobi"])
+@pytest.mark.parametrize("solver", ["pcg", "pcg_jacobi", "pcg_two_level"])
 def test_batched_gradients_match_per_mesh(gpu_device, solver):
     """Each mesh of a batch runs its own CG, so the batch reproduces the per-mesh
     loop up to rounding."""
