@@ -172,6 +172,9 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
         hipLaunchKernelGGL(device::CoarseInvertStep, grid, dim3(256), 0, h.stream,
                            static_cast<double const*>(R.coarse[k & 1].ptr), R.coarse[(k + 1) & 1].ptr, k, c);
     R.coarseFinal.upload(finalBuf, h.stream);
+    PBAT_HIP(hipStreamSynchronize(h.stream)); // finalBuf is a host temporary
+    hipLaunchKernelGGL(device::CoarseGuard, dim3(nM), dim3(256), 0, h.stream, R.coarse[0].ptr, R.coarse[1].ptr,
+                       static_cast<int const*>(R.coarseFinal.ptr), c);
     PBAT_HIP(hipGetLastError());
 }
 

@@ -795,6 +795,32 @@ __global__ __launch_bounds__(256) void CoarseInvertStep(double const* __restrict
     Bk[c.meshOffset[m] + idx] = v;
 }
 
+// Guard after inversion: a mesh block whose inverse has a non-positive or
+// non-finite diagonal entry (an indefinite coarse operator) is zeroed, which
+// leaves that mesh with the block-Jacobi preconditioner. One workgroup per mesh.
+__global__ __launch_bounds__(256) void CoarseGuard(double* __restrict__ buf0, double* __restrict__ buf1,
+                                                   int const* __restrict__ finalBuf, CoarseArgs c)
+{
+    __shared__ int bad;
+    int const m = static_cast<int>(blockIdx.x);
+    int const n = 3 * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
+    double* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m];
+    if (threadIdx.x == 0)
+        bad = 0;
+    __syncthreads();
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += static_cast<int>(blockDim.x))
+    {
+        double const d = A[static_cast<size_t>(i) * n + i];
+        if (!(d > 0.) || !isfinite(d))
+            bad = 1;
+    }
+    __syncthreads();
+    if (!bad)
+        return;
+    for (int64_t k = threadIdx.x; k < static_cast<int64_t>(n) * n; k += blockDim.x)
+        A[k] = 0.;
+}
+
 // zc = (Z^T H Z)^-1 rc per mesh; one wave per coarse row, blockDim = 256.
 __global__ __launch_bounds__(256) void CoarseApply(double const* __restrict__ buf0, double const* __restrict__ buf1,
                                                    int const* __restrict__ finalBuf, double const* __restrict__ rc,
