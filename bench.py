@@ -391,9 +391,16 @@ def main():
         t0 = time.perf_counter()
         ref.step(dt, iters, 1)
         cpu_s = time.perf_counter() - t0
+        model = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+        except OSError:
+            pass
         cpu = {"value": nUpd * iters / cpu_s, "unit": "vertex-updates/s", "cores": threads, "kind": "port",
                "sample": f"1 step x {iters} iterations of the same 64^3 workload, FP64 oracle, "
-                         f"{threads} threads, {cpu_s:.2f}s"}
+                         f"{threads} threads, {cpu_s:.2f}s",
+               "host": {"cpu_model": model, "os_cpu_count": os.cpu_count()}}
         log(f"[bench] cpu baseline {cpu['value']:.4e} vu/s on {threads} threads")
 
     out = {
