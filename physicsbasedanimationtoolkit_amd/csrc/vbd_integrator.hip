@@ -4,10 +4,12 @@
 // with a different device design:
 //  - vertices are renumbered colour-major, so each colour is one contiguous
 //    range and a sweep over it reads per-vertex data coalesced; within a colour
-//    vertices are sorted by degree (stable) so lanes of a wave do equal work;
-//  - each vertex-tet incidence is a 64-byte record stored in sweep order: the
+//    vertices are grouped by mesh (batches) and sorted by degree (stable) so
+//    lanes of a wave do equal work;
+//  - each vertex-tet incidence is a 56-byte record stored in sweep order: the
 //    tet's other three vertex ids, their three shape-function-gradient rows
-//    and the tet's w*mu, w*lambda, alpha. A colour pass streams its records
+//    and the tet's w*mu, w*lambda (alpha = 1 + mu/lambda is derived in the
+//    kernel). A colour pass streams its records
 //    (instead of gathering per-tet data that straddles cache lines), and only
 //    the neighbour positions are gathered, one dependent load after the record;
 //  - positions carry the lumped mass in their 4th component, so one 16-byte
@@ -641,8 +643,8 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     }
     // Incidence records in internal numbering and sweep order: for each
     // incidence the tet's other three vertices (local order) with their GP
-    // rows, and the tet's w*mu, w*lambda, alpha. Only updated vertices are
-    // swept, so only their rows are stored.
+    // rows, and the tet's w*mu (in the id word) and w*lambda. Only updated
+    // vertices are swept, so only their rows are stored.
     std::vector<int> P(static_cast<size_t>(nV) + 1, 0);
     for (int64_t v = 0; v < nV; ++v)
     {
