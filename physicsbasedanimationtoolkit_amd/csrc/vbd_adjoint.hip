@@ -48,6 +48,8 @@ device::AdjointArgs MakeAdjointArgs(pbat_vbd_integrator& h, float4 const* xs, fl
 template <int L, int K, AdjMode M>
 void LaunchAdjointLK(pbat_vbd_integrator& h, device::AdjointArgs a)
 {
+    if (a.vend <= a.vbeg)
+        return;  // an empty colour (or no updated vertex at all)
     int const block = h.blockSize;
     int64_t const threads = static_cast<int64_t>(a.vend - a.vbeg) * L;
     int64_t grid64 = (threads + block - 1) / block;
