@@ -56,11 +56,14 @@ extern "C" {
 
 /* sim/vbd/Enums.h:21-28 EAccelerationStrategy; gpu::vbd::Integrator's factory
  * (gpu/vbd/Integrator.cu:15-33) picks the integrator from it. None, Chebyshev
- * (gpu/impl/vbd/ChebyshevIntegrator.cu) and Anderson
- * (gpu/impl/vbd/AndersonIntegrator.cu; single meshes, not batches) are available. */
+ * (gpu/impl/vbd/ChebyshevIntegrator.cu), Anderson (gpu/impl/vbd/AndersonIntegrator.cu)
+ * and trust region (gpu/impl/vbd/TrustRegionIntegrator.cu) are available; Anderson
+ * and trust region couple every vertex of a handle and take single meshes, not
+ * batches. */
 #define PBAT_VBD_ACCELERATION_NONE 0
 #define PBAT_VBD_ACCELERATION_CHEBYSHEV 1
 #define PBAT_VBD_ACCELERATION_ANDERSON 2
+#define PBAT_VBD_ACCELERATION_TRUST_REGION 3
 #define PBAT_VBD_MAX_ANDERSON_WINDOW 16
 
 /* Message of the last failure on the calling thread ("" if none). */
@@ -126,6 +129,9 @@ typedef struct pbat_vbd_data {
     int32_t acceleration;    /* PBAT_VBD_ACCELERATION_* (sim/vbd/Enums.h:21-28) */
     double rho;              /* Chebyshev estimated spectral radius, 0 < rho < 1 (sim/vbd/Data.h:233-234) */
     int32_t window;          /* Anderson window size, 1..PBAT_VBD_MAX_ANDERSON_WINDOW (sim/vbd/Data.h:236-237) */
+    double eta;              /* trust region: energy reduction accuracy threshold, >= 0 (sim/vbd/Data.h:244) */
+    double tau;              /* trust region: radius increase factor, > 1 (sim/vbd/Data.h:245) */
+    int32_t curved;          /* trust region: curved (1) or linear (0) accelerated path (sim/vbd/Data.h:246) */
 } pbat_vbd_data;
 
 typedef struct pbat_vbd_integrator pbat_vbd_integrator;
@@ -292,6 +298,15 @@ int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);
 /* Number of colours (vertex partitions) and the number of vertices updated per sweep. */
 int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPartitions, int64_t* nUpdated,
                                int64_t* nIncidences);
+
+/* Decisions of the linear trust-region path in the last Step, one entry per
+ * iteration k >= 2 of each substep, in order: the step length t along the VBD
+ * step and whether the step was kept. The structured form of the reference's
+ * profiling log lines "t={},t*={}" and "Accepted/Rejected TR step"
+ * (gpu/impl/vbd/TrustRegionIntegrator.cu:111,133,146). Writes at most
+ * capacity entries; *count receives the number recorded. */
+int pbat_vbd_integrator_trust_region_log(pbat_vbd_integrator* h, double* t, int32_t* accepted, int64_t capacity,
+                                         int64_t* count);
 
 /* Kernel-time instrumentation: when enabled, Step launches eagerly and each
  * substep's back-to-back sweep launches are bracketed by one pair of HIP

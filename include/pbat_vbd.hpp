@@ -82,6 +82,9 @@ struct Data
     int32_t acceleration = PBAT_VBD_ACCELERATION_NONE; // sim/vbd/Data.h:231
     double rho = 1.;                                   // sim/vbd/Data.h:234
     int32_t window = 5;                                // sim/vbd/Data.h:237
+    double eta = 0.2;                                  // sim/vbd/Data.h:244
+    double tau = 2.;                                   // sim/vbd/Data.h:245
+    bool curved = true;                                // sim/vbd/Data.h:246
     std::vector<double> mue, lambdae;
 
     template <class TX, class TE>
@@ -145,6 +148,15 @@ struct Data
     {
         window = windowIn;
         acceleration = PBAT_VBD_ACCELERATION_ANDERSON;
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:170-177 */
+    Data& WithTrustRegionAcceleration(double etaIn, double tauIn, bool curvedIn = true)
+    {
+        eta = etaIn;
+        tau = tauIn;
+        curved = curvedIn;
+        acceleration = PBAT_VBD_ACCELERATION_TRUST_REGION;
         return *this;
     }
     Data& WithHessianDeterminantZeroUnder(double zero)
@@ -235,6 +247,9 @@ class Integrator
         d.acceleration = data.acceleration;
         d.rho = data.rho;
         d.window = data.window;
+        d.eta = data.eta;
+        d.tau = data.tau;
+        d.curved = data.curved ? 1 : 0;
         Check(pbat_vbd_integrator_create(&d, &mImpl));
         mVertices = d.nV;
     }

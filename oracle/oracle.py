@@ -343,6 +343,119 @@ class OracleVbd:
                 self.x.reshape(-1)[:] = Gk - DGK[:, :mk] @ alpha
             L.ora_substep_end(ctypes.byref(s), sdt)
 
+    def step_trust_region(self, dt: float, iterations: int, substeps: int = 1, eta: float = 0.2,
+                          tau: float = 2.0, curved: bool = True, log=None, forced=None) -> None:
+        """TrustRegionIntegrator::Solve (gpu/impl/vbd/TrustRegionIntegrator.cu:36-46) in FP64
+        around the same substep pieces as step(); f is ObjectiveFunction (:265-364, no
+        contact term: single-body scenes have no active contacts).
+
+        Linear path (:48-165): iterations 0 and 1 are plain VBD iterations; from k = 2 a
+        quadratic model m(t) through the last three (t, f) pairs, time shifted so the
+        current iterate sits at t = 0 (ConstructModel, :452-468), is minimised
+        (ModelOptimalStep, :400-430) over the line x(t) = x_k + t (x_VBD - x_k) with t
+        clamped to [1, R / |x_VBD - x_k|]; the step (:491-506) is kept when
+        (f_k - f(x(t))) / (f_k - m(t)) > eta, else rolled back (:508-522) to x_VBD.
+        Curved path (:167-263): its constraint solve returns 0 (:700-703), so every
+        clamped step sits at the lower bound and each iteration is the fallback VBD
+        iteration (:240-251); positions are those of plain VBD.
+
+        log, if a list, receives one (k, t, accepted, ratio) tuple per iteration k >= 2
+        (ratio: the actual over the model's decrease, None when no step was tried).
+        forced, if given, is a sequence of (t, accepted) pairs, one per iteration
+        k >= 2 of each substep, that replaces the path's own choices (the log still
+        records the oracle's own t and decision at that state)."""
+        L = lib()
+        for f in ("ora_substep_begin", "ora_sweep", "ora_substep_end"):
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_double]
+            getattr(L, f).restype = None
+        sdt = float(dt) / substeps
+        flt_min = float(np.finfo(np.float32).tiny)
+        flt_max = float(np.finfo(np.float32).max)
+        small = float(np.float32(1e-6))  # the linear path's 1e-6f tolerance
+
+        def objective():
+            return self.objective(self.x.T, self.xtilde.T, sdt)
+
+        n_forced = 0
+        for _ in range(substeps):
+            s = self._state()
+            L.ora_substep_begin(ctypes.byref(s), sdt)
+            fk, tk = objective(), -1.0
+            fkm1 = fkm2 = tkm1 = tkm2 = 0.0
+            R2 = 0.0 if not curved else float(np.finfo(np.float64).eps)
+            for k in range(iterations):
+                if k < 2 or curved:
+                    if k >= 2:
+                        # curved: SolveCurvedTrustRegionConstraint returns 0 and
+                        # clamp(t*, FLT_MIN, 0) is at the lower bound either way
+                        R2 = max(R2 / (tau * tau), float(np.finfo(np.float64).eps))
+                        if log is not None:
+                            log.append((k, 0.0, False, None))
+                    fkm2, fkm1, tkm2, tkm1 = fkm1, fk, tkm1, tk
+                    xkm1 = self.x.copy()
+                    L.ora_sweep(ctypes.byref(s), sdt)
+                    fk = objective()
+                    tk = (tkm1 + 1.0) if k >= 2 else float(k)
+                    continue
+                # ConstructModel (linear): shift time so t_k = 0, fit a t^2 + b t + c
+                tkm2, tkm1, tk = tkm2 - tk, tkm1 - tk, 0.0
+                Q = np.array([[tkm2 * tkm2, tkm2, 1.0], [tkm1 * tkm1, tkm1, 1.0], [0.0, 0.0, 1.0]])
+                aQ = np.linalg.solve(Q, np.array([fkm2, fkm1, fk]))
+                # UpdateIterates
+                fkm2, fkm1, tkm2, tkm1 = fkm1, fk, tkm1, tk
+                xkm1 = self.x.copy()
+                L.ora_sweep(ctypes.byref(s), sdt)
+                dx = self.x - xkm1
+                dx2 = float((dx * dx).sum())
+                if R2 < dx2 + small:
+                    R2 = tau * tau * dx2
+                lower = 1.0
+                upper = np.sqrt(R2 / dx2) if dx2 > 0 else float("nan")
+                # ModelOptimalStep
+                if abs(aQ[0]) > flt_min:
+                    tstar = -aQ[1] / (2.0 * aQ[0]) if aQ[0] > flt_min else flt_max
+                elif abs(aQ[1]) > flt_min:
+                    tstar = -flt_max if aQ[1] > 0 else flt_max
+                else:
+                    tstar = 0.0
+                t = lower if tstar < lower else (upper if upper < tstar else tstar)
+                x_vbd = self.x.copy()
+
+                def try_step(tt):
+                    # TakeLinearStep, then the actual over the model's decrease
+                    self.x[:] = xkm1 + tt * dx
+                    f_try = objective()
+                    m_try = aQ[0] * tt * tt + aQ[1] * tt + aQ[2]
+                    return f_try, float((fkm1 - f_try) / (fkm1 - m_try))
+
+                attempt = not (abs(t - lower) < small)
+                accepted, ratio = False, None
+                if attempt:
+                    fk, ratio = try_step(t)
+                    accepted = ratio > eta
+                own = (t, accepted, ratio)
+                if forced is not None:
+                    t, accepted = forced[n_forced]
+                    accepted = bool(accepted)
+                    n_forced += 1
+                    self.x[:] = x_vbd
+                    attempt = not (abs(t - lower) < small)
+                    if attempt:
+                        fk, _ = try_step(t)
+                if accepted:
+                    if (upper - t) < small:
+                        R2 *= tau * tau
+                    tk = t
+                else:
+                    R2 /= tau * tau
+                    if attempt:
+                        self.x[:] = xkm1 + (self.x - xkm1) / t
+                    fk = objective()
+                    tk = tkm1 + 1.0
+                if log is not None:
+                    log.append((k, float(own[0]), bool(own[1]), own[2]))
+            L.ora_substep_end(ctypes.byref(s), sdt)
+
     def objective(self, x, xtilde, dt) -> float:
         s = self._state()
         xv = np.ascontiguousarray(np.asarray(x, np.float64).T)

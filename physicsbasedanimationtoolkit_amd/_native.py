@@ -22,6 +22,11 @@ ERROR_NONFINITE = -6
 
 MAX_COLORS = 128
 
+ACCELERATION_NONE = 0
+ACCELERATION_CHEBYSHEV = 1
+ACCELERATION_ANDERSON = 2
+ACCELERATION_TRUST_REGION = 3
+
 _lib = None
 
 c_i64 = ctypes.c_int64
@@ -44,6 +49,7 @@ class Data(ctypes.Structure):
         ("nPartitions", c_i64), ("Pptr", p_i64), ("Padj", p_i64),
         ("strategy", c_i32), ("kD", c_f64), ("detHZero", c_f64),
         ("acceleration", c_i32), ("rho", c_f64), ("window", c_i32),
+        ("eta", c_f64), ("tau", c_f64), ("curved", c_i32),
     ]
 
 
@@ -109,6 +115,8 @@ SIGNATURES = [
     ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("pbat_vbd_device_memory", ctypes.c_int, [p_i64, p_i64, c_i32]),
     ("pbat_vbd_integrator_counts", ctypes.c_int, [ctypes.c_void_p, p_i64, p_i64, p_i64]),
+    ("pbat_vbd_integrator_trust_region_log", ctypes.c_int,
+     [ctypes.c_void_p, p_f64, ctypes.POINTER(c_i32), c_i64, p_i64]),
     ("pbat_vbd_integrator_set_timing", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_sweep_timing", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(c_f64), p_i64, c_i32]),

@@ -220,6 +220,11 @@ struct pbat_vbd_integrator
     pbat_vbd::DeviceArray<float4> xkm1, xkm2;   // Chebyshev: previous two iterates; Anderson: x_{k-1}
     pbat_vbd::DeviceArray<float4> gkm1, fkm1, dgk, dfk; // Anderson: G_{k-1}, F_{k-1}, window columns
     pbat_vbd::DeviceArray<double> andersonAcc, andersonGram; // Anderson: dot products, Gram matrices
+    double eta = 0.2, tau = 2.;               // trust region: acceptance threshold, radius factor
+    bool curved = true;                       // trust region: curved (plain VBD iterates) or linear path
+    pbat_vbd::DeviceArray<double> trPartials; // trust region: per-workgroup reduction partials + sums
+    std::vector<double> trT;                  // trust region: t of each iteration k >= 2 of the last Step
+    std::vector<int> trAccepted;              // trust region: whether that step was kept
     float kD = 0.f;
     float detHZero = 1e-7f;
     int blockSize = 256;

@@ -35,6 +35,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <limits>
 #include <memory>
 #include <numeric>
 #include <sstream>
@@ -234,6 +235,147 @@ void LaunchAcceleration(pbat_vbd_integrator& h, int k, float& omega)
                        h.andersonGram.ptr + (k & 1) * gram);
 }
 
+// Sweep every colour once (one VBD iteration); returns the number of launches.
+int SweepAllColours(pbat_vbd_integrator& h, device::SweepArgs& a)
+{
+    int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
+    int launched = 0;
+    for (int c = 0; c < nColors; ++c)
+    {
+        a.vbeg = h.colorBegin[c];
+        a.vend = h.colorBegin[c + 1];
+        int const nc = a.vend - a.vbeg;
+        if (nc <= 0)
+            continue;
+        LaunchSweep(h, a, nc);
+        ++launched;
+    }
+    return launched;
+}
+
+constexpr int kObjectiveBlocks = 512;
+
+// (sum m |x - x~|^2, sum w Psi(F), sum |x - xprev|^2) at the current iterate, in
+// FP64 with a fixed reduction order, read back on the host (synchronises).
+void ObjectiveTermsOnHost(pbat_vbd_integrator& h, float4 const* xprev, double out[3])
+{
+    double* part = h.trPartials.ptr;
+    double* sums = part + 3 * kObjectiveBlocks;
+    hipLaunchKernelGGL(device::ObjectiveTerms, dim3(kObjectiveBlocks), dim3(256), 0, h.stream,
+                       static_cast<float4 const*>(h.xm.ptr), static_cast<float4 const*>(h.xtil.ptr), xprev,
+                       static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr),
+                       static_cast<int>(h.nV), static_cast<int>(h.nT), part);
+    hipLaunchKernelGGL(device::SumPartials3, dim3(1), dim3(256), 0, h.stream, static_cast<double const*>(part),
+                       kObjectiveBlocks, sums);
+    PBAT_HIP(hipMemcpyAsync(out, sums, 3 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+}
+
+// TrustRegionIntegrator::SolveWithLinearAcceleratedPath
+// (gpu/impl/vbd/TrustRegionIntegrator.cu:48-165). f = 1/2 |x - x~|_M^2 + h^2 sum w Psi
+// (:265-364 without the contact term: no active contacts on a single body). The
+// first two iterations are plain VBD iterations; from k = 2, m(t) = a t^2 + b t + c
+// through the last three (t, f) pairs, shifted so the current iterate sits at
+// t = 0 (:452-468), picks t in [1, R / |dx|] along x(t) = x_k + t dx, dx the VBD
+// step (:400-430); the step is kept if (f_k - f(x(t))) / (f_k - m(t)) > eta, else
+// rolled back to the VBD iterate; R grows or shrinks by tau. Scalars live on the host in FP64
+// (the reference keeps them in FP32); each iteration reads one or two reductions.
+// Returns the number of sweep launches.
+int TrustRegionSolve(pbat_vbd_integrator& h, device::SweepArgs& a, int iterations, float sdt)
+{
+    double const dt2 = double(sdt) * double(sdt);
+    double const small = double(1e-6f);
+    double const fltMin = double(std::numeric_limits<float>::min());
+    double const fltMax = double(std::numeric_limits<float>::max());
+    double const tau2 = h.tau * h.tau;
+    int const n = static_cast<int>(h.nV);
+    int const grid = (n + 255) / 256;
+    size_t const bytes = sizeof(float4) * static_cast<size_t>(h.nV);
+    double r[3];
+    auto objective = [&](float4 const* xprev) {
+        ObjectiveTermsOnHost(h, xprev, r);
+        return 0.5 * r[0] + dt2 * r[1];
+    };
+    auto remember = [&] {
+        PBAT_HIP(hipMemcpyAsync(h.xkm1.ptr, h.xm.ptr, bytes, hipMemcpyDeviceToDevice, h.stream));
+    };
+    int launched = 0;
+    double fk = objective(nullptr), tk = -1., fkm1 = 0., fkm2 = 0., tkm1 = 0., tkm2 = 0., R2 = 0.;
+    for (int k = 0; k < iterations; ++k)
+    {
+        if (k < 2)
+        {
+            fkm2 = fkm1, fkm1 = fk, tkm2 = tkm1, tkm1 = tk;
+            remember();
+            launched += SweepAllColours(h, a);
+            fk = objective(nullptr);
+            tk = static_cast<double>(k);
+            continue;
+        }
+        // ConstructModel: shift time so t_k = 0; c = f_k and a, b from the other two
+        // points by Cramer's rule (the times are distinct and nonzero)
+        tkm2 -= tk, tkm1 -= tk, tk = 0.;
+        double const det = tkm2 * tkm2 * tkm1 - tkm1 * tkm1 * tkm2;
+        double const r2 = fkm2 - fk, r1 = fkm1 - fk;
+        double const qa = (r2 * tkm1 - r1 * tkm2) / det;
+        double const qb = (tkm2 * tkm2 * r1 - tkm1 * tkm1 * r2) / det;
+        double const qc = fk;
+        // UpdateIterates
+        fkm2 = fkm1, fkm1 = fk, tkm2 = tkm1, tkm1 = tk;
+        remember();
+        launched += SweepAllColours(h, a);
+        double const fVbd = objective(h.xkm1.ptr);
+        double const dx2 = r[2];
+        if (R2 < dx2 + small)
+            R2 = tau2 * dx2;
+        double const lower = 1., upper = std::sqrt(R2 / dx2);
+        // ModelOptimalStep
+        double tstar = 0.;
+        if (std::abs(qa) > fltMin)
+            tstar = qa > fltMin ? -qb / (2. * qa) : fltMax;
+        else if (std::abs(qb) > fltMin)
+            tstar = qb > 0. ? -fltMax : fltMax;
+        double const t = tstar < lower ? lower : (upper < tstar ? upper : tstar);
+        bool const attempt = !(std::abs(t - lower) < small);
+        bool accepted = false;
+        if (attempt)
+        {
+            hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
+                               static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<float>(t), 0);
+            fk = objective(nullptr);
+            double const mNext = qa * t * t + qb * t + qc;
+            accepted = (fkm1 - fk) / (fkm1 - mNext) > h.eta;
+        }
+        if (accepted)
+        {
+            if ((upper - t) < small)
+                R2 *= tau2;
+            tk = t;
+        }
+        else
+        {
+            R2 /= tau2;
+            if (attempt)
+            {
+                hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
+                                   static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<float>(t), 1);
+                fk = objective(nullptr);
+            }
+            else
+                fk = fVbd;
+            tk = tkm1 + 1.;
+        }
+        h.trT.push_back(t);
+        h.trAccepted.push_back(accepted ? 1 : 0);
+    }
+    return launched;
+}
+
+bool LinearTrustRegion(pbat_vbd_integrator const& h)
+{
+    return h.acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION && !h.curved;
+}
+
 int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool eventBrackets)
 {
     float const sdt = dt / static_cast<float>(substeps);
@@ -241,7 +383,8 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
     int const n = static_cast<int>(h.nV);
     int const eb = 256, eg = (n + eb - 1) / eb;
     device::SweepArgs a = MakeSweepArgs(h, sdt, sdt2);
-    int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
+    h.trT.clear();
+    h.trAccepted.clear();
     int total = 0;
     for (int s = 0; s < substeps; ++s)
     {
@@ -254,20 +397,14 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
             PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
         float omega = 0.f;
         BeginAcceleration(h);
-        for (int k = 0; k < iterations; ++k)
-        {
-            for (int c = 0; c < nColors; ++c)
+        if (LinearTrustRegion(h))
+            launched += TrustRegionSolve(h, a, iterations, sdt);
+        else
+            for (int k = 0; k < iterations; ++k)
             {
-                a.vbeg = h.colorBegin[c];
-                a.vend = h.colorBegin[c + 1];
-                int const nc = a.vend - a.vbeg;
-                if (nc <= 0)
-                    continue;
-                LaunchSweep(h, a, nc);
-                ++launched;
+                launched += SweepAllColours(h, a);
+                LaunchAcceleration(h, k, omega);
             }
-            LaunchAcceleration(h, k, omega);
-        }
         if (eventBrackets)
         {
             PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
@@ -300,7 +437,9 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     if (iterations < 0 || substeps < 1)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "expected iterations >= 0 and substeps >= 1");
     PBAT_HIP(hipSetDevice(h.device));
-    if (!h.useGraph)
+    // The linear trust-region path decides every iteration on the host from
+    // reductions it reads back, so its steps are launched directly, not captured.
+    if (!h.useGraph || LinearTrustRegion(h))
     {
         InvalidateGraph(h);
         EnqueueStep(h, dt, iterations, substeps, h.timing);
@@ -510,9 +649,14 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     if (d->strategy < 0 || d->strategy > 4)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid initialization strategy");
     if (d->acceleration != PBAT_VBD_ACCELERATION_NONE && d->acceleration != PBAT_VBD_ACCELERATION_CHEBYSHEV &&
-        d->acceleration != PBAT_VBD_ACCELERATION_ANDERSON)
+        d->acceleration != PBAT_VBD_ACCELERATION_ANDERSON && d->acceleration != PBAT_VBD_ACCELERATION_TRUST_REGION)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
-             "only the base, Chebyshev- and Anderson-accelerated VBD integrators are available");
+             "only the base, Chebyshev-, Anderson- and trust-region-accelerated VBD integrators are available");
+    // sim/vbd/Data.cpp:294-302
+    if (d->acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION && !(d->eta >= 0.))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "Expected eta >= 0");
+    if (d->acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION && !(d->tau > 1.))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "Expected tau > 1");
     if (d->acceleration == PBAT_VBD_ACCELERATION_CHEBYSHEV && !(d->rho > 0. && d->rho < 1.))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "Expected 0 < rho < 1");
     if (d->acceleration == PBAT_VBD_ACCELERATION_ANDERSON &&
@@ -549,6 +693,8 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     h->acceleration = d->acceleration;
     h->rho = static_cast<float>(d->rho);
     h->window = d->acceleration == PBAT_VBD_ACCELERATION_ANDERSON ? d->window : 0;
+    if (d->acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION)
+        h->eta = d->eta, h->tau = d->tau, h->curved = d->curved != 0;
     h->kD = static_cast<float>(d->kD);
     h->detHZero = static_cast<float>(d->detHZero);
 
@@ -760,6 +906,11 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         h->andersonAcc.allocate(2 * static_cast<size_t>(device::AndersonAccDoubles()));
         h->andersonGram.allocate(2 * m * m);
     }
+    if (LinearTrustRegion(*h))
+    {
+        h->xkm1.allocate(static_cast<size_t>(nV));
+        h->trPartials.allocate(3 * static_cast<size_t>(kObjectiveBlocks) + 3);
+    }
     h->energy.allocate(3);
     PBAT_HIP(hipMemsetAsync(h->xtil.ptr, 0, sizeof(float4) * nV, h->stream));
     PBAT_HIP(hipMemsetAsync(h->vprev.ptr, 0, sizeof(float4) * nV, h->stream));
@@ -878,26 +1029,18 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
     float const sdt2 = sdt * sdt;
     int const n = static_cast<int>(h.nV);
     int const eb = 256, eg = (n + eb - 1) / eb;
-    int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
     device::SweepArgs a = MakeSweepArgs(h, sdt, sdt2);
     for (int s = 0; s < substeps; ++s)
     {
         hipLaunchKernelGGL(device::SubstepBegin, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xtil.ptr, h.xt.ptr,
                            h.vel.ptr, h.vprev.ptr, h.aext.ptr, n, sdt, sdt2, h.strategy);
         WriteVertexMatrix(h, h.xtil.ptr, Join(dir, name("xtilde", t, s, -1)));
-        float omega = 0.f;
-        BeginAcceleration(h);
+        // TracedSolve (gpu/impl/vbd/Integrator.cu:284-301) is not overridden by the
+        // accelerated integrators: traced iterations are plain VBD iterations.
         for (int k = 0; k < iterations; ++k)
         {
             WriteVertexMatrix(h, h.xm.ptr, Join(dir, name("x", t, s, k)));
-            for (int c = 0; c < nColors; ++c)
-            {
-                a.vbeg = h.colorBegin[c];
-                a.vend = h.colorBegin[c + 1];
-                if (a.vend > a.vbeg)
-                    LaunchSweep(h, a, a.vend - a.vbeg);
-            }
-            LaunchAcceleration(h, k, omega);
+            SweepAllColours(h, a);
         }
         WriteVertexMatrix(h, h.xm.ptr, Join(dir, name("x", t, s, iterations)));
         hipLaunchKernelGGL(device::SubstepEnd, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xt.ptr, h.vel.ptr,
@@ -1043,6 +1186,9 @@ void BuildUnion(pbat_vbd_data const* meshes, int count, UnionData& U)
         if (m.acceleration == PBAT_VBD_ACCELERATION_ANDERSON)
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
                  "Anderson acceleration couples every vertex of a handle; batches do not support it");
+        if (m.acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
+                 "trust-region acceleration couples every vertex of a handle; batches do not support it");
         nV += m.nV;
         nT += m.nT;
         nInc += m.GVGp[m.nV];
@@ -1385,6 +1531,20 @@ extern "C" int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPart
             s = P.back() - P.front();
             *nIncidences = s;
         }
+    });
+}
+
+extern "C" int pbat_vbd_integrator_trust_region_log(pbat_vbd_integrator* h, double* t, int32_t* accepted,
+                                                    int64_t capacity, int64_t* count)
+{
+    return Guard([&] {
+        auto& H = Handle(h);
+        if (capacity < 0 || (capacity > 0 && (!t || !accepted)) || !count)
+            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid trust-region log buffers");
+        int64_t const n = static_cast<int64_t>(H.trT.size());
+        for (int64_t k = 0; k < std::min(n, capacity); ++k)
+            t[k] = H.trT[k], accepted[k] = H.trAccepted[k];
+        *count = n;
     });
 }
 

@@ -638,6 +638,59 @@ __global__ __launch_bounds__(256) void AndersonWindow(float4 const* __restrict__
     xkm1[i] = make_float4(x0, x1, x2, 0.f);
 }
 
+// w Psi(F) of tet e in FP64: Stable Neo-Hookean (physics/StableNeoHookeanEnergy.h:737-755)
+// with the quadrature weight folded into the tgp record's (w mu, w lambda).
+__device__ inline double TetElasticEnergy(int4 const* __restrict__ tetv, float4 const* __restrict__ tgp,
+                                          float4 const* __restrict__ xm, int e)
+{
+    int4 const t = tetv[e];
+    float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
+    float4 const p0 = xm[t.x], p1 = xm[t.y], p2 = xm[t.z], p3 = xm[t.w];
+    double const r[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+    double const d[3][3] = {{double(p1.x) - p0.x, double(p1.y) - p0.y, double(p1.z) - p0.z},
+                            {double(p2.x) - p0.x, double(p2.y) - p0.y, double(p2.z) - p0.z},
+                            {double(p3.x) - p0.x, double(p3.y) - p0.y, double(p3.z) - p0.z}};
+    double F[3][3]; // F[row][col] = sum_k d[k][row] r[k][col]
+    double I2 = 0.;
+    for (int rr = 0; rr < 3; ++rr)
+        for (int c = 0; c < 3; ++c)
+        {
+            F[rr][c] = d[0][rr] * r[0][c] + d[1][rr] * r[1][c] + d[2][rr] * r[2][c];
+            I2 += F[rr][c] * F[rr][c];
+        }
+    double const J = F[0][0] * (F[1][1] * F[2][2] - F[2][1] * F[1][2]) -
+                     F[0][1] * (F[1][0] * F[2][2] - F[2][0] * F[1][2]) +
+                     F[0][2] * (F[1][0] * F[2][1] - F[2][0] * F[1][1]);
+    double const q = J - double(q2.w);
+    return 0.5 * double(q2.y) * (I2 - 3.) + 0.5 * double(q2.z) * q * q;
+}
+
+// Sum three doubles over a 256-thread workgroup; the result is valid in thread 0.
+__device__ inline void BlockSum3(double& a, double& b, double& c)
+{
+    __shared__ double red[3][4];
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        c += __shfl_xor(c, off);
+    }
+    int const w = static_cast<int>(threadIdx.x) / 64;
+    if ((threadIdx.x & 63) == 0)
+    {
+        red[0][w] = a;
+        red[1][w] = b;
+        red[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        c = red[2][0] + red[2][1] + red[2][2] + red[2][3];
+    }
+}
+
 // FP64 energy reduction: out[0] += 1/2 m |v|^2, out[2] += -m a.x over vertices,
 // out[1] += w Psi(F) over tets. One atomic per block and term. blockDim = 256.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void EnergyTerms(float4 const* __restrict__ xm, float4 const* __restrict__ vel,
@@ -645,7 +698,6 @@ __global__ __launch_bounds__(256) void AndersonWindow(float4 const* __restrict__
                                                    float4 const* __restrict__ tgp, int nV, int nT,
                                                    double* __restrict__ out)
 {
-    __shared__ double red[3][4];
     double K = 0., U = 0., P = 0.;
     int const stride = static_cast<int>(gridDim.x * blockDim.x);
     for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < nV; i += stride)
@@ -656,55 +708,86 @@ __global__ __launch_bounds__(256) void AndersonWindow(float4 const* __restrict__
         P -= m * (double(g.x) * x.x + double(g.y) * x.y + double(g.z) * x.z);
     }
     for (int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); e < nT; e += stride)
-    {
-        int4 const t = tetv[e];
-        float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
-        float4 const p0 = xm[t.x], p1 = xm[t.y], p2 = xm[t.z], p3 = xm[t.w];
-        double const r[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
-        double const d[3][3] = {{double(p1.x) - p0.x, double(p1.y) - p0.y, double(p1.z) - p0.z},
-                                {double(p2.x) - p0.x, double(p2.y) - p0.y, double(p2.z) - p0.z},
-                                {double(p3.x) - p0.x, double(p3.y) - p0.y, double(p3.z) - p0.z}};
-        double F[3][3]; // F[row][col] = sum_k d[k][row] r[k][col]
-        double I2 = 0.;
-        for (int rr = 0; rr < 3; ++rr)
-            for (int c = 0; c < 3; ++c)
-            {
-                F[rr][c] = d[0][rr] * r[0][c] + d[1][rr] * r[1][c] + d[2][rr] * r[2][c];
-                I2 += F[rr][c] * F[rr][c];
-            }
-        double const J = F[0][0] * (F[1][1] * F[2][2] - F[2][1] * F[1][2]) -
-                         F[0][1] * (F[1][0] * F[2][2] - F[2][0] * F[1][2]) +
-                         F[0][2] * (F[1][0] * F[2][1] - F[2][0] * F[1][1]);
-        double const q = J - double(q2.w);
-        U += 0.5 * double(q2.y) * (I2 - 3.) + 0.5 * double(q2.z) * q * q;
-    }
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        K += __shfl_xor(K, off);
-        U += __shfl_xor(U, off);
-        P += __shfl_xor(P, off);
-    }
-    int const w = static_cast<int>(threadIdx.x) / 64;
-    if ((threadIdx.x & 63) == 0)
-    {
-        red[0][w] = K;
-        red[1][w] = U;
-        red[2][w] = P;
-    }
-    __syncthreads();
+        U += TetElasticEnergy(tetv, tgp, xm, e);
+    BlockSum3(K, U, P);
     if (threadIdx.x == 0)
     {
-        double s0 = 0., s1 = 0., s2 = 0.;
-        for (int k = 0; k < 4; ++k)
-        {
-            s0 += red[0][k];
-            s1 += red[1][k];
-            s2 += red[2][k];
-        }
-        atomicAdd(out + 0, s0);
-        atomicAdd(out + 1, s1);
-        atomicAdd(out + 2, s2);
+        atomicAdd(out + 0, K);
+        atomicAdd(out + 1, U);
+        atomicAdd(out + 2, P);
     }
+}
+
+// Trust-region reductions (gpu/impl/vbd/TrustRegionIntegrator.cu:265-364 objective
+// terms without contact, :471-489 squared step): per workgroup partial sums of
+// m |x - x~|^2, w Psi(F) and, when xprev is given, |x - xprev|^2, written to
+// part[3 * blockIdx.x + j]. No atomics, so SumPartials3 gives the same bits on
+// every run. blockDim = 256.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void ObjectiveTerms(
+    float4 const* __restrict__ xm, float4 const* __restrict__ xtil, float4 const* __restrict__ xprev,
+    int4 const* __restrict__ tetv, float4 const* __restrict__ tgp, int nV, int nT, double* __restrict__ part)
+{
+    double K = 0., U = 0., D = 0.;
+    int const stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < nV; i += stride)
+    {
+        float4 const x = xm[i], y = xtil[i];
+        double const dx = double(x.x) - y.x, dy = double(x.y) - y.y, dz = double(x.z) - y.z;
+        K += double(x.w) * (dx * dx + dy * dy + dz * dz);
+        if (xprev)
+        {
+            float4 const p = xprev[i];
+            double const ex = double(x.x) - p.x, ey = double(x.y) - p.y, ez = double(x.z) - p.z;
+            D += ex * ex + ey * ey + ez * ez;
+        }
+    }
+    for (int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); e < nT; e += stride)
+        U += TetElasticEnergy(tetv, tgp, xm, e);
+    BlockSum3(K, U, D);
+    if (threadIdx.x == 0)
+    {
+        part[3 * blockIdx.x + 0] = K;
+        part[3 * blockIdx.x + 1] = U;
+        part[3 * blockIdx.x + 2] = D;
+    }
+}
+
+// out[j] = sum over b < nb of part[3 b + j], in a fixed order. One workgroup of 256.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void SumPartials3(double const* __restrict__ part, int nb,
+                                                                           double* __restrict__ out)
+{
+    double a = 0., b = 0., c = 0.;
+    for (int k = static_cast<int>(threadIdx.x); k < nb; k += static_cast<int>(blockDim.x))
+    {
+        a += part[3 * k];
+        b += part[3 * k + 1];
+        c += part[3 * k + 2];
+    }
+    BlockSum3(a, b, c);
+    if (threadIdx.x == 0)
+    {
+        out[0] = a;
+        out[1] = b;
+        out[2] = c;
+    }
+}
+
+// Trust-region step along the line through x_k (xkm1) and the VBD iterate
+// (gpu/impl/vbd/TrustRegionIntegrator.cu:491-522): x = x_k + t (x - x_k), or, with
+// rollback, x = x_k + (x - x_k) / t. Masses in xm.w are kept.
+[[maybe_unused]] static __global__ void LinearPathStep(float4* __restrict__ xm, float4 const* __restrict__ xkm1, int n,
+                                                       float t, int rollback)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 const x = xm[i], p = xkm1[i];
+    float4 r;
+    if (rollback)
+        r = make_float4(p.x + (x.x - p.x) / t, p.y + (x.y - p.y) / t, p.z + (x.z - p.z) / t, x.w);
+    else
+        r = make_float4(p.x + t * (x.x - p.x), p.y + t * (x.y - p.y), p.z + t * (x.z - p.z), x.w);
+    xm[i] = r;
 }
 
 [[maybe_unused]] static __global__ void ZeroDoubles(double* p, int n)

@@ -32,15 +32,21 @@ def _write_only(name):
     return getter
 
 
+# gpu/vbd/Integrator.cu:15-33: the integrators the device factory builds
+_ACCELERATION = {AccelerationStrategy.Base: _native.ACCELERATION_NONE,
+                 AccelerationStrategy.Chebyshev: _native.ACCELERATION_CHEBYSHEV,
+                 AccelerationStrategy.Anderson: _native.ACCELERATION_ANDERSON,
+                 AccelerationStrategy.TrustRegion: _native.ACCELERATION_TRUST_REGION}
+
+
 def _marshal(data: Data):
     """pbat_vbd_data view of a constructed Data, plus the arrays it points into."""
     if not isinstance(data, Data):
         raise TypeError("expected a pbat.sim.vbd.Data")
-    if data.accelerator not in (AccelerationStrategy.Base, AccelerationStrategy.Chebyshev,
-                                AccelerationStrategy.Anderson):
+    if data.accelerator not in _ACCELERATION:
         raise ValueError(
             f"acceleration strategy {data.accelerator.name} is not available on this device path; "
-            "only the base, Chebyshev- and Anderson-accelerated VBD integrators are implemented")
+            "only the base, Chebyshev-, Anderson- and trust-region-accelerated VBD integrators are implemented")
     if data.m.size == 0:
         raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
     nV, nT = data.x.shape[1], data.E.shape[1]
@@ -75,9 +81,10 @@ def _marshal(data: Data):
     d.strategy = int(data.strategy)
     d.kD = float(data.kD)
     d.detHZero = float(data.detH_zero)
-    d.acceleration = {AccelerationStrategy.Chebyshev: 1, AccelerationStrategy.Anderson: 2}.get(data.accelerator, 0)
+    d.acceleration = _ACCELERATION[data.accelerator]
     d.rho = float(data.rho)
     d.window = int(data.manderson)
+    d.eta, d.tau, d.curved = float(data.eta), float(data.tau), int(bool(data.curved))
     return d, keep
 
 
@@ -231,6 +238,18 @@ class Integrator:
         _native.check(self._lib.pbat_vbd_integrator_counts(self._h, ctypes.byref(p), ctypes.byref(u),
                                                            ctypes.byref(i)))
         return p.value, u.value, i.value
+
+    def trust_region_log(self):
+        """[(t, accepted)] of the linear trust-region path in the last step: one entry per
+        iteration k >= 2 of each substep (the reference logs the same through its
+        profiler, gpu/impl/vbd/TrustRegionIntegrator.cu:111,133,146)."""
+        n = ctypes.c_int64()
+        _native.check(self._lib.pbat_vbd_integrator_trust_region_log(self._h, None, None, 0, ctypes.byref(n)))
+        t = np.zeros(max(n.value, 1))
+        acc = np.zeros(max(n.value, 1), dtype=np.int32)
+        _native.check(self._lib.pbat_vbd_integrator_trust_region_log(
+            self._h, _native.ptr(t, ctypes.c_double), _native.ptr(acc, ctypes.c_int32), n.value, ctypes.byref(n)))
+        return [(float(t[k]), bool(acc[k])) for k in range(n.value)]
 
     def synchronize(self) -> None:
         _native.check(self._lib.pbat_vbd_integrator_synchronize(self._h))

@@ -255,9 +255,15 @@ def test_chebyshev_trajectory_and_graph_replay(gpu_device):
 
 def test_unavailable_accelerators_raise(gpu_device):
     s = meshes.plain_cpu_scene(3)
-    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_trust_region_acceleration(0.2, 2.0).construct())
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_nesterov_acceleration(1.0).construct())
     with pytest.raises(ValueError):
         pbat.gpu.vbd.Integrator(data)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_broyden_acceleration(3).construct())
+    with pytest.raises(ValueError):
+        pbat.gpu.vbd.Integrator(data)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_trust_region_acceleration(0.2, 2.0).construct())
+    with pytest.raises(ValueError):
+        pbat.gpu.vbd.BatchIntegrator([data, data])
     data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_anderson_acceleration(17).construct())
     with pytest.raises(ValueError):
         pbat.gpu.vbd.Integrator(data)
@@ -335,6 +341,157 @@ def test_anderson_moving_and_graph_replay(gpu_device):
         e = rel(g1.x, ref.positions)
         print(f"rel pos err {e:.3e}")
         assert e <= ANDERSON_POS_TOL
+
+
+# ---- trust-region VBD (gpu/impl/vbd/TrustRegionIntegrator.cu) ------------------
+# The linear path extrapolates along the last VBD step by up to ~15x, and its
+# accept/reject test compares objective differences that FP32 positions resolve
+# to ~1e-7 of f; one step stays within TR_POS_TOL of the FP64 oracle stepping
+# from the same state when both take the same decisions.
+TR_POS_TOL = 5e-5
+
+
+def build_tr_pair(s, eta=0.2, tau=2.0, curved=False, v=None):
+    data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+            .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc)
+            .with_trust_region_acceleration(eta, tau, curved))
+    if v is not None:
+        data = data.with_velocity(v)
+    data = data.construct()
+    gpu = pbat.gpu.vbd.Integrator(data)
+    ref = O.OracleVbd(s.X, s.E, v=v, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
+                      nthreads=NTHREADS)
+    return gpu, ref
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_trust_region_reference_cube(gpu_device, curved):
+    """The free-fall case of the (commented-out) test at
+    gpu/impl/vbd/TrustRegionIntegrator.cu:787-851: eta 0.1, tau 2, Y 1e7, nu 0.45,
+    dt 1e-2, 10 iterations on the 8-vertex cube: it falls along z only."""
+    P = np.array([[0., 1., 0., 1., 0., 1., 0., 1.], [0., 0., 1., 1., 0., 0., 1., 1.],
+                  [0., 0., 0., 0., 1., 1., 1., 1.]])
+    T = np.array([[0, 3, 5, 6, 0], [1, 2, 4, 7, 5], [3, 0, 6, 5, 3], [5, 6, 0, 3, 6]])
+    mu, lam = meshes.lame_coefficients(np.full(5, 1e7), 0.45)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(P, T).with_material(np.full(5, 1e3), mu, lam)
+            .with_trust_region_acceleration(0.1, 2.0, curved).construct())
+    vbd = pbat.gpu.vbd.Integrator(data)
+    vbd.step(1e-2, 10, 1)
+    dx = vbd.x - P.astype(np.float32)
+    assert (dx[2] < 0).all()
+    assert (np.abs(dx[:2]) < 1e-4).all()
+
+
+def test_trust_region_curved_path_is_plain_vbd_bitwise(gpu_device):
+    """The curved path never takes an accelerated step (its constraint solve returns
+    0, TrustRegionIntegrator.cu:700-703): bitwise the plain integrator, graph or not."""
+    s = meshes.plain_gpu_scene(16, seed=3)
+    v0 = np.random.default_rng(5).normal(size=(3, s.n_vertices)) * 0.3
+    g1, _ = build_tr_pair(s, curved=True, v=v0)
+    _, g2, _ = build_pair(s, v=v0)
+    g3, _ = build_tr_pair(s, curved=True, v=v0)
+    g3.graph_capture = False
+    for _ in range(3):
+        g1.step(s.dt, 12, 2)
+        g2.step(s.dt, 12, 2)
+        g3.step(s.dt, 12, 2)
+        assert np.array_equal(g1.x, g2.x) and np.array_equal(g1.v, g2.v)
+        assert np.array_equal(g3.x, g2.x)
+
+
+@pytest.mark.parametrize("iterations", [10, 20])
+def test_trust_region_linear_one_step_plain_10(gpu_device, iterations):
+    """The 10^3 workload: extrapolated steps up to t ~ 18 are tried, some kept, some
+    rolled back; the device keeps and rejects the same steps as the oracle, and the
+    positions are within TR_POS_TOL of the FP64 oracle."""
+    s = meshes.plain_cpu_scene(10)
+    gpu, ref = build_tr_pair(s)
+    log = []
+    gpu.step(s.dt, iterations, 1)
+    ref.step_trust_region(s.dt, iterations, 1, eta=0.2, tau=2.0, curved=False, log=log)
+    assert any(acc for _, _, acc, _ in log) and not all(acc for _, _, acc, _ in log)
+    dev = gpu.trust_region_log()
+    for (td, ad), (k, t, acc, _) in zip(dev, log):
+        print(f"  k={k} device t={td:.5f} kept={ad}  oracle t={t:.5f} kept={acc}")
+    # every verdict agrees; the model's t agrees to 1e-3 on the first tried steps
+    # (later models fit objective differences near FP32 resolution and move more)
+    assert [a for _, a in dev] == [acc for _, _, acc, _ in log]
+    tried = [(td, t) for (td, _), (_, t, _, _) in zip(dev, log) if t != 1.0][:3]
+    assert len(tried) == 3 and all(abs(td - t) <= 1e-3 * t for td, t in tried)
+    e = rel(gpu.x, ref.positions)
+    ed = rel(gpu.x - s.X, ref.positions - s.X)
+    print(f"{iterations} iterations: rel pos err {e:.3e} rel disp err {ed:.3e}")
+    assert e <= TR_POS_TOL
+
+
+def test_trust_region_rejecting_every_step_is_plain_vbd(gpu_device):
+    """eta = 1e30: every tried step is rolled back to the VBD iterate; the positions
+    are plain VBD's up to the FP32 rounding of x_k + ((x_k + t dx) - x_k) / t."""
+    s = meshes.plain_cpu_scene(10)
+    gpu, _ = build_tr_pair(s, eta=1e30)
+    _, plain, _ = build_pair(s)
+    gpu.step(s.dt, 20, 1)
+    plain.step(s.dt, 20, 1)
+    assert rel(gpu.x, plain.x.astype(np.float64)) <= 1e-6
+
+
+def test_trust_region_moving_16(gpu_device):
+    """A jittered 16^3 mesh with an initial velocity field, 3 steps of 2 substeps x 12
+    iterations, each from the device's state. The device's decisions (t, kept) replayed
+    by the oracle give positions within POS_TOL. The oracle's own decisions at the
+    same states: the first tried step of each substep has the device's t to 1% and
+    the same verdict; later ones rest on objective differences of ~1e-5 of f and on
+    second differences below that, which the FP32 trajectory (displacements ~1e-3
+    off the FP64 ones) moves, so only 75% of all verdicts must agree. Graph capture
+    on or off gives the same bits (the path runs uncaptured)."""
+    s = meshes.plain_gpu_scene(16, seed=3)
+    v0 = np.random.default_rng(5).normal(size=(3, s.n_vertices)) * 0.3
+    g1, ref = build_tr_pair(s, v=v0)
+    g2, _ = build_tr_pair(s, v=v0)
+    g2.graph_capture = False
+    for _ in range(3):
+        ref.x[:] = g1.x.T
+        ref.v[:] = g1.v.T
+        g1.step(s.dt, 12, 2)
+        g2.step(s.dt, 12, 2)
+        assert np.array_equal(g1.x, g2.x)
+        dev = g1.trust_region_log()
+        assert len(dev) == 2 * (12 - 2)
+        own = []
+        ref.step_trust_region(s.dt, 12, 2, eta=0.2, tau=2.0, curved=False, log=own, forced=dev)
+        e = rel(g1.x, ref.positions)
+        print(f"rel pos err {e:.3e} (device decisions replayed)")
+        for (t_dev, acc_dev), (k, t_own, acc_own, ratio) in zip(dev, own):
+            print(f"  k={k} device t={t_dev:.4f} kept={acc_dev}  oracle t={t_own:.4f} kept={acc_own} ratio={ratio}")
+        assert e <= POS_TOL
+        pairs = list(zip(dev, own))
+        for half in (pairs[:10], pairs[10:]):
+            first = next(((d, o) for d, o in half if d[0] != 1.0 or o[1] != 1.0), None)
+            if first is not None:
+                (t_dev, acc_dev), (_, t_own, acc_own, _) = first
+                assert abs(t_dev - t_own) <= 1e-2 * t_own and acc_dev == acc_own
+        assert sum(d[1] == o[2] for d, o in pairs) >= 0.75 * len(pairs)
+
+
+def test_trust_region_invalid_parameters_raise(gpu_device):
+    """sim/vbd/Data.cpp:294-302: eta >= 0 and tau > 1."""
+    s = meshes.plain_cpu_scene(3)
+    for eta, tau in ((-0.1, 2.0), (0.2, 1.0)):
+        data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_trust_region_acceleration(eta, tau)
+                .construct(validate=False))
+        with pytest.raises(ValueError):
+            pbat.gpu.vbd.Integrator(data)
+
+
+def test_traced_step_runs_plain_iterations_when_accelerated(gpu_device, tmp_path):
+    """TracedSolve (gpu/impl/vbd/Integrator.cu:284-301) is not overridden by the
+    accelerated integrators: a traced Chebyshev step equals a plain step bitwise."""
+    s = meshes.plain_gpu_scene(6, seed=2)
+    _, g1, _ = build_pair(s, chebyshev_rho=0.9)
+    _, g2, _ = build_pair(s)
+    g1.traced_step(s.dt, 5, 1, t=0, dir=str(tmp_path))
+    g2.step(s.dt, 5, 1)
+    np.testing.assert_array_equal(g1.x, g2.x)
 
 
 def test_traced_step_writes_reference_files(gpu_device, tmp_path):

@@ -497,3 +497,84 @@ def test_anderson_converges_to_the_vbd_fixed_point():
     err_c = np.abs(c.positions - b.positions).max()
     assert err_a < 1e-12
     assert err_c > 1e-7
+
+
+# ---- trust-region VBD (gpu/impl/vbd/TrustRegionIntegrator.cu) ---------------
+def _tr_scene(seed=7):
+    X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=seed)
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+    kw = dict(aext=meshes.gravity(X.shape[1]), dbc=meshes.bottom_vertices(X), mue=mu, lambdae=lam,
+              v=np.random.default_rng(3).normal(size=X.shape) * 0.3)
+    return X, E, kw
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_trust_region_reference_cube_kat(curved):
+    """The free-fall case of the (commented-out) test at
+    gpu/impl/vbd/TrustRegionIntegrator.cu:787-851: eta 0.1, tau 2, Y 1e7, nu 0.45,
+    dt 1e-2, 10 iterations: the cube falls along z only."""
+    nT = CUBE_T.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e7), 0.45)
+    o = O.OracleVbd(CUBE_P, CUBE_T, mue=mu, lambdae=lam)
+    o.step_trust_region(1e-2, 10, 1, eta=0.1, tau=2.0, curved=curved)
+    dx = o.positions - CUBE_P
+    assert np.all(dx[2] < 0)
+    assert np.all(np.abs(dx[:2]) < 1e-4)
+
+
+def test_trust_region_curved_path_is_plain_vbd():
+    """The curved path's constraint solve returns 0 (TrustRegionIntegrator.cu:700-703),
+    so every iteration takes the fallback VBD iteration: bitwise plain VBD."""
+    X, E, kw = _tr_scene()
+    a, b = O.OracleVbd(X, E, **kw), O.OracleVbd(X, E, **kw)
+    a.step_trust_region(1.0 / 60.0, 12, 2, eta=0.2, tau=2.0, curved=True)
+    b.step(1.0 / 60.0, 12, 2)
+    np.testing.assert_array_equal(a.positions, b.positions)
+    np.testing.assert_array_equal(a.velocities, b.velocities)
+
+
+def test_trust_region_rejecting_every_step_is_plain_vbd():
+    """With eta = 1e30 no step passes the ratio test: every tried step x_k + t dx is
+    rolled back by x_k + (x - x_k) / t, which returns the VBD iterate, so the
+    positions are plain VBD's; steps with t > 1 are tried all the same."""
+    X, E, kw = _tr_scene()
+    a, b = O.OracleVbd(X, E, **kw), O.OracleVbd(X, E, **kw)
+    log = []
+    a.step_trust_region(1.0 / 60.0, 40, 1, eta=1e30, tau=2.0, curved=False, log=log)
+    b.step(1.0 / 60.0, 40, 1)
+    assert sum(t != 1.0 for _, t, _, _ in log) >= 10
+    assert not any(acc for _, _, acc, _ in log)
+    assert np.abs(a.positions - b.positions).max() <= 1e-14 * np.abs(b.positions).max()
+
+
+def test_trust_region_converges_to_the_vbd_fixed_point():
+    """The linear path changes how fast VBD converges, not where to: 160 iterations
+    reach the minimiser of 10,000 plain iterations to 1e-11 where 160 plain ones are
+    ~5e-10 away, and at 40 iterations it is 10x closer than plain VBD."""
+    X, E, kw = _tr_scene()
+    ref = O.OracleVbd(X, E, **kw)
+    ref.step(1.0 / 60.0, 10000, 1)
+    err = {}
+    for it in (40, 160):
+        a, b = O.OracleVbd(X, E, **kw), O.OracleVbd(X, E, **kw)
+        log = []
+        a.step_trust_region(1.0 / 60.0, it, 1, eta=0.2, tau=2.0, curved=False, log=log)
+        b.step(1.0 / 60.0, it, 1)
+        err[it] = (np.abs(a.positions - ref.positions).max(), np.abs(b.positions - ref.positions).max())
+        assert any(acc for _, _, acc, _ in log)
+    assert err[40][0] < 0.1 * err[40][1]
+    assert err[160][0] < 1e-11 < 1e-10 < err[160][1]
+
+
+def test_trust_region_replaying_its_own_decisions_is_bitwise():
+    """forced= replays a decision sequence; fed the path's own decisions it reproduces
+    the unforced result bit for bit, and its log still records those decisions."""
+    X, E, kw = _tr_scene()
+    a, b = O.OracleVbd(X, E, **kw), O.OracleVbd(X, E, **kw)
+    la, lb = [], []
+    a.step_trust_region(1.0 / 60.0, 30, 2, eta=0.2, tau=2.0, curved=False, log=la)
+    b.step_trust_region(1.0 / 60.0, 30, 2, eta=0.2, tau=2.0, curved=False, log=lb,
+                        forced=[(t, acc) for _, t, acc, _ in la])
+    np.testing.assert_array_equal(a.positions, b.positions)
+    assert la == lb
