@@ -132,6 +132,20 @@ typedef struct pbat_vbd_data {
     double eta;              /* trust region: energy reduction accuracy threshold, >= 0 (sim/vbd/Data.h:244) */
     double tau;              /* trust region: radius increase factor, > 1 (sim/vbd/Data.h:245) */
     int32_t curved;          /* trust region: curved (1) or linear (0) accelerated path (sim/vbd/Data.h:246) */
+    /* Collision mesh and contact model (sim/vbd/Data.h:165-178, 225-229; Data.cpp:34-60,
+     * 121-133). Contact runs when two bodies share the collision mesh; zeros/NULL
+     * leave it off. */
+    int64_t nCollisionVertices;
+    const int64_t* V;        /* collision vertices (indices into x) */
+    int64_t nCollisionTriangles;
+    const int64_t* F;        /* 3 x nCollisionTriangles, column-major (F[3f + k]) */
+    const int64_t* B;        /* nV body index of every vertex (NULL: one body) */
+    const double* XVA;       /* nV vertex areas */
+    const double* FA;        /* nCollisionTriangles triangle areas */
+    double muC;              /* collision penalty */
+    double muF;              /* friction coefficient */
+    double epsv;             /* IPC friction's relative velocity threshold */
+    int32_t activeSetUpdateFrequency; /* substeps between contact updates, >= 1 */
 } pbat_vbd_data;
 
 typedef struct pbat_vbd_integrator pbat_vbd_integrator;
@@ -298,6 +312,14 @@ int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);
 /* Number of colours (vertex partitions) and the number of vertices updated per sweep. */
 int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPartitions, int64_t* nUpdated,
                                int64_t* nIncidences);
+
+/* Contact state after the last Step (the reference keeps it in
+ * VertexTriangleMixedCcdDcd::active and Integrator::fc,
+ * gpu/impl/contact/VertexTriangleMixedCcdDcd.cuh:151-160, gpu/impl/vbd/Integrator.cuh:165):
+ * active[c] for each collision vertex c in the order of pbat_vbd_data::V, and up to
+ * 8 contacting triangles per vertex (fc[8 i + k], -1 padded, vertices in original
+ * order). Either may be NULL; fails when contact is off. */
+int pbat_vbd_integrator_contacts(pbat_vbd_integrator* h, int32_t* active, int32_t* fc);
 
 /* Decisions of the linear trust-region path in the last Step, one entry per
  * iteration k >= 2 of each substep, in order: the step length t along the VBD

@@ -107,6 +107,8 @@ class _State(ctypes.Structure):
         ("nthreads", ctypes.c_int), ("gpu_velocity_semantics", ctypes.c_int),
         ("chebyshev", ctypes.c_int), ("rho", ctypes.c_double), ("xkm1", ctypes.c_void_p),
         ("xkm2", ctypes.c_void_p),
+        ("fc", ctypes.c_void_p), ("F", ctypes.c_void_p), ("XVA", ctypes.c_void_p), ("FA", ctypes.c_void_p),
+        ("muC", ctypes.c_double), ("muF", ctypes.c_double), ("epsv", ctypes.c_double), ("xb", ctypes.c_void_p),
     ]
 
 
@@ -114,6 +116,30 @@ def chebyshev_omega(k: int, rho2: float, omega: float) -> float:
     lib().ora_chebyshev_omega.restype = ctypes.c_double
     lib().ora_chebyshev_omega.argtypes = [ctypes.c_int64, ctypes.c_double, ctypes.c_double]
     return lib().ora_chebyshev_omega(int(k), float(rho2), float(omega))
+
+
+def point_triangle_sqdist(p, a, b, c) -> float:
+    """Squared point-triangle distance (geometry/DistanceQueries.h:217-222)."""
+    L = lib()
+    L.ora_point_triangle_sqdist.restype = ctypes.c_double
+    L.ora_point_triangle_sqdist.argtypes = [ctypes.POINTER(ctypes.c_double)] * 4
+    args = [np.ascontiguousarray(np.asarray(q, np.float64)) for q in (p, a, b, c)]
+    return L.ora_point_triangle_sqdist(*[_dp(q) for q in args])
+
+
+def vertex_triangle_contact(xtv, xv, xtf, xf, dt, muC, muF, epsv):
+    """(E, g, H) of AccumulateVertexTriangleContact (sim/vbd/Kernels.h:215-302); xtf, xf
+    are 3x3 with one triangle vertex per column."""
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.ora_vertex_triangle_contact.restype = ctypes.c_double
+    L.ora_vertex_triangle_contact.argtypes = [dp, dp, dp, dp] + [ctypes.c_double] * 4 + [dp, dp]
+    a = [np.ascontiguousarray(np.asarray(q, np.float64)) for q in (xtv, xv)]
+    m = [np.ascontiguousarray(np.asarray(q, np.float64).T) for q in (xtf, xf)]  # column k = vertex k
+    g, H = np.zeros(3), np.zeros(9)
+    E = L.ora_vertex_triangle_contact(_dp(a[0]), _dp(a[1]), _dp(m[0]), _dp(m[1]), float(dt), float(muC), float(muF),
+                                      float(epsv), _dp(g), _dp(H))
+    return E, g, H.reshape(3, 3).T.copy()
 
 
 # ---- constitutive model ---------------------------------------------------
@@ -233,7 +259,7 @@ class OracleVbd:
     def __init__(self, X, E, *, v=None, aext=None, rhoe=None, mue=None, lambdae=None,
                  dbc=None, ordering=ORDER_LARGEST_DEGREE, selection=SELECT_LEAST_USED,
                  strategy=INIT_ADAPTIVE_PBAT, kD=0.0, detHZero=1e-7, nthreads=1,
-                 gpu_velocity_semantics=False, colors=None, chebyshev_rho=None):
+                 gpu_velocity_semantics=False, colors=None, chebyshev_rho=None, contact=None):
         Xv, Ev = _prep(X, E)
         nV, nT = Xv.shape[0], Ev.shape[0]
         self.nV, self.nT = nV, nT
@@ -276,6 +302,112 @@ class OracleVbd:
         self.rho = 1.0 if chebyshev_rho is None else float(chebyshev_rho)
         self.xkm1 = self.x.copy()
         self.xkm2 = self.x.copy()
+        self.contact = contact is not None
+        if self.contact:
+            self._init_contact(**contact)
+
+    # ---- vertex-triangle contact (gpu/impl/vbd/Integrator.cu:82-103,163-301;
+    # gpu/impl/contact/VertexTriangleMixedCcdDcd.cu) --------------------------------
+    def _init_contact(self, V, F, B, XVA, FA, muC=1e6, muF=0.3, epsv=1e-3, frequency=1):
+        """Collision vertices V (into X), triangles F (3 x nF), body of every vertex B,
+        vertex areas XVA (nV), triangle areas FA (nF) (sim/vbd/Data.cpp:34-60), the contact
+        parameters (sim/vbd/Data.h:225-229). The active set starts empty, the search
+        radii at FLT_MAX and the contact lists at -1
+        (gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:22-48, gpu/impl/vbd/Integrator.cu:60)."""
+        self.cV = np.asarray(V, dtype=np.int64).reshape(-1)
+        self.cF = np.ascontiguousarray(np.asarray(F, dtype=np.int64).T)  # nF x 3
+        self.B = np.asarray(B, dtype=np.int64).reshape(-1)
+        self.XVA = np.ascontiguousarray(np.asarray(XVA, np.float64))
+        self.FA = np.ascontiguousarray(np.asarray(FA, np.float64))
+        self.muC, self.muF, self.epsv = float(muC), float(muF), float(epsv)
+        self.frequency = int(frequency)
+        self.fc = np.full((self.nV, 8), -1, dtype=np.int64)
+        self.cactive = np.zeros(self.cV.size, dtype=bool)
+        self.in_av = np.zeros(self.cV.size, dtype=bool)
+        self.dupper = np.full(self.cV.size, float(np.finfo(np.float32).max))
+        self.xb = np.zeros((self.nV, 3))
+
+    def initialize_active_set(self, x0, x1):
+        """VertexTriangleMixedCcdDcd::InitializeActiveSet (:53-142): a collision vertex
+        becomes active when the box of its segment x0 -> x1 overlaps the box of a swept
+        triangle of another body (boxes closed); its search radius becomes the squared
+        diagonal of the two boxes' union, the largest over such triangles (the reference
+        keeps the last one its BVH traversal meets). Vertices stay active from before."""
+        p0, p1 = x0[self.cV], x1[self.cV]
+        lo_v, hi_v = np.minimum(p0, p1), np.maximum(p0, p1)
+        tri = np.concatenate([x0[self.cF], x1[self.cF]], axis=1)  # nF x 6 x 3
+        lo_f, hi_f = tri.min(axis=1), tri.max(axis=1)
+        other = self.B[self.cV][:, None] != self.B[self.cF[:, 0]][None, :]
+        overlap = other & np.all((lo_v[:, None, :] <= hi_f[None]) & (lo_f[None] <= hi_v[:, None, :]), axis=2)
+        diag = np.maximum(hi_v[:, None, :], hi_f[None]) - np.minimum(lo_v[:, None, :], lo_f[None])
+        d2 = np.where(overlap, (diag * diag).sum(axis=2), -np.inf).max(axis=1)
+        hit = overlap.any(axis=1)
+        self.dupper[hit] = d2[hit]
+        self.cactive |= hit
+        self.in_av = self.cactive.copy()
+
+    def nearest_triangles(self, x, v):
+        """Bvh::NearestNeighbours (gpu/impl/geometry/Bvh.cuh:355-466) as used by
+        VertexTriangleMixedCcdDcd (:117-164) for collision vertex v: squared distances to
+        the triangles of other bodies, d* the smallest, eps = FLT_EPSILON. With
+        d* < r - eps (r the vertex's search radius) the triangles within d* + eps, with
+        d* within r +- eps those within r + eps, else none; in triangle order, at most 8
+        (the reference's order and tie-breaking follow its BVH traversal)."""
+        i = self.cV[v]
+        fs = np.nonzero(self.B[self.cF[:, 0]] != self.B[i])[0]
+        if fs.size == 0:
+            return []
+        d = np.array([point_triangle_sqdist(x[i], *x[self.cF[f]]) for f in fs])
+        eps = float(np.finfo(np.float32).eps)
+        ds, r = d.min(), self.dupper[v]
+        if ds < r - eps:
+            thr = ds + eps
+        elif ds <= r + eps:
+            thr = r + eps
+        else:
+            return []
+        return [int(f) for f in fs[d <= thr][:8]]
+
+    def update_active_set(self, x):
+        """Integrator::UpdateActiveSet (gpu/impl/vbd/Integrator.cu:250-271): each vertex
+        of this step's active list gets its nearest triangles as contacts; the lists of
+        other vertices are left as they are."""
+        for v in np.nonzero(self.in_av)[0]:
+            S = self.nearest_triangles(x, v)
+            self.fc[self.cV[v]] = -1
+            self.fc[self.cV[v], :len(S)] = S
+
+    def finalize_active_set(self, x):
+        """VertexTriangleMixedCcdDcd::FinalizeActiveSet (:185-215): a vertex of this step's
+        active list stays active when it lies below the plane of the last of its nearest
+        triangles (signed distance < 0); vertices without any keep their flag."""
+        for v in np.nonzero(self.in_av)[0]:
+            S = self.nearest_triangles(x, v)
+            if S:
+                a, b, c = x[self.cF[S[-1]]]
+                self.cactive[v] = float(np.dot(x[self.cV[v]] - a, np.cross(b - a, c - a))) < 0.0
+
+    def step_contact(self, dt: float, iterations: int, substeps: int = 1) -> None:
+        """Integrator::Step with contact (gpu/impl/vbd/Integrator.cu:82-103): the active
+        set from the full step's sweep x_t -> x_t + dt v + dt^2 a, contacts refreshed every
+        `frequency` substeps at the substep's initial iterate, VBD sweeps with contact
+        terms (each colour reading positions from before its pass), the active set
+        finalised at the end."""
+        L = lib()
+        for f in ("ora_substep_begin", "ora_sweep", "ora_substep_end"):
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_double]
+            getattr(L, f).restype = None
+        sdt = float(dt) / substeps
+        self.initialize_active_set(self.x.copy(), self.x + dt * self.v + dt * dt * self.aext)
+        for k in range(substeps):
+            s = self._state()
+            L.ora_substep_begin(ctypes.byref(s), sdt)
+            if k % self.frequency == 0:
+                self.update_active_set(self.x)
+            for _ in range(iterations):
+                L.ora_sweep(ctypes.byref(s), sdt)
+            L.ora_substep_end(ctypes.byref(s), sdt)
+        self.finalize_active_set(self.x)
 
     def _state(self) -> _State:
         s = _State()
@@ -294,6 +426,12 @@ class OracleVbd:
         s.gpu_velocity_semantics = int(self.gpu_velocity_semantics)
         s.chebyshev, s.rho = int(self.chebyshev), self.rho
         s.xkm1, s.xkm2 = self.xkm1.ctypes.data, self.xkm2.ctypes.data
+        if self.contact:
+            for name in ("fc", "XVA", "FA", "xb"):
+                assert getattr(self, name).flags.c_contiguous, name
+            s.fc, s.F = self.fc.ctypes.data, self.cF.ctypes.data
+            s.XVA, s.FA, s.xb = self.XVA.ctypes.data, self.FA.ctypes.data, self.xb.ctypes.data
+            s.muC, s.muF, s.epsv = self.muC, self.muF, self.epsv
         return s
 
     # sim/vbd/Integrator.cpp:22-45

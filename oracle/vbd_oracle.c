@@ -564,8 +564,168 @@ void ora_vertex_elastic_block(const ora_vbd* s, int64_t i, double H[9], double g
     }
 }
 
-/* Integrator::SolveVertex (sim/vbd/Integrator.cpp:98-136) */
-static void solve_vertex(ora_vbd* s, int64_t i, double sdt, double sdt2)
+static void sub3(const double* a, const double* b, double* r)
+{
+    r[0] = a[0] - b[0], r[1] = a[1] - b[1], r[2] = a[2] - b[2];
+}
+
+static double dot3(const double* a, const double* b)
+{
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+double ora_vertex_triangle_contact(const double xtv[3], const double xv[3], const double xtf[9], const double xf[9],
+                                   double dt, double muC, double muF, double epsv, double* g, double* H)
+{
+    const double *A = xf, *Bv = xf + 3, *C = xf + 6;
+    double e0[3], e1[3], n[3];
+    sub3(Bv, A, e0);
+    sub3(C, A, e1);
+    cross3(e0, e1, n);
+    double const dbl = sqrt(dot3(n, n));
+    if (dbl <= 1e-8) /* degenerate triangle */
+        return 0.;
+    for (int d = 0; d < 3; ++d)
+        n[d] /= dbl;
+    /* project xv on the triangle's plane; barycentric coordinates of the projection */
+    double ap[3];
+    sub3(xv, A, ap);
+    double const t = dot3(n, ap);
+    for (int d = 0; d < 3; ++d)
+        ap[d] -= t * n[d];
+    double const d00 = dot3(e0, e0), d01 = dot3(e0, e1), d11 = dot3(e1, e1);
+    double const d20 = dot3(ap, e0), d21 = dot3(ap, e1);
+    double const den = d00 * d11 - d01 * d01;
+    double const bv = (d11 * d20 - d01 * d21) / den, bw = (d00 * d21 - d01 * d20) / den;
+    double const bu = 1. - bv - bw;
+    if (!(bu >= 0. && bu <= 1. && bv >= 0. && bv <= 1. && bw >= 0. && bw <= 1.))
+        return 0.;
+    /* normal penalty 1/2 muC d^2 on the penetration depth d = min(0, (xv - xb).n) */
+    double xb[3], xtb[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        xb[d] = bu * A[d] + bv * Bv[d] + bw * C[d];
+        xtb[d] = bu * xtf[d] + bv * xtf[3 + d] + bw * xtf[6 + d];
+    }
+    double r[3];
+    sub3(xv, xb, r);
+    double dpen = dot3(r, n);
+    if (dpen > 0.)
+        dpen = 0.;
+    double const lambda = muC * dpen;
+    double E = 0.5 * lambda * dpen;
+    if (g)
+        for (int d = 0; d < 3; ++d)
+            g[d] += lambda * n[d];
+    if (H)
+        for (int c = 0; c < 3; ++c)
+            for (int rr = 0; rr < 3; ++rr)
+                H[3 * c + rr] += muC * n[rr] * n[c];
+    /* lagged IPC friction in the tangent frame (e0, n x e0) on the relative
+     * displacement of the contact point over the substep */
+    double b1[3];
+    cross3(n, e0, b1);
+    double du[3];
+    for (int d = 0; d < 3; ++d)
+        du[d] = (xv[d] - xtv[d]) - (xb[d] - xtb[d]);
+    double const u0 = dot3(e0, du), u1 = dot3(b1, du);
+    double const unorm = sqrt(u0 * u0 + u1 * u1) + DBL_EPSILON;
+    double const epsvh = epsv * dt;
+    double const muFl = muF * fabs(lambda);
+    double const y = unorm / epsvh;
+    double const f1 = (y < 1.) ? 2. * y - y * y : 1.;
+    double const c1 = muFl * f1 / unorm;
+    if (g)
+        for (int d = 0; d < 3; ++d)
+            g[d] += c1 * (e0[d] * u0 + b1[d] * u1);
+    if (H)
+        for (int c = 0; c < 3; ++c)
+            for (int rr = 0; rr < 3; ++rr)
+                H[3 * c + rr] += c1 * (e0[rr] * e0[c] + b1[rr] * b1[c]);
+    double const u2 = unorm * unorm;
+    E += muFl * (epsvh / 3. + u2 / epsvh - u2 * unorm / (3. * epsvh * epsvh));
+    return E;
+}
+
+double ora_point_triangle_sqdist(const double p[3], const double a[3], const double b[3], const double c[3])
+{
+    double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+    sub3(b, a, ab);
+    sub3(c, a, ac);
+    sub3(p, a, ap);
+    double const d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+    double u = 1., v = 0., w = 0.;
+    sub3(p, b, bp);
+    double const d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+    sub3(p, c, cp);
+    double const d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+    double const vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+    if (d1 <= 0. && d2 <= 0.)
+        u = 1., v = 0., w = 0.; /* vertex region a */
+    else if (d3 >= 0. && d4 <= d3)
+        u = 0., v = 1., w = 0.; /* vertex region b */
+    else if (vc <= 0. && d1 >= 0. && d3 <= 0.)
+    {
+        v = d1 / (d1 - d3); /* edge ab */
+        u = 1. - v, w = 0.;
+    }
+    else if (d6 >= 0. && d5 <= d6)
+        u = 0., v = 0., w = 1.; /* vertex region c */
+    else if (vb <= 0. && d2 >= 0. && d6 <= 0.)
+    {
+        w = d2 / (d2 - d6); /* edge ac */
+        u = 1. - w, v = 0.;
+    }
+    else if (va <= 0. && (d4 - d3) >= 0. && (d5 - d6) >= 0.)
+    {
+        w = (d4 - d3) / ((d4 - d3) + (d5 - d6)); /* edge bc */
+        u = 0., v = 1. - w;
+    }
+    else
+    {
+        double const den = 1. / (va + vb + vc); /* face */
+        v = vb * den, w = vc * den, u = 1. - v - w;
+    }
+    for (int d = 0; d < 3; ++d)
+        q[d] = p[d] - (u * a[d] + v * b[d] + w * c[d]);
+    return dot3(q, q);
+}
+
+/* ContactPenalty + AccumulateVertexTriangleContact over vertex i's contacts
+ * (gpu/impl/vbd/Kernels.cuh:80-114, 204-223): the penalty muC is scaled by the
+ * vertex area over the summed areas of its contacting triangles, times each
+ * triangle's area. */
+static void add_contacts(const ora_vbd* s, int64_t i, double sdt, double* g, double* H)
+{
+    const int64_t* f = s->fc + 8 * i;
+    int nc = 0;
+    for (int c = 0; c < 8; ++c)
+        nc += f[c] >= 0;
+    if (nc == 0)
+        return;
+    double sumfa = 0.;
+    for (int c = 0; c < nc; ++c)
+        sumfa += s->FA[f[c]];
+    double const kC = s->XVA[i] * s->muC / sumfa;
+    for (int c = 0; c < nc; ++c)
+    {
+        double xf[9], xtf[9];
+        for (int k = 0; k < 3; ++k)
+        {
+            int64_t const j = s->F[3 * f[c] + k];
+            for (int d = 0; d < 3; ++d)
+            {
+                xf[3 * k + d] = s->x[3 * j + d];
+                xtf[3 * k + d] = s->xt[3 * j + d];
+            }
+        }
+        ora_vertex_triangle_contact(s->xt + 3 * i, s->x + 3 * i, xtf, xf, sdt, kC * s->FA[f[c]], s->muF, s->epsv, g,
+                                    H);
+    }
+}
+
+/* Integrator::SolveVertex (sim/vbd/Integrator.cpp:98-136); the new position goes to out */
+static void solve_vertex(ora_vbd* s, int64_t i, double sdt, double sdt2, double* out)
 {
     double H[9], g[3];
     ora_vertex_elastic_block(s, i, H, g);
@@ -584,6 +744,8 @@ static void solve_vertex(ora_vbd* s, int64_t i, double sdt, double sdt2)
     }
     for (int k = 0; k < 9; ++k)
         H[k] *= 1. + D;
+    if (s->fc)
+        add_contacts(s, i, sdt, g, H);
     /* AddInertiaDerivatives (sim/vbd/Kernels.h:304-322) */
     double const K = s->m[i] / sdt2;
     for (int d = 0; d < 3; ++d)
@@ -593,14 +755,18 @@ static void solve_vertex(ora_vbd* s, int64_t i, double sdt, double sdt2)
     }
     /* IntegratePositions (sim/vbd/Kernels.h:324-339): absolute determinant test */
     if (fabs(det3(H)) <= s->detHZero)
+    {
+        for (int d = 0; d < 3; ++d)
+            out[d] = xi[d];
         return;
+    }
     double Hi[9];
     inv3(H, Hi);
     double step[3];
     for (int r = 0; r < 3; ++r)
         step[r] = Hi[r] * g[0] + Hi[3 + r] * g[1] + Hi[6 + r] * g[2];
     for (int d = 0; d < 3; ++d)
-        xi[d] -= step[d];
+        out[d] = xi[d] - step[d];
 }
 
 /* ---- thread pool over the vertices of one colour ----------------------- */
@@ -690,7 +856,7 @@ static void* worker(void* arg)
                 int64_t pb, pe;
                 range_split(s->Pptr[p], s->Pptr[p + 1], w->tid, w->nthreads, &pb, &pe);
                 for (int64_t q = pb; q < pe; ++q)
-                    solve_vertex(s, s->Padj[q], w->sdt, w->sdt2);
+                    solve_vertex(s, s->Padj[q], w->sdt, w->sdt2, s->x + 3 * s->Padj[q]);
                 pthread_barrier_wait(w->bar);
             }
             if (s->chebyshev)
@@ -717,8 +883,17 @@ void ora_substep_begin(ora_vbd* s, double sdt)
 void ora_sweep(ora_vbd* s, double sdt)
 {
     for (int64_t p = 0; p < s->nPartitions; ++p)
+    {
         for (int64_t q = s->Pptr[p]; q < s->Pptr[p + 1]; ++q)
-            solve_vertex(s, s->Padj[q], sdt, sdt * sdt);
+        {
+            int64_t const i = s->Padj[q];
+            solve_vertex(s, i, sdt, sdt * sdt, s->fc ? s->xb + 3 * i : s->x + 3 * i);
+        }
+        if (s->fc)
+            for (int64_t q = s->Pptr[p]; q < s->Pptr[p + 1]; ++q)
+                for (int d = 0; d < 3; ++d)
+                    s->x[3 * s->Padj[q] + d] = s->xb[3 * s->Padj[q] + d];
+    }
 }
 
 void ora_substep_end(ora_vbd* s, double sdt)
@@ -731,6 +906,8 @@ int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps)
     if (substeps < 1 || iterations < 0)
         return ORA_ERR_ARG;
     if (s->chebyshev && (!s->xkm1 || !s->xkm2))
+        return ORA_ERR_ARG;
+    if (s->fc) /* contact steps run through the substep pieces */
         return ORA_ERR_ARG;
     double const sdt = dt / (double)substeps;
     double const sdt2 = sdt * sdt;
@@ -746,7 +923,7 @@ int ora_step(ora_vbd* s, double dt, int64_t iterations, int64_t substeps)
             {
                 for (int64_t p = 0; p < s->nPartitions; ++p)
                     for (int64_t q = s->Pptr[p]; q < s->Pptr[p + 1]; ++q)
-                        solve_vertex(s, s->Padj[q], sdt, sdt2);
+                        solve_vertex(s, s->Padj[q], sdt, sdt2, s->x + 3 * s->Padj[q]);
                 if (s->chebyshev)
                 {
                     /* ChebyshevIntegrator::Solve (sim/vbd/ChebyshevIntegrator.cpp:15-32) */

@@ -113,7 +113,34 @@ typedef struct ora_vbd {
     double rho;
     double* xkm1;
     double* xkm2;
+    /* Vertex-triangle contact of the GPU integrator (gpu/impl/vbd/Kernels.cuh:80-114,
+     * 204-223): off when fc is NULL. fc holds up to 8 contacting triangles per vertex
+     * (8*nV, packed, -1 padded), F the triangles (3 vertex indices each), XVA the
+     * vertex areas (nV), FA the triangle areas. With contact on, ora_sweep writes
+     * each colour's new positions to xb (3*nV) and copies them back after the colour,
+     * so contact terms read the positions from before the colour pass, as the GPU
+     * integrator's per-colour write buffer does (gpu/impl/vbd/Integrator.cu:303-326). */
+    const int64_t* fc;
+    const int64_t* F;
+    const double* XVA;
+    const double* FA;
+    double muC;
+    double muF;
+    double epsv;
+    double* xb;
 } ora_vbd;
+
+/* AccumulateVertexTriangleContact (sim/vbd/Kernels.h:215-302) for vertex position xv
+ * (xtv at the start of the substep) against the triangle with vertex positions xf
+ * (xtf), each 3x3 column-major (column k = vertex k): adds the penalty and lagged
+ * IPC friction gradient to g and Hessian approximation to H (column-major; either may
+ * be NULL) and returns the contact energy. */
+double ora_vertex_triangle_contact(const double xtv[3], const double xv[3], const double xtf[9], const double xf[9],
+                                   double dt, double muC, double muF, double epsv, double* g, double* H);
+
+/* Squared distance from p to the triangle (a, b, c): closest point by Voronoi regions
+ * (Ericson, Real-Time Collision Detection, 5.1.5), as geometry/DistanceQueries.h:217-222. */
+double ora_point_triangle_sqdist(const double p[3], const double a[3], const double b[3], const double c[3]);
 
 /* sim/vbd/Kernels.h:97-102, including its operator precedence: for k >= 2 the
  * result is 4/4 - rho2*omega = 1 - rho2*omega. */

@@ -50,6 +50,9 @@ class Data(ctypes.Structure):
         ("strategy", c_i32), ("kD", c_f64), ("detHZero", c_f64),
         ("acceleration", c_i32), ("rho", c_f64), ("window", c_i32),
         ("eta", c_f64), ("tau", c_f64), ("curved", c_i32),
+        ("nCollisionVertices", c_i64), ("V", p_i64), ("nCollisionTriangles", c_i64), ("F", p_i64),
+        ("B", p_i64), ("XVA", p_f64), ("FA", p_f64), ("muC", c_f64), ("muF", c_f64), ("epsv", c_f64),
+        ("activeSetUpdateFrequency", c_i32),
     ]
 
 
@@ -115,6 +118,7 @@ SIGNATURES = [
     ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("pbat_vbd_device_memory", ctypes.c_int, [p_i64, p_i64, c_i32]),
     ("pbat_vbd_integrator_counts", ctypes.c_int, [ctypes.c_void_p, p_i64, p_i64, p_i64]),
+    ("pbat_vbd_integrator_contacts", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
     ("pbat_vbd_integrator_trust_region_log", ctypes.c_int,
      [ctypes.c_void_p, p_f64, ctypes.POINTER(c_i32), c_i64, p_i64]),
     ("pbat_vbd_integrator_set_timing", ctypes.c_int, [ctypes.c_void_p, c_i32]),

@@ -225,6 +225,15 @@ struct pbat_vbd_integrator
     pbat_vbd::DeviceArray<double> trPartials; // trust region: per-workgroup reduction partials + sums
     std::vector<double> trT;                  // trust region: t of each iteration k >= 2 of the last Step
     std::vector<int> trAccepted;              // trust region: whether that step was kept
+    // Vertex-triangle contact (on when two bodies share the collision mesh)
+    bool contact = false;
+    int nCV = 0, nF = 0, activeSetFrequency = 1;
+    float muC = 1e6f, muF = 0.3f, epsv = 1e-3f;
+    pbat_vbd::DeviceArray<int> cvIdx, cvBody, fc, cvActive, cvInAv;
+    pbat_vbd::DeviceArray<int4> tri;
+    pbat_vbd::DeviceArray<float> triArea, xva, dupper;
+    pbat_vbd::DeviceArray<float4> triLo, triHi, xfull, xb;
+    std::vector<int64_t> cvOrig; // original index of each collision vertex
     float kD = 0.f;
     float detHZero = 1e-7f;
     int blockSize = 256;

@@ -431,10 +431,18 @@ void RestoreState(pbat_vbd_integrator& h, float4 const* x, float4 const* v, floa
     PBAT_HIP(hipMemcpyAsync(h.vprev.ptr, vp, bytes, hipMemcpyDeviceToDevice, h.stream));
 }
 
+// The adjoint differentiates the elastic step only.
+void NoContact(pbat_vbd_integrator const& h)
+{
+    if (h.contact)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "differentiable rollouts do not support contact");
+}
+
 void RolloutForward(pbat_vbd_integrator& h, float dt, int iterations, int steps, int every)
 {
     if (!(dt > 0.f) || iterations < 0 || steps < 1 || every < 1)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "rollout needs dt > 0, iterations >= 0, steps >= 1, every >= 1");
+    NoContact(h);
     PBAT_HIP(hipSetDevice(h.device));
     RolloutState& R = h.rollout;
     R.valid = false;
@@ -586,6 +594,7 @@ void HessianVectorProduct(pbat_vbd_integrator& h, float const* x, float dt, floa
 {
     if (!x || !p || !out || !(dt > 0.f))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid Hessian-vector arguments");
+    NoContact(h);
     PBAT_HIP(hipSetDevice(h.device));
     EnsureBuffers(h);
     RolloutState& R = h.rollout;
@@ -606,6 +615,7 @@ void AdjointSolve(pbat_vbd_integrator& h, float const* x, float dt, float const*
 {
     if (!x || !b || !y || !(dt > 0.f))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid adjoint solve arguments");
+    NoContact(h);
     pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5};
     if (cfgIn)
         cfg = *cfgIn;

@@ -1,0 +1,352 @@
+// Vertex-triangle contact of the GPU VBD integrator: active-set detection
+// (gpu/impl/contact/VertexTriangleMixedCcdDcd.cu) and the per-vertex contact
+// terms of the sweep (gpu/impl/vbd/Kernels.cuh:80-114, 204-223;
+// sim/vbd/Kernels.h:215-302).
+//
+// Detection is a tiled all-pairs search: each workgroup stages 256 triangles in
+// LDS and every thread tests its collision vertex against them, so a launch
+// reads each triangle once per workgroup. It is exhaustive, which makes its
+// answers independent of any traversal order: a vertex's search radius is the
+// largest over its overlapping triangles, and its nearest triangles come in
+// triangle order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace pbat_vbd::device
+{
+
+constexpr int kMaxContacts = 8;     // contacting triangles per vertex (Kernels.cuh:64-65)
+constexpr int kContactTile = 256;   // triangles staged per LDS tile = workgroup size
+constexpr float kFltEpsilon = 1.1920928955078125e-07f; // FLT_EPSILON
+
+struct ContactArgs
+{
+    int const* __restrict__ cvIdx;     // internal vertex of each collision vertex
+    int const* __restrict__ cvBody;    // body of each collision vertex
+    int4 const* __restrict__ tri;      // internal vertices of each triangle, body of its first vertex in .w
+    float const* __restrict__ triArea; // triangle areas
+    float const* __restrict__ xva;     // vertex areas (internal order)
+    int* __restrict__ fc;              // kMaxContacts contacting triangles per internal vertex, -1 padded
+    int* __restrict__ active;          // per collision vertex, kept across steps
+    int* __restrict__ inAv;            // per collision vertex: in this step's active list
+    float* __restrict__ dupper;        // per collision vertex: squared search radius
+    float4* __restrict__ triLo;        // swept triangle boxes
+    float4* __restrict__ triHi;
+    int nCV, nF;
+};
+
+__device__ inline float3 Sub(float4 a, float4 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline float Dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline float3 Cross(float3 a, float3 b)
+{
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// x1 = x0 + h v + h^2 a over all vertices (gpu/impl/vbd/Integrator.cu:163-188).
+[[maybe_unused]] static __global__ __launch_bounds__(256) void ContactFullStep(
+    float4 const* __restrict__ xt, float4 const* __restrict__ vel, float4 const* __restrict__ aext, int n, float dt,
+    float dt2, float4* __restrict__ x1)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 const p = xt[i], u = vel[i], g = aext[i];
+    x1[i] = make_float4(p.x + dt * u.x + dt2 * g.x, p.y + dt * u.y + dt2 * g.y, p.z + dt * u.z + dt2 * g.z, 0.f);
+}
+
+// Boxes of the triangles swept from x0 to x1 (VertexTriangleMixedCcdDcd.cu:87-101).
+[[maybe_unused]] static __global__ __launch_bounds__(256) void ContactSweptBoxes(ContactArgs c,
+                                                                                 float4 const* __restrict__ x0,
+                                                                                 float4 const* __restrict__ x1)
+{
+    int const f = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (f >= c.nF)
+        return;
+    int4 const t = c.tri[f];
+    float4 const p[6] = {x0[t.x], x0[t.y], x0[t.z], x1[t.x], x1[t.y], x1[t.z]};
+    float lx = p[0].x, ly = p[0].y, lz = p[0].z, hx = lx, hy = ly, hz = lz;
+    for (int k = 1; k < 6; ++k)
+    {
+        lx = fminf(lx, p[k].x), ly = fminf(ly, p[k].y), lz = fminf(lz, p[k].z);
+        hx = fmaxf(hx, p[k].x), hy = fmaxf(hy, p[k].y), hz = fmaxf(hz, p[k].z);
+    }
+    c.triLo[f] = make_float4(lx, ly, lz, __int_as_float(t.w));
+    c.triHi[f] = make_float4(hx, hy, hz, 0.f);
+}
+
+// Broad phase of InitializeActiveSet (VertexTriangleMixedCcdDcd.cu:53-142): a
+// collision vertex whose segment x0 -> x1 has a box that overlaps (closed boxes)
+// the swept box of a triangle of another body becomes active, with search radius
+// the squared diagonal of the union of the two boxes, largest over such
+// triangles. Vertices active from earlier steps stay active. inAv receives the
+// step's active list. One thread per collision vertex; blockDim = kContactTile.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactBroadphase(
+    ContactArgs c, float4 const* __restrict__ x0, float4 const* __restrict__ x1)
+{
+    __shared__ float4 sLo[kContactTile], sHi[kContactTile];
+    int const v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    bool const mine = v < c.nCV;
+    float3 lo = make_float3(0.f, 0.f, 0.f), hi = lo;
+    int body = 0;
+    if (mine)
+    {
+        int const i = c.cvIdx[v];
+        float4 const a = x0[i], b = x1[i];
+        lo = make_float3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z));
+        hi = make_float3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z));
+        body = c.cvBody[v];
+    }
+    bool hit = false;
+    float d2max = 0.f;
+    for (int base = 0; base < c.nF; base += kContactTile)
+    {
+        int const f = base + static_cast<int>(threadIdx.x);
+        if (f < c.nF)
+        {
+            sLo[threadIdx.x] = c.triLo[f];
+            sHi[threadIdx.x] = c.triHi[f];
+        }
+        __syncthreads();
+        int const m = min(kContactTile, c.nF - base);
+        if (mine)
+            for (int k = 0; k < m; ++k)
+            {
+                float4 const L = sLo[k], U = sHi[k];
+                if (__float_as_int(L.w) == body)
+                    continue;
+                bool const overlap = lo.x <= U.x && lo.y <= U.y && lo.z <= U.z && L.x <= hi.x && L.y <= hi.y &&
+                                     L.z <= hi.z;
+                if (!overlap)
+                    continue;
+                float const dx = fmaxf(hi.x, U.x) - fminf(lo.x, L.x);
+                float const dy = fmaxf(hi.y, U.y) - fminf(lo.y, L.y);
+                float const dz = fmaxf(hi.z, U.z) - fminf(lo.z, L.z);
+                d2max = hit ? fmaxf(d2max, dx * dx + dy * dy + dz * dz) : dx * dx + dy * dy + dz * dz;
+                hit = true;
+            }
+        __syncthreads();
+    }
+    if (!mine)
+        return;
+    int act = c.active[v];
+    if (hit)
+    {
+        act = 1;
+        c.active[v] = 1;
+        c.dupper[v] = d2max;
+    }
+    c.inAv[v] = act;
+}
+
+// Squared distance from p to triangle (a, b, c): closest point by Voronoi
+// regions (Ericson, Real-Time Collision Detection, 5.1.5), as
+// geometry/DistanceQueries.h:217-222 evaluates it.
+__device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4 c)
+{
+    float3 const ab = Sub(b, a), ac = Sub(c, a), ap = Sub(p, a);
+    float const d1 = Dot(ab, ap), d2 = Dot(ac, ap);
+    float3 const bp = Sub(p, b);
+    float const d3 = Dot(ab, bp), d4 = Dot(ac, bp);
+    float3 const cp = Sub(p, c);
+    float const d5 = Dot(ab, cp), d6 = Dot(ac, cp);
+    float const vc = d1 * d4 - d3 * d2, vb = d5 * d2 - d1 * d6, va = d3 * d6 - d5 * d4;
+    float u, v, w;
+    if (d1 <= 0.f && d2 <= 0.f)
+        u = 1.f, v = 0.f, w = 0.f;
+    else if (d3 >= 0.f && d4 <= d3)
+        u = 0.f, v = 1.f, w = 0.f;
+    else if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f)
+        v = d1 / (d1 - d3), u = 1.f - v, w = 0.f;
+    else if (d6 >= 0.f && d5 <= d6)
+        u = 0.f, v = 0.f, w = 1.f;
+    else if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f)
+        w = d2 / (d2 - d6), u = 1.f - w, v = 0.f;
+    else if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f)
+        w = (d4 - d3) / ((d4 - d3) + (d5 - d6)), u = 0.f, v = 1.f - w;
+    else
+    {
+        float const den = 1.f / (va + vb + vc);
+        v = vb * den, w = vc * den, u = 1.f - v - w;
+    }
+    float const qx = p.x - (u * a.x + v * b.x + w * c.x);
+    float const qy = p.y - (u * a.y + v * b.y + w * c.y);
+    float const qz = p.z - (u * a.z + v * b.z + w * c.z);
+    return qx * qx + qy * qy + qz * qz;
+}
+
+// Nearest triangles of the step's active vertices (Bvh::NearestNeighbours,
+// gpu/impl/geometry/Bvh.cuh:355-466, as VertexTriangleMixedCcdDcd.cu:117-215
+// uses it): squared distances to the triangles of other bodies, d* the
+// smallest, r the vertex's search radius; with d* < r - eps the triangles
+// within d* + eps, with d* within r +- eps those within r + eps, none
+// otherwise; in triangle order, at most kMaxContacts. finalize = 0
+// (UpdateActiveSet, gpu/impl/vbd/Integrator.cu:250-271): they become the
+// vertex's contacts. finalize = 1 (FinalizeActiveSet): the vertex stays active
+// when it lies below the plane of the last of them. One thread per collision
+// vertex, two passes over LDS tiles of triangle positions; blockDim = kContactTile.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactNearest(
+    ContactArgs c, float4 const* __restrict__ x, int finalize)
+{
+    __shared__ float4 sA[kContactTile], sB[kContactTile], sC[kContactTile];
+    int const v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    bool const mine = v < c.nCV && c.inAv[v] != 0;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    int body = 0;
+    float r = 0.f;
+    if (mine)
+    {
+        p = x[c.cvIdx[v]];
+        body = c.cvBody[v];
+        r = c.dupper[v];
+    }
+    // pass 0: d*; pass 1: collect (contacts straight into fc, or the last one)
+    float dstar = 3.402823466e+38f;
+    bool any = false;
+    float thr = -1.f;
+    int nfound = 0, last = -1;
+    int* out = mine ? c.fc + static_cast<size_t>(c.cvIdx[v]) * kMaxContacts : nullptr;
+    for (int pass = 0; pass < 2; ++pass)
+    {
+        if (pass == 1 && mine && any)
+        {
+            if (dstar < r - kFltEpsilon)
+                thr = dstar + kFltEpsilon;
+            else if (dstar <= r + kFltEpsilon)
+                thr = r + kFltEpsilon;
+        }
+        bool const work = mine && (pass == 0 || thr >= 0.f);
+        for (int base = 0; base < c.nF; base += kContactTile)
+        {
+            int const f = base + static_cast<int>(threadIdx.x);
+            if (f < c.nF)
+            {
+                int4 const t = c.tri[f];
+                float4 a = x[t.x];
+                a.w = __int_as_float(t.w);
+                sA[threadIdx.x] = a;
+                sB[threadIdx.x] = x[t.y];
+                sC[threadIdx.x] = x[t.z];
+            }
+            __syncthreads();
+            int const m = min(kContactTile, c.nF - base);
+            if (work)
+                for (int k = 0; k < m; ++k)
+                {
+                    float4 const a = sA[k];
+                    if (__float_as_int(a.w) == body)
+                        continue;
+                    float const d = PointTriangleSqDist(p, a, sB[k], sC[k]);
+                    if (pass == 0)
+                    {
+                        dstar = any ? fminf(dstar, d) : d;
+                        any = true;
+                    }
+                    else if (d <= thr && nfound < kMaxContacts)
+                    {
+                        if (!finalize)
+                            out[nfound] = base + k;
+                        last = base + k;
+                        ++nfound;
+                    }
+                }
+            __syncthreads();
+        }
+    }
+    if (!mine)
+        return;
+    if (!finalize)
+    {
+        for (int k = nfound; k < kMaxContacts; ++k)
+            out[k] = -1;
+        return;
+    }
+    if (last >= 0)
+    {
+        int4 const t = c.tri[last];
+        float4 const a = x[t.x], b = x[t.y], cc = x[t.z];
+        float3 const n = Cross(Sub(b, a), Sub(cc, a));
+        c.active[v] = Dot(Sub(p, a), n) < 0.f ? 1 : 0;
+    }
+}
+
+// Contact terms of vertex v in the sweep (ContactPenalty, Kernels.cuh:80-114, and
+// AccumulateVertexTriangleContact, sim/vbd/Kernels.h:215-302): the penalty muC is
+// scaled by the vertex area over the summed areas of its contacting triangles,
+// times each triangle's area; the normal penalty 1/2 muC' d^2 on the penetration
+// d = min(0, (x - x_b).n) below a triangle the vertex projects into, and lagged
+// IPC friction on the substep's relative tangential displacement. Triangle
+// positions are read from xm, which holds the positions from before the colour
+// pass (the contact sweep writes to a separate buffer).
+struct ContactSweep
+{
+    int const* __restrict__ fc;
+    int4 const* __restrict__ tri;
+    float const* __restrict__ triArea;
+    float const* __restrict__ xva;
+    float4* __restrict__ xb; // new positions of the colour
+    float muC, muF, epsv;
+};
+
+template <class TBlock>
+__device__ inline void AccumulateContact(float3 xtv, float3 xv, float4 const* xm, float4 const* xt, int4 t, float dt,
+                                         float muC, float muF, float epsv, TBlock& B)
+{
+    float4 const A4 = xm[t.x], B4 = xm[t.y], C4 = xm[t.z];
+    float3 const A = make_float3(A4.x, A4.y, A4.z);
+    float3 const e0 = make_float3(B4.x - A4.x, B4.y - A4.y, B4.z - A4.z);
+    float3 const e1 = make_float3(C4.x - A4.x, C4.y - A4.y, C4.z - A4.z);
+    float3 n = Cross(e0, e1);
+    float const dbl = sqrtf(Dot(n, n));
+    if (dbl <= 1e-8f)
+        return;
+    n = make_float3(n.x / dbl, n.y / dbl, n.z / dbl);
+    float3 ap = make_float3(xv.x - A.x, xv.y - A.y, xv.z - A.z);
+    float const tt = Dot(n, ap);
+    ap = make_float3(ap.x - tt * n.x, ap.y - tt * n.y, ap.z - tt * n.z);
+    float const d00 = Dot(e0, e0), d01 = Dot(e0, e1), d11 = Dot(e1, e1);
+    float const d20 = Dot(ap, e0), d21 = Dot(ap, e1);
+    float const den = d00 * d11 - d01 * d01;
+    float const bv = (d11 * d20 - d01 * d21) / den, bw = (d00 * d21 - d01 * d20) / den;
+    float const bu = 1.f - bv - bw;
+    if (!(bu >= 0.f && bu <= 1.f && bv >= 0.f && bv <= 1.f && bw >= 0.f && bw <= 1.f))
+        return;
+    float3 const xb = make_float3(bu * A4.x + bv * B4.x + bw * C4.x, bu * A4.y + bv * B4.y + bw * C4.y,
+                                  bu * A4.z + bv * B4.z + bw * C4.z);
+    float const dpen = fminf(0.f, (xv.x - xb.x) * n.x + (xv.y - xb.y) * n.y + (xv.z - xb.z) * n.z);
+    float const lambda = muC * dpen;
+    B.g0 += lambda * n.x, B.g1 += lambda * n.y, B.g2 += lambda * n.z;
+    B.h00 += muC * n.x * n.x, B.h01 += muC * n.x * n.y, B.h02 += muC * n.x * n.z;
+    B.h11 += muC * n.y * n.y, B.h12 += muC * n.y * n.z, B.h22 += muC * n.z * n.z;
+    // friction in the tangent frame (e0, n x e0)
+    float4 const tA = xt[t.x], tB = xt[t.y], tC = xt[t.z];
+    float3 const xtb = make_float3(bu * tA.x + bv * tB.x + bw * tC.x, bu * tA.y + bv * tB.y + bw * tC.y,
+                                   bu * tA.z + bv * tB.z + bw * tC.z);
+    float3 const b1 = Cross(n, e0);
+    float3 const du = make_float3((xv.x - xtv.x) - (xb.x - xtb.x), (xv.y - xtv.y) - (xb.y - xtb.y),
+                                  (xv.z - xtv.z) - (xb.z - xtb.z));
+    float const u0 = Dot(e0, du), u1 = Dot(b1, du);
+    float const unorm = sqrtf(u0 * u0 + u1 * u1) + kFltEpsilon;
+    float const epsvh = epsv * dt;
+    float const y = unorm / epsvh;
+    float const f1 = (y < 1.f) ? 2.f * y - y * y : 1.f;
+    float const c1 = muF * fabsf(lambda) * f1 / unorm;
+    B.g0 += c1 * (e0.x * u0 + b1.x * u1);
+    B.g1 += c1 * (e0.y * u0 + b1.y * u1);
+    B.g2 += c1 * (e0.z * u0 + b1.z * u1);
+    B.h00 += c1 * (e0.x * e0.x + b1.x * b1.x), B.h01 += c1 * (e0.x * e0.y + b1.x * b1.y);
+    B.h02 += c1 * (e0.x * e0.z + b1.x * b1.z), B.h11 += c1 * (e0.y * e0.y + b1.y * b1.y);
+    B.h12 += c1 * (e0.y * e0.z + b1.y * b1.z), B.h22 += c1 * (e0.z * e0.z + b1.z * b1.z);
+}
+
+// xm[v] = xb[v] over a colour's range, after its contact sweep.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void CopyColour(float4* __restrict__ xm,
+                                                                          float4 const* __restrict__ xb, int vbeg,
+                                                                          int vend)
+{
+    int const v = vbeg + static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v < vend)
+        xm[v] = xb[v];
+}
+
+} // namespace pbat_vbd::device

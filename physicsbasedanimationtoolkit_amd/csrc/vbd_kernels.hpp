@@ -20,6 +20,8 @@
 
 #include <cstdint>
 
+#include "vbd_contact_kernels.hpp"
+
 namespace pbat_vbd::device {
 
 // ---- cross-lane sums over groups of L consecutive lanes (L | 64) ----------
@@ -137,10 +139,8 @@ struct StepConstants
     float detHZero; // skip threshold on |det H|
 };
 
-// Damping, inertia and the 3x3 Newton step of one vertex
-// (sim/vbd/Kernels.h:173-191, 304-339). Returns the displacement to subtract.
-__device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 const& xl, float4 const& xtv,
-                                             StepConstants const& c)
+// Rayleigh damping of the elastic block (sim/vbd/Kernels.h:173-191).
+__device__ __forceinline__ void Damp(Block& B, float4 const& xv, float4 const& xtv, StepConstants const& c)
 {
     if (c.kD != 0.f)
     {
@@ -152,6 +152,24 @@ __device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 c
         float const s = 1.f + D;
         B.h00 *= s, B.h01 *= s, B.h02 *= s, B.h11 *= s, B.h12 *= s, B.h22 *= s;
     }
+}
+
+__device__ __forceinline__ float3 InertiaNewtonStep(Block B, float4 const& xv, float4 const& xl,
+                                                    StepConstants const& c);
+
+// Damping, inertia and the 3x3 Newton step of one vertex
+// (sim/vbd/Kernels.h:173-191, 304-339). Returns the displacement to subtract.
+__device__ __forceinline__ float3 NewtonStep(Block B, float4 const& xv, float4 const& xl, float4 const& xtv,
+                                             StepConstants const& c)
+{
+    Damp(B, xv, xtv, c);
+    return InertiaNewtonStep(B, xv, xl, c);
+}
+
+// Inertia and the 3x3 Newton step (sim/vbd/Kernels.h:304-339).
+__device__ __forceinline__ float3 InertiaNewtonStep(Block B, float4 const& xv, float4 const& xl,
+                                                    StepConstants const& c)
+{
     float const K = xv.w / c.dt2;
     B.h00 += K, B.h11 += K, B.h22 += K;
     B.g0 += K * (xv.x - xl.x);
@@ -208,6 +226,7 @@ struct SweepArgs
     int ntStream;  // != 0: read the incidence records with non-temporal loads
     int ellStride; // > 0: fixed-stride rows of this many records
     StepConstants c;
+    ContactSweep contact; // contact sweeps only
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -252,7 +271,11 @@ __device__ __forceinline__ float2 LoadConst(float2 const* p, bool nt)
 // The group reduces its 9 partial sums with DPP and its first lane applies the
 // Newton update in place: vertices of one colour share no tet, so none reads
 // what another vertex of the colour writes.
-template <int L, int K>
+// With C (scenes in contact), the vertex's contact terms join its block after
+// damping and before inertia (gpu/impl/vbd/Kernels.cuh:200-230), and the new
+// positions go to contact.xb, so contact terms of the colour read positions from
+// before its pass; the host copies them back after the launch.
+template <int L, int K, bool C = false>
 __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 {
     int block = static_cast<int>(blockIdx.x);
@@ -339,6 +362,33 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     GroupSum<L>(B);
     if (!active || lane != 0)
         return;
+    if constexpr (C)
+    {
+        float4 const xto = a.xt[v];
+        Damp(B, own, xto, a.c);
+        ContactSweep const& cs = a.contact;
+        int4 const fa = *reinterpret_cast<int4 const*>(cs.fc + static_cast<size_t>(v) * kMaxContacts);
+        if (fa.x >= 0)
+        {
+            int4 const fb = *reinterpret_cast<int4 const*>(cs.fc + static_cast<size_t>(v) * kMaxContacts + 4);
+            int const f[kMaxContacts] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
+            int n = 0;
+#pragma unroll
+            for (int k = 0; k < kMaxContacts; ++k)
+                n += f[k] >= 0 ? 1 : 0;
+            float sumfa = 0.f;
+            for (int k = 0; k < n; ++k)
+                sumfa += cs.triArea[f[k]];
+            float const kC = cs.xva[v] * cs.muC / sumfa;
+            float3 const xo = make_float3(own.x, own.y, own.z), xt3 = make_float3(xto.x, xto.y, xto.z);
+            for (int k = 0; k < n; ++k)
+                AccumulateContact(xt3, xo, a.xm, a.xt, cs.tri[f[k]], a.c.dt, kC * cs.triArea[f[k]], cs.muF, cs.epsv,
+                                  B);
+        }
+        float3 const dx = InertiaNewtonStep(B, own, xl, a.c);
+        cs.xb[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
+        return;
+    }
     float3 const dx = NewtonStep(B, own, xl, (a.c.kD != 0.f) ? xtv : own, a.c);
     a.xm[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
 }

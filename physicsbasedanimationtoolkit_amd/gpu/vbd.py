@@ -85,6 +85,20 @@ def _marshal(data: Data):
     d.rho = float(data.rho)
     d.window = int(data.manderson)
     d.eta, d.tau, d.curved = float(data.eta), float(data.tau), int(bool(data.curved))
+    # collision mesh and contact model (sim/vbd/Data.h:165-178, 225-229)
+    if data.V.size and data.F.size:
+        keep["V"] = np.ascontiguousarray(data.V, dtype=np.int64).reshape(-1)
+        keep["F"] = np.ascontiguousarray(np.asarray(data.F, dtype=np.int64).T)
+        keep["B"] = np.ascontiguousarray(data.B, dtype=np.int64).reshape(-1)
+        keep["XVA"] = np.ascontiguousarray(data.XVA, dtype=np.float64)
+        keep["FA"] = np.ascontiguousarray(data.FA, dtype=np.float64)
+        if keep["B"].size != nV or keep["XVA"].size != nV or keep["FA"].size != keep["F"].shape[0]:
+            raise ValueError("B and XVA must have |#vertices| entries and FA one per collision triangle")
+        d.nCollisionVertices, d.V = keep["V"].size, _native.ptr(keep["V"], i64)
+        d.nCollisionTriangles, d.F = keep["F"].shape[0], _native.ptr(keep["F"], i64)
+        d.B, d.XVA, d.FA = _native.ptr(keep["B"], i64), _native.ptr(keep["XVA"], f64), _native.ptr(keep["FA"], f64)
+    d.muC, d.muF, d.epsv = float(data.muC), float(data.muF), float(data.epsv)
+    d.activeSetUpdateFrequency = int(data.active_set_update_frequency)
     return d, keep
 
 
@@ -96,6 +110,7 @@ class Integrator:
         d, keep = _marshal(data)
         self._nV = d.nV
         self._nT = d.nT
+        self._nCV = d.nCollisionVertices
         handle = ctypes.c_void_p()
         _native.check(self._lib.pbat_vbd_integrator_create(ctypes.byref(d), ctypes.byref(handle)))
         del keep
@@ -239,6 +254,16 @@ class Integrator:
                                                            ctypes.byref(i)))
         return p.value, u.value, i.value
 
+    def contacts(self):
+        """(active, fc) after the last step: the active flag of each collision vertex (in
+        the order of data.V) and up to 8 contacting triangles per vertex (|#vertices| x 8,
+        -1 padded). Raises ValueError when the integrator runs no contact."""
+        active = np.zeros(max(self._nCV, 1), dtype=np.int32)
+        fc = np.zeros((self._nV, 8), dtype=np.int32)
+        _native.check(self._lib.pbat_vbd_integrator_contacts(self._h, _native.ptr(active, ctypes.c_int32),
+                                                             _native.ptr(fc, ctypes.c_int32)))
+        return active[: self._nCV].astype(bool), fc
+
     def trust_region_log(self):
         """[(t, accepted)] of the linear trust-region path in the last step: one entry per
         iteration k >= 2 of each substep (the reference logs the same through its
@@ -357,6 +382,7 @@ class BatchIntegrator(Integrator):
             raise ValueError("BatchIntegrator needs at least one Data")
         views = [_marshal(d) for d in datas]
         self._nT = int(sum(v[0].nT for v in views))
+        self._nCV = 0
         arr = (_native.Data * len(views))(*[v[0] for v in views])
         handle = ctypes.c_void_p()
         _native.check(self._lib.pbat_vbd_batch_create(arr, len(views), ctypes.byref(handle)))

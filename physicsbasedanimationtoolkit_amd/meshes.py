@@ -86,6 +86,17 @@ def signed_volumes(X: np.ndarray, E: np.ndarray) -> np.ndarray:
     return np.linalg.det(np.moveaxis(D, 2, 0)) / 6.0
 
 
+def boundary_triangles(E: np.ndarray) -> np.ndarray:
+    """Boundary faces of a tet mesh with positively oriented tets, ``3 x nF``, each
+    ordered so that cross(b - a, c - a) points out of its tet."""
+    a, b, c, d = E
+    faces = np.concatenate([np.stack([a, c, b]), np.stack([a, b, d]), np.stack([a, d, c]), np.stack([b, c, d])],
+                           axis=1)
+    key = np.sort(faces, axis=0)
+    _, inv, counts = np.unique(key, axis=1, return_inverse=True, return_counts=True)
+    return np.ascontiguousarray(faces[:, counts[inv.reshape(-1)] == 1])
+
+
 def bottom_vertices(X: np.ndarray, axis: int = 1) -> np.ndarray:
     """Indices of the vertices on the minimum-coordinate face along ``axis``."""
     lo = X[axis].min()

@@ -1,0 +1,207 @@
+"""Pins of the oracle's vertex-triangle contact (oracle/vbd_oracle.c, oracle/oracle.py):
+closest points against an exhaustive candidate search, the contact gradient and
+Hessian against finite differences of its energy, the active-set known answer of the
+reference's two-tetrahedron test, and whole contact steps against invariants."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from physicsbasedanimationtoolkit_amd import meshes, pbat
+
+DT = 1.0 / 60.0
+
+
+def _closest_exhaustive(p, a, b, c):
+    """min over the plane projection (if inside) and the three clamped edge projections."""
+    cands = [a, b, c]
+    for u, v in ((a, b), (b, c), (c, a)):
+        t = np.clip(np.dot(p - u, v - u) / np.dot(v - u, v - u), 0.0, 1.0)
+        cands.append(u + t * (v - u))
+    n = np.cross(b - a, c - a)
+    q = p - np.dot(p - a, n) / np.dot(n, n) * n
+    M = np.stack([b - a, c - a], axis=1)
+    st = np.linalg.lstsq(M, q - a, rcond=None)[0]
+    if st.min() >= 0 and st.sum() <= 1:
+        cands.append(q)
+    return min(float(np.dot(p - x, p - x)) for x in cands)
+
+
+def test_point_triangle_distance_matches_exhaustive_search():
+    rng = np.random.default_rng(0)
+    for _ in range(500):
+        a, b, c = rng.normal(size=(3, 3))
+        p = rng.normal(size=3) * 2
+        d = O.point_triangle_sqdist(p, a, b, c)
+        assert abs(d - _closest_exhaustive(p, a, b, c)) <= 1e-12 * max(1.0, d)
+
+
+def _tri():
+    xf = np.array([[0.0, 1.2, 0.1], [0.0, 0.05, 1.1], [0.02, -0.01, 0.03]])  # columns = vertices
+    return xf
+
+
+def test_normal_penalty_gradient_and_hessian_match_fd():
+    """muF = 0: E = 1/2 muC d^2 for a vertex below the triangle, g = dE/dx, H = dg/dx."""
+    xf = _tri()
+    xtf = xf - 0.001
+    xv = np.array([0.3, 0.25, -0.02])
+    xtv = xv + np.array([0.0, 0.0, 0.03])
+    muC = 1e5
+    E, g, H = O.vertex_triangle_contact(xtv, xv, xtf, xf, DT, muC, 0.0, 1e-3)
+    assert E > 0
+    eps = 1e-6
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = eps
+        Ep, gp, _ = O.vertex_triangle_contact(xtv, xv + e, xtf, xf, DT, muC, 0.0, 1e-3)
+        Em, gm, _ = O.vertex_triangle_contact(xtv, xv - e, xtf, xf, DT, muC, 0.0, 1e-3)
+        assert abs((Ep - Em) / (2 * eps) - g[k]) <= 1e-6 * np.abs(g).max()
+        np.testing.assert_allclose((gp - gm) / (2 * eps), H[:, k], rtol=0, atol=1e-6 * np.abs(H).max())
+
+
+def test_friction_gradient_is_the_lagged_energy_gradient():
+    """With the normal force lambda held fixed (IPC's lagged friction), the friction part
+    of g is the gradient of muF |lambda| f0(|u|): FD of the friction energy, minus the
+    part that comes from lambda = muC d changing (d(|lambda|)/dx = -muC n below the
+    triangle), in the smooth regime |u| < epsv h."""
+    xf = _tri()
+    xtf = xf.copy()
+    xv = np.array([0.3, 0.25, -0.02])
+    xtv = xv - np.array([2e-6, -1e-6, 0.0])  # small tangential slip
+    muC, muF, epsv = 1e5, 0.4, 1e-2
+    n = np.cross(xf[:, 1] - xf[:, 0], xf[:, 2] - xf[:, 0])
+    n /= np.linalg.norm(n)
+
+    def parts(x):
+        E1, g1, _ = O.vertex_triangle_contact(xtv, x, xtf, xf, DT, muC, muF, epsv)
+        E0, g0, _ = O.vertex_triangle_contact(xtv, x, xtf, xf, DT, muC, 0.0, epsv)
+        return E1 - E0, g1 - g0, E0
+
+    Ef, gf, E0 = parts(xv)
+    lam = np.sqrt(2 * E0 * muC)  # |lambda| = muC |d|, E0 = 1/2 muC d^2
+    assert Ef > 0 and np.linalg.norm(gf) > 0
+    eps = 1e-8
+    fd = np.zeros(3)
+    for k in range(3):
+        e = np.zeros(3)
+        e[k] = eps
+        fd[k] = (parts(xv + e)[0] - parts(xv - e)[0]) / (2 * eps)
+    expected = fd + (Ef / lam) * muC * n
+    np.testing.assert_allclose(gf, expected, rtol=1e-5, atol=1e-6 * np.abs(gf).max())
+
+
+def test_contact_known_answers():
+    xf = np.array([[0.0, 1.0, 0.0], [0.0, 0.0, 1.0], [0.0, 0.0, 0.0]])
+    n = np.array([0.0, 0.0, 1.0])
+    muC = 2e4
+    # above the plane, projecting inside: no force, the Hessian still gains muC n n^T
+    E, g, H = O.vertex_triangle_contact([0.2, 0.2, 0.1], [0.2, 0.2, 0.1], xf, xf, DT, muC, 0.3, 1e-3)
+    assert E == 0 and np.all(g == 0)
+    np.testing.assert_allclose(H, muC * np.outer(n, n))
+    # projecting outside the triangle: nothing
+    E, g, H = O.vertex_triangle_contact([0.9, 0.9, -0.1], [0.9, 0.9, -0.1], xf, xf, DT, muC, 0.3, 1e-3)
+    assert E == 0 and np.all(g == 0) and np.all(H == 0)
+    # degenerate triangle: nothing
+    E, g, H = O.vertex_triangle_contact([0.2, 0.2, -0.1], [0.2, 0.2, -0.1], xf * [[1], [0], [0]], xf * [[1], [0], [0]],
+                                        DT, muC, 0.3, 1e-3)
+    assert E == 0 and np.all(H == 0)
+    # below, no slip: E = 1/2 muC d^2 + muF muC |d| epsv h / 3, g = muC d n
+    d = -0.01
+    E, g, H = O.vertex_triangle_contact([0.2, 0.2, d], [0.2, 0.2, d], xf, xf, DT, muC, 0.3, 1e-3)
+    assert abs(E - (0.5 * muC * d * d + 0.3 * muC * abs(d) * 1e-3 * DT / 3)) <= 1e-9 * E
+    np.testing.assert_allclose(g, muC * d * n, atol=1e-9)
+
+
+def two_tet_scene():
+    """gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:221-289: the bottom tet's top vertex
+    passes through the top tet's bottom face."""
+    XT = np.array([[0.0, 1.0, 0.0, 0.1, 0.0, 1.0, 0.0, 0.1],
+                   [0.0, 0.0, 1.0, 0.1, 0.0, 0.0, 1.0, 0.1],
+                   [0.0, 0.0, 0.0, 1.0, 1.01, 1.01, 1.01, 2.01]])
+    T = np.array([[0, 4], [1, 5], [2, 6], [3, 7]])
+    F = np.array([[0, 1, 2, 0, 4, 5, 6, 4], [1, 2, 0, 2, 5, 6, 4, 6], [3, 3, 3, 1, 7, 7, 7, 5]])
+    X = XT.copy()
+    X[2, :4] += 0.01
+    X[2, 4:] -= 0.01
+    B = np.array([0, 0, 0, 0, 1, 1, 1, 1])
+    return XT, X, T, F, B
+
+
+def test_active_set_known_answer():
+    XT, X, T, F, B = two_tet_scene()
+    o = O.OracleVbd(XT, T, contact=dict(V=np.arange(8), F=F, B=B, XVA=np.ones(8), FA=np.ones(8)))
+    o.initialize_active_set(XT.T.copy(), X.T.copy())
+    # the bottom tet's top vertex and the top tet's three bottom-face vertices
+    assert o.in_av.sum() == 4 and o.in_av[3]
+    assert set(np.nonzero(o.in_av)[0]) == {3, 4, 5, 6}
+    o.update_active_set(X.T.copy())
+    for v in range(8):
+        assert (o.fc[v] >= 0).any() == bool(o.in_av[v])
+    # the penetrating vertex 3 is nearest to the top tet's bottom face (triangle 7)
+    assert list(o.fc[3][o.fc[3] >= 0]) == [7]
+    o.finalize_active_set(X.T.copy())
+    assert o.cactive[3]
+
+
+def cube_on_slab(n=3, gap=0.002, speed=1.0, offset=(0.13, 0.0, 0.17), seed=0):
+    """A jittered n^3 cube (body 1) moving down onto a fixed 1 x 0.2 x 1 slab (body 0)
+    whose grid does not line up with the cube's."""
+    Xs, Es = meshes.cube_tet_grid(4, 1, 4, size=1.0)
+    Xs = Xs * np.array([[1.0], [0.2], [1.0]]) - np.array([[0.0], [0.2], [0.0]])
+    Xc, Ec = meshes.cube_tet_grid(n, jitter=0.1, seed=seed, size=0.5)
+    Xc = Xc + np.array([[offset[0]], [gap], [offset[2]]])
+    X = np.concatenate([Xs, Xc], axis=1)
+    E = np.concatenate([Es, Ec + Xs.shape[1]], axis=1)
+    B = np.r_[np.zeros(Xs.shape[1], np.int64), np.ones(Xc.shape[1], np.int64)]
+    F = meshes.boundary_triangles(E)
+    V = np.unique(F)
+    v = np.zeros(X.shape)
+    v[1, Xs.shape[1]:] = -speed
+    dbc = np.arange(Xs.shape[1])
+    return X, E, F, V, B, v, dbc
+
+
+def _contact_oracle(X, E, F, V, B, v, dbc, muC=1e6, muF=0.3, epsv=1e-3):
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B))
+    return O.OracleVbd(X, E, v=v, aext=meshes.gravity(X.shape[1]), mue=mu, lambdae=lam, dbc=dbc,
+                       contact=dict(V=V, F=F, B=B, XVA=data.XVA, FA=data.FA, muC=muC, muF=muF, epsv=epsv))
+
+
+def test_contact_stops_the_cube_on_the_slab():
+    """Without contact the cube falls through the slab; with contact (a penalty: the
+    cube sinks in and springs back) its lowest vertex stays within 3 cm of the slab's
+    top face, the cube's velocity turns upward, and the slab never moves."""
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    o = _contact_oracle(X, E, F, V, B, v, dbc)
+    free = O.OracleVbd(X, E, v=v, aext=meshes.gravity(X.shape[1]), dbc=dbc)
+    cube = B == 1
+    lowest, active, vy = [], [], []
+    for _ in range(12):
+        o.step_contact(DT, 20, 1)
+        free.step(DT, 20, 1)
+        lowest.append(o.positions[1, cube].min())
+        active.append(o.in_av.sum())
+        vy.append(o.velocities[1, cube].mean())
+    assert free.positions[1, cube].min() < -0.1
+    assert min(lowest) > -0.03
+    assert max(active) > 0 and max(vy) > 0
+    np.testing.assert_array_equal(o.positions[:, ~cube], X[:, ~cube])
+
+
+def test_single_body_contact_is_inert():
+    """One body: no triangle of another body, no contact: bitwise plain VBD (the
+    reference's detector finds nothing for a single body)."""
+    X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=2)
+    F = meshes.boundary_triangles(E)
+    V = np.unique(F)
+    B = np.zeros(X.shape[1], np.int64)
+    v = np.random.default_rng(1).normal(size=X.shape) * 0.3
+    a = _contact_oracle(X, E, F, V, B, v, meshes.bottom_vertices(X))
+    b = O.OracleVbd(X, E, v=v, aext=meshes.gravity(X.shape[1]), mue=a.lame[:, 0], lambdae=a.lame[:, 1],
+                    dbc=meshes.bottom_vertices(X))
+    a.step_contact(DT, 10, 2)
+    b.step(DT, 10, 2)
+    np.testing.assert_array_equal(a.positions, b.positions)
+    assert not a.cactive.any()

@@ -1,0 +1,143 @@
+"""GPU vertex-triangle contact against the FP64 oracle (through the C ABI).
+
+Active sets and contact lists are compared exactly (same vertices, same triangles);
+positions within CONTACT_POS_TOL of the oracle stepping from the device's state."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from physicsbasedanimationtoolkit_amd import meshes, pbat
+from test_contact_oracle import DT, cube_on_slab, two_tet_scene
+
+pytestmark = pytest.mark.gpu
+
+CONTACT_POS_TOL = 1e-5
+
+
+def contact_pair(X, E, F, V, B, v, dbc, aext=None, muC=1e6, muF=0.3, epsv=1e-3, frequency=1):
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+    aext = meshes.gravity(X.shape[1]) if aext is None else aext
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B)
+            .with_material(np.full(nT, 1e3), mu, lam).with_acceleration(aext).with_velocity(v)
+            .with_contact_parameters(muC, muF, epsv).with_active_set_update_frequency(frequency))
+    if dbc is not None and len(dbc):
+        data = data.with_dirichlet_vertices(dbc)
+    data = data.construct()
+    gpu = pbat.gpu.vbd.Integrator(data)
+    ref = O.OracleVbd(X, E, v=v, aext=aext, rhoe=np.full(nT, 1e3), mue=mu, lambdae=lam, dbc=dbc,
+                      contact=dict(V=V, F=F, B=B, XVA=data.XVA, FA=data.FA, muC=muC, muF=muF, epsv=epsv,
+                                   frequency=frequency))
+    return gpu, ref
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def test_two_tet_active_set_matches_oracle(gpu_device):
+    """The reference's detector test (VertexTriangleMixedCcdDcd.cu:221-289) run as one
+    step with dt = 1 and velocities that carry XT to X: the same active vertices and
+    the same contact triangles as the oracle, which matches the reference's counts."""
+    XT, X, T, F, B = two_tet_scene()
+    v = X - XT
+    gpu, ref = contact_pair(XT, T, F, np.arange(8), B, v, None, aext=np.zeros_like(XT))
+    gpu.step(1.0, 0, 1)
+    ref.step_contact(1.0, 0, 1)
+    assert ref.in_av.sum() == 4 and set(np.nonzero(ref.in_av)[0]) == {3, 4, 5, 6}
+    active, fc = gpu.contacts()
+    np.testing.assert_array_equal(active, ref.cactive)
+    np.testing.assert_array_equal(fc, ref.fc)
+    assert list(fc[3][fc[3] >= 0]) == [7]
+
+
+def test_cube_on_slab_matches_oracle(gpu_device):
+    """A jittered 3^3 cube dropped at 1 m/s onto a fixed slab, 12 steps x 20 iterations:
+    each step from the device's state, the oracle finds the same active vertices and
+    contact triangles and lands within CONTACT_POS_TOL."""
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    gpu, ref = contact_pair(X, E, F, V, B, v, dbc)
+    touched = 0
+    for step in range(12):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 20, 1)
+        ref.step_contact(DT, 20, 1)
+        active, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        np.testing.assert_array_equal(active, ref.cactive)
+        touched += int(ref.in_av.sum())
+        e = rel(gpu.x, ref.positions)
+        print(f"step {step}: active {int(ref.in_av.sum())}  rel pos err {e:.3e}")
+        assert e <= CONTACT_POS_TOL
+    assert touched > 0
+
+
+def test_contact_graph_replay_is_bitwise(gpu_device):
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    g1, _ = contact_pair(X, E, F, V, B, v, dbc)
+    g2, _ = contact_pair(X, E, F, V, B, v, dbc)
+    g2.graph_capture = False
+    for _ in range(6):
+        g1.step(DT, 10, 2)
+        g2.step(DT, 10, 2)
+        assert np.array_equal(g1.x, g2.x)
+        a1, f1 = g1.contacts()
+        a2, f2 = g2.contacts()
+        assert np.array_equal(a1, a2) and np.array_equal(f1, f2)
+
+
+def test_active_set_update_frequency(gpu_device):
+    """frequency 2 with 3 substeps refreshes the contacts at substeps 0 and 2
+    (gpu/impl/vbd/Integrator.cu:96)."""
+    X, E, F, V, B, v, dbc = cube_on_slab(gap=0.001)
+    gpu, ref = contact_pair(X, E, F, V, B, v, dbc, frequency=2)
+    for _ in range(4):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 10, 3)
+        ref.step_contact(DT, 10, 3)
+        _, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        assert rel(gpu.x, ref.positions) <= CONTACT_POS_TOL
+
+
+def test_single_body_runs_no_contact(gpu_device):
+    """One body: the reference's detector finds nothing; the device sets nothing up and
+    steps bitwise like an integrator without a collision mesh."""
+    X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=2)
+    F = meshes.boundary_triangles(E)
+    V = np.unique(F)
+    B = np.zeros(X.shape[1], np.int64)
+    v = np.random.default_rng(1).normal(size=X.shape) * 0.3
+    g1, _ = contact_pair(X, E, F, V, B, v, meshes.bottom_vertices(X))
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(np.full(nT, 1e3), mu, lam)
+            .with_acceleration(meshes.gravity(X.shape[1])).with_velocity(v)
+            .with_dirichlet_vertices(meshes.bottom_vertices(X)).construct())
+    g2 = pbat.gpu.vbd.Integrator(data)
+    for _ in range(3):
+        g1.step(DT, 10, 2)
+        g2.step(DT, 10, 2)
+    np.testing.assert_array_equal(g1.x, g2.x)
+    with pytest.raises(ValueError):
+        g1.contacts()
+
+
+def test_contact_unsupported_combinations_raise(gpu_device):
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    nT = E.shape[1]
+    base = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B)
+            .with_dirichlet_vertices(dbc))
+    data = base.construct()
+    with pytest.raises(ValueError):
+        pbat.gpu.vbd.BatchIntegrator([data, data])
+    g = pbat.gpu.vbd.Integrator(data)
+    with pytest.raises(ValueError):
+        g.rollout_forward(DT, 5, 2)
+    tr = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B)
+          .with_trust_region_acceleration(0.2, 2.0, False).construct())
+    with pytest.raises(ValueError):
+        pbat.gpu.vbd.Integrator(tr)
+    assert nT > 0
