@@ -7,7 +7,7 @@ import pytest
 
 from oracle import oracle as O
 from physicsbasedanimationtoolkit_amd import meshes, pbat
-from test_contact_oracle import DT, cube_on_slab, two_tet_scene
+from test_contact_oracle import DT, cube_on_slab, two_cubes, two_tet_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -141,3 +141,42 @@ def test_contact_unsupported_combinations_raise(gpu_device):
     with pytest.raises(ValueError):
         pbat.gpu.vbd.Integrator(tr)
     assert nT > 0
+
+
+def test_colliding_cubes_match_oracle(gpu_device):
+    """Two free cubes colliding (contacts both ways, vertices of one colour coupled
+    across bodies): per step from the device's state, the same active sets and
+    contacts and positions within CONTACT_POS_TOL of the oracle."""
+    X, E, F, V, B, v = two_cubes()
+    gpu, ref = contact_pair(X, E, F, V, B, v, None, aext=np.zeros_like(X))
+    seen = 0
+    for step in range(8):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 20, 1)
+        ref.step_contact(DT, 20, 1)
+        active, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        np.testing.assert_array_equal(active, ref.cactive)
+        seen += int(ref.in_av.sum())
+        e = rel(gpu.x, ref.positions)
+        print(f"step {step}: active {int(ref.in_av.sum())}  rel pos err {e:.3e}")
+        assert e <= CONTACT_POS_TOL
+    assert seen > 0
+
+
+def test_sliding_cube_with_friction_matches_oracle(gpu_device):
+    """A cube sliding at 1 m/s on the slab (friction 0.5), 8 steps of 2 substeps."""
+    X, E, F, V, B, v, dbc = cube_on_slab(gap=0.0005, speed=0.0)
+    v[0, B == 1] = 1.0
+    gpu, ref = contact_pair(X, E, F, V, B, v, dbc, muF=0.5)
+    for step in range(8):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 15, 2)
+        ref.step_contact(DT, 15, 2)
+        _, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        e = rel(gpu.x, ref.positions)
+        print(f"step {step}: rel pos err {e:.3e}")
+        assert e <= CONTACT_POS_TOL
