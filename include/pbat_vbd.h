@@ -181,8 +181,9 @@ int pbat_vbd_integrator_set_initialization_strategy(pbat_vbd_integrator* h, int3
 /* gpu::vbd::Integrator::SetBlockSize (gpu/vbd/Integrator.cu:108-111; clamp as gpu/impl/vbd/Integrator.cu:158-161).
  * Threads per workgroup of the sweep kernel; clamped to [64, 256] and rounded down to a multiple of 64. */
 int pbat_vbd_integrator_set_block_size(pbat_vbd_integrator* h, int32_t block_size);
-/* gpu::vbd::Integrator::SetSceneBoundingBox (gpu/vbd/Integrator.cu:113-116). Stored only: the
- * bounding box feeds the contact detector, which single-body VBD does not use. */
+/* gpu::vbd::Integrator::SetSceneBoundingBox (gpu/vbd/Integrator.cu:113-116). Stored only: in
+ * the reference the box bounds the Morton codes of the contact detector's BVH; the
+ * exhaustive detector here needs no such bounds. */
 int pbat_vbd_integrator_set_scene_bounding_box(pbat_vbd_integrator* h, const float min[3], const float max[3]);
 
 /* Lanes of a wavefront that cooperate on one vertex (1, 2, 4, 8, 16, 32 or 64;
