@@ -19,7 +19,10 @@ The JSON line also carries
   batched: 256 meshes stepped together vs the plain per-mesh loop;
   adjoint: fwd+adjoint steps/s of a differentiable 20-step rollout on a 32^3 mesh;
   batched_adjoint: fwd+adjoint steps/s of 64 meshes over 200 steps vs the
-            plain per-mesh loop, with the peak device memory.
+            plain per-mesh loop, with the peak device memory;
+  variants: Chebyshev, Anderson and trust-region steps on the 64^3 workload
+            (time and distance to the step's minimiser), and contact on 8
+            cubes over a slab (time per step with and without contact).
 """
 from __future__ import annotations
 
@@ -264,6 +267,113 @@ def bench_batched_adjoint(args, torch):
             "solver": solver, "adjoint_stats": rb["stats"], "sweep_shape": shape}
 
 
+def _time_steps(g, dt, iters, substeps, steps, torch):
+    for _ in range(2):
+        g.step(dt, iters, substeps)
+    g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(dt, iters, substeps)
+    g.synchronize()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def bench_variants(args, torch):
+    """Accelerated integrators on the 64^3 workload (one step from rest, 20 iterations):
+    time per step and the distance of the result to the step's minimiser (plain VBD
+    with 400 iterations), next to plain VBD; and vertex-triangle contact on 8 jittered
+    12^3 cubes falling onto a fixed slab: time per step with and without contact."""
+    import numpy as np
+
+    from physicsbasedanimationtoolkit_amd import meshes, pbat
+
+    s = meshes.plain_gpu_scene(args.grid, seed=0)
+    dt, iters = s.dt, args.iterations
+
+    def data(**acc):
+        d = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+             .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc))
+        if "rho" in acc:
+            d = d.with_chebyshev_acceleration(acc["rho"])
+        if "window" in acc:
+            d = d.with_anderson_acceleration(acc["window"])
+        if "eta" in acc:
+            d = d.with_trust_region_acceleration(acc["eta"], 2.0, False)
+        return d.construct()
+
+    g = pbat.gpu.vbd.Integrator(data())
+    g.step(dt, 400, 1)
+    x_star = g.x.astype(np.float64)
+    x0 = s.X.astype(np.float64)
+    out = {"workload": f"{args.grid}^3 plain workload, one step from rest, {iters} iterations; "
+                       "error = |x - x*| / |x* - x0|, x* = plain VBD with 400 iterations"}
+    for name, acc in (("plain", {}), ("chebyshev_rho_0.9", {"rho": 0.9}), ("anderson_m5", {"window": 5}),
+                      ("trust_region_linear", {"eta": 0.2})):
+        g = pbat.gpu.vbd.Integrator(data(**acc))
+        g.step(dt, iters, 1)
+        err = float(np.linalg.norm(g.x - x_star) / np.linalg.norm(x_star - x0))
+        reps = 10
+        t = 0.0
+        for _ in range(reps):
+            g.x = s.X.astype(np.float32)
+            g.v = np.zeros_like(s.X, dtype=np.float32)
+            g.synchronize()
+            t0 = time.perf_counter()
+            g.step(dt, iters, 1)
+            g.synchronize()
+            t += time.perf_counter() - t0
+        out[name] = {"ms_per_step": 1e3 * t / reps, "error_vs_minimiser": err}
+        log(f"[bench] {name}: {1e3 * t / reps:.2f} ms/step, error vs minimiser {err:.3e}")
+
+    # contact: 8 cubes in a 2 x 2 x 2 arrangement above a fixed slab
+    n = 12
+    parts, elems, bodies = [], [], []
+    Xs, Es = meshes.cube_tet_grid(8, 1, 8)
+    Xs = Xs * np.array([[1.2], [0.1], [1.2]]) - np.array([[0.1], [0.1], [0.1]])
+    parts.append(Xs)
+    elems.append(Es)
+    bodies.append(np.zeros(Xs.shape[1], np.int64))
+    off = Xs.shape[1]
+    for b, (i, j, k) in enumerate(np.ndindex(2, 2, 2)):
+        Xc, Ec = meshes.cube_tet_grid(n, jitter=0.1, seed=b, size=0.45)
+        Xc = Xc + np.array([[0.5 * i + 0.013 * b], [0.004 + 0.47 * j], [0.5 * k + 0.007 * b]])
+        parts.append(Xc)
+        elems.append(Ec + off)
+        bodies.append(np.full(Xc.shape[1], b + 1, np.int64))
+        off += Xc.shape[1]
+    X = np.concatenate(parts, axis=1)
+    E = np.concatenate(elems, axis=1)
+    B = np.concatenate(bodies)
+    F = meshes.boundary_triangles(E)
+    V = np.unique(F)
+    v = np.zeros(X.shape)
+    v[1, B > 0] = -0.5
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+    base = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(np.full(nT, 1e3), mu, lam)
+            .with_acceleration(meshes.gravity(X.shape[1])).with_velocity(v)
+            .with_dirichlet_vertices(np.arange(Xs.shape[1])))
+    with_contact = pbat.gpu.vbd.Integrator(base.with_surface_mesh(V, F).with_bodies(B).construct())
+    base2 = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(np.full(nT, 1e3), mu, lam)
+             .with_acceleration(meshes.gravity(X.shape[1])).with_velocity(v)
+             .with_dirichlet_vertices(np.arange(Xs.shape[1])))
+    without = pbat.gpu.vbd.Integrator(base2.construct())
+    steps = 30
+    tc = _time_steps(with_contact, 1.0 / 60.0, iters, 1, steps, torch)
+    tp = _time_steps(without, 1.0 / 60.0, iters, 1, steps, torch)
+    active, _ = with_contact.contacts()
+    out["contact"] = {"workload": f"8 jittered {n}^3 cubes (size 0.45) over a fixed slab, {X.shape[1]} vertices, "
+                                  f"{nT} tets, {V.size} collision vertices, {F.shape[1]} collision triangles, "
+                                  f"h=1/60, {iters} iterations, {steps + 2} steps from 0.5 m/s",
+                      "ms_per_step": 1e3 * tc, "ms_per_step_without_contact": 1e3 * tp,
+                      "active_vertices_at_end": int(active.sum())}
+    log(f"[bench] contact: {1e3 * tc:.2f} ms/step vs {1e3 * tp:.2f} ms/step without contact "
+        f"({V.size} collision vertices, {F.shape[1]} triangles, {int(active.sum())} active at the end)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -289,7 +399,9 @@ def main():
     ap.add_argument("--adjoint-fd", action="store_true", help="central differences on 64 log Y entries")
     ap.add_argument("--batch-adjoint-meshes", type=int, default=64)
     ap.add_argument("--batch-adjoint-steps", type=int, default=200)
-    ap.add_argument("--only", choices=["batched", "adjoint", "batched_adjoint"], default=None,
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the accelerated-integrator and contact measurements")
+    ap.add_argument("--only", choices=["batched", "adjoint", "batched_adjoint", "variants"], default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MB (counter calibration) and exit")
@@ -317,7 +429,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}), flush=True)
         return
     if args.only:
-        run = {"batched": bench_batched, "adjoint": bench_adjoint, "batched_adjoint": bench_batched_adjoint}
+        run = {"batched": bench_batched, "adjoint": bench_adjoint, "batched_adjoint": bench_batched_adjoint,
+               "variants": bench_variants}
         print(json.dumps({args.only: run[args.only](args, torch)}), flush=True)
         return
     s = meshes.plain_gpu_scene(args.grid, seed=0)
@@ -376,6 +489,7 @@ def main():
         f"{bytes_per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
 
     batched = None if args.no_batched else bench_batched(args, torch)
+    variants = None if args.no_variants else bench_variants(args, torch)
     adjoint = batched_adjoint = None
     if not args.no_adjoint:
         adjoint = bench_adjoint(args, torch)
@@ -427,6 +541,7 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
         "batched": batched,
+        "variants": variants,
         "adjoint": adjoint,
         "batched_adjoint": batched_adjoint,
     }

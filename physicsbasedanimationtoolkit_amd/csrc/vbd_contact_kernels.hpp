@@ -3,11 +3,12 @@
 // terms of the sweep (gpu/impl/vbd/Kernels.cuh:80-114, 204-223;
 // sim/vbd/Kernels.h:215-302).
 //
-// Detection is a tiled all-pairs search: each workgroup stages 256 triangles in
-// LDS and every thread tests its collision vertex against them, so a launch
-// reads each triangle once per workgroup. It is exhaustive, which makes its
-// answers independent of any traversal order: a vertex's search radius is the
-// largest over its overlapping triangles, and its nearest triangles come in
+// Detection is a tiled all-pairs search: a workgroup takes kGroup collision
+// vertices (kPerWave per wave), stages kContactTile triangles at a time in LDS,
+// and each wave's 64 lanes stride over the tile for each of its vertices, so a
+// tile read from memory serves kGroup vertices. It is exhaustive, which makes
+// its answers independent of any traversal order: a vertex's search radius is
+// the largest over its overlapping triangles, and its nearest triangles come in
 // triangle order.
 #pragma once
 
@@ -18,6 +19,8 @@ namespace pbat_vbd::device
 
 constexpr int kMaxContacts = 8;     // contacting triangles per vertex (Kernels.cuh:64-65)
 constexpr int kContactTile = 256;   // triangles staged per LDS tile = workgroup size
+constexpr int kPerWave = 4;         // collision vertices per wave in the detection kernels
+constexpr int kGroup = 4 * kPerWave; // collision vertices per workgroup (4 waves)
 constexpr float kFltEpsilon = 1.1920928955078125e-07f; // FLT_EPSILON
 
 struct ContactArgs
@@ -75,30 +78,60 @@ __device__ inline float3 Cross(float3 a, float3 b)
     c.triHi[f] = make_float4(hx, hy, hz, 0.f);
 }
 
+__device__ inline float WaveMax(float x)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        x = fmaxf(x, __shfl_xor(x, off));
+    return x;
+}
+
+__device__ inline float WaveMin(float x)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        x = fminf(x, __shfl_xor(x, off));
+    return x;
+}
+
+__device__ inline int WaveMaxInt(int x)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        x = max(x, __shfl_xor(x, off));
+    return x;
+}
+
 // Broad phase of InitializeActiveSet (VertexTriangleMixedCcdDcd.cu:53-142): a
 // collision vertex whose segment x0 -> x1 has a box that overlaps (closed boxes)
 // the swept box of a triangle of another body becomes active, with search radius
 // the squared diagonal of the union of the two boxes, largest over such
 // triangles. Vertices active from earlier steps stay active. inAv receives the
-// step's active list. One thread per collision vertex; blockDim = kContactTile.
+// step's active list. kGroup collision vertices per workgroup of kContactTile.
 [[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactBroadphase(
     ContactArgs c, float4 const* __restrict__ x0, float4 const* __restrict__ x1)
 {
     __shared__ float4 sLo[kContactTile], sHi[kContactTile];
-    int const v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    bool const mine = v < c.nCV;
-    float3 lo = make_float3(0.f, 0.f, 0.f), hi = lo;
-    int body = 0;
-    if (mine)
+    int const wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
+    int const v0 = static_cast<int>(blockIdx.x) * kGroup + wave * kPerWave;
+    float3 lo[kPerWave], hi[kPerWave];
+    int body[kPerWave];
+    bool hit[kPerWave];
+    float d2[kPerWave];
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
     {
-        int const i = c.cvIdx[v];
-        float4 const a = x0[i], b = x1[i];
-        lo = make_float3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z));
-        hi = make_float3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z));
-        body = c.cvBody[v];
+        int const v = v0 + j;
+        lo[j] = hi[j] = make_float3(0.f, 0.f, 0.f);
+        body[j] = 0;
+        hit[j] = false;
+        d2[j] = 0.f;
+        if (v < c.nCV)
+        {
+            int const i = c.cvIdx[v];
+            float4 const a = x0[i], b = x1[i];
+            lo[j] = make_float3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z));
+            hi[j] = make_float3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z));
+            body[j] = c.cvBody[v];
+        }
     }
-    bool hit = false;
-    float d2max = 0.f;
     for (int base = 0; base < c.nF; base += kContactTile)
     {
         int const f = base + static_cast<int>(threadIdx.x);
@@ -109,34 +142,49 @@ __device__ inline float3 Cross(float3 a, float3 b)
         }
         __syncthreads();
         int const m = min(kContactTile, c.nF - base);
-        if (mine)
-            for (int k = 0; k < m; ++k)
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j)
+        {
+            if (v0 + j >= c.nCV)
+                continue;
+            for (int k = lane; k < m; k += 64)
             {
                 float4 const L = sLo[k], U = sHi[k];
-                if (__float_as_int(L.w) == body)
+                if (__float_as_int(L.w) == body[j])
                     continue;
-                bool const overlap = lo.x <= U.x && lo.y <= U.y && lo.z <= U.z && L.x <= hi.x && L.y <= hi.y &&
-                                     L.z <= hi.z;
+                bool const overlap = lo[j].x <= U.x && lo[j].y <= U.y && lo[j].z <= U.z && L.x <= hi[j].x &&
+                                     L.y <= hi[j].y && L.z <= hi[j].z;
                 if (!overlap)
                     continue;
-                float const dx = fmaxf(hi.x, U.x) - fminf(lo.x, L.x);
-                float const dy = fmaxf(hi.y, U.y) - fminf(lo.y, L.y);
-                float const dz = fmaxf(hi.z, U.z) - fminf(lo.z, L.z);
-                d2max = hit ? fmaxf(d2max, dx * dx + dy * dy + dz * dz) : dx * dx + dy * dy + dz * dz;
-                hit = true;
+                float const dx = fmaxf(hi[j].x, U.x) - fminf(lo[j].x, L.x);
+                float const dy = fmaxf(hi[j].y, U.y) - fminf(lo[j].y, L.y);
+                float const dz = fmaxf(hi[j].z, U.z) - fminf(lo[j].z, L.z);
+                float const q = dx * dx + dy * dy + dz * dz;
+                d2[j] = hit[j] ? fmaxf(d2[j], q) : q;
+                hit[j] = true;
             }
+        }
         __syncthreads();
     }
-    if (!mine)
-        return;
-    int act = c.active[v];
-    if (hit)
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
     {
-        act = 1;
-        c.active[v] = 1;
-        c.dupper[v] = d2max;
+        int const v = v0 + j;
+        if (v >= c.nCV)
+            continue;
+        bool const any = __ballot(hit[j]) != 0ull;
+        float const dmax = WaveMax(hit[j] ? d2[j] : -1.f);
+        if (lane != 0)
+            continue;
+        int act = c.active[v];
+        if (any)
+        {
+            act = 1;
+            c.active[v] = 1;
+            c.dupper[v] = dmax;
+        }
+        c.inAv[v] = act;
     }
-    c.inAv[v] = act;
 }
 
 // Squared distance from p to triangle (a, b, c): closest point by Voronoi
@@ -183,39 +231,65 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
 // otherwise; in triangle order, at most kMaxContacts. finalize = 0
 // (UpdateActiveSet, gpu/impl/vbd/Integrator.cu:250-271): they become the
 // vertex's contacts. finalize = 1 (FinalizeActiveSet): the vertex stays active
-// when it lies below the plane of the last of them. One thread per collision
-// vertex, two passes over LDS tiles of triangle positions; blockDim = kContactTile.
+// when it lies below the plane of the last of them. kGroup collision vertices
+// per workgroup of kContactTile; two passes over LDS tiles of triangle
+// positions, the second collecting in triangle order with wave ballots.
 [[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactNearest(
     ContactArgs c, float4 const* __restrict__ x, int finalize)
 {
     __shared__ float4 sA[kContactTile], sB[kContactTile], sC[kContactTile];
-    int const v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    bool const mine = v < c.nCV && c.inAv[v] != 0;
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    int body = 0;
-    float r = 0.f;
-    if (mine)
+    int const wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
+    int const v0 = static_cast<int>(blockIdx.x) * kGroup + wave * kPerWave;
+    float4 p[kPerWave];
+    int body[kPerWave];
+    bool mine[kPerWave];
+    float r[kPerWave], dmin[kPerWave], thr[kPerWave];
+    int count[kPerWave], last[kPerWave];
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
     {
-        p = x[c.cvIdx[v]];
-        body = c.cvBody[v];
-        r = c.dupper[v];
+        int const v = v0 + j;
+        mine[j] = v < c.nCV && c.inAv[v] != 0;
+        p[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        body[j] = 0;
+        r[j] = 0.f;
+        if (mine[j])
+        {
+            p[j] = x[c.cvIdx[v]];
+            body[j] = c.cvBody[v];
+            r[j] = c.dupper[v];
+        }
+        dmin[j] = 3.402823466e+38f;
+        thr[j] = -1.f;
+        count[j] = 0;
+        last[j] = -1;
     }
-    // pass 0: d*; pass 1: collect (contacts straight into fc, or the last one)
-    float dstar = 3.402823466e+38f;
-    bool any = false;
-    float thr = -1.f;
-    int nfound = 0, last = -1;
-    int* out = mine ? c.fc + static_cast<size_t>(c.cvIdx[v]) * kMaxContacts : nullptr;
+    bool anyWork = false;
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
+        anyWork = anyWork || mine[j];
+    // the whole workgroup skips when none of its vertices is active
+    if (!__syncthreads_or(anyWork ? 1 : 0))
+        return;
+    bool any[kPerWave] = {};
     for (int pass = 0; pass < 2; ++pass)
     {
-        if (pass == 1 && mine && any)
+        if (pass == 1)
         {
-            if (dstar < r - kFltEpsilon)
-                thr = dstar + kFltEpsilon;
-            else if (dstar <= r + kFltEpsilon)
-                thr = r + kFltEpsilon;
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+            {
+                if (!mine[j])
+                    continue;
+                float const ds = WaveMin(dmin[j]);
+                if (!any[j])
+                    continue;
+                if (ds < r[j] - kFltEpsilon)
+                    thr[j] = ds + kFltEpsilon;
+                else if (ds <= r[j] + kFltEpsilon)
+                    thr[j] = r[j] + kFltEpsilon;
+            }
         }
-        bool const work = mine && (pass == 0 || thr >= 0.f);
         for (int base = 0; base < c.nF; base += kContactTile)
         {
             int const f = base + static_cast<int>(threadIdx.x);
@@ -230,43 +304,73 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
             }
             __syncthreads();
             int const m = min(kContactTile, c.nF - base);
-            if (work)
-                for (int k = 0; k < m; ++k)
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+            {
+                if (!mine[j])
+                    continue;
+                if (pass == 0)
                 {
-                    float4 const a = sA[k];
-                    if (__float_as_int(a.w) == body)
-                        continue;
-                    float const d = PointTriangleSqDist(p, a, sB[k], sC[k]);
-                    if (pass == 0)
+                    for (int k = lane; k < m; k += 64)
                     {
-                        dstar = any ? fminf(dstar, d) : d;
-                        any = true;
+                        float4 const a = sA[k];
+                        if (__float_as_int(a.w) == body[j])
+                            continue;
+                        dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], a, sB[k], sC[k]));
                     }
-                    else if (d <= thr && nfound < kMaxContacts)
-                    {
-                        if (!finalize)
-                            out[nfound] = base + k;
-                        last = base + k;
-                        ++nfound;
-                    }
+                    continue;
                 }
+                if (thr[j] < 0.f || count[j] >= kMaxContacts)
+                    continue;
+                // chunks of 64 triangles in order; lane l takes triangle base + q + l
+                for (int q = 0; q < m && count[j] < kMaxContacts; q += 64)
+                {
+                    int const k = q + lane;
+                    bool take = false;
+                    if (k < m)
+                    {
+                        float4 const a = sA[k];
+                        take = __float_as_int(a.w) != body[j] &&
+                               PointTriangleSqDist(p[j], a, sB[k], sC[k]) <= thr[j];
+                    }
+                    unsigned long long const mask = __ballot(take);
+                    int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
+                    bool const kept = take && slot < kMaxContacts;
+                    if (kept && !finalize)
+                        c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = base + k;
+                    last[j] = max(last[j], WaveMaxInt(kept ? base + k : -1));
+                    count[j] += __popcll(mask);
+                }
+            }
             __syncthreads();
         }
+        if (pass == 0)
+        {
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+                any[j] = mine[j] && __ballot(dmin[j] < 3.402823466e+38f) != 0ull;
+        }
     }
-    if (!mine)
-        return;
-    if (!finalize)
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
     {
-        for (int k = nfound; k < kMaxContacts; ++k)
-            out[k] = -1;
-        return;
-    }
-    if (last >= 0)
-    {
-        int4 const t = c.tri[last];
-        float4 const a = x[t.x], b = x[t.y], cc = x[t.z];
-        float3 const n = Cross(Sub(b, a), Sub(cc, a));
-        c.active[v] = Dot(Sub(p, a), n) < 0.f ? 1 : 0;
+        if (!mine[j])
+            continue;
+        int const v = v0 + j;
+        int const n = min(count[j], kMaxContacts);
+        if (!finalize)
+        {
+            if (lane >= n && lane < kMaxContacts)
+                c.fc[static_cast<size_t>(c.cvIdx[v]) * kMaxContacts + lane] = -1;
+            continue;
+        }
+        if (lane == 0 && last[j] >= 0)
+        {
+            int4 const t = c.tri[last[j]];
+            float4 const a = x[t.x], b = x[t.y], cc = x[t.z];
+            float3 const nrm = Cross(Sub(b, a), Sub(cc, a));
+            c.active[v] = Dot(Sub(p[j], a), nrm) < 0.f ? 1 : 0;
+        }
     }
 }
 

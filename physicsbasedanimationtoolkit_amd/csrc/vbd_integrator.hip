@@ -488,7 +488,7 @@ void ContactBeginStep(pbat_vbd_integrator& h, float dt)
     device::ContactArgs const c = MakeContactArgs(h);
     hipLaunchKernelGGL(device::ContactSweptBoxes, dim3((h.nF + 255) / 256), dim3(256), 0, h.stream, c,
                        static_cast<float4 const*>(h.xt.ptr), static_cast<float4 const*>(h.xfull.ptr));
-    hipLaunchKernelGGL(device::ContactBroadphase, dim3((h.nCV + device::kContactTile - 1) / device::kContactTile),
+    hipLaunchKernelGGL(device::ContactBroadphase, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
                        dim3(device::kContactTile), 0, h.stream, c, static_cast<float4 const*>(h.xt.ptr),
                        static_cast<float4 const*>(h.xfull.ptr));
 }
@@ -498,7 +498,7 @@ void ContactBeginStep(pbat_vbd_integrator& h, float dt)
 // at the current positions.
 void ContactNearestPass(pbat_vbd_integrator& h, int finalize)
 {
-    hipLaunchKernelGGL(device::ContactNearest, dim3((h.nCV + device::kContactTile - 1) / device::kContactTile),
+    hipLaunchKernelGGL(device::ContactNearest, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
                        dim3(device::kContactTile), 0, h.stream, MakeContactArgs(h),
                        static_cast<float4 const*>(h.xm.ptr), finalize);
 }
