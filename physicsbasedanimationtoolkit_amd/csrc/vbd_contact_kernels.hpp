@@ -36,6 +36,8 @@ struct ContactArgs
     float* __restrict__ dupper;        // per collision vertex: squared search radius
     float4* __restrict__ triLo;        // swept triangle boxes
     float4* __restrict__ triHi;
+    float4* __restrict__ tileLo;       // box of each tile of kContactTile triangles
+    float4* __restrict__ tileHi;
     int nCV, nF;
 };
 
@@ -76,6 +78,69 @@ __device__ inline float3 Cross(float3 a, float3 b)
     }
     c.triLo[f] = make_float4(lx, ly, lz, __int_as_float(t.w));
     c.triHi[f] = make_float4(hx, hy, hz, 0.f);
+}
+
+// Box of each tile of kContactTile consecutive triangles: of their swept boxes
+// (swept != 0) or of their vertices at x. One workgroup of kContactTile per tile.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactTileBoxes(
+    ContactArgs c, float4 const* __restrict__ x, int swept)
+{
+    __shared__ float red[6][4];
+    int const f = static_cast<int>(blockIdx.x) * kContactTile + static_cast<int>(threadIdx.x);
+    float lx = 3.402823466e+38f, ly = lx, lz = lx, hx = -lx, hy = -lx, hz = -lx;
+    if (f < c.nF)
+    {
+        if (swept)
+        {
+            float4 const L = c.triLo[f], U = c.triHi[f];
+            lx = L.x, ly = L.y, lz = L.z, hx = U.x, hy = U.y, hz = U.z;
+        }
+        else
+        {
+            int4 const t = c.tri[f];
+            float4 const a = x[t.x], b = x[t.y], cc = x[t.z];
+            lx = fminf(a.x, fminf(b.x, cc.x)), ly = fminf(a.y, fminf(b.y, cc.y)), lz = fminf(a.z, fminf(b.z, cc.z));
+            hx = fmaxf(a.x, fmaxf(b.x, cc.x)), hy = fmaxf(a.y, fmaxf(b.y, cc.y)), hz = fmaxf(a.z, fmaxf(b.z, cc.z));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        lx = fminf(lx, __shfl_xor(lx, off)), ly = fminf(ly, __shfl_xor(ly, off));
+        lz = fminf(lz, __shfl_xor(lz, off)), hx = fmaxf(hx, __shfl_xor(hx, off));
+        hy = fmaxf(hy, __shfl_xor(hy, off)), hz = fmaxf(hz, __shfl_xor(hz, off));
+    }
+    int const w = static_cast<int>(threadIdx.x) >> 6;
+    if ((threadIdx.x & 63) == 0)
+        red[0][w] = lx, red[1][w] = ly, red[2][w] = lz, red[3][w] = hx, red[4][w] = hy, red[5][w] = hz;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        for (int k = 1; k < 4; ++k)
+        {
+            red[0][0] = fminf(red[0][0], red[0][k]), red[1][0] = fminf(red[1][0], red[1][k]);
+            red[2][0] = fminf(red[2][0], red[2][k]), red[3][0] = fmaxf(red[3][0], red[3][k]);
+            red[4][0] = fmaxf(red[4][0], red[4][k]), red[5][0] = fmaxf(red[5][0], red[5][k]);
+        }
+        c.tileLo[blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.f);
+        c.tileHi[blockIdx.x] = make_float4(red[3][0], red[4][0], red[5][0], 0.f);
+    }
+}
+
+// Squared distance from p to the box [L, U] (0 inside).
+__device__ inline float PointBoxSqDist(float4 p, float4 L, float4 U)
+{
+    float const dx = p.x - fminf(fmaxf(p.x, L.x), U.x);
+    float const dy = p.y - fminf(fmaxf(p.y, L.y), U.y);
+    float const dz = p.z - fminf(fmaxf(p.z, L.z), U.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// A tile can be skipped for a query when its box lies beyond the query's bound;
+// the margin keeps the test conservative against the rounding of the exact
+// per-triangle distances.
+__device__ inline bool TileWithin(float dbox, float bound)
+{
+    return dbox <= bound + 1e-5f * (bound + dbox) + 1e-30f;
 }
 
 __device__ inline float WaveMax(float x)
@@ -134,6 +199,19 @@ __device__ inline int WaveMaxInt(int x)
     }
     for (int base = 0; base < c.nF; base += kContactTile)
     {
+        // skip tiles whose box overlaps none of the workgroup's segments
+        float4 const TL = c.tileLo[base / kContactTile], TU = c.tileHi[base / kContactTile];
+        bool need[kPerWave];
+        int anyNeed = 0;
+#pragma unroll
+        for (int j = 0; j < kPerWave; ++j)
+        {
+            need[j] = v0 + j < c.nCV && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
+                      TL.y <= hi[j].y && TL.z <= hi[j].z;
+            anyNeed |= need[j] ? 1 : 0;
+        }
+        if (!__syncthreads_or(anyNeed))
+            continue;
         int const f = base + static_cast<int>(threadIdx.x);
         if (f < c.nF)
         {
@@ -145,7 +223,7 @@ __device__ inline int WaveMaxInt(int x)
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
         {
-            if (v0 + j >= c.nCV)
+            if (!need[j])
                 continue;
             for (int k = lane; k < m; k += 64)
             {
@@ -292,6 +370,22 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
         }
         for (int base = 0; base < c.nF; base += kContactTile)
         {
+            // skip tiles whose box lies beyond every query's bound: r + eps in
+            // pass 0 (no triangle farther can matter), the collection threshold
+            // in pass 1
+            float4 const TL = c.tileLo[base / kContactTile], TU = c.tileHi[base / kContactTile];
+            bool need[kPerWave];
+            int anyNeed = 0;
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+            {
+                float const bound = pass == 0 ? r[j] + kFltEpsilon : thr[j];
+                need[j] = mine[j] && (pass == 0 || (thr[j] >= 0.f && count[j] < kMaxContacts)) &&
+                          TileWithin(PointBoxSqDist(p[j], TL, TU), bound);
+                anyNeed |= need[j] ? 1 : 0;
+            }
+            if (!__syncthreads_or(anyNeed))
+                continue;
             int const f = base + static_cast<int>(threadIdx.x);
             if (f < c.nF)
             {
@@ -307,7 +401,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
 #pragma unroll
             for (int j = 0; j < kPerWave; ++j)
             {
-                if (!mine[j])
+                if (!need[j])
                     continue;
                 if (pass == 0)
                 {
