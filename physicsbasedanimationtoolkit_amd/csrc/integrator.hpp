@@ -234,6 +234,7 @@ struct pbat_vbd_integrator
     pbat_vbd::DeviceArray<float> triArea, xva, dupper;
     pbat_vbd::DeviceArray<float4> triLo, triHi, tileLo, tileHi, xfull, xb;
     std::vector<int64_t> cvOrig; // original index of each collision vertex
+    int pendBeg = 0, pendEnd = 0; // colour range whose new positions are still in xb
     float kD = 0.f;
     float detHZero = 1e-7f;
     int blockSize = 256;

@@ -484,13 +484,25 @@ struct ContactSweep
     float const* __restrict__ xva;
     float4* __restrict__ xb; // new positions of the colour
     float muC, muF, epsv;
+    // The previous colour's range, whose current positions are still in xb: the
+    // sweep reads them there and copies them back to xm (nobody reads xm there).
+    int pendBeg, pendEnd;
 };
 
-template <class TBlock>
-__device__ inline void AccumulateContact(float3 xtv, float3 xv, float4 const* xm, float4 const* xt, int4 t, float dt,
-                                         float muC, float muF, float epsv, TBlock& B)
+// Position of vertex j during a contact sweep: in xb when j belongs to the
+// previous colour's pending range, else in xm.
+__device__ __forceinline__ float4 ContactPosition(ContactSweep const& cs, float4 const* xm, int j)
 {
-    float4 const A4 = xm[t.x], B4 = xm[t.y], C4 = xm[t.z];
+    return (j >= cs.pendBeg && j < cs.pendEnd) ? cs.xb[j] : xm[j];
+}
+
+template <class TBlock>
+__device__ inline void AccumulateContact(float3 xtv, float3 xv, ContactSweep const& cs, float4 const* xm,
+                                         float4 const* xt, int4 t, float dt, float muC, float muF, float epsv,
+                                         TBlock& B)
+{
+    float4 const A4 = ContactPosition(cs, xm, t.x), B4 = ContactPosition(cs, xm, t.y),
+                 C4 = ContactPosition(cs, xm, t.z);
     float3 const A = make_float3(A4.x, A4.y, A4.z);
     float3 const e0 = make_float3(B4.x - A4.x, B4.y - A4.y, B4.z - A4.z);
     float3 const e1 = make_float3(C4.x - A4.x, C4.y - A4.y, C4.z - A4.z);
@@ -537,7 +549,7 @@ __device__ inline void AccumulateContact(float3 xtv, float3 xv, float4 const* xm
     B.h12 += c1 * (e0.y * e0.z + b1.y * b1.z), B.h22 += c1 * (e0.z * e0.z + b1.z * b1.z);
 }
 
-// xm[v] = xb[v] over a colour's range, after its contact sweep.
+// xm[v] = xb[v] over a colour's range, after the last contact sweep of an iteration.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void CopyColour(float4* __restrict__ xm,
                                                                           float4 const* __restrict__ xb, int vbeg,
                                                                           int vend)

@@ -69,6 +69,16 @@ int EffectivePerLane(pbat_vbd_integrator const& h)
     return std::clamp((d + L - 1) / L, 1, 4);
 }
 
+// Copy the last contact sweep's colour back from the write buffer.
+void FlushContactSweep(pbat_vbd_integrator& h)
+{
+    int const n = h.pendEnd - h.pendBeg;
+    if (n > 0)
+        hipLaunchKernelGGL(device::CopyColour, dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr,
+                           static_cast<float4 const*>(h.xb.ptr), h.pendBeg, h.pendEnd);
+    h.pendBeg = h.pendEnd = 0;
+}
+
 template <int L, int K>
 void LaunchSweepLK(pbat_vbd_integrator& h, device::SweepArgs a, int nVerts)
 {
@@ -91,12 +101,17 @@ void LaunchSweepLK(pbat_vbd_integrator& h, device::SweepArgs a, int nVerts)
         hipLaunchKernelGGL((device::VbdSweep<L, K>), dim3(static_cast<int>(grid64)), dim3(block), 0, h.stream, a);
         return;
     }
-    // gpu/impl/vbd/Integrator.cu:303-326: the colour writes a buffer copied back after it
+    // gpu/impl/vbd/Integrator.cu:303-326: the colour writes a buffer; the next colour's
+    // launch reads the pending range there and copies it back
+    // the same colour again (a single colour), or a pending range larger than this
+    // launch can copy: copy it back first
+    if ((h.pendBeg == vbeg && h.pendEnd > h.pendBeg) || static_cast<int64_t>(h.pendEnd - h.pendBeg) > grid64 * block)
+        FlushContactSweep(h);
     a.contact.fc = h.fc.ptr, a.contact.tri = h.tri.ptr, a.contact.triArea = h.triArea.ptr, a.contact.xva = h.xva.ptr;
     a.contact.xb = h.xb.ptr, a.contact.muC = h.muC, a.contact.muF = h.muF, a.contact.epsv = h.epsv;
+    a.contact.pendBeg = h.pendBeg, a.contact.pendEnd = h.pendEnd;
     hipLaunchKernelGGL((device::VbdSweep<L, K, true>), dim3(static_cast<int>(grid64)), dim3(block), 0, h.stream, a);
-    hipLaunchKernelGGL(device::CopyColour, dim3((nVerts + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr,
-                       static_cast<float4 const*>(h.xb.ptr), vbeg, vbeg + nVerts);
+    h.pendBeg = vbeg, h.pendEnd = vbeg + nVerts;
 }
 
 template <int L>
@@ -261,6 +276,7 @@ int SweepAllColours(pbat_vbd_integrator& h, device::SweepArgs& a)
         LaunchSweep(h, a, nc);
         ++launched;
     }
+    FlushContactSweep(h);
     return launched;
 }
 
@@ -1237,6 +1253,7 @@ void Autotune(pbat_vbd_integrator& h, float dt, int sweeps, double* timingsUs, i
             if (a.vend > a.vbeg)
                 LaunchSweep(h, a, a.vend - a.vbeg);
         }
+        FlushContactSweep(h);
     };
     struct Candidate
     {

@@ -274,7 +274,9 @@ __device__ __forceinline__ float2 LoadConst(float2 const* p, bool nt)
 // With C (scenes in contact), the vertex's contact terms join its block after
 // damping and before inertia (gpu/impl/vbd/Kernels.cuh:200-230), and the new
 // positions go to contact.xb, so contact terms of the colour read positions from
-// before its pass; the host copies them back after the launch.
+// before its pass (the reference's write buffer, gpu/impl/vbd/Integrator.cu:303-326).
+// The previous colour's positions are read from xb and copied back to xm by
+// this launch; the last colour of an iteration is copied back by the host.
 template <int L, int K, bool C = false>
 __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 {
@@ -318,6 +320,13 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     }
     Block B{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool const nt = a.ntStream != 0;
+    // copy the previous colour's new positions back (contact sweeps only)
+    if constexpr (C)
+    {
+        int const pend = a.contact.pendBeg + tid;
+        if (pend < a.contact.pendEnd)
+            a.xm[pend] = a.contact.xb[pend];
+    }
     int4 I[K];
     float4 q0[K], q1[K];
     float2 q2[K];
@@ -341,9 +350,18 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     {
         if (I[j].x >= 0)
         {
-            pa[j] = a.xm[I[j].x];
-            pb[j] = a.xm[I[j].y];
-            pc[j] = a.xm[I[j].z];
+            if constexpr (C)
+            {
+                pa[j] = ContactPosition(a.contact, a.xm, I[j].x);
+                pb[j] = ContactPosition(a.contact, a.xm, I[j].y);
+                pc[j] = ContactPosition(a.contact, a.xm, I[j].z);
+            }
+            else
+            {
+                pa[j] = a.xm[I[j].x];
+                pb[j] = a.xm[I[j].y];
+                pc[j] = a.xm[I[j].z];
+            }
         }
     }
 #pragma unroll
@@ -356,8 +374,13 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
         int4 const J = LoadIds(a.ids + k, nt);
         if (J.x < 0)
             continue;
-        AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
-                           __int_as_float(J.w), own, a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
+        if constexpr (C)
+            AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
+                               __int_as_float(J.w), own, ContactPosition(a.contact, a.xm, J.x),
+                               ContactPosition(a.contact, a.xm, J.y), ContactPosition(a.contact, a.xm, J.z), B);
+        else
+            AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
+                               __int_as_float(J.w), own, a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
     }
     GroupSum<L>(B);
     if (!active || lane != 0)
@@ -382,8 +405,8 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
             float const kC = cs.xva[v] * cs.muC / sumfa;
             float3 const xo = make_float3(own.x, own.y, own.z), xt3 = make_float3(xto.x, xto.y, xto.z);
             for (int k = 0; k < n; ++k)
-                AccumulateContact(xt3, xo, a.xm, a.xt, cs.tri[f[k]], a.c.dt, kC * cs.triArea[f[k]], cs.muF, cs.epsv,
-                                  B);
+                AccumulateContact(xt3, xo, cs, a.xm, a.xt, cs.tri[f[k]], a.c.dt, kC * cs.triArea[f[k]], cs.muF,
+                                  cs.epsv, B);
         }
         float3 const dx = InertiaNewtonStep(B, own, xl, a.c);
         cs.xb[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
