@@ -14,9 +14,9 @@ pytestmark = pytest.mark.gpu
 CONTACT_POS_TOL = 1e-5
 
 
-def contact_pair(X, E, F, V, B, v, dbc, aext=None, muC=1e6, muF=0.3, epsv=1e-3, frequency=1):
+def contact_pair(X, E, F, V, B, v, dbc, aext=None, muC=1e6, muF=0.3, epsv=1e-3, frequency=1, Y=1e6):
     nT = E.shape[1]
-    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+    mu, lam = meshes.lame_coefficients(np.full(nT, Y), 0.45)
     aext = meshes.gravity(X.shape[1]) if aext is None else aext
     data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B)
             .with_material(np.full(nT, 1e3), mu, lam).with_acceleration(aext).with_velocity(v)
@@ -180,3 +180,37 @@ def test_sliding_cube_with_friction_matches_oracle(gpu_device):
         e = rel(gpu.x, ref.positions)
         print(f"step {step}: rel pos err {e:.3e}")
         assert e <= CONTACT_POS_TOL
+
+
+def test_single_colour_and_narrow_launches_match_oracle(gpu_device):
+    """Write-buffer corner cases of the contact sweep: a scene whose only updated
+    vertices (the two tets' apexes) form a single colour, so each iteration sweeps the
+    same colour again; and one lane per vertex, so a colour's launch can be smaller than
+    the previous colour it has to copy back."""
+    XT, X, T, F, B = two_tet_scene()
+    v = (X - XT) * 60.0
+    dbc = np.array([0, 1, 2, 4, 5, 6])
+    gpu, ref = contact_pair(XT, T, F, np.arange(8), B, v, dbc, aext=np.zeros_like(XT), muC=1e3, Y=1e3)
+    assert gpu.counts()[1] == 2
+    touched = 0
+    for _ in range(3):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 6, 1)
+        ref.step_contact(DT, 6, 1)
+        _, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        touched += int(fc[3, 0] >= 0)
+        assert rel(gpu.x, ref.positions) <= CONTACT_POS_TOL
+    assert touched > 0
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    gpu, ref = contact_pair(X, E, F, V, B, v, dbc)
+    gpu.lanes_per_vertex = 1
+    for _ in range(4):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 10, 1)
+        ref.step_contact(DT, 10, 1)
+        _, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        assert rel(gpu.x, ref.positions) <= CONTACT_POS_TOL
