@@ -232,7 +232,7 @@ struct pbat_vbd_integrator
     pbat_vbd::DeviceArray<int> cvIdx, cvBody, fc, cvActive, cvInAv;
     pbat_vbd::DeviceArray<int4> tri;
     pbat_vbd::DeviceArray<float> triArea, xva, dupper;
-    pbat_vbd::DeviceArray<float4> triLo, triHi, tileLo, tileHi, xfull, xb;
+    pbat_vbd::DeviceArray<float4> triLo, triHi, chunkLo, chunkHi, xfull, xb;
     std::vector<int64_t> cvOrig; // original index of each collision vertex
     int pendBeg = 0, pendEnd = 0; // colour range whose new positions are still in xb
     float kD = 0.f;

@@ -3,13 +3,13 @@
 // terms of the sweep (gpu/impl/vbd/Kernels.cuh:80-114, 204-223;
 // sim/vbd/Kernels.h:215-302).
 //
-// Detection is a tiled all-pairs search: a workgroup takes kGroup collision
-// vertices (kPerWave per wave), stages kContactTile triangles at a time in LDS,
-// and each wave's 64 lanes stride over the tile for each of its vertices, so a
-// tile read from memory serves kGroup vertices. It is exhaustive, which makes
-// its answers independent of any traversal order: a vertex's search radius is
-// the largest over its overlapping triangles, and its nearest triangles come in
-// triangle order.
+// Detection is an all-pairs search over chunks of 64 consecutive triangles,
+// each with a bounding box: a wave takes kPerWave collision vertices, skips the
+// chunks whose box lies beyond all of their query bounds, and for the others
+// gives one triangle to each lane. It is exhaustive up to those exact bounds,
+// which makes its answers independent of any traversal order: a vertex's search
+// radius is the largest over its overlapping triangles, and its nearest
+// triangles come in triangle order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,9 +18,10 @@ namespace pbat_vbd::device
 {
 
 constexpr int kMaxContacts = 8;     // contacting triangles per vertex (Kernels.cuh:64-65)
-constexpr int kContactTile = 256;   // triangles staged per LDS tile = workgroup size
+constexpr int kContactBlock = 256;  // workgroup size of the detection kernels (4 waves)
+constexpr int kChunk = 64;          // triangles per chunk box = lanes per wave
 constexpr int kPerWave = 4;         // collision vertices per wave in the detection kernels
-constexpr int kGroup = 4 * kPerWave; // collision vertices per workgroup (4 waves)
+constexpr int kGroup = 4 * kPerWave; // collision vertices per workgroup
 constexpr float kFltEpsilon = 1.1920928955078125e-07f; // FLT_EPSILON
 
 struct ContactArgs
@@ -36,8 +37,8 @@ struct ContactArgs
     float* __restrict__ dupper;        // per collision vertex: squared search radius
     float4* __restrict__ triLo;        // swept triangle boxes
     float4* __restrict__ triHi;
-    float4* __restrict__ tileLo;       // box of each tile of kContactTile triangles
-    float4* __restrict__ tileHi;
+    float4* __restrict__ chunkLo;      // box of each chunk of kChunk consecutive triangles
+    float4* __restrict__ chunkHi;
     int nCV, nF;
 };
 
@@ -80,13 +81,13 @@ __device__ inline float3 Cross(float3 a, float3 b)
     c.triHi[f] = make_float4(hx, hy, hz, 0.f);
 }
 
-// Box of each tile of kContactTile consecutive triangles: of their swept boxes
-// (swept != 0) or of their vertices at x. One workgroup of kContactTile per tile.
-[[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactTileBoxes(
+// Box of each chunk of kChunk consecutive triangles: of their swept boxes
+// (swept != 0) or of their vertices at x. One wave per chunk.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactChunkBoxes(
     ContactArgs c, float4 const* __restrict__ x, int swept)
 {
-    __shared__ float red[6][4];
-    int const f = static_cast<int>(blockIdx.x) * kContactTile + static_cast<int>(threadIdx.x);
+    int const chunk = static_cast<int>(blockIdx.x) * (kContactBlock / kChunk) + static_cast<int>(threadIdx.x >> 6);
+    int const f = chunk * kChunk + static_cast<int>(threadIdx.x & 63);
     float lx = 3.402823466e+38f, ly = lx, lz = lx, hx = -lx, hy = -lx, hz = -lx;
     if (f < c.nF)
     {
@@ -109,20 +110,10 @@ __device__ inline float3 Cross(float3 a, float3 b)
         lz = fminf(lz, __shfl_xor(lz, off)), hx = fmaxf(hx, __shfl_xor(hx, off));
         hy = fmaxf(hy, __shfl_xor(hy, off)), hz = fmaxf(hz, __shfl_xor(hz, off));
     }
-    int const w = static_cast<int>(threadIdx.x) >> 6;
-    if ((threadIdx.x & 63) == 0)
-        red[0][w] = lx, red[1][w] = ly, red[2][w] = lz, red[3][w] = hx, red[4][w] = hy, red[5][w] = hz;
-    __syncthreads();
-    if (threadIdx.x == 0)
+    if ((threadIdx.x & 63) == 0 && chunk * kChunk < c.nF)
     {
-        for (int k = 1; k < 4; ++k)
-        {
-            red[0][0] = fminf(red[0][0], red[0][k]), red[1][0] = fminf(red[1][0], red[1][k]);
-            red[2][0] = fminf(red[2][0], red[2][k]), red[3][0] = fmaxf(red[3][0], red[3][k]);
-            red[4][0] = fmaxf(red[4][0], red[4][k]), red[5][0] = fmaxf(red[5][0], red[5][k]);
-        }
-        c.tileLo[blockIdx.x] = make_float4(red[0][0], red[1][0], red[2][0], 0.f);
-        c.tileHi[blockIdx.x] = make_float4(red[3][0], red[4][0], red[5][0], 0.f);
+        c.chunkLo[chunk] = make_float4(lx, ly, lz, 0.f);
+        c.chunkHi[chunk] = make_float4(hx, hy, hz, 0.f);
     }
 }
 
@@ -169,13 +160,14 @@ __device__ inline int WaveMaxInt(int x)
 // the swept box of a triangle of another body becomes active, with search radius
 // the squared diagonal of the union of the two boxes, largest over such
 // triangles. Vertices active from earlier steps stay active. inAv receives the
-// step's active list. kGroup collision vertices per workgroup of kContactTile.
-[[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactBroadphase(
+// step's active list. kPerWave collision vertices per wave.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactBroadphase(
     ContactArgs c, float4 const* __restrict__ x0, float4 const* __restrict__ x1)
 {
-    __shared__ float4 sLo[kContactTile], sHi[kContactTile];
     int const wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
     int const v0 = static_cast<int>(blockIdx.x) * kGroup + wave * kPerWave;
+    if (v0 >= c.nCV)
+        return;
     float3 lo[kPerWave], hi[kPerWave];
     int body[kPerWave];
     bool hit[kPerWave];
@@ -197,52 +189,41 @@ __device__ inline int WaveMaxInt(int x)
             body[j] = c.cvBody[v];
         }
     }
-    for (int base = 0; base < c.nF; base += kContactTile)
+    int const nChunks = (c.nF + kChunk - 1) / kChunk;
+    for (int ch = 0; ch < nChunks; ++ch)
     {
-        // skip tiles whose box overlaps none of the workgroup's segments
-        float4 const TL = c.tileLo[base / kContactTile], TU = c.tileHi[base / kContactTile];
+        float4 const TL = c.chunkLo[ch], TU = c.chunkHi[ch];
         bool need[kPerWave];
-        int anyNeed = 0;
+        bool anyNeed = false;
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
         {
             need[j] = v0 + j < c.nCV && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
                       TL.y <= hi[j].y && TL.z <= hi[j].z;
-            anyNeed |= need[j] ? 1 : 0;
+            anyNeed = anyNeed || need[j];
         }
-        if (!__syncthreads_or(anyNeed))
+        if (!anyNeed)
             continue;
-        int const f = base + static_cast<int>(threadIdx.x);
-        if (f < c.nF)
-        {
-            sLo[threadIdx.x] = c.triLo[f];
-            sHi[threadIdx.x] = c.triHi[f];
-        }
-        __syncthreads();
-        int const m = min(kContactTile, c.nF - base);
+        int const f = ch * kChunk + lane;
+        if (f >= c.nF)
+            continue;
+        float4 const L = c.triLo[f], U = c.triHi[f];
 #pragma unroll
         for (int j = 0; j < kPerWave; ++j)
         {
-            if (!need[j])
+            if (!need[j] || __float_as_int(L.w) == body[j])
                 continue;
-            for (int k = lane; k < m; k += 64)
-            {
-                float4 const L = sLo[k], U = sHi[k];
-                if (__float_as_int(L.w) == body[j])
-                    continue;
-                bool const overlap = lo[j].x <= U.x && lo[j].y <= U.y && lo[j].z <= U.z && L.x <= hi[j].x &&
-                                     L.y <= hi[j].y && L.z <= hi[j].z;
-                if (!overlap)
-                    continue;
-                float const dx = fmaxf(hi[j].x, U.x) - fminf(lo[j].x, L.x);
-                float const dy = fmaxf(hi[j].y, U.y) - fminf(lo[j].y, L.y);
-                float const dz = fmaxf(hi[j].z, U.z) - fminf(lo[j].z, L.z);
-                float const q = dx * dx + dy * dy + dz * dz;
-                d2[j] = hit[j] ? fmaxf(d2[j], q) : q;
-                hit[j] = true;
-            }
+            bool const overlap = lo[j].x <= U.x && lo[j].y <= U.y && lo[j].z <= U.z && L.x <= hi[j].x &&
+                                 L.y <= hi[j].y && L.z <= hi[j].z;
+            if (!overlap)
+                continue;
+            float const dx = fmaxf(hi[j].x, U.x) - fminf(lo[j].x, L.x);
+            float const dy = fmaxf(hi[j].y, U.y) - fminf(lo[j].y, L.y);
+            float const dz = fmaxf(hi[j].z, U.z) - fminf(lo[j].z, L.z);
+            float const q = dx * dx + dy * dy + dz * dz;
+            d2[j] = hit[j] ? fmaxf(d2[j], q) : q;
+            hit[j] = true;
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
@@ -309,13 +290,12 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
 // otherwise; in triangle order, at most kMaxContacts. finalize = 0
 // (UpdateActiveSet, gpu/impl/vbd/Integrator.cu:250-271): they become the
 // vertex's contacts. finalize = 1 (FinalizeActiveSet): the vertex stays active
-// when it lies below the plane of the last of them. kGroup collision vertices
-// per workgroup of kContactTile; two passes over LDS tiles of triangle
-// positions, the second collecting in triangle order with wave ballots.
-[[maybe_unused]] static __global__ __launch_bounds__(kContactTile) void ContactNearest(
+// when it lies below the plane of the last of them. kPerWave collision vertices
+// per wave; two passes over the triangle chunks, the second collecting in
+// triangle order with wave ballots.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactNearest(
     ContactArgs c, float4 const* __restrict__ x, int finalize)
 {
-    __shared__ float4 sA[kContactTile], sB[kContactTile], sC[kContactTile];
     int const wave = static_cast<int>(threadIdx.x) >> 6, lane = static_cast<int>(threadIdx.x) & 63;
     int const v0 = static_cast<int>(blockIdx.x) * kGroup + wave * kPerWave;
     float4 p[kPerWave];
@@ -323,6 +303,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
     bool mine[kPerWave];
     float r[kPerWave], dmin[kPerWave], thr[kPerWave];
     int count[kPerWave], last[kPerWave];
+    bool anyWork = false;
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
     {
@@ -341,15 +322,11 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
         thr[j] = -1.f;
         count[j] = 0;
         last[j] = -1;
-    }
-    bool anyWork = false;
-#pragma unroll
-    for (int j = 0; j < kPerWave; ++j)
         anyWork = anyWork || mine[j];
-    // the whole workgroup skips when none of its vertices is active
-    if (!__syncthreads_or(anyWork ? 1 : 0))
+    }
+    if (!anyWork)
         return;
-    bool any[kPerWave] = {};
+    int const nChunks = (c.nF + kChunk - 1) / kChunk;
     for (int pass = 0; pass < 2; ++pass)
     {
         if (pass == 1)
@@ -360,89 +337,63 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
                 if (!mine[j])
                     continue;
                 float const ds = WaveMin(dmin[j]);
-                if (!any[j])
-                    continue;
+                if (!(ds < 3.402823466e+38f))
+                    continue; // no triangle of another body within reach
                 if (ds < r[j] - kFltEpsilon)
                     thr[j] = ds + kFltEpsilon;
                 else if (ds <= r[j] + kFltEpsilon)
                     thr[j] = r[j] + kFltEpsilon;
             }
         }
-        for (int base = 0; base < c.nF; base += kContactTile)
+        for (int ch = 0; ch < nChunks; ++ch)
         {
-            // skip tiles whose box lies beyond every query's bound: r + eps in
+            // skip chunks whose box lies beyond every query's bound: r + eps in
             // pass 0 (no triangle farther can matter), the collection threshold
             // in pass 1
-            float4 const TL = c.tileLo[base / kContactTile], TU = c.tileHi[base / kContactTile];
+            float4 const TL = c.chunkLo[ch], TU = c.chunkHi[ch];
             bool need[kPerWave];
-            int anyNeed = 0;
+            bool anyNeed = false;
 #pragma unroll
             for (int j = 0; j < kPerWave; ++j)
             {
                 float const bound = pass == 0 ? r[j] + kFltEpsilon : thr[j];
                 need[j] = mine[j] && (pass == 0 || (thr[j] >= 0.f && count[j] < kMaxContacts)) &&
                           TileWithin(PointBoxSqDist(p[j], TL, TU), bound);
-                anyNeed |= need[j] ? 1 : 0;
+                anyNeed = anyNeed || need[j];
             }
-            if (!__syncthreads_or(anyNeed))
+            if (!anyNeed)
                 continue;
-            int const f = base + static_cast<int>(threadIdx.x);
-            if (f < c.nF)
+            int const f = ch * kChunk + lane;
+            bool const valid = f < c.nF;
+            float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Bv = A, Cv = A;
+            int tb = 0;
+            if (valid)
             {
                 int4 const t = c.tri[f];
-                float4 a = x[t.x];
-                a.w = __int_as_float(t.w);
-                sA[threadIdx.x] = a;
-                sB[threadIdx.x] = x[t.y];
-                sC[threadIdx.x] = x[t.z];
+                A = x[t.x], Bv = x[t.y], Cv = x[t.z];
+                tb = t.w;
             }
-            __syncthreads();
-            int const m = min(kContactTile, c.nF - base);
 #pragma unroll
             for (int j = 0; j < kPerWave; ++j)
             {
                 if (!need[j])
                     continue;
+                bool const other = valid && tb != body[j];
                 if (pass == 0)
                 {
-                    for (int k = lane; k < m; k += 64)
-                    {
-                        float4 const a = sA[k];
-                        if (__float_as_int(a.w) == body[j])
-                            continue;
-                        dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], a, sB[k], sC[k]));
-                    }
+                    if (other)
+                        dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], A, Bv, Cv));
                     continue;
                 }
-                if (thr[j] < 0.f || count[j] >= kMaxContacts)
-                    continue;
-                // chunks of 64 triangles in order; lane l takes triangle base + q + l
-                for (int q = 0; q < m && count[j] < kMaxContacts; q += 64)
-                {
-                    int const k = q + lane;
-                    bool take = false;
-                    if (k < m)
-                    {
-                        float4 const a = sA[k];
-                        take = __float_as_int(a.w) != body[j] &&
-                               PointTriangleSqDist(p[j], a, sB[k], sC[k]) <= thr[j];
-                    }
-                    unsigned long long const mask = __ballot(take);
-                    int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
-                    bool const kept = take && slot < kMaxContacts;
-                    if (kept && !finalize)
-                        c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = base + k;
-                    last[j] = max(last[j], WaveMaxInt(kept ? base + k : -1));
-                    count[j] += __popcll(mask);
-                }
+                bool const take = other && PointTriangleSqDist(p[j], A, Bv, Cv) <= thr[j];
+                unsigned long long const mask = __ballot(take);
+                int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
+                bool const kept = take && slot < kMaxContacts;
+                if (kept && !finalize)
+                    c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = f;
+                last[j] = max(last[j], WaveMaxInt(kept ? f : -1));
+                count[j] += __popcll(mask);
             }
-            __syncthreads();
-        }
-        if (pass == 0)
-        {
-#pragma unroll
-            for (int j = 0; j < kPerWave; ++j)
-                any[j] = mine[j] && __ballot(dmin[j] < 3.402823466e+38f) != 0ull;
         }
     }
 #pragma unroll

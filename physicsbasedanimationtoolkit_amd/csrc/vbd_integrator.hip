@@ -479,9 +479,9 @@ void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d)
     h.dupper.upload(std::vector<float>(static_cast<size_t>(nCV), std::numeric_limits<float>::max()), h.stream);
     h.triLo.allocate(static_cast<size_t>(nF));
     h.triHi.allocate(static_cast<size_t>(nF));
-    size_t const nTiles = static_cast<size_t>((nF + device::kContactTile - 1) / device::kContactTile);
-    h.tileLo.allocate(nTiles);
-    h.tileHi.allocate(nTiles);
+    size_t const nChunks = static_cast<size_t>((nF + device::kChunk - 1) / device::kChunk);
+    h.chunkLo.allocate(nChunks);
+    h.chunkHi.allocate(nChunks);
     h.xfull.allocate(static_cast<size_t>(nV));
     h.xb.allocate(static_cast<size_t>(nV));
 }
@@ -492,7 +492,7 @@ device::ContactArgs MakeContactArgs(pbat_vbd_integrator& h)
     c.cvIdx = h.cvIdx.ptr, c.cvBody = h.cvBody.ptr, c.tri = h.tri.ptr, c.triArea = h.triArea.ptr;
     c.xva = h.xva.ptr, c.fc = h.fc.ptr, c.active = h.cvActive.ptr, c.inAv = h.cvInAv.ptr;
     c.dupper = h.dupper.ptr, c.triLo = h.triLo.ptr, c.triHi = h.triHi.ptr;
-    c.tileLo = h.tileLo.ptr, c.tileHi = h.tileHi.ptr;
+    c.chunkLo = h.chunkLo.ptr, c.chunkHi = h.chunkHi.ptr;
     c.nCV = h.nCV, c.nF = h.nF;
     return c;
 }
@@ -506,13 +506,13 @@ void ContactBeginStep(pbat_vbd_integrator& h, float dt)
                        static_cast<float4 const*>(h.xt.ptr), static_cast<float4 const*>(h.vel.ptr),
                        static_cast<float4 const*>(h.aext.ptr), n, dt, dt * dt, h.xfull.ptr);
     device::ContactArgs const c = MakeContactArgs(h);
-    int const nTiles = (h.nF + device::kContactTile - 1) / device::kContactTile;
+    int const boxBlocks = (h.nF + device::kContactBlock - 1) / device::kContactBlock;
     hipLaunchKernelGGL(device::ContactSweptBoxes, dim3((h.nF + 255) / 256), dim3(256), 0, h.stream, c,
                        static_cast<float4 const*>(h.xt.ptr), static_cast<float4 const*>(h.xfull.ptr));
-    hipLaunchKernelGGL(device::ContactTileBoxes, dim3(nTiles), dim3(device::kContactTile), 0, h.stream, c,
+    hipLaunchKernelGGL(device::ContactChunkBoxes, dim3(boxBlocks), dim3(device::kContactBlock), 0, h.stream, c,
                        static_cast<float4 const*>(nullptr), 1);
     hipLaunchKernelGGL(device::ContactBroadphase, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
-                       dim3(device::kContactTile), 0, h.stream, c, static_cast<float4 const*>(h.xt.ptr),
+                       dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xt.ptr),
                        static_cast<float4 const*>(h.xfull.ptr));
 }
 
@@ -522,11 +522,11 @@ void ContactBeginStep(pbat_vbd_integrator& h, float dt)
 void ContactNearestPass(pbat_vbd_integrator& h, int finalize)
 {
     device::ContactArgs const c = MakeContactArgs(h);
-    int const nTiles = (h.nF + device::kContactTile - 1) / device::kContactTile;
-    hipLaunchKernelGGL(device::ContactTileBoxes, dim3(nTiles), dim3(device::kContactTile), 0, h.stream, c,
+    int const boxBlocks = (h.nF + device::kContactBlock - 1) / device::kContactBlock;
+    hipLaunchKernelGGL(device::ContactChunkBoxes, dim3(boxBlocks), dim3(device::kContactBlock), 0, h.stream, c,
                        static_cast<float4 const*>(h.xm.ptr), 0);
     hipLaunchKernelGGL(device::ContactNearest, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
-                       dim3(device::kContactTile), 0, h.stream, c, static_cast<float4 const*>(h.xm.ptr), finalize);
+                       dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xm.ptr), finalize);
 }
 
 int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool eventBrackets)

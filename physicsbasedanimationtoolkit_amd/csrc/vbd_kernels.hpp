@@ -320,9 +320,18 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     }
     Block B{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool const nt = a.ntStream != 0;
-    // copy the previous colour's new positions back (contact sweeps only)
+    // contact sweeps: the vertex's contact list and start position, issued now;
+    // then copy the previous colour's new positions back
+    int4 fa = make_int4(-1, -1, -1, -1), fb = fa;
+    float4 xto = own;
     if constexpr (C)
     {
+        if (active)
+        {
+            xto = a.xt[v];
+            fa = *reinterpret_cast<int4 const*>(a.contact.fc + static_cast<size_t>(v) * kMaxContacts);
+            fb = *reinterpret_cast<int4 const*>(a.contact.fc + static_cast<size_t>(v) * kMaxContacts + 4);
+        }
         int const pend = a.contact.pendBeg + tid;
         if (pend < a.contact.pendEnd)
             a.xm[pend] = a.contact.xb[pend];
@@ -382,36 +391,50 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
             AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
                                __int_as_float(J.w), own, a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
     }
-    GroupSum<L>(B);
-    if (!active || lane != 0)
-        return;
     if constexpr (C)
     {
-        float4 const xto = a.xt[v];
-        Damp(B, own, xto, a.c);
+        // contact terms (after damping, before inertia): the group's lanes take
+        // the vertex's contacts k = lane, lane + L, ... and sum them apart from
+        // the elastic block, which is damped alone
         ContactSweep const& cs = a.contact;
-        int4 const fa = *reinterpret_cast<int4 const*>(cs.fc + static_cast<size_t>(v) * kMaxContacts);
-        if (fa.x >= 0)
+        Block Cb{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (active)
         {
-            int4 const fb = *reinterpret_cast<int4 const*>(cs.fc + static_cast<size_t>(v) * kMaxContacts + 4);
-            int const f[kMaxContacts] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
-            int n = 0;
+            if (fa.x >= 0)
+            {
+                int const f[kMaxContacts] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
+                int n = 0;
+                float sumfa = 0.f;
 #pragma unroll
-            for (int k = 0; k < kMaxContacts; ++k)
-                n += f[k] >= 0 ? 1 : 0;
-            float sumfa = 0.f;
-            for (int k = 0; k < n; ++k)
-                sumfa += cs.triArea[f[k]];
-            float const kC = cs.xva[v] * cs.muC / sumfa;
-            float3 const xo = make_float3(own.x, own.y, own.z), xt3 = make_float3(xto.x, xto.y, xto.z);
-            for (int k = 0; k < n; ++k)
-                AccumulateContact(xt3, xo, cs, a.xm, a.xt, cs.tri[f[k]], a.c.dt, kC * cs.triArea[f[k]], cs.muF,
-                                  cs.epsv, B);
+                for (int k = 0; k < kMaxContacts; ++k)
+                    if (f[k] >= 0)
+                    {
+                        ++n;
+                        sumfa += cs.triArea[f[k]];
+                    }
+                float const kC = cs.xva[v] * cs.muC / sumfa;
+                float3 const xo = make_float3(own.x, own.y, own.z), xt3 = make_float3(xto.x, xto.y, xto.z);
+#pragma unroll
+                for (int k = 0; k < kMaxContacts; ++k)
+                    if (k < n && (k & (L - 1)) == lane)
+                        AccumulateContact(xt3, xo, cs, a.xm, a.xt, cs.tri[f[k]], a.c.dt, kC * cs.triArea[f[k]],
+                                          cs.muF, cs.epsv, Cb);
+            }
         }
+        GroupSum<L>(B);
+        GroupSum<L>(Cb);
+        if (!active || lane != 0)
+            return;
+        Damp(B, own, xto, a.c);
+        B.h00 += Cb.h00, B.h01 += Cb.h01, B.h02 += Cb.h02, B.h11 += Cb.h11, B.h12 += Cb.h12, B.h22 += Cb.h22;
+        B.g0 += Cb.g0, B.g1 += Cb.g1, B.g2 += Cb.g2;
         float3 const dx = InertiaNewtonStep(B, own, xl, a.c);
         cs.xb[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
         return;
     }
+    GroupSum<L>(B);
+    if (!active || lane != 0)
+        return;
     float3 const dx = NewtonStep(B, own, xl, (a.c.kD != 0.f) ? xtv : own, a.c);
     a.xm[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
 }
