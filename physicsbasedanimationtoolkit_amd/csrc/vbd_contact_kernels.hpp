@@ -190,39 +190,46 @@ __device__ inline int WaveMaxInt(int x)
         }
     }
     int const nChunks = (c.nF + kChunk - 1) / kChunk;
-    for (int ch = 0; ch < nChunks; ++ch)
+    // 64 chunk boxes at a time, one per lane: the lanes test them against the
+    // wave's segments, and the wave visits the chunks that any segment overlaps
+    for (int cb = 0; cb < nChunks; cb += 64)
     {
-        float4 const TL = c.chunkLo[ch], TU = c.chunkHi[ch];
-        bool need[kPerWave];
-        bool anyNeed = false;
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j)
+        int bits = 0;
+        if (cb + lane < nChunks)
         {
-            need[j] = v0 + j < c.nCV && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
-                      TL.y <= hi[j].y && TL.z <= hi[j].z;
-            anyNeed = anyNeed || need[j];
+            float4 const TL = c.chunkLo[cb + lane], TU = c.chunkHi[cb + lane];
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+                if (v0 + j < c.nCV && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
+                    TL.y <= hi[j].y && TL.z <= hi[j].z)
+                    bits |= 1 << j;
         }
-        if (!anyNeed)
-            continue;
-        int const f = ch * kChunk + lane;
-        if (f >= c.nF)
-            continue;
-        float4 const L = c.triLo[f], U = c.triHi[f];
-#pragma unroll
-        for (int j = 0; j < kPerWave; ++j)
+        unsigned long long todo = __ballot(bits != 0);
+        while (todo)
         {
-            if (!need[j] || __float_as_int(L.w) == body[j])
+            int const k = __ffsll(static_cast<long long>(todo)) - 1;
+            todo &= todo - 1ull;
+            int const need = __shfl(bits, k);
+            int const f = (cb + k) * kChunk + lane;
+            if (f >= c.nF)
                 continue;
-            bool const overlap = lo[j].x <= U.x && lo[j].y <= U.y && lo[j].z <= U.z && L.x <= hi[j].x &&
-                                 L.y <= hi[j].y && L.z <= hi[j].z;
-            if (!overlap)
-                continue;
-            float const dx = fmaxf(hi[j].x, U.x) - fminf(lo[j].x, L.x);
-            float const dy = fmaxf(hi[j].y, U.y) - fminf(lo[j].y, L.y);
-            float const dz = fmaxf(hi[j].z, U.z) - fminf(lo[j].z, L.z);
-            float const q = dx * dx + dy * dy + dz * dz;
-            d2[j] = hit[j] ? fmaxf(d2[j], q) : q;
-            hit[j] = true;
+            float4 const L = c.triLo[f], U = c.triHi[f];
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+            {
+                if (!((need >> j) & 1) || __float_as_int(L.w) == body[j])
+                    continue;
+                bool const overlap = lo[j].x <= U.x && lo[j].y <= U.y && lo[j].z <= U.z && L.x <= hi[j].x &&
+                                     L.y <= hi[j].y && L.z <= hi[j].z;
+                if (!overlap)
+                    continue;
+                float const dx = fmaxf(hi[j].x, U.x) - fminf(lo[j].x, L.x);
+                float const dy = fmaxf(hi[j].y, U.y) - fminf(lo[j].y, L.y);
+                float const dz = fmaxf(hi[j].z, U.z) - fminf(lo[j].z, L.z);
+                float const q = dx * dx + dy * dy + dz * dz;
+                d2[j] = hit[j] ? fmaxf(d2[j], q) : q;
+                hit[j] = true;
+            }
         }
     }
 #pragma unroll
@@ -345,54 +352,64 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
                     thr[j] = r[j] + kFltEpsilon;
             }
         }
-        for (int ch = 0; ch < nChunks; ++ch)
+        // 64 chunk boxes at a time, one per lane; the wave visits, in order, the
+        // chunks whose box lies within some query's bound: r + eps in pass 0 (no
+        // triangle farther can matter), the collection threshold in pass 1
+        for (int cb = 0; cb < nChunks; cb += 64)
         {
-            // skip chunks whose box lies beyond every query's bound: r + eps in
-            // pass 0 (no triangle farther can matter), the collection threshold
-            // in pass 1
-            float4 const TL = c.chunkLo[ch], TU = c.chunkHi[ch];
-            bool need[kPerWave];
-            bool anyNeed = false;
+            int bits = 0;
+            if (cb + lane < nChunks)
+            {
+                float4 const TL = c.chunkLo[cb + lane], TU = c.chunkHi[cb + lane];
 #pragma unroll
-            for (int j = 0; j < kPerWave; ++j)
-            {
-                float const bound = pass == 0 ? r[j] + kFltEpsilon : thr[j];
-                need[j] = mine[j] && (pass == 0 || (thr[j] >= 0.f && count[j] < kMaxContacts)) &&
-                          TileWithin(PointBoxSqDist(p[j], TL, TU), bound);
-                anyNeed = anyNeed || need[j];
-            }
-            if (!anyNeed)
-                continue;
-            int const f = ch * kChunk + lane;
-            bool const valid = f < c.nF;
-            float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Bv = A, Cv = A;
-            int tb = 0;
-            if (valid)
-            {
-                int4 const t = c.tri[f];
-                A = x[t.x], Bv = x[t.y], Cv = x[t.z];
-                tb = t.w;
-            }
-#pragma unroll
-            for (int j = 0; j < kPerWave; ++j)
-            {
-                if (!need[j])
-                    continue;
-                bool const other = valid && tb != body[j];
-                if (pass == 0)
+                for (int j = 0; j < kPerWave; ++j)
                 {
-                    if (other)
-                        dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], A, Bv, Cv));
-                    continue;
+                    float const bound = pass == 0 ? r[j] + kFltEpsilon : thr[j];
+                    if (mine[j] && (pass == 0 || thr[j] >= 0.f) && TileWithin(PointBoxSqDist(p[j], TL, TU), bound))
+                        bits |= 1 << j;
                 }
-                bool const take = other && PointTriangleSqDist(p[j], A, Bv, Cv) <= thr[j];
-                unsigned long long const mask = __ballot(take);
-                int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
-                bool const kept = take && slot < kMaxContacts;
-                if (kept && !finalize)
-                    c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = f;
-                last[j] = max(last[j], WaveMaxInt(kept ? f : -1));
-                count[j] += __popcll(mask);
+            }
+            unsigned long long todo = __ballot(bits != 0);
+            while (todo)
+            {
+                int const k = __ffsll(static_cast<long long>(todo)) - 1;
+                todo &= todo - 1ull;
+                int const needBits = __shfl(bits, k);
+                bool need[kPerWave];
+#pragma unroll
+                for (int j = 0; j < kPerWave; ++j)
+                    need[j] = ((needBits >> j) & 1) && (pass == 0 || count[j] < kMaxContacts);
+                int const f = (cb + k) * kChunk + lane;
+                bool const valid = f < c.nF;
+                float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Bv = A, Cv = A;
+                int tb = 0;
+                if (valid)
+                {
+                    int4 const t = c.tri[f];
+                    A = x[t.x], Bv = x[t.y], Cv = x[t.z];
+                    tb = t.w;
+                }
+#pragma unroll
+                for (int j = 0; j < kPerWave; ++j)
+                {
+                    if (!need[j])
+                        continue;
+                    bool const other = valid && tb != body[j];
+                    if (pass == 0)
+                    {
+                        if (other)
+                            dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], A, Bv, Cv));
+                        continue;
+                    }
+                    bool const take = other && PointTriangleSqDist(p[j], A, Bv, Cv) <= thr[j];
+                    unsigned long long const mask = __ballot(take);
+                    int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
+                    bool const kept = take && slot < kMaxContacts;
+                    if (kept && !finalize)
+                        c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = f;
+                    last[j] = max(last[j], WaveMaxInt(kept ? f : -1));
+                    count[j] += __popcll(mask);
+                }
             }
         }
     }
