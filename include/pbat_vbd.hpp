@@ -18,6 +18,7 @@
 
 #include "pbat_vbd.h"
 
+#include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -86,6 +87,12 @@ struct Data
     double tau = 2.;                                   // sim/vbd/Data.h:245
     bool curved = true;                                // sim/vbd/Data.h:246
     std::vector<double> mue, lambdae;
+    // collision mesh and contact model (sim/vbd/Data.h:165-178, 225-229)
+    std::vector<int64_t> B, V;
+    Matrix<int64_t> F;
+    std::vector<double> XVA, FA;
+    double muC = 1e6, muF = 0.3, epsv = 1e-3;
+    int32_t activeSetUpdateFrequency = 1;
 
     template <class TX, class TE>
     Data& WithVolumeMesh(TX const& Xin, TE const& Ein)
@@ -94,6 +101,54 @@ struct Data
             throw std::invalid_argument("expected 3 x |#nodes| positions and 4 x |#elements| elements");
         X = ToMatrix<double>(Xin);
         E = ToMatrix<int64_t>(Ein);
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:34-54: collision vertices V and triangles F (3 x |#triangles|);
+     * vertex areas get a sixth of each incident triangle's double area, triangle
+     * areas half of it. Call after WithVolumeMesh. */
+    template <class TF>
+    Data& WithSurfaceMesh(std::vector<int64_t> Vin, TF const& Fin)
+    {
+        if (Fin.rows() != 3)
+            throw std::invalid_argument("expected 3 x |#collision triangles| triangles");
+        V = std::move(Vin);
+        F = ToMatrix<int64_t>(Fin);
+        int64_t const nV = X.cols(), nF = F.cols();
+        XVA.assign(static_cast<size_t>(nV), 0.);
+        FA.assign(static_cast<size_t>(nF), 0.);
+        for (int64_t f = 0; f < nF; ++f)
+        {
+            double ab[3], ac[3];
+            for (int d = 0; d < 3; ++d)
+            {
+                ab[d] = X(d, F(1, f)) - X(d, F(0, f));
+                ac[d] = X(d, F(2, f)) - X(d, F(0, f));
+            }
+            double const cx = ab[1] * ac[2] - ab[2] * ac[1], cy = ab[2] * ac[0] - ab[0] * ac[2],
+                         cz = ab[0] * ac[1] - ab[1] * ac[0];
+            double const dbl = std::sqrt(cx * cx + cy * cy + cz * cz);
+            for (int k = 0; k < 3; ++k)
+                XVA[static_cast<size_t>(F(k, f))] += dbl / 6.;
+            FA[static_cast<size_t>(f)] = dbl / 2.;
+        }
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:56-60 */
+    Data& WithBodies(std::vector<int64_t> Bin)
+    {
+        B = std::move(Bin);
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:121-127 */
+    Data& WithContactParameters(double muCIn, double muFIn, double epsvIn)
+    {
+        muC = muCIn, muF = muFIn, epsv = epsvIn;
+        return *this;
+    }
+    /** sim/vbd/Data.cpp:129-133 */
+    Data& WithActiveSetUpdateFrequency(int32_t frequency)
+    {
+        activeSetUpdateFrequency = frequency;
         return *this;
     }
     template <class TV>
@@ -250,6 +305,22 @@ class Integrator
         d.eta = data.eta;
         d.tau = data.tau;
         d.curved = data.curved ? 1 : 0;
+        if (!data.V.empty() && data.F.cols() > 0)
+        {
+            if (data.B.size() != static_cast<size_t>(d.nV) || data.XVA.size() != static_cast<size_t>(d.nV))
+                throw std::invalid_argument("B and XVA must have |#nodes| entries");
+            d.nCollisionVertices = static_cast<int64_t>(data.V.size());
+            d.V = data.V.data();
+            d.nCollisionTriangles = data.F.cols();
+            d.F = data.F.data();
+            d.B = data.B.data();
+            d.XVA = data.XVA.data();
+            d.FA = data.FA.data();
+        }
+        d.muC = data.muC;
+        d.muF = data.muF;
+        d.epsv = data.epsv;
+        d.activeSetUpdateFrequency = data.activeSetUpdateFrequency;
         Check(pbat_vbd_integrator_create(&d, &mImpl));
         mVertices = d.nV;
     }

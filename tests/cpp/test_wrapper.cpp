@@ -47,6 +47,25 @@ int main(int argc, char** argv)
     EXPECT(std::fabs(total - 1000. * 1.0000008) < 1e-9);
     EXPECT(std::fabs(data.aext(2, 0) + 9.81) < 1e-15);
 
+    // collision mesh: the cube's 12 boundary triangles; areas sum to the surface area 6
+    int64_t const Fc[3][12] = {{0, 0, 4, 4, 0, 0, 2, 2, 0, 0, 1, 1},
+                               {2, 3, 5, 7, 1, 5, 6, 7, 4, 6, 3, 7},
+                               {3, 1, 7, 6, 5, 4, 7, 3, 6, 2, 7, 5}};
+    Matrix<int64_t> F(3, 12);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 12; ++c)
+            F(r, c) = Fc[r][c];
+    pbat_amd::sim::vbd::Data surf;
+    surf.WithVolumeMesh(X, E).WithSurfaceMesh({0, 1, 2, 3, 4, 5, 6, 7}, F).WithContactParameters(1e5, 0.2, 1e-3);
+    double sumFA = 0., sumXVA = 0.;
+    for (double a : surf.FA)
+        sumFA += a;
+    for (double a : surf.XVA)
+        sumXVA += a;
+    EXPECT(std::fabs(sumFA - 6.) < 1e-12 && std::fabs(sumXVA - 6.) < 1e-12);
+    EXPECT(std::fabs(surf.XVA[0] - 1.) < 1e-12); // corner 0 lies in 6 triangles of double area 1: 6 x 1/6
+    EXPECT(surf.muC == 1e5 && surf.muF == 0.2);
+
     bool threw = false;
     try
     {
