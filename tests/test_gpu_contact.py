@@ -214,3 +214,53 @@ def test_single_colour_and_narrow_launches_match_oracle(gpu_device):
         _, fc = gpu.contacts()
         np.testing.assert_array_equal(fc, ref.fc)
         assert rel(gpu.x, ref.positions) <= CONTACT_POS_TOL
+
+
+@pytest.mark.parametrize("accel", ["chebyshev", "anderson", "trust_region_curved"])
+def test_contact_with_accelerated_integrators(gpu_device, accel):
+    """Contact composes with the accelerated integrators (each acts between iterations):
+    graph replay equals eager launches bit for bit, the cube stays on the slab, and the
+    curved trust-region path equals plain VBD with contact bit for bit."""
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(np.full(nT, 1e6), 0.45)
+
+    def make(acc):
+        d = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B)
+             .with_material(np.full(nT, 1e3), mu, lam).with_acceleration(meshes.gravity(X.shape[1]))
+             .with_velocity(v).with_dirichlet_vertices(dbc))
+        if acc == "chebyshev":
+            d = d.with_chebyshev_acceleration(0.5)
+        elif acc == "anderson":
+            d = d.with_anderson_acceleration(3)
+        elif acc == "trust_region_curved":
+            d = d.with_trust_region_acceleration(0.2, 2.0, True)
+        return pbat.gpu.vbd.Integrator(d.construct())
+
+    g1, g2 = make(accel), make(accel)
+    g2.graph_capture = False
+    plain = make(None) if accel == "trust_region_curved" else None
+    lowest = []
+    for _ in range(10):
+        g1.step(DT, 15, 1)
+        g2.step(DT, 15, 1)
+        assert np.array_equal(g1.x, g2.x)
+        if plain is not None:
+            plain.step(DT, 15, 1)
+            assert np.array_equal(g1.x, plain.x)
+        lowest.append(g1.x[1, B == 1].min())
+    assert np.isfinite(g1.x).all() and min(lowest) > -0.03
+
+
+def test_traced_step_with_contact_equals_step(gpu_device, tmp_path):
+    """A traced step runs the same contact detection as a step: same positions, same
+    contacts."""
+    X, E, F, V, B, v, dbc = cube_on_slab(gap=0.001)
+    g1, _ = contact_pair(X, E, F, V, B, v, dbc)
+    g2, _ = contact_pair(X, E, F, V, B, v, dbc)
+    g2.graph_capture = False
+    for t in range(3):
+        g1.traced_step(DT, 5, 2, t=t, dir=str(tmp_path))
+        g2.step(DT, 5, 2)
+        assert np.array_equal(g1.x, g2.x)
+        assert all(np.array_equal(a, b) for a, b in zip(g1.contacts(), g2.contacts()))
