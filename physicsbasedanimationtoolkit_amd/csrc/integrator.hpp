@@ -275,6 +275,9 @@ struct pbat_vbd_integrator
         aggOf.reset(), aggMesh.reset(), meshAggBegin.reset(), meshCoarseOffset.reset();
         aggVertPtr.reset(), aggVerts.reset();
         gkm1.reset(), fkm1.reset(), dgk.reset(), dfk.reset(), andersonAcc.reset(), andersonGram.reset();
+        trPartials.reset(), cvIdx.reset(), cvBody.reset(), fc.reset(), cvActive.reset(), cvInAv.reset();
+        tri.reset(), triArea.reset(), xva.reset(), dupper.reset();
+        triLo.reset(), triHi.reset(), chunkLo.reset(), chunkHi.reset(), xfull.reset(), xb.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)
             (void)hipStreamDestroy(stream);
@@ -298,5 +301,13 @@ void UploadVec(pbat_vbd_integrator& h, float4* dst, float const* src, float cons
 void DownloadVec(pbat_vbd_integrator& h, float4 const* src, float* dst);
 void CheckVertexArray(pbat_vbd_integrator const& h, void const* p, int64_t n, char const* what);
 pbat_vbd_integrator& Handle(pbat_vbd_integrator* h);
+// Vertex-triangle contact (vbd_contact.hip): set-up from the data, the step's active
+// set, contacts (finalize = 0) or the end-of-step active flags (finalize = 1), and
+// the copy of the last contact-swept colour back from the write buffer.
+bool CollisionMeshSpansBodies(pbat_vbd_data const& d);
+void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d);
+void ContactBeginStep(pbat_vbd_integrator& h, float dt);
+void ContactNearestPass(pbat_vbd_integrator& h, int finalize);
+void FlushContactSweep(pbat_vbd_integrator& h);
 
 } // namespace pbat_vbd

@@ -69,16 +69,6 @@ int EffectivePerLane(pbat_vbd_integrator const& h)
     return std::clamp((d + L - 1) / L, 1, 4);
 }
 
-// Copy the last contact sweep's colour back from the write buffer.
-void FlushContactSweep(pbat_vbd_integrator& h)
-{
-    int const n = h.pendEnd - h.pendBeg;
-    if (n > 0)
-        hipLaunchKernelGGL(device::CopyColour, dim3((n + 255) / 256), dim3(256), 0, h.stream, h.xm.ptr,
-                           static_cast<float4 const*>(h.xb.ptr), h.pendBeg, h.pendEnd);
-    h.pendBeg = h.pendEnd = 0;
-}
-
 template <int L, int K>
 void LaunchSweepLK(pbat_vbd_integrator& h, device::SweepArgs a, int nVerts)
 {
@@ -401,132 +391,6 @@ int TrustRegionSolve(pbat_vbd_integrator& h, device::SweepArgs& a, int iteration
 bool LinearTrustRegion(pbat_vbd_integrator const& h)
 {
     return h.acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION && !h.curved;
-}
-
-// Whether the collision mesh of d holds vertices or triangles of two bodies.
-bool CollisionMeshSpansBodies(pbat_vbd_data const& d)
-{
-    if (d.nCollisionVertices <= 0 || d.nCollisionTriangles <= 0 || !d.B || !d.V || !d.F)
-        return false;
-    int64_t const b0 = d.B[d.V[0]];
-    for (int64_t k = 0; k < d.nCollisionVertices; ++k)
-        if (d.B[d.V[k]] != b0)
-            return true;
-    for (int64_t k = 0; k < d.nCollisionTriangles; ++k)
-        if (d.B[d.F[3 * k]] != b0)
-            return true;
-    return false;
-}
-
-// Collision mesh of the data (sim/vbd/Data.cpp:34-60): contact is on when the
-// collision mesh holds vertices or triangles of two bodies; otherwise the
-// reference's detector can never find a pair and nothing is set up.
-void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d)
-{
-    int64_t const nV = h.nV, nCV = d->nCollisionVertices, nF = d->nCollisionTriangles;
-    if (nCV < 0 || nF < 0 || (nCV > 0 && !d->V) || (nF > 0 && !d->F))
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid collision mesh");
-    if (nCV == 0 || nF == 0 || !d->B)
-        return;
-    for (int64_t k = 0; k < nCV; ++k)
-        if (d->V[k] < 0 || d->V[k] >= nV)
-            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "collision vertex index out of range");
-    for (int64_t k = 0; k < 3 * nF; ++k)
-        if (d->F[k] < 0 || d->F[k] >= nV)
-            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "collision triangle vertex index out of range");
-    if (!CollisionMeshSpansBodies(*d))
-        return;
-    if (!d->XVA || !d->FA)
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "contact needs vertex areas XVA and triangle areas FA");
-    if (nCV > (int64_t{1} << 30) || nF > (int64_t{1} << 30))
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "collision mesh too large");
-    if (d->activeSetUpdateFrequency < 1)
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "active set update frequency must be >= 1");
-    if (h.acceleration == PBAT_VBD_ACCELERATION_TRUST_REGION && !h.curved)
-        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT,
-             "the linear trust-region path is not available with contact (its objective has no contact term)");
-    h.contact = true;
-    h.nCV = static_cast<int>(nCV), h.nF = static_cast<int>(nF);
-    h.muC = static_cast<float>(d->muC), h.muF = static_cast<float>(d->muF), h.epsv = static_cast<float>(d->epsv);
-    h.activeSetFrequency = d->activeSetUpdateFrequency;
-    std::vector<int> cvIdx(static_cast<size_t>(nCV)), cvBody(cvIdx.size());
-    h.cvOrig.assign(d->V, d->V + nCV);
-    for (int64_t k = 0; k < nCV; ++k)
-    {
-        cvIdx[k] = h.iperm[d->V[k]];
-        cvBody[k] = static_cast<int>(d->B[d->V[k]]);
-    }
-    std::vector<int4> tri(static_cast<size_t>(nF));
-    std::vector<float> area(tri.size()), xva(static_cast<size_t>(nV));
-    for (int64_t f = 0; f < nF; ++f)
-    {
-        tri[f] = make_int4(h.iperm[d->F[3 * f]], h.iperm[d->F[3 * f + 1]], h.iperm[d->F[3 * f + 2]],
-                           static_cast<int>(d->B[d->F[3 * f]]));
-        area[f] = static_cast<float>(d->FA[f]);
-    }
-    for (int64_t i = 0; i < nV; ++i)
-        xva[h.iperm[i]] = static_cast<float>(d->XVA[i]);
-    h.cvIdx.upload(cvIdx, h.stream);
-    h.cvBody.upload(cvBody, h.stream);
-    h.tri.upload(tri, h.stream);
-    h.triArea.upload(area, h.stream);
-    h.xva.upload(xva, h.stream);
-    // no contacts, nothing active, search radii at FLT_MAX
-    // (gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:22-48, gpu/impl/vbd/Integrator.cu:60)
-    h.fc.upload(std::vector<int>(static_cast<size_t>(nV) * device::kMaxContacts, -1), h.stream);
-    h.cvActive.upload(std::vector<int>(static_cast<size_t>(nCV), 0), h.stream);
-    h.cvInAv.upload(std::vector<int>(static_cast<size_t>(nCV), 0), h.stream);
-    h.dupper.upload(std::vector<float>(static_cast<size_t>(nCV), std::numeric_limits<float>::max()), h.stream);
-    h.triLo.allocate(static_cast<size_t>(nF));
-    h.triHi.allocate(static_cast<size_t>(nF));
-    size_t const nChunks = static_cast<size_t>((nF + device::kChunk - 1) / device::kChunk);
-    h.chunkLo.allocate(nChunks);
-    h.chunkHi.allocate(nChunks);
-    h.xfull.allocate(static_cast<size_t>(nV));
-    h.xb.allocate(static_cast<size_t>(nV));
-}
-
-device::ContactArgs MakeContactArgs(pbat_vbd_integrator& h)
-{
-    device::ContactArgs c{};
-    c.cvIdx = h.cvIdx.ptr, c.cvBody = h.cvBody.ptr, c.tri = h.tri.ptr, c.triArea = h.triArea.ptr;
-    c.xva = h.xva.ptr, c.fc = h.fc.ptr, c.active = h.cvActive.ptr, c.inAv = h.cvInAv.ptr;
-    c.dupper = h.dupper.ptr, c.triLo = h.triLo.ptr, c.triHi = h.triHi.ptr;
-    c.chunkLo = h.chunkLo.ptr, c.chunkHi = h.chunkHi.ptr;
-    c.nCV = h.nCV, c.nF = h.nF;
-    return c;
-}
-
-// InitializeActiveSet (gpu/impl/vbd/Integrator.cu:163-188): the active set of the
-// step from the sweep of every vertex over the full step, x_t -> x_t + dt v + dt^2 a.
-void ContactBeginStep(pbat_vbd_integrator& h, float dt)
-{
-    int const n = static_cast<int>(h.nV);
-    hipLaunchKernelGGL(device::ContactFullStep, dim3((n + 255) / 256), dim3(256), 0, h.stream,
-                       static_cast<float4 const*>(h.xt.ptr), static_cast<float4 const*>(h.vel.ptr),
-                       static_cast<float4 const*>(h.aext.ptr), n, dt, dt * dt, h.xfull.ptr);
-    device::ContactArgs const c = MakeContactArgs(h);
-    int const boxBlocks = (h.nF + device::kContactBlock - 1) / device::kContactBlock;
-    hipLaunchKernelGGL(device::ContactSweptBoxes, dim3((h.nF + 255) / 256), dim3(256), 0, h.stream, c,
-                       static_cast<float4 const*>(h.xt.ptr), static_cast<float4 const*>(h.xfull.ptr));
-    hipLaunchKernelGGL(device::ContactChunkBoxes, dim3(boxBlocks), dim3(device::kContactBlock), 0, h.stream, c,
-                       static_cast<float4 const*>(nullptr), 1);
-    hipLaunchKernelGGL(device::ContactBroadphase, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
-                       dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xt.ptr),
-                       static_cast<float4 const*>(h.xfull.ptr));
-}
-
-// UpdateActiveSet (finalize = 0, gpu/impl/vbd/Integrator.cu:250-271) or
-// FinalizeActiveSet (finalize = 1, gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:185-215)
-// at the current positions.
-void ContactNearestPass(pbat_vbd_integrator& h, int finalize)
-{
-    device::ContactArgs const c = MakeContactArgs(h);
-    int const boxBlocks = (h.nF + device::kContactBlock - 1) / device::kContactBlock;
-    hipLaunchKernelGGL(device::ContactChunkBoxes, dim3(boxBlocks), dim3(device::kContactBlock), 0, h.stream, c,
-                       static_cast<float4 const*>(h.xm.ptr), 0);
-    hipLaunchKernelGGL(device::ContactNearest, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
-                       dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xm.ptr), finalize);
 }
 
 int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool eventBrackets)
@@ -1700,33 +1564,6 @@ extern "C" int pbat_vbd_integrator_counts(pbat_vbd_integrator* h, int64_t* nPart
             PBAT_HIP(hipMemcpy(P.data(), H.vtp.ptr, P.size() * sizeof(int), hipMemcpyDeviceToHost));
             s = P.back() - P.front();
             *nIncidences = s;
-        }
-    });
-}
-
-extern "C" int pbat_vbd_integrator_contacts(pbat_vbd_integrator* h, int32_t* active, int32_t* fc)
-{
-    return Guard([&] {
-        auto& H = Handle(h);
-        if (!H.contact)
-            Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "contact is off for this integrator");
-        PBAT_HIP(hipSetDevice(H.device));
-        PBAT_HIP(hipStreamSynchronize(H.stream));
-        if (active)
-        {
-            std::vector<int> a(static_cast<size_t>(H.nCV));
-            PBAT_HIP(hipMemcpy(a.data(), H.cvActive.ptr, a.size() * sizeof(int), hipMemcpyDeviceToHost));
-            for (int k = 0; k < H.nCV; ++k)
-                active[k] = a[k];
-        }
-        if (fc)
-        {
-            size_t const n = static_cast<size_t>(H.nV) * device::kMaxContacts;
-            std::vector<int> f(n);
-            PBAT_HIP(hipMemcpy(f.data(), H.fc.ptr, n * sizeof(int), hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < H.nV; ++i)
-                for (int k = 0; k < device::kMaxContacts; ++k)
-                    fc[device::kMaxContacts * i + k] = f[device::kMaxContacts * static_cast<size_t>(H.iperm[i]) + k];
         }
     });
 }
