@@ -291,16 +291,22 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     float4 const* dinv = jacobi ? R.dinv.ptr : nullptr;
     device::CoarseArgs const cargs = MakeCoarseArgs(h);
     double* const rc = twoLevel ? R.rc.ptr : nullptr;
-    // two-level: rc = Z^T r, zc = (Z^T H Z)^-1 rc, z += Z zc with r.z
+    // two-level: rc = Z^T r, zc = (Z^T H Z)^-1 rc, z += Z zc with r.z. Rows of the
+    // coarse inverse are read by 1, 2 or 4 waves, by the largest mesh's row length.
+    int maxCoarse = 0;
+    for (int m = 0; m < nM && twoLevel; ++m)
+        maxCoarse = std::max(maxCoarse, 3 * (h.aggBegin[m + 1] - h.aggBegin[m]));
+    int const coarseWavesPerRow = maxCoarse > 512 ? 4 : maxCoarse > 256 ? 2 : 1;
     auto coarse = [&] {
         int const rows = 3 * h.nAggregates;
         hipLaunchKernelGGL(device::CoarseRestrict, dim3((h.nAggregates * 64 + 255) / 256), dim3(256), 0, h.stream,
                            static_cast<float4 const*>(R.r.ptr), static_cast<int const*>(h.aggVertPtr.ptr),
                            static_cast<int const*>(h.aggVerts.ptr), h.nAggregates, R.rc.ptr);
-        hipLaunchKernelGGL(device::CoarseApply, dim3((rows * 64 + 255) / 256), dim3(256), 0, h.stream,
-                           static_cast<double const*>(R.coarse[0].ptr), static_cast<double const*>(R.coarse[1].ptr),
-                           static_cast<int const*>(R.coarseFinal.ptr), static_cast<double const*>(R.rc.ptr),
-                           R.zc.ptr, cargs);
+        int const rowsPerBlock = 4 / coarseWavesPerRow;
+        hipLaunchKernelGGL(device::CoarseApply, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
+                           h.stream, static_cast<double const*>(R.coarse[0].ptr),
+                           static_cast<double const*>(R.coarse[1].ptr), static_cast<int const*>(R.coarseFinal.ptr),
+                           static_cast<double const*>(R.rc.ptr), R.zc.ptr, cargs, coarseWavesPerRow);
         hipLaunchKernelGGL(device::CoarseProlong, dim3(gridN), dim3(256), 0, h.stream, R.z.ptr,
                            static_cast<float4 const*>(R.r.ptr), n, cargs, static_cast<double const*>(R.zc.ptr),
                            static_cast<int const*>(h.meshOf.ptr), nM, cg.A);

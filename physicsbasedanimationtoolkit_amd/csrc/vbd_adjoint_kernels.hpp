@@ -821,26 +821,38 @@ __global__ __launch_bounds__(256) void CoarseGuard(double* __restrict__ buf0, do
         A[k] = 0.;
 }
 
-// zc = (Z^T H Z)^-1 rc per mesh; one wave per coarse row, blockDim = 256.
+// zc = (Z^T H Z)^-1 rc per mesh: wpr waves (1, 2 or 4) per coarse row, so long
+// rows (large meshes) are read by more waves in parallel; blockDim = 256.
 __global__ __launch_bounds__(256) void CoarseApply(double const* __restrict__ buf0, double const* __restrict__ buf1,
                                                    int const* __restrict__ finalBuf, double const* __restrict__ rc,
-                                                   double* __restrict__ zc, CoarseArgs c)
+                                                   double* __restrict__ zc, CoarseArgs c, int wpr)
 {
-    int const row = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    int const lane = static_cast<int>(threadIdx.x & 63);
-    if (row >= 3 * c.nAgg)
-        return;
-    int const m = c.aggMesh[row / 3];
-    int const b0 = 3 * c.meshAggBegin[m];
-    int const n = 3 * c.meshAggBegin[m + 1] - b0;
-    double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
+    __shared__ double part[4];
+    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+    int const row = static_cast<int>(blockIdx.x) * (4 / wpr) + wave / wpr;
+    int const sub = wave % wpr;
     double sum = 0.;
-    for (int j = lane; j < n; j += 64)
-        sum += A[j] * rc[b0 + j];
+    if (row < 3 * c.nAgg)
+    {
+        int const m = c.aggMesh[row / 3];
+        int const b0 = 3 * c.meshAggBegin[m];
+        int const n = 3 * c.meshAggBegin[m + 1] - b0;
+        double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
+        for (int j = sub * 64 + lane; j < n; j += 64 * wpr)
+            sum += A[j] * rc[b0 + j];
+    }
     for (int off = 32; off > 0; off >>= 1)
         sum += __shfl_xor(sum, off);
     if (lane == 0)
-        zc[row] = sum;
+        part[wave] = sum;
+    __syncthreads();
+    if (lane == 0 && sub == 0 && row < 3 * c.nAgg)
+    {
+        double t = part[wave];
+        for (int k = 1; k < wpr; ++k)
+            t += part[wave + k];
+        zc[row] = t;
+    }
 }
 
 // rc = Z^T r: one wave per aggregate sums r over the aggregate's vertices.
