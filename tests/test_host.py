@@ -117,3 +117,29 @@ def test_mesh_generator_matches_workload_table():
         assert X.shape[1] == nv and E.shape[1] == nt
         assert meshes.bottom_vertices(X).size == nd
         assert meshes.signed_volumes(X, E).min() > 0
+
+
+def test_marshal_carries_collision_mesh_and_accelerators():
+    """The C ABI view of a Data (gpu/vbd.py _marshal): collision mesh, bodies, areas and
+    contact parameters, trust-region and Anderson settings; mismatched sizes raise."""
+    from physicsbasedanimationtoolkit_amd.gpu import vbd as gvbd
+
+    X, E = meshes.cube_tet_grid(2)
+    F = meshes.boundary_triangles(E)
+    V = np.unique(F)
+    B = (X[0] > 0.5).astype(np.int64)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_surface_mesh(V, F).with_bodies(B)
+            .with_contact_parameters(2e5, 0.4, 2e-3).with_active_set_update_frequency(3)
+            .with_trust_region_acceleration(0.3, 3.0, False).construct())
+    d, keep = gvbd._marshal(data)
+    assert d.nCollisionVertices == V.size and d.nCollisionTriangles == F.shape[1]
+    assert (d.muC, d.muF, d.epsv, d.activeSetUpdateFrequency) == (2e5, 0.4, 2e-3, 3)
+    assert d.acceleration == 3 and (d.eta, d.tau, d.curved) == (0.3, 3.0, 0)
+    assert np.array_equal(keep["F"].T, F) and np.array_equal(keep["B"], B)
+    assert abs(keep["FA"].sum() - 6.0) < 1e-12 and abs(keep["XVA"].sum() - 6.0) < 1e-12
+    data.B = B[:-1]
+    with pytest.raises(ValueError):
+        gvbd._marshal(data)
+    plain = pbat.sim.vbd.Data().with_volume_mesh(X, E).with_anderson_acceleration(4).construct()
+    d, _ = gvbd._marshal(plain)
+    assert d.nCollisionVertices == 0 and d.acceleration == 2 and d.window == 4
