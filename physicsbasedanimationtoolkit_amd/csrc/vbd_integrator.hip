@@ -693,6 +693,10 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         if (d->GVGp[i + 1] < d->GVGp[i])
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "GVGp must be non-decreasing");
     int64_t const nInc = d->GVGp[nV];
+    // device kernels index vertices, incidence records and 3 or 4 entries per tet with
+    // 32-bit integers
+    if (nV >= (int64_t{1} << 31) || 4 * nT >= (int64_t{1} << 31) || nInc >= (int64_t{1} << 31))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "mesh too large for the device's 32-bit indices");
     for (int64_t k = 0; k < nInc; ++k)
         if (d->GVGe[k] < 0 || d->GVGe[k] >= nT || d->GVGilocal[k] < 0 || d->GVGilocal[k] > 3)
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "GVGe/GVGilocal entry out of range");
