@@ -531,6 +531,16 @@ def main():
                          f"{threads} threads, {cpu_s:.2f}s",
                "host": {"cpu_model": model, "os_cpu_count": os.cpu_count()}}
         log(f"[bench] cpu baseline {cpu['value']:.4e} vu/s on {threads} threads")
+        # one thread, on a shorter sample (the same first step, fewer iterations)
+        iters1 = 2
+        ref1 = O.OracleVbd(s.X, s.E, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
+                           nthreads=1)
+        t0 = time.perf_counter()
+        ref1.step(dt, iters1, 1)
+        cpu1_s = time.perf_counter() - t0
+        cpu["single_thread"] = {"value": nUpd * iters1 / cpu1_s, "unit": "vertex-updates/s", "cores": 1,
+                                "sample": f"1 step x {iters1} iterations of the same workload, {cpu1_s:.2f}s"}
+        log(f"[bench] cpu baseline {cpu['single_thread']['value']:.4e} vu/s on 1 thread")
 
     out = {
         "metric": "VBD vertex-updates/s + fwd+adjoint steps/s vs plain, 1 GPU, \u22641e-5 rel err",

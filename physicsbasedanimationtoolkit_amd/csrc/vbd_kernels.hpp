@@ -295,7 +295,7 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     bool const active = v < a.vend;
     int kb = 0, ke = 0;
     float4 own = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 xl = own, xtv = own;
+    float4 xl = own;
     if (active)
     {
         if (a.ellStride > 0)
@@ -310,13 +310,8 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
             ke = a.vtp[v + 1];
         }
         own = a.xm[v];
-        if (lane == 0)
-        {
-            // issued now, used after the reduction
-            xl = a.xtil[v];
-            if (a.c.kD != 0.f)
-                xtv = a.xt[v];
-        }
+        if (lane == 0 && K < 3)
+            xl = a.xtil[v]; // issued now, used after the reduction
     }
     Block B{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool const nt = a.ntStream != 0;
@@ -435,7 +430,11 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     GroupSum<L>(B);
     if (!active || lane != 0)
         return;
-    float3 const dx = NewtonStep(B, own, xl, (a.c.kD != 0.f) ? xtv : own, a.c);
+    if constexpr (K >= 3)
+        xl = a.xtil[v];
+    // the start position only matters with damping: loaded here rather than
+    // held in registers through the incidence loop
+    float3 const dx = NewtonStep(B, own, xl, (a.c.kD != 0.f) ? a.xt[v] : own, a.c);
     a.xm[v] = make_float4(own.x - dx.x, own.y - dx.y, own.z - dx.z, own.w);
 }
 
