@@ -233,6 +233,8 @@ struct pbat_vbd_integrator
     pbat_vbd::DeviceArray<int4> tri;
     pbat_vbd::DeviceArray<float> triArea, xva, dupper;
     pbat_vbd::DeviceArray<float4> triLo, triHi, chunkLo, chunkHi, xfull, xb;
+    pbat_vbd::DeviceArray<float4> triA, triB, triC; // triangle positions of the nearest pass
+    pbat_vbd::DeviceArray<int4> crec;               // per-slot contact records of the sweep
     std::vector<int64_t> cvOrig; // original index of each collision vertex
     int pendBeg = 0, pendEnd = 0; // colour range whose new positions are still in xb
     float kD = 0.f;
@@ -278,6 +280,7 @@ struct pbat_vbd_integrator
         trPartials.reset(), cvIdx.reset(), cvBody.reset(), fc.reset(), cvActive.reset(), cvInAv.reset();
         tri.reset(), triArea.reset(), xva.reset(), dupper.reset();
         triLo.reset(), triHi.reset(), chunkLo.reset(), chunkHi.reset(), xfull.reset(), xb.reset();
+        triA.reset(), triB.reset(), triC.reset(), crec.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)
             (void)hipStreamDestroy(stream);

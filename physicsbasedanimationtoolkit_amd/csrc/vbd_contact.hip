@@ -45,10 +45,16 @@ struct ContactArgs
     float* __restrict__ dupper;        // per collision vertex: squared search radius
     float4* __restrict__ triLo;        // swept triangle boxes
     float4* __restrict__ triHi;
-    float4* __restrict__ chunkLo;      // box of each chunk of kChunk consecutive triangles
-    float4* __restrict__ chunkHi;
+    float4* __restrict__ chunkLo;      // box of each chunk of kChunk consecutive triangles; when all
+    float4* __restrict__ chunkHi;      // belong to one body, chunkHi.w != 0 and chunkLo.w = that body
+    // triangle vertex positions at the nearest pass's x (body of the triangle in triA.w)
+    float4* __restrict__ triA;
+    float4* __restrict__ triB;
+    float4* __restrict__ triC;
     int nCV, nF;
 };
+
+constexpr int kChunksPerRound = 4; // chunks whose triangles a nearest-pass round loads together
 
 // x1 = x0 + h v + h^2 a over all vertices (gpu/impl/vbd/Integrator.cu:163-188).
 [[maybe_unused]] static __global__ __launch_bounds__(256) void ContactFullStep(
@@ -83,15 +89,19 @@ struct ContactArgs
 }
 
 // Box of each chunk of kChunk consecutive triangles: of their swept boxes
-// (swept != 0) or of their vertices at x. One wave per chunk.
+// (swept != 0) or of their vertices at x, whose positions per triangle it
+// also stores for the nearest pass; and the body of the chunk's triangles when
+// they all belong to one. One wave per chunk.
 [[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactChunkBoxes(
     ContactArgs c, float4 const* __restrict__ x, int swept)
 {
     int const chunk = static_cast<int>(blockIdx.x) * (kContactBlock / kChunk) + static_cast<int>(threadIdx.x >> 6);
     int const f = chunk * kChunk + static_cast<int>(threadIdx.x & 63);
     float lx = 3.402823466e+38f, ly = lx, lz = lx, hx = -lx, hy = -lx, hz = -lx;
+    int blo = 0x7fffffff, bhi = static_cast<int>(0x80000000u);
     if (f < c.nF)
     {
+        blo = bhi = c.tri[f].w;
         if (swept)
         {
             float4 const L = c.triLo[f], U = c.triHi[f];
@@ -103,6 +113,9 @@ struct ContactArgs
             float4 const a = x[t.x], b = x[t.y], cc = x[t.z];
             lx = fminf(a.x, fminf(b.x, cc.x)), ly = fminf(a.y, fminf(b.y, cc.y)), lz = fminf(a.z, fminf(b.z, cc.z));
             hx = fmaxf(a.x, fmaxf(b.x, cc.x)), hy = fmaxf(a.y, fmaxf(b.y, cc.y)), hz = fmaxf(a.z, fmaxf(b.z, cc.z));
+            c.triA[f] = make_float4(a.x, a.y, a.z, __int_as_float(t.w));
+            c.triB[f] = b;
+            c.triC[f] = cc;
         }
     }
     for (int off = 32; off > 0; off >>= 1)
@@ -110,11 +123,12 @@ struct ContactArgs
         lx = fminf(lx, __shfl_xor(lx, off)), ly = fminf(ly, __shfl_xor(ly, off));
         lz = fminf(lz, __shfl_xor(lz, off)), hx = fmaxf(hx, __shfl_xor(hx, off));
         hy = fmaxf(hy, __shfl_xor(hy, off)), hz = fmaxf(hz, __shfl_xor(hz, off));
+        blo = min(blo, __shfl_xor(blo, off)), bhi = max(bhi, __shfl_xor(bhi, off));
     }
     if ((threadIdx.x & 63) == 0 && chunk * kChunk < c.nF)
     {
-        c.chunkLo[chunk] = make_float4(lx, ly, lz, 0.f);
-        c.chunkHi[chunk] = make_float4(hx, hy, hz, 0.f);
+        c.chunkLo[chunk] = make_float4(lx, ly, lz, __int_as_float(blo));
+        c.chunkHi[chunk] = make_float4(hx, hy, hz, blo == bhi ? 1.f : 0.f);
     }
 }
 
@@ -199,9 +213,12 @@ __device__ inline int WaveMaxInt(int x)
         if (cb + lane < nChunks)
         {
             float4 const TL = c.chunkLo[cb + lane], TU = c.chunkHi[cb + lane];
+            // a chunk of the vertex's own body holds no pair
+            bool const oneBody = TU.w != 0.f;
+            int const chunkBody = __float_as_int(TL.w);
 #pragma unroll
             for (int j = 0; j < kPerWave; ++j)
-                if (v0 + j < c.nCV && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
+                if (v0 + j < c.nCV && !(oneBody && chunkBody == body[j]) && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
                     TL.y <= hi[j].y && TL.z <= hi[j].z)
                     bits |= 1 << j;
         }
@@ -300,7 +317,9 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
 // vertex's contacts. finalize = 1 (FinalizeActiveSet): the vertex stays active
 // when it lies below the plane of the last of them. kPerWave collision vertices
 // per wave; two passes over the triangle chunks, the second collecting in
-// triangle order with wave ballots.
+// triangle order with wave ballots. A round loads the triangles of up to
+// kChunksPerRound chunks at once (positions stored by ContactChunkBoxes) and
+// then treats the chunks in order, so the collection order is unchanged.
 [[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactNearest(
     ContactArgs c, float4 const* __restrict__ x, int finalize)
 {
@@ -334,6 +353,22 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
     }
     if (!anyWork)
         return;
+    // seed the nearest distance with the vertex's first contact of its last list
+    // when that triangle lies within reach: it belongs to the searched set, so
+    // the minimum is unchanged, and the search starts from a tight bound
+#pragma unroll
+    for (int j = 0; j < kPerWave; ++j)
+    {
+        if (!mine[j])
+            continue;
+        int const f0 = c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts];
+        if (f0 >= 0)
+        {
+            float const d0 = PointTriangleSqDist(p[j], c.triA[f0], c.triB[f0], c.triC[f0]);
+            if (d0 <= r[j] + kFltEpsilon)
+                dmin[j] = d0;
+        }
+    }
     int const nChunks = (c.nF + kChunk - 1) / kChunk;
     for (int pass = 0; pass < 2; ++pass)
     {
@@ -354,62 +389,103 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
             }
         }
         // 64 chunk boxes at a time, one per lane; the wave visits, in order, the
-        // chunks whose box lies within some query's bound: r + eps in pass 0 (no
-        // triangle farther can matter), the collection threshold in pass 1
+        // chunks of other bodies whose box lies within some query's bound: r + eps
+        // in pass 0 (no triangle farther can matter), the collection threshold in
+        // pass 1. In pass 0 the bound shrinks to 4 d + 1e-6 with d the nearest
+        // squared distance found so far (before each group of boxes and after each
+        // round): a box beyond it holds no triangle nearer than d, by a margin far
+        // wider than the rounding of either distance, so the minimum is the one of
+        // the full search.
         for (int cb = 0; cb < nChunks; cb += 64)
         {
+            float bound[kPerWave];
+#pragma unroll
+            for (int j = 0; j < kPerWave; ++j)
+                bound[j] = pass == 0 ? fminf(r[j] + kFltEpsilon, 4.f * WaveMin(dmin[j]) + 1e-6f) : thr[j];
             int bits = 0;
+            float4 TL = make_float4(0.f, 0.f, 0.f, 0.f), TU = TL;
             if (cb + lane < nChunks)
             {
-                float4 const TL = c.chunkLo[cb + lane], TU = c.chunkHi[cb + lane];
+                TL = c.chunkLo[cb + lane], TU = c.chunkHi[cb + lane];
+                bool const oneBody = TU.w != 0.f;
+                int const chunkBody = __float_as_int(TL.w);
 #pragma unroll
                 for (int j = 0; j < kPerWave; ++j)
                 {
-                    float const bound = pass == 0 ? r[j] + kFltEpsilon : thr[j];
-                    if (mine[j] && (pass == 0 || thr[j] >= 0.f) && TileWithin(PointBoxSqDist(p[j], TL, TU), bound))
+                    if (mine[j] && !(oneBody && chunkBody == body[j]) && (pass == 0 || thr[j] >= 0.f) &&
+                        TileWithin(PointBoxSqDist(p[j], TL, TU), bound[j]))
                         bits |= 1 << j;
                 }
             }
             unsigned long long todo = __ballot(bits != 0);
             while (todo)
             {
-                int const k = __ffsll(static_cast<long long>(todo)) - 1;
-                todo &= todo - 1ull;
-                int const needBits = __shfl(bits, k);
-                bool need[kPerWave];
+                int ks[kChunksPerRound];
+                float4 A[kChunksPerRound], Bv[kChunksPerRound], Cv[kChunksPerRound];
 #pragma unroll
-                for (int j = 0; j < kPerWave; ++j)
-                    need[j] = ((needBits >> j) & 1) && (pass == 0 || count[j] < kMaxContacts);
-                int const f = (cb + k) * kChunk + lane;
-                bool const valid = f < c.nF;
-                float4 A = make_float4(0.f, 0.f, 0.f, 0.f), Bv = A, Cv = A;
-                int tb = 0;
-                if (valid)
+                for (int i = 0; i < kChunksPerRound; ++i)
                 {
-                    int4 const t = c.tri[f];
-                    A = x[t.x], Bv = x[t.y], Cv = x[t.z];
-                    tb = t.w;
+                    ks[i] = -1;
+                    A[i] = Bv[i] = Cv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (todo)
+                    {
+                        ks[i] = __ffsll(static_cast<long long>(todo)) - 1;
+                        todo &= todo - 1ull;
+                        int const f = (cb + ks[i]) * kChunk + lane;
+                        if (f < c.nF)
+                            A[i] = c.triA[f], Bv[i] = c.triB[f], Cv[i] = c.triC[f];
+                    }
                 }
 #pragma unroll
-                for (int j = 0; j < kPerWave; ++j)
+                for (int i = 0; i < kChunksPerRound; ++i)
                 {
-                    if (!need[j])
-                        continue;
-                    bool const other = valid && tb != body[j];
-                    if (pass == 0)
+                    if (ks[i] < 0)
+                        break;
+                    int const needBits = __shfl(bits, ks[i]);
+                    bool need[kPerWave];
+#pragma unroll
+                    for (int j = 0; j < kPerWave; ++j)
+                        need[j] = ((needBits >> j) & 1) && (pass == 0 || count[j] < kMaxContacts);
+                    int const f = (cb + ks[i]) * kChunk + lane;
+                    bool const valid = f < c.nF;
+                    int const tb = __float_as_int(A[i].w);
+#pragma unroll
+                    for (int j = 0; j < kPerWave; ++j)
                     {
-                        if (other)
-                            dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], A, Bv, Cv));
-                        continue;
+                        if (!need[j])
+                            continue;
+                        bool const other = valid && tb != body[j];
+                        if (pass == 0)
+                        {
+                            if (other)
+                                dmin[j] = fminf(dmin[j], PointTriangleSqDist(p[j], A[i], Bv[i], Cv[i]));
+                            continue;
+                        }
+                        bool const take = other && PointTriangleSqDist(p[j], A[i], Bv[i], Cv[i]) <= thr[j];
+                        unsigned long long const mask = __ballot(take);
+                        int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
+                        bool const kept = take && slot < kMaxContacts;
+                        if (kept && !finalize)
+                            c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = f;
+                        last[j] = max(last[j], WaveMaxInt(kept ? f : -1));
+                        count[j] += __popcll(mask);
                     }
-                    bool const take = other && PointTriangleSqDist(p[j], A, Bv, Cv) <= thr[j];
-                    unsigned long long const mask = __ballot(take);
-                    int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
-                    bool const kept = take && slot < kMaxContacts;
-                    if (kept && !finalize)
-                        c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = f;
-                    last[j] = max(last[j], WaveMaxInt(kept ? f : -1));
-                    count[j] += __popcll(mask);
+                }
+                if (pass == 0 && todo)
+                {
+#pragma unroll
+                    for (int j = 0; j < kPerWave; ++j)
+                        bound[j] = fminf(r[j] + kFltEpsilon, 4.f * WaveMin(dmin[j]) + 1e-6f);
+                    int left = 0;
+                    if ((todo >> lane) & 1ull)
+                    {
+#pragma unroll
+                        for (int j = 0; j < kPerWave; ++j)
+                            if (((bits >> j) & 1) && TileWithin(PointBoxSqDist(p[j], TL, TU), bound[j]))
+                                left |= 1 << j;
+                    }
+                    bits = left;
+                    todo = __ballot(left != 0);
                 }
             }
         }
@@ -434,6 +510,40 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
             float3 const nrm = Cross(Sub(b, a), Sub(cc, a));
             c.active[v] = Dot(Sub(p[j], a), nrm) < 0.f ? 1 : 0;
         }
+    }
+}
+
+// The sweep's per-slot contact records from the contact lists (kept, stale or
+// not, for every collision vertex): the triangle's vertices and its penalty
+// muC XVA_v / sum_k FA_k * FA_f (ContactPenalty, gpu/impl/vbd/Kernels.cuh:80-114;
+// the sum over the vertex's contacting triangles in slot order), or x = -1 for
+// an empty slot. One thread per collision vertex.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void ContactPack(ContactArgs c, float muC,
+                                                                           int4* __restrict__ rec)
+{
+    int const k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= c.nCV)
+        return;
+    size_t const v = static_cast<size_t>(c.cvIdx[k]);
+    int4 const fa = *reinterpret_cast<int4 const*>(c.fc + v * kMaxContacts);
+    int4 const fb = *reinterpret_cast<int4 const*>(c.fc + v * kMaxContacts + 4);
+    int const f[kMaxContacts] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
+    float sumfa = 0.f;
+#pragma unroll
+    for (int j = 0; j < kMaxContacts; ++j)
+        if (f[j] >= 0)
+            sumfa += c.triArea[f[j]];
+    float const kC = c.xva[v] * muC / sumfa;
+#pragma unroll
+    for (int j = 0; j < kMaxContacts; ++j)
+    {
+        int4 r = make_int4(-1, -1, -1, 0);
+        if (f[j] >= 0)
+        {
+            int4 const t = c.tri[f[j]];
+            r = make_int4(t.x, t.y, t.z, __float_as_int(kC * c.triArea[f[j]]));
+        }
+        rec[v * kMaxContacts + j] = r;
     }
 }
 
@@ -522,6 +632,8 @@ void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d)
     // no contacts, nothing active, search radii at FLT_MAX
     // (gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:22-48, gpu/impl/vbd/Integrator.cu:60)
     h.fc.upload(std::vector<int>(static_cast<size_t>(nV) * device::kMaxContacts, -1), h.stream);
+    h.crec.upload(std::vector<int4>(static_cast<size_t>(nV) * device::kMaxContacts, make_int4(-1, -1, -1, 0)),
+                  h.stream);
     h.cvActive.upload(std::vector<int>(static_cast<size_t>(nCV), 0), h.stream);
     h.cvInAv.upload(std::vector<int>(static_cast<size_t>(nCV), 0), h.stream);
     h.dupper.upload(std::vector<float>(static_cast<size_t>(nCV), std::numeric_limits<float>::max()), h.stream);
@@ -530,6 +642,9 @@ void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d)
     size_t const nChunks = static_cast<size_t>((nF + device::kChunk - 1) / device::kChunk);
     h.chunkLo.allocate(nChunks);
     h.chunkHi.allocate(nChunks);
+    h.triA.allocate(static_cast<size_t>(nF));
+    h.triB.allocate(static_cast<size_t>(nF));
+    h.triC.allocate(static_cast<size_t>(nF));
     h.xfull.allocate(static_cast<size_t>(nV));
     h.xb.allocate(static_cast<size_t>(nV));
 }
@@ -541,6 +656,7 @@ device::ContactArgs MakeContactArgs(pbat_vbd_integrator& h)
     c.xva = h.xva.ptr, c.fc = h.fc.ptr, c.active = h.cvActive.ptr, c.inAv = h.cvInAv.ptr;
     c.dupper = h.dupper.ptr, c.triLo = h.triLo.ptr, c.triHi = h.triHi.ptr;
     c.chunkLo = h.chunkLo.ptr, c.chunkHi = h.chunkHi.ptr;
+    c.triA = h.triA.ptr, c.triB = h.triB.ptr, c.triC = h.triC.ptr;
     c.nCV = h.nCV, c.nF = h.nF;
     return c;
 }
@@ -564,9 +680,9 @@ void ContactBeginStep(pbat_vbd_integrator& h, float dt)
                        static_cast<float4 const*>(h.xfull.ptr));
 }
 
-// UpdateActiveSet (finalize = 0, gpu/impl/vbd/Integrator.cu:250-271) or
-// FinalizeActiveSet (finalize = 1, gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:185-215)
-// at the current positions.
+// UpdateActiveSet (finalize = 0, gpu/impl/vbd/Integrator.cu:250-271; then the
+// sweep's contact records) or FinalizeActiveSet (finalize = 1,
+// gpu/impl/contact/VertexTriangleMixedCcdDcd.cu:185-215) at the current positions.
 void ContactNearestPass(pbat_vbd_integrator& h, int finalize)
 {
     device::ContactArgs const c = MakeContactArgs(h);
@@ -575,6 +691,9 @@ void ContactNearestPass(pbat_vbd_integrator& h, int finalize)
                        static_cast<float4 const*>(h.xm.ptr), 0);
     hipLaunchKernelGGL(device::ContactNearest, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
                        dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xm.ptr), finalize);
+    if (!finalize)
+        hipLaunchKernelGGL(device::ContactPack, dim3((h.nCV + 255) / 256), dim3(256), 0, h.stream, c, h.muC,
+                           h.crec.ptr);
 }
 
 } // namespace pbat_vbd

@@ -28,12 +28,11 @@ __device__ inline float3 Cross(float3 a, float3 b)
 // pass (the contact sweep writes to a separate buffer).
 struct ContactSweep
 {
-    int const* __restrict__ fc;
-    int4 const* __restrict__ tri;
-    float const* __restrict__ triArea;
-    float const* __restrict__ xva;
+    // kMaxContacts records per vertex, one per contact slot: the triangle's
+    // vertices and, as bits in .w, its scaled penalty; x = -1 for an empty slot
+    int4 const* __restrict__ rec;
     float4* __restrict__ xb; // new positions of the colour
-    float muC, muF, epsv;
+    float muF, epsv;
     // The previous colour's range, whose current positions are still in xb: the
     // sweep reads them there and copies them back to xm (nobody reads xm there).
     int pendBeg, pendEnd;

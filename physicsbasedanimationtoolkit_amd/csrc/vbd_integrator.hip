@@ -97,8 +97,7 @@ void LaunchSweepLK(pbat_vbd_integrator& h, device::SweepArgs a, int nVerts)
     // launch can copy: copy it back first
     if ((h.pendBeg == vbeg && h.pendEnd > h.pendBeg) || static_cast<int64_t>(h.pendEnd - h.pendBeg) > grid64 * block)
         FlushContactSweep(h);
-    a.contact.fc = h.fc.ptr, a.contact.tri = h.tri.ptr, a.contact.triArea = h.triArea.ptr, a.contact.xva = h.xva.ptr;
-    a.contact.xb = h.xb.ptr, a.contact.muC = h.muC, a.contact.muF = h.muF, a.contact.epsv = h.epsv;
+    a.contact.rec = h.crec.ptr, a.contact.xb = h.xb.ptr, a.contact.muF = h.muF, a.contact.epsv = h.epsv;
     a.contact.pendBeg = h.pendBeg, a.contact.pendEnd = h.pendEnd;
     hipLaunchKernelGGL((device::VbdSweep<L, K, true>), dim3(static_cast<int>(grid64)), dim3(block), 0, h.stream, a);
     h.pendBeg = vbeg, h.pendEnd = vbeg + nVerts;

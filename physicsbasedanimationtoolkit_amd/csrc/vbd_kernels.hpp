@@ -315,18 +315,23 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     }
     Block B{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     bool const nt = a.ntStream != 0;
-    // contact sweeps: the vertex's contact list and start position, issued now;
-    // then copy the previous colour's new positions back
-    int4 fa = make_int4(-1, -1, -1, -1), fb = fa;
+    // contact sweeps: the start position and the lane's contact records (slots
+    // lane, lane + L, ...), issued now; then copy the previous colour's new
+    // positions back
+    constexpr int kSlots = C ? (kMaxContacts + L - 1) / L : 1;
+    int4 crec[kSlots];
     float4 xto = own;
     if constexpr (C)
     {
-        if (active)
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j)
         {
-            xto = a.xt[v];
-            fa = *reinterpret_cast<int4 const*>(a.contact.fc + static_cast<size_t>(v) * kMaxContacts);
-            fb = *reinterpret_cast<int4 const*>(a.contact.fc + static_cast<size_t>(v) * kMaxContacts + 4);
+            int const k = lane + j * L;
+            crec[j] = (active && k < kMaxContacts) ? a.contact.rec[static_cast<size_t>(v) * kMaxContacts + k]
+                                                   : make_int4(-1, -1, -1, 0);
         }
+        if (active)
+            xto = a.xt[v];
         int const pend = a.contact.pendBeg + tid;
         if (pend < a.contact.pendEnd)
             a.xm[pend] = a.contact.xb[pend];
@@ -389,37 +394,22 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     if constexpr (C)
     {
         // contact terms (after damping, before inertia): the group's lanes take
-        // the vertex's contacts k = lane, lane + L, ... and sum them apart from
-        // the elastic block, which is damped alone
+        // the vertex's contact slots k = lane, lane + L, ... and sum them apart
+        // from the elastic block, which is damped alone
         ContactSweep const& cs = a.contact;
         Block Cb{0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (active)
-        {
-            if (fa.x >= 0)
-            {
-                int const f[kMaxContacts] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w};
-                int n = 0;
-                float sumfa = 0.f;
+        float3 const xo = make_float3(own.x, own.y, own.z), xt3 = make_float3(xto.x, xto.y, xto.z);
 #pragma unroll
-                for (int k = 0; k < kMaxContacts; ++k)
-                    if (f[k] >= 0)
-                    {
-                        ++n;
-                        sumfa += cs.triArea[f[k]];
-                    }
-                float const kC = cs.xva[v] * cs.muC / sumfa;
-                float3 const xo = make_float3(own.x, own.y, own.z), xt3 = make_float3(xto.x, xto.y, xto.z);
-#pragma unroll
-                for (int k = 0; k < kMaxContacts; ++k)
-                    if (k < n && (k & (L - 1)) == lane)
-                        AccumulateContact(xt3, xo, cs, a.xm, a.xt, cs.tri[f[k]], a.c.dt, kC * cs.triArea[f[k]],
-                                          cs.muF, cs.epsv, Cb);
-            }
-        }
+        for (int j = 0; j < kSlots; ++j)
+            if (crec[j].x >= 0)
+                AccumulateContact(xt3, xo, cs, a.xm, a.xt, crec[j], a.c.dt, __int_as_float(crec[j].w), cs.muF,
+                                  cs.epsv, Cb);
         GroupSum<L>(B);
         GroupSum<L>(Cb);
         if (!active || lane != 0)
             return;
+        if constexpr (K >= 3)
+            xl = a.xtil[v];
         Damp(B, own, xto, a.c);
         B.h00 += Cb.h00, B.h01 += Cb.h01, B.h02 += Cb.h02, B.h11 += Cb.h11, B.h12 += Cb.h12, B.h22 += Cb.h22;
         B.g0 += Cb.g0, B.g1 += Cb.g1, B.g2 += Cb.g2;

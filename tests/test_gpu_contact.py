@@ -74,6 +74,31 @@ def test_cube_on_slab_matches_oracle(gpu_device):
     assert touched > 0
 
 
+@pytest.mark.parametrize("lanes,per_lane", [(1, 4), (2, 4), (4, 3), (8, 3), (16, 2), (32, 1), (64, 1)])
+def test_contact_sweep_shapes_match_oracle(gpu_device, lanes, per_lane):
+    """Each sweep width spreads a vertex's 8 contact slots over its lanes differently
+    (all 8 in one lane at 1 lane per vertex, one slot per lane from 8 lanes up), and each
+    prefetch depth loads the inertial target at its own point: every shape finds the
+    oracle's contacts and lands within CONTACT_POS_TOL of its positions."""
+    X, E, F, V, B, v, dbc = cube_on_slab()
+    gpu, ref = contact_pair(X, E, F, V, B, v, dbc)
+    gpu.set_sweep_config(lanes, per_lane)
+    cfg = gpu.sweep_config()
+    assert (cfg["lanes_per_vertex"], cfg["incidences_per_lane"]) == (lanes, per_lane)
+    touched = 0
+    for step in range(12):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 20, 1)
+        ref.step_contact(DT, 20, 1)
+        active, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        np.testing.assert_array_equal(active, ref.cactive)
+        touched += int((fc >= 0).any(axis=1).sum())
+        assert rel(gpu.x, ref.positions) <= CONTACT_POS_TOL
+    assert touched > 0
+
+
 def test_contact_graph_replay_is_bitwise(gpu_device):
     X, E, F, V, B, v, dbc = cube_on_slab()
     g1, _ = contact_pair(X, E, F, V, B, v, dbc)
