@@ -127,6 +127,19 @@ def point_triangle_sqdist(p, a, b, c) -> float:
     return L.ora_point_triangle_sqdist(*[_dp(q) for q in args])
 
 
+def point_triangle_sqdists(p, tris) -> np.ndarray:
+    """point_triangle_sqdist from p to each triangle of tris (n x 3 vertices x 3)."""
+    L = lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.ora_point_triangle_sqdist_many.restype = None
+    L.ora_point_triangle_sqdist_many.argtypes = [dp, dp, ctypes.c_int64, dp]
+    q = np.ascontiguousarray(np.asarray(p, np.float64))
+    t = np.ascontiguousarray(np.asarray(tris, np.float64).reshape(-1, 9))
+    out = np.zeros(t.shape[0])
+    L.ora_point_triangle_sqdist_many(_dp(q), _dp(t), t.shape[0], _dp(out))
+    return out
+
+
 def vertex_triangle_contact(xtv, xv, xtf, xf, dt, muC, muF, epsv):
     """(E, g, H) of AccumulateVertexTriangleContact (sim/vbd/Kernels.h:215-302); xtf, xf
     are 3x3 with one triangle vertex per column."""
@@ -357,7 +370,7 @@ class OracleVbd:
         fs = np.nonzero(self.B[self.cF[:, 0]] != self.B[i])[0]
         if fs.size == 0:
             return []
-        d = np.array([point_triangle_sqdist(x[i], *x[self.cF[f]]) for f in fs])
+        d = point_triangle_sqdists(x[i], x[self.cF[fs]])
         eps = float(np.finfo(np.float32).eps)
         ds, r = d.min(), self.dupper[v]
         if ds < r - eps:

@@ -691,6 +691,12 @@ double ora_point_triangle_sqdist(const double p[3], const double a[3], const dou
     return dot3(q, q);
 }
 
+void ora_point_triangle_sqdist_many(const double p[3], const double *tri, int64_t n, double *out)
+{
+    for (int64_t k = 0; k < n; ++k)
+        out[k] = ora_point_triangle_sqdist(p, tri + 9 * k, tri + 9 * k + 3, tri + 9 * k + 6);
+}
+
 /* ContactPenalty + AccumulateVertexTriangleContact over vertex i's contacts
  * (gpu/impl/vbd/Kernels.cuh:80-114, 204-223): the penalty muC is scaled by the
  * vertex area over the summed areas of its contacting triangles, times each

@@ -142,6 +142,10 @@ double ora_vertex_triangle_contact(const double xtv[3], const double xv[3], cons
  * (Ericson, Real-Time Collision Detection, 5.1.5), as geometry/DistanceQueries.h:217-222. */
 double ora_point_triangle_sqdist(const double p[3], const double a[3], const double b[3], const double c[3]);
 
+/* ora_point_triangle_sqdist from p to each of n triangles, tri holding (a, b, c) of
+ * triangle k at tri[9k .. 9k+8]. */
+void ora_point_triangle_sqdist_many(const double p[3], const double *tri, int64_t n, double *out);
+
 /* sim/vbd/Kernels.h:97-102, including its operator precedence: for k >= 2 the
  * result is 4/4 - rho2*omega = 1 - rho2*omega. */
 double ora_chebyshev_omega(int64_t k, double rho2, double omega);

@@ -232,7 +232,7 @@ struct pbat_vbd_integrator
     pbat_vbd::DeviceArray<int> cvIdx, cvBody, fc, cvActive, cvInAv;
     pbat_vbd::DeviceArray<int4> tri;
     pbat_vbd::DeviceArray<float> triArea, xva, dupper;
-    pbat_vbd::DeviceArray<float4> triLo, triHi, chunkLo, chunkHi, xfull, xb;
+    pbat_vbd::DeviceArray<float4> triLo, triHi, chunkLo, chunkHi, superLo, superHi, xfull, xb;
     pbat_vbd::DeviceArray<float4> triA, triB, triC; // triangle positions of the nearest pass
     pbat_vbd::DeviceArray<int4> crec;               // per-slot contact records of the sweep
     std::vector<int64_t> cvOrig; // original index of each collision vertex
@@ -279,7 +279,8 @@ struct pbat_vbd_integrator
         gkm1.reset(), fkm1.reset(), dgk.reset(), dfk.reset(), andersonAcc.reset(), andersonGram.reset();
         trPartials.reset(), cvIdx.reset(), cvBody.reset(), fc.reset(), cvActive.reset(), cvInAv.reset();
         tri.reset(), triArea.reset(), xva.reset(), dupper.reset();
-        triLo.reset(), triHi.reset(), chunkLo.reset(), chunkHi.reset(), xfull.reset(), xb.reset();
+        triLo.reset(), triHi.reset(), chunkLo.reset(), chunkHi.reset(), superLo.reset(), superHi.reset();
+        xfull.reset(), xb.reset();
         triA.reset(), triB.reset(), triC.reset(), crec.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         if (stream)

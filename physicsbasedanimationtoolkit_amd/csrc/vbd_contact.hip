@@ -47,6 +47,8 @@ struct ContactArgs
     float4* __restrict__ triHi;
     float4* __restrict__ chunkLo;      // box of each chunk of kChunk consecutive triangles; when all
     float4* __restrict__ chunkHi;      // belong to one body, chunkHi.w != 0 and chunkLo.w = that body
+    float4* __restrict__ superLo;      // the same for each super-chunk of 64 consecutive chunks
+    float4* __restrict__ superHi;
     // triangle vertex positions at the nearest pass's x (body of the triangle in triA.w)
     float4* __restrict__ triA;
     float4* __restrict__ triB;
@@ -132,6 +134,38 @@ constexpr int kChunksPerRound = 4; // chunks whose triangles a nearest-pass roun
     }
 }
 
+// Box of each super-chunk of 64 consecutive chunk boxes, and its body when all
+// its triangles belong to one. One wave per super-chunk.
+[[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactSuperBoxes(ContactArgs c)
+{
+    int const nChunks = (c.nF + kChunk - 1) / kChunk, nSuper = (nChunks + 63) / 64;
+    int const sc = static_cast<int>(blockIdx.x) * (kContactBlock / 64) + static_cast<int>(threadIdx.x >> 6);
+    int const k = sc * 64 + static_cast<int>(threadIdx.x & 63);
+    float lx = 3.402823466e+38f, ly = lx, lz = lx, hx = -lx, hy = -lx, hz = -lx;
+    int blo = 0x7fffffff, bhi = static_cast<int>(0x80000000u);
+    bool mixed = false;
+    if (k < nChunks)
+    {
+        float4 const L = c.chunkLo[k], U = c.chunkHi[k];
+        lx = L.x, ly = L.y, lz = L.z, hx = U.x, hy = U.y, hz = U.z;
+        blo = bhi = __float_as_int(L.w);
+        mixed = U.w == 0.f;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        lx = fminf(lx, __shfl_xor(lx, off)), ly = fminf(ly, __shfl_xor(ly, off));
+        lz = fminf(lz, __shfl_xor(lz, off)), hx = fmaxf(hx, __shfl_xor(hx, off));
+        hy = fmaxf(hy, __shfl_xor(hy, off)), hz = fmaxf(hz, __shfl_xor(hz, off));
+        blo = min(blo, __shfl_xor(blo, off)), bhi = max(bhi, __shfl_xor(bhi, off));
+    }
+    bool const anyMixed = __ballot(mixed) != 0ull;
+    if ((threadIdx.x & 63) == 0 && sc < nSuper)
+    {
+        c.superLo[sc] = make_float4(lx, ly, lz, __int_as_float(blo));
+        c.superHi[sc] = make_float4(hx, hy, hz, (!anyMixed && blo == bhi) ? 1.f : 0.f);
+    }
+}
+
 // Squared distance from p to the box [L, U] (0 inside).
 __device__ inline float PointBoxSqDist(float4 p, float4 L, float4 U)
 {
@@ -204,11 +238,35 @@ __device__ inline int WaveMaxInt(int x)
             body[j] = c.cvBody[v];
         }
     }
-    int const nChunks = (c.nF + kChunk - 1) / kChunk;
-    // 64 chunk boxes at a time, one per lane: the lanes test them against the
-    // wave's segments, and the wave visits the chunks that any segment overlaps
+    int const nChunks = (c.nF + kChunk - 1) / kChunk, nSuper = (nChunks + 63) / 64;
+    // 64 chunk boxes at a time (one super-chunk), one per lane: the lanes test
+    // them against the wave's segments, and the wave visits the chunks that any
+    // segment overlaps. Super-chunk boxes, tested 64 at a time, skip the groups
+    // that no segment meets (a chunk box lies inside its super-chunk's box).
+    unsigned long long sTodo = 0ull;
+    int sbits = 0;
     for (int cb = 0; cb < nChunks; cb += 64)
     {
+        int const sIdx = cb >> 6;
+        if ((sIdx & 63) == 0)
+        {
+            sbits = 0;
+            if (sIdx + lane < nSuper)
+            {
+                float4 const SL = c.superLo[sIdx + lane], SU = c.superHi[sIdx + lane];
+                bool const oneBody = SU.w != 0.f;
+                int const superBody = __float_as_int(SL.w);
+#pragma unroll
+                for (int j = 0; j < kPerWave; ++j)
+                    if (v0 + j < c.nCV && !(oneBody && superBody == body[j]) && lo[j].x <= SU.x &&
+                        lo[j].y <= SU.y && lo[j].z <= SU.z && SL.x <= hi[j].x && SL.y <= hi[j].y && SL.z <= hi[j].z)
+                        sbits |= 1 << j;
+            }
+            sTodo = __ballot(sbits != 0);
+        }
+        if (!((sTodo >> (sIdx & 63)) & 1ull))
+            continue;
+        int const qmask = __shfl(sbits, sIdx & 63);
         int bits = 0;
         if (cb + lane < nChunks)
         {
@@ -218,7 +276,7 @@ __device__ inline int WaveMaxInt(int x)
             int const chunkBody = __float_as_int(TL.w);
 #pragma unroll
             for (int j = 0; j < kPerWave; ++j)
-                if (v0 + j < c.nCV && !(oneBody && chunkBody == body[j]) && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
+                if (((qmask >> j) & 1) && !(oneBody && chunkBody == body[j]) && lo[j].x <= TU.x && lo[j].y <= TU.y && lo[j].z <= TU.z && TL.x <= hi[j].x &&
                     TL.y <= hi[j].y && TL.z <= hi[j].z)
                     bits |= 1 << j;
         }
@@ -369,7 +427,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
                 dmin[j] = d0;
         }
     }
-    int const nChunks = (c.nF + kChunk - 1) / kChunk;
+    int const nChunks = (c.nF + kChunk - 1) / kChunk, nSuper = (nChunks + 63) / 64;
     for (int pass = 0; pass < 2; ++pass)
     {
         if (pass == 1)
@@ -395,9 +453,38 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
         // squared distance found so far (before each group of boxes and after each
         // round): a box beyond it holds no triangle nearer than d, by a margin far
         // wider than the rounding of either distance, so the minimum is the one of
-        // the full search.
+        // the full search. Super-chunk boxes, tested 64 at a time against the bound
+        // (widened by 1e-4 against rounding), skip the groups of 64 chunks that no
+        // query can reach.
+        unsigned long long sTodo = 0ull;
+        int sbits = 0;
         for (int cb = 0; cb < nChunks; cb += 64)
         {
+            int const sIdx = cb >> 6;
+            if ((sIdx & 63) == 0)
+            {
+                float sBound[kPerWave];
+#pragma unroll
+                for (int j = 0; j < kPerWave; ++j)
+                    sBound[j] = (pass == 0 ? fminf(r[j] + kFltEpsilon, 4.f * WaveMin(dmin[j]) + 1e-6f) : thr[j]) *
+                                (1.f + 1e-4f);
+                sbits = 0;
+                if (sIdx + lane < nSuper)
+                {
+                    float4 const SL = c.superLo[sIdx + lane], SU = c.superHi[sIdx + lane];
+                    bool const oneBody = SU.w != 0.f;
+                    int const superBody = __float_as_int(SL.w);
+#pragma unroll
+                    for (int j = 0; j < kPerWave; ++j)
+                        if (mine[j] && !(oneBody && superBody == body[j]) && (pass == 0 || thr[j] >= 0.f) &&
+                            TileWithin(PointBoxSqDist(p[j], SL, SU), sBound[j]))
+                            sbits |= 1 << j;
+                }
+                sTodo = __ballot(sbits != 0);
+            }
+            if (!((sTodo >> (sIdx & 63)) & 1ull))
+                continue;
+            int const qmask = __shfl(sbits, sIdx & 63);
             float bound[kPerWave];
 #pragma unroll
             for (int j = 0; j < kPerWave; ++j)
@@ -412,7 +499,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
 #pragma unroll
                 for (int j = 0; j < kPerWave; ++j)
                 {
-                    if (mine[j] && !(oneBody && chunkBody == body[j]) && (pass == 0 || thr[j] >= 0.f) &&
+                    if (((qmask >> j) & 1) && !(oneBody && chunkBody == body[j]) && (pass == 0 || thr[j] >= 0.f) &&
                         TileWithin(PointBoxSqDist(p[j], TL, TU), bound[j]))
                         bits |= 1 << j;
                 }
@@ -642,11 +729,20 @@ void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d)
     size_t const nChunks = static_cast<size_t>((nF + device::kChunk - 1) / device::kChunk);
     h.chunkLo.allocate(nChunks);
     h.chunkHi.allocate(nChunks);
+    h.superLo.allocate((nChunks + 63) / 64);
+    h.superHi.allocate((nChunks + 63) / 64);
     h.triA.allocate(static_cast<size_t>(nF));
     h.triB.allocate(static_cast<size_t>(nF));
     h.triC.allocate(static_cast<size_t>(nF));
     h.xfull.allocate(static_cast<size_t>(nV));
     h.xb.allocate(static_cast<size_t>(nV));
+}
+
+// Workgroups of ContactSuperBoxes: one wave per super-chunk of 64 chunks.
+int SuperBlocks(pbat_vbd_integrator const& h)
+{
+    int const nChunks = (h.nF + device::kChunk - 1) / device::kChunk, nSuper = (nChunks + 63) / 64;
+    return (nSuper + device::kContactBlock / 64 - 1) / (device::kContactBlock / 64);
 }
 
 device::ContactArgs MakeContactArgs(pbat_vbd_integrator& h)
@@ -655,7 +751,7 @@ device::ContactArgs MakeContactArgs(pbat_vbd_integrator& h)
     c.cvIdx = h.cvIdx.ptr, c.cvBody = h.cvBody.ptr, c.tri = h.tri.ptr, c.triArea = h.triArea.ptr;
     c.xva = h.xva.ptr, c.fc = h.fc.ptr, c.active = h.cvActive.ptr, c.inAv = h.cvInAv.ptr;
     c.dupper = h.dupper.ptr, c.triLo = h.triLo.ptr, c.triHi = h.triHi.ptr;
-    c.chunkLo = h.chunkLo.ptr, c.chunkHi = h.chunkHi.ptr;
+    c.chunkLo = h.chunkLo.ptr, c.chunkHi = h.chunkHi.ptr, c.superLo = h.superLo.ptr, c.superHi = h.superHi.ptr;
     c.triA = h.triA.ptr, c.triB = h.triB.ptr, c.triC = h.triC.ptr;
     c.nCV = h.nCV, c.nF = h.nF;
     return c;
@@ -675,6 +771,7 @@ void ContactBeginStep(pbat_vbd_integrator& h, float dt)
                        static_cast<float4 const*>(h.xt.ptr), static_cast<float4 const*>(h.xfull.ptr));
     hipLaunchKernelGGL(device::ContactChunkBoxes, dim3(boxBlocks), dim3(device::kContactBlock), 0, h.stream, c,
                        static_cast<float4 const*>(nullptr), 1);
+    hipLaunchKernelGGL(device::ContactSuperBoxes, dim3(SuperBlocks(h)), dim3(device::kContactBlock), 0, h.stream, c);
     hipLaunchKernelGGL(device::ContactBroadphase, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
                        dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xt.ptr),
                        static_cast<float4 const*>(h.xfull.ptr));
@@ -689,6 +786,7 @@ void ContactNearestPass(pbat_vbd_integrator& h, int finalize)
     int const boxBlocks = (h.nF + device::kContactBlock - 1) / device::kContactBlock;
     hipLaunchKernelGGL(device::ContactChunkBoxes, dim3(boxBlocks), dim3(device::kContactBlock), 0, h.stream, c,
                        static_cast<float4 const*>(h.xm.ptr), 0);
+    hipLaunchKernelGGL(device::ContactSuperBoxes, dim3(SuperBlocks(h)), dim3(device::kContactBlock), 0, h.stream, c);
     hipLaunchKernelGGL(device::ContactNearest, dim3((h.nCV + device::kGroup - 1) / device::kGroup),
                        dim3(device::kContactBlock), 0, h.stream, c, static_cast<float4 const*>(h.xm.ptr), finalize);
     if (!finalize)

@@ -169,6 +169,53 @@ def test_contact_unsupported_combinations_raise(gpu_device):
     assert nT > 0
 
 
+def stacked_cubes_on_slab(n=8):
+    """Two layers of 2 x 2 jittered n^3 cubes (bodies 1..8) falling at 0.5 m/s onto a
+    fixed slab (body 0), 0.005 apart: with n = 8, 6,464 collision triangles, so the
+    detection's boxes span two groups of 64 chunks of 64 triangles."""
+    Xs, Es = meshes.cube_tet_grid(8, 1, 8)
+    Xs = Xs * np.array([[1.2], [0.1], [1.2]]) - np.array([[0.1], [0.1], [0.1]])
+    parts, elems, bodies, off = [Xs], [Es], [np.zeros(Xs.shape[1], np.int64)], Xs.shape[1]
+    for b, (i, j, k) in enumerate(np.ndindex(2, 2, 2)):
+        Xc, Ec = meshes.cube_tet_grid(n, jitter=0.1, seed=b, size=0.45)
+        Xc = Xc + np.array([[0.5 * i + 0.013 * b], [0.004 + 0.455 * j], [0.5 * k + 0.007 * b]])
+        parts.append(Xc)
+        elems.append(Ec + off)
+        bodies.append(np.full(Xc.shape[1], b + 1, np.int64))
+        off += Xc.shape[1]
+    X, E, B = np.concatenate(parts, axis=1), np.concatenate(elems, axis=1), np.concatenate(bodies)
+    F = meshes.boundary_triangles(E)
+    v = np.zeros(X.shape)
+    v[1, B > 0] = -0.5
+    return X, E, F, np.unique(F), B, v, np.arange(Xs.shape[1])
+
+
+def test_stacked_cubes_match_oracle(gpu_device):
+    """Nine bodies and more than 64 chunks of triangles: the culled detection (chunk and
+    super-chunk boxes, own-body chunks skipped, the nearest search started from the last
+    first contact) finds the oracle's exhaustive contacts and active flags each step,
+    bottom cubes on the slab and top cubes on the bottom ones, and the positions agree."""
+    X, E, F, V, B, v, dbc = stacked_cubes_on_slab()
+    assert F.shape[1] > 64 * 64
+    gpu, ref = contact_pair(X, E, F, V, B, v, dbc)
+    pairs = set()
+    for step in range(5):
+        ref.x[:] = gpu.x.T
+        ref.v[:] = gpu.v.T
+        gpu.step(DT, 10, 1)
+        ref.step_contact(DT, 10, 1)
+        active, fc = gpu.contacts()
+        np.testing.assert_array_equal(fc, ref.fc)
+        np.testing.assert_array_equal(active, ref.cactive)
+        for i in np.nonzero((fc >= 0).any(axis=1))[0]:
+            pairs.add((int(B[i]), int(B[F[0, fc[i, 0]]])))
+        e = rel(gpu.x, ref.positions)
+        print(f"step {step}: active {int(ref.in_av.sum())}  contacting {int((fc >= 0).any(axis=1).sum())}  "
+              f"rel pos err {e:.3e}")
+        assert e <= CONTACT_POS_TOL
+    assert any(b1 > 0 and b2 == 0 for b1, b2 in pairs) and any(b1 > 0 and b2 > 0 for b1, b2 in pairs)
+
+
 def test_colliding_cubes_match_oracle(gpu_device):
     """Two free cubes colliding (contacts both ways, vertices of one colour coupled
     across bodies): per step from the device's state, the same active sets and
