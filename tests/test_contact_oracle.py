@@ -33,6 +33,12 @@ def test_point_triangle_distance_matches_exhaustive_search():
         p = rng.normal(size=3) * 2
         d = O.point_triangle_sqdist(p, a, b, c)
         assert abs(d - _closest_exhaustive(p, a, b, c)) <= 1e-12 * max(1.0, d)
+    # the batched form (one call per vertex in the nearest-triangle search) is the same
+    # function applied triangle by triangle
+    p = rng.normal(size=3)
+    tris = rng.normal(size=(64, 3, 3))
+    assert np.array_equal(O.point_triangle_sqdists(p, tris),
+                          np.array([O.point_triangle_sqdist(p, *t) for t in tris]))
 
 
 def _tri():
