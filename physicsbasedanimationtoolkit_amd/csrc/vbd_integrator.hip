@@ -251,7 +251,10 @@ void LaunchAcceleration(pbat_vbd_integrator& h, int k, float& omega)
 }
 
 // Sweep every colour once (one VBD iteration); returns the number of launches.
-int SweepAllColours(pbat_vbd_integrator& h, device::SweepArgs& a)
+// One sweep over every colour. With contact, the last colour's new positions are
+// left in the write buffer when flush is false: the next sweep's first launch
+// reads them there and copies them back, as between colours.
+int SweepAllColours(pbat_vbd_integrator& h, device::SweepArgs& a, bool flush = true)
 {
     int const nColors = static_cast<int>(h.colorBegin.size()) - 1;
     int launched = 0;
@@ -265,7 +268,8 @@ int SweepAllColours(pbat_vbd_integrator& h, device::SweepArgs& a)
         LaunchSweep(h, a, nc);
         ++launched;
     }
-    FlushContactSweep(h);
+    if (flush)
+        FlushContactSweep(h);
     return launched;
 }
 
@@ -422,7 +426,10 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         else
             for (int k = 0; k < iterations; ++k)
             {
-                launched += SweepAllColours(h, a);
+                // plain VBD reads nothing between sweeps: the write buffer is
+                // copied back after the last one
+                bool const keep = h.acceleration == PBAT_VBD_ACCELERATION_NONE && k + 1 < iterations;
+                launched += SweepAllColours(h, a, !keep);
                 LaunchAcceleration(h, k, omega);
             }
         if (eventBrackets)
