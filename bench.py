@@ -119,7 +119,8 @@ def bench_adjoint(args, torch):
     hidden per-tet log Y log-uniform in [1e5, 1e7], target = 20-step rollout
     (h = 1/60, 20 iterations) from v0 = 0.5 x random unit vectors; loss
     L = sum |x_T - x_T^target|^2 from log Y = ln 1e6. One fwd+adjoint pass = the
-    20-step forward (state kept every 10 steps) + the IFT backward (one
+    20-step forward (state kept after every step: 34 MB, so the backward pass
+    recomputes nothing) + the IFT backward (one
     two-level PCG solve per step, tolerance 1e-6 per mesh) giving dL/dlogY,
     dL/dv0, dL/dx0. Also: central differences of the device forward on 64
     random log Y entries (the forward is truncated at 20 iterations, so these
@@ -131,7 +132,7 @@ def bench_adjoint(args, torch):
     g = pbat.gpu.vbd.Integrator(_diff_data(pbat, s))
     g.autotune(s.dt, sweeps=5)
     v0 = g.v.copy()
-    dt, it, T, every = s.dt, s.iterations, s.steps, 10
+    dt, it, T, every = s.dt, s.iterations, s.steps, 1
     solver = dict(solver="pcg_two_level", tolerance=1e-6, max_iterations=2000, check_every=10)
 
     def forward(logy):

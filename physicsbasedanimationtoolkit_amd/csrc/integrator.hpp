@@ -128,6 +128,7 @@ struct RolloutState
     int iterations = 0, steps = 0, every = 1;
     std::vector<DeviceArray<float4>> ckptX, ckptV, ckptVprev; // every `every` steps
     std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
+    DeviceArray<float4> lastX;                                // positions after the last step
     DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
     DeviceArray<float4> dinv;                                 // inverse diagonal blocks, 2 per vertex
     DeviceArray<double> coarse[2];                            // coarse operator / its inverse (ping-pong)
@@ -283,6 +284,7 @@ struct pbat_vbd_integrator
         xfull.reset(), xb.reset();
         triA.reset(), triB.reset(), triC.reset(), crec.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
+        rollout.lastX.reset();
         if (stream)
             (void)hipStreamDestroy(stream);
     }

@@ -493,6 +493,9 @@ void RolloutForward(pbat_vbd_integrator& h, float dt, int iterations, int steps,
         }
         Step(h, dt, iterations, 1);
     }
+    if (R.lastX.n != n)
+        R.lastX.allocate(n);
+    PBAT_HIP(hipMemcpyAsync(R.lastX.ptr, h.xm.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, h.stream));
     R.dt = dt, R.iterations = iterations, R.steps = steps, R.every = every;
     R.valid = true;
 }
@@ -544,14 +547,21 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
     int const nSeg = static_cast<int>(R.ckptX.size());
     for (int sgm = nSeg - 1; sgm >= 0; --sgm)
     {
-        RestoreState(h, R.ckptX[sgm].ptr, R.ckptV[sgm].ptr, R.ckptVprev[sgm].ptr);
+        // the segment's minimisers: all but the last recomputed from its checkpoint
+        // (bitwise equal to the forward), the last one being the positions the next
+        // checkpoint (or the end of the rollout) saved
         int const t0 = sgm * R.every, t1 = std::min(R.steps, t0 + R.every);
-        for (int t = t0; t < t1; ++t)
+        if (t1 - t0 > 1)
+            RestoreState(h, R.ckptX[sgm].ptr, R.ckptV[sgm].ptr, R.ckptVprev[sgm].ptr);
+        for (int t = t0; t < t1 - 1; ++t)
         {
             Step(h, dt, R.iterations, 1);
             PBAT_HIP(hipMemcpyAsync(R.segX[t - t0].ptr, h.xm.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice,
                                     h.stream));
         }
+        float4 const* last = (sgm + 1 < nSeg) ? R.ckptX[sgm + 1].ptr : R.lastX.ptr;
+        PBAT_HIP(hipMemcpyAsync(R.segX[t1 - 1 - t0].ptr, last, sizeof(float4) * n, hipMemcpyDeviceToDevice,
+                                h.stream));
         for (int t = t1 - 1; t >= t0; --t)
         {
             // the step's minimiser, in the solver's fixed buffer

@@ -218,6 +218,30 @@ def test_checkpoint_interval_does_not_change_gradients(gpu_device):
             assert rel(r[k], out[0][k]) <= 1e-5, k
 
 
+def test_backward_reads_the_rollout_not_the_current_state(gpu_device):
+    """The backward pass takes each step's minimiser from the rollout (the last one from
+    the forward's own copy of its end positions), so stepping the integrator between the
+    forward and the backward pass (which keeps the rollout) changes no gradient, at every
+    checkpoint interval."""
+    X, E, mu, lam, v0 = diff_scene(6, seed=7)
+    g = make_gpu(X, E, mu, lam, v0)
+    w = np.random.default_rng(2).normal(size=X.shape)
+    for every in (1, 2, 5):
+        g.x = X
+        g.v = v0
+        g.rollout_forward(DT, 10, 5, checkpoint_every=every)
+        ref = g.rollout_backward(w, None, tolerance=1e-6)
+        g.x = X
+        g.v = v0
+        g.rollout_forward(DT, 10, 5, checkpoint_every=every)
+        g.step(DT, 10, 1)
+        moved = g.x
+        out = g.rollout_backward(w, None, tolerance=1e-6)
+        np.testing.assert_array_equal(g.x, moved)
+        for k in ("log_young_modulus", "v0", "x0"):
+            assert rel(out[k], ref[k]) <= 1e-5, (every, k)
+
+
 def test_backward_restores_end_state(gpu_device):
     X, E, mu, lam, v0 = diff_scene(6, seed=6)
     a = make_gpu(X, E, mu, lam, v0)
