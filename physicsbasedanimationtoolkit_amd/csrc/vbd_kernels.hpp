@@ -85,7 +85,7 @@ __device__ __forceinline__ void GroupSum(Block& B)
 //   Hess_i += w mu |g|^2 I + w lambda c c^T
 // The d2J/dF2 term of the SNH Hessian contracts to zero for one vertex:
 // det(F0 + x_i g^T) is affine in x_i.
-// alpha = 1 + mu/lambda = 1 + (w mu)/(w lambda).
+// alpha = 1 + mu/lambda, so w lambda (J - alpha) = w lambda (J - 1) - w mu.
 __device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 const& q1, float2 const& q2, float wmu,
                                                    float4 const& own, float4 const& pa, float4 const& pb,
                                                    float4 const& pc, Block& B)
@@ -93,7 +93,7 @@ __device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 cons
     float const gax = q0.x, gay = q0.y, gaz = q0.z;
     float const gbx = q0.w, gby = q1.x, gbz = q1.y;
     float const gcx = q1.z, gcy = q1.w, gcz = q2.x;
-    float const wlam = q2.y, alpha = 1.f + wmu / wlam;
+    float const wlam = q2.y;
     float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
     float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
     float const dcx = pc.x - own.x, dcy = pc.y - own.y, dcz = pc.z - own.z;
@@ -117,7 +117,7 @@ __device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 cons
     float const Fgx = f0x * gx + f1x * gy + f2x * gz;
     float const Fgy = f0y * gx + f1y * gy + f2y * gz;
     float const Fgz = f0z * gx + f1z * gy + f2z * gz;
-    float const s = wlam * (J - alpha);
+    float const s = wlam * (J - 1.f) - wmu; // = w lambda (J - alpha), without a division
     B.g0 += wmu * Fgx + s * kx;
     B.g1 += wmu * Fgy + s * ky;
     B.g2 += wmu * Fgz + s * kz;
