@@ -412,8 +412,9 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
     if (!anyWork)
         return;
     // seed the nearest distance with the vertex's first contact of its last list
-    // when that triangle lies within reach: it belongs to the searched set, so
-    // the minimum is unchanged, and the search starts from a tight bound
+    // when that triangle lies well within reach (by 0.1%, far beyond the rounding
+    // of the distances): it belongs to the searched set, so the minimum is
+    // unchanged, and the search starts from a tight bound
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
     {
@@ -423,7 +424,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
         if (f0 >= 0)
         {
             float const d0 = PointTriangleSqDist(p[j], c.triA[f0], c.triB[f0], c.triC[f0]);
-            if (d0 <= r[j] + kFltEpsilon)
+            if (d0 <= (r[j] + kFltEpsilon) * 0.999f)
                 dmin[j] = d0;
         }
     }
