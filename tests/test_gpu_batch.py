@@ -95,3 +95,24 @@ def test_chebyshev_batch_equals_per_mesh_integrators_bitwise(gpu_device):
             g.step(dt, it, 1)
     for xb, g in zip(batch.mesh_positions(), singles):
         assert np.array_equal(xb, g.x)
+
+
+def test_full_batched_workload_equals_per_mesh(gpu_device):
+    """The batched benchmark's workload at full size (256 jittered meshes, grid dims
+    U{4..24} per axis, per-mesh Y and nu, h = 1/60, 10 iterations): one batched step
+    (fixed-stride rows) equals the 256 per-mesh integrators (CSR rows, the layout every
+    mesh has) bit for bit at the same lanes and prefetch depth, which give each lane the
+    same incidences in the same order; and no Dirichlet vertex moves."""
+    scenes = meshes.batched_scenes(256, 4, 24, seed=0)
+    datas = [construct(s) for s in scenes]
+    batch = pbat.gpu.vbd.BatchIntegrator(datas)
+    batch.set_sweep_config(16, 2, xcd_remap=True, fixed_stride=True)
+    dt, it = scenes[0].dt, scenes[0].iterations
+    batch.step(dt, it, 1)
+    for xb, d, s in zip(batch.mesh_positions(), datas, scenes):
+        g = pbat.gpu.vbd.Integrator(d)
+        g.set_sweep_config(16, 2, xcd_remap=False, fixed_stride=False)
+        g.step(dt, it, 1)
+        assert np.array_equal(xb, g.x)
+        assert np.array_equal(xb[:, s.dbc], s.X[:, s.dbc].astype(np.float32))
+        del g
