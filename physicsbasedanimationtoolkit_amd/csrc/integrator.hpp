@@ -224,8 +224,10 @@ struct pbat_vbd_integrator
     double eta = 0.2, tau = 2.;               // trust region: acceptance threshold, radius factor
     bool curved = true;                       // trust region: curved (plain VBD iterates) or linear path
     pbat_vbd::DeviceArray<double> trPartials; // trust region: per-workgroup reduction partials + sums
-    std::vector<double> trT;                  // trust region: t of each iteration k >= 2 of the last Step
-    std::vector<int> trAccepted;              // trust region: whether that step was kept
+    pbat_vbd::DeviceArray<double> trState;    // trust region: the model's FP64 scalars (device::TrSlot)
+    pbat_vbd::DeviceArray<double> trLogT;     // trust region: t of each iteration k >= 2 of the last Step
+    pbat_vbd::DeviceArray<int> trLogAccepted; // trust region: whether that step was kept
+    int64_t trLogCount = 0;                   // trust region: entries of the last Step's log
     // Vertex-triangle contact (on when two bodies share the collision mesh)
     bool contact = false;
     int nCV = 0, nF = 0, activeSetFrequency = 1;
@@ -278,7 +280,8 @@ struct pbat_vbd_integrator
         aggOf.reset(), aggMesh.reset(), meshAggBegin.reset(), meshCoarseOffset.reset();
         aggVertPtr.reset(), aggVerts.reset();
         gkm1.reset(), fkm1.reset(), dgk.reset(), dfk.reset(), andersonAcc.reset(), andersonGram.reset();
-        trPartials.reset(), cvIdx.reset(), cvBody.reset(), fc.reset(), cvActive.reset(), cvInAv.reset();
+        trPartials.reset(), trState.reset(), trLogT.reset(), trLogAccepted.reset();
+        cvIdx.reset(), cvBody.reset(), fc.reset(), cvActive.reset(), cvInAv.reset();
         tri.reset(), triArea.reset(), xva.reset(), dupper.reset();
         triLo.reset(), triHi.reset(), chunkLo.reset(), chunkHi.reset(), superLo.reset(), superHi.reset();
         xfull.reset(), xb.reset();

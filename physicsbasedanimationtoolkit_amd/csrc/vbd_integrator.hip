@@ -275,20 +275,36 @@ int SweepAllColours(pbat_vbd_integrator& h, device::SweepArgs& a, bool flush = t
 
 constexpr int kObjectiveBlocks = 512;
 
-// (sum m |x - x~|^2, sum w Psi(F), sum |x - xprev|^2) at the current iterate, in
-// FP64 with a fixed reduction order, read back on the host (synchronises).
-void ObjectiveTermsOnHost(pbat_vbd_integrator& h, float4 const* xprev, double out[3])
+// (sum m |x - x~|^2, sum w Psi(F), sum |x - xprev|^2) at the current iterate into
+// the device sums, in FP64 with a fixed reduction order; skipped when gate points
+// at 0.
+void ObjectiveTermsOnDevice(pbat_vbd_integrator& h, float4 const* xprev, double const* gate)
 {
     double* part = h.trPartials.ptr;
     double* sums = part + 3 * kObjectiveBlocks;
     hipLaunchKernelGGL(device::ObjectiveTerms, dim3(kObjectiveBlocks), dim3(256), 0, h.stream,
                        static_cast<float4 const*>(h.xm.ptr), static_cast<float4 const*>(h.xtil.ptr), xprev,
                        static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr),
-                       static_cast<int>(h.nV), static_cast<int>(h.nT), part);
+                       static_cast<int>(h.nV), static_cast<int>(h.nT), part, gate);
     hipLaunchKernelGGL(device::SumPartials3, dim3(1), dim3(256), 0, h.stream, static_cast<double const*>(part),
-                       kObjectiveBlocks, sums);
-    PBAT_HIP(hipMemcpyAsync(out, sums, 3 * sizeof(double), hipMemcpyDeviceToHost, h.stream));
-    PBAT_HIP(hipStreamSynchronize(h.stream));
+                       kObjectiveBlocks, sums, gate);
+}
+
+// Trust-region log slots of one step: one per iteration k >= 2 of each substep.
+int64_t TrLogSlots(int iterations, int substeps)
+{
+    return static_cast<int64_t>(std::max(iterations - 2, 0)) * std::max(substeps, 0);
+}
+
+// Device buffers of the linear trust-region path for a step of this shape
+// (allocated before any capture).
+void EnsureTrustRegionBuffers(pbat_vbd_integrator& h, int iterations, int substeps)
+{
+    if (!h.trState.ptr)
+        h.trState.allocate(device::kTrSlots);
+    size_t const slots = static_cast<size_t>(std::max<int64_t>(TrLogSlots(iterations, substeps), 1));
+    if (h.trLogT.n < slots)
+        h.trLogT.allocate(slots), h.trLogAccepted.allocate(slots);
 }
 
 // TrustRegionIntegrator::SolveWithLinearAcceleratedPath
@@ -298,95 +314,47 @@ void ObjectiveTermsOnHost(pbat_vbd_integrator& h, float4 const* xprev, double ou
 // through the last three (t, f) pairs, shifted so the current iterate sits at
 // t = 0 (:452-468), picks t in [1, R / |dx|] along x(t) = x_k + t dx, dx the VBD
 // step (:400-430); the step is kept if (f_k - f(x(t))) / (f_k - m(t)) > eta, else
-// rolled back to the VBD iterate; R grows or shrinks by tau. Scalars live on the host in FP64
-// (the reference keeps them in FP32); each iteration reads one or two reductions.
+// rolled back to the VBD iterate; R grows or shrinks by tau. The scalars are FP64
+// on the device (the reference keeps them in FP32 and reads each reduction back);
+// the objective after a step that is not tried, or not rolled back, is skipped.
+// Logs (t, accepted) of substep `substep` from slot substep * (iterations - 2).
 // Returns the number of sweep launches.
-int TrustRegionSolve(pbat_vbd_integrator& h, device::SweepArgs& a, int iterations, float sdt)
+int TrustRegionSolve(pbat_vbd_integrator& h, device::SweepArgs& a, int iterations, float sdt, int substep)
 {
-    double const dt2 = double(sdt) * double(sdt);
-    double const small = double(1e-6f);
-    double const fltMin = double(std::numeric_limits<float>::min());
-    double const fltMax = double(std::numeric_limits<float>::max());
-    double const tau2 = h.tau * h.tau;
+    device::TrConstants const c{double(sdt) * double(sdt), h.tau * h.tau, h.eta, double(1e-6f),
+                                double(std::numeric_limits<float>::min()), double(std::numeric_limits<float>::max())};
+    double* st = h.trState.ptr;
+    double const* sums = h.trPartials.ptr + 3 * kObjectiveBlocks;
     int const n = static_cast<int>(h.nV);
     int const grid = (n + 255) / 256;
     size_t const bytes = sizeof(float4) * static_cast<size_t>(h.nV);
-    double r[3];
-    auto objective = [&](float4 const* xprev) {
-        ObjectiveTermsOnHost(h, xprev, r);
-        return 0.5 * r[0] + dt2 * r[1];
-    };
-    auto remember = [&] {
-        PBAT_HIP(hipMemcpyAsync(h.xkm1.ptr, h.xm.ptr, bytes, hipMemcpyDeviceToDevice, h.stream));
-    };
     int launched = 0;
-    double fk = objective(nullptr), tk = -1., fkm1 = 0., fkm2 = 0., tkm1 = 0., tkm2 = 0., R2 = 0.;
+    ObjectiveTermsOnDevice(h, nullptr, nullptr);
+    hipLaunchKernelGGL(device::TrBegin, dim3(1), dim3(64), 0, h.stream, st, sums, c);
     for (int k = 0; k < iterations; ++k)
     {
         if (k < 2)
         {
-            fkm2 = fkm1, fkm1 = fk, tkm2 = tkm1, tkm1 = tk;
-            remember();
             launched += SweepAllColours(h, a);
-            fk = objective(nullptr);
-            tk = static_cast<double>(k);
+            ObjectiveTermsOnDevice(h, nullptr, nullptr);
+            hipLaunchKernelGGL(device::TrPlain, dim3(1), dim3(64), 0, h.stream, st, sums, c, k);
             continue;
         }
-        // ConstructModel: shift time so t_k = 0; c = f_k and a, b from the other two
-        // points by Cramer's rule (the times are distinct and nonzero)
-        tkm2 -= tk, tkm1 -= tk, tk = 0.;
-        double const det = tkm2 * tkm2 * tkm1 - tkm1 * tkm1 * tkm2;
-        double const r2 = fkm2 - fk, r1 = fkm1 - fk;
-        double const qa = (r2 * tkm1 - r1 * tkm2) / det;
-        double const qb = (tkm2 * tkm2 * r1 - tkm1 * tkm1 * r2) / det;
-        double const qc = fk;
-        // UpdateIterates
-        fkm2 = fkm1, fkm1 = fk, tkm2 = tkm1, tkm1 = tk;
-        remember();
+        hipLaunchKernelGGL(device::TrModel, dim3(1), dim3(64), 0, h.stream, st);
+        PBAT_HIP(hipMemcpyAsync(h.xkm1.ptr, h.xm.ptr, bytes, hipMemcpyDeviceToDevice, h.stream));
         launched += SweepAllColours(h, a);
-        double const fVbd = objective(h.xkm1.ptr);
-        double const dx2 = r[2];
-        if (R2 < dx2 + small)
-            R2 = tau2 * dx2;
-        double const lower = 1., upper = std::sqrt(R2 / dx2);
-        // ModelOptimalStep
-        double tstar = 0.;
-        if (std::abs(qa) > fltMin)
-            tstar = qa > fltMin ? -qb / (2. * qa) : fltMax;
-        else if (std::abs(qb) > fltMin)
-            tstar = qb > 0. ? -fltMax : fltMax;
-        double const t = tstar < lower ? lower : (upper < tstar ? upper : tstar);
-        bool const attempt = !(std::abs(t - lower) < small);
-        bool accepted = false;
-        if (attempt)
-        {
-            hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
-                               static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<float>(t), 0);
-            fk = objective(nullptr);
-            double const mNext = qa * t * t + qb * t + qc;
-            accepted = (fkm1 - fk) / (fkm1 - mNext) > h.eta;
-        }
-        if (accepted)
-        {
-            if ((upper - t) < small)
-                R2 *= tau2;
-            tk = t;
-        }
-        else
-        {
-            R2 /= tau2;
-            if (attempt)
-            {
-                hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
-                                   static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<float>(t), 1);
-                fk = objective(nullptr);
-            }
-            else
-                fk = fVbd;
-            tk = tkm1 + 1.;
-        }
-        h.trT.push_back(t);
-        h.trAccepted.push_back(accepted ? 1 : 0);
+        ObjectiveTermsOnDevice(h, h.xkm1.ptr, nullptr);
+        hipLaunchKernelGGL(device::TrDecide, dim3(1), dim3(64), 0, h.stream, st, sums, c);
+        hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
+                           static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<double const*>(st), 0);
+        ObjectiveTermsOnDevice(h, nullptr, st + device::kTrAttempt);
+        int const slot = substep * (iterations - 2) + (k - 2);
+        hipLaunchKernelGGL(device::TrAccept, dim3(1), dim3(64), 0, h.stream, st, sums, c, h.trLogT.ptr,
+                           h.trLogAccepted.ptr, slot);
+        hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
+                           static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<double const*>(st), 1);
+        ObjectiveTermsOnDevice(h, nullptr, st + device::kTrRevert);
+        hipLaunchKernelGGL(device::TrRevert, dim3(1), dim3(64), 0, h.stream, st, sums, c);
     }
     return launched;
 }
@@ -403,8 +371,7 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
     int const n = static_cast<int>(h.nV);
     int const eb = 256, eg = (n + eb - 1) / eb;
     device::SweepArgs a = MakeSweepArgs(h, sdt, sdt2);
-    h.trT.clear();
-    h.trAccepted.clear();
+    h.trLogCount = LinearTrustRegion(h) ? TrLogSlots(iterations, substeps) : 0;
     int total = 0;
     if (h.contact)
         ContactBeginStep(h, dt);
@@ -422,7 +389,7 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         float omega = 0.f;
         BeginAcceleration(h);
         if (LinearTrustRegion(h))
-            launched += TrustRegionSolve(h, a, iterations, sdt);
+            launched += TrustRegionSolve(h, a, iterations, sdt, s);
         else
             for (int k = 0; k < iterations; ++k)
             {
@@ -466,9 +433,9 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     if (iterations < 0 || substeps < 1)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "expected iterations >= 0 and substeps >= 1");
     PBAT_HIP(hipSetDevice(h.device));
-    // The linear trust-region path decides every iteration on the host from
-    // reductions it reads back, so its steps are launched directly, not captured.
-    if (!h.useGraph || LinearTrustRegion(h))
+    if (LinearTrustRegion(h))
+        EnsureTrustRegionBuffers(h, iterations, substeps);
+    if (!h.useGraph)
     {
         InvalidateGraph(h);
         EnqueueStep(h, dt, iterations, substeps, h.timing);
@@ -1585,9 +1552,17 @@ extern "C" int pbat_vbd_integrator_trust_region_log(pbat_vbd_integrator* h, doub
         auto& H = Handle(h);
         if (capacity < 0 || (capacity > 0 && (!t || !accepted)) || !count)
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid trust-region log buffers");
-        int64_t const n = static_cast<int64_t>(H.trT.size());
-        for (int64_t k = 0; k < std::min(n, capacity); ++k)
-            t[k] = H.trT[k], accepted[k] = H.trAccepted[k];
+        int64_t const n = H.trLogCount, m = std::min(n, capacity);
+        if (m > 0)
+        {
+            PBAT_HIP(hipSetDevice(H.device));
+            PBAT_HIP(hipStreamSynchronize(H.stream));
+            std::vector<int> acc(static_cast<size_t>(m));
+            PBAT_HIP(hipMemcpy(t, H.trLogT.ptr, sizeof(double) * m, hipMemcpyDeviceToHost));
+            PBAT_HIP(hipMemcpy(acc.data(), H.trLogAccepted.ptr, sizeof(int) * m, hipMemcpyDeviceToHost));
+            for (int64_t k = 0; k < m; ++k)
+                accepted[k] = acc[k];
+        }
         *count = n;
     });
 }

@@ -810,8 +810,11 @@ __device__ inline void BlockSum3(double& a, double& b, double& c)
 // every run. blockDim = 256.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void ObjectiveTerms(
     float4 const* __restrict__ xm, float4 const* __restrict__ xtil, float4 const* __restrict__ xprev,
-    int4 const* __restrict__ tetv, float4 const* __restrict__ tgp, int nV, int nT, double* __restrict__ part)
+    int4 const* __restrict__ tetv, float4 const* __restrict__ tgp, int nV, int nT, double* __restrict__ part,
+    double const* __restrict__ gate)
 {
+    if (gate && *gate == 0.)
+        return; // not needed this time: the partials keep their last values
     double K = 0., U = 0., D = 0.;
     int const stride = static_cast<int>(gridDim.x * blockDim.x);
     for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < nV; i += stride)
@@ -838,9 +841,13 @@ __device__ inline void BlockSum3(double& a, double& b, double& c)
 }
 
 // out[j] = sum over b < nb of part[3 b + j], in a fixed order. One workgroup of 256.
+// Nothing happens when gate points at 0.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void SumPartials3(double const* __restrict__ part, int nb,
-                                                                           double* __restrict__ out)
+                                                                           double* __restrict__ out,
+                                                                           double const* __restrict__ gate)
 {
+    if (gate && *gate == 0.)
+        return;
     double a = 0., b = 0., c = 0.;
     for (int k = static_cast<int>(threadIdx.x); k < nb; k += static_cast<int>(blockDim.x))
     {
@@ -857,15 +864,166 @@ __device__ inline void BlockSum3(double& a, double& b, double& c)
     }
 }
 
+// ---- linear trust-region path (gpu/impl/vbd/TrustRegionIntegrator.cu:36-263) ----
+// The model's FP64 scalars live in device memory and single-thread kernels
+// update them between the sweeps, so a step reads nothing back to the host and
+// can be replayed from a graph.
+enum TrSlot
+{
+    kTrF = 0,   // f_k
+    kTrT,       // t_k
+    kTrFm1,     // f_{k-1}
+    kTrFm2,     // f_{k-2}
+    kTrTm1,     // t_{k-1}
+    kTrTm2,     // t_{k-2}
+    kTrR2,      // squared trust radius
+    kTrQa,      // model m(t) = qa t^2 + qb t + qc
+    kTrQb,
+    kTrQc,
+    kTrFVbd,    // f at the VBD iterate
+    kTrStep,    // step length t tried this iteration
+    kTrUpper,   // upper bound of t
+    kTrAttempt, // 1 when t is tried (t != 1)
+    kTrRevert,  // 1 when the tried step is rolled back
+    kTrSlots
+};
+
+struct TrConstants
+{
+    double dt2, tau2, eta, small, fltMin, fltMax;
+};
+
+// f = 1/2 sum m |x - x~|^2 + h^2 sum w Psi from the reduced sums (K, U, .)
+__device__ inline double TrObjective(double const* sums, double dt2) { return 0.5 * sums[0] + dt2 * sums[1]; }
+
+// Start of a substep: f_0 at the initial iterate, empty history, R = 0.
+[[maybe_unused]] static __global__ void TrBegin(double* __restrict__ s, double const* __restrict__ sums,
+                                                TrConstants c)
+{
+    if (threadIdx.x != 0)
+        return;
+    s[kTrF] = TrObjective(sums, c.dt2);
+    s[kTrT] = -1.;
+    s[kTrFm1] = s[kTrFm2] = s[kTrTm1] = s[kTrTm2] = s[kTrR2] = 0.;
+}
+
+// After a plain iteration k < 2: the history shifts, then f_k, t_k = k.
+[[maybe_unused]] static __global__ void TrPlain(double* __restrict__ s, double const* __restrict__ sums,
+                                                TrConstants c, int k)
+{
+    if (threadIdx.x != 0)
+        return;
+    s[kTrFm2] = s[kTrFm1], s[kTrFm1] = s[kTrF];
+    s[kTrTm2] = s[kTrTm1], s[kTrTm1] = s[kTrT];
+    s[kTrF] = TrObjective(sums, c.dt2);
+    s[kTrT] = static_cast<double>(k);
+}
+
+// Before the sweep of iteration k >= 2 (ConstructModel and UpdateIterates): times
+// shifted so that t_k = 0, the quadratic through the last three (t, f) by
+// Cramer's rule, then the history shifts.
+[[maybe_unused]] static __global__ void TrModel(double* __restrict__ s)
+{
+    if (threadIdx.x != 0)
+        return;
+    double const tk = s[kTrT];
+    double const tkm2 = s[kTrTm2] - tk, tkm1 = s[kTrTm1] - tk;
+    double const fk = s[kTrF], fkm1 = s[kTrFm1], fkm2 = s[kTrFm2];
+    double const det = tkm2 * tkm2 * tkm1 - tkm1 * tkm1 * tkm2;
+    double const r2 = fkm2 - fk, r1 = fkm1 - fk;
+    s[kTrQa] = (r2 * tkm1 - r1 * tkm2) / det;
+    s[kTrQb] = (tkm2 * tkm2 * r1 - tkm1 * tkm1 * r2) / det;
+    s[kTrQc] = fk;
+    s[kTrFm2] = fkm1, s[kTrFm1] = fk;
+    s[kTrTm2] = tkm1, s[kTrTm1] = 0.;
+    s[kTrT] = 0.;
+}
+
+// After the sweep (sums at the VBD iterate, sums[2] = |dx|^2): the radius, the
+// model's optimal step clamped to [1, R/|dx|] (ModelOptimalStep), and whether
+// it is tried.
+[[maybe_unused]] static __global__ void TrDecide(double* __restrict__ s, double const* __restrict__ sums,
+                                                 TrConstants c)
+{
+    if (threadIdx.x != 0)
+        return;
+    double const fVbd = TrObjective(sums, c.dt2), dx2 = sums[2];
+    double R2 = s[kTrR2];
+    if (R2 < dx2 + c.small)
+        R2 = c.tau2 * dx2;
+    double const lower = 1., upper = sqrt(R2 / dx2);
+    double const qa = s[kTrQa], qb = s[kTrQb];
+    double tstar = 0.;
+    if (fabs(qa) > c.fltMin)
+        tstar = qa > c.fltMin ? -qb / (2. * qa) : c.fltMax;
+    else if (fabs(qb) > c.fltMin)
+        tstar = qb > 0. ? -c.fltMax : c.fltMax;
+    double const t = tstar < lower ? lower : (upper < tstar ? upper : tstar);
+    s[kTrR2] = R2, s[kTrFVbd] = fVbd, s[kTrUpper] = upper, s[kTrStep] = t;
+    s[kTrAttempt] = (fabs(t - lower) < c.small) ? 0. : 1.;
+}
+
+// After the tried step (sums at x(t), read only when a step was tried): keep it
+// when the actual over the predicted decrease exceeds eta, growing R at the
+// bound; otherwise shrink R and roll back (or keep the VBD iterate when nothing
+// was tried). Logs (t, accepted) at slot.
+[[maybe_unused]] static __global__ void TrAccept(double* __restrict__ s, double const* __restrict__ sums,
+                                                 TrConstants c, double* __restrict__ logT,
+                                                 int* __restrict__ logAccepted, int slot)
+{
+    if (threadIdx.x != 0)
+        return;
+    bool const attempt = s[kTrAttempt] != 0.;
+    double const t = s[kTrStep];
+    bool accepted = false;
+    if (attempt)
+    {
+        double const fk = TrObjective(sums, c.dt2);
+        s[kTrF] = fk;
+        double const mNext = s[kTrQa] * t * t + s[kTrQb] * t + s[kTrQc];
+        double const fkm1 = s[kTrFm1];
+        accepted = (fkm1 - fk) / (fkm1 - mNext) > c.eta;
+    }
+    if (accepted)
+    {
+        if ((s[kTrUpper] - t) < c.small)
+            s[kTrR2] *= c.tau2;
+        s[kTrT] = t;
+        s[kTrRevert] = 0.;
+    }
+    else
+    {
+        s[kTrR2] /= c.tau2;
+        s[kTrRevert] = attempt ? 1. : 0.;
+        if (!attempt)
+            s[kTrF] = s[kTrFVbd];
+        s[kTrT] = s[kTrTm1] + 1.;
+    }
+    logT[slot] = t;
+    logAccepted[slot] = accepted ? 1 : 0;
+}
+
+// After a rollback: f_k at the restored VBD iterate.
+[[maybe_unused]] static __global__ void TrRevert(double* __restrict__ s, double const* __restrict__ sums,
+                                                 TrConstants c)
+{
+    if (threadIdx.x != 0)
+        return;
+    if (s[kTrRevert] != 0.)
+        s[kTrF] = TrObjective(sums, c.dt2);
+}
+
 // Trust-region step along the line through x_k (xkm1) and the VBD iterate
 // (gpu/impl/vbd/TrustRegionIntegrator.cu:491-522): x = x_k + t (x - x_k), or, with
-// rollback, x = x_k + (x - x_k) / t. Masses in xm.w are kept.
+// rollback, x = x_k + (x - x_k) / t, t from the device state; nothing happens
+// unless the state says the step is tried (rolled back). Masses in xm.w are kept.
 [[maybe_unused]] static __global__ void LinearPathStep(float4* __restrict__ xm, float4 const* __restrict__ xkm1, int n,
-                                                       float t, int rollback)
+                                                       double const* __restrict__ s, int rollback)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n)
+    if (i >= n || s[rollback ? kTrRevert : kTrAttempt] == 0.)
         return;
+    float const t = static_cast<float>(s[kTrStep]);
     float4 const x = xm[i], p = xkm1[i];
     float4 r;
     if (rollback)

@@ -442,8 +442,8 @@ def test_trust_region_moving_16(gpu_device):
     same states: the first tried step of each substep has the device's t to 1% and
     the same verdict; later ones rest on objective differences of ~1e-5 of f and on
     second differences below that, which the FP32 trajectory (displacements ~1e-3
-    off the FP64 ones) moves, so only 75% of all verdicts must agree. Graph capture
-    on or off gives the same bits (the path runs uncaptured)."""
+    off the FP64 ones) moves, so only 75% of all verdicts must agree. The captured
+    step (decisions on the device) replays the eager launches bit for bit."""
     s = meshes.plain_gpu_scene(16, seed=3)
     v0 = np.random.default_rng(5).normal(size=(3, s.n_vertices)) * 0.3
     g1, ref = build_tr_pair(s, v=v0)
