@@ -327,15 +327,17 @@ int pbat_vbd_integrator_contacts(pbat_vbd_integrator* h, int32_t* active, int32_
  * step and whether the step was kept. The structured form of the reference's
  * profiling log lines "t={},t*={}" and "Accepted/Rejected TR step"
  * (gpu/impl/vbd/TrustRegionIntegrator.cu:111,133,146). Writes at most
- * capacity entries; *count receives the number recorded. */
+ * capacity entries; *count receives the number recorded. The log is kept in
+ * device memory during the step; this call synchronises the handle's stream. */
 int pbat_vbd_integrator_trust_region_log(pbat_vbd_integrator* h, double* t, int32_t* accepted, int64_t capacity,
                                          int64_t* count);
 
-/* Kernel-time instrumentation: when enabled, Step launches eagerly and each
- * substep's back-to-back sweep launches are bracketed by one pair of HIP
- * events on the handle's stream. pbat_vbd_integrator_sweep_timing returns
- * (summed event ms, number of sweep launches inside the brackets) since the
- * last reset. No reference counterpart. */
+/* Kernel-time instrumentation: when enabled, HIP events on the handle's stream
+ * bracket each replay of the captured step graph (which also holds the substep
+ * begin/end kernels), or, with graph capture off, each substep's back-to-back
+ * sweep launches. pbat_vbd_integrator_sweep_timing returns (summed event ms,
+ * number of sweep launches inside the brackets) since the last reset. No
+ * reference counterpart. */
 int pbat_vbd_integrator_set_timing(pbat_vbd_integrator* h, int32_t enabled);
 int pbat_vbd_integrator_sweep_timing(pbat_vbd_integrator* h, double* total_ms, int64_t* launches, int32_t reset);
 
