@@ -602,6 +602,8 @@ __global__ __launch_bounds__(256) void AndersonWindow(float4 const* __restrict__
 #pragma unroll
     for (int j = 0; j < 2 * M; ++j)
     {
+        if ((j < M ? j : j - M) >= mk)
+            continue; // a window column not filled yet (uniform over the launch)
         double v = vals[j];
         for (int off = 32; off > 0; off >>= 1)
             v += __shfl_xor(v, off);
