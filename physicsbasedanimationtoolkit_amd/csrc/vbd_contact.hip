@@ -29,7 +29,10 @@ namespace pbat_vbd::device
 
 constexpr int kContactBlock = 256;   // workgroup size of the detection kernels (4 waves)
 constexpr int kChunk = 64;           // triangles per chunk box = lanes per wave
-constexpr int kPerWave = 4;          // collision vertices per wave in the detection kernels
+// Collision vertices per wave in the detection kernels. One: a wave's chunk
+// visits then serve one query, so the few queries with far nearest triangles do
+// not hold up three others (8 to 200 cubes: 2-15% faster steps than with 4).
+constexpr int kPerWave = 1;
 constexpr int kGroup = 4 * kPerWave; // collision vertices per workgroup
 
 struct ContactArgs
