@@ -50,10 +50,11 @@ int EffectiveLanes(pbat_vbd_integrator const& h)
 {
     if (h.lanes > 0)
         return h.lanes;
-    // enough lanes that each handles about 3 incidences of the busiest vertex
+    // enough lanes that each handles about 2 incidences of the busiest vertex
+    // (degree 24: 16 lanes x 2, the shape the autotuner picks on the 64^3 grid)
     int const d = static_cast<int>(std::max<int64_t>(h.maxDegree, 1));
     int L = 1;
-    while (L < 64 && (d + L - 1) / L > 3)
+    while (L < 64 && (d + L - 1) / L > 2)
         L <<= 1;
     return L;
 }
