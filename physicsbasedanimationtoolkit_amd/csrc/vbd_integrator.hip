@@ -278,8 +278,9 @@ constexpr int kObjectiveBlocks = 512;
 
 // (sum m |x - x~|^2, sum w Psi(F), sum |x - xprev|^2) at the current iterate into
 // the device sums, in FP64 with a fixed reduction order; skipped when gate points
-// at 0.
-void ObjectiveTermsOnDevice(pbat_vbd_integrator& h, float4 const* xprev, double const* gate)
+// at 0. The trust-region update e runs after the sums (or in their place).
+void ObjectiveTermsOnDevice(pbat_vbd_integrator& h, float4 const* xprev, double const* gate,
+                            device::TrEpilogue const& e)
 {
     double* part = h.trPartials.ptr;
     double* sums = part + 3 * kObjectiveBlocks;
@@ -288,7 +289,7 @@ void ObjectiveTermsOnDevice(pbat_vbd_integrator& h, float4 const* xprev, double 
                        static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr),
                        static_cast<int>(h.nV), static_cast<int>(h.nT), part, gate);
     hipLaunchKernelGGL(device::SumPartials3, dim3(1), dim3(256), 0, h.stream, static_cast<double const*>(part),
-                       kObjectiveBlocks, sums, gate);
+                       kObjectiveBlocks, sums, gate, e);
 }
 
 // Trust-region log slots of one step: one per iteration k >= 2 of each substep.
@@ -325,37 +326,38 @@ int TrustRegionSolve(pbat_vbd_integrator& h, device::SweepArgs& a, int iteration
     device::TrConstants const c{double(sdt) * double(sdt), h.tau * h.tau, h.eta, double(1e-6f),
                                 double(std::numeric_limits<float>::min()), double(std::numeric_limits<float>::max())};
     double* st = h.trState.ptr;
-    double const* sums = h.trPartials.ptr + 3 * kObjectiveBlocks;
     int const n = static_cast<int>(h.nV);
     int const grid = (n + 255) / 256;
     size_t const bytes = sizeof(float4) * static_cast<size_t>(h.nV);
+    auto update = [&](int op) {
+        device::TrEpilogue e{};
+        e.op = op, e.s = st, e.c = c, e.logT = h.trLogT.ptr, e.logAccepted = h.trLogAccepted.ptr;
+        return e;
+    };
     int launched = 0;
-    ObjectiveTermsOnDevice(h, nullptr, nullptr);
-    hipLaunchKernelGGL(device::TrBegin, dim3(1), dim3(64), 0, h.stream, st, sums, c);
+    ObjectiveTermsOnDevice(h, nullptr, nullptr, update(device::kTrOpBegin));
     for (int k = 0; k < iterations; ++k)
     {
         if (k < 2)
         {
             launched += SweepAllColours(h, a);
-            ObjectiveTermsOnDevice(h, nullptr, nullptr);
-            hipLaunchKernelGGL(device::TrPlain, dim3(1), dim3(64), 0, h.stream, st, sums, c, k);
+            device::TrEpilogue e = update(device::kTrOpPlain);
+            e.k = k;
+            ObjectiveTermsOnDevice(h, nullptr, nullptr, e);
             continue;
         }
         hipLaunchKernelGGL(device::TrModel, dim3(1), dim3(64), 0, h.stream, st);
         PBAT_HIP(hipMemcpyAsync(h.xkm1.ptr, h.xm.ptr, bytes, hipMemcpyDeviceToDevice, h.stream));
         launched += SweepAllColours(h, a);
-        ObjectiveTermsOnDevice(h, h.xkm1.ptr, nullptr);
-        hipLaunchKernelGGL(device::TrDecide, dim3(1), dim3(64), 0, h.stream, st, sums, c);
+        ObjectiveTermsOnDevice(h, h.xkm1.ptr, nullptr, update(device::kTrOpDecide));
         hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
                            static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<double const*>(st), 0);
-        ObjectiveTermsOnDevice(h, nullptr, st + device::kTrAttempt);
-        int const slot = substep * (iterations - 2) + (k - 2);
-        hipLaunchKernelGGL(device::TrAccept, dim3(1), dim3(64), 0, h.stream, st, sums, c, h.trLogT.ptr,
-                           h.trLogAccepted.ptr, slot);
+        device::TrEpilogue accept = update(device::kTrOpAccept);
+        accept.slot = substep * (iterations - 2) + (k - 2);
+        ObjectiveTermsOnDevice(h, nullptr, st + device::kTrAttempt, accept);
         hipLaunchKernelGGL(device::LinearPathStep, dim3(grid), dim3(256), 0, h.stream, h.xm.ptr,
                            static_cast<float4 const*>(h.xkm1.ptr), n, static_cast<double const*>(st), 1);
-        ObjectiveTermsOnDevice(h, nullptr, st + device::kTrRevert);
-        hipLaunchKernelGGL(device::TrRevert, dim3(1), dim3(64), 0, h.stream, st, sums, c);
+        ObjectiveTermsOnDevice(h, nullptr, st + device::kTrRevert, update(device::kTrOpRevert));
     }
     return launched;
 }

@@ -842,34 +842,10 @@ __device__ inline void BlockSum3(double& a, double& b, double& c)
     }
 }
 
-// out[j] = sum over b < nb of part[3 b + j], in a fixed order. One workgroup of 256.
-// Nothing happens when gate points at 0.
-[[maybe_unused]] static __global__ __launch_bounds__(256) void SumPartials3(double const* __restrict__ part, int nb,
-                                                                           double* __restrict__ out,
-                                                                           double const* __restrict__ gate)
-{
-    if (gate && *gate == 0.)
-        return;
-    double a = 0., b = 0., c = 0.;
-    for (int k = static_cast<int>(threadIdx.x); k < nb; k += static_cast<int>(blockDim.x))
-    {
-        a += part[3 * k];
-        b += part[3 * k + 1];
-        c += part[3 * k + 2];
-    }
-    BlockSum3(a, b, c);
-    if (threadIdx.x == 0)
-    {
-        out[0] = a;
-        out[1] = b;
-        out[2] = c;
-    }
-}
-
 // ---- linear trust-region path (gpu/impl/vbd/TrustRegionIntegrator.cu:36-263) ----
-// The model's FP64 scalars live in device memory and single-thread kernels
-// update them between the sweeps, so a step reads nothing back to the host and
-// can be replayed from a graph.
+// The model's FP64 scalars live in device memory; TrModel and the epilogue of
+// the objective's last reduction (SumPartials3) update them between the sweeps,
+// so a step reads nothing back to the host and can be replayed from a graph.
 enum TrSlot
 {
     kTrF = 0,   // f_k
@@ -899,22 +875,16 @@ struct TrConstants
 __device__ inline double TrObjective(double const* sums, double dt2) { return 0.5 * sums[0] + dt2 * sums[1]; }
 
 // Start of a substep: f_0 at the initial iterate, empty history, R = 0.
-[[maybe_unused]] static __global__ void TrBegin(double* __restrict__ s, double const* __restrict__ sums,
-                                                TrConstants c)
+__device__ inline void TrBegin(double* __restrict__ s, double const* __restrict__ sums, TrConstants const& c)
 {
-    if (threadIdx.x != 0)
-        return;
     s[kTrF] = TrObjective(sums, c.dt2);
     s[kTrT] = -1.;
     s[kTrFm1] = s[kTrFm2] = s[kTrTm1] = s[kTrTm2] = s[kTrR2] = 0.;
 }
 
 // After a plain iteration k < 2: the history shifts, then f_k, t_k = k.
-[[maybe_unused]] static __global__ void TrPlain(double* __restrict__ s, double const* __restrict__ sums,
-                                                TrConstants c, int k)
+__device__ inline void TrPlain(double* __restrict__ s, double const* __restrict__ sums, TrConstants const& c, int k)
 {
-    if (threadIdx.x != 0)
-        return;
     s[kTrFm2] = s[kTrFm1], s[kTrFm1] = s[kTrF];
     s[kTrTm2] = s[kTrTm1], s[kTrTm1] = s[kTrT];
     s[kTrF] = TrObjective(sums, c.dt2);
@@ -944,11 +914,8 @@ __device__ inline double TrObjective(double const* sums, double dt2) { return 0.
 // After the sweep (sums at the VBD iterate, sums[2] = |dx|^2): the radius, the
 // model's optimal step clamped to [1, R/|dx|] (ModelOptimalStep), and whether
 // it is tried.
-[[maybe_unused]] static __global__ void TrDecide(double* __restrict__ s, double const* __restrict__ sums,
-                                                 TrConstants c)
+__device__ inline void TrDecide(double* __restrict__ s, double const* __restrict__ sums, TrConstants const& c)
 {
-    if (threadIdx.x != 0)
-        return;
     double const fVbd = TrObjective(sums, c.dt2), dx2 = sums[2];
     double R2 = s[kTrR2];
     if (R2 < dx2 + c.small)
@@ -969,12 +936,9 @@ __device__ inline double TrObjective(double const* sums, double dt2) { return 0.
 // when the actual over the predicted decrease exceeds eta, growing R at the
 // bound; otherwise shrink R and roll back (or keep the VBD iterate when nothing
 // was tried). Logs (t, accepted) at slot.
-[[maybe_unused]] static __global__ void TrAccept(double* __restrict__ s, double const* __restrict__ sums,
-                                                 TrConstants c, double* __restrict__ logT,
-                                                 int* __restrict__ logAccepted, int slot)
+__device__ inline void TrAccept(double* __restrict__ s, double const* __restrict__ sums, TrConstants const& c,
+                                double* __restrict__ logT, int* __restrict__ logAccepted, int slot)
 {
-    if (threadIdx.x != 0)
-        return;
     bool const attempt = s[kTrAttempt] != 0.;
     double const t = s[kTrStep];
     bool accepted = false;
@@ -1006,13 +970,81 @@ __device__ inline double TrObjective(double const* sums, double dt2) { return 0.
 }
 
 // After a rollback: f_k at the restored VBD iterate.
-[[maybe_unused]] static __global__ void TrRevert(double* __restrict__ s, double const* __restrict__ sums,
-                                                 TrConstants c)
+__device__ inline void TrRevert(double* __restrict__ s, double const* __restrict__ sums, TrConstants const& c)
 {
-    if (threadIdx.x != 0)
-        return;
     if (s[kTrRevert] != 0.)
         s[kTrF] = TrObjective(sums, c.dt2);
+}
+
+// Trust-region update that runs after a reduction, on its sums (SumPartials3).
+enum TrOp : int
+{
+    kTrOpNone = 0,
+    kTrOpBegin,
+    kTrOpPlain,
+    kTrOpDecide,
+    kTrOpAccept,
+    kTrOpRevert
+};
+
+struct TrEpilogue
+{
+    int op;          // TrOp
+    double* s;       // trust-region state
+    TrConstants c;
+    double* logT;    // kTrOpAccept: decision log
+    int* logAccepted;
+    int slot;        // kTrOpAccept: log slot
+    int k;           // kTrOpPlain: iteration
+};
+
+// out[j] = sum over b < nb of part[3 b + j], in a fixed order. One workgroup of 256.
+// The sums are skipped when gate points at 0; the epilogue's trust-region update
+// runs either way (it reads the sums only when they were formed).
+[[maybe_unused]] static __global__ __launch_bounds__(256) void SumPartials3(double const* __restrict__ part, int nb,
+                                                                           double* __restrict__ out,
+                                                                           double const* __restrict__ gate,
+                                                                           TrEpilogue e)
+{
+    if (!(gate && *gate == 0.))
+    {
+        double a = 0., b = 0., c = 0.;
+        for (int k = static_cast<int>(threadIdx.x); k < nb; k += static_cast<int>(blockDim.x))
+        {
+            a += part[3 * k];
+            b += part[3 * k + 1];
+            c += part[3 * k + 2];
+        }
+        BlockSum3(a, b, c);
+        if (threadIdx.x == 0)
+        {
+            out[0] = a;
+            out[1] = b;
+            out[2] = c;
+        }
+    }
+    if (threadIdx.x != 0)
+        return;
+    switch (e.op)
+    {
+    case kTrOpBegin:
+        TrBegin(e.s, out, e.c);
+        break;
+    case kTrOpPlain:
+        TrPlain(e.s, out, e.c, e.k);
+        break;
+    case kTrOpDecide:
+        TrDecide(e.s, out, e.c);
+        break;
+    case kTrOpAccept:
+        TrAccept(e.s, out, e.c, e.logT, e.logAccepted, e.slot);
+        break;
+    case kTrOpRevert:
+        TrRevert(e.s, out, e.c);
+        break;
+    default:
+        break;
+    }
 }
 
 // Trust-region step along the line through x_k (xkm1) and the VBD iterate
