@@ -45,6 +45,35 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _kernel_source_sha256():
+    import hashlib
+
+    path = os.path.join(ROOT, "physicsbasedanimationtoolkit_amd", "csrc", "vbd_kernels.hpp")
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def _counter_traffic(name, shape):
+    """HBM bytes per sweep launch from a committed rocprofv3 counter summary
+    (profiles/<name>: FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
+    reads for a device copy of known size). Used only when the summary was measured
+    on this build's sweep kernel source and the sweep shape now in effect; returns
+    (summary or None, provenance note)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None, f"none: profiles/{name} absent"
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("kernel_source_sha256") != _kernel_source_sha256():
+        return None, f"none: profiles/{name} was measured on another version of the sweep kernel source"
+    want = {k: shape[k] for k in ("lanes_per_vertex", "incidences_per_lane", "xcd_remap", "nontemporal",
+                                  "fixed_stride")}
+    if t.get("sweep_shape") != want:
+        return None, f"none: profiles/{name} was measured with sweep shape {t.get('sweep_shape')}, now {want}"
+    return t, (f"profiles/{name} (rocprofv3 --pmc FETCH_SIZE x {t['fetch_scale']:.3f} + WRITE_SIZE x "
+               f"{t['write_scale']:.3f}, per launch, same kernel source and sweep shape)")
+
+
 def bench_batched(args, torch):
     """Batched VBD: 256 jittered cube meshes with grid dims U{4..24} per axis, per-mesh
     Y log-uniform in [1e5, 1e7] and nu uniform in [0.3, 0.48], h = 1/60, 10 iterations.
@@ -496,13 +525,9 @@ def main():
     bytes_per_sweep = BYTES_PER_VERTEX * nUpd + BYTES_PER_INCIDENCE * nInc
     bytes_per_launch = bytes_per_sweep / nColors
     achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = None
-    pmc = os.path.join(ROOT, "profiles", "vbd_sweep_64cube_traffic.json")
-    if os.path.exists(pmc):
-        with open(pmc) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+    traffic, traffic_note = _counter_traffic("vbd_sweep_64cube_traffic.json", shape)
     log(f"[bench] sweep kernel: {launches} launches, avg {avg_launch_s * 1e6:.2f} us, "
-        f"{bytes_per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s")
+        f"{bytes_per_launch / 1e6:.2f} MB/launch -> {achieved:.1f} GB/s; counter traffic {traffic_note}")
 
     batched = None if args.no_batched else bench_batched(args, torch)
     variants = None if args.no_variants else bench_variants(args, torch)
@@ -515,11 +540,18 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle as O
 
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # every core this process may run on (its affinity mask), within the job's
+        # CPU share when the environment states one (OMP_NUM_THREADS)
+        affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = args.cpu_threads or (min(affinity, share) if share > 0 else affinity)
         ref = O.OracleVbd(s.X, s.E, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
                           nthreads=threads)
+        ref.step(dt, iters, 1)  # warm-up step
+        cpu_steps = 3
         t0 = time.perf_counter()
-        ref.step(dt, iters, 1)
+        for _ in range(cpu_steps):
+            ref.step(dt, iters, 1)
         cpu_s = time.perf_counter() - t0
         model = ""
         try:
@@ -527,20 +559,25 @@ def main():
                 model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
         except OSError:
             pass
-        cpu = {"value": nUpd * iters / cpu_s, "unit": "vertex-updates/s", "cores": threads, "kind": "port",
-               "sample": f"1 step x {iters} iterations of the same 64^3 workload, FP64 oracle, "
-                         f"{threads} threads, {cpu_s:.2f}s",
-               "host": {"cpu_model": model, "os_cpu_count": os.cpu_count()}}
+        cpu = {"value": nUpd * iters * cpu_steps / cpu_s, "unit": "vertex-updates/s", "cores": threads,
+               "kind": "port",
+               "sample": f"1 warm-up + {cpu_steps} timed steps x {iters} iterations of the same 64^3 workload "
+                         f"(from rest), FP64 oracle, {threads} threads, {cpu_s:.2f}s timed",
+               "host": {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cpus": affinity,
+                        "omp_num_threads": share or None}}
         log(f"[bench] cpu baseline {cpu['value']:.4e} vu/s on {threads} threads")
-        # one thread, on a shorter sample (the same first step, fewer iterations)
-        iters1 = 2
+        # one thread, on a shorter sample: a warm-up step of 1 iteration, then one
+        # step of 4 iterations
+        iters1 = 4
         ref1 = O.OracleVbd(s.X, s.E, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
                            nthreads=1)
+        ref1.step(dt, 1, 1)
         t0 = time.perf_counter()
         ref1.step(dt, iters1, 1)
         cpu1_s = time.perf_counter() - t0
         cpu["single_thread"] = {"value": nUpd * iters1 / cpu1_s, "unit": "vertex-updates/s", "cores": 1,
-                                "sample": f"1 step x {iters1} iterations of the same workload, {cpu1_s:.2f}s"}
+                                "sample": f"1 warm-up step x 1 iteration, then 1 step x {iters1} iterations of "
+                                          f"the same workload, {cpu1_s:.2f}s"}
         log(f"[bench] cpu baseline {cpu['single_thread']['value']:.4e} vu/s on 1 thread")
 
     out = {
@@ -562,7 +599,12 @@ def main():
                    "incidences": nInc, "parallelism": "single GPU",
                    "sweep_shape": dict(shape, hip_graph=not args.no_graph)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic["hbm_bytes_per_launch"],
+                     "traffic_source": traffic_note,
+                     "traffic_gbs": None if traffic is None else traffic["hbm_bytes_per_launch"] / avg_launch_s / 1e9,
+                     "traffic_frac": None if traffic is None
+                     else traffic["hbm_bytes_per_launch"] / avg_launch_s / 1e9 / HBM_PEAK_GBS,
                      "kernel": "VbdSweep", "avg_launch_us": avg_launch_s * 1e6,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,

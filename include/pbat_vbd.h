@@ -156,7 +156,10 @@ int pbat_vbd_integrator_create(const pbat_vbd_data* data, pbat_vbd_integrator** 
 /* gpu::vbd::Integrator::~Integrator (gpu/vbd/Integrator.cu:52-56) */
 int pbat_vbd_integrator_destroy(pbat_vbd_integrator* h);
 
-/* gpu::vbd::Integrator::Step (gpu/vbd/Integrator.h:72; gpu/impl/vbd/Integrator.cu:82-103) */
+/* gpu::vbd::Integrator::Step (gpu/vbd/Integrator.h:72; gpu/impl/vbd/Integrator.cu:82-103).
+ * Returns PBAT_VBD_ERROR_NONFINITE when a substep ended with a non-finite
+ * position or velocity at any vertex (the reference has no such check and
+ * carries NaN on); the state is left as computed. */
 int pbat_vbd_integrator_step(pbat_vbd_integrator* h, float dt, int32_t iterations, int32_t substeps);
 /* gpu::vbd::Integrator::TracedStep (gpu/vbd/Integrator.h:85-90; gpu/impl/vbd/Integrator.cu:105-148).
  * Writes MatrixMarket files named {variable}.t.{t}.s.{s}[.k.{k}].mtx into dir. */
@@ -237,6 +240,28 @@ int pbat_vbd_batch_vertex_offsets(pbat_vbd_integrator* h, int64_t* offsets, int3
  * gpu/impl/vbd/TrustRegionIntegrator.cu:265-364 and sim/vbd/Integrator.cpp:138-166. */
 int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3]);
 
+/* sim::vbd::Integrator::ObjectiveFunction and ObjectiveFunctionGradient
+ * (sim/vbd/Integrator.h:41-49, Integrator.cpp:138-200) on the device, reduced in
+ * FP64 in a fixed order: *f = 1/2 sum m |x - x~|^2 + dt^2 sum_e w_e Psi_e at
+ * positions x and inertial target xtilde (each 3 x nV FP32; NULL = the handle's
+ * current positions / the inertial target of its last substep); grad (3 x nV
+ * FP64, vertex-major as the reference's reshaped gradient, Dirichlet vertices
+ * included) its gradient; *residual2 the squared gradient norm over the
+ * non-Dirichlet vertices, the quantity the VBD iteration drives to zero. Each
+ * output pointer may be NULL. */
+int pbat_vbd_integrator_objective(pbat_vbd_integrator* h, const float* x, const float* xtilde, float dt, double* f,
+                                  double* grad, double* residual2);
+/* sim::vbd::Integrator::TraceNextStep followed by Step (sim/vbd/Integrator.cpp:22-52,
+ * 73-87, 202-235): one step whose iterates are traced. Before each VBD iteration and
+ * after the last, the FP64 objective, its gradient and the positions are recorded;
+ * per substep s they are written to {dir}/{t}.{s}.f.mtx ((iterations + 1) x 1),
+ * {t}.{s}.grad.mtx and {t}.{s}.x.mtx (3 nV x (iterations + 1), columns vertex-major),
+ * MatrixMarket dense arrays as the reference's Eigen::saveMarketDense writes them.
+ * The step itself is the handle's Step, run eagerly (the linear trust-region path
+ * runs plain VBD iterations here). */
+int pbat_vbd_integrator_trace_objective_step(pbat_vbd_integrator* h, float dt, int32_t iterations, int32_t substeps,
+                                             int32_t t, const char* dir);
+
 /* ------------------------------------------------------------------------- */
 /* Differentiable rollouts (no reference counterpart)                        */
 /* ------------------------------------------------------------------------- */
@@ -294,6 +319,13 @@ int pbat_vbd_rollout_forward(pbat_vbd_integrator* h, float dt, int32_t iteration
 int pbat_vbd_rollout_backward(pbat_vbd_integrator* h, const float* dL_dx, const float* dL_dv, float* dL_dlogY,
                               float* dL_dv0, float* dL_dx0, const pbat_vbd_adjoint_config* config,
                               pbat_vbd_adjoint_stats* stats);
+/* Identity of the rollout that pbat_vbd_rollout_backward would differentiate:
+ * *generation changes whenever a rollout is started (rollout_forward) or
+ * discarded (any of the setters listed above); *valid is 1 while one is held.
+ * A caller that keeps a rollout across other work (an autograd graph, say)
+ * compares the generation before calling rollout_backward. Either pointer may
+ * be NULL. */
+int pbat_vbd_rollout_generation(pbat_vbd_integrator* h, int64_t* generation, int32_t* valid);
 /* out = (M / h^2 + d2U/dx2(x)) p on free vertices (0 on Dirichlet ones);
  * x, p, out: 3 x nV. */
 int pbat_vbd_hessian_vector(pbat_vbd_integrator* h, const float* x, float dt, const float* p, float* out);
@@ -306,6 +338,14 @@ int pbat_vbd_adjoint_solve(pbat_vbd_integrator* h, const float* x, float dt, con
  * in bytes, and its peak since load or the last reset_peak != 0. No reference
  * counterpart. */
 int pbat_vbd_device_memory(int64_t* current, int64_t* peak, int32_t reset_peak);
+
+/* Number of 3x3 block solves of the sweep that took the FP64 path since the
+ * handle was created or last reset (reset != 0 clears it after reading). The
+ * sweep solves a vertex's block in FP32 by its adjugate, and in FP64 from the
+ * same FP32 entries when |det H| < 1e-4 |H00 H11 H22| (ill-conditioned; the
+ * reference solves in its build's scalar type, sim/vbd/Kernels.h:324-339). No
+ * reference counterpart. */
+int pbat_vbd_integrator_fp64_solves(pbat_vbd_integrator* h, int64_t* count, int32_t reset);
 
 /* Wait for all work queued on the handle's stream. */
 int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h);

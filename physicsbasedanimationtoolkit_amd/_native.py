@@ -116,6 +116,10 @@ SIGNATURES = [
     ("pbat_vbd_integrator_set_graph_capture", ctypes.c_int, [ctypes.c_void_p, c_i32]),
     ("pbat_vbd_integrator_energy", ctypes.c_int, [ctypes.c_void_p, p_f64]),
     ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("pbat_vbd_integrator_objective", ctypes.c_int, [ctypes.c_void_p, p_f32, p_f32, c_f32, p_f64, p_f64, p_f64]),
+    ("pbat_vbd_integrator_trace_objective_step", ctypes.c_int,
+     [ctypes.c_void_p, c_f32, c_i32, c_i32, c_i32, ctypes.c_char_p]),
+    ("pbat_vbd_integrator_fp64_solves", ctypes.c_int, [ctypes.c_void_p, p_i64, c_i32]),
     ("pbat_vbd_device_memory", ctypes.c_int, [p_i64, p_i64, c_i32]),
     ("pbat_vbd_integrator_counts", ctypes.c_int, [ctypes.c_void_p, p_i64, p_i64, p_i64]),
     ("pbat_vbd_integrator_contacts", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(c_i32), ctypes.POINTER(c_i32)]),
@@ -130,6 +134,7 @@ SIGNATURES = [
     ("pbat_vbd_rollout_backward", ctypes.c_int,
      [ctypes.c_void_p, p_f32, p_f32, p_f32, p_f32, p_f32, ctypes.POINTER(AdjointConfig),
       ctypes.POINTER(AdjointStats)]),
+    ("pbat_vbd_rollout_generation", ctypes.c_int, [ctypes.c_void_p, p_i64, ctypes.POINTER(c_i32)]),
     ("pbat_vbd_hessian_vector", ctypes.c_int, [ctypes.c_void_p, p_f32, c_f32, p_f32, p_f32]),
     ("pbat_vbd_adjoint_solve", ctypes.c_int,
      [ctypes.c_void_p, p_f32, c_f32, p_f32, p_f32, ctypes.POINTER(AdjointConfig), ctypes.POINTER(AdjointStats)]),

@@ -124,6 +124,14 @@ struct HostIncidenceRecords
 struct RolloutState
 {
     bool valid = false;
+    // changes whenever a rollout is started or discarded, so a caller holding a
+    // rollout can tell whether rollout_backward would still differentiate it
+    int64_t generation = 0;
+    void Discard()
+    {
+        valid = false;
+        ++generation;
+    }
     float dt = 0.f;
     int iterations = 0, steps = 0, every = 1;
     std::vector<DeviceArray<float4>> ckptX, ckptV, ckptVprev; // every `every` steps
@@ -138,10 +146,21 @@ struct RolloutState
     DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count
     DeviceArray<double> grad;                                 // d L / d log Y per tet
     DeviceArray<float> logY;                                  // parameter upload buffer
+    // step Hessian assembled per solve as 3x3 blocks over the updated vertices:
+    // the pattern and assembly sources (built once per handle), the values, the
+    // longest row, and the fused iteration's second direction buffer and
+    // last-workgroup counters
+    DeviceArray<int> bsrRowPtr, bsrCol, bsrBlockRow, bsrSrcPtr, bsrSrc;
+    DeviceArray<float4> bsrA, bsrB;
+    DeviceArray<float> bsrC;
+    int bsrMaxRow = 0;
+    DeviceArray<float4> p2;
+    DeviceArray<unsigned int> counters;
     // captured chunk of CG iterations and the settings it was captured for
     hipGraph_t cgGraph = nullptr;
     hipGraphExec_t cgExec = nullptr;
     std::vector<double> cgKey;
+    std::vector<void const*> cgPtrs; // device addresses the captured kernels take
     void ResetGraph()
     {
         if (cgExec)
@@ -151,6 +170,7 @@ struct RolloutState
         cgExec = nullptr;
         cgGraph = nullptr;
         cgKey.clear();
+        cgPtrs.clear();
     }
 };
 
@@ -242,6 +262,19 @@ struct pbat_vbd_integrator
     int pendBeg = 0, pendEnd = 0; // colour range whose new positions are still in xb
     float kD = 0.f;
     float detHZero = 1e-7f;
+    // Step status the kernels report: statusHost[0] = 1 when a substep ended with a
+    // non-finite position or velocity. Pinned, mapped, coherent host memory
+    // (statusDev is its device address), so the host reads it after its stream
+    // synchronisation with no copy; written by the device only in that case.
+    int* statusHost = nullptr;
+    int* statusDev = nullptr;
+    pbat_vbd::DeviceArray<unsigned int> fp64Solves; // 3x3 block solves that took the FP64 path
+    // FP64 objective and its gradient (allocated on first use): every vertex's
+    // incident (tet, local index) as 4 e + l (CSR), staged x / x~, the gradient
+    // and the reductions' per-workgroup partials
+    pbat_vbd::DeviceArray<int> objVtp, objInc;
+    pbat_vbd::DeviceArray<float4> objX, objXtil;
+    pbat_vbd::DeviceArray<double> objGrad, objPart;
     int blockSize = 256;
     int lanes = 0;         // 0 = automatic
     int perLane = 0;       // incidences prefetched per lane, 0 = automatic
@@ -288,6 +321,10 @@ struct pbat_vbd_integrator
         triA.reset(), triB.reset(), triC.reset(), crec.reset();
         rollout.ckptX.clear(), rollout.ckptV.clear(), rollout.ckptVprev.clear(), rollout.segX.clear();
         rollout.lastX.reset();
+        fp64Solves.reset();
+        objVtp.reset(), objInc.reset(), objX.reset(), objXtil.reset(), objGrad.reset(), objPart.reset();
+        if (statusHost)
+            (void)hipHostFree(statusHost);
         if (stream)
             (void)hipStreamDestroy(stream);
     }
@@ -303,6 +340,11 @@ void LaunchSweep(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts)
 device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2);
 // One forward Step (gpu/impl/vbd/Integrator.cu:82-103): graph replay or eager launches.
 void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps);
+// Status reported by the kernels of the work queued since the last ClearStatus,
+// read after a synchronisation of the handle's stream: fails with
+// PBAT_VBD_ERROR_NONFINITE when a substep ended with non-finite positions.
+void ClearStatus(pbat_vbd_integrator& h);
+void CheckStatus(pbat_vbd_integrator& h);
 void InvalidateGraph(pbat_vbd_integrator& h);
 // Host <-> device copies of 3 x nV column-major matrices (original vertex order)
 // into / out of float4 arrays (internal order); w components from wsrc (or 0).

@@ -16,9 +16,11 @@
 // conjugate-gradient iteration driven by the Hessian-vector kernel.
 #include "integrator.hpp"
 #include "vbd_adjoint_kernels.hpp"
+#include "vbd_hessian_kernels.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -127,9 +129,16 @@ void EnsureBuffers(pbat_vbd_integrator& h)
 {
     RolloutState& R = h.rollout;
     size_t const n = static_cast<size_t>(h.nV);
-    for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
+    for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
+    if (R.counters.n != 2)
+    {
+        // last-workgroup tickets of the fused CG iteration; each launch leaves them at 0
+        R.counters.allocate(2);
+        hipLaunchKernelGGL(device::ZeroScalars, dim3(1), dim3(64), 0, h.stream, static_cast<double*>(nullptr), 0,
+                           reinterpret_cast<int*>(R.counters.ptr), 2);
+    }
 
     if (R.dinv.n != 2 * n)
         R.dinv.allocate(2 * n);
@@ -195,7 +204,11 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
     for (int k = 0; k < maxN; ++k)
         hipLaunchKernelGGL(device::CoarseInvertStep, grid, dim3(256), 0, h.stream,
                            static_cast<double const*>(R.coarse[k & 1].ptr), R.coarse[(k + 1) & 1].ptr, k, c);
-    R.coarseFinal.upload(finalBuf, h.stream);
+    // allocated once per size: the captured CG chunk graph holds this pointer
+    if (R.coarseFinal.n != finalBuf.size())
+        R.coarseFinal.allocate(finalBuf.size());
+    PBAT_HIP(hipMemcpyAsync(R.coarseFinal.ptr, finalBuf.data(), finalBuf.size() * sizeof(int),
+                            hipMemcpyHostToDevice, h.stream));
     PBAT_HIP(hipStreamSynchronize(h.stream)); // finalBuf is a host temporary
     hipLaunchKernelGGL(device::CoarseGuard, dim3(nM), dim3(256), 0, h.stream, R.coarse[0].ptr, R.coarse[1].ptr,
                        static_cast<int const*>(R.coarseFinal.ptr), c);
@@ -225,22 +238,203 @@ void VertexPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, f
     LaunchAdjoint<M>(h, a);
 }
 
-// One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b.
-// With acc (a reverse pass after a forward pass, which updates each vertex
-// last), the accumulator r.z of each mesh receives b . y of the final values.
-void GaussSeidelPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b, bool reverse,
+// ---- assembled step Hessian (vbd_hessian_kernels.hpp) ------------------------
+
+// Block pattern of the step Hessian over the updated vertices (row i lists the
+// updated vertices sharing a tet with i, itself included, ascending) and each
+// block's assembly sources, from the element table; built once per handle.
+void BuildBsrPattern(pbat_vbd_integrator& h)
+{
+    RolloutState& R = h.rollout;
+    int const nUpd = static_cast<int>(h.nUpdated);
+    if (R.bsrRowPtr.n == static_cast<size_t>(nUpd) + 1)
+        return;
+    int64_t const nT = h.nT;
+    if (nT >= (int64_t{1} << 27))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "mesh too large for the adjoint's assembled Hessian (2^27 tets)");
+    std::vector<int4> TV(static_cast<size_t>(nT));
+    PBAT_HIP(hipMemcpy(TV.data(), h.tetv.ptr, TV.size() * sizeof(int4), hipMemcpyDeviceToHost));
+    auto vid = [&](int64_t e, int l) {
+        int4 const& t = TV[e];
+        return l == 0 ? t.x : l == 1 ? t.y : l == 2 ? t.z : t.w;
+    };
+    // updated vertex -> 4 e + l, ascending e
+    std::vector<int> vp(static_cast<size_t>(nUpd) + 1, 0);
+    for (int64_t e = 0; e < nT; ++e)
+        for (int l = 0; l < 4; ++l)
+            if (vid(e, l) < nUpd)
+                ++vp[vid(e, l) + 1];
+    for (int i = 0; i < nUpd; ++i)
+        vp[i + 1] += vp[i];
+    std::vector<int> vinc(static_cast<size_t>(vp[nUpd]));
+    {
+        std::vector<int> fill(vp.begin(), vp.end() - 1);
+        for (int64_t e = 0; e < nT; ++e)
+            for (int l = 0; l < 4; ++l)
+                if (vid(e, l) < nUpd)
+                    vinc[fill[vid(e, l)]++] = static_cast<int>(4 * e + l);
+    }
+    std::vector<int> rowPtr(static_cast<size_t>(nUpd) + 1, 0), col, blockRow, cols;
+    int maxRow = 0;
+    for (int i = 0; i < nUpd; ++i)
+    {
+        cols.clear();
+        for (int k = vp[i]; k < vp[i + 1]; ++k)
+            for (int l = 0; l < 4; ++l)
+            {
+                int const j = vid(vinc[k] >> 2, l);
+                if (j < nUpd)
+                    cols.push_back(j);
+            }
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        col.insert(col.end(), cols.begin(), cols.end());
+        blockRow.insert(blockRow.end(), cols.size(), i);
+        rowPtr[i + 1] = rowPtr[i] + static_cast<int>(cols.size());
+        maxRow = std::max(maxRow, static_cast<int>(cols.size()));
+    }
+    size_t const nBlocks = col.size();
+    std::vector<int> srcPtr(nBlocks + 1, 0), src;
+    auto forEachSource = [&](auto&& f) {
+        for (int i = 0; i < nUpd; ++i)
+            for (int k = vp[i]; k < vp[i + 1]; ++k)
+            {
+                int const e = vinc[k] >> 2, la = vinc[k] & 3;
+                for (int lb = 0; lb < 4; ++lb)
+                {
+                    int const j = vid(e, lb);
+                    if (j >= nUpd)
+                        continue;
+                    int const slot = static_cast<int>(
+                        std::lower_bound(col.begin() + rowPtr[i], col.begin() + rowPtr[i + 1], j) - col.begin());
+                    f(slot, 16 * e + 4 * la + lb);
+                }
+            }
+    };
+    forEachSource([&](int slot, int) { ++srcPtr[slot + 1]; });
+    for (size_t k = 0; k < nBlocks; ++k)
+        srcPtr[k + 1] += srcPtr[k];
+    src.resize(static_cast<size_t>(srcPtr[nBlocks]));
+    {
+        std::vector<int> fill(srcPtr.begin(), srcPtr.end() - 1);
+        forEachSource([&](int slot, int code) { src[fill[slot]++] = code; });
+    }
+    R.bsrRowPtr.upload(rowPtr, h.stream);
+    R.bsrCol.upload(col, h.stream);
+    R.bsrBlockRow.upload(blockRow, h.stream);
+    R.bsrSrcPtr.upload(srcPtr, h.stream);
+    R.bsrSrc.upload(src, h.stream);
+    R.bsrA.allocate(nBlocks), R.bsrB.allocate(nBlocks), R.bsrC.allocate(nBlocks);
+    R.bsrMaxRow = maxRow;
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+}
+
+// H = M / h^2 + sum_e w_e d2Psi_e/dx2 at xs into the block values, and the
+// inverse diagonal blocks into dinv.
+void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt)
+{
+    BuildBsrPattern(h);
+    RolloutState& R = h.rollout;
+    device::BsrPattern p{};
+    p.rowPtr = R.bsrRowPtr.ptr, p.col = R.bsrCol.ptr, p.blockRow = R.bsrBlockRow.ptr;
+    p.srcPtr = R.bsrSrcPtr.ptr, p.src = R.bsrSrc.ptr;
+    p.nRows = static_cast<int>(h.nUpdated), p.nBlocks = static_cast<int>(R.bsrCol.n);
+    if (p.nBlocks == 0)
+        return;
+    device::BsrValues const v{R.bsrA.ptr, R.bsrB.ptr, R.bsrC.ptr};
+    hipLaunchKernelGGL(device::BsrAssemble, dim3(Blocks(p.nBlocks)), dim3(256), 0, h.stream, xs,
+                       static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr), p,
+                       1. / (double(dt) * double(dt)), v, R.dinv.ptr);
+}
+
+// Lanes per row of the block products: the longest row rounded up to a power of
+// two (at most 64), one block per lane; longer rows take more per lane.
+int BsrLanes(pbat_vbd_integrator const& h)
+{
+    int const d = std::max(h.rollout.bsrMaxRow, 1);
+    int L = 1;
+    while (L < 64 && L < d)
+        L <<= 1;
+    return L;
+}
+
+int BsrPerLane(pbat_vbd_integrator const& h)
+{
+    int const d = std::max(h.rollout.bsrMaxRow, 1);
+    return std::clamp((d + BsrLanes(h) - 1) / BsrLanes(h), 1, 4);
+}
+
+using device::BsrMode;
+
+template <int L, int K, BsrMode M>
+void LaunchBsrLK(pbat_vbd_integrator& h, device::BsrArgs a)
+{
+    int const block = 256;
+    int64_t const threads = static_cast<int64_t>(a.vend - a.vbeg) * L;
+    int64_t grid64 = (threads + block - 1) / block;
+    a.xcdChunk = 0;
+    if (h.xcdRemap && grid64 >= 16)
+    {
+        grid64 = (grid64 + 7) / 8 * 8;
+        a.xcdChunk = static_cast<int>(grid64 / 8);
+    }
+    if (grid64 * block >= (int64_t{1} << 31))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "row range too large for the block product launch");
+    hipLaunchKernelGGL((device::BsrKernel<L, K, M>), dim3(static_cast<int>(grid64)), dim3(block), 0, h.stream, a);
+}
+
+template <int L, BsrMode M>
+void LaunchBsrL(pbat_vbd_integrator& h, device::BsrArgs const& a)
+{
+    switch (BsrPerLane(h))
+    {
+        case 1: LaunchBsrLK<L, 1, M>(h, a); break;
+        case 2: LaunchBsrLK<L, 2, M>(h, a); break;
+        case 3: LaunchBsrLK<L, 3, M>(h, a); break;
+        default: LaunchBsrLK<L, 4, M>(h, a); break;
+    }
+}
+
+// Block product of mode M over rows [vbeg, vend) (nothing for an empty range).
+template <BsrMode M>
+void BsrPass(pbat_vbd_integrator& h, device::BsrArgs a, int vbeg, int vend)
+{
+    if (vend <= vbeg)
+        return;
+    RolloutState& R = h.rollout;
+    a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
+    a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
+    a.meshOf = h.meshOf.ptr, a.aggOf = h.aggOf.ptr, a.dinv = R.dinv.ptr;
+    a.vbeg = vbeg, a.vend = vend;
+    switch (BsrLanes(h))
+    {
+        case 1: LaunchBsrL<1, M>(h, a); break;
+        case 2: LaunchBsrL<2, M>(h, a); break;
+        case 4: LaunchBsrL<4, M>(h, a); break;
+        case 8: LaunchBsrL<8, M>(h, a); break;
+        case 16: LaunchBsrL<16, M>(h, a); break;
+        case 32: LaunchBsrL<32, M>(h, a); break;
+        default: LaunchBsrL<64, M>(h, a); break;
+    }
+}
+
+// One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b
+// with the assembled blocks. With acc (a reverse pass after a forward pass,
+// which updates each vertex last), the accumulator r.z of each mesh receives
+// b . y of the final values.
+void GaussSeidelPass(pbat_vbd_integrator& h, device::CgState const& cg, float4* y, float4 const* b, bool reverse,
                      double* acc = nullptr)
 {
     int const nC = static_cast<int>(h.colorBegin.size()) - 1;
+    device::BsrArgs a{};
+    a.y = y, a.b = b, a.cg = cg, a.acc = acc, a.accArray = device::kAccRzNew;
     for (int k = 0; k < nC; ++k)
     {
         int const c = reverse ? nC - 1 - k : k;
         if (acc)
-            VertexPass<AdjMode::kGaussSeidelDot>(h, xs, dt, y, b, nullptr, acc, device::kAccRzNew, h.colorBegin[c],
-                                                 h.colorBegin[c + 1]);
+            BsrPass<BsrMode::kGaussSeidelDot>(h, a, h.colorBegin[c], h.colorBegin[c + 1]);
         else
-            VertexPass<AdjMode::kGaussSeidel>(h, xs, dt, y, b, nullptr, nullptr, 0, h.colorBegin[c],
-                                              h.colorBegin[c + 1]);
+            BsrPass<BsrMode::kGaussSeidel>(h, a, h.colorBegin[c], h.colorBegin[c + 1]);
     }
 }
 
@@ -280,21 +474,26 @@ SolveResult ReadResult(pbat_vbd_integrator& h)
 }
 
 // Solve H y = b (y holds the initial guess) for every mesh of the handle, each
-// with its own CG scalars and stopping test |b - H y| <= tol |b|; the host
-// polls the number of live meshes every cfg.check_every iterations.
+// with its own CG scalars and stopping test |b - H y| <= tol |b|. H is assembled
+// once per solve (AssembleBsr); the host polls the number of live meshes every
+// cfg.check_every iterations, and the iteration kernels do nothing once every
+// mesh has converged.
 //
-// One CG iteration is four launches with the block-Jacobi preconditioner:
-//   (1) q = H p, p.q;
-//   (2) y += alpha p, r -= alpha q (alpha = rz / pq), z = D^-1 r, r.r, r.z;
-//   (3) one workgroup: stopping test, beta, rotation, cleared accumulators;
-//   (4) p = z + beta p (finished meshes keep p);
-// the symmetric Gauss-Seidel preconditioner replaces the z of (2) by one
-// forward and one reverse coloured pass.
+// Block Jacobi and two-level preconditioners (the fused iteration), per iteration:
+//   (1) p = z + Z zc + beta p_old formed on the fly, q = H p, p . q;
+//   (2) one wave per aggregate: y += alpha p, r -= alpha q (alpha = rz / pq),
+//       z = D^-1 r, r . z, r . r and the restriction rc = Z^T r;
+//   (3) two-level: zc = (Z^T H Z)^-1 rc and rc . zc; the last workgroup ends the
+//       iteration (stopping test, beta with r . z = r . D^-1 r + rc . zc);
+//       block Jacobi: (2)'s last workgroup ends it.
+// The symmetric Gauss-Seidel preconditioner keeps the four-launch iteration
+//   q = H p and p . q; y, r update with z = 0; a forward and a reverse coloured
+//   Gauss-Seidel pass for z with r . z; one workgroup: scalars; p = z + beta p.
 //
-// With graph capture on (the integrator's setting), each chunk of check_every
-// iterations is captured once into a hipGraph and replayed; the graph's
-// kernels take the handle's fixed work vectors (x = hold, y, b = rhs), so it
-// stays valid across solves with the same settings.
+// With graph capture on (the integrator's setting), each chunk of iterations is
+// captured once into a hipGraph and replayed; the graph's kernels take the
+// handle's fixed work vectors (x = hold, y, b = rhs), so it stays valid across
+// solves with the same settings.
 SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b,
                   pbat_vbd_adjoint_config const& cfg)
 {
@@ -306,43 +505,24 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     cg.F = R.flags.ptr;
     cg.nM = nM;
     cg.tol2 = static_cast<double>(cfg.tolerance) * cfg.tolerance;
-    int const check = std::max(1, cfg.check_every);
     int const gridN = Blocks(n);
     bool const twoLevel = cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL && h.nAggregates > 0;
-    bool const jacobi = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI || cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL;
-    float4 const* dinv = jacobi ? R.dinv.ptr : nullptr;
-    device::CoarseArgs const cargs = MakeCoarseArgs(h);
-    double* const rc = twoLevel ? R.rc.ptr : nullptr;
-    // two-level: rc = Z^T r, zc = (Z^T H Z)^-1 rc, z += Z zc with r.z. Rows of the
-    // coarse inverse are read by 1, 2 or 4 waves, by the largest mesh's row length.
-    int maxCoarse = 0;
-    for (int m = 0; m < nM && twoLevel; ++m)
-        maxCoarse = std::max(maxCoarse, 3 * (h.aggBegin[m + 1] - h.aggBegin[m]));
-    int const coarseWavesPerRow = maxCoarse > 512 ? 4 : maxCoarse > 256 ? 2 : 1;
-    auto coarse = [&] {
-        int const rows = 3 * h.nAggregates;
-        hipLaunchKernelGGL(device::CoarseRestrict, dim3((h.nAggregates * 64 + 255) / 256), dim3(256), 0, h.stream,
-                           static_cast<float4 const*>(R.r.ptr), static_cast<int const*>(h.aggVertPtr.ptr),
-                           static_cast<int const*>(h.aggVerts.ptr), h.nAggregates, R.rc.ptr);
-        int const rowsPerBlock = 4 / coarseWavesPerRow;
-        hipLaunchKernelGGL(device::CoarseApply, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
-                           h.stream, static_cast<double const*>(R.coarse[0].ptr),
-                           static_cast<double const*>(R.coarse[1].ptr), static_cast<int const*>(R.coarseFinal.ptr),
-                           static_cast<double const*>(R.rc.ptr), R.zc.ptr, cargs, coarseWavesPerRow);
-        hipLaunchKernelGGL(device::CoarseProlong, dim3(gridN), dim3(256), 0, h.stream, R.z.ptr,
-                           static_cast<float4 const*>(R.r.ptr), n, cargs, static_cast<double const*>(R.zc.ptr),
-                           static_cast<int const*>(h.meshOf.ptr), nM, cg.A);
-    };
+    bool const fused = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI || cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL;
+    // fused chunks hold an even number of iterations: the two direction buffers swap
+    int const check = fused ? (std::max(1, cfg.check_every) + 1) / 2 * 2 : std::max(1, cfg.check_every);
+    AssembleBsr(h, xs, dt);
     auto zeroScalars = [&] {
         int const nd = static_cast<int>(ScalarDoubles(h));
         hipLaunchKernelGGL(device::ZeroScalars, dim3(std::min(1024, Blocks(nd + 2 * nM + 1))), dim3(256), 0,
                            h.stream, cg.P, nd, cg.F, 2 * nM + 1);
     };
-    // r = b - H y with per-mesh |b|^2, |r|^2 (and r . D^-1 r with dinv)
-    auto residual = [&](float4 const* pre, double* restrictTo) {
-        VertexPass<AdjMode::kHessVec>(h, xs, dt, y, nullptr, R.q.ptr, nullptr, 0, 0, n);
-        hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr,
-                           R.z.ptr, pre, h.meshOf.ptr, n, nM, cg.A, static_cast<int const*>(h.aggOf.ptr), restrictTo);
+    auto product = [&](float4* v, float4* out, double* acc) {
+        device::BsrArgs a{};
+        a.y = v, a.out = out, a.cg = cg, a.acc = acc, a.accArray = device::kAccPq;
+        if (acc)
+            BsrPass<BsrMode::kProductDot>(h, a, 0, n);
+        else
+            BsrPass<BsrMode::kProduct>(h, a, 0, n);
     };
     zeroScalars();
     Zero4(h, R.z.ptr, nV);
@@ -353,10 +533,13 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         {
             int const chunk = std::min(check, cfg.max_iterations - it);
             for (int k = 0; k < chunk; ++k)
-                GaussSeidelPass(h, xs, dt, y, b, false);
+                GaussSeidelPass(h, cg, y, b, false);
             it += chunk;
             zeroScalars();
-            residual(nullptr, nullptr);
+            product(y, R.q.ptr, nullptr);
+            hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr,
+                               R.z.ptr, static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, nM, cg.A,
+                               static_cast<int const*>(h.aggOf.ptr), static_cast<double*>(nullptr));
             hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
             if (ReadLive(h) == 0)
                 break;
@@ -366,51 +549,101 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         PBAT_HIP(hipGetLastError());
         return out;
     }
-    if (jacobi)
-        VertexPass<AdjMode::kDiagInverse>(h, xs, dt, y, nullptr, R.dinv.ptr, nullptr, 0, 0, n);
-    residual(dinv, rc);
-    if (twoLevel)
-        coarse();
-    if (!jacobi)
-    {
-        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
-        GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, cg.A);
-    }
-    hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
-    hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr, nV);
-    auto iteration = [&] {
-        VertexPass<AdjMode::kHessVecDot>(h, xs, dt, R.p.ptr, nullptr, R.q.ptr, cg.A, device::kAccPq, 0, n);
-        hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
-                           static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr), dinv,
-                           h.meshOf.ptr, n, cg, static_cast<int const*>(h.aggOf.ptr), rc);
-        if (twoLevel)
-            coarse();
-        if (!jacobi)
-        {
-            GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, false);
-            GaussSeidelPass(h, xs, dt, R.z.ptr, R.r.ptr, true, cg.A);
-        }
-        hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 0);
-        hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr,
-                           static_cast<float4 const*>(R.z.ptr), h.meshOf.ptr, n, cg);
+    device::CoarseArgs const cargs = MakeCoarseArgs(h);
+    // two-level: rows of the coarse inverse are read by 1, 2 or 4 waves, by the
+    // largest mesh's row length
+    int maxCoarse = 0;
+    for (int m = 0; m < nM && twoLevel; ++m)
+        maxCoarse = std::max(maxCoarse, 3 * (h.aggBegin[m + 1] - h.aggBegin[m]));
+    int const coarseWavesPerRow = maxCoarse > 512 ? 4 : maxCoarse > 256 ? 2 : 1;
+    auto aggUpdate = [&](int init, float4 const* p) {
+        device::AggUpdateArgs u{};
+        u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = h.aggVerts.ptr, u.aggMesh = h.aggMesh.ptr;
+        u.y = y, u.r = R.r.ptr, u.z = R.z.ptr, u.p = p, u.q = R.q.ptr, u.b = b, u.dinv = R.dinv.ptr;
+        u.rc = twoLevel ? R.rc.ptr : nullptr;
+        u.cg = cg, u.counter = R.counters.ptr, u.nAgg = h.nAggregates, u.init = init, u.finish = twoLevel ? 0 : 1;
+        if (h.nAggregates > 0)
+            hipLaunchKernelGGL(device::AggUpdate, dim3(Blocks(int64_t(h.nAggregates) * 64)), dim3(256), 0, h.stream,
+                               u);
     };
+    auto coarseFinish = [&](int first) {
+        int const rows = 3 * h.nAggregates, rowsPerBlock = 4 / coarseWavesPerRow;
+        hipLaunchKernelGGL(device::CoarseApplyFinish, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
+                           h.stream, static_cast<double const*>(R.coarse[0].ptr),
+                           static_cast<double const*>(R.coarse[1].ptr), static_cast<int const*>(R.coarseFinal.ptr),
+                           static_cast<double const*>(R.rc.ptr), R.zc.ptr, cargs, coarseWavesPerRow, cg,
+                           R.counters.ptr + 1, first);
+    };
+    std::function<void(int)> iteration;
+    if (fused)
+    {
+        // r = b - H y, z = D^-1 r, (rc, zc); p_old = 0 for the first direction
+        product(y, R.q.ptr, nullptr);
+        aggUpdate(1, nullptr);
+        if (twoLevel)
+            coarseFinish(1);
+        Zero4(h, R.p.ptr, nV);
+        iteration = [&](int k) {
+            float4* const pOld = (k & 1) ? R.p2.ptr : R.p.ptr;
+            float4* const pNew = (k & 1) ? R.p.ptr : R.p2.ptr;
+            device::BsrArgs a{};
+            a.y = pNew, a.out = R.q.ptr, a.z = R.z.ptr, a.pOld = pOld, a.zc = twoLevel ? R.zc.ptr : nullptr;
+            a.cg = cg, a.acc = cg.A, a.accArray = device::kAccPq;
+            BsrPass<BsrMode::kDirectionDot>(h, a, 0, n);
+            aggUpdate(0, pNew);
+            if (twoLevel)
+                coarseFinish(0);
+        };
+    }
+    else
+    {
+        product(y, R.q.ptr, nullptr);
+        hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr, R.z.ptr,
+                           static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, nM, cg.A,
+                           static_cast<int const*>(h.aggOf.ptr), static_cast<double*>(nullptr));
+        GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, false);
+        GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, true, cg.A);
+        hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
+        hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.p.ptr, R.z.ptr, nV);
+        iteration = [&](int) {
+            product(R.p.ptr, R.q.ptr, cg.A);
+            hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
+                               static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr),
+                               static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, cg,
+                               static_cast<int const*>(h.aggOf.ptr), static_cast<double*>(nullptr));
+            GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, false);
+            GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, true, cg.A);
+            hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 0);
+            hipLaunchKernelGGL(device::CgDirection, dim3(gridN), dim3(256), 0, h.stream, R.p.ptr,
+                               static_cast<float4 const*>(R.z.ptr), h.meshOf.ptr, n, cg);
+        };
+    }
     bool const graph = h.useGraph && xs == R.hold.ptr && y == R.y.ptr && b == R.rhs.ptr;
     if (graph)
     {
-        std::vector<double> const key{double(dt),          double(cfg.solver),    double(check),
-                                      cg.tol2,             double(AdjointLanes(h)), double(AdjointPerLane(h)),
-                                      double(h.xcdRemap),  double(h.useEll),      double(h.blockSize)};
-        if (!R.cgExec || R.cgKey != key)
+        std::vector<double> const key{double(dt),        double(cfg.solver), double(check),       cg.tol2,
+                                      double(BsrLanes(h)), double(BsrPerLane(h)), double(h.xcdRemap),
+                                      double(coarseWavesPerRow)};
+        // every device address the captured kernels take: a buffer reallocated at
+        // another address forces a new capture
+        std::vector<void const*> const ptrs{
+            R.y.ptr,           R.r.ptr,         R.z.ptr,       R.p.ptr,          R.p2.ptr,        R.q.ptr,
+            R.rhs.ptr,         R.hold.ptr,      R.dinv.ptr,    R.scal.ptr,       R.flags.ptr,     R.counters.ptr,
+            R.coarse[0].ptr,   R.coarse[1].ptr, R.coarseFinal.ptr, R.rc.ptr,     R.zc.ptr,        R.bsrRowPtr.ptr,
+            R.bsrCol.ptr,      R.bsrA.ptr,      R.bsrB.ptr,    R.bsrC.ptr,       h.meshOf.ptr,    h.aggOf.ptr,
+            h.aggVertPtr.ptr,  h.aggVerts.ptr,  h.aggMesh.ptr};
+        if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();
             PBAT_HIP(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
             for (int k = 0; k < check; ++k)
-                iteration();
+                iteration(k);
             hipGraph_t g = nullptr;
             PBAT_HIP(hipStreamEndCapture(h.stream, &g));
             R.cgGraph = g;
             PBAT_HIP(hipGraphInstantiate(&R.cgExec, g, nullptr, nullptr, 0));
             R.cgKey = key;
+            R.cgPtrs = ptrs;
         }
     }
     if (ReadLive(h) > 0)
@@ -423,7 +656,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
                 PBAT_HIP(hipGraphLaunch(R.cgExec, h.stream));
             else
                 for (int k = 0; k < chunk; ++k)
-                    iteration();
+                    iteration(it + k);
             it += chunk;
             if (ReadLive(h) == 0)
                 break;
@@ -432,7 +665,6 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     PBAT_HIP(hipGetLastError());
     return ReadResult(h);
 }
-
 void CheckConfig(pbat_vbd_adjoint_config const& cfg)
 {
     if (cfg.max_iterations < 1 || !(cfg.tolerance >= 0.f) ||
@@ -473,7 +705,7 @@ void RolloutForward(pbat_vbd_integrator& h, float dt, int iterations, int steps,
     NoContact(h);
     PBAT_HIP(hipSetDevice(h.device));
     RolloutState& R = h.rollout;
-    R.valid = false;
+    R.Discard();
     int const nCkpt = (steps + every - 1) / every;
     size_t const n = static_cast<size_t>(h.nV);
     R.ckptX.resize(static_cast<size_t>(nCkpt));
@@ -625,7 +857,7 @@ void SetLogYoung(pbat_vbd_integrator& h, float const* logY, int64_t nT)
         h.lameHost[2 * e + 1] *= scale;
     }
     h.logYCurrent = v;
-    R.valid = false;
+    R.Discard();
 }
 
 void HessianVectorProduct(pbat_vbd_integrator& h, float const* x, float dt, float const* p, float* out)
@@ -706,6 +938,17 @@ extern "C" int pbat_vbd_rollout_backward(pbat_vbd_integrator* h, const float* dL
                                          const pbat_vbd_adjoint_config* config, pbat_vbd_adjoint_stats* stats)
 {
     return Guard([&] { RolloutBackward(Handle(h), dL_dx, dL_dv, dL_dlogY, dL_dv0, dL_dx0, config, stats); });
+}
+
+extern "C" int pbat_vbd_rollout_generation(pbat_vbd_integrator* h, int64_t* generation, int32_t* valid)
+{
+    return Guard([&] {
+        auto& H = Handle(h);
+        if (generation)
+            *generation = H.rollout.generation;
+        if (valid)
+            *valid = H.rollout.valid ? 1 : 0;
+    });
 }
 
 extern "C" int pbat_vbd_hessian_vector(pbat_vbd_integrator* h, const float* x, float dt, const float* p, float* out)

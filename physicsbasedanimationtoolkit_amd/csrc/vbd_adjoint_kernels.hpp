@@ -106,7 +106,8 @@ enum AccArray : int
     kAccPq = 1,    // p . H p
     kAccRr = 2,    // r . r
     kAccBb = 3,    // b . b
-    kAccCount = 4
+    kAccRcZc = 4,  // (Z^T r) . zc: the coarse part of r . z (two-level, fused iteration)
+    kAccCount = 5
 };
 constexpr int kSpread = 16;
 constexpr int kSlotStride = 16;
@@ -170,7 +171,7 @@ __device__ __forceinline__ void FinishIteration(CgState const& c, int first)
         if (hl < kSpread)
         {
             rr = *AccSlot(c.A, kAccRr, nM, m, hl);
-            rz = *AccSlot(c.A, kAccRzNew, nM, m, hl);
+            rz = *AccSlot(c.A, kAccRzNew, nM, m, hl) + *AccSlot(c.A, kAccRcZc, nM, m, hl);
             bb = *AccSlot(c.A, kAccBb, nM, m, hl);
             for (int arr = 0; arr < kAccCount; ++arr)
                 *AccSlot(c.A, arr, nM, m, hl) = 0.;

@@ -1,0 +1,460 @@
+// Device kernels of the adjoint's assembled step Hessian and of its fused
+// conjugate-gradient iteration.
+//
+// The step Hessian H = M / h^2 + sum_e w_e d2Psi_e/dx2 is fixed during one
+// linear solve (it is evaluated at the step's minimiser), so it is assembled
+// once per solve into 3x3 blocks over the updated vertices (block-sparse rows,
+// columns ascending) and every product of the solve is a block-sparse
+// matrix-vector product: 40 bytes per block (36 B of values + the column)
+// instead of re-evaluating the Stable Neo-Hookean Hessian of every incident tet
+// per product. For tet e with shape-function gradient rows g_a, g_b of two of
+// its vertices, F its deformation gradient and J = det F, the block is
+//   w mu (g_a . g_b) I + w lambda (cof F g_a)(cof F g_b)^T
+//     - w lambda (J - alpha) [F (g_a x g_b)]_x
+// (d2Psi/dF2 = mu I + lambda cof F (x) cof F + lambda (J - alpha) d2J/dF2
+// contracted with g_a and g_b; d2J/dF2 contracts to the cross-product matrix).
+// Blocks are summed in FP64 in a fixed order and stored in FP32.
+#pragma once
+
+#include "vbd_adjoint_kernels.hpp"
+
+namespace pbat_vbd::device {
+
+// Values of the block-sparse step Hessian, structure of arrays per block:
+// a = (h00, h01, h02, h10), b = (h11, h12, h20, h21), c = h22.
+struct BsrValues
+{
+    float4* __restrict__ a;
+    float4* __restrict__ b;
+    float* __restrict__ c;
+};
+
+// Sparsity pattern and assembly sources: block k sits in row blockRow[k] at
+// column col[k]; its sources src[srcPtr[k] .. srcPtr[k+1]) are 16 e + 4 la + lb
+// (tet e, local vertex la of the row vertex, lb of the column vertex),
+// ascending in e.
+struct BsrPattern
+{
+    int const* __restrict__ rowPtr; // nRows + 1
+    int const* __restrict__ col;
+    int const* __restrict__ blockRow;
+    int const* __restrict__ srcPtr;
+    int const* __restrict__ src;
+    int nRows;
+    int nBlocks;
+};
+
+// One thread per block. Diagonal blocks also receive m_i / h^2 I and write the
+// FP64-inverted block to dinv[2 i], dinv[2 i + 1] as (00, 01, 02, 11), (12, 22).
+__global__ __launch_bounds__(256) void BsrAssemble(float4 const* __restrict__ xs, int4 const* __restrict__ tetv,
+                                                   float4 const* __restrict__ tgp, BsrPattern p, double invDt2,
+                                                   BsrValues v, float4* __restrict__ dinv)
+{
+    int const k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= p.nBlocks)
+        return;
+    int const i = p.blockRow[k], j = p.col[k];
+    double H[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    for (int s = p.srcPtr[k]; s < p.srcPtr[k + 1]; ++s)
+    {
+        int const code = p.src[s];
+        int const e = code >> 4, la = (code >> 2) & 3, lb = code & 3;
+        int4 const t = tetv[e];
+        float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
+        float4 const x0 = xs[t.x], x1 = xs[t.y], x2 = xs[t.z], x3 = xs[t.w];
+        double g[4][3] = {{0., 0., 0.}, {q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+        for (int c = 0; c < 3; ++c)
+            g[0][c] = -(g[1][c] + g[2][c] + g[3][c]);
+        double const d[3][3] = {{double(x1.x) - x0.x, double(x1.y) - x0.y, double(x1.z) - x0.z},
+                                {double(x2.x) - x0.x, double(x2.y) - x0.y, double(x2.z) - x0.z},
+                                {double(x3.x) - x0.x, double(x3.y) - x0.y, double(x3.z) - x0.z}};
+        double F[3][3]; // F[row][col] = sum_k d[k][row] g[k + 1][col]
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                F[r][c] = d[0][r] * g[1][c] + d[1][r] * g[2][c] + d[2][r] * g[3][c];
+        double cof[3][3]; // column c = f_{c+1} x f_{c+2}
+        for (int c = 0; c < 3; ++c)
+        {
+            int const c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            cof[0][c] = F[1][c1] * F[2][c2] - F[2][c1] * F[1][c2];
+            cof[1][c] = F[2][c1] * F[0][c2] - F[0][c1] * F[2][c2];
+            cof[2][c] = F[0][c1] * F[1][c2] - F[1][c1] * F[0][c2];
+        }
+        double const J = F[0][0] * cof[0][0] + F[1][0] * cof[1][0] + F[2][0] * cof[2][0];
+        double const* ga = g[la];
+        double const* gb = g[lb];
+        double ca[3], cb[3], w[3];
+        double const gx[3] = {ga[1] * gb[2] - ga[2] * gb[1], ga[2] * gb[0] - ga[0] * gb[2],
+                              ga[0] * gb[1] - ga[1] * gb[0]};
+        for (int r = 0; r < 3; ++r)
+        {
+            ca[r] = cof[r][0] * ga[0] + cof[r][1] * ga[1] + cof[r][2] * ga[2];
+            cb[r] = cof[r][0] * gb[0] + cof[r][1] * gb[1] + cof[r][2] * gb[2];
+            w[r] = F[r][0] * gx[0] + F[r][1] * gx[1] + F[r][2] * gx[2];
+        }
+        double const wmu = q2.y, wlam = q2.z, sk = double(q2.z) * (J - double(q2.w));
+        double const dg = wmu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+        // - sk [w]_x with [w]_x = [[0, -w2, w1], [w2, 0, -w0], [-w1, w0, 0]]
+        double const S[3][3] = {{0., sk * w[2], -sk * w[1]}, {-sk * w[2], 0., sk * w[0]}, {sk * w[1], -sk * w[0], 0.}};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                H[r][c] += (r == c ? dg : 0.) + wlam * ca[r] * cb[c] + S[r][c];
+    }
+    if (i == j)
+    {
+        double const m = double(xs[i].w) * invDt2;
+        H[0][0] += m, H[1][1] += m, H[2][2] += m;
+        Inverse3 const C(H[0][0], 0.5 * (H[0][1] + H[1][0]), 0.5 * (H[0][2] + H[2][0]), H[1][1],
+                         0.5 * (H[1][2] + H[2][1]), H[2][2]);
+        dinv[2 * i] = make_float4(static_cast<float>(C.c00 * C.inv), static_cast<float>(C.c01 * C.inv),
+                                  static_cast<float>(C.c02 * C.inv), static_cast<float>(C.c11 * C.inv));
+        dinv[2 * i + 1] = make_float4(static_cast<float>(C.c12 * C.inv), static_cast<float>(C.c22 * C.inv), 0.f, 0.f);
+    }
+    v.a[k] = make_float4(static_cast<float>(H[0][0]), static_cast<float>(H[0][1]), static_cast<float>(H[0][2]),
+                         static_cast<float>(H[1][0]));
+    v.b[k] = make_float4(static_cast<float>(H[1][1]), static_cast<float>(H[1][2]), static_cast<float>(H[2][0]),
+                         static_cast<float>(H[2][1]));
+    v.c[k] = static_cast<float>(H[2][2]);
+}
+
+// Returns true in every thread of the workgroup that finishes last among the
+// launch's workgroups (each one calls this once, after its own atomics and
+// stores). The counter returns to 0 for the next launch. Release before the
+// ticket, acquire after it: the last workgroup sees every other one's writes.
+__device__ __forceinline__ bool LastWorkgroup(unsigned int* counter)
+{
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __threadfence();
+        unsigned int const ticket = atomicAdd(counter, 1u);
+        last = (ticket == gridDim.x - 1) ? 1 : 0;
+        if (last)
+            atomicExch(counter, 0u);
+    }
+    __syncthreads();
+    if (last)
+        __threadfence();
+    return last != 0;
+}
+
+enum class BsrMode : int
+{
+    kProduct = 0,        // out = H y
+    kProductDot = 1,     // out = H y, acc[mesh] += y . H y
+    kGaussSeidel = 2,    // rows of one colour: y_i += D_i^-1 (b_i - (H y)_i)
+    kGaussSeidelDot = 3, // same, then acc[mesh] += b_i . y_i
+    kDirectionDot = 4    // p = z + Z zc + beta p_old formed on the fly (written to y);
+                         // out = H p, acc[mesh] += p . H p
+};
+
+struct BsrArgs
+{
+    int const* __restrict__ rowPtr;
+    int const* __restrict__ col;
+    float4 const* __restrict__ va;
+    float4 const* __restrict__ vb;
+    float const* __restrict__ vc;
+    float4* __restrict__ y;          // vector multiplied (Gauss-Seidel: updated in place)
+    float4 const* __restrict__ b;    // right-hand side (Gauss-Seidel)
+    float4* __restrict__ out;        // product
+    float4 const* __restrict__ dinv; // inverse diagonal blocks (Gauss-Seidel)
+    int const* __restrict__ meshOf;
+    // kDirectionDot: p_j = z_j + zc[aggOf j] + beta p_old_j of mesh m (p_old_j when m is done)
+    float4 const* __restrict__ z;
+    float4 const* __restrict__ pOld;
+    double const* __restrict__ zc; // nullable (block Jacobi): no coarse term
+    int const* __restrict__ aggOf;
+    CgState cg;
+    double* __restrict__ acc; // accumulators (dot modes), array accArray
+    int accArray;
+    int vbeg, vend;
+    int xcdChunk;
+};
+
+template <BsrMode M>
+__device__ __forceinline__ float4 BsrVector(BsrArgs const& a, int j, float beta, bool frozen)
+{
+    if constexpr (M == BsrMode::kDirectionDot)
+    {
+        float4 const po = a.pOld[j];
+        if (frozen)
+            return po;
+        float4 const zj = a.z[j];
+        float4 r = make_float4(zj.x + beta * po.x, zj.y + beta * po.y, zj.z + beta * po.z, 0.f);
+        if (a.zc)
+        {
+            int const g = a.aggOf[j];
+            r.x += static_cast<float>(a.zc[3 * g]);
+            r.y += static_cast<float>(a.zc[3 * g + 1]);
+            r.z += static_cast<float>(a.zc[3 * g + 2]);
+        }
+        return r;
+    }
+    else
+    {
+        return a.y[j];
+    }
+}
+
+// L lanes per row of [vbeg, vend); lane l takes blocks l, l + L, ... of the row,
+// the first K of them loaded together before accumulating.
+template <int L, int K, BsrMode M>
+__global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
+{
+    constexpr bool kGS = M == BsrMode::kGaussSeidel || M == BsrMode::kGaussSeidelDot;
+    constexpr bool kDot = M == BsrMode::kGaussSeidelDot || M == BsrMode::kDirectionDot || M == BsrMode::kProductDot;
+    constexpr bool kDir = M == BsrMode::kDirectionDot;
+    if constexpr (kDir)
+    {
+        if (a.cg.F[2 * a.cg.nM] == 0)
+            return; // every mesh has converged: nothing left to do
+    }
+    int block = static_cast<int>(blockIdx.x);
+    if (a.xcdChunk > 0)
+        block = (block & 7) * a.xcdChunk + (block >> 3);
+    int const tid = block * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    int const lane = tid & (L - 1);
+    int const row = a.vbeg + tid / L;
+    bool const active = row < a.vend;
+    int kb = 0, ke = 0, m = 0;
+    float beta = 0.f;
+    bool frozen = false;
+    if (active)
+    {
+        kb = a.rowPtr[row];
+        ke = a.rowPtr[row + 1];
+        if constexpr (kDir)
+        {
+            m = a.meshOf[row];
+            frozen = a.cg.F[kDone * a.cg.nM + m] != 0;
+            beta = static_cast<float>(a.cg.P[kBeta * a.cg.nM + m]);
+        }
+    }
+    float hx = 0.f, hy = 0.f, hz = 0.f;
+    int cj[K];
+    float4 A[K], B[K];
+    float C[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+    {
+        int const k = kb + lane + u * L;
+        cj[u] = -1;
+        if (k < ke)
+        {
+            cj[u] = a.col[k];
+            A[u] = a.va[k];
+            B[u] = a.vb[k];
+            C[u] = a.vc[k];
+        }
+    }
+    float4 P[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+        if (cj[u] >= 0)
+            P[u] = BsrVector<M>(a, cj[u], beta, frozen);
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+        if (cj[u] >= 0)
+        {
+            hx += A[u].x * P[u].x + A[u].y * P[u].y + A[u].z * P[u].z;
+            hy += A[u].w * P[u].x + B[u].x * P[u].y + B[u].y * P[u].z;
+            hz += B[u].z * P[u].x + B[u].w * P[u].y + C[u] * P[u].z;
+        }
+    for (int k = kb + lane + K * L; k < ke; k += L)
+    {
+        float4 const Ak = a.va[k], Bk = a.vb[k];
+        float const Ck = a.vc[k];
+        float4 const Pk = BsrVector<M>(a, a.col[k], beta, frozen);
+        hx += Ak.x * Pk.x + Ak.y * Pk.y + Ak.z * Pk.z;
+        hy += Ak.w * Pk.x + Bk.x * Pk.y + Bk.y * Pk.z;
+        hz += Bk.z * Pk.x + Bk.w * Pk.y + Ck * Pk.z;
+    }
+    hx = GroupSum<L>(hx);
+    hy = GroupSum<L>(hy);
+    hz = GroupSum<L>(hz);
+    bool const leader = active && lane == 0;
+    double dot = 0.;
+    if (leader)
+    {
+        if constexpr (kGS)
+        {
+            float4 const bo = a.b[row], yo = a.y[row];
+            float const rx = bo.x - hx, ry = bo.y - hy, rz = bo.z - hz;
+            float4 const yn = make_float4(yo.x, yo.y, yo.z, 0.f);
+            float4 const D0 = a.dinv[2 * row], D1 = a.dinv[2 * row + 1];
+            float4 const r = make_float4(yn.x + D0.x * rx + D0.y * ry + D0.z * rz,
+                                         yn.y + D0.y * rx + D0.w * ry + D1.x * rz,
+                                         yn.z + D0.z * rx + D1.x * ry + D1.y * rz, 0.f);
+            a.y[row] = r;
+            dot = double(bo.x) * r.x + double(bo.y) * r.y + double(bo.z) * r.z;
+        }
+        else
+        {
+            a.out[row] = make_float4(hx, hy, hz, 0.f);
+            if constexpr (M == BsrMode::kProductDot)
+            {
+                float4 const yr = a.y[row];
+                dot = double(yr.x) * hx + double(yr.y) * hy + double(yr.z) * hz;
+            }
+            if constexpr (kDir)
+            {
+                float4 const pr = BsrVector<M>(a, row, beta, frozen);
+                a.y[row] = pr; // the direction of this iteration (the p_old of the next)
+                dot = double(pr.x) * hx + double(pr.y) * hy + double(pr.z) * hz;
+            }
+        }
+    }
+    if constexpr (kDot)
+        BlockAccumulate(dot, leader ? a.meshOf[row] : -1, a.acc, a.accArray, a.cg.nM);
+}
+
+// Per mesh m with alpha = rz / pq (0 once converged), for the vertices of each
+// aggregate (one wave per aggregate, lanes over its vertex list):
+//   init:   r = b - q               (q = H y0)
+//   update: y += alpha p, r -= alpha q
+// then z = D^-1 r, the aggregate's restriction rc = sum r (FP64, when rc) and
+// the mesh's r . z and r . r (and b . b at init) accumulated. Every updated
+// vertex belongs to exactly one aggregate and every aggregate to one mesh.
+// With finish (block Jacobi: no coarse step follows), the last workgroup ends
+// the iteration (FinishIteration).
+struct AggUpdateArgs
+{
+    int const* __restrict__ aggPtr;
+    int const* __restrict__ aggVerts;
+    int const* __restrict__ aggMesh;
+    float4* __restrict__ y;
+    float4* __restrict__ r;
+    float4* __restrict__ z;
+    float4 const* __restrict__ p;
+    float4 const* __restrict__ q;
+    float4 const* __restrict__ b; // init only
+    float4 const* __restrict__ dinv;
+    double* __restrict__ rc;      // nullable
+    CgState cg;
+    unsigned int* counter;        // finish only
+    int nAgg;
+    int init;
+    int finish;
+};
+
+__global__ __launch_bounds__(256) void AggUpdate(AggUpdateArgs u)
+{
+    int const nM = u.cg.nM;
+    if (!u.init && u.cg.F[2 * nM] == 0)
+        return; // every mesh has converged
+    int const g = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    int const lane = static_cast<int>(threadIdx.x & 63);
+    double sx = 0., sy = 0., sz = 0., rz = 0., rr = 0., bb = 0.;
+    int m = -1;
+    if (g < u.nAgg)
+    {
+        m = u.aggMesh[g];
+        float alpha = 0.f;
+        if (!u.init)
+        {
+            double const pq = AccSum(u.cg.A, kAccPq, nM, m);
+            bool const done = u.cg.F[kDone * nM + m] != 0;
+            alpha = (!done && pq != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq) : 0.f;
+        }
+        for (int k = u.aggPtr[g] + lane; k < u.aggPtr[g + 1]; k += 64)
+        {
+            int const i = u.aggVerts[k];
+            float4 const qi = u.q[i];
+            float4 ri;
+            if (u.init)
+            {
+                float4 const bi = u.b[i];
+                ri = make_float4(bi.x - qi.x, bi.y - qi.y, bi.z - qi.z, 0.f);
+                bb += Dot3(bi, bi);
+            }
+            else
+            {
+                float4 const pi = u.p[i];
+                float4 yi = u.y[i];
+                ri = u.r[i];
+                yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
+                ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
+                u.y[i] = yi;
+            }
+            u.r[i] = ri;
+            float4 const zi = ApplyInverse(u.dinv, i, ri);
+            u.z[i] = zi;
+            sx += ri.x, sy += ri.y, sz += ri.z;
+            rz += Dot3(ri, zi);
+            rr += Dot3(ri, ri);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        sx += __shfl_xor(sx, off);
+        sy += __shfl_xor(sy, off);
+        sz += __shfl_xor(sz, off);
+        rz += __shfl_xor(rz, off);
+        rr += __shfl_xor(rr, off);
+        bb += __shfl_xor(bb, off);
+    }
+    if (lane == 0 && m >= 0)
+    {
+        if (u.rc)
+        {
+            u.rc[3 * g] = sx;
+            u.rc[3 * g + 1] = sy;
+            u.rc[3 * g + 2] = sz;
+        }
+        int const slot = g & (kSpread - 1);
+        atomicAdd(AccSlot(u.cg.A, kAccRzNew, nM, m, slot), rz);
+        atomicAdd(AccSlot(u.cg.A, kAccRr, nM, m, slot), rr);
+        if (u.init)
+            atomicAdd(AccSlot(u.cg.A, kAccBb, nM, m, slot), bb);
+    }
+    if (u.finish && LastWorkgroup(u.counter))
+        FinishIteration(u.cg, u.init);
+}
+
+// zc = (Z^T H Z)^-1 rc per mesh (wpr waves per coarse row, as CoarseApply), the
+// coarse part rc . zc of each mesh's r . z, then the last workgroup ends the
+// iteration (FinishIteration: r . z = (r . D^-1 r) + rc . zc, stopping test, beta).
+__global__ __launch_bounds__(256) void CoarseApplyFinish(double const* __restrict__ buf0,
+                                                         double const* __restrict__ buf1,
+                                                         int const* __restrict__ finalBuf,
+                                                         double const* __restrict__ rc, double* __restrict__ zc,
+                                                         CoarseArgs c, int wpr, CgState cg,
+                                                         unsigned int* __restrict__ counter, int first)
+{
+    if (!first && cg.F[2 * cg.nM] == 0)
+        return; // every mesh has converged
+    __shared__ double part[4];
+    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+    int const row = static_cast<int>(blockIdx.x) * (4 / wpr) + wave / wpr;
+    int const sub = wave % wpr;
+    double sum = 0.;
+    int m = -1;
+    if (row < 3 * c.nAgg)
+    {
+        m = c.aggMesh[row / 3];
+        int const b0 = 3 * c.meshAggBegin[m];
+        int const n = 3 * c.meshAggBegin[m + 1] - b0;
+        double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
+        for (int j = sub * 64 + lane; j < n; j += 64 * wpr)
+            sum += A[j] * rc[b0 + j];
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        sum += __shfl_xor(sum, off);
+    if (lane == 0)
+        part[wave] = sum;
+    __syncthreads();
+    if (lane == 0 && sub == 0 && row < 3 * c.nAgg)
+    {
+        double t = part[wave];
+        for (int k = 1; k < wpr; ++k)
+            t += part[wave + k];
+        zc[row] = t;
+        atomicAdd(AccSlot(cg.A, kAccRcZc, cg.nM, m, row & (kSpread - 1)), t * rc[row]);
+    }
+    if (LastWorkgroup(counter))
+        FinishIteration(cg, first);
+}
+
+} // namespace pbat_vbd::device

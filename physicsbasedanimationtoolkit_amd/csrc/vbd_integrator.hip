@@ -146,6 +146,7 @@ device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2)
     a.c.dt2 = sdt2;
     a.c.kD = h.kD;
     a.c.detHZero = h.detHZero;
+    a.c.fp64Solves = h.fp64Solves.ptr;
     a.ntStream = h.ntStream ? 1 : 0;
     if (h.useEll && h.ellStride > 0)
         a.ellStride = h.ellStride;
@@ -409,7 +410,7 @@ int EnqueueStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         }
         total += launched;
         hipLaunchKernelGGL(device::SubstepEnd, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xt.ptr, h.vel.ptr,
-                           h.vprev.ptr, n, sdt);
+                           h.vprev.ptr, n, sdt, h.statusDev);
     }
     if (h.contact)
         ContactNearestPass(h, 1);
@@ -429,6 +430,16 @@ void InvalidateGraph(pbat_vbd_integrator& h)
     h.graphSweeps = 0;
 }
 
+void ClearStatus(pbat_vbd_integrator& h) { __atomic_store_n(h.statusHost, 0, __ATOMIC_RELEASE); }
+
+void CheckStatus(pbat_vbd_integrator& h)
+{
+    if (__atomic_load_n(h.statusHost, __ATOMIC_ACQUIRE) != 0)
+        Fail(PBAT_VBD_ERROR_NONFINITE,
+             "non-finite positions or velocities after the step (a diverged solve or a non-finite input state); "
+             "the state is left as computed");
+}
+
 void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
 {
     if (!(dt > 0.f) || !std::isfinite(dt))
@@ -438,6 +449,7 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     PBAT_HIP(hipSetDevice(h.device));
     if (LinearTrustRegion(h))
         EnsureTrustRegionBuffers(h, iterations, substeps);
+    ClearStatus(h);
     if (!h.useGraph)
     {
         InvalidateGraph(h);
@@ -445,6 +457,7 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
         if (h.timing)
             CollectTiming(h);
         PBAT_HIP(hipStreamSynchronize(h.stream));
+        CheckStatus(h);
         return;
     }
     GraphKey key;
@@ -490,6 +503,7 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     PBAT_HIP(hipStreamSynchronize(h.stream));
     if (h.timing)
         CollectTiming(h);
+    CheckStatus(h);
 }
 
 void CheckVertexArray(pbat_vbd_integrator const& h, void const* p, int64_t n, char const* what)
@@ -915,6 +929,20 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         h->trPartials.allocate(3 * static_cast<size_t>(kObjectiveBlocks) + 3);
     }
     h->energy.allocate(3);
+    {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess || !p)
+            Fail(PBAT_VBD_ERROR_OUT_OF_MEMORY, "hipHostMalloc of the step status failed");
+        h->statusHost = static_cast<int*>(p);
+        std::memset(p, 0, 64);
+        void* dp = nullptr;
+        PBAT_HIP(hipHostGetDevicePointer(&dp, p, 0));
+        h->statusDev = static_cast<int*>(dp);
+        h->fp64Solves.allocate(1);
+        unsigned int const zero = 0;
+        PBAT_HIP(hipMemcpyAsync(h->fp64Solves.ptr, &zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+        PBAT_HIP(hipStreamSynchronize(h->stream));
+    }
     PBAT_HIP(hipMemsetAsync(h->xtil.ptr, 0, sizeof(float4) * nV, h->stream));
     PBAT_HIP(hipMemsetAsync(h->vprev.ptr, 0, sizeof(float4) * nV, h->stream));
 
@@ -937,6 +965,183 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
     *out = h.release();
 }
 
+// MatrixMarket dense array file (column-major values), `digits` significant digits.
+void WriteDense(std::string const& path, int64_t rows, int64_t cols, std::vector<double> const& colMajor,
+                int digits = 9);
+
+// Buffers of the FP64 objective (allocated on first use). The vertex -> (tet,
+// local index) CSR over every vertex, Dirichlet ones included, comes from the
+// device's element table, in ascending tet order per vertex.
+void EnsureObjectiveBuffers(pbat_vbd_integrator& h)
+{
+    size_t const nV = static_cast<size_t>(h.nV);
+    if (h.objVtp.n != nV + 1)
+    {
+        std::vector<int4> TV(static_cast<size_t>(h.nT));
+        PBAT_HIP(hipMemcpy(TV.data(), h.tetv.ptr, TV.size() * sizeof(int4), hipMemcpyDeviceToHost));
+        std::vector<int> ptr(nV + 1, 0), inc(4 * TV.size());
+        for (int4 const& t : TV)
+            for (int id : {t.x, t.y, t.z, t.w})
+                ++ptr[id + 1];
+        for (size_t v = 0; v < nV; ++v)
+            ptr[v + 1] += ptr[v];
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (size_t e = 0; e < TV.size(); ++e)
+        {
+            int const ids[4] = {TV[e].x, TV[e].y, TV[e].z, TV[e].w};
+            for (int l = 0; l < 4; ++l)
+                inc[fill[ids[l]]++] = static_cast<int>(4 * e + l);
+        }
+        h.objVtp.upload(ptr, h.stream);
+        h.objInc.upload(inc, h.stream);
+        PBAT_HIP(hipStreamSynchronize(h.stream));
+    }
+    if (h.objX.n != nV)
+    {
+        h.objX.allocate(nV), h.objXtil.allocate(nV);
+        h.objGrad.allocate(3 * nV);
+    }
+    size_t const nb = static_cast<size_t>(std::max<int64_t>((h.nV + 255) / 256, kObjectiveBlocks));
+    if (h.objPart.n != 6 * nb + 6)
+        h.objPart.allocate(6 * nb + 6);
+}
+
+// sim::vbd::Integrator::ObjectiveFunction and ObjectiveFunctionGradient
+// (sim/vbd/Integrator.cpp:138-200) at device positions x and inertial target xtil
+// (float4, internal order), in FP64: f = 1/2 sum m |x - x~|^2 + dt^2 sum_e w_e Psi_e,
+// its gradient (3 x nV, original vertex order, nullable) and the squared gradient
+// norm over the non-Dirichlet vertices (nullable).
+void ObjectiveOnDevice(pbat_vbd_integrator& h, float4 const* x, float4 const* xtil, float dt, double* f,
+                       double* gradOrig, double* r2)
+{
+    EnsureObjectiveBuffers(h);
+    int const nV = static_cast<int>(h.nV), nT = static_cast<int>(h.nT);
+    int const gridG = (nV + 255) / 256;
+    size_t const nb = static_cast<size_t>(std::max(gridG, kObjectiveBlocks));
+    double* partF = h.objPart.ptr;
+    double* partG = h.objPart.ptr + 3 * nb;
+    double* sums = h.objPart.ptr + 6 * nb; // (K, U, .) then (|g|^2, ., .)
+    device::TrEpilogue none{};
+    hipLaunchKernelGGL(device::ObjectiveTerms, dim3(kObjectiveBlocks), dim3(256), 0, h.stream, x, xtil,
+                       static_cast<float4 const*>(nullptr), static_cast<int4 const*>(h.tetv.ptr),
+                       static_cast<float4 const*>(h.tgp.ptr), nV, nT, partF, static_cast<double const*>(nullptr));
+    hipLaunchKernelGGL(device::SumPartials3, dim3(1), dim3(256), 0, h.stream, static_cast<double const*>(partF),
+                       kObjectiveBlocks, sums, static_cast<double const*>(nullptr), none);
+    double const dt2 = double(dt) * double(dt);
+    hipLaunchKernelGGL(device::ObjectiveGradient, dim3(std::max(gridG, 1)), dim3(256), 0, h.stream, x, xtil,
+                       static_cast<int const*>(h.objVtp.ptr), static_cast<int const*>(h.objInc.ptr),
+                       static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr), nV,
+                       static_cast<int>(h.nUpdated), dt2, h.objGrad.ptr, partG);
+    hipLaunchKernelGGL(device::SumPartials3, dim3(1), dim3(256), 0, h.stream, static_cast<double const*>(partG),
+                       std::max(gridG, 1), sums + 3, static_cast<double const*>(nullptr), none);
+    PBAT_HIP(hipGetLastError());
+    double s[6];
+    PBAT_HIP(hipMemcpyAsync(s, sums, sizeof(s), hipMemcpyDeviceToHost, h.stream));
+    std::vector<double> g;
+    if (gradOrig)
+    {
+        g.resize(3 * static_cast<size_t>(nV));
+        PBAT_HIP(hipMemcpyAsync(g.data(), h.objGrad.ptr, g.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                h.stream));
+    }
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    if (f)
+        *f = 0.5 * s[0] + dt2 * s[1];
+    if (r2)
+        *r2 = s[3];
+    if (gradOrig)
+        for (int v = 0; v < nV; ++v)
+            for (int d = 0; d < 3; ++d)
+                gradOrig[3 * static_cast<size_t>(h.perm[v]) + d] = g[3 * static_cast<size_t>(v) + d];
+}
+
+void Objective(pbat_vbd_integrator& h, float const* x, float const* xtilde, float dt, double* f, double* grad,
+               double* r2)
+{
+    if (!(dt > 0.f) || !std::isfinite(dt))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "dt must be positive and finite");
+    PBAT_HIP(hipSetDevice(h.device));
+    EnsureObjectiveBuffers(h);
+    float4 const* xd = h.xm.ptr;
+    float4 const* xl = h.xtil.ptr;
+    if (x)
+    {
+        UploadVec(h, h.objX.ptr, x, h.massOrig.data());
+        xd = h.objX.ptr;
+    }
+    if (xtilde)
+    {
+        UploadVec(h, h.objXtil.ptr, xtilde, nullptr);
+        xl = h.objXtil.ptr;
+    }
+    ObjectiveOnDevice(h, xd, xl, dt, f, grad, r2);
+}
+
+// sim::vbd::Integrator::TraceNextStep followed by Step (sim/vbd/Integrator.cpp:22-52,
+// 73-87, 202-235): one step whose iterates are traced. Before every VBD iteration
+// and after the last, the objective, its gradient and the positions at that iterate
+// are recorded (FP64 reductions on the device); per substep s they go to
+// {dir}/{t}.{s}.f.mtx ((iterations + 1) x 1), {t}.{s}.grad.mtx and {t}.{s}.x.mtx
+// (3 nV x (iterations + 1), each column vertex-major as the reference's
+// MatrixX::reshaped()). Accelerated integrators trace between their sweeps as the
+// reference's do (RunVbdIteration records before each sweep); the linear
+// trust-region path traces plain VBD iterations. Runs eagerly, no graph.
+void TraceObjectiveStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, int t, char const* dir)
+{
+    if (!(dt > 0.f) || !std::isfinite(dt) || iterations < 0 || substeps < 1)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid step arguments");
+    PBAT_HIP(hipSetDevice(h.device));
+    EnsureObjectiveBuffers(h);
+    ClearStatus(h);
+    float const sdt = dt / static_cast<float>(substeps);
+    float const sdt2 = sdt * sdt;
+    int const n = static_cast<int>(h.nV);
+    int const eb = 256, eg = (n + eb - 1) / eb;
+    size_t const nd = 3 * static_cast<size_t>(n);
+    device::SweepArgs a = MakeSweepArgs(h, sdt, sdt2);
+    if (h.contact)
+        ContactBeginStep(h, dt);
+    std::string const d = (dir && *dir) ? dir : ".";
+    for (int s = 0; s < substeps; ++s)
+    {
+        hipLaunchKernelGGL(device::SubstepBegin, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xtil.ptr, h.xt.ptr,
+                           h.vel.ptr, h.vprev.ptr, h.aext.ptr, n, sdt, sdt2, h.strategy);
+        if (h.contact && s % h.activeSetFrequency == 0)
+            ContactNearestPass(h, 0);
+        std::vector<double> F, G, X;
+        std::vector<float> xbuf(nd);
+        auto record = [&] {
+            double fk = 0.;
+            G.resize(G.size() + nd);
+            ObjectiveOnDevice(h, h.xm.ptr, h.xtil.ptr, sdt, &fk, G.data() + G.size() - nd, nullptr);
+            F.push_back(fk);
+            DownloadVec(h, h.xm.ptr, xbuf.data());
+            X.insert(X.end(), xbuf.begin(), xbuf.end());
+        };
+        float omega = 0.f;
+        BeginAcceleration(h);
+        for (int k = 0; k < iterations; ++k)
+        {
+            record();
+            SweepAllColours(h, a);
+            LaunchAcceleration(h, k, omega);
+        }
+        record();
+        hipLaunchKernelGGL(device::SubstepEnd, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xt.ptr, h.vel.ptr,
+                           h.vprev.ptr, n, sdt, h.statusDev);
+        std::string const stem = d + "/" + std::to_string(t) + "." + std::to_string(s) + ".";
+        int64_t const cols = static_cast<int64_t>(F.size());
+        WriteDense(stem + "f.mtx", cols, 1, F, 17);
+        WriteDense(stem + "grad.mtx", static_cast<int64_t>(nd), cols, G, 17);
+        WriteDense(stem + "x.mtx", static_cast<int64_t>(nd), cols, X, 17);
+    }
+    if (h.contact)
+        ContactNearestPass(h, 1);
+    PBAT_HIP(hipGetLastError());
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    CheckStatus(h);
+}
+
 void Energy(pbat_vbd_integrator& h, double out[3])
 {
     PBAT_HIP(hipSetDevice(h.device));
@@ -951,13 +1156,14 @@ void Energy(pbat_vbd_integrator& h, double out[3])
 }
 
 // ---- MatrixMarket dense export (gpu/impl/vbd/Integrator.cu:105-148, 284-301) ----
-void WriteDense(std::string const& path, int64_t rows, int64_t cols, std::vector<double> const& colMajor)
+void WriteDense(std::string const& path, int64_t rows, int64_t cols, std::vector<double> const& colMajor,
+                int digits)
 {
     std::ofstream f(path);
     if (!f)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "cannot open " + path + " for writing");
     f << "%%MatrixMarket matrix array real general\n" << rows << " " << cols << "\n";
-    f << std::setprecision(9);
+    f << std::setprecision(digits);
     for (double v : colMajor)
         f << v << "\n";
 }
@@ -985,6 +1191,7 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
     if (!(dt > 0.f) || iterations < 0 || substeps < 1)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid step arguments");
     PBAT_HIP(hipSetDevice(h.device));
+    ClearStatus(h);
     auto name = [](char const* var, int tt, int s, int k) {
         std::ostringstream ss;
         ss << var << ".t." << tt << ".s." << s;
@@ -1052,12 +1259,13 @@ void TracedStep(pbat_vbd_integrator& h, float dt, int iterations, int substeps, 
         }
         WriteVertexMatrix(h, h.xm.ptr, Join(dir, name("x", t, s, iterations)));
         hipLaunchKernelGGL(device::SubstepEnd, dim3(eg), dim3(eb), 0, h.stream, h.xm.ptr, h.xt.ptr, h.vel.ptr,
-                           h.vprev.ptr, n, sdt);
+                           h.vprev.ptr, n, sdt, h.statusDev);
     }
     if (h.contact)
         ContactNearestPass(h, 1);
     PBAT_HIP(hipGetLastError());
     PBAT_HIP(hipStreamSynchronize(h.stream));
+    CheckStatus(h);
 }
 
 // Time full sweeps (all colours) for each candidate sweep shape and keep the
@@ -1341,7 +1549,7 @@ extern "C" int pbat_vbd_integrator_set_positions(pbat_vbd_integrator* h, const f
         auto& H = Handle(h);
         CheckVertexArray(H, x, n, "positions");
         PBAT_HIP(hipSetDevice(H.device));
-        H.rollout.valid = false;
+        H.rollout.Discard();
         UploadVec(H, H.xm.ptr, x, H.massOrig.data());
         UploadVec(H, H.xt.ptr, x, nullptr);
     });
@@ -1353,7 +1561,7 @@ extern "C" int pbat_vbd_integrator_set_velocities(pbat_vbd_integrator* h, const 
         auto& H = Handle(h);
         CheckVertexArray(H, v, n, "velocities");
         PBAT_HIP(hipSetDevice(H.device));
-        H.rollout.valid = false;
+        H.rollout.Discard();
         UploadVec(H, H.vel.ptr, v, nullptr);
         UploadVec(H, H.vprev.ptr, v, nullptr);
     });
@@ -1365,7 +1573,7 @@ extern "C" int pbat_vbd_integrator_set_external_acceleration(pbat_vbd_integrator
         auto& H = Handle(h);
         CheckVertexArray(H, a, n, "external acceleration");
         PBAT_HIP(hipSetDevice(H.device));
-        H.rollout.valid = false;
+        H.rollout.Discard();
         UploadVec(H, H.aext.ptr, a, nullptr);
     });
 }
@@ -1394,7 +1602,7 @@ extern "C" int pbat_vbd_integrator_set_detH_zero(pbat_vbd_integrator* h, float z
 {
     return Guard([&] {
         Handle(h).detHZero = zero;
-        Handle(h).rollout.valid = false;
+        Handle(h).rollout.Discard();
     });
 }
 
@@ -1402,7 +1610,7 @@ extern "C" int pbat_vbd_integrator_set_rayleigh_damping(pbat_vbd_integrator* h, 
 {
     return Guard([&] {
         Handle(h).kD = kD;
-        Handle(h).rollout.valid = false;
+        Handle(h).rollout.Discard();
     });
 }
 
@@ -1412,7 +1620,7 @@ extern "C" int pbat_vbd_integrator_set_initialization_strategy(pbat_vbd_integrat
         if (strategy < 0 || strategy > 4)
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid initialization strategy");
         Handle(h).strategy = strategy;
-        Handle(h).rollout.valid = false;
+        Handle(h).rollout.Discard();
     });
 }
 
@@ -1442,7 +1650,7 @@ extern "C" int pbat_vbd_integrator_set_lanes_per_vertex(pbat_vbd_integrator* h, 
         if (lanes != 0 && (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0))
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "lanes per vertex must be 0 or a power of two in [1, 64]");
         Handle(h).lanes = lanes;
-        Handle(h).rollout.valid = false;
+        Handle(h).rollout.Discard();
     });
 }
 
@@ -1465,7 +1673,7 @@ extern "C" int pbat_vbd_integrator_set_sweep_config(pbat_vbd_integrator* h, cons
         H.xcdRemap = config->xcd_remap != 0;
         H.ntStream = config->nontemporal != 0;
         H.useEll = config->fixed_stride != 0;
-        H.rollout.valid = false; // the recomputed steps must repeat the forward bit for bit
+        H.rollout.Discard(); // the recomputed steps must repeat the forward bit for bit
     });
 }
 
@@ -1488,7 +1696,7 @@ extern "C" int pbat_vbd_integrator_autotune(pbat_vbd_integrator* h, float dt, in
 {
     return Guard([&] {
         Autotune(Handle(h), dt, sweeps, timings, n_candidates);
-        Handle(h).rollout.valid = false;
+        Handle(h).rollout.Discard();
     });
 }
 
@@ -1509,6 +1717,37 @@ extern "C" int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3])
             Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null output");
         Energy(Handle(h), out);
     });
+}
+
+extern "C" int pbat_vbd_integrator_fp64_solves(pbat_vbd_integrator* h, int64_t* count, int32_t reset)
+{
+    return Guard([&] {
+        auto& H = Handle(h);
+        PBAT_HIP(hipSetDevice(H.device));
+        unsigned int c = 0;
+        PBAT_HIP(hipMemcpyAsync(&c, H.fp64Solves.ptr, sizeof(c), hipMemcpyDeviceToHost, H.stream));
+        PBAT_HIP(hipStreamSynchronize(H.stream));
+        if (count)
+            *count = static_cast<int64_t>(c);
+        if (reset)
+        {
+            unsigned int const zero = 0;
+            PBAT_HIP(hipMemcpyAsync(H.fp64Solves.ptr, &zero, sizeof(zero), hipMemcpyHostToDevice, H.stream));
+            PBAT_HIP(hipStreamSynchronize(H.stream));
+        }
+    });
+}
+
+extern "C" int pbat_vbd_integrator_objective(pbat_vbd_integrator* h, const float* x, const float* xtilde, float dt,
+                                             double* f, double* grad, double* residual2)
+{
+    return Guard([&] { Objective(Handle(h), x, xtilde, dt, f, grad, residual2); });
+}
+
+extern "C" int pbat_vbd_integrator_trace_objective_step(pbat_vbd_integrator* h, float dt, int32_t iterations,
+                                                        int32_t substeps, int32_t t, const char* dir)
+{
+    return Guard([&] { TraceObjectiveStep(Handle(h), dt, iterations, substeps, t, dir); });
 }
 
 extern "C" int pbat_vbd_integrator_synchronize(pbat_vbd_integrator* h)

@@ -137,6 +137,7 @@ struct StepConstants
     float dt2;      // h^2
     float kD;       // Rayleigh damping coefficient
     float detHZero; // skip threshold on |det H|
+    unsigned int* fp64Solves; // counts 3x3 solves taken by the FP64 path (nullable)
 };
 
 // Rayleigh damping of the elastic block (sim/vbd/Kernels.h:173-191).
@@ -186,6 +187,8 @@ __device__ __forceinline__ float3 InertiaNewtonStep(Block B, float4 const& xv, f
     if (fabsf(det) < 1e-4f * dscale)
     {
         // ill-conditioned block: solve in FP64 from the FP32 entries
+        if (c.fp64Solves)
+            atomicAdd(c.fp64Solves, 1u);
         double const h00 = B.h00, h01 = B.h01, h02 = B.h02, h11 = B.h11, h12 = B.h12, h22 = B.h22;
         double const b00 = h11 * h22 - h12 * h12, b01 = h02 * h12 - h01 * h22, b02 = h01 * h12 - h02 * h11;
         double const b11 = h00 * h22 - h02 * h02, b12 = h01 * h02 - h00 * h12, b22 = h00 * h11 - h01 * h01;
@@ -476,9 +479,12 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 }
 
 // Substep end (gpu/impl/vbd/Integrator.cu:329-347): vprev <- v, v = (x - xt)/h, xt <- x.
+// Every vertex passes here once per substep, so this is where a non-finite
+// position or velocity (a diverged solve, a NaN set by the caller) is flagged:
+// *nonfinite = 1 (nullable; written only then).
 [[maybe_unused]] static __global__ __launch_bounds__(256) void SubstepEnd(float4 const* __restrict__ xm, float4* __restrict__ xt,
                                                   float4* __restrict__ vel, float4* __restrict__ vprev, int n,
-                                                  float dt)
+                                                  float dt, int* __restrict__ nonfinite)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n)
@@ -486,8 +492,12 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     float4 const x = xm[i];
     float4 const p = xt[i];
     vprev[i] = vel[i];
-    vel[i] = make_float4((x.x - p.x) / dt, (x.y - p.y) / dt, (x.z - p.z) / dt, 0.f);
+    float4 const u = make_float4((x.x - p.x) / dt, (x.y - p.y) / dt, (x.z - p.z) / dt, 0.f);
+    vel[i] = u;
     xt[i] = make_float4(x.x, x.y, x.z, 0.f);
+    if (nonfinite && !(isfinite(x.x) && isfinite(x.y) && isfinite(x.z) && isfinite(u.x) && isfinite(u.y) &&
+                       isfinite(u.z)))
+        *nonfinite = 1;
 }
 
 // Chebyshev semi-iterative update after iteration k (sim/vbd/Kernels.h:104-118,
@@ -839,6 +849,78 @@ __device__ inline void BlockSum3(double& a, double& b, double& c)
         part[3 * blockIdx.x + 0] = K;
         part[3 * blockIdx.x + 1] = U;
         part[3 * blockIdx.x + 2] = D;
+    }
+}
+
+// Gradient of the incremental potential f = 1/2 |x - x~|_M^2 + h^2 sum_e w_e Psi_e
+// (sim::vbd::Integrator::ObjectiveFunctionGradient, sim/vbd/Integrator.cpp:168-200) in
+// FP64 from the FP32 state, one thread per vertex v of [0, nV):
+//   g_v = m_v (x_v - x~_v) + h^2 sum_{(e, l) incident to v} (w_e P(F_e)) gp_{e,l}
+// with P(F) = mu F + lambda (J - alpha) cof F the Stable Neo-Hookean stress
+// (physics/StableNeoHookeanEnergy.h:766-795) and gp_{e,l} row l of the tet's shape
+// function gradients. inc lists 4 e + l per vertex (CSR vtp), ascending e, so each
+// vertex's sum has a fixed order (no atomics). grad: 3 doubles per vertex (internal
+// order). part[3 b] receives workgroup b's sum of |g_v|^2 over v < nUpd (the
+// non-Dirichlet vertices), part[3 b + 1..2] = 0. blockDim = 256.
+[[maybe_unused]] static __global__ __launch_bounds__(256) void ObjectiveGradient(
+    float4 const* __restrict__ xm, float4 const* __restrict__ xtil, int const* __restrict__ vtp,
+    int const* __restrict__ inc, int4 const* __restrict__ tetv, float4 const* __restrict__ tgp, int nV, int nUpd,
+    double dt2, double* __restrict__ grad, double* __restrict__ part)
+{
+    int const v = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    double r2 = 0.;
+    if (v < nV)
+    {
+        double gx = 0., gy = 0., gz = 0.;
+        for (int k = vtp[v]; k < vtp[v + 1]; ++k)
+        {
+            int const e = inc[k] >> 2, l = inc[k] & 3;
+            int4 const t = tetv[e];
+            float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
+            float4 const p0 = xm[t.x], p1 = xm[t.y], p2 = xm[t.z], p3 = xm[t.w];
+            double const r[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+            double const d[3][3] = {{double(p1.x) - p0.x, double(p1.y) - p0.y, double(p1.z) - p0.z},
+                                    {double(p2.x) - p0.x, double(p2.y) - p0.y, double(p2.z) - p0.z},
+                                    {double(p3.x) - p0.x, double(p3.y) - p0.y, double(p3.z) - p0.z}};
+            double F[3][3]; // F[row][col] = sum_k d[k][row] r[k][col]
+            for (int rr = 0; rr < 3; ++rr)
+                for (int c = 0; c < 3; ++c)
+                    F[rr][c] = d[0][rr] * r[0][c] + d[1][rr] * r[1][c] + d[2][rr] * r[2][c];
+            double cof[3][3]; // column c = f_{c+1} x f_{c+2}
+            for (int c = 0; c < 3; ++c)
+            {
+                int const c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+                cof[0][c] = F[1][c1] * F[2][c2] - F[2][c1] * F[1][c2];
+                cof[1][c] = F[2][c1] * F[0][c2] - F[0][c1] * F[2][c2];
+                cof[2][c] = F[0][c1] * F[1][c2] - F[1][c1] * F[0][c2];
+            }
+            double const J = F[0][0] * cof[0][0] + F[1][0] * cof[1][0] + F[2][0] * cof[2][0];
+            double const wmu = q2.y, s = double(q2.z) * (J - double(q2.w));
+            double gp[3];
+            for (int c = 0; c < 3; ++c)
+                gp[c] = (l == 0) ? -(r[0][c] + r[1][c] + r[2][c]) : r[l - 1][c];
+            double out[3];
+            for (int rr = 0; rr < 3; ++rr)
+                out[rr] = (wmu * F[rr][0] + s * cof[rr][0]) * gp[0] + (wmu * F[rr][1] + s * cof[rr][1]) * gp[1] +
+                          (wmu * F[rr][2] + s * cof[rr][2]) * gp[2];
+            gx += out[0], gy += out[1], gz += out[2];
+        }
+        float4 const x = xm[v], y = xtil[v];
+        double const m = x.w;
+        gx = m * (double(x.x) - y.x) + dt2 * gx;
+        gy = m * (double(x.y) - y.y) + dt2 * gy;
+        gz = m * (double(x.z) - y.z) + dt2 * gz;
+        grad[3 * v] = gx, grad[3 * v + 1] = gy, grad[3 * v + 2] = gz;
+        if (v < nUpd)
+            r2 = gx * gx + gy * gy + gz * gz;
+    }
+    double z0 = 0., z1 = 0.;
+    BlockSum3(r2, z0, z1);
+    if (threadIdx.x == 0)
+    {
+        part[3 * blockIdx.x] = r2;
+        part[3 * blockIdx.x + 1] = 0.;
+        part[3 * blockIdx.x + 2] = 0.;
     }
 }
 

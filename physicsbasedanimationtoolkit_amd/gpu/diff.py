@@ -33,6 +33,7 @@ def _make_function():
             integrator.v = v0.detach().cpu().numpy()
             integrator.rollout_forward(dt, iterations, steps, checkpoint_every)
             ctx.integrator = integrator
+            ctx.generation = integrator.rollout_generation()[0]
             ctx.solver = dict(solver or {})
             ctx.like = (log_y, x0, v0)
             xT = torch.from_numpy(np.ascontiguousarray(integrator.x)).to(dtype=x0.dtype, device=x0.device)
@@ -42,6 +43,14 @@ def _make_function():
         @staticmethod
         def backward(ctx, gx, gv):
             log_y, x0, v0 = ctx.like
+            generation, held = ctx.integrator.rollout_generation()
+            if not held or generation != ctx.generation:
+                # the integrator holds one rollout: another rollout, or a change of its
+                # state, moduli or settings, replaced the one this graph recorded
+                raise RuntimeError(
+                    "the integrator's rollout changed after this graph's forward pass (another "
+                    "rollout, or a new state, moduli or solver setting); its gradients would "
+                    "belong to that rollout. Use one integrator per pending backward pass.")
             gxa = np.zeros(tuple(x0.shape), dtype=np.float32) if gx is None else gx.detach().cpu().numpy()
             gva = None if gv is None else gv.detach().cpu().numpy()
             r = ctx.integrator.rollout_backward(gxa, gva, **ctx.solver)

@@ -39,13 +39,15 @@ _ACCELERATION = {AccelerationStrategy.Base: _native.ACCELERATION_NONE,
                  AccelerationStrategy.TrustRegion: _native.ACCELERATION_TRUST_REGION}
 
 
-def _marshal(data: Data):
-    """pbat_vbd_data view of a constructed Data, plus the arrays it points into."""
+def _marshal(data: Data, contact: bool = True, accelerator=None):
+    """pbat_vbd_data view of a constructed Data, plus the arrays it points into.
+    contact=False leaves the collision mesh out; accelerator overrides data.accelerator."""
     if not isinstance(data, Data):
         raise TypeError("expected a pbat.sim.vbd.Data")
-    if data.accelerator not in _ACCELERATION:
+    accelerator = data.accelerator if accelerator is None else accelerator
+    if accelerator not in _ACCELERATION:
         raise ValueError(
-            f"acceleration strategy {data.accelerator.name} is not available on this device path; "
+            f"acceleration strategy {AccelerationStrategy(accelerator).name} is not available on this device path; "
             "only the base, Chebyshev-, Anderson- and trust-region-accelerated VBD integrators are implemented")
     if data.m.size == 0:
         raise ValueError("data must be constructed (call data.construct()) before creating an Integrator")
@@ -81,12 +83,12 @@ def _marshal(data: Data):
     d.strategy = int(data.strategy)
     d.kD = float(data.kD)
     d.detHZero = float(data.detH_zero)
-    d.acceleration = _ACCELERATION[data.accelerator]
+    d.acceleration = _ACCELERATION[accelerator]
     d.rho = float(data.rho)
     d.window = int(data.manderson)
     d.eta, d.tau, d.curved = float(data.eta), float(data.tau), int(bool(data.curved))
     # collision mesh and contact model (sim/vbd/Data.h:165-178, 225-229)
-    if data.V.size and data.F.size:
+    if contact and data.V.size and data.F.size:
         keep["V"] = np.ascontiguousarray(data.V, dtype=np.int64).reshape(-1)
         keep["F"] = np.ascontiguousarray(np.asarray(data.F, dtype=np.int64).T)
         keep["B"] = np.ascontiguousarray(data.B, dtype=np.int64).reshape(-1)
@@ -106,8 +108,11 @@ class Integrator:
     """VBD time integrator running on the current HIP device."""
 
     def __init__(self, data: Data):
+        self._create(data)
+
+    def _create(self, data: Data, contact: bool = True, accelerator=None):
         self._lib = _native.lib()
-        d, keep = _marshal(data)
+        d, keep = _marshal(data, contact, accelerator)
         self._nV = d.nV
         self._nT = d.nT
         self._nCV = d.nCollisionVertices
@@ -276,6 +281,38 @@ class Integrator:
             self._h, _native.ptr(t, ctypes.c_double), _native.ptr(acc, ctypes.c_int32), n.value, ctypes.byref(n)))
         return [(float(t[k]), bool(acc[k])) for k in range(n.value)]
 
+    def objective(self, dt: float, x=None, xtilde=None, gradient: bool = False):
+        """FP64 objective of the step (sim::vbd::Integrator::ObjectiveFunction /
+        ObjectiveFunctionGradient, sim/vbd/Integrator.cpp:138-200) reduced on the device:
+        f = 1/2 |x - x~|_M^2 + dt^2 sum_e w_e Psi_e at x and xtilde (3 x nV; None = the
+        current positions / the inertial target of the last substep). Returns
+        (f, squared gradient norm over the non-Dirichlet vertices) and, with
+        gradient=True, also the gradient as a vertex-major (3 nV,) vector."""
+        xa = None if x is None else _vertex_matrix_f32(x, self._nV, "x")
+        la = None if xtilde is None else _vertex_matrix_f32(xtilde, self._nV, "xtilde")
+        f, r2 = ctypes.c_double(), ctypes.c_double()
+        g = np.zeros(3 * self._nV) if gradient else None
+        fp = ctypes.c_float
+        _native.check(self._lib.pbat_vbd_integrator_objective(
+            self._h, None if xa is None else _native.ptr(xa, fp), None if la is None else _native.ptr(la, fp),
+            float(dt), ctypes.byref(f), None if g is None else _native.ptr(g, ctypes.c_double), ctypes.byref(r2)))
+        return (f.value, r2.value, g) if gradient else (f.value, r2.value)
+
+    def trace_objective_step(self, dt: float, iterations: int, substeps: int = 1, t: int = -1,
+                             path: str = ".") -> None:
+        """One step whose iterates are traced as sim::vbd::Integrator::TraceNextStep does
+        (sim/vbd/Integrator.cpp:202-235): {path}/{t}.{s}.f.mtx, .grad.mtx and .x.mtx per substep."""
+        os.makedirs(path, exist_ok=True)
+        _native.check(self._lib.pbat_vbd_integrator_trace_objective_step(
+            self._h, float(dt), int(iterations), int(substeps), int(t), str(path).encode()))
+
+    def fp64_solves(self, reset: bool = False) -> int:
+        """Vertex block solves of the sweep that took the FP64 path (ill-conditioned
+        3x3 blocks) since creation or the last reset."""
+        n = ctypes.c_int64()
+        _native.check(self._lib.pbat_vbd_integrator_fp64_solves(self._h, ctypes.byref(n), int(bool(reset))))
+        return n.value
+
     def synchronize(self) -> None:
         _native.check(self._lib.pbat_vbd_integrator_synchronize(self._h))
 
@@ -312,6 +349,14 @@ class Integrator:
         `checkpoint_every` steps for rollout_backward."""
         _native.check(self._lib.pbat_vbd_rollout_forward(self._h, float(dt), int(iterations), int(steps),
                                                          int(checkpoint_every)))
+
+    def rollout_generation(self):
+        """(generation, held): identifies the rollout rollout_backward would
+        differentiate; the generation changes whenever a rollout is started or
+        discarded (a setter of the state, moduli or solver settings)."""
+        g, v = ctypes.c_int64(), ctypes.c_int32()
+        _native.check(self._lib.pbat_vbd_rollout_generation(self._h, ctypes.byref(g), ctypes.byref(v)))
+        return g.value, bool(v.value)
 
     @staticmethod
     def adjoint_config(solver: str = "pcg", max_iterations: int = 200, tolerance: float = 1e-6,

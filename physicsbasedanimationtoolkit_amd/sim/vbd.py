@@ -307,3 +307,136 @@ class Data:
                 raise ValueError("Expected eta >= 0")
             if self.tau <= 1:
                 raise ValueError("Expected tau > 1")
+
+
+class Integrator:
+    """``pbat.sim.vbd.Integrator`` (bindings/pypbat/sim/vbd/Integrator.cpp:30-90) over
+    the HIP path: the reference's CPU VBD integrator (sim/vbd/Integrator.h:17-69,
+    Integrator.cpp:22-136), stepped by the same device kernels as
+    ``pbat.gpu.vbd.Integrator`` but with the CPU integrator's semantics:
+
+    - the integrator the CPU binding's factory builds from ``data.accelerator``:
+      Chebyshev and Anderson accelerated, anything else plain VBD (the binding has
+      no trust-region integrator); Nesterov and Broyden (CPU-only in the reference)
+      are not built here and raise ``ValueError``;
+    - ``vt = v`` before the initial guess (Integrator.cpp:31-32), so the
+      AdaptiveVbd guess is ``x_t + h v`` (its acceleration estimate is 0);
+    - no contact: the CPU integrator reads no collision mesh;
+    - ``trace_next_step`` writes the reference's ``{t}.{s}.f.mtx``,
+      ``{t}.{s}.grad.mtx`` and ``{t}.{s}.x.mtx`` (Integrator.cpp:202-235), with the
+      objective and its gradient reduced in FP64 on the device.
+
+    The state lives on the device in FP32 (the reference CPU path is FP64); ``x``
+    and ``v`` read it back as float64 arrays, and ``data`` returns the Data with
+    ``x`` and ``v`` refreshed from the device. Assigning ``data`` rebuilds the
+    integrator from it; editing the arrays of a returned ``data`` in place does not
+    reach the device (assign ``x``/``v`` instead)."""
+
+    def __init__(self, data: Data):
+        self._trace = None
+        self._build(data)
+
+    def _build(self, data: Data):
+        from ..gpu.vbd import Integrator as _DeviceIntegrator
+
+        if data.accelerator in (AccelerationStrategy.Nesterov, AccelerationStrategy.Broyden):
+            raise ValueError(f"the {data.accelerator.name}-accelerated VBD integrator (CPU-only in the reference) "
+                             "is not available here; Base, Chebyshev and Anderson are")
+        accel = data.accelerator if data.accelerator in (AccelerationStrategy.Chebyshev,
+                                                         AccelerationStrategy.Anderson) else AccelerationStrategy.Base
+        dev = _DeviceIntegrator.__new__(_DeviceIntegrator)
+        dev._create(data, contact=False, accelerator=accel)
+        self._dev = dev
+        self._data = data
+        self._strategy = InitializationStrategy(int(data.strategy))
+        self._apply_strategy()
+
+    def _apply_strategy(self):
+        # sim/vbd/Integrator.cpp:31-32 sets vt = v before InitialPositionsForSolve, whose
+        # AdaptiveVbd estimate (vt - vtm1) / h . a then vanishes: x0 = xt + h vt, the
+        # Inertia guess (sim/vbd/Kernels.h:45-94)
+        s = self._strategy
+        self._dev.strategy = InitializationStrategy.Inertia if s == InitializationStrategy.AdaptiveVbd else s
+
+    def step(self, dt: float, iterations: int, substeps: int = 1) -> None:
+        """Integrate the VBD simulation 1 time step (sim/vbd/Integrator.cpp:22-45)."""
+        if self._trace is not None:
+            path, t = self._trace
+            self._trace = None  # mTraceIterates = false after the step (Integrator.cpp:44)
+            self._dev.trace_objective_step(dt, iterations, substeps, t, path)
+        else:
+            self._dev.step(dt, iterations, substeps)
+
+    def trace_next_step(self, path: str = ".", t: int = -1) -> None:
+        """Mark the next time step for optimization tracing (sim/vbd/Integrator.cpp:47-52)."""
+        self._trace = (str(path), int(t))
+
+    # ---- the reference's debugging methods (sim/vbd/Integrator.h:41-49) ----------
+    def objective_function(self, xk, xtilde, dt: float) -> float:
+        """1/2 |xk - xtilde|_M^2 + dt^2 sum_e w_e Psi_e (sim/vbd/Integrator.cpp:138-166)."""
+        return self._dev.objective(dt, xk, xtilde)[0]
+
+    def objective_function_gradient(self, xk, xtilde, dt: float) -> np.ndarray:
+        """Its gradient, vertex-major (3 |#nodes|,) (sim/vbd/Integrator.cpp:168-200)."""
+        return self._dev.objective(dt, xk, xtilde, gradient=True)[2]
+
+    # ---- properties of the binding ----------------------------------------------
+    @property
+    def x(self) -> np.ndarray:
+        """3x|#nodes| nodal positions"""
+        return self._dev.x.astype(np.float64)
+
+    @x.setter
+    def x(self, value):
+        self._dev.x = value
+
+    @property
+    def v(self) -> np.ndarray:
+        """3x|#nodes| nodal velocities"""
+        return self._dev.v.astype(np.float64)
+
+    @v.setter
+    def v(self, value):
+        self._dev.v = value
+
+    @property
+    def strategy(self) -> InitializationStrategy:
+        """Initialization strategy of the step's optimisation"""
+        return self._strategy
+
+    @strategy.setter
+    def strategy(self, value):
+        self._strategy = InitializationStrategy(int(value))
+        self._data.strategy = self._strategy
+        self._apply_strategy()
+
+    @property
+    def kD(self) -> float:
+        """Rayleigh damping coefficient"""
+        return float(self._data.kD)
+
+    @kD.setter
+    def kD(self, value):
+        self._data.kD = float(value)
+        self._dev.kD = float(value)
+
+    @property
+    def detH_residual(self) -> float:
+        """Determinant of the residual Hessian for pseudo singularity check"""
+        return float(self._data.detH_zero)
+
+    @detH_residual.setter
+    def detH_residual(self, value):
+        self._data.detH_zero = float(value)
+        self._dev.detH_residual = float(value)
+
+    @property
+    def data(self) -> Data:
+        """The problem data, with x and v read back from the device"""
+        self._data.x = self.x
+        self._data.v = self.v
+        return self._data
+
+    @data.setter
+    def data(self, value: Data):
+        self._build(value)

@@ -361,6 +361,63 @@ def test_rayleigh_damping_dissipates_energy():
     assert damped.energy().sum() < plain.energy().sum() < e0
 
 
+# ---- total energy (kinetic, elastic, external potential) ----------------------
+# The oracle's energy(): K = 1/2 sum m |v|^2, U = sum_e w_e Psi(F_e), P = -sum m a.x
+# (the terms of sim/vbd/Integrator.cpp:138-166 at the step's state). Pinned by what
+# mechanics fixes, not by retyping the sums.
+
+def _translated_cube(n, c, v, a):
+    s = meshes.plain_cpu_scene(n)
+    nV = s.X.shape[1]
+    X = s.X + np.asarray(c, float)[:, None]
+    o = O.OracleVbd(X, s.E, v=np.tile(np.asarray(v, float)[:, None], (1, nV)),
+                    aext=np.tile(np.asarray(a, float)[:, None], (1, nV)), mue=s.mue, lambdae=s.lambdae,
+                    strategy=O.INIT_KINETIC_ENERGY_MINIMUM)
+    return s, o
+
+
+def test_energy_of_a_rigid_translation():
+    """Uniform velocity v, positions X + c: K = 1/2 M |v|^2 (M the total lumped mass),
+    U = sum_e w_e mu_e^2 / (2 lambda_e) (F = I everywhere, the reference's Psi(I)
+    known answer, physics/StableNeoHookeanEnergy.cpp:10-42) and P = -M a . x_com
+    with x_com the mass centre; moving the body by d changes P by -M a . d exactly."""
+    v, a = np.array([0.3, -1.2, 0.7]), np.array([1.0, -9.81, 2.5])
+    s, o = _translated_cube(4, [0.0, 0.0, 0.0], v, a)
+    _, o2 = _translated_cube(4, [0.25, -0.5, 2.0], v, a)
+    M = o.m.sum()
+    K, U, P = o.energy()
+    assert K == pytest.approx(0.5 * M * v @ v, rel=1e-13)
+    mu, lam = o.lame[:, 0], o.lame[:, 1]
+    assert U == pytest.approx(np.sum(o.wg * mu * mu / (2 * lam)), rel=1e-9)
+    com = (o.m[:, None] * o.x).sum(axis=0) / M
+    assert P == pytest.approx(-M * a @ com, rel=1e-12)
+    K2, U2, P2 = o2.energy()
+    assert K2 == pytest.approx(K, rel=1e-13) and U2 == pytest.approx(U, rel=1e-9)
+    assert P2 - P == pytest.approx(-M * a @ np.array([0.25, -0.5, 2.0]), rel=1e-10)
+
+
+@pytest.mark.parametrize("steps", [1, 5])
+def test_energy_loss_of_backward_euler_free_fall(steps):
+    """A free body under uniform acceleration a (no Dirichlet vertices), started at rest
+    shape with uniform velocity: the step's minimiser is the inertial target exactly
+    (F = I, grad Psi(I) = 0), so x' = x + h v + h^2 a and v' = v + h a. Then per step
+    dK + dP = -1/2 M h^2 |a|^2 (backward Euler's numerical dissipation) and dU = 0.
+    A dropped 1/2 on K or a flipped sign on P breaks the identity."""
+    v, a = np.array([0.5, 1.0, -0.25]), np.array([0.0, -9.81, 3.0])
+    h = 1.0 / 60.0
+    _, o = _translated_cube(3, [0.1, 0.2, 0.3], v, a)
+    M = o.m.sum()
+    e0 = o.energy()
+    for _ in range(steps):
+        o.step(h, 10, 1)
+    e1 = o.energy()
+    loss = -0.5 * M * h * h * (a @ a) * steps
+    assert (e1[0] - e0[0]) + (e1[2] - e0[2]) == pytest.approx(loss, rel=1e-8)
+    assert e1[1] == pytest.approx(e0[1], rel=1e-9)
+    # the motion itself: uniform acceleration
+    assert np.allclose(o.velocities, (v + steps * h * a)[:, None], atol=1e-9)
+
+
 # ---- Chebyshev-accelerated VBD (sim/vbd/ChebyshevIntegrator.cpp) ------------
 
 def test_chebyshev_omega_sequence_keeps_reference_precedence():
