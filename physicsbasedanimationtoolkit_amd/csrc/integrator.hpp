@@ -90,26 +90,30 @@ struct DeviceArray
     }
 };
 
-// Incidence records of the sweep (structure of arrays, see SweepArgs).
+// Incidence records (structure of arrays, see SweepArgs): the sweep reads
+// ids, geo and mat (40 B); the adjoint's vertex kernel reads ids, g0, g1, g2
+// (the three GP rows, 56 B).
 struct IncidenceRecords
 {
     DeviceArray<int4> ids;
-    DeviceArray<float4> g0, g1;
-    DeviceArray<float2> g2;
-    void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(); }
+    DeviceArray<float4> g0, g1, geo;
+    DeviceArray<float2> g2, mat;
+    void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(), geo.reset(), mat.reset(); }
 };
 
 struct HostIncidenceRecords
 {
     std::vector<int4> ids;
-    std::vector<float4> g0, g1;
-    std::vector<float2> g2;
+    std::vector<float4> g0, g1, geo;
+    std::vector<float2> g2, mat;
     void resize(size_t n)
     {
         ids.assign(n, make_int4(-1, 0, 0, 0)); // a = -1: unused fixed-stride slot
         g0.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
         g1 = g0;
+        geo = g0;
         g2.assign(n, make_float2(0.f, 0.f));
+        mat = g2;
     }
     void upload(IncidenceRecords& d, hipStream_t s) const
     {
@@ -117,6 +121,8 @@ struct HostIncidenceRecords
         d.g0.upload(g0, s);
         d.g1.upload(g1, s);
         d.g2.upload(g2, s);
+        d.geo.upload(geo, s);
+        d.mat.upload(mat, s);
     }
 };
 
@@ -148,14 +154,22 @@ struct RolloutState
     DeviceArray<float> logY;                                  // parameter upload buffer
     // step Hessian assembled per solve as 3x3 blocks over the updated vertices:
     // the pattern and assembly sources (built once per handle), the values, the
-    // longest row, and the fused iteration's second direction buffer and
-    // last-workgroup counters
+    // longest row, and the fused iteration's second direction buffer
     DeviceArray<int> bsrRowPtr, bsrCol, bsrBlockRow, bsrSrcPtr, bsrSrc;
     DeviceArray<float4> bsrA, bsrB;
     DeviceArray<float> bsrC;
     int bsrMaxRow = 0;
     DeviceArray<float4> p2;
-    DeviceArray<unsigned int> counters;
+    // row chunks of the fused iteration (begin, end), mesh-major, and the first
+    // chunk of each mesh (nMeshes + 1); chunks hold at most bsrChunkRows rows
+    DeviceArray<int2> bsrChunks;
+    DeviceArray<int> meshChunkBegin;
+    int bsrChunkRows = 0;
+    // per-chunk p . H p, per-aggregate r . z, r . r, b . b, per-coarse-row rc . zc
+    DeviceArray<double> pqPart, rzPart, rrPart, bbPart, rczcPart;
+    // Galerkin coarse operator from the blocks: coarse block k couples aggregates
+    // (cbA[k], cbB[k]) and sums the fine blocks cbSrc[cbSrcPtr[k] .. cbSrcPtr[k + 1])
+    DeviceArray<int> cbA, cbB, cbSrcPtr, cbSrc;
     // captured chunk of CG iterations and the settings it was captured for
     hipGraph_t cgGraph = nullptr;
     hipGraphExec_t cgExec = nullptr;

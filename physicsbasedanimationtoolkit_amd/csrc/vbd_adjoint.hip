@@ -19,6 +19,7 @@
 #include "vbd_hessian_kernels.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <functional>
 #include <string>
@@ -132,14 +133,6 @@ void EnsureBuffers(pbat_vbd_integrator& h)
     for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
-    if (R.counters.n != 2)
-    {
-        // last-workgroup tickets of the fused CG iteration; each launch leaves them at 0
-        R.counters.allocate(2);
-        hipLaunchKernelGGL(device::ZeroScalars, dim3(1), dim3(64), 0, h.stream, static_cast<double*>(nullptr), 0,
-                           reinterpret_cast<int*>(R.counters.ptr), 2);
-    }
-
     if (R.dinv.n != 2 * n)
         R.dinv.allocate(2 * n);
     size_t const nM = static_cast<size_t>(h.nMeshes);
@@ -162,8 +155,11 @@ device::CoarseArgs MakeCoarseArgs(pbat_vbd_integrator& h)
     return c;
 }
 
-// Assemble the coarse operator Z^T H Z at xs (positions with masses) and
-// invert each mesh's block in place.
+void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt);
+
+// The coarse operator Z^T H Z at xs (positions with masses): the step Hessian
+// assembled there, its blocks summed per pair of aggregates in a fixed order,
+// then each mesh's dense block inverted in place.
 void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
 {
     RolloutState& R = h.rollout;
@@ -186,11 +182,14 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
                        static_cast<long long>(total));
     hipLaunchKernelGGL(device::ZeroDoublesLong, dim3(64), dim3(256), 0, h.stream, R.rc.ptr,
                        static_cast<long long>(R.rc.n));
-    int const nT = static_cast<int>(h.nT), nUpd = static_cast<int>(h.nUpdated);
-    hipLaunchKernelGGL(device::CoarseAssembleElastic, dim3(Blocks(nT)), dim3(256), 0, h.stream, xs, h.tetv.ptr,
-                       h.tgp.ptr, nT, c, R.coarse[0].ptr);
-    hipLaunchKernelGGL(device::CoarseAssembleMass, dim3(Blocks(nUpd)), dim3(256), 0, h.stream, xs, nUpd, c,
-                       1. / (double(dt) * double(dt)), R.coarse[0].ptr);
+    AssembleBsr(h, xs, dt);
+    int const nCB = static_cast<int>(R.cbA.n);
+    if (nCB > 0)
+        hipLaunchKernelGGL(device::CoarseFromBsr, dim3(Blocks(nCB)), dim3(256), 0, h.stream,
+                           static_cast<int const*>(R.cbA.ptr), static_cast<int const*>(R.cbB.ptr),
+                           static_cast<int const*>(R.cbSrcPtr.ptr), static_cast<int const*>(R.cbSrc.ptr), nCB,
+                           static_cast<float4 const*>(R.bsrA.ptr), static_cast<float4 const*>(R.bsrB.ptr),
+                           static_cast<float const*>(R.bsrC.ptr), c, R.coarse[0].ptr);
     int maxN = 0;
     std::vector<int> finalBuf(static_cast<size_t>(nM));
     for (int m = 0; m < nM; ++m)
@@ -239,6 +238,8 @@ void VertexPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, f
 }
 
 // ---- assembled step Hessian (vbd_hessian_kernels.hpp) ------------------------
+
+int BsrLanes(pbat_vbd_integrator const& h);
 
 // Block pattern of the step Hessian over the updated vertices (row i lists the
 // updated vertices sharing a tet with i, itself included, ascending) and each
@@ -326,6 +327,79 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
     R.bsrSrc.upload(src, h.stream);
     R.bsrA.allocate(nBlocks), R.bsrB.allocate(nBlocks), R.bsrC.allocate(nBlocks);
     R.bsrMaxRow = maxRow;
+
+    // mesh and aggregate of every updated vertex
+    std::vector<int> meshOf(static_cast<size_t>(nUpd)), aggOf(static_cast<size_t>(nUpd));
+    if (nUpd > 0)
+    {
+        PBAT_HIP(hipMemcpy(meshOf.data(), h.meshOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
+        PBAT_HIP(hipMemcpy(aggOf.data(), h.aggOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
+    }
+    // row chunks: every (mesh, colour) run of rows cut into chunks of at most
+    // 256 / L rows, then ordered mesh-major (stable: colour, then row order)
+    int const nM = h.nMeshes;
+    int const chunkRows = 256 / BsrLanes(h);
+    struct Chunk
+    {
+        int mesh, begin, end;
+    };
+    std::vector<Chunk> chunks;
+    int const nC = static_cast<int>(h.colorBegin.size()) - 1;
+    for (int c = 0; c < nC; ++c)
+    {
+        int r = h.colorBegin[c];
+        int const re = std::min(h.colorBegin[c + 1], nUpd);
+        while (r < re)
+        {
+            int runEnd = r + 1;
+            while (runEnd < re && meshOf[runEnd] == meshOf[r])
+                ++runEnd;
+            for (int b = r; b < runEnd; b += chunkRows)
+                chunks.push_back({meshOf[r], b, std::min(b + chunkRows, runEnd)});
+            r = runEnd;
+        }
+    }
+    std::stable_sort(chunks.begin(), chunks.end(), [](Chunk const& a, Chunk const& b) { return a.mesh < b.mesh; });
+    std::vector<int2> chunkRange(chunks.size());
+    std::vector<int> meshChunk(static_cast<size_t>(nM) + 1, 0);
+    for (size_t k = 0; k < chunks.size(); ++k)
+    {
+        chunkRange[k] = make_int2(chunks[k].begin, chunks[k].end);
+        ++meshChunk[chunks[k].mesh + 1];
+    }
+    for (int m = 0; m < nM; ++m)
+        meshChunk[m + 1] += meshChunk[m];
+    R.bsrChunks.upload(chunkRange, h.stream);
+    R.meshChunkBegin.upload(meshChunk, h.stream);
+    R.bsrChunkRows = chunkRows;
+    R.pqPart.allocate(std::max<size_t>(chunks.size(), 1));
+    size_t const nAgg = static_cast<size_t>(std::max(h.nAggregates, 1));
+    R.rzPart.allocate(nAgg), R.rrPart.allocate(nAgg), R.bbPart.allocate(nAgg), R.rczcPart.allocate(3 * nAgg);
+
+    // coarse blocks: the fine blocks grouped by (aggregate of the row, aggregate
+    // of the column), ascending within each group
+    std::vector<std::array<int, 3>> keys(nBlocks);
+    for (size_t k = 0; k < nBlocks; ++k)
+        keys[k] = {aggOf[blockRow[k]], aggOf[col[k]], static_cast<int>(k)};
+    std::sort(keys.begin(), keys.end());
+    std::vector<int> cbA, cbB, cbPtr{0}, cbSrc(nBlocks);
+    for (size_t k = 0; k < nBlocks; ++k)
+    {
+        if (k == 0 || keys[k][0] != keys[k - 1][0] || keys[k][1] != keys[k - 1][1])
+        {
+            if (k > 0)
+                cbPtr.push_back(static_cast<int>(k));
+            cbA.push_back(keys[k][0]);
+            cbB.push_back(keys[k][1]);
+        }
+        cbSrc[k] = keys[k][2];
+    }
+    if (nBlocks > 0)
+        cbPtr.push_back(static_cast<int>(nBlocks));
+    R.cbA.upload(cbA, h.stream);
+    R.cbB.upload(cbB, h.stream);
+    R.cbSrcPtr.upload(cbPtr, h.stream);
+    R.cbSrc.upload(cbSrc, h.stream);
     PBAT_HIP(hipStreamSynchronize(h.stream));
 }
 
@@ -418,6 +492,58 @@ void BsrPass(pbat_vbd_integrator& h, device::BsrArgs a, int vbeg, int vend)
     }
 }
 
+template <int L, int K>
+void LaunchDirectionLK(pbat_vbd_integrator& h, device::BsrArgs a)
+{
+    RolloutState& R = h.rollout;
+    int const nChunks = static_cast<int>(R.bsrChunks.n);
+    int64_t grid64 = std::max(nChunks, 1);
+    a.xcdChunk = 0;
+    if (h.xcdRemap && grid64 >= 16)
+    {
+        grid64 = (grid64 + 7) / 8 * 8;
+        a.xcdChunk = static_cast<int>(grid64 / 8);
+    }
+    hipLaunchKernelGGL((device::BsrDirection<L, K>), dim3(static_cast<int>(grid64)), dim3(256), 0, h.stream, a,
+                       static_cast<int2 const*>(R.bsrChunks.ptr), nChunks, R.pqPart.ptr);
+}
+
+template <int L>
+void LaunchDirectionL(pbat_vbd_integrator& h, device::BsrArgs const& a)
+{
+    switch (BsrPerLane(h))
+    {
+        case 1: LaunchDirectionLK<L, 1>(h, a); break;
+        case 2: LaunchDirectionLK<L, 2>(h, a); break;
+        case 3: LaunchDirectionLK<L, 3>(h, a); break;
+        default: LaunchDirectionLK<L, 4>(h, a); break;
+    }
+}
+
+// The fused iteration's direction product over the row chunks (BsrDirection):
+// a.y = p (written), a.out = H p, a.z, a.pOld, a.zc as BsrArgs describes them.
+void DirectionPass(pbat_vbd_integrator& h, device::BsrArgs a)
+{
+    RolloutState& R = h.rollout;
+    if (R.bsrChunks.n == 0)
+        return;
+    if (256 / BsrLanes(h) != R.bsrChunkRows)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "row chunks built for another lane count");
+    a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
+    a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
+    a.meshOf = h.meshOf.ptr, a.aggOf = h.aggOf.ptr;
+    switch (BsrLanes(h))
+    {
+        case 1: LaunchDirectionL<1>(h, a); break;
+        case 2: LaunchDirectionL<2>(h, a); break;
+        case 4: LaunchDirectionL<4>(h, a); break;
+        case 8: LaunchDirectionL<8>(h, a); break;
+        case 16: LaunchDirectionL<16>(h, a); break;
+        case 32: LaunchDirectionL<32>(h, a); break;
+        default: LaunchDirectionL<64>(h, a); break;
+    }
+}
+
 // One Gauss-Seidel pass over the colours (forward or reverse order) on H y = b
 // with the assembled blocks. With acc (a reverse pass after a forward pass,
 // which updates each vertex last), the accumulator r.z of each mesh receives
@@ -480,12 +606,17 @@ SolveResult ReadResult(pbat_vbd_integrator& h)
 // mesh has converged.
 //
 // Block Jacobi and two-level preconditioners (the fused iteration), per iteration:
-//   (1) p = z + Z zc + beta p_old formed on the fly, q = H p, p . q;
-//   (2) one wave per aggregate: y += alpha p, r -= alpha q (alpha = rz / pq),
-//       z = D^-1 r, r . z, r . r and the restriction rc = Z^T r;
-//   (3) two-level: zc = (Z^T H Z)^-1 rc and rc . zc; the last workgroup ends the
-//       iteration (stopping test, beta with r . z = r . D^-1 r + rc . zc);
-//       block Jacobi: (2)'s last workgroup ends it.
+//   (1) per row chunk: p = z + Z zc + beta p_old formed on the fly, q = H p,
+//       the chunk's p . q;
+//   (2) one workgroup per aggregate: alpha = rz / pq from its mesh's chunk sums,
+//       y += alpha p, r -= alpha q, z = D^-1 r, the aggregate's r . z, r . r and
+//       restriction rc = Z^T r;
+//   (3) two-level: zc = (Z^T H Z)^-1 rc and each coarse row's rc . zc;
+//   (4) one workgroup: per mesh, r . z = r . D^-1 r + rc . zc and r . r summed
+//       over its partials, stopping test, beta.
+// No atomics: every sum is taken in a fixed order, so a solve gives the same
+// bits on every run, and a mesh gives the same bits alone or in a batch (its
+// row chunks, aggregates and coarse rows are the same).
 // The symmetric Gauss-Seidel preconditioner keeps the four-launch iteration
 //   q = H p and p . q; y, r update with z = 0; a forward and a reverse coloured
 //   Gauss-Seidel pass for z with r . z; one workgroup: scalars; p = z + beta p.
@@ -561,18 +692,26 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = h.aggVerts.ptr, u.aggMesh = h.aggMesh.ptr;
         u.y = y, u.r = R.r.ptr, u.z = R.z.ptr, u.p = p, u.q = R.q.ptr, u.b = b, u.dinv = R.dinv.ptr;
         u.rc = twoLevel ? R.rc.ptr : nullptr;
-        u.cg = cg, u.counter = R.counters.ptr, u.nAgg = h.nAggregates, u.init = init, u.finish = twoLevel ? 0 : 1;
+        u.pqPart = R.pqPart.ptr, u.meshChunkBegin = R.meshChunkBegin.ptr;
+        u.rzPart = R.rzPart.ptr, u.rrPart = R.rrPart.ptr, u.bbPart = R.bbPart.ptr;
+        u.cg = cg, u.init = init;
         if (h.nAggregates > 0)
-            hipLaunchKernelGGL(device::AggUpdate, dim3(Blocks(int64_t(h.nAggregates) * 64)), dim3(256), 0, h.stream,
-                               u);
+            hipLaunchKernelGGL(device::AggUpdate, dim3(h.nAggregates), dim3(128), 0, h.stream, u);
     };
-    auto coarseFinish = [&](int first) {
+    auto coarseRows = [&](int first) {
         int const rows = 3 * h.nAggregates, rowsPerBlock = 4 / coarseWavesPerRow;
-        hipLaunchKernelGGL(device::CoarseApplyFinish, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
+        hipLaunchKernelGGL(device::CoarseApplyRows, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
                            h.stream, static_cast<double const*>(R.coarse[0].ptr),
                            static_cast<double const*>(R.coarse[1].ptr), static_cast<int const*>(R.coarseFinal.ptr),
                            static_cast<double const*>(R.rc.ptr), R.zc.ptr, cargs, coarseWavesPerRow, cg,
-                           R.counters.ptr + 1, first);
+                           R.rczcPart.ptr, first);
+    };
+    auto finish = [&](int first) {
+        device::CgFinishArgs f{};
+        f.cg = cg, f.meshAggBegin = h.meshAggBegin.ptr;
+        f.rzPart = R.rzPart.ptr, f.rrPart = R.rrPart.ptr, f.bbPart = R.bbPart.ptr;
+        f.rczcPart = twoLevel ? R.rczcPart.ptr : nullptr;
+        hipLaunchKernelGGL(device::CgFinish, dim3(1), dim3(256), 0, h.stream, f, first);
     };
     std::function<void(int)> iteration;
     if (fused)
@@ -580,19 +719,21 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         // r = b - H y, z = D^-1 r, (rc, zc); p_old = 0 for the first direction
         product(y, R.q.ptr, nullptr);
         aggUpdate(1, nullptr);
-        if (twoLevel)
-            coarseFinish(1);
+        if (twoLevel && h.nAggregates > 0)
+            coarseRows(1);
+        finish(1);
         Zero4(h, R.p.ptr, nV);
         iteration = [&](int k) {
             float4* const pOld = (k & 1) ? R.p2.ptr : R.p.ptr;
             float4* const pNew = (k & 1) ? R.p.ptr : R.p2.ptr;
             device::BsrArgs a{};
             a.y = pNew, a.out = R.q.ptr, a.z = R.z.ptr, a.pOld = pOld, a.zc = twoLevel ? R.zc.ptr : nullptr;
-            a.cg = cg, a.acc = cg.A, a.accArray = device::kAccPq;
-            BsrPass<BsrMode::kDirectionDot>(h, a, 0, n);
+            a.cg = cg;
+            DirectionPass(h, a);
             aggUpdate(0, pNew);
-            if (twoLevel)
-                coarseFinish(0);
+            if (twoLevel && h.nAggregates > 0)
+                coarseRows(0);
+            finish(0);
         };
     }
     else
@@ -628,10 +769,11 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         // another address forces a new capture
         std::vector<void const*> const ptrs{
             R.y.ptr,           R.r.ptr,         R.z.ptr,       R.p.ptr,          R.p2.ptr,        R.q.ptr,
-            R.rhs.ptr,         R.hold.ptr,      R.dinv.ptr,    R.scal.ptr,       R.flags.ptr,     R.counters.ptr,
-            R.coarse[0].ptr,   R.coarse[1].ptr, R.coarseFinal.ptr, R.rc.ptr,     R.zc.ptr,        R.bsrRowPtr.ptr,
-            R.bsrCol.ptr,      R.bsrA.ptr,      R.bsrB.ptr,    R.bsrC.ptr,       h.meshOf.ptr,    h.aggOf.ptr,
-            h.aggVertPtr.ptr,  h.aggVerts.ptr,  h.aggMesh.ptr};
+            R.rhs.ptr,         R.hold.ptr,      R.dinv.ptr,    R.scal.ptr,       R.flags.ptr,     R.coarse[0].ptr,
+            R.coarse[1].ptr,   R.coarseFinal.ptr, R.rc.ptr,    R.zc.ptr,         R.bsrRowPtr.ptr, R.bsrCol.ptr,
+            R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
+            h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.meshChunkBegin.ptr,
+            R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();

@@ -106,8 +106,7 @@ enum AccArray : int
     kAccPq = 1,    // p . H p
     kAccRr = 2,    // r . r
     kAccBb = 3,    // b . b
-    kAccRcZc = 4,  // (Z^T r) . zc: the coarse part of r . z (two-level, fused iteration)
-    kAccCount = 5
+    kAccCount = 4
 };
 constexpr int kSpread = 16;
 constexpr int kSlotStride = 16;
@@ -149,11 +148,37 @@ struct CgState
     double tol2; // squared relative tolerance
 };
 
+// End of one CG iteration of mesh m (one thread) from its sums r . r, r . z
+// and, at the initialisation (first), b . b: the stopping test |r| <= tol |b|,
+// beta = rz / rz_old, rz rotated, the iteration counted. Returns whether the
+// mesh is still live.
+__device__ __forceinline__ bool FinishMesh(CgState const& c, int m, double rr, double rz, double bb, int first)
+{
+    int const nM = c.nM;
+    if (first)
+        c.P[kBb * nM + m] = bb;
+    int done = c.F[kDone * nM + m];
+    double const b2 = c.P[kBb * nM + m];
+    if (!done)
+    {
+        if (!first)
+            c.F[kIters * nM + m] += 1;
+        c.P[kRrLast * nM + m] = rr;
+        if (!(b2 > 0.) || rr <= c.tol2 * b2)
+            done = 1;
+    }
+    double const rzOld = c.P[kRz * nM + m];
+    c.P[kBeta * nM + m] = (!done && rzOld != 0.) ? rz / rzOld : 0.;
+    c.P[kRz * nM + m] = rz;
+    c.F[kDone * nM + m] = done;
+    return !done;
+}
+
 // Per mesh, after an iteration's accumulations (one workgroup, blockDim a
 // multiple of 64): each mesh's kSpread targets are summed by one half-wave in
-// a fixed order; then the stopping test |r| <= tol |b|, beta = rz_new / rz,
-// rz rotated, every accumulator of the mesh cleared, live meshes counted.
-// first: the initialisation (takes b.b, counts no iteration).
+// a fixed order, every accumulator of the mesh cleared, then FinishMesh and
+// the live meshes counted. first: the initialisation (takes b.b, counts no
+// iteration).
 __device__ __forceinline__ void FinishIteration(CgState const& c, int first)
 {
     static_assert(kSpread <= 32, "one half-wave per mesh");
@@ -171,7 +196,7 @@ __device__ __forceinline__ void FinishIteration(CgState const& c, int first)
         if (hl < kSpread)
         {
             rr = *AccSlot(c.A, kAccRr, nM, m, hl);
-            rz = *AccSlot(c.A, kAccRzNew, nM, m, hl) + *AccSlot(c.A, kAccRcZc, nM, m, hl);
+            rz = *AccSlot(c.A, kAccRzNew, nM, m, hl);
             bb = *AccSlot(c.A, kAccBb, nM, m, hl);
             for (int arr = 0; arr < kAccCount; ++arr)
                 *AccSlot(c.A, arr, nM, m, hl) = 0.;
@@ -183,25 +208,7 @@ __device__ __forceinline__ void FinishIteration(CgState const& c, int first)
             bb += __shfl_xor(bb, off);
         }
         if (hl == 0)
-        {
-            if (first)
-                c.P[kBb * nM + m] = bb;
-            int done = c.F[kDone * nM + m];
-            double const b2 = c.P[kBb * nM + m];
-            if (!done)
-            {
-                if (!first)
-                    c.F[kIters * nM + m] += 1;
-                c.P[kRrLast * nM + m] = rr;
-                if (!(b2 > 0.) || rr <= c.tol2 * b2)
-                    done = 1;
-            }
-            double const rzOld = c.P[kRz * nM + m];
-            c.P[kBeta * nM + m] = (!done && rzOld != 0.) ? rz / rzOld : 0.;
-            c.P[kRz * nM + m] = rz;
-            c.F[kDone * nM + m] = done;
-            count += done ? 0 : 1;
-        }
+            count += FinishMesh(c, m, rr, rz, bb, first) ? 1 : 0;
     }
     if (count)
         atomicAdd(&live, count);
@@ -628,9 +635,10 @@ __global__ __launch_bounds__(256) void CgDirection(float4* __restrict__ p, float
 
 // ---- two-level preconditioner: translations of vertex aggregates -----------
 // Coarse space Z: one 3-column block per aggregate (unit translations of its
-// vertices). The coarse operator Z^T H Z is assembled per mesh as a dense
-// block, inverted in place by Gauss-Jordan steps (SPD: no pivoting), and the
-// preconditioner adds Z (Z^T H Z)^-1 Z^T r to the block-Jacobi one.
+// vertices). The coarse operator Z^T H Z is formed per mesh as a dense block
+// from the assembled step Hessian (CoarseFromBsr), inverted in place by
+// Gauss-Jordan steps (SPD: no pivoting), and the preconditioner adds
+// Z (Z^T H Z)^-1 Z^T r to the block-Jacobi one.
 
 struct CoarseArgs
 {
@@ -640,133 +648,6 @@ struct CoarseArgs
     long long const* __restrict__ meshOffset; // first double of each mesh's dense block
     int nAgg;
 };
-
-__device__ __forceinline__ void Cross(double const a[3], double const b[3], double c[3])
-{
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
-}
-
-// Z^T H_e Z for every tet that spans several aggregates or holds a vertex that
-// is not updated (the other tets' blocks sum to zero: the elastic energy is
-// translation invariant). With gA, gB the sums of the shape-function gradient
-// rows of the tet's vertices in aggregates A and B,
-//   block(A, B) = w mu (gA . gB) I + w lambda (cof F gA)(cof F gB)^T
-//                 - w lambda (J - alpha) [F (gA x gB)]_x
-// (the SNH Hessian contracted with gA, gB). FP64 from the FP32 state.
-__global__ __launch_bounds__(256) void CoarseAssembleElastic(float4 const* __restrict__ xs,
-                                                             int4 const* __restrict__ tetv,
-                                                             float4 const* __restrict__ tgp, int nT, CoarseArgs c,
-                                                             double* __restrict__ Ac)
-{
-    int const e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (e >= nT)
-        return;
-    int4 const t = tetv[e];
-    int const vid[4] = {t.x, t.y, t.z, t.w};
-    int agg[4], gid[4], grp[4], ng = 0;
-    bool allIn = true;
-    for (int k = 0; k < 4; ++k)
-    {
-        agg[k] = c.aggOf[vid[k]];
-        grp[k] = -1;
-        if (agg[k] < 0)
-        {
-            allIn = false;
-            continue;
-        }
-        int j = 0;
-        while (j < ng && gid[j] != agg[k])
-            ++j;
-        if (j == ng)
-            gid[ng++] = agg[k];
-        grp[k] = j;
-    }
-    if (ng == 0 || (ng == 1 && allIn))
-        return;
-    float4 const q0 = tgp[3 * e], q1 = tgp[3 * e + 1], q2 = tgp[3 * e + 2];
-    double g[4][3] = {{0., 0., 0.},
-                      {q0.x, q0.y, q0.z},
-                      {q0.w, q1.x, q1.y},
-                      {q1.z, q1.w, q2.x}};
-    for (int d = 0; d < 3; ++d)
-        g[0][d] = -(g[1][d] + g[2][d] + g[3][d]);
-    double F[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}}; // F[row][col]
-    for (int k = 0; k < 4; ++k)
-    {
-        float4 const x = xs[vid[k]];
-        double const xk[3] = {x.x, x.y, x.z};
-        for (int r = 0; r < 3; ++r)
-            for (int cc = 0; cc < 3; ++cc)
-                F[r][cc] += xk[r] * g[k][cc];
-    }
-    double col[3][3], cof[3][3]; // columns
-    for (int cc = 0; cc < 3; ++cc)
-        for (int r = 0; r < 3; ++r)
-            col[cc][r] = F[r][cc];
-    Cross(col[1], col[2], cof[0]);
-    Cross(col[2], col[0], cof[1]);
-    Cross(col[0], col[1], cof[2]);
-    double const J = col[0][0] * cof[0][0] + col[0][1] * cof[0][1] + col[0][2] * cof[0][2];
-    double const wmu = q2.y, wlam = q2.z, s = double(q2.z) * (J - double(q2.w));
-    double gs[4][3];
-    for (int j = 0; j < ng; ++j)
-        gs[j][0] = gs[j][1] = gs[j][2] = 0.;
-    for (int k = 0; k < 4; ++k)
-        if (grp[k] >= 0)
-            for (int d = 0; d < 3; ++d)
-                gs[grp[k]][d] += g[k][d];
-    int const m = c.aggMesh[gid[0]];
-    int const a0 = c.meshAggBegin[m];
-    int const n = 3 * (c.meshAggBegin[m + 1] - a0);
-    double* const A = Ac + c.meshOffset[m];
-    for (int a = 0; a < ng; ++a)
-    {
-        double ca[3]; // cof F gA
-        for (int r = 0; r < 3; ++r)
-            ca[r] = cof[0][r] * gs[a][0] + cof[1][r] * gs[a][1] + cof[2][r] * gs[a][2];
-        for (int b = 0; b < ng; ++b)
-        {
-            double cb[3], w[3], fw[3];
-            for (int r = 0; r < 3; ++r)
-                cb[r] = cof[0][r] * gs[b][0] + cof[1][r] * gs[b][1] + cof[2][r] * gs[b][2];
-            Cross(gs[a], gs[b], w);
-            for (int r = 0; r < 3; ++r)
-                fw[r] = F[r][0] * w[0] + F[r][1] * w[1] + F[r][2] * w[2];
-            double const dg = wmu * (gs[a][0] * gs[b][0] + gs[a][1] * gs[b][1] + gs[a][2] * gs[b][2]);
-            // [fw]_x
-            double const sk[3][3] = {{0., -fw[2], fw[1]}, {fw[2], 0., -fw[0]}, {-fw[1], fw[0], 0.}};
-            int const ra = 3 * (gid[a] - a0), cbase = 3 * (gid[b] - a0);
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j)
-                {
-                    double const v = (i == j ? dg : 0.) + wlam * ca[i] * cb[j] - s * sk[i][j];
-                    atomicAdd(A + static_cast<size_t>(ra + i) * n + cbase + j, v);
-                }
-        }
-    }
-}
-
-// Inertia: diagonal += m_i / h^2 for every updated vertex
-__global__ __launch_bounds__(256) void CoarseAssembleMass(float4 const* __restrict__ xs, int nUpd, CoarseArgs c,
-                                                          double invDt2, double* __restrict__ Ac)
-{
-    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= nUpd)
-        return;
-    int const g = c.aggOf[i];
-    if (g < 0)
-        return;
-    int const m = c.aggMesh[g];
-    int const a0 = c.meshAggBegin[m];
-    int const n = 3 * (c.meshAggBegin[m + 1] - a0);
-    double* const A = Ac + c.meshOffset[m];
-    double const k = double(xs[i].w) * invDt2;
-    int const r = 3 * (g - a0);
-    for (int d = 0; d < 3; ++d)
-        atomicAdd(A + static_cast<size_t>(r + d) * n + r + d, k);
-}
 
 // Gauss-Jordan step k on every mesh block (blockIdx.y = mesh): Bk = step(Ak);
 // after n steps an n x n SPD block holds its inverse.
@@ -820,95 +701,6 @@ __global__ __launch_bounds__(256) void CoarseGuard(double* __restrict__ buf0, do
         return;
     for (int64_t k = threadIdx.x; k < static_cast<int64_t>(n) * n; k += blockDim.x)
         A[k] = 0.;
-}
-
-// zc = (Z^T H Z)^-1 rc per mesh: wpr waves (1, 2 or 4) per coarse row, so long
-// rows (large meshes) are read by more waves in parallel; blockDim = 256.
-__global__ __launch_bounds__(256) void CoarseApply(double const* __restrict__ buf0, double const* __restrict__ buf1,
-                                                   int const* __restrict__ finalBuf, double const* __restrict__ rc,
-                                                   double* __restrict__ zc, CoarseArgs c, int wpr)
-{
-    __shared__ double part[4];
-    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
-    int const row = static_cast<int>(blockIdx.x) * (4 / wpr) + wave / wpr;
-    int const sub = wave % wpr;
-    double sum = 0.;
-    if (row < 3 * c.nAgg)
-    {
-        int const m = c.aggMesh[row / 3];
-        int const b0 = 3 * c.meshAggBegin[m];
-        int const n = 3 * c.meshAggBegin[m + 1] - b0;
-        double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
-        for (int j = sub * 64 + lane; j < n; j += 64 * wpr)
-            sum += A[j] * rc[b0 + j];
-    }
-    for (int off = 32; off > 0; off >>= 1)
-        sum += __shfl_xor(sum, off);
-    if (lane == 0)
-        part[wave] = sum;
-    __syncthreads();
-    if (lane == 0 && sub == 0 && row < 3 * c.nAgg)
-    {
-        double t = part[wave];
-        for (int k = 1; k < wpr; ++k)
-            t += part[wave + k];
-        zc[row] = t;
-    }
-}
-
-// rc = Z^T r: one wave per aggregate sums r over the aggregate's vertices.
-__global__ __launch_bounds__(256) void CoarseRestrict(float4 const* __restrict__ r, int const* __restrict__ ptr,
-                                                      int const* __restrict__ verts, int nAgg,
-                                                      double* __restrict__ rc)
-{
-    int const g = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    int const lane = static_cast<int>(threadIdx.x & 63);
-    if (g >= nAgg)
-        return;
-    double sx = 0., sy = 0., sz = 0.;
-    for (int k = ptr[g] + lane; k < ptr[g + 1]; k += 64)
-    {
-        float4 const ri = r[verts[k]];
-        sx += ri.x, sy += ri.y, sz += ri.z;
-    }
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        sx += __shfl_xor(sx, off);
-        sy += __shfl_xor(sy, off);
-        sz += __shfl_xor(sz, off);
-    }
-    if (lane == 0)
-    {
-        rc[3 * g] = sx;
-        rc[3 * g + 1] = sy;
-        rc[3 * g + 2] = sz;
-    }
-}
-
-// z += Z zc (each vertex its aggregate's translation); r.z accumulated per mesh.
-__global__ __launch_bounds__(256) void CoarseProlong(float4* __restrict__ z, float4 const* __restrict__ r, int n,
-                                                     CoarseArgs c, double const* __restrict__ zc,
-                                                     int const* __restrict__ meshOf, int nM, double* __restrict__ A)
-{
-    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    double rz = 0.;
-    int m = -1;
-    if (i < n)
-    {
-        m = meshOf[i];
-        int const g = c.aggOf[i];
-        float4 zi = z[i];
-        if (g >= 0)
-        {
-            zi.x += static_cast<float>(zc[3 * g]);
-            zi.y += static_cast<float>(zc[3 * g + 1]);
-            zi.z += static_cast<float>(zc[3 * g + 2]);
-            z[i] = zi;
-        }
-        float4 const ri = r[i];
-        rz = Dot3(ri, zi);
-    }
-    BlockAccumulate(rz, m, A, kAccRzNew, nM);
 }
 
 __global__ __launch_bounds__(256) void ZeroDoublesLong(double* __restrict__ p, long long n)

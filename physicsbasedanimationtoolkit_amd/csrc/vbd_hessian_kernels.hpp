@@ -117,36 +117,14 @@ __global__ __launch_bounds__(256) void BsrAssemble(float4 const* __restrict__ xs
     v.c[k] = static_cast<float>(H[2][2]);
 }
 
-// Returns true in every thread of the workgroup that finishes last among the
-// launch's workgroups (each one calls this once, after its own atomics and
-// stores). The counter returns to 0 for the next launch. Release before the
-// ticket, acquire after it: the last workgroup sees every other one's writes.
-__device__ __forceinline__ bool LastWorkgroup(unsigned int* counter)
-{
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0)
-    {
-        __threadfence();
-        unsigned int const ticket = atomicAdd(counter, 1u);
-        last = (ticket == gridDim.x - 1) ? 1 : 0;
-        if (last)
-            atomicExch(counter, 0u);
-    }
-    __syncthreads();
-    if (last)
-        __threadfence();
-    return last != 0;
-}
-
 enum class BsrMode : int
 {
     kProduct = 0,        // out = H y
     kProductDot = 1,     // out = H y, acc[mesh] += y . H y
     kGaussSeidel = 2,    // rows of one colour: y_i += D_i^-1 (b_i - (H y)_i)
     kGaussSeidelDot = 3, // same, then acc[mesh] += b_i . y_i
-    kDirectionDot = 4    // p = z + Z zc + beta p_old formed on the fly (written to y);
-                         // out = H p, acc[mesh] += p . H p
+    kDirection = 4       // p = z + Z zc + beta p_old formed on the fly (written to y);
+                         // out = H p, p . H p per row chunk (BsrDirection)
 };
 
 struct BsrArgs
@@ -161,7 +139,7 @@ struct BsrArgs
     float4* __restrict__ out;        // product
     float4 const* __restrict__ dinv; // inverse diagonal blocks (Gauss-Seidel)
     int const* __restrict__ meshOf;
-    // kDirectionDot: p_j = z_j + zc[aggOf j] + beta p_old_j of mesh m (p_old_j when m is done)
+    // kDirection: p_j = z_j + zc[aggOf j] + beta p_old_j of mesh m (p_old_j when m is done)
     float4 const* __restrict__ z;
     float4 const* __restrict__ pOld;
     double const* __restrict__ zc; // nullable (block Jacobi): no coarse term
@@ -176,7 +154,7 @@ struct BsrArgs
 template <BsrMode M>
 __device__ __forceinline__ float4 BsrVector(BsrArgs const& a, int j, float beta, bool frozen)
 {
-    if constexpr (M == BsrMode::kDirectionDot)
+    if constexpr (M == BsrMode::kDirection)
     {
         float4 const po = a.pOld[j];
         if (frozen)
@@ -198,40 +176,12 @@ __device__ __forceinline__ float4 BsrVector(BsrArgs const& a, int j, float beta,
     }
 }
 
-// L lanes per row of [vbeg, vend); lane l takes blocks l, l + L, ... of the row,
-// the first K of them loaded together before accumulating.
+// (H v)_row over blocks [kb, ke) of one row: lane l of the row's L lanes takes
+// blocks l, l + L, ..., the first K of them loaded together before
+// accumulating; the group sum leaves the product in every lane of the group.
 template <int L, int K, BsrMode M>
-__global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
+__device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, int lane, float beta, bool frozen)
 {
-    constexpr bool kGS = M == BsrMode::kGaussSeidel || M == BsrMode::kGaussSeidelDot;
-    constexpr bool kDot = M == BsrMode::kGaussSeidelDot || M == BsrMode::kDirectionDot || M == BsrMode::kProductDot;
-    constexpr bool kDir = M == BsrMode::kDirectionDot;
-    if constexpr (kDir)
-    {
-        if (a.cg.F[2 * a.cg.nM] == 0)
-            return; // every mesh has converged: nothing left to do
-    }
-    int block = static_cast<int>(blockIdx.x);
-    if (a.xcdChunk > 0)
-        block = (block & 7) * a.xcdChunk + (block >> 3);
-    int const tid = block * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-    int const lane = tid & (L - 1);
-    int const row = a.vbeg + tid / L;
-    bool const active = row < a.vend;
-    int kb = 0, ke = 0, m = 0;
-    float beta = 0.f;
-    bool frozen = false;
-    if (active)
-    {
-        kb = a.rowPtr[row];
-        ke = a.rowPtr[row + 1];
-        if constexpr (kDir)
-        {
-            m = a.meshOf[row];
-            frozen = a.cg.F[kDone * a.cg.nM + m] != 0;
-            beta = static_cast<float>(a.cg.P[kBeta * a.cg.nM + m]);
-        }
-    }
     float hx = 0.f, hy = 0.f, hz = 0.f;
     int cj[K];
     float4 A[K], B[K];
@@ -271,9 +221,30 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
         hy += Ak.w * Pk.x + Bk.x * Pk.y + Bk.y * Pk.z;
         hz += Bk.z * Pk.x + Bk.w * Pk.y + Ck * Pk.z;
     }
-    hx = GroupSum<L>(hx);
-    hy = GroupSum<L>(hy);
-    hz = GroupSum<L>(hz);
+    return make_float3(GroupSum<L>(hx), GroupSum<L>(hy), GroupSum<L>(hz));
+}
+
+// L lanes per row of [vbeg, vend): products and Gauss-Seidel passes.
+template <int L, int K, BsrMode M>
+__global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
+{
+    static_assert(M != BsrMode::kDirection, "the direction product runs over row chunks (BsrDirection)");
+    constexpr bool kGS = M == BsrMode::kGaussSeidel || M == BsrMode::kGaussSeidelDot;
+    constexpr bool kDot = M == BsrMode::kGaussSeidelDot || M == BsrMode::kProductDot;
+    int block = static_cast<int>(blockIdx.x);
+    if (a.xcdChunk > 0)
+        block = (block & 7) * a.xcdChunk + (block >> 3);
+    int const tid = block * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
+    int const lane = tid & (L - 1);
+    int const row = a.vbeg + tid / L;
+    bool const active = row < a.vend;
+    int kb = 0, ke = 0;
+    if (active)
+    {
+        kb = a.rowPtr[row];
+        ke = a.rowPtr[row + 1];
+    }
+    float3 const h = RowProduct<L, K, M>(a, kb, ke, lane, 0.f, false);
     bool const leader = active && lane == 0;
     double dot = 0.;
     if (leader)
@@ -281,28 +252,21 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
         if constexpr (kGS)
         {
             float4 const bo = a.b[row], yo = a.y[row];
-            float const rx = bo.x - hx, ry = bo.y - hy, rz = bo.z - hz;
-            float4 const yn = make_float4(yo.x, yo.y, yo.z, 0.f);
+            float const rx = bo.x - h.x, ry = bo.y - h.y, rz = bo.z - h.z;
             float4 const D0 = a.dinv[2 * row], D1 = a.dinv[2 * row + 1];
-            float4 const r = make_float4(yn.x + D0.x * rx + D0.y * ry + D0.z * rz,
-                                         yn.y + D0.y * rx + D0.w * ry + D1.x * rz,
-                                         yn.z + D0.z * rx + D1.x * ry + D1.y * rz, 0.f);
+            float4 const r = make_float4(yo.x + D0.x * rx + D0.y * ry + D0.z * rz,
+                                         yo.y + D0.y * rx + D0.w * ry + D1.x * rz,
+                                         yo.z + D0.z * rx + D1.x * ry + D1.y * rz, 0.f);
             a.y[row] = r;
             dot = double(bo.x) * r.x + double(bo.y) * r.y + double(bo.z) * r.z;
         }
         else
         {
-            a.out[row] = make_float4(hx, hy, hz, 0.f);
+            a.out[row] = make_float4(h.x, h.y, h.z, 0.f);
             if constexpr (M == BsrMode::kProductDot)
             {
                 float4 const yr = a.y[row];
-                dot = double(yr.x) * hx + double(yr.y) * hy + double(yr.z) * hz;
-            }
-            if constexpr (kDir)
-            {
-                float4 const pr = BsrVector<M>(a, row, beta, frozen);
-                a.y[row] = pr; // the direction of this iteration (the p_old of the next)
-                dot = double(pr.x) * hx + double(pr.y) * hy + double(pr.z) * hz;
+                dot = double(yr.x) * h.x + double(yr.y) * h.y + double(yr.z) * h.z;
             }
         }
     }
@@ -310,15 +274,88 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
         BlockAccumulate(dot, leader ? a.meshOf[row] : -1, a.acc, a.accArray, a.cg.nM);
 }
 
-// Per mesh m with alpha = rz / pq (0 once converged), for the vertices of each
-// aggregate (one wave per aggregate, lanes over its vertex list):
+// Row chunks of the fused CG iteration: each (mesh, colour) run of rows is cut
+// into chunks of at most 256 / L rows, numbered mesh-major, so the chunks of
+// mesh m are [meshChunkBegin[m], meshChunkBegin[m + 1]) and every chunk holds
+// rows of one mesh. A chunk's sums depend only on its own rows, so a mesh gets
+// the same partial sums whether it is stepped alone or in a batch.
+//
+// One workgroup per chunk (XCD-aware order when xcdChunk > 0): the direction
+// p = z + Z zc + beta p_old of each of its rows is formed on the fly for every
+// block (p_old when the mesh has converged), written to y, q = H p to out, and
+// the chunk's p . q, summed over its rows in order, to pqPart[chunk].
+template <int L, int K>
+__global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __restrict__ chunks, int nChunks,
+                                                    double* __restrict__ pqPart)
+{
+    constexpr int R = 256 / L;
+    int const nM = a.cg.nM;
+    if (a.cg.F[2 * nM] == 0)
+        return; // every mesh has converged: nothing left to do
+    int w = static_cast<int>(blockIdx.x);
+    if (a.xcdChunk > 0)
+        w = (w & 7) * a.xcdChunk + (w >> 3);
+    if (w >= nChunks)
+        return; // padding of the XCD order
+    int2 const ch = chunks[w];
+    int const r = static_cast<int>(threadIdx.x) / L, lane = static_cast<int>(threadIdx.x) & (L - 1);
+    int const row = ch.x + r;
+    bool const active = row < ch.y;
+    int const m = a.meshOf[ch.x];
+    bool const frozen = a.cg.F[kDone * nM + m] != 0;
+    float const beta = static_cast<float>(a.cg.P[kBeta * nM + m]);
+    int kb = 0, ke = 0;
+    if (active)
+    {
+        kb = a.rowPtr[row];
+        ke = a.rowPtr[row + 1];
+    }
+    float3 const h = RowProduct<L, K, BsrMode::kDirection>(a, kb, ke, lane, beta, frozen);
+    __shared__ double sdot[R];
+    if (lane == 0)
+    {
+        double dot = 0.;
+        if (active)
+        {
+            float4 const pr = BsrVector<BsrMode::kDirection>(a, row, beta, frozen);
+            a.y[row] = pr; // the direction of this iteration (the p_old of the next)
+            a.out[row] = make_float4(h.x, h.y, h.z, 0.f);
+            dot = double(pr.x) * h.x + double(pr.y) * h.y + double(pr.z) * h.z;
+        }
+        sdot[r] = dot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double s = 0.;
+        for (int k = 0; k < R; ++k)
+            s += sdot[k];
+        pqPart[w] = s;
+    }
+}
+
+// Sum over the 128 threads of a workgroup in a fixed order (both waves'
+// butterflies, then wave 0 + wave 1); every thread calls it, thread 0 gets it.
+__device__ __forceinline__ double Sum128(double v, double* __restrict__ lds)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return lds[0] + lds[1];
+}
+
+// One workgroup of 128 threads per aggregate g of mesh m, alpha = rz / pq with
+// pq the sum of m's chunk partials in order (0 once m has converged); over the
+// aggregate's vertices:
 //   init:   r = b - q               (q = H y0)
 //   update: y += alpha p, r -= alpha q
-// then z = D^-1 r, the aggregate's restriction rc = sum r (FP64, when rc) and
-// the mesh's r . z and r . r (and b . b at init) accumulated. Every updated
-// vertex belongs to exactly one aggregate and every aggregate to one mesh.
-// With finish (block Jacobi: no coarse step follows), the last workgroup ends
-// the iteration (FinishIteration).
+// then z = D^-1 r, and the aggregate's r . z, r . r (and b . b at init) and,
+// when rc is given, its restriction rc = sum r, summed in a fixed order and
+// written to the per-aggregate partials. Every updated vertex belongs to
+// exactly one aggregate and every aggregate to one mesh.
 struct AggUpdateArgs
 {
     int const* __restrict__ aggPtr;
@@ -332,70 +369,72 @@ struct AggUpdateArgs
     float4 const* __restrict__ b; // init only
     float4 const* __restrict__ dinv;
     double* __restrict__ rc;      // nullable
+    double const* __restrict__ pqPart;
+    int const* __restrict__ meshChunkBegin;
+    double* __restrict__ rzPart;
+    double* __restrict__ rrPart;
+    double* __restrict__ bbPart; // init only
     CgState cg;
-    unsigned int* counter;        // finish only
-    int nAgg;
     int init;
-    int finish;
 };
 
-__global__ __launch_bounds__(256) void AggUpdate(AggUpdateArgs u)
+__global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 {
     int const nM = u.cg.nM;
     if (!u.init && u.cg.F[2 * nM] == 0)
         return; // every mesh has converged
-    int const g = static_cast<int>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-    int const lane = static_cast<int>(threadIdx.x & 63);
+    __shared__ double lds[2];
+    int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
+    int const m = u.aggMesh[g];
+    float alpha = 0.f;
+    if (!u.init)
+    {
+        double s = 0.;
+        for (int k = u.meshChunkBegin[m] + t; k < u.meshChunkBegin[m + 1]; k += 128)
+            s += u.pqPart[k];
+        double const pq = Sum128(s, lds);
+        bool const done = u.cg.F[kDone * nM + m] != 0;
+        alpha = (!done && pq != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq) : 0.f;
+    }
     double sx = 0., sy = 0., sz = 0., rz = 0., rr = 0., bb = 0.;
-    int m = -1;
-    if (g < u.nAgg)
+    for (int k = u.aggPtr[g] + t; k < u.aggPtr[g + 1]; k += 128)
     {
-        m = u.aggMesh[g];
-        float alpha = 0.f;
-        if (!u.init)
+        int const i = u.aggVerts[k];
+        float4 const qi = u.q[i];
+        float4 ri;
+        if (u.init)
         {
-            double const pq = AccSum(u.cg.A, kAccPq, nM, m);
-            bool const done = u.cg.F[kDone * nM + m] != 0;
-            alpha = (!done && pq != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq) : 0.f;
+            float4 const bi = u.b[i];
+            ri = make_float4(bi.x - qi.x, bi.y - qi.y, bi.z - qi.z, 0.f);
+            bb += Dot3(bi, bi);
         }
-        for (int k = u.aggPtr[g] + lane; k < u.aggPtr[g + 1]; k += 64)
+        else
         {
-            int const i = u.aggVerts[k];
-            float4 const qi = u.q[i];
-            float4 ri;
-            if (u.init)
-            {
-                float4 const bi = u.b[i];
-                ri = make_float4(bi.x - qi.x, bi.y - qi.y, bi.z - qi.z, 0.f);
-                bb += Dot3(bi, bi);
-            }
-            else
-            {
-                float4 const pi = u.p[i];
-                float4 yi = u.y[i];
-                ri = u.r[i];
-                yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
-                ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
-                u.y[i] = yi;
-            }
-            u.r[i] = ri;
-            float4 const zi = ApplyInverse(u.dinv, i, ri);
-            u.z[i] = zi;
-            sx += ri.x, sy += ri.y, sz += ri.z;
-            rz += Dot3(ri, zi);
-            rr += Dot3(ri, ri);
+            float4 const pi = u.p[i];
+            float4 yi = u.y[i];
+            ri = u.r[i];
+            yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
+            ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
+            u.y[i] = yi;
         }
+        u.r[i] = ri;
+        float4 const zi = ApplyInverse(u.dinv, i, ri);
+        u.z[i] = zi;
+        sx += ri.x, sy += ri.y, sz += ri.z;
+        rz += Dot3(ri, zi);
+        rr += Dot3(ri, ri);
     }
-    for (int off = 32; off > 0; off >>= 1)
+    rz = Sum128(rz, lds);
+    rr = Sum128(rr, lds);
+    if (u.init)
+        bb = Sum128(bb, lds);
+    if (u.rc)
     {
-        sx += __shfl_xor(sx, off);
-        sy += __shfl_xor(sy, off);
-        sz += __shfl_xor(sz, off);
-        rz += __shfl_xor(rz, off);
-        rr += __shfl_xor(rr, off);
-        bb += __shfl_xor(bb, off);
+        sx = Sum128(sx, lds);
+        sy = Sum128(sy, lds);
+        sz = Sum128(sz, lds);
     }
-    if (lane == 0 && m >= 0)
+    if (t == 0)
     {
         if (u.rc)
         {
@@ -403,25 +442,21 @@ __global__ __launch_bounds__(256) void AggUpdate(AggUpdateArgs u)
             u.rc[3 * g + 1] = sy;
             u.rc[3 * g + 2] = sz;
         }
-        int const slot = g & (kSpread - 1);
-        atomicAdd(AccSlot(u.cg.A, kAccRzNew, nM, m, slot), rz);
-        atomicAdd(AccSlot(u.cg.A, kAccRr, nM, m, slot), rr);
+        u.rzPart[g] = rz;
+        u.rrPart[g] = rr;
         if (u.init)
-            atomicAdd(AccSlot(u.cg.A, kAccBb, nM, m, slot), bb);
+            u.bbPart[g] = bb;
     }
-    if (u.finish && LastWorkgroup(u.counter))
-        FinishIteration(u.cg, u.init);
 }
 
-// zc = (Z^T H Z)^-1 rc per mesh (wpr waves per coarse row, as CoarseApply), the
-// coarse part rc . zc of each mesh's r . z, then the last workgroup ends the
-// iteration (FinishIteration: r . z = (r . D^-1 r) + rc . zc, stopping test, beta).
-__global__ __launch_bounds__(256) void CoarseApplyFinish(double const* __restrict__ buf0,
-                                                         double const* __restrict__ buf1,
-                                                         int const* __restrict__ finalBuf,
-                                                         double const* __restrict__ rc, double* __restrict__ zc,
-                                                         CoarseArgs c, int wpr, CgState cg,
-                                                         unsigned int* __restrict__ counter, int first)
+// zc = (Z^T H Z)^-1 rc per mesh (wpr waves per coarse row) and each coarse
+// row's share of rc . zc, the coarse part of the mesh's r . z, into rczcPart.
+__global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict__ buf0,
+                                                       double const* __restrict__ buf1,
+                                                       int const* __restrict__ finalBuf,
+                                                       double const* __restrict__ rc, double* __restrict__ zc,
+                                                       CoarseArgs c, int wpr, CgState cg,
+                                                       double* __restrict__ rczcPart, int first)
 {
     if (!first && cg.F[2 * cg.nM] == 0)
         return; // every mesh has converged
@@ -430,10 +465,9 @@ __global__ __launch_bounds__(256) void CoarseApplyFinish(double const* __restric
     int const row = static_cast<int>(blockIdx.x) * (4 / wpr) + wave / wpr;
     int const sub = wave % wpr;
     double sum = 0.;
-    int m = -1;
     if (row < 3 * c.nAgg)
     {
-        m = c.aggMesh[row / 3];
+        int const m = c.aggMesh[row / 3];
         int const b0 = 3 * c.meshAggBegin[m];
         int const n = 3 * c.meshAggBegin[m + 1] - b0;
         double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
@@ -451,10 +485,98 @@ __global__ __launch_bounds__(256) void CoarseApplyFinish(double const* __restric
         for (int k = 1; k < wpr; ++k)
             t += part[wave + k];
         zc[row] = t;
-        atomicAdd(AccSlot(cg.A, kAccRcZc, cg.nM, m, row & (kSpread - 1)), t * rc[row]);
+        rczcPart[row] = t * rc[row];
     }
-    if (LastWorkgroup(counter))
-        FinishIteration(cg, first);
+}
+
+// End of an iteration of the fused CG (one workgroup of 256 threads, one wave
+// per mesh at a time): each mesh's partials summed in a fixed order, r . z =
+// sum of the aggregates' r . D^-1 r (+ the coarse rows' rc . zc), then the
+// stopping test and beta (FinishMesh).
+struct CgFinishArgs
+{
+    CgState cg;
+    int const* __restrict__ meshAggBegin;
+    double const* __restrict__ rzPart;
+    double const* __restrict__ rrPart;
+    double const* __restrict__ bbPart;
+    double const* __restrict__ rczcPart; // nullable (block Jacobi)
+};
+
+__global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
+{
+    int const nM = f.cg.nM;
+    if (!first && f.cg.F[2 * nM] == 0)
+        return; // every mesh has converged
+    __shared__ int live;
+    if (threadIdx.x == 0)
+        live = 0;
+    __syncthreads();
+    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+    int count = 0;
+    for (int m = wave; m < nM; m += 4)
+    {
+        int const a0 = f.meshAggBegin[m], a1 = f.meshAggBegin[m + 1];
+        double rz = 0., rr = 0., bb = 0.;
+        for (int k = a0 + lane; k < a1; k += 64)
+        {
+            rz += f.rzPart[k];
+            rr += f.rrPart[k];
+            if (first)
+                bb += f.bbPart[k];
+        }
+        if (f.rczcPart)
+            for (int k = 3 * a0 + lane; k < 3 * a1; k += 64)
+                rz += f.rczcPart[k];
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            rz += __shfl_xor(rz, off);
+            rr += __shfl_xor(rr, off);
+            bb += __shfl_xor(bb, off);
+        }
+        if (lane == 0)
+            count += FinishMesh(f.cg, m, rr, rz, bb, first) ? 1 : 0;
+    }
+    if (count)
+        atomicAdd(&live, count);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        f.cg.F[2 * nM] = live;
+}
+
+// Galerkin coarse operator Z^T H Z from the assembled blocks: coarse block k
+// (aggregates cbA[k], cbB[k] of one mesh) is the sum, in ascending order, of
+// the fine blocks H_ij with i in cbA[k] and j in cbB[k]; one thread per coarse
+// block writes its 3x3 entries into the mesh's dense row-major matrix (zeroed
+// beforehand), so the operator is the same on every run.
+__global__ __launch_bounds__(256) void CoarseFromBsr(int const* __restrict__ cbA, int const* __restrict__ cbB,
+                                                     int const* __restrict__ cbSrcPtr,
+                                                     int const* __restrict__ cbSrc, int nCoarseBlocks,
+                                                     float4 const* __restrict__ va, float4 const* __restrict__ vb,
+                                                     float const* __restrict__ vc, CoarseArgs c,
+                                                     double* __restrict__ Ac)
+{
+    int const k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k >= nCoarseBlocks)
+        return;
+    double s[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    for (int q = cbSrcPtr[k]; q < cbSrcPtr[k + 1]; ++q)
+    {
+        int const f = cbSrc[q];
+        float4 const A = va[f], B = vb[f];
+        s[0] += A.x, s[1] += A.y, s[2] += A.z;
+        s[3] += A.w, s[4] += B.x, s[5] += B.y;
+        s[6] += B.z, s[7] += B.w, s[8] += vc[f];
+    }
+    int const ga = cbA[k], gb = cbB[k];
+    int const m = c.aggMesh[ga];
+    int const a0 = c.meshAggBegin[m];
+    int const n = 3 * (c.meshAggBegin[m + 1] - a0);
+    double* const A = Ac + c.meshOffset[m];
+    int const ra = 3 * (ga - a0), cb = 3 * (gb - a0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            A[static_cast<size_t>(ra + i) * n + cb + j] = s[3 * i + j];
 }
 
 } // namespace pbat_vbd::device
