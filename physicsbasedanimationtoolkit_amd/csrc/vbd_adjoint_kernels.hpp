@@ -11,7 +11,7 @@
 namespace pbat_vbd::device {
 
 // Directional derivative of one incidence's elastic gradient: with
-// dF = sum_k (p_k - p_i) g_k^T (the own frame, as AccumulateOwnFrame),
+// dF = sum_k (p_k - p_i) g_k^T (the own frame: F = sum_k (x_k - x_i) g_k^T),
 //   (H_e p)_i = [w mu dF + w lambda (cof F : dF) cof F + w lambda (J - alpha) dcof(F)[dF]] g
 // where dcof's columns are df1 x f2 + f1 x df2, df2 x f0 + f2 x df0,
 // df0 x f1 + f0 x df1. Adds the block-diagonal part w (mu |g|^2 I + lambda c c^T)
@@ -738,13 +738,14 @@ __global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)
 }
 
 // Rescale per-tet Lame coefficients to Young's moduli exp(logY) (nu fixed):
-// w*mu and w*lambda of the tet constants and of its incidence records
-// (w*mu in the records' id words, w*lambda in their last float pair).
+// w*mu and w*lambda of the tet constants and of its incidence records (w*mu in
+// the records' id words, w*lambda in the .y of their g2 and mat pairs).
 __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ logY, float2 const* __restrict__ base,
                                                    float4* __restrict__ tgp, int4 const* __restrict__ slots,
                                                    int4* __restrict__ ids, float2* __restrict__ g2,
-                                                   int4 const* __restrict__ slotsEll, int4* __restrict__ idsEll,
-                                                   float2* __restrict__ g2Ell, int nT)
+                                                   float2* __restrict__ mat, int4 const* __restrict__ slotsEll,
+                                                   int4* __restrict__ idsEll, float2* __restrict__ g2Ell,
+                                                   float2* __restrict__ matEll, int nT)
 {
     int const e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= nT)
@@ -755,7 +756,7 @@ __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ log
     float4 t = tgp[3 * e + 2];
     t.y = wmu, t.z = wlam;
     tgp[3 * e + 2] = t;
-    auto update = [&](int4 const& sl, int4* I, float2* G) {
+    auto update = [&](int4 const& sl, int4* I, float2* G, float2* Mt) {
         int const s4[4] = {sl.x, sl.y, sl.z, sl.w};
         for (int k = 0; k < 4; ++k)
             if (s4[k] >= 0)
@@ -764,11 +765,12 @@ __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ log
                 r.w = __float_as_int(wmu);
                 I[s4[k]] = r;
                 G[s4[k]].y = wlam;
+                Mt[s4[k]].y = wlam;
             }
     };
-    update(slots[e], ids, g2);
+    update(slots[e], ids, g2, mat);
     if (g2Ell)
-        update(slotsEll[e], idsEll, g2Ell);
+        update(slotsEll[e], idsEll, g2Ell, matEll);
 }
 
 } // namespace pbat_vbd::device

@@ -136,9 +136,8 @@ device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2)
     pbat_vbd::IncidenceRecords const& r = (h.useEll && h.ellStride > 0) ? h.ell : h.rows;
     a.vtp = h.vtp.ptr;
     a.ids = r.ids.ptr;
-    a.g0 = r.g0.ptr;
-    a.g1 = r.g1.ptr;
-    a.g2 = r.g2.ptr;
+    a.geo = r.geo.ptr;
+    a.mat = r.mat.ptr;
     a.xm = h.xm.ptr;
     a.xtil = h.xtil.ptr;
     a.xt = h.xt.ptr;
@@ -836,14 +835,20 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         double const* G = d->GP + 12 * e; // G(r, c) = G[4c + r]
         int ids[3];
         float g[3][3];
+        double gd[3][3], own[3];
         int n = 0;
+        for (int c = 0; c < 3; ++c)
+            own[c] = G[4 * c + il];
         for (int j = 0; j < 4; ++j)
         {
             if (j == il)
                 continue;
             ids[n] = h->iperm[d->E[4 * e + j]];
             for (int c = 0; c < 3; ++c)
-                g[n][c] = static_cast<float>(G[4 * c + j]);
+            {
+                gd[n][c] = G[4 * c + j];
+                g[n][c] = static_cast<float>(gd[n][c]);
+            }
             ++n;
         }
         float4 const m = G4[3 * e + 2]; // (., w*mu, w*lambda, alpha)
@@ -853,6 +858,15 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         R.g0[slot] = make_float4(g[0][0], g[0][1], g[0][2], g[1][0]);
         R.g1[slot] = make_float4(g[1][1], g[1][2], g[2][0], g[2][1]);
         R.g2[slot] = make_float2(g[2][2], m.z);
+        // sweep form (FP64 from the FP64 GP): g_k . g of the other rows with the
+        // own row g, |g|^2, and det of the other three rows
+        auto dot = [](double const* a, double const* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        double const detG = gd[0][0] * (gd[1][1] * gd[2][2] - gd[1][2] * gd[2][1]) -
+                            gd[0][1] * (gd[1][0] * gd[2][2] - gd[1][2] * gd[2][0]) +
+                            gd[0][2] * (gd[1][0] * gd[2][1] - gd[1][1] * gd[2][0]);
+        R.geo[slot] = make_float4(static_cast<float>(dot(gd[0], own)), static_cast<float>(dot(gd[1], own)),
+                                  static_cast<float>(dot(gd[2], own)), static_cast<float>(dot(own, own)));
+        R.mat[slot] = make_float2(static_cast<float>(detG), m.z);
     };
     {
         HostIncidenceRecords R;

@@ -7,9 +7,9 @@
 //   xt[v]    float4 (x_t, 0)          positions at the start of the substep
 //   vel[v], vprev[v], aext[v]  float4 velocities and external acceleration
 //   vtp[v]   int32 CSR prefix over vertex-tet incidences
-//   incidence records (56 B, structure of arrays, in sweep order): the other
-//            three vertex ids of the tet with their GP rows and the tet's
-//            w*mu, w*lambda (see SweepArgs)
+//   incidence records (40 B, structure of arrays, in sweep order): the other
+//            three vertex ids of the tet, rest-shape constants of its GP rows
+//            and the tet's w*mu, w*lambda (see SweepArgs)
 //   tgp[3e..3e+2]  float4 x3 (48 B) per tet: rows 1..3 of the shape function
 //            gradients GP (row 0 = -(row1+row2+row3)), w*mu, w*lambda, alpha
 //   tetv[e]  int4 vertex ids of tet e
@@ -76,52 +76,40 @@ __device__ __forceinline__ void GroupSum(Block& B)
 }
 
 // Elastic contribution of one incident tet to vertex i's block (Stable
-// Neo-Hookean, physics/StableNeoHookeanEnergy.h), in the own-vertex frame.
-// With a, b, c the tet's other vertices and ga, gb, gc their rows of the
-// shape function gradients GP (so the own row is g = -(ga + gb + gc)),
-//   F = (xa - x) ga^T + (xb - x) gb^T + (xc - x) gc^T,
-// which equals x_e GP because the rows of GP sum to zero. With c = cof(F) g,
+// Neo-Hookean, physics/StableNeoHookeanEnergy.h), with c = cof(F) g:
 //   grad_i += w mu F g + w lambda (J - alpha) c
 //   Hess_i += w mu |g|^2 I + w lambda c c^T
-// The d2J/dF2 term of the SNH Hessian contracts to zero for one vertex:
-// det(F0 + x_i g^T) is affine in x_i.
-// alpha = 1 + mu/lambda, so w lambda (J - alpha) = w lambda (J - 1) - w mu.
-__device__ __forceinline__ void AccumulateOwnFrame(float4 const& q0, float4 const& q1, float2 const& q2, float wmu,
-                                                   float4 const& own, float4 const& pa, float4 const& pb,
-                                                   float4 const& pc, Block& B)
+// (the d2J/dF2 term of the SNH Hessian contracts to zero for one vertex:
+// det(F0 + x_i g^T) is affine in x_i). alpha = 1 + mu/lambda, so
+// w lambda (J - alpha) = w lambda (J - 1) - w mu.
+// With a, b, c the tet's other vertices, d_k = x_k - x, G the matrix of their
+// shape-function gradient rows g_a, g_b, g_c and g = -(g_a + g_b + g_c) the own
+// row, F = D G with D = [d_a d_b d_c], so
+//   F g = sum_k (g_k . g) d_k,   J = det D det G = det G (d_a . n),
+//   c = cof(D) cof(G) g = -det G n,
+// where n = (x_b - x_a) x (x_c - x_a) (the column sum of cof D) and G^-T g =
+// -(1, 1, 1). The record holds geo = (g_a . g, g_b . g, g_c . g, |g|^2) and
+// mat = (det G, w lambda), all rest-shape constants computed in FP64.
+__device__ __forceinline__ void AccumulateElastic(float4 const& geo, float2 const& mat, float wmu, float4 const& own,
+                                                  float4 const& pa, float4 const& pb, float4 const& pc, Block& B)
 {
-    float const gax = q0.x, gay = q0.y, gaz = q0.z;
-    float const gbx = q0.w, gby = q1.x, gbz = q1.y;
-    float const gcx = q1.z, gcy = q1.w, gcz = q2.x;
-    float const wlam = q2.y;
     float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
     float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
     float const dcx = pc.x - own.x, dcy = pc.y - own.y, dcz = pc.z - own.z;
-    float const f0x = dax * gax + dbx * gbx + dcx * gcx;
-    float const f0y = day * gax + dby * gbx + dcy * gcx;
-    float const f0z = daz * gax + dbz * gbx + dcz * gcx;
-    float const f1x = dax * gay + dbx * gby + dcx * gcy;
-    float const f1y = day * gay + dby * gby + dcy * gcy;
-    float const f1z = daz * gay + dbz * gby + dcz * gcy;
-    float const f2x = dax * gaz + dbx * gbz + dcx * gcz;
-    float const f2y = day * gaz + dby * gbz + dcy * gcz;
-    float const f2z = daz * gaz + dbz * gbz + dcz * gcz;
-    float const ax = f1y * f2z - f1z * f2y, ay = f1z * f2x - f1x * f2z, az = f1x * f2y - f1y * f2x;
-    float const bx = f2y * f0z - f2z * f0y, by = f2z * f0x - f2x * f0z, bz = f2x * f0y - f2y * f0x;
-    float const cx = f0y * f1z - f0z * f1y, cy = f0z * f1x - f0x * f1z, cz = f0x * f1y - f0y * f1x;
-    float const J = f0x * ax + f0y * ay + f0z * az;
-    float const gx = -(gax + gbx + gcx), gy = -(gay + gby + gcy), gz = -(gaz + gbz + gcz);
-    float const kx = gx * ax + gy * bx + gz * cx;
-    float const ky = gx * ay + gy * by + gz * cy;
-    float const kz = gx * az + gy * bz + gz * cz;
-    float const Fgx = f0x * gx + f1x * gy + f2x * gz;
-    float const Fgy = f0y * gx + f1y * gy + f2y * gz;
-    float const Fgz = f0z * gx + f1z * gy + f2z * gz;
+    float const ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
+    float const fx = pc.x - pa.x, fy = pc.y - pa.y, fz = pc.z - pa.z;
+    float const nx = ey * fz - ez * fy, ny = ez * fx - ex * fz, nz = ex * fy - ey * fx;
+    float const detG = mat.x, wlam = mat.y;
+    float const J = detG * (dax * nx + day * ny + daz * nz);
+    float const kx = -detG * nx, ky = -detG * ny, kz = -detG * nz;
+    float const Fgx = geo.x * dax + geo.y * dbx + geo.z * dcx;
+    float const Fgy = geo.x * day + geo.y * dby + geo.z * dcy;
+    float const Fgz = geo.x * daz + geo.y * dbz + geo.z * dcz;
     float const s = wlam * (J - 1.f) - wmu; // = w lambda (J - alpha), without a division
     B.g0 += wmu * Fgx + s * kx;
     B.g1 += wmu * Fgy + s * ky;
     B.g2 += wmu * Fgz + s * kz;
-    float const dg = wmu * (gx * gx + gy * gy + gz * gz);
+    float const dg = wmu * geo.w;
     float const lkx = wlam * kx, lky = wlam * ky, lkz = wlam * kz;
     B.h00 += dg + lkx * kx;
     B.h11 += dg + lky * ky;
@@ -207,19 +195,19 @@ __device__ __forceinline__ float3 InertiaNewtonStep(Block B, float4 const& xv, f
 }
 
 // Incidence records, structure of arrays, row k of vertex v's incidences
-// (CSR rows through vtp, or fixed-stride rows of ellStride entries), 56 B:
+// (CSR rows through vtp, or fixed-stride rows of ellStride entries), 40 B:
 //   ids[k] int4   (a, b, c, w*mu as bits): the tet's other three vertices in
 //                 local order; a = -1 marks an unused fixed-stride slot
-//   g0[k], g1[k] float4, g2[k] float2: (ga, gb.x), (gb.yz, gc.xy),
-//                 (gc.z, w*lambda) with ga, gb, gc the GP rows of a, b, c
+//   geo[k] float4 (ga . g, gb . g, gc . g, |g|^2) and mat[k] float2
+//                 (det [ga; gb; gc], w*lambda), with ga, gb, gc the GP rows of
+//                 a, b, c and g the vertex's own row (AccumulateElastic)
 // Records are stored in sweep order, so a colour pass streams them.
 struct SweepArgs
 {
     int const* __restrict__ vtp;
     int4 const* __restrict__ ids;
-    float4 const* __restrict__ g0;
-    float4 const* __restrict__ g1;
-    float2 const* __restrict__ g2;
+    float4 const* __restrict__ geo;
+    float2 const* __restrict__ mat;
     float4* __restrict__ xm;
     float4 const* __restrict__ xtil;
     float4 const* __restrict__ xt;
@@ -340,7 +328,7 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
             a.xm[pend] = a.contact.xb[pend];
     }
     int4 I[K];
-    float4 q0[K], q1[K];
+    float4 q0[K];
     float2 q2[K];
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -349,9 +337,8 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
         if (k < ke)
         {
             I[j] = LoadIds(a.ids + k, nt);
-            q0[j] = LoadConst(a.g0 + k, nt);
-            q1[j] = LoadConst(a.g1 + k, nt);
-            q2[j] = LoadConst(a.g2 + k, nt);
+            q0[j] = LoadConst(a.geo + k, nt);
+            q2[j] = LoadConst(a.mat + k, nt);
         }
         else
             I[j] = make_int4(-1, 0, 0, 0);
@@ -379,7 +366,7 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 #pragma unroll
     for (int j = 0; j < K; ++j)
         if (I[j].x >= 0)
-            AccumulateOwnFrame(q0[j], q1[j], q2[j], __int_as_float(I[j].w), own, pa[j], pb[j], pc[j], B);
+            AccumulateElastic(q0[j], q2[j], __int_as_float(I[j].w), own, pa[j], pb[j], pc[j], B);
     // incidences beyond K per lane (vertices of unusually high degree)
     for (int k = kb + lane + K * L; k < ke; k += L)
     {
@@ -387,12 +374,12 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
         if (J.x < 0)
             continue;
         if constexpr (C)
-            AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
-                               __int_as_float(J.w), own, ContactPosition(a.contact, a.xm, J.x),
-                               ContactPosition(a.contact, a.xm, J.y), ContactPosition(a.contact, a.xm, J.z), B);
+            AccumulateElastic(LoadConst(a.geo + k, nt), LoadConst(a.mat + k, nt), __int_as_float(J.w), own,
+                              ContactPosition(a.contact, a.xm, J.x), ContactPosition(a.contact, a.xm, J.y),
+                              ContactPosition(a.contact, a.xm, J.z), B);
         else
-            AccumulateOwnFrame(LoadConst(a.g0 + k, nt), LoadConst(a.g1 + k, nt), LoadConst(a.g2 + k, nt),
-                               __int_as_float(J.w), own, a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
+            AccumulateElastic(LoadConst(a.geo + k, nt), LoadConst(a.mat + k, nt), __int_as_float(J.w), own,
+                              a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
     }
     if constexpr (C)
     {

@@ -311,6 +311,45 @@ def test_vertex_block_equals_closed_form():
         assert np.allclose(g9, g, rtol=1e-8, atol=1e-9 * np.abs(H).max())
 
 
+def test_vertex_block_equals_record_form():
+    """The sweep's per-incidence form from its 40-byte records equals the reference's
+    9x9 contraction: with the other vertices a, b, c in local order, d_k = x_k - x,
+    G = [g_a; g_b; g_c] and the own row g, the records hold (g_k . g, |g|^2) and det G,
+    and the kernel takes F g = sum_k (g_k . g) d_k, J = det G (d_a . n) and
+    c = cof(F) g = -det G n with n = (x_b - x_a) x (x_c - x_a). A dropped sign on
+    c or a wrong local order fails here (the 9x9 side is the oracle's own code)."""
+    X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=21)
+    rng = np.random.default_rng(22)
+    nT = E.shape[1]
+    mu = rng.uniform(1e5, 1e6, nT)
+    lam = mu * rng.uniform(3, 10, nT)
+    o = O.OracleVbd(X, E, mue=mu, lambdae=lam)
+    o.x[:] = (X + 0.05 * rng.normal(size=X.shape)).T
+    GP = o.GP.reshape(nT, 3, 4).transpose(0, 2, 1)  # per tet 4 x 3
+    for i in rng.choice(X.shape[1], 10, replace=False):
+        H9, g9 = o.vertex_elastic_block(i)
+        H = np.zeros((3, 3))
+        gr = np.zeros(3)
+        for k in range(o.GVGp[i], o.GVGp[i + 1]):
+            e, il = o.GVGe[k], o.GVGilocal[k]
+            others = [j for j in range(4) if j != il]
+            g = GP[e][il]
+            Gk = GP[e][others]
+            geo = Gk @ g
+            detG = np.linalg.det(Gk)
+            xo = o.x[E[others, e]]
+            d = xo - o.x[i]
+            n = np.cross(xo[1] - xo[0], xo[2] - xo[0])
+            J = detG * (d[0] @ n)
+            c = -detG * n
+            Fg = geo @ d
+            w = o.wg[e]
+            H += w * (mu[e] * (g @ g) * np.eye(3) + lam[e] * np.outer(c, c))
+            gr += w * (mu[e] * Fg + (lam[e] * (J - 1) - mu[e]) * c)
+        assert np.allclose(H9, H, rtol=1e-10, atol=1e-10 * np.abs(H).max())
+        assert np.allclose(g9, gr, rtol=1e-8, atol=1e-9 * np.abs(H).max())
+
+
 def test_multithreaded_step_matches_single_thread():
     s = meshes.plain_gpu_scene(6, seed=1)
     kw = dict(aext=s.aext, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc)
