@@ -55,10 +55,10 @@ def _kernel_source_sha256():
 
 def _counter_traffic(name, shape):
     """HBM bytes per sweep launch from a committed rocprofv3 counter summary
-    (profiles/<name>: FETCH_SIZE and WRITE_SIZE runs, each scaled by what the counter
-    reads for a device copy of known size). Used only when the summary was measured
-    on this build's sweep kernel source and the sweep shape now in effect; returns
-    (summary or None, provenance note)."""
+    (profiles/<name>: FETCH_SIZE and WRITE_SIZE runs per sweep shape, each scaled by
+    what the counter reads for a device copy of known size). Used only when the
+    summary was measured on this build's sweep kernel source and holds the sweep
+    shape now in effect; returns (entry or None, provenance note)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None, f"none: profiles/{name} absent"
@@ -68,10 +68,11 @@ def _counter_traffic(name, shape):
         return None, f"none: profiles/{name} was measured on another version of the sweep kernel source"
     want = {k: shape[k] for k in ("lanes_per_vertex", "incidences_per_lane", "xcd_remap", "nontemporal",
                                   "fixed_stride")}
-    if t.get("sweep_shape") != want:
-        return None, f"none: profiles/{name} was measured with sweep shape {t.get('sweep_shape')}, now {want}"
-    return t, (f"profiles/{name} (rocprofv3 --pmc FETCH_SIZE x {t['fetch_scale']:.3f} + WRITE_SIZE x "
-               f"{t['write_scale']:.3f}, per launch, same kernel source and sweep shape)")
+    entry = next((e for e in t.get("shapes", []) if e.get("sweep_shape") == want), None)
+    if entry is None:
+        return None, f"none: profiles/{name} holds no measurement of sweep shape {want}"
+    return entry, (f"profiles/{name} (rocprofv3 --pmc FETCH_SIZE x {t['fetch_scale']:.3f} + WRITE_SIZE x "
+                   f"{t['write_scale']:.3f}, per launch, same kernel source and sweep shape)")
 
 
 def bench_batched(args, torch):
@@ -91,17 +92,30 @@ def bench_batched(args, torch):
     nColors, nUpd, nInc = batch.counts()
     log(f"[bench] batch: {count} meshes, {batch.offsets[-1]} vertices, {sum(s.n_tets for s in scenes)} tets, "
         f"{nColors} colours, built in {time.time() - t0:.1f}s")
-    tune = batch.autotune(dt, sweeps=5)
-    log(f"[bench] batch sweep shape: {batch.sweep_config()} ({min(r[-1] for r in tune):.1f} us/sweep)")
+    if args.lanes:
+        batch.set_sweep_config(args.lanes, args.per_lane, not args.no_xcd, args.nontemporal,
+                               None if args.stride == "auto" else args.stride == "fixed")
+        log(f"[bench] batch sweep shape: {batch.sweep_config()} (set)")
+    else:
+        tune = batch.autotune(dt, sweeps=5)
+        log(f"[bench] batch sweep shape: {batch.sweep_config()} ({min(r[-1] for r in tune):.1f} us/sweep)")
+    if args.no_graph:
+        batch.graph_capture = False
     for _ in range(2):
         batch.step(dt, iters, 1)
     batch.synchronize()
     steps = args.batch_steps
+    batch.set_timing(True)
+    batch.sweep_timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.step(dt, iters, 1)
     batch.synchronize()
     tb = (time.perf_counter() - t0) / steps
+    ms, launches = batch.sweep_timing(reset=True)
+    batch.set_timing(False)
+    if args.batch_no_loop:
+        return {"steps_per_s": 1 / tb, "sweep_launches": launches, "sweep_ms": ms}
     # plain per-mesh loop on the same meshes
     singles = [pbat.gpu.vbd.Integrator(d) for d in datas]
     for g in singles:
@@ -116,15 +130,27 @@ def bench_batched(args, torch):
     tl = (time.perf_counter() - t0) / loop_steps
     log(f"[bench] batched {1 / tb:.1f} steps/s ({1e3 * tb:.2f} ms) vs per-mesh loop {1 / tl:.2f} steps/s "
         f"({1e3 * tl:.1f} ms): {tl / tb:.1f}x")
-    # algorithmic bytes of one sweep over the batch (all colours), per step time
-    achieved = (BYTES_PER_VERTEX * nUpd + BYTES_PER_INCIDENCE * nInc) * iters / tb / 1e9
-    log(f"[bench] batched sweeps: {achieved:.0f} GB/s algorithmic, step-averaged "
-        f"({achieved / HBM_PEAK_GBS:.3f} of 8 TB/s)")
+    # the sweep kernel: algorithmic bytes per launch (the 64^3 model, 60 B per
+    # vertex + 120 B per incidence, averaged over the colours) over the average
+    # launch time from HIP events around the replayed step graphs, and the
+    # HBM bytes the counters measured for this shape
+    shape = batch.sweep_config()
+    avg_launch_s = ms * 1e-3 / max(launches, 1)
+    bytes_per_launch = (BYTES_PER_VERTEX * nUpd + BYTES_PER_INCIDENCE * nInc) / nColors
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    traffic, traffic_note = _counter_traffic("batched_sweep_256meshes_traffic.json", shape)
+    tgbs = None if traffic is None else traffic["hbm_bytes_per_launch"] / avg_launch_s / 1e9
+    log(f"[bench] batched sweep kernel: {launches} launches, avg {avg_launch_s * 1e6:.2f} us, "
+        f"{bytes_per_launch / 1e6:.2f} MB/launch -> {achieved:.0f} GB/s algorithmic; counter traffic "
+        f"{traffic_note}" + ("" if tgbs is None else f": {tgbs:.0f} GB/s"))
     return {"workload": f"{count} jittered cube meshes, grid dims U{{4..24}} per axis, per-mesh Y, nu; "
                         f"h=1/60, {iters} iterations",
-            "roofline_step_averaged": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                       "frac": achieved / HBM_PEAK_GBS,
-                                       "note": "algorithmic bytes of the step's sweeps / whole step time"},
+            "roofline": {"bound": "hbm", "kernel": "VbdSweep", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "avg_launch_us": avg_launch_s * 1e6,
+                         "algorithmic_bytes_per_launch": bytes_per_launch,
+                         "traffic": None if traffic is None else traffic["hbm_bytes_per_launch"],
+                         "traffic_source": traffic_note, "traffic_gbs": tgbs,
+                         "traffic_frac": None if tgbs is None else tgbs / HBM_PEAK_GBS},
             "meshes": count, "vertices": int(batch.offsets[-1]), "updated_vertices": nUpd,
             "steps_per_s": 1 / tb, "per_mesh_loop_steps_per_s": 1 / tl, "speedup_vs_loop": tl / tb,
             "vertex_updates_per_s": nUpd * iters / tb, "steps": steps, "loop_steps": loop_steps,
@@ -437,6 +463,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-mesh measurement")
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--batch-no-loop", action="store_true",
+                    help="batched measurement without the per-mesh loop baseline (profiling runs)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout measurements")
     ap.add_argument("--adjoint-grid", type=int, default=32)

@@ -334,17 +334,48 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __res
     }
 }
 
-// Sum over the 128 threads of a workgroup in a fixed order (both waves'
-// butterflies, then wave 0 + wave 1); every thread calls it, thread 0 gets it.
-__device__ __forceinline__ double Sum128(double v, double* __restrict__ lds)
+// Thread t of a group of N threads sums v[b + t], v[b + t + N], ... in that
+// order, issuing the loads 8 at a time (the order, and so the bits, do not
+// depend on the batching: out-of-range entries add an exact 0).
+template <int N>
+__device__ __forceinline__ double StridedSum(double const* __restrict__ v, int b, int e, int t)
 {
-    for (int off = 32; off > 0; off >>= 1)
-        v += __shfl_xor(v, off);
+    double s = 0.;
+    for (int k0 = b + t; k0 < e; k0 += 8 * N)
+    {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+        {
+            int const k = k0 + u * N;
+            x[u] = k < e ? v[k] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            s += x[u];
+    }
+    return s;
+}
+
+// Sums of V values over the 128 threads of a workgroup in a fixed order (each
+// wave's butterfly, then wave 0 + wave 1); every thread calls it and gets the
+// sums in v.
+template <int V>
+__device__ __forceinline__ void Sum128(double (&v)[V], double (*lds)[V])
+{
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+        for (int off = 32; off > 0; off >>= 1)
+            v[i] += __shfl_xor(v[i], off);
     __syncthreads();
     if ((threadIdx.x & 63) == 0)
-        lds[threadIdx.x >> 6] = v;
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+            lds[threadIdx.x >> 6][i] = v[i];
     __syncthreads();
-    return lds[0] + lds[1];
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+        v[i] = lds[0][i] + lds[1][i];
 }
 
 // One workgroup of 128 threads per aggregate g of mesh m, alpha = rz / pq with
@@ -383,18 +414,17 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
     int const nM = u.cg.nM;
     if (!u.init && u.cg.F[2 * nM] == 0)
         return; // every mesh has converged
-    __shared__ double lds[2];
+    __shared__ double lds1[2][1];
+    __shared__ double lds6[2][6];
     int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
     int const m = u.aggMesh[g];
     float alpha = 0.f;
     if (!u.init)
     {
-        double s = 0.;
-        for (int k = u.meshChunkBegin[m] + t; k < u.meshChunkBegin[m + 1]; k += 128)
-            s += u.pqPart[k];
-        double const pq = Sum128(s, lds);
+        double pq[1] = {StridedSum<128>(u.pqPart, u.meshChunkBegin[m], u.meshChunkBegin[m + 1], t)};
+        Sum128<1>(pq, lds1);
         bool const done = u.cg.F[kDone * nM + m] != 0;
-        alpha = (!done && pq != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq) : 0.f;
+        alpha = (!done && pq[0] != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq[0]) : 0.f;
     }
     double sx = 0., sy = 0., sz = 0., rz = 0., rr = 0., bb = 0.;
     for (int k = u.aggPtr[g] + t; k < u.aggPtr[g + 1]; k += 128)
@@ -424,28 +454,20 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
         rz += Dot3(ri, zi);
         rr += Dot3(ri, ri);
     }
-    rz = Sum128(rz, lds);
-    rr = Sum128(rr, lds);
-    if (u.init)
-        bb = Sum128(bb, lds);
-    if (u.rc)
-    {
-        sx = Sum128(sx, lds);
-        sy = Sum128(sy, lds);
-        sz = Sum128(sz, lds);
-    }
+    double sums[6] = {rz, rr, bb, sx, sy, sz};
+    Sum128<6>(sums, lds6);
     if (t == 0)
     {
         if (u.rc)
         {
-            u.rc[3 * g] = sx;
-            u.rc[3 * g + 1] = sy;
-            u.rc[3 * g + 2] = sz;
+            u.rc[3 * g] = sums[3];
+            u.rc[3 * g + 1] = sums[4];
+            u.rc[3 * g + 2] = sums[5];
         }
-        u.rzPart[g] = rz;
-        u.rrPart[g] = rr;
+        u.rzPart[g] = sums[0];
+        u.rrPart[g] = sums[1];
         if (u.init)
-            u.bbPart[g] = bb;
+            u.bbPart[g] = sums[2];
     }
 }
 
@@ -517,17 +539,11 @@ __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
     for (int m = wave; m < nM; m += 4)
     {
         int const a0 = f.meshAggBegin[m], a1 = f.meshAggBegin[m + 1];
-        double rz = 0., rr = 0., bb = 0.;
-        for (int k = a0 + lane; k < a1; k += 64)
-        {
-            rz += f.rzPart[k];
-            rr += f.rrPart[k];
-            if (first)
-                bb += f.bbPart[k];
-        }
+        double rz = StridedSum<64>(f.rzPart, a0, a1, lane);
+        double rr = StridedSum<64>(f.rrPart, a0, a1, lane);
+        double bb = first ? StridedSum<64>(f.bbPart, a0, a1, lane) : 0.;
         if (f.rczcPart)
-            for (int k = 3 * a0 + lane; k < 3 * a1; k += 64)
-                rz += f.rczcPart[k];
+            rz += StridedSum<64>(f.rczcPart, 3 * a0, 3 * a1, lane);
         for (int off = 32; off > 0; off >>= 1)
         {
             rz += __shfl_xor(rz, off);
