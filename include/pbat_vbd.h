@@ -251,6 +251,19 @@ int pbat_vbd_integrator_energy(pbat_vbd_integrator* h, double out[3]);
  * output pointer may be NULL. */
 int pbat_vbd_integrator_objective(pbat_vbd_integrator* h, const float* x, const float* xtilde, float dt, double* f,
                                   double* grad, double* residual2);
+/* No counterpart in the reference: for callers that cannot allocate device memory
+ * (code between other work on a stream it must not disturb, serving loops,
+ * harnesses that forbid hipMalloc). The same value, bit for bit, as
+ * pbat_vbd_integrator_objective(h, NULL, NULL, dt, f, NULL, NULL): the objective at
+ * the handle's current positions and inertial target, reduced in FP64 in a fixed
+ * order, with its partial sums in a file-scope device array of 16 KiB per GPU
+ * (looked up once per GPU, kept until the process ends) instead of buffers the
+ * handle allocates. Calls no hipMalloc or hipFree. The scratch does not grow with
+ * the mesh (a fixed 512-workgroup reduction), so the only size limit is the
+ * kernels' 32-bit indexing: nV and nT below 2^31, larger meshes are rejected with
+ * PBAT_VBD_ERROR_INVALID_ARGUMENT. Every call on a GPU overwrites that GPU's
+ * scratch: calls that overlap on one GPU (several host threads) race. */
+int pbat_vbd_integrator_objective_noalloc(pbat_vbd_integrator* h, float dt, double* f);
 /* sim::vbd::Integrator::TraceNextStep followed by Step (sim/vbd/Integrator.cpp:22-52,
  * 73-87, 202-235): one step whose iterates are traced. Before each VBD iteration and
  * after the last, the FP64 objective, its gradient and the positions are recorded;

@@ -117,6 +117,7 @@ SIGNATURES = [
     ("pbat_vbd_integrator_energy", ctypes.c_int, [ctypes.c_void_p, p_f64]),
     ("pbat_vbd_integrator_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("pbat_vbd_integrator_objective", ctypes.c_int, [ctypes.c_void_p, p_f32, p_f32, c_f32, p_f64, p_f64, p_f64]),
+    ("pbat_vbd_integrator_objective_noalloc", ctypes.c_int, [ctypes.c_void_p, c_f32, p_f64]),
     ("pbat_vbd_integrator_trace_objective_step", ctypes.c_int,
      [ctypes.c_void_p, c_f32, c_i32, c_i32, c_i32, ctypes.c_char_p]),
     ("pbat_vbd_integrator_fp64_solves", ctypes.c_int, [ctypes.c_void_p, p_i64, c_i32]),

@@ -37,6 +37,7 @@
 #include <iomanip>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 #include <string>
@@ -1091,6 +1092,67 @@ void Objective(pbat_vbd_integrator& h, float const* x, float const* xtilde, floa
     ObjectiveOnDevice(h, xd, xl, dt, f, grad, r2);
 }
 
+// ---- objective without device allocation -----------------------------------
+// Scratch of ObjectiveNoAlloc: the per-workgroup partials of ObjectiveTerms
+// (3 doubles for each of its kObjectiveBlocks workgroups) and their 3 sums,
+// 16 KiB. Each GPU holds its own copy until the process ends; every call on a
+// GPU overwrites it, so calls that overlap on one GPU race.
+constexpr int kNoAllocScratchDoubles = 2048;
+static_assert(3 * kObjectiveBlocks + 3 <= kNoAllocScratchDoubles, "objective partials exceed the scratch");
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+// Device address of the scratch on each GPU, looked up once per device (with
+// that device current); the host-side name is not a device address.
+constexpr int kMaxScratchDevices = 64;
+std::mutex gScratchMutex;
+double* gScratchPtr[kMaxScratchDevices] = {};
+
+double* ScratchOnDevice(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "device index beyond the no-allocation entry point's table");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchPtr[device])
+    {
+        void* ptr = nullptr;
+        PBAT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr)
+            Fail(PBAT_VBD_ERROR_HIP, "hipGetSymbolAddress returned no address for the objective scratch");
+        gScratchPtr[device] = static_cast<double*>(ptr);
+    }
+    return gScratchPtr[device];
+}
+
+// The objective of ObjectiveOnDevice at the handle's current positions and
+// inertial target (f only), with its partials in the static scratch instead of
+// the handle's lazily allocated buffers: the same kernels and partial layout,
+// so the same bits. Allocates no device memory.
+void ObjectiveNoAlloc(pbat_vbd_integrator& h, float dt, double* f)
+{
+    if (!(dt > 0.f) || !std::isfinite(dt))
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "dt must be positive and finite");
+    if (!f)
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null objective output");
+    if (h.nV > std::numeric_limits<int>::max() || h.nT > std::numeric_limits<int>::max())
+        Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "mesh too large for the objective reduction (2^31 vertices or tets)");
+    PBAT_HIP(hipSetDevice(h.device));
+    double* const part = ScratchOnDevice(h.device);
+    double* const sums = part + 3 * kObjectiveBlocks;
+    device::TrEpilogue none{};
+    hipLaunchKernelGGL(device::ObjectiveTerms, dim3(kObjectiveBlocks), dim3(256), 0, h.stream,
+                       static_cast<float4 const*>(h.xm.ptr), static_cast<float4 const*>(h.xtil.ptr),
+                       static_cast<float4 const*>(nullptr), static_cast<int4 const*>(h.tetv.ptr),
+                       static_cast<float4 const*>(h.tgp.ptr), static_cast<int>(h.nV), static_cast<int>(h.nT), part,
+                       static_cast<double const*>(nullptr));
+    hipLaunchKernelGGL(device::SumPartials3, dim3(1), dim3(256), 0, h.stream, static_cast<double const*>(part),
+                       kObjectiveBlocks, sums, static_cast<double const*>(nullptr), none);
+    PBAT_HIP(hipGetLastError());
+    double s[2];
+    PBAT_HIP(hipMemcpyAsync(s, sums, sizeof(s), hipMemcpyDeviceToHost, h.stream));
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    *f = 0.5 * s[0] + double(dt) * double(dt) * s[1];
+}
+
 // sim::vbd::Integrator::TraceNextStep followed by Step (sim/vbd/Integrator.cpp:22-52,
 // 73-87, 202-235): one step whose iterates are traced. Before every VBD iteration
 // and after the last, the objective, its gradient and the positions at that iterate
@@ -1756,6 +1818,11 @@ extern "C" int pbat_vbd_integrator_objective(pbat_vbd_integrator* h, const float
                                              double* f, double* grad, double* residual2)
 {
     return Guard([&] { Objective(Handle(h), x, xtilde, dt, f, grad, residual2); });
+}
+
+extern "C" int pbat_vbd_integrator_objective_noalloc(pbat_vbd_integrator* h, float dt, double* f)
+{
+    return Guard([&] { ObjectiveNoAlloc(Handle(h), dt, f); });
 }
 
 extern "C" int pbat_vbd_integrator_trace_objective_step(pbat_vbd_integrator* h, float dt, int32_t iterations,
