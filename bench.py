@@ -39,6 +39,10 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VERTEX = 60
 BYTES_PER_INCIDENCE = 120
+# CG tolerance |r| <= tol |b| of the adjoint benches: on the 12^3 stand-in of the
+# differentiable workload the gradients stay at the FP32 floor (4-9e-6 against
+# the FP64 IFT gradient, as at 1e-6) with 14% fewer iterations than at 1e-6
+ADJOINT_TOL = 1e-5
 
 
 def log(msg):
@@ -176,10 +180,9 @@ def bench_adjoint(args, torch):
     L = sum |x_T - x_T^target|^2 from log Y = ln 1e6. One fwd+adjoint pass = the
     20-step forward (state kept after every step: 34 MB, so the backward pass
     recomputes nothing) + the IFT backward (one
-    two-level PCG solve per step, tolerance 1e-6 per mesh) giving dL/dlogY,
-    dL/dv0, dL/dx0. Also: central differences of the device forward on 64
-    random log Y entries (the forward is truncated at 20 iterations, so these
-    measure truncation as much as the adjoint)."""
+    two-level PCG solve per step, tolerance ADJOINT_TOL per mesh) giving dL/dlogY,
+    dL/dv0, dL/dx0. The gradient checks (against the FP64 IFT gradient and FP64
+    finite differences) run in the CPU leg: adjoint_gradient_checks."""
     import numpy as np
     from physicsbasedanimationtoolkit_amd import meshes, pbat, _native
 
@@ -188,7 +191,7 @@ def bench_adjoint(args, torch):
     g.autotune(s.dt, sweeps=5)
     v0 = g.v.copy()
     dt, it, T, every = s.dt, s.iterations, s.steps, 1
-    solver = dict(solver="pcg_two_level", tolerance=1e-6, max_iterations=2000, check_every=10)
+    solver = dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000, check_every=10)
 
     def forward(logy):
         g.set_log_young_modulus(logy)
@@ -215,32 +218,12 @@ def bench_adjoint(args, torch):
         tf += t1 - t0
         stats = r["stats"]
     loss = float(((xT - target) ** 2).sum())
-    fd = None
-    if args.adjoint_fd:
-        rng = np.random.default_rng(1)
-        idx = rng.choice(s.n_tets, 64, replace=False)
-        eps = 1e-2
-        pairs = []
-        for e in idx:
-            lv = []
-            for sgn in (1.0, -1.0):
-                ly = s.log_y_guess.copy()
-                ly[e] += sgn * eps
-                forward(ly)
-                lv.append(float(((g.x.astype(np.float64) - target) ** 2).sum()))
-            pairs.append(((lv[0] - lv[1]) / (2 * eps), float(r["log_young_modulus"][e])))
-        pairs = np.array(pairs)
-        scale = np.abs(pairs[:, 0]).max()
-        err = np.abs(pairs[:, 0] - pairs[:, 1]) / scale
-        fd = {"params": 64, "eps": eps, "median_err_rel_to_max": float(np.median(err)),
-              "max_err_rel_to_max": float(err.max()),
-              "note": "FD of the 20-iteration (unconverged) forward vs the IFT gradient of the converged step"}
     out = {"workload": f"differentiable VBD, {args.adjoint_grid}^3 jittered grid ({s.n_vertices} verts, "
                        f"{s.n_tets} tets), hidden per-tet log Y, {T} steps h=1/60 x {it} iterations, "
                        f"v0 = 0.5 random unit vectors, L = |x_T - target|^2 from log Y = ln 1e6",
            "fwd_adjoint_steps_per_s": T * reps / (tf + tb), "forward_steps_per_s": T * reps / tf,
            "forward_ms": 1e3 * tf / reps, "adjoint_ms": 1e3 * tb / reps, "loss": loss,
-           "solver": solver, "checkpoint_every": every, "adjoint_stats": stats, "fd_check": fd,
+           "solver": solver, "checkpoint_every": every, "adjoint_stats": stats,
            "device_bytes": _native.device_memory()[0]}
     log(f"[bench] adjoint 32^3: fwd {out['forward_ms']:.1f} ms + adjoint {out['adjoint_ms']:.1f} ms per "
         f"{T}-step rollout -> {out['fwd_adjoint_steps_per_s']:.1f} fwd+adj steps/s; {stats}")
@@ -264,7 +247,7 @@ def bench_batched_adjoint(args, torch):
     b.autotune(scenes[0].dt, sweeps=3)
     shape = b.sweep_config()
     dt, it, T, every = scenes[0].dt, scenes[0].iterations, args.batch_adjoint_steps, 10
-    solver = dict(solver="pcg_two_level", tolerance=1e-6, max_iterations=2000, check_every=10)
+    solver = dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000, check_every=10)
     hidden = np.concatenate([s.log_y_hidden for s in scenes])
     guess = np.concatenate([s.log_y_guess for s in scenes])
     Xb = np.concatenate([s.X for s in scenes], axis=1)
@@ -289,6 +272,9 @@ def bench_batched_adjoint(args, torch):
     mem_now, mem_peak = _native.device_memory()
     log(f"[bench] batched adjoint: {len(scenes)} meshes, fwd {tf:.2f}s + adjoint {tb:.2f}s per {T}-step rollout "
         f"-> {T / (tf + tb):.2f} steps/s; peak device memory {mem_peak / 2**20:.0f} MiB; {rb['stats']}")
+    if args.batch_no_loop:
+        return {"fwd_adjoint_steps_per_s": T / (tf + tb), "forward_s": tf, "adjoint_s": tb,
+                "adjoint_stats": rb["stats"]}
     # plain per-mesh loop, same sweep shape so that each forward is bitwise the batch's
     loop_t = 0.0
     errs = []
@@ -445,6 +431,126 @@ def bench_variants(args, torch):
     return out
 
 
+def shape_sweep(args, torch, workload):
+    """Counter-profiling driver: every candidate shape of the sweep autotuner (the
+    list its own run returns) steps the workload once with eager launches, in that
+    order, so that a rocprofv3 --pmc run of this command attributes each group of
+    colours x iterations consecutive VbdSweep dispatches to one shape (the
+    autotuner's own 2 x colours dispatches per candidate come first). Prints the
+    order as JSON."""
+    from physicsbasedanimationtoolkit_amd import meshes, pbat
+
+    if workload == "plain":
+        s = meshes.plain_gpu_scene(args.grid, seed=0)
+        data = (pbat.sim.vbd.Data().with_volume_mesh(s.X, s.E).with_material(s.rhoe, s.mue, s.lambdae)
+                .with_acceleration(s.aext).with_dirichlet_vertices(s.dbc).construct())
+        target = pbat.gpu.vbd.Integrator(data)
+        dt, iters = s.dt, args.iterations
+    else:
+        scenes = meshes.batched_scenes(args.batch_meshes, 4, 24, seed=0)
+        datas = [(pbat.sim.vbd.Data().with_volume_mesh(c.X, c.E).with_material(c.rhoe, c.mue, c.lambdae)
+                  .with_acceleration(c.aext).with_dirichlet_vertices(c.dbc).construct()) for c in scenes]
+        target = pbat.gpu.vbd.BatchIntegrator(datas)
+        dt, iters = scenes[0].dt, scenes[0].iterations
+    nColors = target.counts()[0]
+    rows = target.autotune(dt, sweeps=1)
+    target.graph_capture = False
+    shapes = []
+    for L, K, X, N, S, _ in rows:
+        target.set_sweep_config(int(L), int(K), bool(X), bool(N), bool(S))
+        target.step(dt, iters, 1)
+        shapes.append(target.sweep_config())
+    target.synchronize()
+    return {"workload": workload, "autotune_sweep_dispatches": 2 * nColors * len(rows),
+            "sweep_dispatches_per_shape": nColors * iters, "shapes": shapes}
+
+
+def adjoint_gradient_checks(args, solver):
+    """CPU leg (oracle as the checker, after every timed region):
+    (1) the 32^3 differentiable workload's settings on a 12^3 grid (the FP64 direct
+        solve at 32^3 takes minutes per step): device gradients wrt log Y and v0
+        against the FP64 IFT gradient along the device trajectory, with the bench's
+        solver settings;
+    (2) 64 random log Y entries of a soft 4^3 rollout whose 400 iterations per step
+        converge to round-off: the device gradient against FP64 central differences
+        of oracle rollouts (step 1e-4), which differentiate the minimiser sequence the
+        implicit-function theorem describes."""
+    import numpy as np
+    from oracle import adjoint as A
+    from oracle import oracle as O
+    from physicsbasedanimationtoolkit_amd import meshes, pbat
+
+    out = {}
+    s = meshes.differentiable_scene(12, seed=3)
+    g = pbat.gpu.vbd.Integrator(_diff_data(pbat, s))
+    v0 = g.v.copy()
+    g.set_log_young_modulus(s.log_y_hidden)
+    g.rollout_forward(s.dt, s.iterations, s.steps, 10)
+    target = g.x.astype(np.float64)
+    g.set_log_young_modulus(s.log_y_guess)
+    g.x, g.v = s.X, v0
+    xs = [np.asarray(s.X, np.float64).T.copy()]
+    for _ in range(s.steps):
+        g.step(s.dt, s.iterations, 1)
+        xs.append(g.x.astype(np.float64).T.copy())
+    dLdx = 2 * (xs[-1].T - target)
+    g.x, g.v = s.X, v0
+    g.rollout_forward(s.dt, s.iterations, s.steps, 10)
+    r = g.rollout_backward(dLdx, None, **solver)
+    o = O.OracleVbd(s.X, s.E, v=s.v0, aext=s.aext, rhoe=s.rhoe, mue=s.mue, lambdae=s.lambdae, dbc=s.dbc,
+                    nthreads=8)
+    A.set_log_young_modulus(o, s.log_y_guess)
+    ref = A.ift_gradient(o, xs, s.dt, dLdx.T)
+    out["vs_fp64_ift"] = {"workload": "12^3 jittered grid, hidden per-tet log Y, v0 = 0.5 random unit vectors, "
+                                      f"{s.steps} steps h=1/60 x {s.iterations} iterations, "
+                                      "L = |x_T - target|^2 from log Y = ln 1e6",
+                          "rel_err_log_young_modulus": _rel(r["log_young_modulus"], ref[0]),
+                          "rel_err_v0": _rel(r["v0"], ref[1].T), "solver": solver, "stats": r["stats"]}
+    # (2) finite differences on a converged soft rollout
+    X, E = meshes.cube_tet_grid(4, jitter=0.1, seed=3)
+    rng = np.random.default_rng(3)
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(meshes.log_uniform(rng, 1e4, 1e5, nT), 0.45)
+    v = rng.normal(size=X.shape) * 0.5
+    dbc, a, rho = meshes.bottom_vertices(X), meshes.gravity(X.shape[1]), np.full(nT, 1e3)
+    dt, iters, steps = 1.0 / 60.0, 400, 3
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(rho, mu, lam).with_acceleration(a)
+            .with_dirichlet_vertices(dbc).with_velocity(v).construct())
+    gs = pbat.gpu.vbd.Integrator(data)
+    w = np.random.default_rng(11).normal(size=X.shape)
+    gs.rollout_forward(dt, iters, steps, 1)
+    rs = gs.rollout_backward(w, None, **dict(solver, tolerance=1e-7))
+
+    def make():
+        return O.OracleVbd(X, E, v=v, aext=a, rhoe=rho, mue=mu, lambdae=lam, dbc=dbc, nthreads=4)
+
+    logy0 = np.log(A.young_modulus(make()))
+    idx = np.random.default_rng(1).choice(nT, 64, replace=False)
+    eps = 1e-4
+    fd = []
+    for e in idx:
+        lv = []
+        for sgn in (1.0, -1.0):
+            oe = make()
+            ly = logy0.copy()
+            ly[e] += sgn * eps
+            A.set_log_young_modulus(oe, ly)
+            lv.append(float((w.T * A.rollout(oe, dt, iters, steps)[-1]).sum()))
+        fd.append((lv[0] - lv[1]) / (2 * eps))
+    fd = np.array(fd)
+    dev = np.asarray(rs["log_young_modulus"], np.float64)[idx]
+    err = np.abs(dev - fd) / np.abs(fd).max()
+    out["fd_check"] = {"params": 64, "workload": f"soft 4^3 jittered grid (Y log-uniform in [1e4, 1e5]), "
+                                                 f"{steps} steps x {iters} iterations (converged), L = w . x_T",
+                       "eps": eps, "fd": "FP64 central differences of oracle rollouts",
+                       "rel_err_vector": _rel(dev, fd), "max_err_rel_to_max": float(err.max()),
+                       "median_err_rel_to_max": float(np.median(err))}
+    log(f"[bench] gradient checks: 12^3 vs FP64 IFT log Y {out['vs_fp64_ift']['rel_err_log_young_modulus']:.2e} "
+        f"v0 {out['vs_fp64_ift']['rel_err_v0']:.2e}; 64-entry FD (4^3 converged) "
+        f"rel err {out['fd_check']['rel_err_vector']:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -464,18 +570,21 @@ def main():
     ap.add_argument("--batch-meshes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--batch-no-loop", action="store_true",
-                    help="batched measurement without the per-mesh loop baseline (profiling runs)")
+                    help="batched measurements without their per-mesh loop baselines (profiling runs)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout measurements")
     ap.add_argument("--adjoint-grid", type=int, default=32)
     ap.add_argument("--adjoint-reps", type=int, default=3)
-    ap.add_argument("--adjoint-fd", action="store_true", help="central differences on 64 log Y entries")
     ap.add_argument("--batch-adjoint-meshes", type=int, default=64)
     ap.add_argument("--batch-adjoint-steps", type=int, default=200)
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the accelerated-integrator and contact measurements")
-    ap.add_argument("--only", choices=["batched", "adjoint", "batched_adjoint", "variants"], default=None,
+    ap.add_argument("--only", choices=["batched", "adjoint", "batched_adjoint", "variants", "gradient_checks"],
+                    default=None,
                     help="run one secondary measurement alone and print its JSON (profiling runs)")
+    ap.add_argument("--shape-sweep", choices=["plain", "batched"], default=None,
+                    help="step the workload once per autotuner candidate shape, eagerly, and print the order "
+                         "(counter profiling of every shape in one run)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MB (counter calibration) and exit")
     args = ap.parse_args()
@@ -501,9 +610,14 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": 4 * n, "launches": 5}), flush=True)
         return
+    if args.shape_sweep:
+        print(json.dumps(shape_sweep(args, torch, args.shape_sweep)), flush=True)
+        return
     if args.only:
         run = {"batched": bench_batched, "adjoint": bench_adjoint, "batched_adjoint": bench_batched_adjoint,
-               "variants": bench_variants}
+               "variants": bench_variants,
+               "gradient_checks": lambda a, t: adjoint_gradient_checks(
+                   a, dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000, check_every=10))}
         print(json.dumps({args.only: run[args.only](args, torch)}), flush=True)
         return
     s = meshes.plain_gpu_scene(args.grid, seed=0)
@@ -607,6 +721,8 @@ def main():
                                 "sample": f"1 warm-up step x 1 iteration, then 1 step x {iters1} iterations of "
                                           f"the same workload, {cpu1_s:.2f}s"}
         log(f"[bench] cpu baseline {cpu['single_thread']['value']:.4e} vu/s on 1 thread")
+        if adjoint is not None:
+            adjoint["gradient_checks"] = adjoint_gradient_checks(args, adjoint["solver"])
 
     out = {
         "metric": "VBD vertex-updates/s + fwd+adjoint steps/s vs plain, 1 GPU, \u22641e-5 rel err",

@@ -156,6 +156,7 @@ struct RolloutState
     // the pattern and assembly sources (built once per handle), the values, the
     // longest row, and the fused iteration's second direction buffer
     DeviceArray<int> bsrRowPtr, bsrCol, bsrBlockRow, bsrSrcPtr, bsrSrc;
+    DeviceArray<int> bsrColAgg; // aggregate of each block's column
     DeviceArray<float4> bsrA, bsrB;
     DeviceArray<float> bsrC;
     int bsrMaxRow = 0;

@@ -196,13 +196,13 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
     {
         int const n = 3 * (h.aggBegin[m + 1] - h.aggBegin[m]);
         maxN = std::max(maxN, n);
-        finalBuf[m] = n % 2;
+        finalBuf[m] = ((n + device::kPivot - 1) / device::kPivot) % 2; // parity of its step count
     }
     int64_t const maxEntries = int64_t(maxN) * maxN;
     dim3 const grid(static_cast<unsigned>((maxEntries + 255) / 256), static_cast<unsigned>(nM));
-    for (int k = 0; k < maxN; ++k)
+    for (int s = 0, k = 0; k < maxN; ++s, k += device::kPivot)
         hipLaunchKernelGGL(device::CoarseInvertStep, grid, dim3(256), 0, h.stream,
-                           static_cast<double const*>(R.coarse[k & 1].ptr), R.coarse[(k + 1) & 1].ptr, k, c);
+                           static_cast<double const*>(R.coarse[s & 1].ptr), R.coarse[(s + 1) & 1].ptr, k, c);
     // allocated once per size: the captured CG chunk graph holds this pointer
     if (R.coarseFinal.n != finalBuf.size())
         R.coarseFinal.allocate(finalBuf.size());
@@ -335,6 +335,11 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
         PBAT_HIP(hipMemcpy(meshOf.data(), h.meshOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
         PBAT_HIP(hipMemcpy(aggOf.data(), h.aggOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
     }
+    // aggregate of each block's column (the direction product's coarse term)
+    std::vector<int> colAgg(nBlocks);
+    for (size_t k = 0; k < nBlocks; ++k)
+        colAgg[k] = aggOf[col[k]];
+    R.bsrColAgg.upload(colAgg, h.stream);
     // row chunks: every (mesh, colour) run of rows cut into chunks of at most
     // 256 / L rows, then ordered mesh-major (stable: colour, then row order)
     int const nM = h.nMeshes;
@@ -531,7 +536,7 @@ void DirectionPass(pbat_vbd_integrator& h, device::BsrArgs a)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "row chunks built for another lane count");
     a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
     a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
-    a.meshOf = h.meshOf.ptr, a.aggOf = h.aggOf.ptr;
+    a.meshOf = h.meshOf.ptr, a.aggOf = h.aggOf.ptr, a.colAgg = R.bsrColAgg.ptr;
     switch (BsrLanes(h))
     {
         case 1: LaunchDirectionL<1>(h, a); break;
@@ -773,7 +778,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             R.coarse[1].ptr,   R.coarseFinal.ptr, R.rc.ptr,    R.zc.ptr,         R.bsrRowPtr.ptr, R.bsrCol.ptr,
             R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
             h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.meshChunkBegin.ptr,
-            R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr};
+            R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
+            R.bsrColAgg.ptr};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();

@@ -649,8 +649,15 @@ struct CoarseArgs
     int nAgg;
 };
 
-// Gauss-Jordan step k on every mesh block (blockIdx.y = mesh): Bk = step(Ak);
-// after n steps an n x n SPD block holds its inverse.
+// Block Gauss-Jordan step on every mesh block (blockIdx.y = mesh): with the
+// pivot block P = A[K, K] of the rows and columns K = [k, k + b) (b = min(kPivot,
+// n - k)), Bk = step(Ak) is
+//   B[K, K] = P^-1,   B[K, j] = P^-1 A[K, j],   B[i, K] = -A[i, K] P^-1,
+//   B[i, j] = A[i, j] - A[i, K] P^-1 A[K, j]     (i, j outside K),
+// the same as b scalar steps; after ceil(n / kPivot) steps an n x n SPD block
+// holds its inverse. Every workgroup inverts P in LDS (SPD, no pivoting).
+constexpr int kPivot = 8;
+
 __global__ __launch_bounds__(256) void CoarseInvertStep(double const* __restrict__ Ak, double* __restrict__ Bk,
                                                         int k, CoarseArgs c)
 {
@@ -658,22 +665,72 @@ __global__ __launch_bounds__(256) void CoarseInvertStep(double const* __restrict
     int const n = 3 * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
     if (k >= n)
         return;
+    int const b = min(kPivot, n - k);
+    double const* A = Ak + c.meshOffset[m];
+    __shared__ double P[kPivot][kPivot];
+    int const t = static_cast<int>(threadIdx.x);
+    if (t < b * b)
+        P[t / b][t % b] = A[static_cast<size_t>(k + t / b) * n + k + t % b];
+    __syncthreads();
+    if (t == 0)
+    {
+        // in-place Gauss-Jordan inverse of the b x b pivot block
+        for (int q = 0; q < b; ++q)
+        {
+            double const inv = 1. / P[q][q];
+            for (int r = 0; r < b; ++r)
+            {
+                if (r == q)
+                    continue;
+                double const f = P[r][q] * inv;
+                for (int cc = 0; cc < b; ++cc)
+                    if (cc != q)
+                        P[r][cc] -= f * P[q][cc];
+                P[r][q] = -f;
+            }
+            for (int cc = 0; cc < b; ++cc)
+                if (cc != q)
+                    P[q][cc] *= inv;
+            P[q][q] = inv;
+        }
+    }
+    __syncthreads();
     int64_t const idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= static_cast<int64_t>(n) * n)
         return;
     int const i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
-    double const* A = Ak + c.meshOffset[m];
-    double const p = A[static_cast<size_t>(k) * n + k];
-    double const aik = A[static_cast<size_t>(i) * n + k], akj = A[static_cast<size_t>(k) * n + j];
+    bool const iK = i >= k && i < k + b, jK = j >= k && j < k + b;
     double v;
-    if (i == k && j == k)
-        v = 1. / p;
-    else if (i == k)
-        v = akj / p;
-    else if (j == k)
-        v = -aik / p;
+    if (iK && jK)
+    {
+        v = P[i - k][j - k];
+    }
+    else if (iK)
+    {
+        v = 0.;
+        for (int l = 0; l < b; ++l)
+            v += P[i - k][l] * A[static_cast<size_t>(k + l) * n + j];
+    }
+    else if (jK)
+    {
+        v = 0.;
+        for (int l = 0; l < b; ++l)
+            v -= A[static_cast<size_t>(i) * n + k + l] * P[l][j - k];
+    }
     else
-        v = A[idx] - aik * akj / p;
+    {
+        double w[kPivot];
+        for (int l = 0; l < b; ++l)
+            w[l] = A[static_cast<size_t>(k + l) * n + j];
+        v = A[idx];
+        for (int q = 0; q < b; ++q)
+        {
+            double t2 = 0.;
+            for (int l = 0; l < b; ++l)
+                t2 += P[q][l] * w[l];
+            v -= A[static_cast<size_t>(i) * n + k + q] * t2;
+        }
+    }
     Bk[c.meshOffset[m] + idx] = v;
 }
 

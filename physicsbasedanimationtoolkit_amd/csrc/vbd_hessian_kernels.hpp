@@ -144,6 +144,7 @@ struct BsrArgs
     float4 const* __restrict__ pOld;
     double const* __restrict__ zc; // nullable (block Jacobi): no coarse term
     int const* __restrict__ aggOf;
+    int const* __restrict__ colAgg; // aggregate of each block's column (kDirection)
     CgState cg;
     double* __restrict__ acc; // accumulators (dot modes), array accArray
     int accArray;
@@ -151,29 +152,22 @@ struct BsrArgs
     int xcdChunk;
 };
 
-template <BsrMode M>
-__device__ __forceinline__ float4 BsrVector(BsrArgs const& a, int j, float beta, bool frozen)
+// p_j = z_j + zc[g] + beta p_old_j with g the aggregate of vertex j (p_old_j when
+// the mesh has converged)
+__device__ __forceinline__ float4 DirectionVector(BsrArgs const& a, int j, int g, float beta, bool frozen)
 {
-    if constexpr (M == BsrMode::kDirection)
+    float4 const po = a.pOld[j];
+    if (frozen)
+        return po;
+    float4 const zj = a.z[j];
+    float4 r = make_float4(zj.x + beta * po.x, zj.y + beta * po.y, zj.z + beta * po.z, 0.f);
+    if (a.zc)
     {
-        float4 const po = a.pOld[j];
-        if (frozen)
-            return po;
-        float4 const zj = a.z[j];
-        float4 r = make_float4(zj.x + beta * po.x, zj.y + beta * po.y, zj.z + beta * po.z, 0.f);
-        if (a.zc)
-        {
-            int const g = a.aggOf[j];
-            r.x += static_cast<float>(a.zc[3 * g]);
-            r.y += static_cast<float>(a.zc[3 * g + 1]);
-            r.z += static_cast<float>(a.zc[3 * g + 2]);
-        }
-        return r;
+        r.x += static_cast<float>(a.zc[3 * g]);
+        r.y += static_cast<float>(a.zc[3 * g + 1]);
+        r.z += static_cast<float>(a.zc[3 * g + 2]);
     }
-    else
-    {
-        return a.y[j];
-    }
+    return r;
 }
 
 // (H v)_row over blocks [kb, ke) of one row: lane l of the row's L lanes takes
@@ -182,8 +176,9 @@ __device__ __forceinline__ float4 BsrVector(BsrArgs const& a, int j, float beta,
 template <int L, int K, BsrMode M>
 __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, int lane, float beta, bool frozen)
 {
+    constexpr bool kDir = M == BsrMode::kDirection;
     float hx = 0.f, hy = 0.f, hz = 0.f;
-    int cj[K];
+    int cj[K], cg[K];
     float4 A[K], B[K];
     float C[K];
 #pragma unroll
@@ -191,9 +186,12 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
     {
         int const k = kb + lane + u * L;
         cj[u] = -1;
+        cg[u] = 0;
         if (k < ke)
         {
             cj[u] = a.col[k];
+            if constexpr (kDir)
+                cg[u] = a.colAgg[k];
             A[u] = a.va[k];
             B[u] = a.vb[k];
             C[u] = a.vc[k];
@@ -203,7 +201,12 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
 #pragma unroll
     for (int u = 0; u < K; ++u)
         if (cj[u] >= 0)
-            P[u] = BsrVector<M>(a, cj[u], beta, frozen);
+        {
+            if constexpr (kDir)
+                P[u] = DirectionVector(a, cj[u], cg[u], beta, frozen);
+            else
+                P[u] = a.y[cj[u]];
+        }
 #pragma unroll
     for (int u = 0; u < K; ++u)
         if (cj[u] >= 0)
@@ -216,7 +219,11 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
     {
         float4 const Ak = a.va[k], Bk = a.vb[k];
         float const Ck = a.vc[k];
-        float4 const Pk = BsrVector<M>(a, a.col[k], beta, frozen);
+        float4 Pk;
+        if constexpr (kDir)
+            Pk = DirectionVector(a, a.col[k], a.colAgg[k], beta, frozen);
+        else
+            Pk = a.y[a.col[k]];
         hx += Ak.x * Pk.x + Ak.y * Pk.y + Ak.z * Pk.z;
         hy += Ak.w * Pk.x + Bk.x * Pk.y + Bk.y * Pk.z;
         hz += Bk.z * Pk.x + Bk.w * Pk.y + Ck * Pk.z;
@@ -305,10 +312,13 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __res
     bool const frozen = a.cg.F[kDone * nM + m] != 0;
     float const beta = static_cast<float>(a.cg.P[kBeta * nM + m]);
     int kb = 0, ke = 0;
+    float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
     if (active)
     {
         kb = a.rowPtr[row];
         ke = a.rowPtr[row + 1];
+        if (lane == 0) // the row's own direction, in flight during the product
+            pr = DirectionVector(a, row, a.zc ? a.aggOf[row] : 0, beta, frozen);
     }
     float3 const h = RowProduct<L, K, BsrMode::kDirection>(a, kb, ke, lane, beta, frozen);
     __shared__ double sdot[R];
@@ -317,7 +327,6 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __res
         double dot = 0.;
         if (active)
         {
-            float4 const pr = BsrVector<BsrMode::kDirection>(a, row, beta, frozen);
             a.y[row] = pr; // the direction of this iteration (the p_old of the next)
             a.out[row] = make_float4(h.x, h.y, h.z, 0.f);
             dot = double(pr.x) * h.x + double(pr.y) * h.y + double(pr.z) * h.z;
@@ -335,23 +344,23 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __res
 }
 
 // Thread t of a group of N threads sums v[b + t], v[b + t + N], ... in that
-// order, issuing the loads 8 at a time (the order, and so the bits, do not
+// order, issuing the loads 16 at a time (the order, and so the bits, do not
 // depend on the batching: out-of-range entries add an exact 0).
 template <int N>
 __device__ __forceinline__ double StridedSum(double const* __restrict__ v, int b, int e, int t)
 {
     double s = 0.;
-    for (int k0 = b + t; k0 < e; k0 += 8 * N)
+    for (int k0 = b + t; k0 < e; k0 += 16 * N)
     {
-        double x[8];
+        double x[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < 16; ++u)
         {
             int const k = k0 + u * N;
             x[u] = k < e ? v[k] : 0.;
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < 16; ++u)
             s += x[u];
     }
     return s;
@@ -418,6 +427,24 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
     __shared__ double lds6[2][6];
     int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
     int const m = u.aggMesh[g];
+    int const kBeg = u.aggPtr[g] + t, kEnd = u.aggPtr[g + 1];
+    // the thread's first vertex: loads issued before alpha is known
+    int i0 = -1;
+    float4 const zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 q0 = zero4, a0 = zero4, y0 = zero4, r0 = zero4, d0 = zero4, d1 = zero4;
+    if (kBeg < kEnd)
+    {
+        i0 = u.aggVerts[kBeg];
+        q0 = u.q[i0];
+        a0 = u.init ? u.b[i0] : u.p[i0];
+        if (!u.init)
+        {
+            y0 = u.y[i0];
+            r0 = u.r[i0];
+        }
+        d0 = u.dinv[2 * i0];
+        d1 = u.dinv[2 * i0 + 1];
+    }
     float alpha = 0.f;
     if (!u.init)
     {
@@ -427,32 +454,42 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
         alpha = (!done && pq[0] != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq[0]) : 0.f;
     }
     double sx = 0., sy = 0., sz = 0., rz = 0., rr = 0., bb = 0.;
-    for (int k = u.aggPtr[g] + t; k < u.aggPtr[g + 1]; k += 128)
-    {
-        int const i = u.aggVerts[k];
-        float4 const qi = u.q[i];
-        float4 ri;
+    // init: a = b, r = b - q; update: a = p, y += alpha p, r -= alpha q
+    auto update = [&](int i, float4 const& qi, float4 const& ai, float4 yi, float4 ri, float4 const& D0,
+                      float4 const& D1) {
         if (u.init)
         {
-            float4 const bi = u.b[i];
-            ri = make_float4(bi.x - qi.x, bi.y - qi.y, bi.z - qi.z, 0.f);
-            bb += Dot3(bi, bi);
+            ri = make_float4(ai.x - qi.x, ai.y - qi.y, ai.z - qi.z, 0.f);
+            bb += Dot3(ai, ai);
         }
         else
         {
-            float4 const pi = u.p[i];
-            float4 yi = u.y[i];
-            ri = u.r[i];
-            yi.x += alpha * pi.x, yi.y += alpha * pi.y, yi.z += alpha * pi.z;
+            yi.x += alpha * ai.x, yi.y += alpha * ai.y, yi.z += alpha * ai.z;
             ri.x -= alpha * qi.x, ri.y -= alpha * qi.y, ri.z -= alpha * qi.z;
             u.y[i] = yi;
         }
         u.r[i] = ri;
-        float4 const zi = ApplyInverse(u.dinv, i, ri);
+        float4 const zi = make_float4(D0.x * ri.x + D0.y * ri.y + D0.z * ri.z,
+                                      D0.y * ri.x + D0.w * ri.y + D1.x * ri.z,
+                                      D0.z * ri.x + D1.x * ri.y + D1.y * ri.z, 0.f);
         u.z[i] = zi;
         sx += ri.x, sy += ri.y, sz += ri.z;
         rz += Dot3(ri, zi);
         rr += Dot3(ri, ri);
+    };
+    if (i0 >= 0)
+        update(i0, q0, a0, y0, r0, d0, d1);
+    for (int k = kBeg + 128; k < kEnd; k += 128)
+    {
+        int const i = u.aggVerts[k];
+        float4 const qi = u.q[i], ai = u.init ? u.b[i] : u.p[i];
+        float4 yi = qi, ri = qi;
+        if (!u.init)
+        {
+            yi = u.y[i];
+            ri = u.r[i];
+        }
+        update(i, qi, ai, yi, ri, u.dinv[2 * i], u.dinv[2 * i + 1]);
     }
     double sums[6] = {rz, rr, bb, sx, sy, sz};
     Sum128<6>(sums, lds6);
