@@ -575,13 +575,15 @@ struct SolveResult
     double relResidual = 0.;    // largest per-mesh |r| / |b| at its last iteration
 };
 
+// Meshes whose solve has not converged yet (their done flags).
 int ReadLive(pbat_vbd_integrator& h)
 {
-    int live = 0;
     int const nM = h.nMeshes;
-    PBAT_HIP(hipMemcpyAsync(&live, h.rollout.flags.ptr + 2 * nM, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    std::vector<int> done(static_cast<size_t>(nM));
+    PBAT_HIP(hipMemcpyAsync(done.data(), h.rollout.flags.ptr + device::kDone * nM, sizeof(int) * nM,
+                            hipMemcpyDeviceToHost, h.stream));
     PBAT_HIP(hipStreamSynchronize(h.stream));
-    return live;
+    return static_cast<int>(std::count(done.begin(), done.end(), 0));
 }
 
 SolveResult ReadResult(pbat_vbd_integrator& h)
@@ -716,7 +718,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         f.cg = cg, f.meshAggBegin = h.meshAggBegin.ptr;
         f.rzPart = R.rzPart.ptr, f.rrPart = R.rrPart.ptr, f.bbPart = R.bbPart.ptr;
         f.rczcPart = twoLevel ? R.rczcPart.ptr : nullptr;
-        hipLaunchKernelGGL(device::CgFinish, dim3(1), dim3(256), 0, h.stream, f, first);
+        hipLaunchKernelGGL(device::CgFinish, dim3((nM + 3) / 4), dim3(256), 0, h.stream, f, first);
     };
     std::function<void(int)> iteration;
     if (fused)

@@ -655,7 +655,8 @@ struct CoarseArgs
 //   B[K, K] = P^-1,   B[K, j] = P^-1 A[K, j],   B[i, K] = -A[i, K] P^-1,
 //   B[i, j] = A[i, j] - A[i, K] P^-1 A[K, j]     (i, j outside K),
 // the same as b scalar steps; after ceil(n / kPivot) steps an n x n SPD block
-// holds its inverse. Every workgroup inverts P in LDS (SPD, no pivoting).
+// holds its inverse. Every workgroup inverts P in LDS (SPD, no pivoting), one
+// thread per entry.
 constexpr int kPivot = 8;
 
 __global__ __launch_bounds__(256) void CoarseInvertStep(double const* __restrict__ Ak, double* __restrict__ Bk,
@@ -669,32 +670,32 @@ __global__ __launch_bounds__(256) void CoarseInvertStep(double const* __restrict
     double const* A = Ak + c.meshOffset[m];
     __shared__ double P[kPivot][kPivot];
     int const t = static_cast<int>(threadIdx.x);
-    if (t < b * b)
-        P[t / b][t % b] = A[static_cast<size_t>(k + t / b) * n + k + t % b];
+    int const pr = t / kPivot, pc = t % kPivot;
+    bool const inP = t < kPivot * kPivot && pr < b && pc < b;
+    if (inP)
+        P[pr][pc] = A[static_cast<size_t>(k + pr) * n + k + pc];
     __syncthreads();
-    if (t == 0)
+    // Gauss-Jordan inverse of the b x b pivot block in place, one thread per entry
+    for (int q = 0; q < b; ++q)
     {
-        // in-place Gauss-Jordan inverse of the b x b pivot block
-        for (int q = 0; q < b; ++q)
+        double v = 0.;
+        if (inP)
         {
             double const inv = 1. / P[q][q];
-            for (int r = 0; r < b; ++r)
-            {
-                if (r == q)
-                    continue;
-                double const f = P[r][q] * inv;
-                for (int cc = 0; cc < b; ++cc)
-                    if (cc != q)
-                        P[r][cc] -= f * P[q][cc];
-                P[r][q] = -f;
-            }
-            for (int cc = 0; cc < b; ++cc)
-                if (cc != q)
-                    P[q][cc] *= inv;
-            P[q][q] = inv;
+            if (pr == q && pc == q)
+                v = inv;
+            else if (pr == q)
+                v = P[q][pc] * inv;
+            else if (pc == q)
+                v = -P[pr][q] * inv;
+            else
+                v = P[pr][pc] - P[pr][q] * P[q][pc] * inv;
         }
+        __syncthreads();
+        if (inP)
+            P[pr][pc] = v;
+        __syncthreads();
     }
-    __syncthreads();
     int64_t const idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= static_cast<int64_t>(n) * n)
         return;

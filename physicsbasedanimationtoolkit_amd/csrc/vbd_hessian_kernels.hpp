@@ -289,27 +289,27 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
 //
 // One workgroup per chunk (XCD-aware order when xcdChunk > 0): the direction
 // p = z + Z zc + beta p_old of each of its rows is formed on the fly for every
-// block (p_old when the mesh has converged), written to y, q = H p to out, and
-// the chunk's p . q, summed over its rows in order, to pqPart[chunk].
+// block, written to y, q = H p to out, and the chunk's p . q, summed over its
+// rows in order, to pqPart[chunk]. Chunks of a converged mesh do nothing.
 template <int L, int K>
 __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __restrict__ chunks, int nChunks,
                                                     double* __restrict__ pqPart)
 {
     constexpr int R = 256 / L;
     int const nM = a.cg.nM;
-    if (a.cg.F[2 * nM] == 0)
-        return; // every mesh has converged: nothing left to do
     int w = static_cast<int>(blockIdx.x);
     if (a.xcdChunk > 0)
         w = (w & 7) * a.xcdChunk + (w >> 3);
     if (w >= nChunks)
         return; // padding of the XCD order
     int2 const ch = chunks[w];
+    int const m = a.meshOf[ch.x];
+    if (a.cg.F[kDone * nM + m] != 0)
+        return; // the chunk's mesh has converged: its vectors no longer change
+    constexpr bool frozen = false;
     int const r = static_cast<int>(threadIdx.x) / L, lane = static_cast<int>(threadIdx.x) & (L - 1);
     int const row = ch.x + r;
     bool const active = row < ch.y;
-    int const m = a.meshOf[ch.x];
-    bool const frozen = a.cg.F[kDone * nM + m] != 0;
     float const beta = static_cast<float>(a.cg.P[kBeta * nM + m]);
     int kb = 0, ke = 0;
     float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -421,12 +421,12 @@ struct AggUpdateArgs
 __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 {
     int const nM = u.cg.nM;
-    if (!u.init && u.cg.F[2 * nM] == 0)
-        return; // every mesh has converged
     __shared__ double lds1[2][1];
     __shared__ double lds6[2][6];
     int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
     int const m = u.aggMesh[g];
+    if (!u.init && u.cg.F[kDone * nM + m] != 0)
+        return; // the aggregate's mesh has converged
     int const kBeg = u.aggPtr[g] + t, kEnd = u.aggPtr[g + 1];
     // the thread's first vertex: loads issued before alpha is known
     int i0 = -1;
@@ -517,14 +517,14 @@ __global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict_
                                                        CoarseArgs c, int wpr, CgState cg,
                                                        double* __restrict__ rczcPart, int first)
 {
-    if (!first && cg.F[2 * cg.nM] == 0)
-        return; // every mesh has converged
     __shared__ double part[4];
     int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
     int const row = static_cast<int>(blockIdx.x) * (4 / wpr) + wave / wpr;
     int const sub = wave % wpr;
     double sum = 0.;
-    if (row < 3 * c.nAgg)
+    // rows of converged meshes do nothing
+    bool const live = row < 3 * c.nAgg && (first || cg.F[kDone * cg.nM + c.aggMesh[row / 3]] == 0);
+    if (live)
     {
         int const m = c.aggMesh[row / 3];
         int const b0 = 3 * c.meshAggBegin[m];
@@ -538,7 +538,7 @@ __global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict_
     if (lane == 0)
         part[wave] = sum;
     __syncthreads();
-    if (lane == 0 && sub == 0 && row < 3 * c.nAgg)
+    if (lane == 0 && sub == 0 && live)
     {
         double t = part[wave];
         for (int k = 1; k < wpr; ++k)
@@ -548,10 +548,11 @@ __global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict_
     }
 }
 
-// End of an iteration of the fused CG (one workgroup of 256 threads, one wave
-// per mesh at a time): each mesh's partials summed in a fixed order, r . z =
-// sum of the aggregates' r . D^-1 r (+ the coarse rows' rc . zc), then the
-// stopping test and beta (FinishMesh).
+// End of an iteration of the fused CG, one wave per mesh (4 per workgroup):
+// the mesh's partials summed in a fixed order, r . z = sum of the aggregates'
+// r . D^-1 r (+ the coarse rows' rc . zc), then the stopping test and beta
+// (FinishMesh). Converged meshes are skipped; the host counts live meshes from
+// their done flags.
 struct CgFinishArgs
 {
     CgState cg;
@@ -565,36 +566,24 @@ struct CgFinishArgs
 __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
 {
     int const nM = f.cg.nM;
-    if (!first && f.cg.F[2 * nM] == 0)
-        return; // every mesh has converged
-    __shared__ int live;
-    if (threadIdx.x == 0)
-        live = 0;
-    __syncthreads();
-    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
-    int count = 0;
-    for (int m = wave; m < nM; m += 4)
+    int const m = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
+    int const lane = static_cast<int>(threadIdx.x & 63);
+    if (m >= nM || (!first && f.cg.F[kDone * nM + m] != 0))
+        return;
+    int const a0 = f.meshAggBegin[m], a1 = f.meshAggBegin[m + 1];
+    double rz = StridedSum<64>(f.rzPart, a0, a1, lane);
+    double rr = StridedSum<64>(f.rrPart, a0, a1, lane);
+    double bb = first ? StridedSum<64>(f.bbPart, a0, a1, lane) : 0.;
+    if (f.rczcPart)
+        rz += StridedSum<64>(f.rczcPart, 3 * a0, 3 * a1, lane);
+    for (int off = 32; off > 0; off >>= 1)
     {
-        int const a0 = f.meshAggBegin[m], a1 = f.meshAggBegin[m + 1];
-        double rz = StridedSum<64>(f.rzPart, a0, a1, lane);
-        double rr = StridedSum<64>(f.rrPart, a0, a1, lane);
-        double bb = first ? StridedSum<64>(f.bbPart, a0, a1, lane) : 0.;
-        if (f.rczcPart)
-            rz += StridedSum<64>(f.rczcPart, 3 * a0, 3 * a1, lane);
-        for (int off = 32; off > 0; off >>= 1)
-        {
-            rz += __shfl_xor(rz, off);
-            rr += __shfl_xor(rr, off);
-            bb += __shfl_xor(bb, off);
-        }
-        if (lane == 0)
-            count += FinishMesh(f.cg, m, rr, rz, bb, first) ? 1 : 0;
+        rz += __shfl_xor(rz, off);
+        rr += __shfl_xor(rr, off);
+        bb += __shfl_xor(bb, off);
     }
-    if (count)
-        atomicAdd(&live, count);
-    __syncthreads();
-    if (threadIdx.x == 0)
-        f.cg.F[2 * nM] = live;
+    if (lane == 0)
+        FinishMesh(f.cg, m, rr, rz, bb, first);
 }
 
 // Galerkin coarse operator Z^T H Z from the assembled blocks: coarse block k
