@@ -90,39 +90,42 @@ struct DeviceArray
     }
 };
 
-// Incidence records (structure of arrays, see SweepArgs): the sweep reads
-// ids, geo and mat (40 B); the adjoint's vertex kernel reads ids, g0, g1, g2
-// (the three GP rows, 56 B).
+// Incidence records (structure of arrays, see SweepArgs): the sweep reads sids
+// and sgeo (32 B); the adjoint's vertex kernel reads ids, g0, g1, g2 (the three
+// GP rows, 56 B); shape holds the rest-shape constants the sweep's records are
+// rebuilt from when the moduli change.
 struct IncidenceRecords
 {
-    DeviceArray<int4> ids;
-    DeviceArray<float4> g0, g1, geo;
-    DeviceArray<float2> g2, mat;
-    void reset() { ids.reset(), g0.reset(), g1.reset(), g2.reset(), geo.reset(), mat.reset(); }
+    DeviceArray<int4> ids, sids;
+    DeviceArray<float4> g0, g1, sgeo, shape;
+    DeviceArray<float2> g2;
+    void reset() { ids.reset(), sids.reset(), g0.reset(), g1.reset(), sgeo.reset(), shape.reset(), g2.reset(); }
 };
 
 struct HostIncidenceRecords
 {
-    std::vector<int4> ids;
-    std::vector<float4> g0, g1, geo;
-    std::vector<float2> g2, mat;
+    std::vector<int4> ids, sids;
+    std::vector<float4> g0, g1, sgeo, shape;
+    std::vector<float2> g2;
     void resize(size_t n)
     {
         ids.assign(n, make_int4(-1, 0, 0, 0)); // a = -1: unused fixed-stride slot
+        sids = ids;
         g0.assign(n, make_float4(0.f, 0.f, 0.f, 0.f));
         g1 = g0;
-        geo = g0;
+        sgeo = g0;
+        shape = g0;
         g2.assign(n, make_float2(0.f, 0.f));
-        mat = g2;
     }
     void upload(IncidenceRecords& d, hipStream_t s) const
     {
         d.ids.upload(ids, s);
+        d.sids.upload(sids, s);
         d.g0.upload(g0, s);
         d.g1.upload(g1, s);
         d.g2.upload(g2, s);
-        d.geo.upload(geo, s);
-        d.mat.upload(mat, s);
+        d.sgeo.upload(sgeo, s);
+        d.shape.upload(shape, s);
     }
 };
 

@@ -994,10 +994,13 @@ void SetLogYoung(pbat_vbd_integrator& h, float const* logY, int64_t nT)
     std::vector<float> v(logY, logY + nT);
     R.logY.upload(v, h.stream);
     bool const ell = h.ellStride > 0;
+    auto set = [](int4 const* slots, IncidenceRecords& r) {
+        return device::RecordSet{slots, r.ids.ptr, r.g2.ptr, r.sids.ptr, r.sgeo.ptr, r.shape.ptr};
+    };
+    device::RecordSet const rows = set(h.slotsRows.ptr, h.rows);
+    device::RecordSet const ellRows = ell ? set(h.slotsEll.ptr, h.ell) : device::RecordSet{};
     hipLaunchKernelGGL(device::SetLogYoung, dim3(Blocks(nT)), dim3(256), 0, h.stream, R.logY.ptr, h.matBase.ptr,
-                       h.tgp.ptr, h.slotsRows.ptr, h.rows.ids.ptr, h.rows.g2.ptr, h.rows.mat.ptr,
-                       ell ? h.slotsEll.ptr : nullptr, ell ? h.ell.ids.ptr : nullptr, ell ? h.ell.g2.ptr : nullptr,
-                       ell ? h.ell.mat.ptr : nullptr, static_cast<int>(nT));
+                       h.tgp.ptr, rows, ellRows, static_cast<int>(nT));
     PBAT_HIP(hipGetLastError());
     PBAT_HIP(hipStreamSynchronize(h.stream));
     for (int64_t e = 0; e < nT; ++e)

@@ -796,14 +796,23 @@ __global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)
 }
 
 // Rescale per-tet Lame coefficients to Young's moduli exp(logY) (nu fixed):
-// w*mu and w*lambda of the tet constants and of its incidence records (w*mu in
-// the records' id words, w*lambda in the .y of their g2 and mat pairs).
+// w*mu and w*lambda of the tet constants and of its incidence records: the
+// adjoint's records (w*mu in the id words, w*lambda in g2.y) and the sweep's,
+// rebuilt from their rest-shape constants shape = (gamma_a, gamma_b, gamma_c,
+// kappa): A = w lambda kappa^2 in the id words, sgeo = (w mu gamma_k,
+// kappa (w lambda + w mu)), in FP64 as the host builds them.
+struct RecordSet
+{
+    int4 const* __restrict__ slots; // record slot of each (tet, local vertex), -1: none
+    int4* __restrict__ ids;
+    float2* __restrict__ g2;
+    int4* __restrict__ sids;
+    float4* __restrict__ sgeo;
+    float4 const* __restrict__ shape;
+};
+
 __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ logY, float2 const* __restrict__ base,
-                                                   float4* __restrict__ tgp, int4 const* __restrict__ slots,
-                                                   int4* __restrict__ ids, float2* __restrict__ g2,
-                                                   float2* __restrict__ mat, int4 const* __restrict__ slotsEll,
-                                                   int4* __restrict__ idsEll, float2* __restrict__ g2Ell,
-                                                   float2* __restrict__ matEll, int nT)
+                                                   float4* __restrict__ tgp, RecordSet rows, RecordSet ell, int nT)
 {
     int const e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= nT)
@@ -814,21 +823,31 @@ __global__ __launch_bounds__(256) void SetLogYoung(float const* __restrict__ log
     float4 t = tgp[3 * e + 2];
     t.y = wmu, t.z = wlam;
     tgp[3 * e + 2] = t;
-    auto update = [&](int4 const& sl, int4* I, float2* G, float2* Mt) {
+    double const dmu = wmu, dlam = wlam;
+    auto update = [&](RecordSet const& R) {
+        int4 const sl = R.slots[e];
         int const s4[4] = {sl.x, sl.y, sl.z, sl.w};
         for (int k = 0; k < 4; ++k)
-            if (s4[k] >= 0)
-            {
-                int4 r = I[s4[k]];
-                r.w = __float_as_int(wmu);
-                I[s4[k]] = r;
-                G[s4[k]].y = wlam;
-                Mt[s4[k]].y = wlam;
-            }
+        {
+            int const q = s4[k];
+            if (q < 0)
+                continue;
+            int4 r = R.ids[q];
+            r.w = __float_as_int(wmu);
+            R.ids[q] = r;
+            R.g2[q].y = wlam;
+            float4 const sh = R.shape[q];
+            double const kap = sh.w;
+            int4 si = R.sids[q];
+            si.w = __float_as_int(static_cast<float>(dlam * kap * kap));
+            R.sids[q] = si;
+            R.sgeo[q] = make_float4(static_cast<float>(dmu * sh.x), static_cast<float>(dmu * sh.y),
+                                    static_cast<float>(dmu * sh.z), static_cast<float>(kap * (dlam + dmu)));
+        }
     };
-    update(slots[e], ids, g2, mat);
-    if (g2Ell)
-        update(slotsEll[e], idsEll, g2Ell, matEll);
+    update(rows);
+    if (ell.slots)
+        update(ell);
 }
 
 } // namespace pbat_vbd::device

@@ -136,9 +136,8 @@ device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2)
     device::SweepArgs a{};
     pbat_vbd::IncidenceRecords const& r = (h.useEll && h.ellStride > 0) ? h.ell : h.rows;
     a.vtp = h.vtp.ptr;
-    a.ids = r.ids.ptr;
-    a.geo = r.geo.ptr;
-    a.mat = r.mat.ptr;
+    a.ids = r.sids.ptr;
+    a.geo = r.sgeo.ptr;
     a.xm = h.xm.ptr;
     a.xtil = h.xtil.ptr;
     a.xt = h.xt.ptr;
@@ -834,6 +833,7 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         int const il = static_cast<int>(d->GVGilocal[k]);
         setSlot(slots[e], il, static_cast<int>(slot));
         double const* G = d->GP + 12 * e; // G(r, c) = G[4c + r]
+        double const w = d->wg[e], mu = d->lame[2 * e], lam = d->lame[2 * e + 1];
         int ids[3];
         float g[3][3];
         double gd[3][3], own[3];
@@ -859,15 +859,25 @@ void Create(pbat_vbd_data const* d, pbat_vbd_integrator** out, std::vector<int64
         R.g0[slot] = make_float4(g[0][0], g[0][1], g[0][2], g[1][0]);
         R.g1[slot] = make_float4(g[1][1], g[1][2], g[2][0], g[2][1]);
         R.g2[slot] = make_float2(g[2][2], m.z);
-        // sweep form (FP64 from the FP64 GP): g_k . g of the other rows with the
-        // own row g, |g|^2, and det of the other three rows
+        // sweep record (FP64 from the FP64 GP and material): with gamma_k = g_k . g
+        // (the other rows with the own row g) and kappa = det of the other three
+        // rows, sids.w = bits of w lambda kappa^2 and sgeo = (w mu gamma_a,
+        // w mu gamma_b, w mu gamma_c, kappa (w lambda + w mu)); shape keeps
+        // (gamma_a, gamma_b, gamma_c, kappa) for later changes of the moduli
         auto dot = [](double const* a, double const* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
-        double const detG = gd[0][0] * (gd[1][1] * gd[2][2] - gd[1][2] * gd[2][1]) -
-                            gd[0][1] * (gd[1][0] * gd[2][2] - gd[1][2] * gd[2][0]) +
-                            gd[0][2] * (gd[1][0] * gd[2][1] - gd[1][1] * gd[2][0]);
-        R.geo[slot] = make_float4(static_cast<float>(dot(gd[0], own)), static_cast<float>(dot(gd[1], own)),
-                                  static_cast<float>(dot(gd[2], own)), static_cast<float>(dot(own, own)));
-        R.mat[slot] = make_float2(static_cast<float>(detG), m.z);
+        double const kappa = gd[0][0] * (gd[1][1] * gd[2][2] - gd[1][2] * gd[2][1]) -
+                             gd[0][1] * (gd[1][0] * gd[2][2] - gd[1][2] * gd[2][0]) +
+                             gd[0][2] * (gd[1][0] * gd[2][1] - gd[1][1] * gd[2][0]);
+        double const ga = dot(gd[0], own), gb = dot(gd[1], own), gc = dot(gd[2], own);
+        double const wmu = w * mu, wlam = w * lam;
+        float const A = static_cast<float>(wlam * kappa * kappa);
+        int Abits;
+        std::memcpy(&Abits, &A, sizeof(int));
+        R.sids[slot] = make_int4(ids[0], ids[1], ids[2], Abits);
+        R.sgeo[slot] = make_float4(static_cast<float>(wmu * ga), static_cast<float>(wmu * gb),
+                                   static_cast<float>(wmu * gc), static_cast<float>(kappa * (wlam + wmu)));
+        R.shape[slot] = make_float4(static_cast<float>(ga), static_cast<float>(gb), static_cast<float>(gc),
+                                    static_cast<float>(kappa));
     };
     {
         HostIncidenceRecords R;

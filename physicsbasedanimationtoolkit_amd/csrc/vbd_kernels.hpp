@@ -7,9 +7,9 @@
 //   xt[v]    float4 (x_t, 0)          positions at the start of the substep
 //   vel[v], vprev[v], aext[v]  float4 velocities and external acceleration
 //   vtp[v]   int32 CSR prefix over vertex-tet incidences
-//   incidence records (40 B, structure of arrays, in sweep order): the other
-//            three vertex ids of the tet, rest-shape constants of its GP rows
-//            and the tet's w*mu, w*lambda (see SweepArgs)
+//   incidence records (32 B, structure of arrays, in sweep order): the other
+//            three vertex ids of the tet and four constants of its GP rows and
+//            material (see SweepArgs)
 //   tgp[3e..3e+2]  float4 x3 (48 B) per tet: rows 1..3 of the shape function
 //            gradients GP (row 0 = -(row1+row2+row3)), w*mu, w*lambda, alpha
 //   tetv[e]  int4 vertex ids of tet e
@@ -83,15 +83,18 @@ __device__ __forceinline__ void GroupSum(Block& B)
 // det(F0 + x_i g^T) is affine in x_i). alpha = 1 + mu/lambda, so
 // w lambda (J - alpha) = w lambda (J - 1) - w mu.
 // With a, b, c the tet's other vertices, d_k = x_k - x, G the matrix of their
-// shape-function gradient rows g_a, g_b, g_c and g = -(g_a + g_b + g_c) the own
-// row, F = D G with D = [d_a d_b d_c], so
-//   F g = sum_k (g_k . g) d_k,   J = det D det G = det G (d_a . n),
-//   c = cof(D) cof(G) g = -det G n,
+// shape-function gradient rows g_a, g_b, g_c, g = -(g_a + g_b + g_c) the own
+// row and kappa = det G, F = D G with D = [d_a d_b d_c], so
+//   F g = sum_k gamma_k d_k (gamma_k = g_k . g),   J = kappa (d_a . n),
+//   c = cof(D) cof(G) g = -kappa n,   |g|^2 = -sum_k gamma_k,
 // where n = (x_b - x_a) x (x_c - x_a) (the column sum of cof D) and G^-T g =
-// -(1, 1, 1). The record holds geo = (g_a . g, g_b . g, g_c . g, |g|^2) and
-// mat = (det G, w lambda), all rest-shape constants computed in FP64.
-__device__ __forceinline__ void AccumulateElastic(float4 const& geo, float2 const& mat, float wmu, float4 const& own,
-                                                  float4 const& pa, float4 const& pb, float4 const& pc, Block& B)
+// -(1, 1, 1). With the record's A = w lambda kappa^2, B = kappa (w lambda + w mu)
+// and gamma'_k = w mu gamma_k (rest-shape and material constants, FP64 on the
+// host), the terms are
+//   grad_i += sum_k gamma'_k d_k - (A (d_a . n) - B) n
+//   Hess_i += -(sum_k gamma'_k) I + A n n^T.
+__device__ __forceinline__ void AccumulateElastic(float4 const& geo, float A, float4 const& own, float4 const& pa,
+                                                  float4 const& pb, float4 const& pc, Block& B)
 {
     float const dax = pa.x - own.x, day = pa.y - own.y, daz = pa.z - own.z;
     float const dbx = pb.x - own.x, dby = pb.y - own.y, dbz = pb.z - own.z;
@@ -99,24 +102,18 @@ __device__ __forceinline__ void AccumulateElastic(float4 const& geo, float2 cons
     float const ex = pb.x - pa.x, ey = pb.y - pa.y, ez = pb.z - pa.z;
     float const fx = pc.x - pa.x, fy = pc.y - pa.y, fz = pc.z - pa.z;
     float const nx = ey * fz - ez * fy, ny = ez * fx - ex * fz, nz = ex * fy - ey * fx;
-    float const detG = mat.x, wlam = mat.y;
-    float const J = detG * (dax * nx + day * ny + daz * nz);
-    float const kx = -detG * nx, ky = -detG * ny, kz = -detG * nz;
-    float const Fgx = geo.x * dax + geo.y * dbx + geo.z * dcx;
-    float const Fgy = geo.x * day + geo.y * dby + geo.z * dcy;
-    float const Fgz = geo.x * daz + geo.y * dbz + geo.z * dcz;
-    float const s = wlam * (J - 1.f) - wmu; // = w lambda (J - alpha), without a division
-    B.g0 += wmu * Fgx + s * kx;
-    B.g1 += wmu * Fgy + s * ky;
-    B.g2 += wmu * Fgz + s * kz;
-    float const dg = wmu * geo.w;
-    float const lkx = wlam * kx, lky = wlam * ky, lkz = wlam * kz;
-    B.h00 += dg + lkx * kx;
-    B.h11 += dg + lky * ky;
-    B.h22 += dg + lkz * kz;
-    B.h01 += lkx * ky;
-    B.h02 += lkx * kz;
-    B.h12 += lky * kz;
+    float const coef = A * (dax * nx + day * ny + daz * nz) - geo.w;
+    B.g0 += geo.x * dax + geo.y * dbx + geo.z * dcx - coef * nx;
+    B.g1 += geo.x * day + geo.y * dby + geo.z * dcy - coef * ny;
+    B.g2 += geo.x * daz + geo.y * dbz + geo.z * dcz - coef * nz;
+    float const dg = -(geo.x + geo.y + geo.z);
+    float const anx = A * nx, any = A * ny, anz = A * nz;
+    B.h00 += dg + anx * nx;
+    B.h11 += dg + any * ny;
+    B.h22 += dg + anz * nz;
+    B.h01 += anx * ny;
+    B.h02 += anx * nz;
+    B.h12 += any * nz;
 }
 
 struct StepConstants
@@ -195,19 +192,16 @@ __device__ __forceinline__ float3 InertiaNewtonStep(Block B, float4 const& xv, f
 }
 
 // Incidence records, structure of arrays, row k of vertex v's incidences
-// (CSR rows through vtp, or fixed-stride rows of ellStride entries), 40 B:
-//   ids[k] int4   (a, b, c, w*mu as bits): the tet's other three vertices in
+// (CSR rows through vtp, or fixed-stride rows of ellStride entries), 32 B:
+//   ids[k] int4   (a, b, c, A as bits): the tet's other three vertices in
 //                 local order; a = -1 marks an unused fixed-stride slot
-//   geo[k] float4 (ga . g, gb . g, gc . g, |g|^2) and mat[k] float2
-//                 (det [ga; gb; gc], w*lambda), with ga, gb, gc the GP rows of
-//                 a, b, c and g the vertex's own row (AccumulateElastic)
+//   geo[k] float4 (gamma'_a, gamma'_b, gamma'_c, B) (AccumulateElastic)
 // Records are stored in sweep order, so a colour pass streams them.
 struct SweepArgs
 {
     int const* __restrict__ vtp;
     int4 const* __restrict__ ids;
     float4 const* __restrict__ geo;
-    float2 const* __restrict__ mat;
     float4* __restrict__ xm;
     float4 const* __restrict__ xtil;
     float4 const* __restrict__ xt;
@@ -329,7 +323,6 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
     }
     int4 I[K];
     float4 q0[K];
-    float2 q2[K];
 #pragma unroll
     for (int j = 0; j < K; ++j)
     {
@@ -338,7 +331,6 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
         {
             I[j] = LoadIds(a.ids + k, nt);
             q0[j] = LoadConst(a.geo + k, nt);
-            q2[j] = LoadConst(a.mat + k, nt);
         }
         else
             I[j] = make_int4(-1, 0, 0, 0);
@@ -366,7 +358,7 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
 #pragma unroll
     for (int j = 0; j < K; ++j)
         if (I[j].x >= 0)
-            AccumulateElastic(q0[j], q2[j], __int_as_float(I[j].w), own, pa[j], pb[j], pc[j], B);
+            AccumulateElastic(q0[j], __int_as_float(I[j].w), own, pa[j], pb[j], pc[j], B);
     // incidences beyond K per lane (vertices of unusually high degree)
     for (int k = kb + lane + K * L; k < ke; k += L)
     {
@@ -374,12 +366,12 @@ __global__ __launch_bounds__(256) void VbdSweep(SweepArgs a)
         if (J.x < 0)
             continue;
         if constexpr (C)
-            AccumulateElastic(LoadConst(a.geo + k, nt), LoadConst(a.mat + k, nt), __int_as_float(J.w), own,
+            AccumulateElastic(LoadConst(a.geo + k, nt), __int_as_float(J.w), own,
                               ContactPosition(a.contact, a.xm, J.x), ContactPosition(a.contact, a.xm, J.y),
                               ContactPosition(a.contact, a.xm, J.z), B);
         else
-            AccumulateElastic(LoadConst(a.geo + k, nt), LoadConst(a.mat + k, nt), __int_as_float(J.w), own,
-                              a.xm[J.x], a.xm[J.y], a.xm[J.z], B);
+            AccumulateElastic(LoadConst(a.geo + k, nt), __int_as_float(J.w), own, a.xm[J.x], a.xm[J.y],
+                              a.xm[J.z], B);
     }
     if constexpr (C)
     {

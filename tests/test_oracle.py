@@ -312,12 +312,15 @@ def test_vertex_block_equals_closed_form():
 
 
 def test_vertex_block_equals_record_form():
-    """The sweep's per-incidence form from its 40-byte records equals the reference's
+    """The sweep's per-incidence form from its 32-byte records equals the reference's
     9x9 contraction: with the other vertices a, b, c in local order, d_k = x_k - x,
-    G = [g_a; g_b; g_c] and the own row g, the records hold (g_k . g, |g|^2) and det G,
-    and the kernel takes F g = sum_k (g_k . g) d_k, J = det G (d_a . n) and
-    c = cof(F) g = -det G n with n = (x_b - x_a) x (x_c - x_a). A dropped sign on
-    c or a wrong local order fails here (the 9x9 side is the oracle's own code)."""
+    G = [g_a; g_b; g_c], the own row g, gamma_k = g_k . g and kappa = det G, the
+    record holds A = w lambda kappa^2, B = kappa (w lambda + w mu) and
+    gamma'_k = w mu gamma_k, and the kernel takes
+    grad += sum_k gamma'_k d_k - (A (d_a . n) - B) n and
+    Hess += -(sum_k gamma'_k) I + A n n^T with n = (x_b - x_a) x (x_c - x_a).
+    A dropped sign, a wrong local order or a wrong power of kappa fails here (the
+    9x9 side is the oracle's own code)."""
     X, E = meshes.cube_tet_grid(3, jitter=0.1, seed=21)
     rng = np.random.default_rng(22)
     nT = E.shape[1]
@@ -335,17 +338,14 @@ def test_vertex_block_equals_record_form():
             others = [j for j in range(4) if j != il]
             g = GP[e][il]
             Gk = GP[e][others]
-            geo = Gk @ g
-            detG = np.linalg.det(Gk)
+            kappa = np.linalg.det(Gk)
+            wmu, wlam = o.wg[e] * mu[e], o.wg[e] * lam[e]
+            A, B, gp = wlam * kappa**2, kappa * (wlam + wmu), wmu * (Gk @ g)
             xo = o.x[E[others, e]]
             d = xo - o.x[i]
             n = np.cross(xo[1] - xo[0], xo[2] - xo[0])
-            J = detG * (d[0] @ n)
-            c = -detG * n
-            Fg = geo @ d
-            w = o.wg[e]
-            H += w * (mu[e] * (g @ g) * np.eye(3) + lam[e] * np.outer(c, c))
-            gr += w * (mu[e] * Fg + (lam[e] * (J - 1) - mu[e]) * c)
+            gr += gp @ d - (A * (d[0] @ n) - B) * n
+            H += -gp.sum() * np.eye(3) + A * np.outer(n, n)
         assert np.allclose(H9, H, rtol=1e-10, atol=1e-10 * np.abs(H).max())
         assert np.allclose(g9, gr, rtol=1e-8, atol=1e-9 * np.abs(H).max())
 
