@@ -18,8 +18,10 @@ The JSON line also carries
             bounded sample (one step) using the host's cores;
   batched: 256 meshes stepped together vs the plain per-mesh loop;
   adjoint: fwd+adjoint steps/s of a differentiable 20-step rollout on a 32^3 mesh;
-  batched_adjoint: fwd+adjoint steps/s of 64 meshes over 200 steps vs the
-            plain per-mesh loop, with the peak device memory;
+  batched_adjoint: fwd+adjoint steps/s of 64 meshes over 200 steps (second
+            pass; the first, which also builds the handle's solver set-up,
+            is reported beside it and against the plain per-mesh loop), with
+            the peak device memory;
   variants: Chebyshev, Anderson and trust-region steps on the 64^3 workload
             (time and distance to the step's minimiser), and contact on 8
             cubes over a slab (time per step with and without contact).
@@ -236,8 +238,12 @@ def bench_batched_adjoint(args, torch):
     (236,010 vertices, 1,158,930 tets), hidden per-tet log Y, 200 steps at
     h = 1/120 x 10 iterations, state kept every 10 steps and recomputed in the
     backward pass, L = sum |x_T - x_T^target|^2 over the batch, gradient wrt
-    per-tet log Y (and v0, x0). Compared with the plain per-mesh loop: one
-    Integrator per mesh doing the same fwd+adjoint rollout, same sweep shape."""
+    per-tet log Y (and v0, x0). Two passes: the first also builds the handle's
+    Hessian pattern, coarse pattern and captured CG iteration; the second is
+    the steady state of an optimisation loop (fwd_adjoint_steps_per_s).
+    Compared with the plain per-mesh loop: one Integrator per mesh doing one
+    fwd+adjoint rollout (its set-up included), same sweep shape, against the
+    batch's first pass."""
     import numpy as np
     from physicsbasedanimationtoolkit_amd import meshes, pbat, _native
 
@@ -262,19 +268,27 @@ def bench_batched_adjoint(args, torch):
     forward(hidden)
     target = b.x.astype(np.float64)
     _native.device_memory(reset_peak=True)
-    t0 = time.perf_counter()
-    forward(guess)
-    b.synchronize()
-    t1 = time.perf_counter()
-    rb = b.rollout_backward(2 * (b.x - target), None, **solver)
-    tb = time.perf_counter() - t1
-    tf = t1 - t0
+
+    def rollout():
+        t0 = time.perf_counter()
+        forward(guess)
+        b.synchronize()
+        t1 = time.perf_counter()
+        r = b.rollout_backward(2 * (b.x - target), None, **solver)
+        return r, t1 - t0, time.perf_counter() - t1
+
+    # the first pass also builds the handle's Hessian pattern, coarse operator
+    # pattern and captured CG graph (once per handle); the second is the
+    # steady state of an optimisation loop
+    rb, tf0, tb0 = rollout()
+    rb, tf, tb = rollout()
     mem_now, mem_peak = _native.device_memory()
     log(f"[bench] batched adjoint: {len(scenes)} meshes, fwd {tf:.2f}s + adjoint {tb:.2f}s per {T}-step rollout "
-        f"-> {T / (tf + tb):.2f} steps/s; peak device memory {mem_peak / 2**20:.0f} MiB; {rb['stats']}")
+        f"-> {T / (tf + tb):.2f} steps/s (first pass {tf0:.2f}s + {tb0:.2f}s -> {T / (tf0 + tb0):.2f}); "
+        f"peak device memory {mem_peak / 2**20:.0f} MiB; {rb['stats']}")
     if args.batch_no_loop:
         return {"fwd_adjoint_steps_per_s": T / (tf + tb), "forward_s": tf, "adjoint_s": tb,
-                "adjoint_stats": rb["stats"]}
+                "first_pass_steps_per_s": T / (tf0 + tb0), "adjoint_stats": rb["stats"]}
     # plain per-mesh loop, same sweep shape so that each forward is bitwise the batch's
     loop_t = 0.0
     errs = []
@@ -298,13 +312,17 @@ def bench_batched_adjoint(args, torch):
         errs.append(_rel(rb["log_young_modulus"][t0e:t0e + nT], r["log_young_modulus"]))
         t0e += nT
         del gi
-    log(f"[bench] per-mesh loop: {loop_t:.2f}s -> {T / loop_t:.2f} steps/s; batched is {loop_t / (tf + tb):.1f}x; "
-        f"max rel grad diff batch vs loop {max(errs):.2e}")
+    log(f"[bench] per-mesh loop: {loop_t:.2f}s -> {T / loop_t:.2f} steps/s; batched first pass is "
+        f"{loop_t / (tf0 + tb0):.1f}x; max rel grad diff batch vs loop {max(errs):.2e}")
     return {"workload": f"{len(scenes)} jittered meshes, grid dims U{{8..20}} per axis ({int(b.offsets[-1])} verts, "
                         f"{sum(s.n_tets for s in scenes)} tets), hidden per-tet log Y, {T} steps h=1/120 x {it} "
                         f"iterations, checkpoint every {every}, L = |x_T - target|^2, grad wrt per-tet log Y",
-            "fwd_adjoint_steps_per_s": T / (tf + tb), "per_mesh_loop_steps_per_s": T / loop_t,
-            "speedup_vs_loop": loop_t / (tf + tb), "forward_s": tf, "adjoint_s": tb, "loop_s": loop_t,
+            "fwd_adjoint_steps_per_s": T / (tf + tb), "first_pass_steps_per_s": T / (tf0 + tb0),
+            "per_mesh_loop_steps_per_s": T / loop_t,
+            "speedup_vs_loop": loop_t / (tf0 + tb0),
+            "speedup_basis": "first batched pass (handle set-up included) vs one pass of each per-mesh "
+                             "integrator (its set-up included)",
+            "forward_s": tf, "adjoint_s": tb, "first_pass_s": tf0 + tb0, "loop_s": loop_t,
             "max_rel_grad_diff_vs_loop": max(errs), "peak_device_bytes": mem_peak,
             "solver": solver, "adjoint_stats": rb["stats"], "sweep_shape": shape}
 

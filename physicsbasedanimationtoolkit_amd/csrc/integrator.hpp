@@ -146,7 +146,7 @@ struct RolloutState
     std::vector<DeviceArray<float4>> ckptX, ckptV, ckptVprev; // every `every` steps
     std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
     DeviceArray<float4> lastX;                                // positions after the last step
-    DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
+    DeviceArray<float4> y, yPrev, r, z, p, q, rhs, xbar, vbar, hold;
     DeviceArray<float4> dinv;                                 // inverse diagonal blocks, 2 per vertex
     DeviceArray<double> coarse[2];                            // coarse operator / its inverse (ping-pong)
     DeviceArray<int> coarseFinal;                             // per mesh: buffer holding its inverse
@@ -156,9 +156,13 @@ struct RolloutState
     DeviceArray<double> grad;                                 // d L / d log Y per tet
     DeviceArray<float> logY;                                  // parameter upload buffer
     // step Hessian assembled per solve as 3x3 blocks over the updated vertices:
-    // the pattern and assembly sources (built once per handle), the values, the
-    // longest row, and the fused iteration's second direction buffer
+    // the pattern and assembly sources (built once per handle: per row its
+    // incidences and their tets' row positions, or, for rows too long for the
+    // row-major assembly, per block its sources), the values, the longest row,
+    // and the fused iteration's second direction buffer
     DeviceArray<int> bsrRowPtr, bsrCol, bsrBlockRow, bsrSrcPtr, bsrSrc;
+    DeviceArray<int> incPtr, incCode;
+    DeviceArray<int4> incSlot;
     DeviceArray<int> bsrColAgg; // aggregate of each block's column
     DeviceArray<float4> bsrA, bsrB;
     DeviceArray<float> bsrC;
@@ -177,6 +181,13 @@ struct RolloutState
     // captured chunk of CG iterations and the settings it was captured for
     hipGraph_t cgGraph = nullptr;
     hipGraphExec_t cgExec = nullptr;
+    // host-mapped copy of the per-mesh done flags the CG's finish kernel writes
+    // (doneDev its device address), and the events that mark the ends of the
+    // two most recent chunks of iterations
+    int* doneHost = nullptr;
+    int* doneDev = nullptr;
+    int doneHostMeshes = 0;
+    hipEvent_t chunkEvent[2] = {nullptr, nullptr};
     std::vector<double> cgKey;
     std::vector<void const*> cgPtrs; // device addresses the captured kernels take
     void ResetGraph()
@@ -343,6 +354,11 @@ struct pbat_vbd_integrator
         objVtp.reset(), objInc.reset(), objX.reset(), objXtil.reset(), objGrad.reset(), objPart.reset();
         if (statusHost)
             (void)hipHostFree(statusHost);
+        if (rollout.doneHost)
+            (void)hipHostFree(rollout.doneHost);
+        for (hipEvent_t e : rollout.chunkEvent)
+            if (e)
+                (void)hipEventDestroy(e);
         if (stream)
             (void)hipStreamDestroy(stream);
     }

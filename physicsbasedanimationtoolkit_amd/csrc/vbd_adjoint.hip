@@ -130,7 +130,7 @@ void EnsureBuffers(pbat_vbd_integrator& h)
 {
     RolloutState& R = h.rollout;
     size_t const n = static_cast<size_t>(h.nV);
-    for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
+    for (auto* b : {&R.y, &R.yPrev, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
     if (R.dinv.n != 2 * n)
@@ -241,6 +241,9 @@ void VertexPass(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, f
 
 int BsrLanes(pbat_vbd_integrator const& h);
 
+// Longest block row the row-major assembly takes (4 blocks per lane of 32).
+constexpr int kRowAssembleMaxRow = 128;
+
 // Block pattern of the step Hessian over the updated vertices (row i lists the
 // updated vertices sharing a tet with i, itself included, ascending) and each
 // block's assembly sources, from the element table; built once per handle.
@@ -295,7 +298,6 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
         maxRow = std::max(maxRow, static_cast<int>(cols.size()));
     }
     size_t const nBlocks = col.size();
-    std::vector<int> srcPtr(nBlocks + 1, 0), src;
     auto forEachSource = [&](auto&& f) {
         for (int i = 0; i < nUpd; ++i)
             for (int k = vp[i]; k < vp[i + 1]; ++k)
@@ -312,20 +314,45 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
                 }
             }
     };
-    forEachSource([&](int slot, int) { ++srcPtr[slot + 1]; });
-    for (size_t k = 0; k < nBlocks; ++k)
-        srcPtr[k + 1] += srcPtr[k];
-    src.resize(static_cast<size_t>(srcPtr[nBlocks]));
-    {
-        std::vector<int> fill(srcPtr.begin(), srcPtr.end() - 1);
-        forEachSource([&](int slot, int code) { src[fill[slot]++] = code; });
-    }
     R.bsrRowPtr.upload(rowPtr, h.stream);
     R.bsrCol.upload(col, h.stream);
     R.bsrBlockRow.upload(blockRow, h.stream);
-    R.bsrSrcPtr.upload(srcPtr, h.stream);
-    R.bsrSrc.upload(src, h.stream);
     R.bsrA.allocate(nBlocks), R.bsrB.allocate(nBlocks), R.bsrC.allocate(nBlocks);
+    if (maxRow <= kRowAssembleMaxRow)
+    {
+        // row-major assembly: per incidence, the row positions of its tet's vertices
+        std::vector<int4> incSlot(vinc.size());
+        for (int i = 0; i < nUpd; ++i)
+            for (int k = vp[i]; k < vp[i + 1]; ++k)
+            {
+                int s4[4];
+                for (int l = 0; l < 4; ++l)
+                {
+                    int const j = vid(vinc[k] >> 2, l);
+                    s4[l] = j < nUpd ? static_cast<int>(std::lower_bound(col.begin() + rowPtr[i],
+                                                                         col.begin() + rowPtr[i + 1], j) -
+                                                        (col.begin() + rowPtr[i]))
+                                     : -1;
+                }
+                incSlot[k] = make_int4(s4[0], s4[1], s4[2], s4[3]);
+            }
+        R.incPtr.upload(vp, h.stream);
+        R.incCode.upload(vinc, h.stream);
+        R.incSlot.upload(incSlot, h.stream);
+    }
+    else
+    {
+        // rows too long for it: one thread per block over the block's sources
+        std::vector<int> srcPtr(nBlocks + 1, 0);
+        forEachSource([&](int slot, int) { ++srcPtr[slot + 1]; });
+        for (size_t k = 0; k < nBlocks; ++k)
+            srcPtr[k + 1] += srcPtr[k];
+        std::vector<int> src(static_cast<size_t>(srcPtr[nBlocks]));
+        std::vector<int> fill(srcPtr.begin(), srcPtr.end() - 1);
+        forEachSource([&](int slot, int code) { src[fill[slot]++] = code; });
+        R.bsrSrcPtr.upload(srcPtr, h.stream);
+        R.bsrSrc.upload(src, h.stream);
+    }
     R.bsrMaxRow = maxRow;
 
     // mesh and aggregate of every updated vertex
@@ -420,10 +447,30 @@ void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt)
     p.nRows = static_cast<int>(h.nUpdated), p.nBlocks = static_cast<int>(R.bsrCol.n);
     if (p.nBlocks == 0)
         return;
+    double const invDt2 = 1. / (double(dt) * double(dt));
     device::BsrValues const v{R.bsrA.ptr, R.bsrB.ptr, R.bsrC.ptr};
-    hipLaunchKernelGGL(device::BsrAssemble, dim3(Blocks(p.nBlocks)), dim3(256), 0, h.stream, xs,
-                       static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr), p,
-                       1. / (double(dt) * double(dt)), v, R.dinv.ptr);
+    if (R.bsrMaxRow > kRowAssembleMaxRow)
+    {
+        hipLaunchKernelGGL(device::BsrAssemble, dim3(Blocks(p.nBlocks)), dim3(256), 0, h.stream, xs,
+                           static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr), p, invDt2,
+                           v, R.dinv.ptr);
+        return;
+    }
+    device::RowAssembleArgs a{};
+    a.xs = xs, a.tetv = h.tetv.ptr, a.tgp = h.tgp.ptr;
+    a.incPtr = R.incPtr.ptr, a.incCode = R.incCode.ptr, a.incSlot = R.incSlot.ptr;
+    a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr, a.nRows = p.nRows, a.invDt2 = invDt2, a.v = v;
+    a.dinv = R.dinv.ptr;
+    dim3 const grid((p.nRows + 1) / 2);
+    int const nb = (R.bsrMaxRow + 31) / 32;
+    if (nb <= 1)
+        hipLaunchKernelGGL(device::RowAssemble<1>, grid, dim3(64), 0, h.stream, a);
+    else if (nb == 2)
+        hipLaunchKernelGGL(device::RowAssemble<2>, grid, dim3(64), 0, h.stream, a);
+    else if (nb == 3)
+        hipLaunchKernelGGL(device::RowAssemble<3>, grid, dim3(64), 0, h.stream, a);
+    else
+        hipLaunchKernelGGL(device::RowAssemble<4>, grid, dim3(64), 0, h.stream, a);
 }
 
 // Lanes per row of the block products: the longest row rounded up to a power of
@@ -586,6 +633,45 @@ int ReadLive(pbat_vbd_integrator& h)
     return static_cast<int>(std::count(done.begin(), done.end(), 0));
 }
 
+// Host-mapped copy of the per-mesh done flags (written by CgFinish) and the
+// chunk-end events, allocated on first use.
+void EnsureDoneHost(pbat_vbd_integrator& h)
+{
+    RolloutState& R = h.rollout;
+    int const nM = h.nMeshes;
+    if (R.doneHostMeshes < nM)
+    {
+        if (R.doneHost)
+            PBAT_HIP(hipHostFree(R.doneHost));
+        R.doneHost = nullptr, R.doneDev = nullptr, R.doneHostMeshes = 0;
+        void* p = nullptr;
+        if (hipHostMalloc(&p, sizeof(int) * static_cast<size_t>(nM), hipHostMallocMapped | hipHostMallocCoherent) !=
+                hipSuccess ||
+            !p)
+            Fail(PBAT_VBD_ERROR_OUT_OF_MEMORY, "hipHostMalloc of the live-mesh flags failed");
+        R.doneHost = static_cast<int*>(p), R.doneHostMeshes = nM;
+        void* dp = nullptr;
+        PBAT_HIP(hipHostGetDevicePointer(&dp, p, 0));
+        R.doneDev = static_cast<int*>(dp);
+    }
+    for (hipEvent_t& e : R.chunkEvent)
+        if (!e)
+            PBAT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+}
+
+// Wait for the chunk that event `slot` ends and count the meshes still
+// iterating. The flags may already hold a later chunk's values (a mesh stays
+// done once done within a solve).
+int LiveAfterChunk(pbat_vbd_integrator& h, int slot)
+{
+    RolloutState& R = h.rollout;
+    PBAT_HIP(hipEventSynchronize(R.chunkEvent[slot]));
+    int live = 0;
+    for (int m = 0; m < h.nMeshes; ++m)
+        live += __atomic_load_n(R.doneHost + m, __ATOMIC_ACQUIRE) == 0;
+    return live;
+}
+
 SolveResult ReadResult(pbat_vbd_integrator& h)
 {
     int const nM = h.nMeshes;
@@ -718,11 +804,13 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         f.cg = cg, f.meshAggBegin = h.meshAggBegin.ptr;
         f.rzPart = R.rzPart.ptr, f.rrPart = R.rrPart.ptr, f.bbPart = R.bbPart.ptr;
         f.rczcPart = twoLevel ? R.rczcPart.ptr : nullptr;
+        f.doneOut = R.doneDev;
         hipLaunchKernelGGL(device::CgFinish, dim3((nM + 3) / 4), dim3(256), 0, h.stream, f, first);
     };
     std::function<void(int)> iteration;
     if (fused)
     {
+        EnsureDoneHost(h);
         // r = b - H y, z = D^-1 r, (rc, zc); p_old = 0 for the first direction
         product(y, R.q.ptr, nullptr);
         aggUpdate(1, nullptr);
@@ -781,7 +869,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
             h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.meshChunkBegin.ptr,
             R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
-            R.bsrColAgg.ptr};
+            R.bsrColAgg.ptr,   R.doneDev};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();
@@ -796,19 +884,37 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             R.cgPtrs = ptrs;
         }
     }
-    if (ReadLive(h) > 0)
+    // Fused iterations: the host polls the live-mesh count one chunk behind.
+    // Chunk c + 1 is queued before the host waits for the end of chunk c and
+    // reads the done flags the finish kernel mirrors into host memory, so the
+    // device does not idle during the poll. A chunk queued after every mesh has
+    // converged (or that starts with every mesh converged) changes nothing: each
+    // of its kernels returns at once for a mesh whose done flag is set.
+    auto launchChunk = [&](int it) {
+        int const chunk = std::min(check, cfg.max_iterations - it);
+        if (graph && chunk == check)
+            PBAT_HIP(hipGraphLaunch(R.cgExec, h.stream));
+        else
+            for (int k = 0; k < chunk; ++k)
+                iteration(it + k);
+        return it + chunk;
+    };
+    int it = 0;
+    if (!fused)
     {
-        int it = 0;
-        while (it < cfg.max_iterations)
+        // the matrix-free iteration keeps its flags on the device only: poll
+        // before every chunk
+        while (it < cfg.max_iterations && ReadLive(h) > 0)
+            it = launchChunk(it);
+    }
+    else
+    {
+        it = launchChunk(0);
+        for (int slot = 0; it < cfg.max_iterations; slot ^= 1)
         {
-            int const chunk = std::min(check, cfg.max_iterations - it);
-            if (graph && chunk == check)
-                PBAT_HIP(hipGraphLaunch(R.cgExec, h.stream));
-            else
-                for (int k = 0; k < chunk; ++k)
-                    iteration(it + k);
-            it += chunk;
-            if (ReadLive(h) == 0)
+            PBAT_HIP(hipEventRecord(R.chunkEvent[slot], h.stream));
+            it = launchChunk(it);
+            if (LiveAfterChunk(h, slot) == 0)
                 break;
         }
     }
@@ -952,6 +1058,13 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
             float4 const* xs = R.hold.ptr;
             hipLaunchKernelGGL(device::AdjointRhs, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.xbar.ptr, R.vbar.ptr,
                                R.rhs.ptr, nUpd, nV, 1.f / dt);
+            // initial guess: the last solution, extrapolated from the last two
+            if (st.solves == 1)
+                hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.yPrev.ptr,
+                                   static_cast<float4 const*>(R.y.ptr), nV);
+            else if (st.solves >= 2)
+                hipLaunchKernelGGL(device::Extrapolate4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.y.ptr,
+                                   R.yPrev.ptr, nV);
             SolveResult const sr = Solve(h, xs, dt, R.y.ptr, R.rhs.ptr, cfg);
             ++st.solves;
             st.total_iterations += sr.iterations;

@@ -788,6 +788,18 @@ __global__ __launch_bounds__(256) void Copy4(float4* __restrict__ dst, float4 co
         dst[i] = src[i];
 }
 
+// Initial guess of the next backward solve from the last two solutions:
+// y = 2 y - yPrev (linear extrapolation in the step), yPrev = the old y.
+__global__ __launch_bounds__(256) void Extrapolate4(float4* __restrict__ y, float4* __restrict__ yPrev, int n)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= n)
+        return;
+    float4 const c = y[i], p = yPrev[i];
+    yPrev[i] = c;
+    y[i] = make_float4(2.f * c.x - p.x, 2.f * c.y - p.y, 2.f * c.z - p.z, 0.f);
+}
+
 __global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);

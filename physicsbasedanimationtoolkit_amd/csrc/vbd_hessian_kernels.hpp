@@ -7,13 +7,16 @@
 // columns ascending) and every product of the solve is a block-sparse
 // matrix-vector product: 40 bytes per block (36 B of values + the column)
 // instead of re-evaluating the Stable Neo-Hookean Hessian of every incident tet
-// per product. For tet e with shape-function gradient rows g_a, g_b of two of
-// its vertices, F its deformation gradient and J = det F, the block is
+// per product. RowAssemble forms each tet's blocks once per row vertex and
+// sums every block's terms in a fixed order. For tet e with shape-function gradient rows
+// g_a, g_b of two of its vertices, F its deformation gradient and J = det F, the
+// block is
 //   w mu (g_a . g_b) I + w lambda (cof F g_a)(cof F g_b)^T
 //     - w lambda (J - alpha) [F (g_a x g_b)]_x
 // (d2Psi/dF2 = mu I + lambda cof F (x) cof F + lambda (J - alpha) d2J/dF2
 // contracted with g_a and g_b; d2J/dF2 contracts to the cross-product matrix).
-// Blocks are summed in FP64 in a fixed order and stored in FP32.
+// The terms are formed in FP64 and kept in FP32; each block sums them in FP64
+// in a fixed order (ascending tet) and is stored in FP32.
 #pragma once
 
 #include "vbd_adjoint_kernels.hpp"
@@ -44,8 +47,10 @@ struct BsrPattern
     int nBlocks;
 };
 
-// One thread per block. Diagonal blocks also receive m_i / h^2 I and write the
-// FP64-inverted block to dinv[2 i], dinv[2 i + 1] as (00, 01, 02, 11), (12, 22).
+// General assembly (rows longer than RowAssemble takes): one thread per block,
+// each source's tet recomputed. Diagonal blocks also receive m_i / h^2 I and
+// write the FP64-inverted block to dinv[2 i], dinv[2 i + 1] as
+// (00, 01, 02, 11), (12, 22).
 __global__ __launch_bounds__(256) void BsrAssemble(float4 const* __restrict__ xs, int4 const* __restrict__ tetv,
                                                    float4 const* __restrict__ tgp, BsrPattern p, double invDt2,
                                                    BsrValues v, float4* __restrict__ dinv)
@@ -115,6 +120,203 @@ __global__ __launch_bounds__(256) void BsrAssemble(float4 const* __restrict__ xs
     v.b[k] = make_float4(static_cast<float>(H[1][1]), static_cast<float>(H[1][2]), static_cast<float>(H[2][0]),
                          static_cast<float>(H[2][1]));
     v.c[k] = static_cast<float>(H[2][2]);
+}
+
+// The FP32 state's tet (F, cof F, J in FP64) and the Stable Neo-Hookean
+// constants, for the blocks
+//   H(a, b) = w mu (g_a . g_b) I + w lambda (cof F g_a)(cof F g_b)^T
+//             - w lambda (J - alpha) [F (g_a x g_b)]_x.
+struct TetHessian
+{
+    double g[4][3];  // shape-function gradient rows
+    double F[3][3];  // F[row][col]
+    double cg[4][3]; // cof F g_l
+    double wmu, wlam, sk;
+    __device__ __forceinline__ TetHessian(float4 const* __restrict__ xs, int4 const& t, float4 const& q0,
+                                          float4 const& q1, float4 const& q2)
+    {
+        float4 const x0 = xs[t.x], x1 = xs[t.y], x2 = xs[t.z], x3 = xs[t.w];
+        double const G[3][3] = {{q0.x, q0.y, q0.z}, {q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x}};
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            g[1][c] = G[0][c], g[2][c] = G[1][c], g[3][c] = G[2][c];
+            g[0][c] = -(G[0][c] + G[1][c] + G[2][c]);
+        }
+        double const d[3][3] = {{double(x1.x) - x0.x, double(x1.y) - x0.y, double(x1.z) - x0.z},
+                                {double(x2.x) - x0.x, double(x2.y) - x0.y, double(x2.z) - x0.z},
+                                {double(x3.x) - x0.x, double(x3.y) - x0.y, double(x3.z) - x0.z}};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                F[r][c] = d[0][r] * G[0][c] + d[1][r] * G[1][c] + d[2][r] * G[2][c];
+        double cof[3][3]; // column c = f_{c+1} x f_{c+2}
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+        {
+            int const c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+            cof[0][c] = F[1][c1] * F[2][c2] - F[2][c1] * F[1][c2];
+            cof[1][c] = F[2][c1] * F[0][c2] - F[0][c1] * F[2][c2];
+            cof[2][c] = F[0][c1] * F[1][c2] - F[1][c1] * F[0][c2];
+        }
+        double const J = F[0][0] * cof[0][0] + F[1][0] * cof[1][0] + F[2][0] * cof[2][0];
+        wmu = q2.y, wlam = q2.z, sk = double(q2.z) * (J - double(q2.w));
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                cg[l][r] = cof[r][0] * g[l][0] + cof[r][1] * g[l][1] + cof[r][2] * g[l][2];
+    }
+    // H(a, b) for the row's gradient row ga and cof F ga, column lb
+    template <int lb>
+    __device__ __forceinline__ void Block(double const (&ga)[3], double const (&ca)[3], double (&H)[9]) const
+    {
+        double const* gb = g[lb];
+        double const gx[3] = {ga[1] * gb[2] - ga[2] * gb[1], ga[2] * gb[0] - ga[0] * gb[2],
+                              ga[0] * gb[1] - ga[1] * gb[0]};
+        double w[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            w[r] = sk * (F[r][0] * gx[0] + F[r][1] * gx[1] + F[r][2] * gx[2]);
+        double const dg = wmu * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+        // - sk [F (g_a x g_b)]_x with [w]_x = [[0, -w2, w1], [w2, 0, -w0], [-w1, w0, 0]]
+        double const S[9] = {0., w[2], -w[1], -w[2], 0., w[0], w[1], -w[0], 0.};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                H[3 * r + c] = (r == c ? dg : 0.) + wlam * ca[r] * cg[lb][c] + S[3 * r + c];
+    }
+};
+
+// Row-major assembly: incidences of row i (tet e, local vertex la of the row
+// vertex) in incCode[incPtr[i] .. incPtr[i + 1]) as 4 e + la, ascending e;
+// incSlot holds, per incidence, the position in the row of the tet's local
+// vertices 0..3 (-1: not updated).
+struct RowAssembleArgs
+{
+    float4 const* __restrict__ xs;
+    int4 const* __restrict__ tetv;
+    float4 const* __restrict__ tgp;
+    int const* __restrict__ incPtr;
+    int const* __restrict__ incCode;
+    int4 const* __restrict__ incSlot;
+    int const* __restrict__ rowPtr;
+    int const* __restrict__ col;
+    int nRows;
+    double invDt2;
+    BsrValues v;
+    float4* __restrict__ dinv;
+};
+
+// 32 lanes per row, two rows per wave, one wave per workgroup. The row's
+// incidences are taken 32 at a time, one per lane: the lane forms its tet's
+// F, cof F and J in FP64 and the four blocks (la, lb), lb = 0..3, and stages
+// them in LDS as FP32; lane l then adds, for each block it owns (l, l + 32, ...,
+// NB of them, below the row length), the staged blocks that fall on it in
+// incidence order (ascending tet), in FP64. Diagonal blocks also receive
+// m_i / h^2 I and write the FP64-inverted block to dinv[2 i], dinv[2 i + 1] as
+// (00, 01, 02, 11), (12, 22).
+template <int NB>
+__global__ __launch_bounds__(64) void RowAssemble(RowAssembleArgs a)
+{
+    constexpr int L = 32;
+    __shared__ float stage[2][L][4][9];
+    __shared__ int slotS[2][L][4];
+    int const lane = static_cast<int>(threadIdx.x) & (L - 1), half = static_cast<int>(threadIdx.x) >> 5;
+    int const i = static_cast<int>(blockIdx.x) * 2 + half;
+    int kb = 0, ke = 0, rb = 0, nb = 0;
+    if (i < a.nRows)
+        kb = a.incPtr[i], ke = a.incPtr[i + 1], rb = a.rowPtr[i], nb = a.rowPtr[i + 1] - rb;
+    int const mine = (ke - kb + L - 1) / L;
+    int const batches = max(mine, __shfl_xor(mine, 32)); // uniform over the wave
+    double acc[NB][9];
+#pragma unroll
+    for (int o = 0; o < NB; ++o)
+#pragma unroll
+        for (int c = 0; c < 9; ++c)
+            acc[o][c] = 0.;
+    for (int bt = 0; bt < batches; ++bt)
+    {
+        int const k = kb + bt * L + lane;
+        if (k < ke)
+        {
+            int const code = a.incCode[k];
+            int const e = code >> 2, la = code & 3;
+            int4 const sl = a.incSlot[k];
+            TetHessian const T(a.xs, a.tetv[e], a.tgp[3 * e], a.tgp[3 * e + 1], a.tgp[3 * e + 2]);
+            double ga[3], ca[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+            {
+                ga[c] = la == 0 ? T.g[0][c] : la == 1 ? T.g[1][c] : la == 2 ? T.g[2][c] : T.g[3][c];
+                ca[c] = la == 0 ? T.cg[0][c] : la == 1 ? T.cg[1][c] : la == 2 ? T.cg[2][c] : T.cg[3][c];
+            }
+            double H[9];
+            T.Block<0>(ga, ca, H);
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                stage[half][lane][0][c] = static_cast<float>(H[c]);
+            T.Block<1>(ga, ca, H);
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                stage[half][lane][1][c] = static_cast<float>(H[c]);
+            T.Block<2>(ga, ca, H);
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                stage[half][lane][2][c] = static_cast<float>(H[c]);
+            T.Block<3>(ga, ca, H);
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                stage[half][lane][3][c] = static_cast<float>(H[c]);
+            slotS[half][lane][0] = sl.x, slotS[half][lane][1] = sl.y;
+            slotS[half][lane][2] = sl.z, slotS[half][lane][3] = sl.w;
+        }
+        __syncthreads();
+        int const cnt = min(L, max(0, ke - kb - bt * L));
+#pragma unroll
+        for (int o = 0; o < NB; ++o)
+        {
+            int const b = lane + o * L;
+            if (b >= nb)
+                continue;
+            for (int j = 0; j < cnt; ++j)
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+                    if (slotS[half][j][s] == b)
+                    {
+#pragma unroll
+                        for (int c = 0; c < 9; ++c)
+                            acc[o][c] += stage[half][j][s][c];
+                    }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int o = 0; o < NB; ++o)
+    {
+        int const b = lane + o * L;
+        if (b >= nb)
+            continue;
+        int const kBlock = rb + b;
+        double* H = acc[o];
+        if (a.col[kBlock] == i)
+        {
+            double const m = double(a.xs[i].w) * a.invDt2;
+            H[0] += m, H[4] += m, H[8] += m;
+            Inverse3 const C(H[0], 0.5 * (H[1] + H[3]), 0.5 * (H[2] + H[6]), H[4], 0.5 * (H[5] + H[7]), H[8]);
+            a.dinv[2 * i] = make_float4(static_cast<float>(C.c00 * C.inv), static_cast<float>(C.c01 * C.inv),
+                                        static_cast<float>(C.c02 * C.inv), static_cast<float>(C.c11 * C.inv));
+            a.dinv[2 * i + 1] =
+                make_float4(static_cast<float>(C.c12 * C.inv), static_cast<float>(C.c22 * C.inv), 0.f, 0.f);
+        }
+        a.v.a[kBlock] = make_float4(static_cast<float>(H[0]), static_cast<float>(H[1]), static_cast<float>(H[2]),
+                                    static_cast<float>(H[3]));
+        a.v.b[kBlock] = make_float4(static_cast<float>(H[4]), static_cast<float>(H[5]), static_cast<float>(H[6]),
+                                    static_cast<float>(H[7]));
+        a.v.c[kBlock] = static_cast<float>(H[8]);
+    }
 }
 
 enum class BsrMode : int
@@ -561,6 +763,7 @@ struct CgFinishArgs
     double const* __restrict__ rrPart;
     double const* __restrict__ bbPart;
     double const* __restrict__ rczcPart; // nullable (block Jacobi)
+    int* __restrict__ doneOut;           // nullable: host-mapped copy of the done flags
 };
 
 __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
@@ -583,7 +786,11 @@ __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
         bb += __shfl_xor(bb, off);
     }
     if (lane == 0)
-        FinishMesh(f.cg, m, rr, rz, bb, first);
+    {
+        bool const live = FinishMesh(f.cg, m, rr, rz, bb, first);
+        if (f.doneOut)
+            f.doneOut[m] = live ? 0 : 1;
+    }
 }
 
 // Galerkin coarse operator Z^T H Z from the assembled blocks: coarse block k

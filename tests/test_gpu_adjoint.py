@@ -99,6 +99,61 @@ def test_adjoint_solve_matches_fp64(gpu_device, solver, iters):
     assert e <= 1e-4
 
 
+def fan_scene(n_surface, seed=0):
+    """A ball of tets around one centre vertex: n_surface points on the unit
+    sphere (golden-angle spiral), their convex hull's triangles each joined to
+    the centre. The centre is adjacent to every surface vertex; the surface
+    vertices with y < -0.8 are fixed."""
+    from scipy.spatial import ConvexHull
+
+    k = np.arange(n_surface) + 0.5
+    y = 1.0 - 2.0 * k / n_surface
+    r = np.sqrt(1.0 - y * y)
+    phi = k * np.pi * (3.0 - np.sqrt(5.0))
+    P = np.stack([r * np.cos(phi), y, r * np.sin(phi)], axis=1)
+    tris = ConvexHull(P).simplices
+    X = np.concatenate([np.zeros((1, 3)), P]).T.copy()
+    E = np.concatenate([np.zeros((len(tris), 1), np.int64), tris + 1], axis=1)
+    for t in E:
+        a, b, c, d = X[:, t].T
+        if np.dot(np.cross(b - a, c - a), d - a) < 0:
+            t[2], t[3] = t[3], t[2]
+    E = E.T.copy()
+    rng = np.random.default_rng(seed)
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(meshes.log_uniform(rng, 1e5, 1e7, nT), 0.45)
+    dbc = np.flatnonzero(X[1] < -0.8).astype(np.int64)
+    return X, E, mu, lam, dbc
+
+
+@pytest.mark.parametrize("n_surface", [40, 90, 120, 200])
+@pytest.mark.parametrize("solver", ["pcg_jacobi", "pcg_two_level"])
+def test_adjoint_solve_long_rows_matches_fp64(gpu_device, n_surface, solver):
+    """The centre's row holds 37, 82, 109 and 181 blocks (its updated
+    neighbours and itself): the row-major assembly with 2, 3 and 4 blocks per
+    lane, and the per-block assembly that takes rows beyond 128 blocks."""
+    X, E, mu, lam, dbc = fan_scene(n_surface)
+    nT = E.shape[1]
+    v0 = np.zeros_like(X)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(np.full(nT, 1e3), mu, lam)
+            .with_acceleration(meshes.gravity(X.shape[1])).with_dirichlet_vertices(dbc)
+            .with_velocity(v0).construct())
+    g = pbat.gpu.vbd.Integrator(data)
+    o = O.OracleVbd(X, E, v=v0, aext=meshes.gravity(X.shape[1]), rhoe=np.full(nT, 1e3), mue=mu, lambdae=lam,
+                    dbc=dbc, nthreads=8)
+    x = deformed(X, amp=0.002)
+    b = np.random.default_rng(3).normal(size=X.shape)
+    y, st = g.adjoint_solve(x, DT, b, solver=solver, max_iterations=600, tolerance=1e-6, check_every=10)
+    H, idx = A.hessian(o, x.T, DT)
+    yr = np.zeros(3 * o.nV)
+    yr[idx] = A.spla.spsolve(H, b.T.reshape(-1)[idx])
+    yr = yr.reshape(o.nV, 3).T
+    e = rel(y, yr)
+    print(f"fan {n_surface} {solver}: {st} rel err {e:.3e}")
+    assert st["relative_residual"] <= 1e-5
+    assert e <= 1e-4
+
+
 def gpu_trajectory(g, x0, v0, steps, iterations):
     """Positions x_0..x_T of stepping g one step at a time (device forward)."""
     g.x = x0
