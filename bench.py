@@ -175,6 +175,14 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
+def _adjoint_solver(args):
+    """The differentiable measurements' solver: two-level PCG at ADJOINT_TOL per
+    mesh, each solve started from the projection onto the last
+    args.adjoint_history solutions."""
+    return dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000,
+                check_every=args.adjoint_check_every, history=args.adjoint_history)
+
+
 def bench_adjoint(args, torch):
     """Differentiable VBD on one mesh: 32^3 grid (35,937 vertices, 196,608 tets),
     hidden per-tet log Y log-uniform in [1e5, 1e7], target = 20-step rollout
@@ -193,7 +201,7 @@ def bench_adjoint(args, torch):
     g.autotune(s.dt, sweeps=5)
     v0 = g.v.copy()
     dt, it, T, every = s.dt, s.iterations, s.steps, 1
-    solver = dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000, check_every=10)
+    solver = _adjoint_solver(args)
 
     def forward(logy):
         g.set_log_young_modulus(logy)
@@ -253,7 +261,7 @@ def bench_batched_adjoint(args, torch):
     b.autotune(scenes[0].dt, sweeps=3)
     shape = b.sweep_config()
     dt, it, T, every = scenes[0].dt, scenes[0].iterations, args.batch_adjoint_steps, 10
-    solver = dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000, check_every=10)
+    solver = _adjoint_solver(args)
     hidden = np.concatenate([s.log_y_hidden for s in scenes])
     guess = np.concatenate([s.log_y_guess for s in scenes])
     Xb = np.concatenate([s.X for s in scenes], axis=1)
@@ -593,6 +601,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-rollout measurements")
     ap.add_argument("--adjoint-grid", type=int, default=32)
     ap.add_argument("--adjoint-reps", type=int, default=3)
+    ap.add_argument("--adjoint-history", type=int, default=6,
+                    help="previous solutions each backward solve's initial guess is projected onto (0..8)")
+    ap.add_argument("--adjoint-check-every", type=int, default=10,
+                    help="CG iterations per captured chunk between live-mesh polls")
     ap.add_argument("--batch-adjoint-meshes", type=int, default=64)
     ap.add_argument("--batch-adjoint-steps", type=int, default=200)
     ap.add_argument("--no-variants", action="store_true",
@@ -635,7 +647,7 @@ def main():
         run = {"batched": bench_batched, "adjoint": bench_adjoint, "batched_adjoint": bench_batched_adjoint,
                "variants": bench_variants,
                "gradient_checks": lambda a, t: adjoint_gradient_checks(
-                   a, dict(solver="pcg_two_level", tolerance=ADJOINT_TOL, max_iterations=2000, check_every=10))}
+                   a, _adjoint_solver(args))}
         print(json.dumps({args.only: run[args.only](args, torch)}), flush=True)
         return
     s = meshes.plain_gpu_scene(args.grid, seed=0)

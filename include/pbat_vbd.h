@@ -305,6 +305,9 @@ typedef struct pbat_vbd_adjoint_config {
     int32_t max_iterations; /* per linear solve */
     float tolerance;        /* on |b - H y| / |b| per mesh */
     int32_t check_every;    /* iterations between host polls of the live-mesh count */
+    int32_t history;        /* 0..8: the block Jacobi and two-level solvers start each solve of a
+                               backward pass from the projection (H-norm) of its solution onto the
+                               span of the last `history` solutions; 0: from the last solution */
 } pbat_vbd_adjoint_config;
 
 typedef struct pbat_vbd_adjoint_stats {
@@ -328,7 +331,7 @@ int pbat_vbd_rollout_forward(pbat_vbd_integrator* h, float dt, int32_t iteration
  * nullable = 0), original vertex order. Outputs (each nullable): dL/dlogY
  * (nT), dL/dv_0 and dL/dx_0 (3 x nV; zero on Dirichlet vertices). Segments
  * between checkpoints are recomputed; the integrator's state at the end of
- * the rollout is restored afterwards. config nullable (PCG, 200, 1e-6, 5). */
+ * the rollout is restored afterwards. config nullable (PCG, 200, 1e-6, 5, history 0). */
 int pbat_vbd_rollout_backward(pbat_vbd_integrator* h, const float* dL_dx, const float* dL_dv, float* dL_dlogY,
                               float* dL_dv0, float* dL_dx0, const pbat_vbd_adjoint_config* config,
                               pbat_vbd_adjoint_stats* stats);

@@ -74,7 +74,8 @@ ADJOINT_PCG_TWO_LEVEL = 3
 class AdjointConfig(ctypes.Structure):
     """``pbat_vbd_adjoint_config`` of include/pbat_vbd.h."""
 
-    _fields_ = [("solver", c_i32), ("max_iterations", c_i32), ("tolerance", c_f32), ("check_every", c_i32)]
+    _fields_ = [("solver", c_i32), ("max_iterations", c_i32), ("tolerance", c_f32), ("check_every", c_i32),
+                ("history", c_i32)]
 
 
 class AdjointStats(ctypes.Structure):

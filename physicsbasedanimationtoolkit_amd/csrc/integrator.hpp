@@ -146,10 +146,18 @@ struct RolloutState
     std::vector<DeviceArray<float4>> ckptX, ckptV, ckptVprev; // every `every` steps
     std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
     DeviceArray<float4> lastX;                                // positions after the last step
-    DeviceArray<float4> y, yPrev, r, z, p, q, rhs, xbar, vbar, hold;
+    DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
+    // previous solutions of the backward pass (a ring, newest at histHead - 1),
+    // the projection basis, its products with H, the per-aggregate partial sums
+    // and the per-mesh coefficients of the projected initial guess
+    DeviceArray<float4> hist[8], projBasis[8], projHb[8];
+    DeviceArray<double> projPart, projCoef;
+    DeviceArray<double> solveStats; // per solve of a backward pass: iterations, relative residual
+    int histHead = 0, histCount = 0;
     DeviceArray<float4> dinv;                                 // inverse diagonal blocks, 2 per vertex
     DeviceArray<double> coarse[2];                            // coarse operator / its inverse (ping-pong)
     DeviceArray<int> coarseFinal;                             // per mesh: buffer holding its inverse
+    DeviceArray<float> coarseF;                               // the inverse, symmetrised, FP32
     DeviceArray<double> rc, zc;                               // coarse residual and correction
     DeviceArray<double> scal;                                 // per-mesh solver scalars (kScalarCount x nMeshes)
     DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count
@@ -168,10 +176,11 @@ struct RolloutState
     DeviceArray<float> bsrC;
     int bsrMaxRow = 0;
     DeviceArray<float4> p2;
-    // row chunks of the fused iteration (begin, end), mesh-major, and the first
-    // chunk of each mesh (nMeshes + 1); chunks hold at most bsrChunkRows rows
+    // row chunks of the fused iteration (begin, end), mesh-major, and for each
+    // aggregate the chunks of its mesh (begin, end); chunks hold at most
+    // bsrChunkRows rows
     DeviceArray<int2> bsrChunks;
-    DeviceArray<int> meshChunkBegin;
+    DeviceArray<int2> aggChunks;
     int bsrChunkRows = 0;
     // per-chunk p . H p, per-aggregate r . z, r . r, b . b, per-coarse-row rc . zc
     DeviceArray<double> pqPart, rzPart, rrPart, bbPart, rczcPart;
@@ -373,7 +382,9 @@ int EffectivePerLane(pbat_vbd_integrator const& h);
 void LaunchSweep(pbat_vbd_integrator& h, device::SweepArgs const& a, int nVerts);
 device::SweepArgs MakeSweepArgs(pbat_vbd_integrator& h, float sdt, float sdt2);
 // One forward Step (gpu/impl/vbd/Integrator.cu:82-103): graph replay or eager launches.
-void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps);
+// sync = false (rollouts): the step is only queued; the caller clears the status
+// before its steps and synchronises and checks it after them.
+void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool sync = true);
 // Status reported by the kernels of the work queued since the last ClearStatus,
 // read after a synchronisation of the handle's stream: fails with
 // PBAT_VBD_ERROR_NONFINITE when a substep ended with non-finite positions.

@@ -130,9 +130,19 @@ void EnsureBuffers(pbat_vbd_integrator& h)
 {
     RolloutState& R = h.rollout;
     size_t const n = static_cast<size_t>(h.nV);
-    for (auto* b : {&R.y, &R.yPrev, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
+    for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
+    static_assert(device::kProjMax == sizeof(R.hist) / sizeof(R.hist[0]), "history ring size");
+    for (int j = 0; j < device::kProjMax; ++j)
+        for (auto* b : {&R.hist[j], &R.projBasis[j], &R.projHb[j]})
+            if (b->n != n)
+                b->allocate(n);
+    size_t const nProjPart = static_cast<size_t>(device::kProjTerms) * std::max(h.nAggregates, 1);
+    if (R.projPart.n != nProjPart)
+        R.projPart.allocate(nProjPart);
+    if (R.projCoef.n != static_cast<size_t>(device::kProjMax) * h.nMeshes)
+        R.projCoef.allocate(static_cast<size_t>(device::kProjMax) * h.nMeshes);
     if (R.dinv.n != 2 * n)
         R.dinv.allocate(2 * n);
     size_t const nM = static_cast<size_t>(h.nMeshes);
@@ -211,6 +221,11 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
     PBAT_HIP(hipStreamSynchronize(h.stream)); // finalBuf is a host temporary
     hipLaunchKernelGGL(device::CoarseGuard, dim3(nM), dim3(256), 0, h.stream, R.coarse[0].ptr, R.coarse[1].ptr,
                        static_cast<int const*>(R.coarseFinal.ptr), c);
+    if (R.coarseF.n != total)
+        R.coarseF.allocate(total);
+    hipLaunchKernelGGL(device::CoarseToFloat, grid, dim3(256), 0, h.stream,
+                       static_cast<double const*>(R.coarse[0].ptr), static_cast<double const*>(R.coarse[1].ptr),
+                       static_cast<int const*>(R.coarseFinal.ptr), c, R.coarseF.ptr);
     PBAT_HIP(hipGetLastError());
 }
 
@@ -402,7 +417,11 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
     for (int m = 0; m < nM; ++m)
         meshChunk[m + 1] += meshChunk[m];
     R.bsrChunks.upload(chunkRange, h.stream);
-    R.meshChunkBegin.upload(meshChunk, h.stream);
+    std::vector<int2> aggChunk(static_cast<size_t>(std::max(h.nAggregates, 1)), make_int2(0, 0));
+    for (int m = 0; m < nM; ++m)
+        for (int g = h.aggBegin[m]; g < h.aggBegin[m + 1]; ++g)
+            aggChunk[g] = make_int2(meshChunk[m], meshChunk[m + 1]);
+    R.aggChunks.upload(aggChunk, h.stream);
     R.bsrChunkRows = chunkRows;
     R.pqPart.allocate(std::max<size_t>(chunks.size(), 1));
     size_t const nAgg = static_cast<size_t>(std::max(h.nAggregates, 1));
@@ -718,8 +737,40 @@ SolveResult ReadResult(pbat_vbd_integrator& h)
 // captured once into a hipGraph and replayed; the graph's kernels take the
 // handle's fixed work vectors (x = hold, y, b = rhs), so it stays valid across
 // solves with the same settings.
+// y = the H-norm projection of the solution onto the span of the nHist latest
+// entries of the solution history (vbd_hessian_kernels.hpp: ProjBasis .. ProjCombine),
+// H assembled.
+void ProjectInitialGuess(pbat_vbd_integrator& h, float4* y, float4 const* b, int nHist)
+{
+    RolloutState& R = h.rollout;
+    int const n = static_cast<int>(h.nUpdated), nM = h.nMeshes;
+    device::ProjArgs a{};
+    for (int j = 0; j < device::kProjMax; ++j)
+    {
+        int const slot = ((R.histHead - 1 - j) % device::kProjMax + device::kProjMax) % device::kProjMax;
+        a.hist[j] = R.hist[slot].ptr;
+        a.basis[j] = R.projBasis[j].ptr;
+        a.hb[j] = R.projHb[j].ptr;
+    }
+    a.rhs = b, a.y = y;
+    a.aggPtr = h.aggVertPtr.ptr, a.aggVerts = h.aggVerts.ptr, a.meshAggBegin = h.meshAggBegin.ptr;
+    a.meshOf = h.meshOf.ptr;
+    a.part = R.projPart.ptr, a.coef = R.projCoef.ptr;
+    a.k = nHist, a.n = n, a.nM = nM, a.nAgg = h.nAggregates;
+    hipLaunchKernelGGL(device::ProjBasis, dim3(Blocks(n)), dim3(256), 0, h.stream, a);
+    for (int j = 0; j < nHist; ++j)
+    {
+        device::BsrArgs p{};
+        p.y = a.basis[j], p.out = a.hb[j];
+        BsrPass<BsrMode::kProduct>(h, p, 0, n);
+    }
+    hipLaunchKernelGGL(device::ProjGram, dim3(h.nAggregates), dim3(128), 0, h.stream, a);
+    hipLaunchKernelGGL(device::ProjSolve, dim3(nM), dim3(256), 0, h.stream, a);
+    hipLaunchKernelGGL(device::ProjCombine, dim3(Blocks(n)), dim3(256), 0, h.stream, a);
+}
+
 SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y, float4 const* b,
-                  pbat_vbd_adjoint_config const& cfg)
+                  pbat_vbd_adjoint_config const& cfg, int nHist, double* statsOut)
 {
     RolloutState& R = h.rollout;
     int const n = static_cast<int>(h.nUpdated), nV = static_cast<int>(h.nV), nM = h.nMeshes;
@@ -735,6 +786,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     // fused chunks hold an even number of iterations: the two direction buffers swap
     int const check = fused ? (std::max(1, cfg.check_every) + 1) / 2 * 2 : std::max(1, cfg.check_every);
     AssembleBsr(h, xs, dt);
+    if (fused && nHist > 0 && n > 0 && h.nAggregates > 0)
+        ProjectInitialGuess(h, y, b, nHist);
     auto zeroScalars = [&] {
         int const nd = static_cast<int>(ScalarDoubles(h));
         hipLaunchKernelGGL(device::ZeroScalars, dim3(std::min(1024, Blocks(nd + 2 * nM + 1))), dim3(256), 0,
@@ -785,7 +838,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = h.aggVerts.ptr, u.aggMesh = h.aggMesh.ptr;
         u.y = y, u.r = R.r.ptr, u.z = R.z.ptr, u.p = p, u.q = R.q.ptr, u.b = b, u.dinv = R.dinv.ptr;
         u.rc = twoLevel ? R.rc.ptr : nullptr;
-        u.pqPart = R.pqPart.ptr, u.meshChunkBegin = R.meshChunkBegin.ptr;
+        u.pqPart = R.pqPart.ptr, u.aggChunks = R.aggChunks.ptr;
         u.rzPart = R.rzPart.ptr, u.rrPart = R.rrPart.ptr, u.bbPart = R.bbPart.ptr;
         u.cg = cg, u.init = init;
         if (h.nAggregates > 0)
@@ -794,10 +847,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     auto coarseRows = [&](int first) {
         int const rows = 3 * h.nAggregates, rowsPerBlock = 4 / coarseWavesPerRow;
         hipLaunchKernelGGL(device::CoarseApplyRows, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
-                           h.stream, static_cast<double const*>(R.coarse[0].ptr),
-                           static_cast<double const*>(R.coarse[1].ptr), static_cast<int const*>(R.coarseFinal.ptr),
-                           static_cast<double const*>(R.rc.ptr), R.zc.ptr, cargs, coarseWavesPerRow, cg,
-                           R.rczcPart.ptr, first);
+                           h.stream, static_cast<float const*>(R.coarseF.ptr), static_cast<double const*>(R.rc.ptr),
+                           R.zc.ptr, cargs, coarseWavesPerRow, cg, R.rczcPart.ptr, first);
     };
     auto finish = [&](int first) {
         device::CgFinishArgs f{};
@@ -865,9 +916,9 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         std::vector<void const*> const ptrs{
             R.y.ptr,           R.r.ptr,         R.z.ptr,       R.p.ptr,          R.p2.ptr,        R.q.ptr,
             R.rhs.ptr,         R.hold.ptr,      R.dinv.ptr,    R.scal.ptr,       R.flags.ptr,     R.coarse[0].ptr,
-            R.coarse[1].ptr,   R.coarseFinal.ptr, R.rc.ptr,    R.zc.ptr,         R.bsrRowPtr.ptr, R.bsrCol.ptr,
+            R.coarse[1].ptr,   R.coarseF.ptr,     R.rc.ptr,    R.zc.ptr,         R.bsrRowPtr.ptr, R.bsrCol.ptr,
             R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
-            h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.meshChunkBegin.ptr,
+            h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.aggChunks.ptr,
             R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
             R.bsrColAgg.ptr,   R.doneDev};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
@@ -919,11 +970,17 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         }
     }
     PBAT_HIP(hipGetLastError());
+    if (statsOut)
+    {
+        // the fused solvers' statistics stay on the device: no host round trip
+        hipLaunchKernelGGL(device::SolveStats, dim3(1), dim3(64), 0, h.stream, cg, statsOut);
+        return SolveResult{};
+    }
     return ReadResult(h);
 }
 void CheckConfig(pbat_vbd_adjoint_config const& cfg)
 {
-    if (cfg.max_iterations < 1 || !(cfg.tolerance >= 0.f) ||
+    if (cfg.max_iterations < 1 || !(cfg.tolerance >= 0.f) || cfg.history < 0 || cfg.history > device::kProjMax ||
         (cfg.solver != PBAT_VBD_ADJOINT_PCG && cfg.solver != PBAT_VBD_ADJOINT_GAUSS_SEIDEL &&
          cfg.solver != PBAT_VBD_ADJOINT_PCG_JACOBI && cfg.solver != PBAT_VBD_ADJOINT_PCG_TWO_LEVEL))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid adjoint solver configuration");
@@ -972,6 +1029,7 @@ void RolloutForward(pbat_vbd_integrator& h, float dt, int iterations, int steps,
         if (R.ckptX[c].n != n)
             R.ckptX[c].allocate(n), R.ckptV[c].allocate(n), R.ckptVprev[c].allocate(n);
     }
+    ClearStatus(h);
     for (int t = 0; t < steps; ++t)
     {
         if (t % every == 0)
@@ -979,8 +1037,10 @@ void RolloutForward(pbat_vbd_integrator& h, float dt, int iterations, int steps,
             int const c = t / every;
             SaveState(h, R.ckptX[c].ptr, R.ckptV[c].ptr, R.ckptVprev[c].ptr);
         }
-        Step(h, dt, iterations, 1);
+        Step(h, dt, iterations, 1, false);
     }
+    PBAT_HIP(hipStreamSynchronize(h.stream));
+    CheckStatus(h);
     if (R.lastX.n != n)
         R.lastX.allocate(n);
     PBAT_HIP(hipMemcpyAsync(R.lastX.ptr, h.xm.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice, h.stream));
@@ -998,7 +1058,7 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "the adjoint assumes no Rayleigh damping (kD = 0)");
     if (!dLdx)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "null dL/dx");
-    pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5};
+    pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5, 0};
     if (cfgIn)
         cfg = *cfgIn;
     CheckConfig(cfg);
@@ -1023,6 +1083,10 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
         Zero4(h, R.vbar.ptr + nUpd, nV - nUpd);
     }
     Zero4(h, R.y.ptr, nV);
+    R.histHead = R.histCount = 0;
+    bool const fused = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI || cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL;
+    if (fused && R.solveStats.n < 2 * static_cast<size_t>(R.steps))
+        R.solveStats.allocate(2 * static_cast<size_t>(R.steps));
     hipLaunchKernelGGL(device::ZeroDoubles, dim3(Blocks(std::max(nT, 1))), dim3(256), 0, h.stream, R.grad.ptr, nT);
     if (R.segX.size() != static_cast<size_t>(R.every))
         R.segX.resize(static_cast<size_t>(R.every));
@@ -1041,9 +1105,10 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
         int const t0 = sgm * R.every, t1 = std::min(R.steps, t0 + R.every);
         if (t1 - t0 > 1)
             RestoreState(h, R.ckptX[sgm].ptr, R.ckptV[sgm].ptr, R.ckptVprev[sgm].ptr);
+        // the recomputed steps repeat the forward's bit for bit: no status checks
         for (int t = t0; t < t1 - 1; ++t)
         {
-            Step(h, dt, R.iterations, 1);
+            Step(h, dt, R.iterations, 1, false);
             PBAT_HIP(hipMemcpyAsync(R.segX[t - t0].ptr, h.xm.ptr, sizeof(float4) * n, hipMemcpyDeviceToDevice,
                                     h.stream));
         }
@@ -1058,18 +1123,25 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
             float4 const* xs = R.hold.ptr;
             hipLaunchKernelGGL(device::AdjointRhs, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.xbar.ptr, R.vbar.ptr,
                                R.rhs.ptr, nUpd, nV, 1.f / dt);
-            // initial guess: the last solution, extrapolated from the last two
-            if (st.solves == 1)
-                hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.yPrev.ptr,
-                                   static_cast<float4 const*>(R.y.ptr), nV);
-            else if (st.solves >= 2)
-                hipLaunchKernelGGL(device::Extrapolate4, dim3(Blocks(nV)), dim3(256), 0, h.stream, R.y.ptr,
-                                   R.yPrev.ptr, nV);
-            SolveResult const sr = Solve(h, xs, dt, R.y.ptr, R.rhs.ptr, cfg);
+            // initial guess: the previous solution, or (fused solvers) the
+            // projection onto the span of the last kProjMax solutions
+            double* const statsOut = fused ? R.solveStats.ptr + 2 * static_cast<size_t>(st.solves) : nullptr;
+            SolveResult const sr =
+                Solve(h, xs, dt, R.y.ptr, R.rhs.ptr, cfg, std::min(cfg.history, R.histCount), statsOut);
+            if (cfg.history > 0)
+            {
+                hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream,
+                                   R.hist[R.histHead].ptr, static_cast<float4 const*>(R.y.ptr), nV);
+                R.histHead = (R.histHead + 1) % device::kProjMax;
+                R.histCount = std::min(R.histCount + 1, device::kProjMax);
+            }
             ++st.solves;
-            st.total_iterations += sr.iterations;
-            st.max_iterations = std::max(st.max_iterations, sr.iterations);
-            st.max_relative_residual = std::max(st.max_relative_residual, static_cast<float>(sr.relResidual));
+            if (!fused)
+            {
+                st.total_iterations += sr.iterations;
+                st.max_iterations = std::max(st.max_iterations, sr.iterations);
+                st.max_relative_residual = std::max(st.max_relative_residual, static_cast<float>(sr.relResidual));
+            }
             hipLaunchKernelGGL(device::LogYoungGradient, dim3(Blocks(std::max(nT, 1))), dim3(256), 0, h.stream, xs,
                                R.y.ptr, h.tetv.ptr, h.tgp.ptr, nT, R.grad.ptr);
             hipLaunchKernelGGL(device::AdjointStepBack, dim3(Blocks(nV)), dim3(256), 0, h.stream, xs, R.y.ptr,
@@ -1077,6 +1149,20 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
         }
     }
     PBAT_HIP(hipGetLastError());
+    if (fused && st.solves > 0)
+    {
+        std::vector<double> S(2 * static_cast<size_t>(st.solves));
+        PBAT_HIP(hipMemcpyAsync(S.data(), R.solveStats.ptr, S.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                h.stream));
+        PBAT_HIP(hipStreamSynchronize(h.stream));
+        for (int k = 0; k < st.solves; ++k)
+        {
+            int const it = static_cast<int>(S[2 * k]);
+            st.total_iterations += it;
+            st.max_iterations = std::max(st.max_iterations, it);
+            st.max_relative_residual = std::max(st.max_relative_residual, static_cast<float>(S[2 * k + 1]));
+        }
+    }
     RestoreState(h, endX.ptr, endV.ptr, endVp.ptr);
     if (dv0)
         DownloadVec(h, R.vbar.ptr, dv0);
@@ -1153,7 +1239,7 @@ void AdjointSolve(pbat_vbd_integrator& h, float const* x, float dt, float const*
     if (!x || !b || !y || !(dt > 0.f))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "invalid adjoint solve arguments");
     NoContact(h);
-    pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5};
+    pbat_vbd_adjoint_config cfg{PBAT_VBD_ADJOINT_PCG, 200, 1e-6f, 5, 0};
     if (cfgIn)
         cfg = *cfgIn;
     CheckConfig(cfg);
@@ -1168,7 +1254,7 @@ void AdjointSolve(pbat_vbd_integrator& h, float const* x, float dt, float const*
     Zero4(h, R.y.ptr, nV);
     if (cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL)
         BuildCoarse(h, R.hold.ptr, dt);
-    SolveResult const sr = Solve(h, R.hold.ptr, dt, R.y.ptr, R.rhs.ptr, cfg);
+    SolveResult const sr = Solve(h, R.hold.ptr, dt, R.y.ptr, R.rhs.ptr, cfg, 0, nullptr);
     DownloadVec(h, R.y.ptr, y);
     if (stats)
         *stats = pbat_vbd_adjoint_stats{1, sr.iterations, sr.iterations, static_cast<float>(sr.relResidual)};

@@ -761,6 +761,23 @@ __global__ __launch_bounds__(256) void CoarseGuard(double* __restrict__ buf0, do
         A[k] = 0.;
 }
 
+// The guarded inverse, symmetrised and rounded to FP32 for the CG's coarse
+// products (grid.y = mesh): it halves the bytes every iteration reads; the
+// preconditioner stays symmetric.
+__global__ __launch_bounds__(256) void CoarseToFloat(double const* __restrict__ buf0, double const* __restrict__ buf1,
+                                                     int const* __restrict__ finalBuf, CoarseArgs c,
+                                                     float* __restrict__ out)
+{
+    int const m = static_cast<int>(blockIdx.y);
+    int const n = 3 * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
+    int64_t const idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= static_cast<int64_t>(n) * n)
+        return;
+    int64_t const i = idx / n, j = idx % n;
+    double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m];
+    out[c.meshOffset[m] + idx] = static_cast<float>(0.5 * (A[i * n + j] + A[j * n + i]));
+}
+
 __global__ __launch_bounds__(256) void ZeroDoublesLong(double* __restrict__ p, long long n)
 {
     for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -786,18 +803,6 @@ __global__ __launch_bounds__(256) void Copy4(float4* __restrict__ dst, float4 co
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n)
         dst[i] = src[i];
-}
-
-// Initial guess of the next backward solve from the last two solutions:
-// y = 2 y - yPrev (linear extrapolation in the step), yPrev = the old y.
-__global__ __launch_bounds__(256) void Extrapolate4(float4* __restrict__ y, float4* __restrict__ yPrev, int n)
-{
-    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n)
-        return;
-    float4 const c = y[i], p = yPrev[i];
-    yPrev[i] = c;
-    y[i] = make_float4(2.f * c.x - p.x, 2.f * c.y - p.y, 2.f * c.z - p.z, 0.f);
 }
 
 __global__ __launch_bounds__(256) void Fill4(float4* __restrict__ dst, int n)

@@ -485,7 +485,7 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
 
 // Row chunks of the fused CG iteration: each (mesh, colour) run of rows is cut
 // into chunks of at most 256 / L rows, numbered mesh-major, so the chunks of
-// mesh m are [meshChunkBegin[m], meshChunkBegin[m + 1]) and every chunk holds
+// a mesh are consecutive (aggChunks holds each aggregate's mesh's range) and every chunk holds
 // rows of one mesh. A chunk's sums depend only on its own rows, so a mesh gets
 // the same partial sums whether it is stepped alone or in a batch.
 //
@@ -612,7 +612,7 @@ struct AggUpdateArgs
     float4 const* __restrict__ dinv;
     double* __restrict__ rc;      // nullable
     double const* __restrict__ pqPart;
-    int const* __restrict__ meshChunkBegin;
+    int2 const* __restrict__ aggChunks; // chunks of the aggregate's mesh
     double* __restrict__ rzPart;
     double* __restrict__ rrPart;
     double* __restrict__ bbPart; // init only
@@ -626,17 +626,24 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
     __shared__ double lds1[2][1];
     __shared__ double lds6[2][6];
     int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
+    // independent loads first: the mesh, its chunk range and the aggregate's
+    // vertex range; then the done flag and the first vertex's index. A converged
+    // mesh's workgroups leave after those; the others load the mesh's chunk
+    // partials, r . z and the first vertex's data together.
     int const m = u.aggMesh[g];
-    if (!u.init && u.cg.F[kDone * nM + m] != 0)
-        return; // the aggregate's mesh has converged
+    int2 const cr = u.aggChunks[g];
     int const kBeg = u.aggPtr[g] + t, kEnd = u.aggPtr[g + 1];
+    int const done = u.init ? 0 : u.cg.F[kDone * nM + m];
+    int i0 = kBeg < kEnd ? u.aggVerts[kBeg] : -1;
+    if (done != 0)
+        return; // the aggregate's mesh has converged (uniform over the workgroup)
+    double pq[1] = {u.init ? 0. : StridedSum<128>(u.pqPart, cr.x, cr.y, t)};
+    double const rzOld = u.init ? 0. : u.cg.P[kRz * nM + m];
     // the thread's first vertex: loads issued before alpha is known
-    int i0 = -1;
     float4 const zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 q0 = zero4, a0 = zero4, y0 = zero4, r0 = zero4, d0 = zero4, d1 = zero4;
-    if (kBeg < kEnd)
+    if (i0 >= 0)
     {
-        i0 = u.aggVerts[kBeg];
         q0 = u.q[i0];
         a0 = u.init ? u.b[i0] : u.p[i0];
         if (!u.init)
@@ -650,10 +657,8 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
     float alpha = 0.f;
     if (!u.init)
     {
-        double pq[1] = {StridedSum<128>(u.pqPart, u.meshChunkBegin[m], u.meshChunkBegin[m + 1], t)};
         Sum128<1>(pq, lds1);
-        bool const done = u.cg.F[kDone * nM + m] != 0;
-        alpha = (!done && pq[0] != 0.) ? static_cast<float>(u.cg.P[kRz * nM + m] / pq[0]) : 0.f;
+        alpha = pq[0] != 0. ? static_cast<float>(rzOld / pq[0]) : 0.f;
     }
     double sx = 0., sy = 0., sz = 0., rz = 0., rr = 0., bb = 0.;
     // init: a = b, r = b - q; update: a = p, y += alpha p, r -= alpha q
@@ -712,9 +717,7 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 
 // zc = (Z^T H Z)^-1 rc per mesh (wpr waves per coarse row) and each coarse
 // row's share of rc . zc, the coarse part of the mesh's r . z, into rczcPart.
-__global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict__ buf0,
-                                                       double const* __restrict__ buf1,
-                                                       int const* __restrict__ finalBuf,
+__global__ __launch_bounds__(256) void CoarseApplyRows(float const* __restrict__ coarseF,
                                                        double const* __restrict__ rc, double* __restrict__ zc,
                                                        CoarseArgs c, int wpr, CgState cg,
                                                        double* __restrict__ rczcPart, int first)
@@ -731,9 +734,9 @@ __global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict_
         int const m = c.aggMesh[row / 3];
         int const b0 = 3 * c.meshAggBegin[m];
         int const n = 3 * c.meshAggBegin[m + 1] - b0;
-        double const* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
+        float const* A = coarseF + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
         for (int j = sub * 64 + lane; j < n; j += 64 * wpr)
-            sum += A[j] * rc[b0 + j];
+            sum += double(A[j]) * rc[b0 + j];
     }
     for (int off = 32; off > 0; off >>= 1)
         sum += __shfl_xor(sum, off);
@@ -748,6 +751,256 @@ __global__ __launch_bounds__(256) void CoarseApplyRows(double const* __restrict_
         zc[row] = t;
         rczcPart[row] = t * rc[row];
     }
+}
+
+// A solve's statistics without a host round trip: out = (largest per-mesh
+// iteration count, largest per-mesh |r| / |b| at its last iteration).
+__global__ __launch_bounds__(64) void SolveStats(CgState cg, double* __restrict__ out)
+{
+    if (threadIdx.x != 0)
+        return;
+    int const nM = cg.nM;
+    int it = 0;
+    double rel = 0.;
+    for (int m = 0; m < nM; ++m)
+    {
+        it = max(it, cg.F[kIters * nM + m]);
+        double const bb = cg.P[kBb * nM + m];
+        if (bb > 0.)
+            rel = fmax(rel, sqrt(cg.P[kRrLast * nM + m] / bb));
+    }
+    out[0] = it;
+    out[1] = rel;
+}
+
+// ---- initial guess of a backward solve from the previous solutions ----
+// With s_1, ..., s_k the previous solves' solutions (newest first), the basis
+// b_j = the j-th backward difference at s_1 (b_0 = s_1, b_1 = s_1 - s_2,
+// b_2 = s_1 - 2 s_2 + s_3, ...: the same span, far better conditioned than the
+// nearly parallel solutions themselves) and y0 = sum_j c_j b_j with, per mesh,
+// (B^T H B) c = B^T b: the element of the span closest to the solution in the
+// H-norm, which is the norm CG minimises. Polynomial extrapolation of the
+// solutions is one element of that span.
+constexpr int kProjMax = 8;
+constexpr int kProjPairs = kProjMax * (kProjMax + 1) / 2;
+constexpr int kProjTerms = kProjPairs + kProjMax; // b_p . H b_q (p <= q), b_p . b
+
+struct ProjArgs
+{
+    float4 const* hist[kProjMax]; // previous solutions, newest first
+    float4* basis[kProjMax];
+    float4* hb[kProjMax];         // H b_j
+    float4 const* __restrict__ rhs;
+    float4* __restrict__ y;
+    int const* __restrict__ aggPtr;
+    int const* __restrict__ aggVerts;
+    int const* __restrict__ meshAggBegin;
+    int const* __restrict__ meshOf;
+    double* __restrict__ part; // kProjTerms x nAgg, term-major
+    double* __restrict__ coef; // kProjMax x nM
+    int k, n, nM, nAgg;
+};
+
+__global__ __launch_bounds__(256) void ProjBasis(ProjArgs a)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= a.n)
+        return;
+    float4 s[kProjMax];
+#pragma unroll
+    for (int j = 0; j < kProjMax; ++j)
+        s[j] = j < a.k ? a.hist[j][i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    a.basis[0][i] = s[0];
+    // d holds the differences of the current order, b their first entry
+#pragma unroll
+    for (int o = 1; o < kProjMax; ++o)
+    {
+        if (o >= a.k)
+            break;
+#pragma unroll
+        for (int j = 0; j + o < kProjMax; ++j)
+            s[j] = make_float4(s[j].x - s[j + 1].x, s[j].y - s[j + 1].y, s[j].z - s[j + 1].z, 0.f);
+        a.basis[o][i] = s[0];
+    }
+}
+
+// One 128-thread workgroup per aggregate: its vertices' b_p . H b_q and
+// b_p . b in FP64, summed in a fixed order.
+__global__ __launch_bounds__(128) void ProjGram(ProjArgs a)
+{
+    __shared__ double lds[2][kProjTerms];
+    int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
+    double acc[kProjTerms];
+#pragma unroll
+    for (int u = 0; u < kProjTerms; ++u)
+        acc[u] = 0.;
+    for (int kk = a.aggPtr[g] + t; kk < a.aggPtr[g + 1]; kk += 128)
+    {
+        int const i = a.aggVerts[kk];
+        float4 b[kProjMax], hb[kProjMax];
+#pragma unroll
+        for (int j = 0; j < kProjMax; ++j)
+        {
+            b[j] = j < a.k ? a.basis[j][i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            hb[j] = j < a.k ? a.hb[j][i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        float4 const r = a.rhs[i];
+        int u = 0;
+#pragma unroll
+        for (int p = 0; p < kProjMax; ++p)
+#pragma unroll
+            for (int q = p; q < kProjMax; ++q, ++u)
+                if (q < a.k)
+                    acc[u] += Dot3(b[p], hb[q]);
+#pragma unroll
+        for (int p = 0; p < kProjMax; ++p)
+            if (p < a.k)
+                acc[kProjPairs + p] += Dot3(b[p], r);
+    }
+    Sum128<kProjTerms>(acc, lds);
+    if (t < kProjTerms)
+    {
+        double v = 0.;
+#pragma unroll
+        for (int u = 0; u < kProjTerms; ++u)
+            v = u == t ? acc[u] : v;
+        a.part[static_cast<size_t>(t) * a.nAgg + g] = v;
+    }
+}
+
+// Sum of v[b .. e) over one wave in a fixed order; every lane gets it.
+__device__ __forceinline__ double MeshSum(double const* __restrict__ v, int b, int e, int lane)
+{
+    double s = StridedSum<64>(v, b, e, lane);
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off);
+    return s;
+}
+
+// One workgroup per mesh: the mesh's sums over its aggregates in a fixed order
+// (each by one wave), then (B^T H B) c = B^T b scaled to a unit diagonal and
+// solved by LDL^T, dropping a basis vector whose pivot falls below 1e-6
+// (numerically in the span of the others) or whose diagonal is not positive.
+__global__ __launch_bounds__(256) void ProjSolve(ProjArgs a)
+{
+    __shared__ double sums[kProjTerms];
+    int const m = static_cast<int>(blockIdx.x);
+    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+    int const a0 = a.meshAggBegin[m], a1 = a.meshAggBegin[m + 1];
+    // term u (in use: both basis indices below k) summed by wave u mod 4
+    {
+        int u = 0;
+#pragma unroll
+        for (int p = 0; p < kProjMax; ++p)
+#pragma unroll
+            for (int q = p; q < kProjMax; ++q, ++u)
+                if ((u & 3) == wave)
+                {
+                    double const v = q < a.k ? MeshSum(a.part + static_cast<size_t>(u) * a.nAgg, a0, a1, lane) : 0.;
+                    if (lane == 0)
+                        sums[u] = v;
+                }
+#pragma unroll
+        for (int p = 0; p < kProjMax; ++p, ++u)
+            if ((u & 3) == wave)
+            {
+                double const v = p < a.k ? MeshSum(a.part + static_cast<size_t>(u) * a.nAgg, a0, a1, lane) : 0.;
+                if (lane == 0)
+                    sums[u] = v;
+            }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0)
+        return;
+    double sum[kProjTerms];
+#pragma unroll
+    for (int u = 0; u < kProjTerms; ++u)
+        sum[u] = sums[u];
+
+    int const k = a.k;
+    double G[kProjMax][kProjMax], f[kProjMax], sc[kProjMax], L[kProjMax][kProjMax], d[kProjMax], c[kProjMax];
+    bool use[kProjMax];
+    {
+        int u = 0;
+        for (int p = 0; p < kProjMax; ++p)
+            for (int q = p; q < kProjMax; ++q, ++u)
+                G[p][q] = G[q][p] = sum[u];
+        for (int p = 0; p < kProjMax; ++p)
+            f[p] = sum[kProjPairs + p];
+    }
+    for (int p = 0; p < kProjMax; ++p)
+    {
+        use[p] = p < k && G[p][p] > 0.;
+        sc[p] = use[p] ? 1. / sqrt(G[p][p]) : 0.;
+    }
+    for (int p = 0; p < kProjMax; ++p)
+    {
+        d[p] = 0.;
+        for (int q = 0; q < kProjMax; ++q)
+            L[p][q] = 0.;
+    }
+    for (int p = 0; p < kProjMax; ++p)
+    {
+        if (!use[p])
+            continue;
+        double dp = G[p][p] * sc[p] * sc[p];
+        for (int q = 0; q < p; ++q)
+            if (use[q])
+                dp -= L[p][q] * L[p][q] * d[q];
+        if (!(dp > 1e-6))
+        {
+            use[p] = false;
+            continue;
+        }
+        d[p] = dp;
+        for (int r = p + 1; r < kProjMax; ++r)
+        {
+            if (!use[r])
+                continue;
+            double v = G[r][p] * sc[r] * sc[p];
+            for (int q = 0; q < p; ++q)
+                if (use[q])
+                    v -= L[r][q] * L[p][q] * d[q];
+            L[r][p] = v / dp;
+        }
+    }
+    // L D L^T c~ = f~ over the kept vectors
+    for (int p = 0; p < kProjMax; ++p)
+    {
+        double v = use[p] ? f[p] * sc[p] : 0.;
+        for (int q = 0; q < p; ++q)
+            v -= L[p][q] * c[q];
+        c[p] = use[p] ? v : 0.;
+    }
+    for (int p = 0; p < kProjMax; ++p)
+        c[p] = use[p] ? c[p] / d[p] : 0.;
+    for (int p = kProjMax - 1; p >= 0; --p)
+    {
+        double v = c[p];
+        for (int q = p + 1; q < kProjMax; ++q)
+            v -= L[q][p] * c[q];
+        c[p] = use[p] ? v : 0.;
+    }
+    for (int p = 0; p < kProjMax; ++p)
+        a.coef[static_cast<size_t>(m) * kProjMax + p] = c[p] * sc[p];
+}
+
+__global__ __launch_bounds__(256) void ProjCombine(ProjArgs a)
+{
+    int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= a.n)
+        return;
+    double const* c = a.coef + static_cast<size_t>(a.meshOf[i]) * kProjMax;
+    double x = 0., y = 0., z = 0.;
+#pragma unroll
+    for (int j = 0; j < kProjMax; ++j)
+    {
+        if (j >= a.k)
+            break;
+        float4 const b = a.basis[j][i];
+        x += c[j] * b.x, y += c[j] * b.y, z += c[j] * b.z;
+    }
+    a.y[i] = make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z), 0.f);
 }
 
 // End of an iteration of the fused CG, one wave per mesh (4 per workgroup):

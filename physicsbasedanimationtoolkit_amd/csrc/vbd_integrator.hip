@@ -439,8 +439,9 @@ void CheckStatus(pbat_vbd_integrator& h)
              "the state is left as computed");
 }
 
-void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
+void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps, bool sync)
 {
+    sync = sync || h.timing; // timed launches are collected after a synchronisation
     if (!(dt > 0.f) || !std::isfinite(dt))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "dt must be positive and finite");
     if (iterations < 0 || substeps < 1)
@@ -448,11 +449,14 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
     PBAT_HIP(hipSetDevice(h.device));
     if (LinearTrustRegion(h))
         EnsureTrustRegionBuffers(h, iterations, substeps);
-    ClearStatus(h);
+    if (sync)
+        ClearStatus(h);
     if (!h.useGraph)
     {
         InvalidateGraph(h);
         EnqueueStep(h, dt, iterations, substeps, h.timing);
+        if (!sync)
+            return;
         if (h.timing)
             CollectTiming(h);
         PBAT_HIP(hipStreamSynchronize(h.stream));
@@ -499,6 +503,8 @@ void Step(pbat_vbd_integrator& h, float dt, int iterations, int substeps)
         PBAT_HIP(hipEventRecord(NextEvent(h), h.stream));
         h.pendingLaunches.push_back(h.graphSweeps);
     }
+    if (!sync)
+        return;
     PBAT_HIP(hipStreamSynchronize(h.stream));
     if (h.timing)
         CollectTiming(h);

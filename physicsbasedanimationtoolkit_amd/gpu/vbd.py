@@ -360,12 +360,16 @@ class Integrator:
 
     @staticmethod
     def adjoint_config(solver: str = "pcg", max_iterations: int = 200, tolerance: float = 1e-6,
-                       check_every: int = 5) -> _native.AdjointConfig:
+                       check_every: int = 5, history: int = 6) -> _native.AdjointConfig:
+        """history: the block Jacobi and two-level solvers start each solve of a
+        backward pass from the H-norm projection of its solution onto the span
+        of the last `history` solutions (0..8; 0 starts from the last one)."""
         kinds = {"pcg": _native.ADJOINT_PCG, "gauss_seidel": _native.ADJOINT_GAUSS_SEIDEL,
                  "pcg_jacobi": _native.ADJOINT_PCG_JACOBI, "pcg_two_level": _native.ADJOINT_PCG_TWO_LEVEL}
         if solver not in kinds:
             raise ValueError(f"solver must be one of {sorted(kinds)}")
-        return _native.AdjointConfig(kinds[solver], int(max_iterations), float(tolerance), int(check_every))
+        return _native.AdjointConfig(kinds[solver], int(max_iterations), float(tolerance), int(check_every),
+                                     int(history))
 
     def rollout_backward(self, dL_dx, dL_dv=None, **solver):
         """Gradients of a loss L(x_T, v_T) of the last rollout_forward.

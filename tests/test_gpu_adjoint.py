@@ -408,6 +408,11 @@ def test_adjoint_errors(gpu_device):
         g.set_log_young_modulus(np.zeros(3))
     with pytest.raises(ValueError):
         g.set_log_young_modulus(np.full(E.shape[1], np.nan))
+    g.set_sweep_config(0, 0, True, False)
+    g.rollout_forward(DT, 5, 2)
+    for history in (-1, 9):
+        with pytest.raises(ValueError):
+            g.rollout_backward(np.ones_like(X), solver="pcg_two_level", history=history)
 
 
 def test_differentiable_workload_checkpoint_intervals_agree(gpu_device):
