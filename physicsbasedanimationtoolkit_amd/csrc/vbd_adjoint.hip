@@ -19,7 +19,6 @@
 #include "vbd_hessian_kernels.hpp"
 
 #include <algorithm>
-#include <array>
 #include <cmath>
 #include <functional>
 #include <string>
@@ -429,21 +428,35 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
 
     // coarse blocks: the fine blocks grouped by (aggregate of the row, aggregate
     // of the column), ascending within each group
-    std::vector<std::array<int, 3>> keys(nBlocks);
+    // (two stable counting passes: by column aggregate, then by row aggregate)
+    int const nAggAll = std::max(h.nAggregates, 1);
+    auto countingPass = [&](std::vector<int> const& in, auto&& keyOf) {
+        std::vector<int> start(static_cast<size_t>(nAggAll) + 1, 0), out(in.size());
+        for (int k : in)
+            ++start[keyOf(k) + 1];
+        for (int a = 0; a < nAggAll; ++a)
+            start[a + 1] += start[a];
+        for (int k : in)
+            out[start[keyOf(k)]++] = k;
+        return out;
+    };
+    std::vector<int> order(nBlocks);
     for (size_t k = 0; k < nBlocks; ++k)
-        keys[k] = {aggOf[blockRow[k]], aggOf[col[k]], static_cast<int>(k)};
-    std::sort(keys.begin(), keys.end());
+        order[k] = static_cast<int>(k);
+    order = countingPass(order, [&](int k) { return aggOf[col[k]]; });
+    order = countingPass(order, [&](int k) { return aggOf[blockRow[k]]; });
     std::vector<int> cbA, cbB, cbPtr{0}, cbSrc(nBlocks);
-    for (size_t k = 0; k < nBlocks; ++k)
+    for (size_t t = 0; t < nBlocks; ++t)
     {
-        if (k == 0 || keys[k][0] != keys[k - 1][0] || keys[k][1] != keys[k - 1][1])
+        int const k = order[t], a = aggOf[blockRow[k]], b = aggOf[col[k]];
+        if (t == 0 || a != cbA.back() || b != cbB.back())
         {
-            if (k > 0)
-                cbPtr.push_back(static_cast<int>(k));
-            cbA.push_back(keys[k][0]);
-            cbB.push_back(keys[k][1]);
+            if (t > 0)
+                cbPtr.push_back(static_cast<int>(t));
+            cbA.push_back(a);
+            cbB.push_back(b);
         }
-        cbSrc[k] = keys[k][2];
+        cbSrc[t] = k;
     }
     if (nBlocks > 0)
         cbPtr.push_back(static_cast<int>(nBlocks));
@@ -856,7 +869,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         f.rzPart = R.rzPart.ptr, f.rrPart = R.rrPart.ptr, f.bbPart = R.bbPart.ptr;
         f.rczcPart = twoLevel ? R.rczcPart.ptr : nullptr;
         f.doneOut = R.doneDev;
-        hipLaunchKernelGGL(device::CgFinish, dim3((nM + 3) / 4), dim3(256), 0, h.stream, f, first);
+        hipLaunchKernelGGL(device::CgFinish, dim3(nM), dim3(256), 0, h.stream, f, first);
     };
     std::function<void(int)> iteration;
     if (fused)

@@ -757,20 +757,26 @@ __global__ __launch_bounds__(256) void CoarseApplyRows(float const* __restrict__
 // iteration count, largest per-mesh |r| / |b| at its last iteration).
 __global__ __launch_bounds__(64) void SolveStats(CgState cg, double* __restrict__ out)
 {
-    if (threadIdx.x != 0)
-        return;
-    int const nM = cg.nM;
+    int const nM = cg.nM, lane = static_cast<int>(threadIdx.x);
     int it = 0;
     double rel = 0.;
-    for (int m = 0; m < nM; ++m)
+    for (int m = lane; m < nM; m += 64)
     {
         it = max(it, cg.F[kIters * nM + m]);
         double const bb = cg.P[kBb * nM + m];
         if (bb > 0.)
             rel = fmax(rel, sqrt(cg.P[kRrLast * nM + m] / bb));
     }
-    out[0] = it;
-    out[1] = rel;
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        it = max(it, __shfl_xor(it, off));
+        rel = fmax(rel, __shfl_xor(rel, off));
+    }
+    if (lane == 0)
+    {
+        out[0] = it;
+        out[1] = rel;
+    }
 }
 
 // ---- initial guess of a backward solve from the previous solutions ----
@@ -1003,9 +1009,9 @@ __global__ __launch_bounds__(256) void ProjCombine(ProjArgs a)
     a.y[i] = make_float4(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z), 0.f);
 }
 
-// End of an iteration of the fused CG, one wave per mesh (4 per workgroup):
-// the mesh's partials summed in a fixed order, r . z = sum of the aggregates'
-// r . D^-1 r (+ the coarse rows' rc . zc), then the stopping test and beta
+// End of an iteration of the fused CG, one workgroup per mesh: the mesh's
+// partials summed in a fixed order (one sum per wave), r . z = the aggregates'
+// r . D^-1 r + the coarse rows' rc . zc, then the stopping test and beta
 // (FinishMesh). Converged meshes are skipped; the host counts live meshes from
 // their done flags.
 struct CgFinishArgs
@@ -1021,26 +1027,31 @@ struct CgFinishArgs
 
 __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
 {
+    __shared__ double part[4];
     int const nM = f.cg.nM;
-    int const m = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x >> 6);
-    int const lane = static_cast<int>(threadIdx.x & 63);
-    if (m >= nM || (!first && f.cg.F[kDone * nM + m] != 0))
+    int const m = static_cast<int>(blockIdx.x);
+    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+    if (!first && f.cg.F[kDone * nM + m] != 0)
         return;
     int const a0 = f.meshAggBegin[m], a1 = f.meshAggBegin[m + 1];
-    double rz = StridedSum<64>(f.rzPart, a0, a1, lane);
-    double rr = StridedSum<64>(f.rrPart, a0, a1, lane);
-    double bb = first ? StridedSum<64>(f.bbPart, a0, a1, lane) : 0.;
-    if (f.rczcPart)
-        rz += StridedSum<64>(f.rczcPart, 3 * a0, 3 * a1, lane);
+    // one sum per wave, all in flight together: r . D^-1 r, r . r, rc . zc, b . b
+    double v = 0.;
+    if (wave == 0)
+        v = StridedSum<64>(f.rzPart, a0, a1, lane);
+    else if (wave == 1)
+        v = StridedSum<64>(f.rrPart, a0, a1, lane);
+    else if (wave == 2 && f.rczcPart)
+        v = StridedSum<64>(f.rczcPart, 3 * a0, 3 * a1, lane);
+    else if (wave == 3 && first)
+        v = StridedSum<64>(f.bbPart, a0, a1, lane);
     for (int off = 32; off > 0; off >>= 1)
-    {
-        rz += __shfl_xor(rz, off);
-        rr += __shfl_xor(rr, off);
-        bb += __shfl_xor(bb, off);
-    }
+        v += __shfl_xor(v, off);
     if (lane == 0)
+        part[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
     {
-        bool const live = FinishMesh(f.cg, m, rr, rz, bb, first);
+        bool const live = FinishMesh(f.cg, m, part[1], part[0] + part[2], part[3], first);
         if (f.doneOut)
             f.doneOut[m] = live ? 0 : 1;
     }
