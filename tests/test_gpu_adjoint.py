@@ -191,6 +191,37 @@ def test_rollout_gradient_matches_fp64_ift(gpu_device, solver):
         assert e <= GRAD_TOL, key
 
 
+@pytest.mark.parametrize("history", [0, 1, 3, 8])
+@pytest.mark.parametrize("solver", ["pcg_jacobi", "pcg_two_level"])
+def test_projected_initial_guess_matches_fp64_ift(gpu_device, solver, history):
+    """Each solve of a backward pass started from the last solution (history 0)
+    or from its projection onto the span of the last 1, 3 or 8 solutions: 12
+    steps, so the longest history is reached, and the gradients stay within
+    tolerance of the FP64 IFT gradient; the projected starts take at most one
+    CG iteration per solve more than the plain ones (fewer in practice)."""
+    X, E, mu, lam, v0 = diff_scene(8, seed=3)
+    steps, iters = 12, 20
+    g = make_gpu(X, E, mu, lam, v0)
+    o = make_oracle(X, E, mu, lam, v0)
+    v0f = o.v.T.copy()
+    xs = gpu_trajectory(g, X, v0f, steps, iters)
+    rng = np.random.default_rng(22)
+    w = rng.normal(size=X.shape)
+    ref = A.ift_gradient(o, xs, DT, w.T, np.zeros_like(w.T))
+    g.x = X
+    g.v = v0f
+    g.rollout_forward(DT, iters, steps, checkpoint_every=4)
+    kw = dict(solver=solver, tolerance=1e-6, max_iterations=600, check_every=10)
+    r = g.rollout_backward(w, None, history=history, **kw)
+    r0 = g.rollout_backward(w, None, history=0, **kw)
+    print(solver, history, "stats", r["stats"], r0["stats"])
+    for key, refk in (("log_young_modulus", ref[0]), ("v0", ref[1].T), ("x0", ref[2].T)):
+        e = rel(r[key], refk)
+        print(f"{key}: rel err {e:.3e}")
+        assert e <= GRAD_TOL, key
+    assert r["stats"]["total_iterations"] <= r0["stats"]["total_iterations"] + steps
+
+
 def test_differentiable_workload_gradient_matches_fp64(gpu_device):
     """The 32^3 differentiable workload's settings on a 12^3 grid (the FP64 direct
     solve of the 32^3 step Hessian takes minutes): hidden per-tet log Y, v0 = 0.5 x
