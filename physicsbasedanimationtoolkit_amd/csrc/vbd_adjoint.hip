@@ -292,7 +292,12 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
                 if (vid(e, l) < nUpd)
                     vinc[fill[vid(e, l)]++] = static_cast<int>(4 * e + l);
     }
+    // rows, and per incidence the row positions of its tet's vertices (-1: not
+    // updated) through a dense vertex -> position marker reset after each row
     std::vector<int> rowPtr(static_cast<size_t>(nUpd) + 1, 0), col, blockRow, cols;
+    std::vector<int> slotOf(static_cast<size_t>(nUpd), -1);
+    std::vector<int4> incSlot(vinc.size());
+    col.reserve(16 * static_cast<size_t>(nUpd)), blockRow.reserve(16 * static_cast<size_t>(nUpd));
     int maxRow = 0;
     for (int i = 0; i < nUpd; ++i)
     {
@@ -306,6 +311,20 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
             }
         std::sort(cols.begin(), cols.end());
         cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        for (size_t c = 0; c < cols.size(); ++c)
+            slotOf[cols[c]] = static_cast<int>(c);
+        for (int k = vp[i]; k < vp[i + 1]; ++k)
+        {
+            int s4[4];
+            for (int l = 0; l < 4; ++l)
+            {
+                int const j = vid(vinc[k] >> 2, l);
+                s4[l] = j < nUpd ? slotOf[j] : -1;
+            }
+            incSlot[k] = make_int4(s4[0], s4[1], s4[2], s4[3]);
+        }
+        for (int j : cols)
+            slotOf[j] = -1;
         col.insert(col.end(), cols.begin(), cols.end());
         blockRow.insert(blockRow.end(), cols.size(), i);
         rowPtr[i + 1] = rowPtr[i] + static_cast<int>(cols.size());
@@ -317,15 +336,10 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
             for (int k = vp[i]; k < vp[i + 1]; ++k)
             {
                 int const e = vinc[k] >> 2, la = vinc[k] & 3;
+                int const s4[4] = {incSlot[k].x, incSlot[k].y, incSlot[k].z, incSlot[k].w};
                 for (int lb = 0; lb < 4; ++lb)
-                {
-                    int const j = vid(e, lb);
-                    if (j >= nUpd)
-                        continue;
-                    int const slot = static_cast<int>(
-                        std::lower_bound(col.begin() + rowPtr[i], col.begin() + rowPtr[i + 1], j) - col.begin());
-                    f(slot, 16 * e + 4 * la + lb);
-                }
+                    if (s4[lb] >= 0)
+                        f(rowPtr[i] + s4[lb], 16 * e + 4 * la + lb);
             }
     };
     R.bsrRowPtr.upload(rowPtr, h.stream);
@@ -334,22 +348,7 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
     R.bsrA.allocate(nBlocks), R.bsrB.allocate(nBlocks), R.bsrC.allocate(nBlocks);
     if (maxRow <= kRowAssembleMaxRow)
     {
-        // row-major assembly: per incidence, the row positions of its tet's vertices
-        std::vector<int4> incSlot(vinc.size());
-        for (int i = 0; i < nUpd; ++i)
-            for (int k = vp[i]; k < vp[i + 1]; ++k)
-            {
-                int s4[4];
-                for (int l = 0; l < 4; ++l)
-                {
-                    int const j = vid(vinc[k] >> 2, l);
-                    s4[l] = j < nUpd ? static_cast<int>(std::lower_bound(col.begin() + rowPtr[i],
-                                                                         col.begin() + rowPtr[i + 1], j) -
-                                                        (col.begin() + rowPtr[i]))
-                                     : -1;
-                }
-                incSlot[k] = make_int4(s4[0], s4[1], s4[2], s4[3]);
-            }
+        // row-major assembly: the incidences and their tets' row positions
         R.incPtr.upload(vp, h.stream);
         R.incCode.upload(vinc, h.stream);
         R.incSlot.upload(incSlot, h.stream);
