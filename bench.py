@@ -620,7 +620,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the VBD path has no multi-GPU form in this build "
+        print("bench.py measures one GPU only: the VBD path has no multi-GPU form here "
               "(a batch of meshes would shard by mesh, a single mesh by spatial subdomain with a halo "
               "exchange per colour); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
