@@ -63,7 +63,7 @@ def _counter_traffic(name, shape):
     """HBM bytes per sweep launch from a committed rocprofv3 counter summary
     (profiles/<name>: FETCH_SIZE and WRITE_SIZE runs per sweep shape, each scaled by
     what the counter reads for a device copy of known size). Used only when the
-    summary was measured on this build's sweep kernel source and holds the sweep
+    summary was measured on the current sweep kernel source and holds the sweep
     shape now in effect; returns (entry or None, provenance note)."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
