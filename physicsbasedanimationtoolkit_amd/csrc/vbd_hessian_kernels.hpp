@@ -212,18 +212,21 @@ struct RowAssembleArgs
 
 // 32 lanes per row, two rows per wave, one wave per workgroup. The row's
 // incidences are taken 32 at a time, one per lane: the lane forms its tet's
-// F, cof F and J in FP64 and the four blocks (la, lb), lb = 0..3, and stages
-// them in LDS as FP32; lane l then adds, for each block it owns (l, l + 32, ...,
-// NB of them, below the row length), the staged blocks that fall on it in
-// incidence order (ascending tet), in FP64. Diagonal blocks also receive
-// m_i / h^2 I and write the FP64-inverted block to dinv[2 i], dinv[2 i + 1] as
-// (00, 01, 02, 11), (12, 22).
+// F, cof F and J in FP64 and the four blocks (la, lb), lb = 0..3. The diagonal
+// block (la, la) of every incidence is summed over the row's lanes by a
+// fixed-order butterfly in FP64; the three others are staged in LDS as FP32,
+// and lane l then adds, for each off-diagonal block it owns (l, l + 32, ..., NB
+// of them, below the row length), the staged blocks that fall on it in
+// incidence order (ascending tet), in FP64, visiting only the incidences its
+// block appears in (a bit mask from the staged row positions). Diagonal blocks
+// also receive m_i / h^2 I and write the FP64-inverted block to dinv[2 i],
+// dinv[2 i + 1] as (00, 01, 02, 11), (12, 22).
 template <int NB>
 __global__ __launch_bounds__(64) void RowAssemble(RowAssembleArgs a)
 {
     constexpr int L = 32;
     __shared__ float stage[2][L][4][9];
-    __shared__ int slotS[2][L][4];
+    __shared__ int4 slotS[2][L];
     int const lane = static_cast<int>(threadIdx.x) & (L - 1), half = static_cast<int>(threadIdx.x) >> 5;
     int const i = static_cast<int>(blockIdx.x) * 2 + half;
     int kb = 0, ke = 0, rb = 0, nb = 0;
@@ -237,9 +240,12 @@ __global__ __launch_bounds__(64) void RowAssemble(RowAssembleArgs a)
 #pragma unroll
         for (int c = 0; c < 9; ++c)
             acc[o][c] = 0.;
+    double diag[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
+    int diagSlot = -1; // the row vertex's position in its row (from its first incidence)
     for (int bt = 0; bt < batches; ++bt)
     {
         int const k = kb + bt * L + lane;
+        double Hd[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
         if (k < ke)
         {
             int const code = a.incCode[k];
@@ -254,42 +260,73 @@ __global__ __launch_bounds__(64) void RowAssemble(RowAssembleArgs a)
                 ca[c] = la == 0 ? T.cg[0][c] : la == 1 ? T.cg[1][c] : la == 2 ? T.cg[2][c] : T.cg[3][c];
             }
             double H[9];
+            auto keep = [&](int lb) {
+                if (lb == la)
+                {
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        Hd[c] = H[c];
+                }
+                else
+                {
+#pragma unroll
+                    for (int c = 0; c < 9; ++c)
+                        stage[half][lane][lb][c] = static_cast<float>(H[c]);
+                }
+            };
             T.Block<0>(ga, ca, H);
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-                stage[half][lane][0][c] = static_cast<float>(H[c]);
+            keep(0);
             T.Block<1>(ga, ca, H);
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-                stage[half][lane][1][c] = static_cast<float>(H[c]);
+            keep(1);
             T.Block<2>(ga, ca, H);
-#pragma unroll
-            for (int c = 0; c < 9; ++c)
-                stage[half][lane][2][c] = static_cast<float>(H[c]);
+            keep(2);
             T.Block<3>(ga, ca, H);
+            keep(3);
+            slotS[half][lane] = sl;
+        }
+        // diagonal terms: butterfly over the row's 32 lanes (idle lanes add 0)
 #pragma unroll
-            for (int c = 0; c < 9; ++c)
-                stage[half][lane][3][c] = static_cast<float>(H[c]);
-            slotS[half][lane][0] = sl.x, slotS[half][lane][1] = sl.y;
-            slotS[half][lane][2] = sl.z, slotS[half][lane][3] = sl.w;
+        for (int c = 0; c < 9; ++c)
+        {
+            double v = Hd[c];
+#pragma unroll
+            for (int off = 1; off < L; off <<= 1)
+                v += __shfl_xor(v, off, L);
+            diag[c] += v;
         }
         __syncthreads();
+        if (bt == 0)
+        {
+            // every row has an incidence; lane 0 of the row holds its first
+            int const code0 = kb < ke ? a.incCode[kb] : 0;
+            int4 const s0 = slotS[half][0];
+            int const la0 = code0 & 3;
+            diagSlot = kb < ke ? (la0 == 0 ? s0.x : la0 == 1 ? s0.y : la0 == 2 ? s0.z : s0.w) : -1;
+        }
         int const cnt = min(L, max(0, ke - kb - bt * L));
 #pragma unroll
         for (int o = 0; o < NB; ++o)
         {
             int const b = lane + o * L;
-            if (b >= nb)
+            if (b >= nb || b == diagSlot)
                 continue;
+            unsigned mask = 0u;
             for (int j = 0; j < cnt; ++j)
+            {
+                int4 const sl = slotS[half][j];
+                if (sl.x == b || sl.y == b || sl.z == b || sl.w == b)
+                    mask |= 1u << j;
+            }
+            while (mask)
+            {
+                int const j = __builtin_ctz(mask);
+                mask &= mask - 1u;
+                int4 const sl = slotS[half][j];
+                int const s = sl.x == b ? 0 : sl.y == b ? 1 : sl.z == b ? 2 : 3;
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    if (slotS[half][j][s] == b)
-                    {
-#pragma unroll
-                        for (int c = 0; c < 9; ++c)
-                            acc[o][c] += stage[half][j][s][c];
-                    }
+                for (int c = 0; c < 9; ++c)
+                    acc[o][c] += stage[half][j][s][c];
+            }
         }
         __syncthreads();
     }
@@ -301,9 +338,12 @@ __global__ __launch_bounds__(64) void RowAssemble(RowAssembleArgs a)
             continue;
         int const kBlock = rb + b;
         double* H = acc[o];
-        if (a.col[kBlock] == i)
+        if (b == diagSlot)
         {
             double const m = double(a.xs[i].w) * a.invDt2;
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                H[c] = diag[c];
             H[0] += m, H[4] += m, H[8] += m;
             Inverse3 const C(H[0], 0.5 * (H[1] + H[3]), 0.5 * (H[2] + H[6]), H[4], 0.5 * (H[5] + H[7]), H[8]);
             a.dinv[2 * i] = make_float4(static_cast<float>(C.c00 * C.inv), static_cast<float>(C.c01 * C.inv),
