@@ -176,11 +176,14 @@ struct RolloutState
     DeviceArray<float> bsrC;
     int bsrMaxRow = 0;
     DeviceArray<float4> p2;
-    // row chunks of the fused iteration (begin, end), mesh-major, and for each
-    // aggregate the chunks of its mesh (begin, end); chunks hold at most
-    // bsrChunkRows rows
-    DeviceArray<int2> bsrChunks;
+    // row chunks of the fused iteration (begin, end, mesh, 0), mesh-major, and
+    // for each aggregate the chunks of its mesh (begin, end); chunks hold at
+    // most bsrChunkRows rows
+    DeviceArray<int4> bsrChunks;
     DeviceArray<int2> aggChunks;
+    // per aggregate: its first kAggFirst vertices (-1 past its size) and its
+    // mesh's done flag as of the last finish kernel
+    DeviceArray<int> aggFirst, aggDone;
     int bsrChunkRows = 0;
     // per-chunk p . H p, per-aggregate r . z, r . r, b . b, per-coarse-row rc . zc
     DeviceArray<double> pqPart, rzPart, rrPart, bbPart, rczcPart;

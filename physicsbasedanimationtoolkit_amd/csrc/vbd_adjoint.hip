@@ -405,11 +405,11 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
         }
     }
     std::stable_sort(chunks.begin(), chunks.end(), [](Chunk const& a, Chunk const& b) { return a.mesh < b.mesh; });
-    std::vector<int2> chunkRange(chunks.size());
+    std::vector<int4> chunkRange(chunks.size());
     std::vector<int> meshChunk(static_cast<size_t>(nM) + 1, 0);
     for (size_t k = 0; k < chunks.size(); ++k)
     {
-        chunkRange[k] = make_int2(chunks[k].begin, chunks[k].end);
+        chunkRange[k] = make_int4(chunks[k].begin, chunks[k].end, chunks[k].mesh, 0);
         ++meshChunk[chunks[k].mesh + 1];
     }
     for (int m = 0; m < nM; ++m)
@@ -420,6 +420,22 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
         for (int g = h.aggBegin[m]; g < h.aggBegin[m + 1]; ++g)
             aggChunk[g] = make_int2(meshChunk[m], meshChunk[m + 1]);
     R.aggChunks.upload(aggChunk, h.stream);
+    // each aggregate's first vertices, one per AggUpdate thread
+    {
+        std::vector<int> ptr(static_cast<size_t>(h.nAggregates) + 1), verts(static_cast<size_t>(nUpd));
+        if (h.nAggregates > 0)
+        {
+            PBAT_HIP(hipMemcpy(ptr.data(), h.aggVertPtr.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
+            if (nUpd > 0)
+                PBAT_HIP(hipMemcpy(verts.data(), h.aggVerts.ptr, verts.size() * sizeof(int), hipMemcpyDeviceToHost));
+        }
+        std::vector<int> first(static_cast<size_t>(std::max(h.nAggregates, 1)) * device::kAggFirst, -1);
+        for (int g = 0; g < h.nAggregates; ++g)
+            for (int k = ptr[g]; k < std::min(ptr[g + 1], ptr[g] + device::kAggFirst); ++k)
+                first[static_cast<size_t>(g) * device::kAggFirst + (k - ptr[g])] = verts[k];
+        R.aggFirst.upload(first, h.stream);
+        R.aggDone.allocate(static_cast<size_t>(std::max(h.nAggregates, 1)));
+    }
     R.bsrChunkRows = chunkRows;
     R.pqPart.allocate(std::max<size_t>(chunks.size(), 1));
     size_t const nAgg = static_cast<size_t>(std::max(h.nAggregates, 1));
@@ -588,7 +604,7 @@ void LaunchDirectionLK(pbat_vbd_integrator& h, device::BsrArgs a)
         a.xcdChunk = static_cast<int>(grid64 / 8);
     }
     hipLaunchKernelGGL((device::BsrDirection<L, K>), dim3(static_cast<int>(grid64)), dim3(256), 0, h.stream, a,
-                       static_cast<int2 const*>(R.bsrChunks.ptr), nChunks, R.pqPart.ptr);
+                       static_cast<int4 const*>(R.bsrChunks.ptr), nChunks, R.pqPart.ptr);
 }
 
 template <int L>
@@ -851,6 +867,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         u.y = y, u.r = R.r.ptr, u.z = R.z.ptr, u.p = p, u.q = R.q.ptr, u.b = b, u.dinv = R.dinv.ptr;
         u.rc = twoLevel ? R.rc.ptr : nullptr;
         u.pqPart = R.pqPart.ptr, u.aggChunks = R.aggChunks.ptr;
+        u.aggFirst = R.aggFirst.ptr, u.aggDone = R.aggDone.ptr;
         u.rzPart = R.rzPart.ptr, u.rrPart = R.rrPart.ptr, u.bbPart = R.bbPart.ptr;
         u.cg = cg, u.init = init;
         if (h.nAggregates > 0)
@@ -867,7 +884,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         f.cg = cg, f.meshAggBegin = h.meshAggBegin.ptr;
         f.rzPart = R.rzPart.ptr, f.rrPart = R.rrPart.ptr, f.bbPart = R.bbPart.ptr;
         f.rczcPart = twoLevel ? R.rczcPart.ptr : nullptr;
-        f.doneOut = R.doneDev;
+        f.doneOut = R.doneDev, f.aggDone = R.aggDone.ptr;
         hipLaunchKernelGGL(device::CgFinish, dim3(nM), dim3(256), 0, h.stream, f, first);
     };
     std::function<void(int)> iteration;
@@ -932,7 +949,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
             h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.aggChunks.ptr,
             R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
-            R.bsrColAgg.ptr,   R.doneDev};
+            R.bsrColAgg.ptr,   R.doneDev,       R.aggFirst.ptr, R.aggDone.ptr};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();

@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
 // block, written to y, q = H p to out, and the chunk's p . q, summed over its
 // rows in order, to pqPart[chunk]. Chunks of a converged mesh do nothing.
 template <int L, int K>
-__global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __restrict__ chunks, int nChunks,
+__global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int4 const* __restrict__ chunks, int nChunks,
                                                     double* __restrict__ pqPart)
 {
     constexpr int R = 256 / L;
@@ -544,24 +544,29 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __res
         w = (w & 7) * a.xcdChunk + (w >> 3);
     if (w >= nChunks)
         return; // padding of the XCD order
-    int2 const ch = chunks[w];
-    int const m = a.meshOf[ch.x];
-    if (a.cg.F[kDone * nM + m] != 0)
-        return; // the chunk's mesh has converged: its vectors no longer change
+    // the chunk (rows and mesh), then its mesh's done flag and beta together
+    // with the row's extent and aggregate
+    int4 const ch = chunks[w];
+    int const m = ch.z;
     constexpr bool frozen = false;
     int const r = static_cast<int>(threadIdx.x) / L, lane = static_cast<int>(threadIdx.x) & (L - 1);
     int const row = ch.x + r;
     bool const active = row < ch.y;
+    int const done = a.cg.F[kDone * nM + m];
     float const beta = static_cast<float>(a.cg.P[kBeta * nM + m]);
-    int kb = 0, ke = 0;
-    float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
+    int kb = 0, ke = 0, rowAgg = 0;
     if (active)
     {
         kb = a.rowPtr[row];
         ke = a.rowPtr[row + 1];
-        if (lane == 0) // the row's own direction, in flight during the product
-            pr = DirectionVector(a, row, a.zc ? a.aggOf[row] : 0, beta, frozen);
+        if (lane == 0 && a.zc)
+            rowAgg = a.aggOf[row];
     }
+    if (done != 0)
+        return; // the chunk's mesh has converged: its vectors no longer change
+    float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active && lane == 0) // the row's own direction, in flight during the product
+        pr = DirectionVector(a, row, rowAgg, beta, frozen);
     float3 const h = RowProduct<L, K, BsrMode::kDirection>(a, kb, ke, lane, beta, frozen);
     __shared__ double sdot[R];
     if (lane == 0)
@@ -586,23 +591,23 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int2 const* __res
 }
 
 // Thread t of a group of N threads sums v[b + t], v[b + t + N], ... in that
-// order, issuing the loads 16 at a time (the order, and so the bits, do not
-// depend on the batching: out-of-range entries add an exact 0).
-template <int N>
+// order, issuing the loads U at a time (the order, and so the bits, do not
+// depend on U: out-of-range entries add an exact 0).
+template <int N, int U = 16>
 __device__ __forceinline__ double StridedSum(double const* __restrict__ v, int b, int e, int t)
 {
     double s = 0.;
-    for (int k0 = b + t; k0 < e; k0 += 16 * N)
+    for (int k0 = b + t; k0 < e; k0 += U * N)
     {
-        double x[16];
+        double x[U];
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < U; ++u)
         {
             int const k = k0 + u * N;
             x[u] = k < e ? v[k] : 0.;
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u)
+        for (int u = 0; u < U; ++u)
             s += x[u];
     }
     return s;
@@ -653,6 +658,8 @@ struct AggUpdateArgs
     double* __restrict__ rc;      // nullable
     double const* __restrict__ pqPart;
     int2 const* __restrict__ aggChunks; // chunks of the aggregate's mesh
+    int const* __restrict__ aggFirst;   // kAggFirst vertices per aggregate (-1: none)
+    int const* __restrict__ aggDone;    // done flag of the aggregate's mesh
     double* __restrict__ rzPart;
     double* __restrict__ rrPart;
     double* __restrict__ bbPart; // init only
@@ -660,24 +667,26 @@ struct AggUpdateArgs
     int init;
 };
 
+constexpr int kAggFirst = 128; // = AggUpdate's workgroup size
+
 __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 {
     int const nM = u.cg.nM;
     __shared__ double lds1[2][1];
     __shared__ double lds6[2][6];
     int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
-    // independent loads first: the mesh, its chunk range and the aggregate's
-    // vertex range; then the done flag and the first vertex's index. A converged
+    // independent loads first: the mesh, its chunk range, the aggregate's vertex
+    // range, its mesh's done flag and the thread's first vertex. A converged
     // mesh's workgroups leave after those; the others load the mesh's chunk
     // partials, r . z and the first vertex's data together.
     int const m = u.aggMesh[g];
     int2 const cr = u.aggChunks[g];
     int const kBeg = u.aggPtr[g] + t, kEnd = u.aggPtr[g + 1];
-    int const done = u.init ? 0 : u.cg.F[kDone * nM + m];
-    int i0 = kBeg < kEnd ? u.aggVerts[kBeg] : -1;
+    int const done = u.init ? 0 : u.aggDone[g];
+    int const i0 = u.aggFirst[static_cast<size_t>(g) * kAggFirst + t];
     if (done != 0)
         return; // the aggregate's mesh has converged (uniform over the workgroup)
-    double pq[1] = {u.init ? 0. : StridedSum<128>(u.pqPart, cr.x, cr.y, t)};
+    double pq[1] = {u.init ? 0. : StridedSum<128, 32>(u.pqPart, cr.x, cr.y, t)};
     double const rzOld = u.init ? 0. : u.cg.P[kRz * nM + m];
     // the thread's first vertex: loads issued before alpha is known
     float4 const zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1063,6 +1072,7 @@ struct CgFinishArgs
     double const* __restrict__ bbPart;
     double const* __restrict__ rczcPart; // nullable (block Jacobi)
     int* __restrict__ doneOut;           // nullable: host-mapped copy of the done flags
+    int* __restrict__ aggDone;           // per aggregate: its mesh's done flag
 };
 
 __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
@@ -1081,7 +1091,7 @@ __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
     else if (wave == 1)
         v = StridedSum<64>(f.rrPart, a0, a1, lane);
     else if (wave == 2 && f.rczcPart)
-        v = StridedSum<64>(f.rczcPart, 3 * a0, 3 * a1, lane);
+        v = StridedSum<64, 32>(f.rczcPart, 3 * a0, 3 * a1, lane);
     else if (wave == 3 && first)
         v = StridedSum<64>(f.bbPart, a0, a1, lane);
     for (int off = 32; off > 0; off >>= 1)
@@ -1089,12 +1099,17 @@ __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
     if (lane == 0)
         part[wave] = v;
     __syncthreads();
+    __shared__ int doneS;
     if (threadIdx.x == 0)
     {
         bool const live = FinishMesh(f.cg, m, part[1], part[0] + part[2], part[3], first);
         if (f.doneOut)
             f.doneOut[m] = live ? 0 : 1;
+        doneS = live ? 0 : 1;
     }
+    __syncthreads();
+    for (int g = a0 + static_cast<int>(threadIdx.x); g < a1; g += static_cast<int>(blockDim.x))
+        f.aggDone[g] = doneS;
 }
 
 // Galerkin coarse operator Z^T H Z from the assembled blocks: coarse block k
