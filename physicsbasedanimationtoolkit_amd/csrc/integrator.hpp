@@ -147,6 +147,7 @@ struct RolloutState
     std::vector<DeviceArray<float4>> segX;                    // minimisers of one recomputed segment
     DeviceArray<float4> lastX;                                // positions after the last step
     DeviceArray<float4> y, r, z, p, q, rhs, xbar, vbar, hold;
+    DeviceArray<float4> yS, bS; // the fused solvers' solution and right-hand side in row order
     // previous solutions of the backward pass (a ring, newest at histHead - 1),
     // the projection basis, its products with H, the per-aggregate partial sums
     // and the per-mesh coefficients of the projected initial guess
@@ -169,6 +170,11 @@ struct RolloutState
     // row-major assembly, per block its sources), the values, the longest row,
     // and the fused iteration's second direction buffer
     DeviceArray<int> bsrRowPtr, bsrCol, bsrBlockRow, bsrSrcPtr, bsrSrc;
+    // row order of the pattern (-1: none yet, 0: the vertices' own, 1: aggregate-
+    // major), the vertex of each row, and each row's mesh and aggregate, and each
+    // aggregate's rows (CSR by aggVertPtr)
+    int bsrOrder = -1;
+    DeviceArray<int> rowVertex, meshOfRow, aggOfRow, aggRows;
     DeviceArray<int> incPtr, incCode;
     DeviceArray<int4> incSlot;
     DeviceArray<int> bsrColAgg; // aggregate of each block's column

@@ -129,7 +129,7 @@ void EnsureBuffers(pbat_vbd_integrator& h)
 {
     RolloutState& R = h.rollout;
     size_t const n = static_cast<size_t>(h.nV);
-    for (auto* b : {&R.y, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
+    for (auto* b : {&R.y, &R.yS, &R.bS, &R.r, &R.z, &R.p, &R.p2, &R.q, &R.rhs, &R.xbar, &R.vbar, &R.hold})
         if (b->n != n)
             b->allocate(n);
     static_assert(device::kProjMax == sizeof(R.hist) / sizeof(R.hist[0]), "history ring size");
@@ -164,7 +164,7 @@ device::CoarseArgs MakeCoarseArgs(pbat_vbd_integrator& h)
     return c;
 }
 
-void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt);
+void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt, bool aggregateOrder);
 
 // The coarse operator Z^T H Z at xs (positions with masses): the step Hessian
 // assembled there, its blocks summed per pair of aggregates in a fixed order,
@@ -191,7 +191,7 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
                        static_cast<long long>(total));
     hipLaunchKernelGGL(device::ZeroDoublesLong, dim3(64), dim3(256), 0, h.stream, R.rc.ptr,
                        static_cast<long long>(R.rc.n));
-    AssembleBsr(h, xs, dt);
+    AssembleBsr(h, xs, dt, true); // the two-level solver is a fused one: aggregate-major rows
     int const nCB = static_cast<int>(R.cbA.n);
     if (nCB > 0)
         hipLaunchKernelGGL(device::CoarseFromBsr, dim3(Blocks(nCB)), dim3(256), 0, h.stream,
@@ -258,15 +258,21 @@ int BsrLanes(pbat_vbd_integrator const& h);
 // Longest block row the row-major assembly takes (4 blocks per lane of 32).
 constexpr int kRowAssembleMaxRow = 128;
 
-// Block pattern of the step Hessian over the updated vertices (row i lists the
-// updated vertices sharing a tet with i, itself included, ascending) and each
-// block's assembly sources, from the element table; built once per handle.
-void BuildBsrPattern(pbat_vbd_integrator& h)
+// Block pattern of the step Hessian over the updated vertices and each block's
+// assembly sources, from the element table; built once per handle and row
+// order. Row s stands for updated vertex rowVertex[s]: the vertices in their
+// own (colour-major) order for the Gauss-Seidel solvers, or aggregate-major for
+// the fused CG, so that an aggregate's rows are consecutive and its updates and
+// sums read consecutive entries. Row s lists the rows of the updated vertices
+// sharing a tet with rowVertex[s], itself included, ascending.
+void BuildBsrPattern(pbat_vbd_integrator& h, bool aggregateOrder)
 {
     RolloutState& R = h.rollout;
     int const nUpd = static_cast<int>(h.nUpdated);
-    if (R.bsrRowPtr.n == static_cast<size_t>(nUpd) + 1)
+    int const order = aggregateOrder ? 1 : 0;
+    if (R.bsrOrder == order && R.bsrRowPtr.n == static_cast<size_t>(nUpd) + 1)
         return;
+    R.ResetGraph(); // a captured CG chunk holds the previous pattern's buffers
     int64_t const nT = h.nT;
     if (nT >= (int64_t{1} << 27))
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "mesh too large for the adjoint's assembled Hessian (2^27 tets)");
@@ -292,65 +298,92 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
                 if (vid(e, l) < nUpd)
                     vinc[fill[vid(e, l)]++] = static_cast<int>(4 * e + l);
     }
-    // rows, and per incidence the row positions of its tet's vertices (-1: not
-    // updated) through a dense vertex -> position marker reset after each row
+    // mesh and aggregate of every updated vertex, and the aggregates' vertices
+    std::vector<int> meshOf(static_cast<size_t>(nUpd)), aggOf(static_cast<size_t>(nUpd));
+    std::vector<int> aggPtr(static_cast<size_t>(h.nAggregates) + 1, 0), aggVerts(static_cast<size_t>(nUpd));
+    if (nUpd > 0)
+    {
+        PBAT_HIP(hipMemcpy(meshOf.data(), h.meshOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
+        PBAT_HIP(hipMemcpy(aggOf.data(), h.aggOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
+    }
+    if (h.nAggregates > 0)
+    {
+        PBAT_HIP(hipMemcpy(aggPtr.data(), h.aggVertPtr.ptr, aggPtr.size() * sizeof(int), hipMemcpyDeviceToHost));
+        if (nUpd > 0)
+            PBAT_HIP(hipMemcpy(aggVerts.data(), h.aggVerts.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
+    }
+    // row order: rowVertex[s] = vertex of row s, rowOf its inverse
+    std::vector<int> rowVertex(static_cast<size_t>(nUpd)), rowOf(static_cast<size_t>(nUpd));
+    for (int i = 0; i < nUpd; ++i)
+        rowVertex[i] = aggregateOrder ? aggVerts[i] : i;
+    for (int s = 0; s < nUpd; ++s)
+        rowOf[rowVertex[s]] = s;
+    // rows, and per incidence of a row (in row order) the row positions of its
+    // tet's vertices (-1: not updated) through a dense row -> position marker
+    // reset after each row
     std::vector<int> rowPtr(static_cast<size_t>(nUpd) + 1, 0), col, blockRow, cols;
-    std::vector<int> slotOf(static_cast<size_t>(nUpd), -1);
+    std::vector<int> incPtr(static_cast<size_t>(nUpd) + 1, 0), incCode(vinc.size());
     std::vector<int4> incSlot(vinc.size());
+    std::vector<int> slotOf(static_cast<size_t>(nUpd), -1);
     col.reserve(16 * static_cast<size_t>(nUpd)), blockRow.reserve(16 * static_cast<size_t>(nUpd));
     int maxRow = 0;
-    for (int i = 0; i < nUpd; ++i)
+    for (int s = 0; s < nUpd; ++s)
     {
+        int const v = rowVertex[s];
         cols.clear();
-        for (int k = vp[i]; k < vp[i + 1]; ++k)
+        for (int k = vp[v]; k < vp[v + 1]; ++k)
             for (int l = 0; l < 4; ++l)
             {
                 int const j = vid(vinc[k] >> 2, l);
                 if (j < nUpd)
-                    cols.push_back(j);
+                    cols.push_back(rowOf[j]);
             }
         std::sort(cols.begin(), cols.end());
         cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
         for (size_t c = 0; c < cols.size(); ++c)
             slotOf[cols[c]] = static_cast<int>(c);
-        for (int k = vp[i]; k < vp[i + 1]; ++k)
+        int t = incPtr[s];
+        for (int k = vp[v]; k < vp[v + 1]; ++k, ++t)
         {
             int s4[4];
             for (int l = 0; l < 4; ++l)
             {
                 int const j = vid(vinc[k] >> 2, l);
-                s4[l] = j < nUpd ? slotOf[j] : -1;
+                s4[l] = j < nUpd ? slotOf[rowOf[j]] : -1;
             }
-            incSlot[k] = make_int4(s4[0], s4[1], s4[2], s4[3]);
+            incCode[t] = vinc[k];
+            incSlot[t] = make_int4(s4[0], s4[1], s4[2], s4[3]);
         }
+        incPtr[s + 1] = t;
         for (int j : cols)
             slotOf[j] = -1;
         col.insert(col.end(), cols.begin(), cols.end());
-        blockRow.insert(blockRow.end(), cols.size(), i);
-        rowPtr[i + 1] = rowPtr[i] + static_cast<int>(cols.size());
+        blockRow.insert(blockRow.end(), cols.size(), s);
+        rowPtr[s + 1] = rowPtr[s] + static_cast<int>(cols.size());
         maxRow = std::max(maxRow, static_cast<int>(cols.size()));
     }
     size_t const nBlocks = col.size();
     auto forEachSource = [&](auto&& f) {
-        for (int i = 0; i < nUpd; ++i)
-            for (int k = vp[i]; k < vp[i + 1]; ++k)
+        for (int s = 0; s < nUpd; ++s)
+            for (int t = incPtr[s]; t < incPtr[s + 1]; ++t)
             {
-                int const e = vinc[k] >> 2, la = vinc[k] & 3;
-                int const s4[4] = {incSlot[k].x, incSlot[k].y, incSlot[k].z, incSlot[k].w};
+                int const e = incCode[t] >> 2, la = incCode[t] & 3;
+                int const s4[4] = {incSlot[t].x, incSlot[t].y, incSlot[t].z, incSlot[t].w};
                 for (int lb = 0; lb < 4; ++lb)
                     if (s4[lb] >= 0)
-                        f(rowPtr[i] + s4[lb], 16 * e + 4 * la + lb);
+                        f(rowPtr[s] + s4[lb], 16 * e + 4 * la + lb);
             }
     };
     R.bsrRowPtr.upload(rowPtr, h.stream);
     R.bsrCol.upload(col, h.stream);
     R.bsrBlockRow.upload(blockRow, h.stream);
     R.bsrA.allocate(nBlocks), R.bsrB.allocate(nBlocks), R.bsrC.allocate(nBlocks);
+    R.rowVertex.upload(rowVertex, h.stream);
     if (maxRow <= kRowAssembleMaxRow)
     {
-        // row-major assembly: the incidences and their tets' row positions
-        R.incPtr.upload(vp, h.stream);
-        R.incCode.upload(vinc, h.stream);
+        // row-major assembly: the rows' incidences and their tets' row positions
+        R.incPtr.upload(incPtr, h.stream);
+        R.incCode.upload(incCode, h.stream);
         R.incSlot.upload(incSlot, h.stream);
     }
     else
@@ -368,20 +401,27 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
     }
     R.bsrMaxRow = maxRow;
 
-    // mesh and aggregate of every updated vertex
-    std::vector<int> meshOf(static_cast<size_t>(nUpd)), aggOf(static_cast<size_t>(nUpd));
-    if (nUpd > 0)
+    // mesh and aggregate of every row, each aggregate's rows
+    std::vector<int> meshOfRow(static_cast<size_t>(nUpd)), aggOfRow(static_cast<size_t>(nUpd));
+    std::vector<int> aggRows(static_cast<size_t>(nUpd));
+    for (int s = 0; s < nUpd; ++s)
     {
-        PBAT_HIP(hipMemcpy(meshOf.data(), h.meshOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
-        PBAT_HIP(hipMemcpy(aggOf.data(), h.aggOf.ptr, sizeof(int) * nUpd, hipMemcpyDeviceToHost));
+        meshOfRow[s] = meshOf[rowVertex[s]];
+        aggOfRow[s] = aggOf[rowVertex[s]];
     }
+    for (int k = 0; k < nUpd; ++k)
+        aggRows[k] = rowOf[aggVerts[k]]; // aggregate-major rows: k itself
+    R.meshOfRow.upload(meshOfRow, h.stream);
+    R.aggOfRow.upload(aggOfRow, h.stream);
+    R.aggRows.upload(aggRows, h.stream);
     // aggregate of each block's column (the direction product's coarse term)
     std::vector<int> colAgg(nBlocks);
     for (size_t k = 0; k < nBlocks; ++k)
-        colAgg[k] = aggOf[col[k]];
+        colAgg[k] = aggOfRow[col[k]];
     R.bsrColAgg.upload(colAgg, h.stream);
-    // row chunks: every (mesh, colour) run of rows cut into chunks of at most
-    // 256 / L rows, then ordered mesh-major (stable: colour, then row order)
+    // row chunks: every run of rows of one mesh (and, in the vertices' own order,
+    // one colour) cut into chunks of at most 256 / L rows, then ordered
+    // mesh-major (stable: colour, then row order)
     int const nM = h.nMeshes;
     int const chunkRows = 256 / BsrLanes(h);
     struct Chunk
@@ -389,18 +429,23 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
         int mesh, begin, end;
     };
     std::vector<Chunk> chunks;
-    int const nC = static_cast<int>(h.colorBegin.size()) - 1;
-    for (int c = 0; c < nC; ++c)
+    std::vector<int> bounds;
+    if (aggregateOrder)
+        bounds = {0, nUpd};
+    else
+        for (int b : h.colorBegin)
+            bounds.push_back(std::min(b, nUpd));
+    for (size_t c = 0; c + 1 < bounds.size(); ++c)
     {
-        int r = h.colorBegin[c];
-        int const re = std::min(h.colorBegin[c + 1], nUpd);
+        int r = bounds[c];
+        int const re = bounds[c + 1];
         while (r < re)
         {
             int runEnd = r + 1;
-            while (runEnd < re && meshOf[runEnd] == meshOf[r])
+            while (runEnd < re && meshOfRow[runEnd] == meshOfRow[r])
                 ++runEnd;
             for (int b = r; b < runEnd; b += chunkRows)
-                chunks.push_back({meshOf[r], b, std::min(b + chunkRows, runEnd)});
+                chunks.push_back({meshOfRow[r], b, std::min(b + chunkRows, runEnd)});
             r = runEnd;
         }
     }
@@ -420,19 +465,12 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
         for (int g = h.aggBegin[m]; g < h.aggBegin[m + 1]; ++g)
             aggChunk[g] = make_int2(meshChunk[m], meshChunk[m + 1]);
     R.aggChunks.upload(aggChunk, h.stream);
-    // each aggregate's first vertices, one per AggUpdate thread
+    // each aggregate's first rows, one per AggUpdate thread
     {
-        std::vector<int> ptr(static_cast<size_t>(h.nAggregates) + 1), verts(static_cast<size_t>(nUpd));
-        if (h.nAggregates > 0)
-        {
-            PBAT_HIP(hipMemcpy(ptr.data(), h.aggVertPtr.ptr, ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
-            if (nUpd > 0)
-                PBAT_HIP(hipMemcpy(verts.data(), h.aggVerts.ptr, verts.size() * sizeof(int), hipMemcpyDeviceToHost));
-        }
         std::vector<int> first(static_cast<size_t>(std::max(h.nAggregates, 1)) * device::kAggFirst, -1);
         for (int g = 0; g < h.nAggregates; ++g)
-            for (int k = ptr[g]; k < std::min(ptr[g + 1], ptr[g] + device::kAggFirst); ++k)
-                first[static_cast<size_t>(g) * device::kAggFirst + (k - ptr[g])] = verts[k];
+            for (int k = aggPtr[g]; k < std::min(aggPtr[g + 1], aggPtr[g] + device::kAggFirst); ++k)
+                first[static_cast<size_t>(g) * device::kAggFirst + (k - aggPtr[g])] = aggRows[k];
         R.aggFirst.upload(first, h.stream);
         R.aggDone.allocate(static_cast<size_t>(std::max(h.nAggregates, 1)));
     }
@@ -455,15 +493,15 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
             out[start[keyOf(k)]++] = k;
         return out;
     };
-    std::vector<int> order(nBlocks);
+    std::vector<int> blockOrder(nBlocks);
     for (size_t k = 0; k < nBlocks; ++k)
-        order[k] = static_cast<int>(k);
-    order = countingPass(order, [&](int k) { return aggOf[col[k]]; });
-    order = countingPass(order, [&](int k) { return aggOf[blockRow[k]]; });
+        blockOrder[k] = static_cast<int>(k);
+    blockOrder = countingPass(blockOrder, [&](int k) { return aggOfRow[col[k]]; });
+    blockOrder = countingPass(blockOrder, [&](int k) { return aggOfRow[blockRow[k]]; });
     std::vector<int> cbA, cbB, cbPtr{0}, cbSrc(nBlocks);
     for (size_t t = 0; t < nBlocks; ++t)
     {
-        int const k = order[t], a = aggOf[blockRow[k]], b = aggOf[col[k]];
+        int const k = blockOrder[t], a = aggOfRow[blockRow[k]], b = aggOfRow[col[k]];
         if (t == 0 || a != cbA.back() || b != cbB.back())
         {
             if (t > 0)
@@ -480,13 +518,14 @@ void BuildBsrPattern(pbat_vbd_integrator& h)
     R.cbSrcPtr.upload(cbPtr, h.stream);
     R.cbSrc.upload(cbSrc, h.stream);
     PBAT_HIP(hipStreamSynchronize(h.stream));
+    R.bsrOrder = order;
 }
 
 // H = M / h^2 + sum_e w_e d2Psi_e/dx2 at xs into the block values, and the
 // inverse diagonal blocks into dinv.
-void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt)
+void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt, bool aggregateOrder)
 {
-    BuildBsrPattern(h);
+    BuildBsrPattern(h, aggregateOrder);
     RolloutState& R = h.rollout;
     device::BsrPattern p{};
     p.rowPtr = R.bsrRowPtr.ptr, p.col = R.bsrCol.ptr, p.blockRow = R.bsrBlockRow.ptr;
@@ -500,14 +539,14 @@ void AssembleBsr(pbat_vbd_integrator& h, float4 const* xs, float dt)
     {
         hipLaunchKernelGGL(device::BsrAssemble, dim3(Blocks(p.nBlocks)), dim3(256), 0, h.stream, xs,
                            static_cast<int4 const*>(h.tetv.ptr), static_cast<float4 const*>(h.tgp.ptr), p, invDt2,
-                           v, R.dinv.ptr);
+                           v, R.dinv.ptr, static_cast<int const*>(R.rowVertex.ptr));
         return;
     }
     device::RowAssembleArgs a{};
     a.xs = xs, a.tetv = h.tetv.ptr, a.tgp = h.tgp.ptr;
     a.incPtr = R.incPtr.ptr, a.incCode = R.incCode.ptr, a.incSlot = R.incSlot.ptr;
     a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr, a.nRows = p.nRows, a.invDt2 = invDt2, a.v = v;
-    a.dinv = R.dinv.ptr;
+    a.dinv = R.dinv.ptr, a.rowVertex = R.rowVertex.ptr;
     dim3 const grid((p.nRows + 1) / 2);
     int const nb = (R.bsrMaxRow + 31) / 32;
     if (nb <= 1)
@@ -577,7 +616,7 @@ void BsrPass(pbat_vbd_integrator& h, device::BsrArgs a, int vbeg, int vend)
     RolloutState& R = h.rollout;
     a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
     a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
-    a.meshOf = h.meshOf.ptr, a.aggOf = h.aggOf.ptr, a.dinv = R.dinv.ptr;
+    a.meshOf = R.meshOfRow.ptr, a.aggOf = R.aggOfRow.ptr, a.dinv = R.dinv.ptr;
     a.vbeg = vbeg, a.vend = vend;
     switch (BsrLanes(h))
     {
@@ -630,7 +669,7 @@ void DirectionPass(pbat_vbd_integrator& h, device::BsrArgs a)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "row chunks built for another lane count");
     a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
     a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
-    a.meshOf = h.meshOf.ptr, a.aggOf = h.aggOf.ptr, a.colAgg = R.bsrColAgg.ptr;
+    a.meshOf = R.meshOfRow.ptr, a.aggOf = R.aggOfRow.ptr, a.colAgg = R.bsrColAgg.ptr;
     switch (BsrLanes(h))
     {
         case 1: LaunchDirectionL<1>(h, a); break;
@@ -781,8 +820,8 @@ void ProjectInitialGuess(pbat_vbd_integrator& h, float4* y, float4 const* b, int
         a.hb[j] = R.projHb[j].ptr;
     }
     a.rhs = b, a.y = y;
-    a.aggPtr = h.aggVertPtr.ptr, a.aggVerts = h.aggVerts.ptr, a.meshAggBegin = h.meshAggBegin.ptr;
-    a.meshOf = h.meshOf.ptr;
+    a.aggPtr = h.aggVertPtr.ptr, a.aggVerts = R.aggRows.ptr, a.meshAggBegin = h.meshAggBegin.ptr;
+    a.meshOf = R.meshOfRow.ptr;
     a.part = R.projPart.ptr, a.coef = R.projCoef.ptr;
     a.k = nHist, a.n = n, a.nM = nM, a.nAgg = h.nAggregates;
     hipLaunchKernelGGL(device::ProjBasis, dim3(Blocks(n)), dim3(256), 0, h.stream, a);
@@ -813,7 +852,20 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     bool const fused = cfg.solver == PBAT_VBD_ADJOINT_PCG_JACOBI || cfg.solver == PBAT_VBD_ADJOINT_PCG_TWO_LEVEL;
     // fused chunks hold an even number of iterations: the two direction buffers swap
     int const check = fused ? (std::max(1, cfg.check_every) + 1) / 2 * 2 : std::max(1, cfg.check_every);
-    AssembleBsr(h, xs, dt);
+    AssembleBsr(h, xs, dt, fused);
+    // the fused solvers work on aggregate-major rows: y and b gathered into that
+    // order here, y scattered back at the end
+    float4* const yIn = y;
+    bool const fixedBuffers = xs == R.hold.ptr && y == R.y.ptr && b == R.rhs.ptr;
+    if (fused && n > 0)
+    {
+        hipLaunchKernelGGL(device::GatherRows, dim3(gridN), dim3(256), 0, h.stream, R.yS.ptr,
+                           static_cast<float4 const*>(y), static_cast<int const*>(R.rowVertex.ptr), n);
+        hipLaunchKernelGGL(device::GatherRows, dim3(gridN), dim3(256), 0, h.stream, R.bS.ptr, b,
+                           static_cast<int const*>(R.rowVertex.ptr), n);
+        y = R.yS.ptr;
+        b = R.bS.ptr;
+    }
     if (fused && nHist > 0 && n > 0 && h.nAggregates > 0)
         ProjectInitialGuess(h, y, b, nHist);
     auto zeroScalars = [&] {
@@ -863,7 +915,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     int const coarseWavesPerRow = maxCoarse > 512 ? 4 : maxCoarse > 256 ? 2 : 1;
     auto aggUpdate = [&](int init, float4 const* p) {
         device::AggUpdateArgs u{};
-        u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = h.aggVerts.ptr, u.aggMesh = h.aggMesh.ptr;
+        u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = R.aggRows.ptr, u.aggMesh = h.aggMesh.ptr;
         u.y = y, u.r = R.r.ptr, u.z = R.z.ptr, u.p = p, u.q = R.q.ptr, u.b = b, u.dinv = R.dinv.ptr;
         u.rc = twoLevel ? R.rc.ptr : nullptr;
         u.pqPart = R.pqPart.ptr, u.aggChunks = R.aggChunks.ptr;
@@ -934,7 +986,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
                                static_cast<float4 const*>(R.z.ptr), h.meshOf.ptr, n, cg);
         };
     }
-    bool const graph = h.useGraph && xs == R.hold.ptr && y == R.y.ptr && b == R.rhs.ptr;
+    bool const graph = h.useGraph && fixedBuffers;
     if (graph)
     {
         std::vector<double> const key{double(dt),        double(cfg.solver), double(check),       cg.tol2,
@@ -949,7 +1001,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
             h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.aggChunks.ptr,
             R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
-            R.bsrColAgg.ptr,   R.doneDev,       R.aggFirst.ptr, R.aggDone.ptr};
+            R.bsrColAgg.ptr,   R.doneDev,       R.aggFirst.ptr, R.aggDone.ptr,    R.yS.ptr,        R.bS.ptr,
+            R.aggRows.ptr,     R.aggOfRow.ptr,  R.meshOfRow.ptr};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();
@@ -998,6 +1051,9 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
                 break;
         }
     }
+    if (fused && n > 0)
+        hipLaunchKernelGGL(device::ScatterRows, dim3(gridN), dim3(256), 0, h.stream, yIn,
+                           static_cast<float4 const*>(y), static_cast<int const*>(R.rowVertex.ptr), n);
     PBAT_HIP(hipGetLastError());
     if (statsOut)
     {
@@ -1160,7 +1216,8 @@ void RolloutBackward(pbat_vbd_integrator& h, float const* dLdx, float const* dLd
             if (cfg.history > 0)
             {
                 hipLaunchKernelGGL(device::Copy4, dim3(Blocks(nV)), dim3(256), 0, h.stream,
-                                   R.hist[R.histHead].ptr, static_cast<float4 const*>(R.y.ptr), nV);
+                                   R.hist[R.histHead].ptr, static_cast<float4 const*>(fused ? R.yS.ptr : R.y.ptr),
+                                   nV);
                 R.histHead = (R.histHead + 1) % device::kProjMax;
                 R.histCount = std::min(R.histCount + 1, device::kProjMax);
             }

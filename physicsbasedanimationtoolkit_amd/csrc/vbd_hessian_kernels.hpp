@@ -53,7 +53,8 @@ struct BsrPattern
 // (00, 01, 02, 11), (12, 22).
 __global__ __launch_bounds__(256) void BsrAssemble(float4 const* __restrict__ xs, int4 const* __restrict__ tetv,
                                                    float4 const* __restrict__ tgp, BsrPattern p, double invDt2,
-                                                   BsrValues v, float4* __restrict__ dinv)
+                                                   BsrValues v, float4* __restrict__ dinv,
+                                                   int const* __restrict__ rowVertex)
 {
     int const k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (k >= p.nBlocks)
@@ -107,7 +108,7 @@ __global__ __launch_bounds__(256) void BsrAssemble(float4 const* __restrict__ xs
     }
     if (i == j)
     {
-        double const m = double(xs[i].w) * invDt2;
+        double const m = double(xs[rowVertex[i]].w) * invDt2;
         H[0][0] += m, H[1][1] += m, H[2][2] += m;
         Inverse3 const C(H[0][0], 0.5 * (H[0][1] + H[1][0]), 0.5 * (H[0][2] + H[2][0]), H[1][1],
                          0.5 * (H[1][2] + H[2][1]), H[2][2]);
@@ -208,6 +209,7 @@ struct RowAssembleArgs
     double invDt2;
     BsrValues v;
     float4* __restrict__ dinv;
+    int const* __restrict__ rowVertex; // vertex of each row
 };
 
 // 32 lanes per row, two rows per wave, one wave per workgroup. The row's
@@ -340,7 +342,7 @@ __global__ __launch_bounds__(64) void RowAssemble(RowAssembleArgs a)
         double* H = acc[o];
         if (b == diagSlot)
         {
-            double const m = double(a.xs[i].w) * a.invDt2;
+            double const m = double(a.xs[a.rowVertex[i]].w) * a.invDt2;
 #pragma unroll
             for (int c = 0; c < 9; ++c)
                 H[c] = diag[c];
@@ -826,6 +828,23 @@ __global__ __launch_bounds__(64) void SolveStats(CgState cg, double* __restrict_
         out[0] = it;
         out[1] = rel;
     }
+}
+
+// Vectors between the vertices' order and the solver's row order.
+__global__ __launch_bounds__(256) void GatherRows(float4* __restrict__ out, float4 const* __restrict__ in,
+                                                  int const* __restrict__ rowVertex, int n)
+{
+    int const s = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s < n)
+        out[s] = in[rowVertex[s]];
+}
+
+__global__ __launch_bounds__(256) void ScatterRows(float4* __restrict__ out, float4 const* __restrict__ in,
+                                                   int const* __restrict__ rowVertex, int n)
+{
+    int const s = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s < n)
+        out[rowVertex[s]] = in[s];
 }
 
 // ---- initial guess of a backward solve from the previous solutions ----
