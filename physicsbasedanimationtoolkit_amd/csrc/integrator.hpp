@@ -301,6 +301,10 @@ struct pbat_vbd_integrator
     float muC = 1e6f, muF = 0.3f, epsv = 1e-3f;
     pbat_vbd::DeviceArray<int> cvIdx, cvBody, fc, cvActive, cvInAv;
     pbat_vbd::DeviceArray<int4> tri;
+    // the triangles in the detection kernels' storage order (grouped by body,
+    // then along a Morton curve of their rest centroids) and each one's index
+    pbat_vbd::DeviceArray<int4> triS;
+    pbat_vbd::DeviceArray<int> triId;
     pbat_vbd::DeviceArray<float> triArea, xva, dupper;
     pbat_vbd::DeviceArray<float4> triLo, triHi, chunkLo, chunkHi, superLo, superHi, xfull, xb;
     pbat_vbd::DeviceArray<float4> triA, triB, triC; // triangle positions of the nearest pass
@@ -362,7 +366,7 @@ struct pbat_vbd_integrator
         gkm1.reset(), fkm1.reset(), dgk.reset(), dfk.reset(), andersonAcc.reset(), andersonGram.reset();
         trPartials.reset(), trState.reset(), trLogT.reset(), trLogAccepted.reset();
         cvIdx.reset(), cvBody.reset(), fc.reset(), cvActive.reset(), cvInAv.reset();
-        tri.reset(), triArea.reset(), xva.reset(), dupper.reset();
+        tri.reset(), triS.reset(), triId.reset(), triArea.reset(), xva.reset(), dupper.reset();
         triLo.reset(), triHi.reset(), chunkLo.reset(), chunkHi.reset(), superLo.reset(), superHi.reset();
         xfull.reset(), xb.reset();
         triA.reset(), triB.reset(), triC.reset(), crec.reset();

@@ -7,10 +7,12 @@
 // Detection is an all-pairs search over chunks of 64 consecutive triangles,
 // each with a bounding box: a wave takes kPerWave collision vertices, skips the
 // chunks whose box lies beyond all of their query bounds, and for the others
-// gives one triangle to each lane. It is exhaustive up to those exact bounds,
-// which makes its answers independent of any traversal order: a vertex's search
-// radius is the largest over its overlapping triangles, and its nearest
-// triangles come in triangle order.
+// gives one triangle to each lane. The triangles are stored grouped by body and
+// along a Morton curve of their rest centroids, so a chunk's box is tight. The
+// search is exhaustive up to those exact bounds, which makes its answers
+// independent of the storage and traversal order: a vertex's search radius is
+// the largest over its overlapping triangles, and its nearest triangles are kept
+// as the kMaxContacts smallest triangle indices, ascending (triangle order).
 
 #include "pbat_vbd.h"
 #include "status.hpp"
@@ -18,6 +20,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
 #include <limits>
 #include <string>
 #include <vector>
@@ -40,6 +45,8 @@ struct ContactArgs
     int const* __restrict__ cvIdx;     // internal vertex of each collision vertex
     int const* __restrict__ cvBody;    // body of each collision vertex
     int4 const* __restrict__ tri;      // internal vertices of each triangle, body of its first vertex in .w
+    int4 const* __restrict__ triS;     // the same in storage order
+    int const* __restrict__ triId;     // triangle index of each stored triangle
     float const* __restrict__ triArea; // triangle areas
     float const* __restrict__ xva;     // vertex areas (internal order)
     int* __restrict__ fc;              // kMaxContacts contacting triangles per internal vertex, -1 padded
@@ -81,7 +88,7 @@ constexpr int kChunksPerRound = 4; // chunks whose triangles a nearest-pass roun
     int const f = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (f >= c.nF)
         return;
-    int4 const t = c.tri[f];
+    int4 const t = c.triS[f];
     float4 const p[6] = {x0[t.x], x0[t.y], x0[t.z], x1[t.x], x1[t.y], x1[t.z]};
     float lx = p[0].x, ly = p[0].y, lz = p[0].z, hx = lx, hy = ly, hz = lz;
     for (int k = 1; k < 6; ++k)
@@ -93,7 +100,7 @@ constexpr int kChunksPerRound = 4; // chunks whose triangles a nearest-pass roun
     c.triHi[f] = make_float4(hx, hy, hz, 0.f);
 }
 
-// Box of each chunk of kChunk consecutive triangles: of their swept boxes
+// Box of each chunk of kChunk consecutive stored triangles: of their swept boxes
 // (swept != 0) or of their vertices at x, whose positions per triangle it
 // also stores for the nearest pass; and the body of the chunk's triangles when
 // they all belong to one. One wave per chunk.
@@ -106,7 +113,7 @@ constexpr int kChunksPerRound = 4; // chunks whose triangles a nearest-pass roun
     int blo = 0x7fffffff, bhi = static_cast<int>(0x80000000u);
     if (f < c.nF)
     {
-        blo = bhi = c.tri[f].w;
+        blo = bhi = c.triS[f].w;
         if (swept)
         {
             float4 const L = c.triLo[f], U = c.triHi[f];
@@ -114,7 +121,7 @@ constexpr int kChunksPerRound = 4; // chunks whose triangles a nearest-pass roun
         }
         else
         {
-            int4 const t = c.tri[f];
+            int4 const t = c.triS[f];
             float4 const a = x[t.x], b = x[t.y], cc = x[t.z];
             lx = fminf(a.x, fminf(b.x, cc.x)), ly = fminf(a.y, fminf(b.y, cc.y)), lz = fminf(a.z, fminf(b.z, cc.z));
             hx = fmaxf(a.x, fmaxf(b.x, cc.x)), hy = fmaxf(a.y, fmaxf(b.y, cc.y)), hz = fmaxf(a.z, fmaxf(b.z, cc.z));
@@ -373,14 +380,15 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
 // uses it): squared distances to the triangles of other bodies, d* the
 // smallest, r the vertex's search radius; with d* < r - eps the triangles
 // within d* + eps, with d* within r +- eps those within r + eps, none
-// otherwise; in triangle order, at most kMaxContacts. finalize = 0
+// otherwise; the kMaxContacts with the smallest triangle indices, ascending
+// (triangle order), whatever the storage order scanned. finalize = 0
 // (UpdateActiveSet, gpu/impl/vbd/Integrator.cu:250-271): they become the
 // vertex's contacts. finalize = 1 (FinalizeActiveSet): the vertex stays active
 // when it lies below the plane of the last of them. kPerWave collision vertices
-// per wave; two passes over the triangle chunks, the second collecting in
-// triangle order with wave ballots. A round loads the triangles of up to
-// kChunksPerRound chunks at once (positions stored by ContactChunkBoxes) and
-// then treats the chunks in order, so the collection order is unchanged.
+// per wave; two passes over the triangle chunks, the second collecting with
+// wave ballots into a sorted list held by the first kMaxContacts lanes. A round
+// loads the triangles of up to kChunksPerRound chunks at once (positions stored
+// by ContactChunkBoxes).
 [[maybe_unused]] static __global__ __launch_bounds__(kContactBlock) void ContactNearest(
     ContactArgs c, float4 const* __restrict__ x, int finalize)
 {
@@ -391,6 +399,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
     bool mine[kPerWave];
     float r[kPerWave], dmin[kPerWave], thr[kPerWave];
     int count[kPerWave], last[kPerWave];
+    int kept[kPerWave]; // lane s < kMaxContacts: the s-th smallest triangle index collected
     bool anyWork = false;
 #pragma unroll
     for (int j = 0; j < kPerWave; ++j)
@@ -410,6 +419,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
         thr[j] = -1.f;
         count[j] = 0;
         last[j] = -1;
+        kept[j] = 0x7fffffff;
         anyWork = anyWork || mine[j];
     }
     if (!anyWork)
@@ -426,7 +436,8 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
         int const f0 = c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts];
         if (f0 >= 0)
         {
-            float const d0 = PointTriangleSqDist(p[j], c.triA[f0], c.triB[f0], c.triC[f0]);
+            int4 const t0 = c.tri[f0];
+            float const d0 = PointTriangleSqDist(p[j], x[t0.x], x[t0.y], x[t0.z]);
             if (d0 <= (r[j] + kFltEpsilon) * 0.999f)
                 dmin[j] = d0;
         }
@@ -536,7 +547,7 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
                     bool need[kPerWave];
 #pragma unroll
                     for (int j = 0; j < kPerWave; ++j)
-                        need[j] = ((needBits >> j) & 1) && (pass == 0 || count[j] < kMaxContacts);
+                        need[j] = (needBits >> j) & 1;
                     int const f = (cb + ks[i]) * kChunk + lane;
                     bool const valid = f < c.nF;
                     int const tb = __float_as_int(A[i].w);
@@ -553,13 +564,21 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
                             continue;
                         }
                         bool const take = other && PointTriangleSqDist(p[j], A[i], Bv[i], Cv[i]) <= thr[j];
-                        unsigned long long const mask = __ballot(take);
-                        int const slot = count[j] + __popcll(mask & ((1ull << lane) - 1ull));
-                        bool const kept = take && slot < kMaxContacts;
-                        if (kept && !finalize)
-                            c.fc[static_cast<size_t>(c.cvIdx[v0 + j]) * kMaxContacts + slot] = f;
-                        last[j] = max(last[j], WaveMaxInt(kept ? f : -1));
+                        unsigned long long mask = __ballot(take);
                         count[j] += __popcll(mask);
+                        int const id = take ? c.triId[f] : 0;
+                        // insert each taken index into the ascending list of the
+                        // kMaxContacts smallest (lanes 0 .. kMaxContacts - 1)
+                        while (mask)
+                        {
+                            int const l = __ffsll(static_cast<long long>(mask)) - 1;
+                            mask &= mask - 1ull;
+                            int const t = __shfl(id, l);
+                            int const pos = __popcll(__ballot(lane < kMaxContacts && kept[j] < t));
+                            int const up = __shfl_up(kept[j], 1);
+                            if (lane < kMaxContacts && pos < kMaxContacts)
+                                kept[j] = lane < pos ? kept[j] : (lane == pos ? t : up);
+                        }
                     }
                 }
                 if (pass == 0 && todo)
@@ -588,10 +607,11 @@ __device__ inline float PointTriangleSqDist(float4 p, float4 a, float4 b, float4
             continue;
         int const v = v0 + j;
         int const n = min(count[j], kMaxContacts);
+        last[j] = n > 0 ? __shfl(kept[j], n - 1) : -1;
         if (!finalize)
         {
-            if (lane >= n && lane < kMaxContacts)
-                c.fc[static_cast<size_t>(c.cvIdx[v]) * kMaxContacts + lane] = -1;
+            if (lane < kMaxContacts)
+                c.fc[static_cast<size_t>(c.cvIdx[v]) * kMaxContacts + lane] = lane < n ? kept[j] : -1;
             continue;
         }
         if (lane == 0 && last[j] >= 0)
@@ -715,9 +735,52 @@ void SetUpContact(pbat_vbd_integrator& h, pbat_vbd_data const* d)
     }
     for (int64_t i = 0; i < nV; ++i)
         xva[h.iperm[i]] = static_cast<float>(d->XVA[i]);
+    // storage order of the detection kernels: by body, then along a Morton curve
+    // of the rest centroids over the collision mesh's box (10 bits per axis)
+    std::vector<int> order(static_cast<size_t>(nF));
+    {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t k = 0; k < 3 * nF; ++k)
+            for (int a = 0; a < 3; ++a)
+            {
+                double const x = d->x[3 * d->F[k] + a];
+                lo[a] = std::min(lo[a], x), hi[a] = std::max(hi[a], x);
+            }
+        auto spread = [](uint32_t v) { // 10 bits -> every third of 30 bits
+            v &= 0x3ffu;
+            v = (v | (v << 16)) & 0x030000ffu;
+            v = (v | (v << 8)) & 0x0300f00fu;
+            v = (v | (v << 4)) & 0x030c30c3u;
+            v = (v | (v << 2)) & 0x09249249u;
+            return v;
+        };
+        std::vector<uint32_t> code(static_cast<size_t>(nF));
+        for (int64_t f = 0; f < nF; ++f)
+        {
+            uint32_t key = 0;
+            for (int a = 0; a < 3; ++a)
+            {
+                double const cen = (d->x[3 * d->F[3 * f] + a] + d->x[3 * d->F[3 * f + 1] + a] +
+                                    d->x[3 * d->F[3 * f + 2] + a]) / 3.;
+                double const ext = hi[a] - lo[a];
+                double const t = ext > 0. ? (cen - lo[a]) / ext : 0.;
+                key |= spread(static_cast<uint32_t>(std::min(1023., std::max(0., std::floor(t * 1024.))))) << a;
+            }
+            code[f] = key;
+            order[f] = static_cast<int>(f);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return tri[a].w != tri[b].w ? tri[a].w < tri[b].w : code[a] < code[b];
+        });
+    }
+    std::vector<int4> triS(static_cast<size_t>(nF));
+    for (int64_t s = 0; s < nF; ++s)
+        triS[s] = tri[order[s]];
     h.cvIdx.upload(cvIdx, h.stream);
     h.cvBody.upload(cvBody, h.stream);
     h.tri.upload(tri, h.stream);
+    h.triS.upload(triS, h.stream);
+    h.triId.upload(order, h.stream);
     h.triArea.upload(area, h.stream);
     h.xva.upload(xva, h.stream);
     // no contacts, nothing active, search radii at FLT_MAX
@@ -753,6 +816,7 @@ device::ContactArgs MakeContactArgs(pbat_vbd_integrator& h)
 {
     device::ContactArgs c{};
     c.cvIdx = h.cvIdx.ptr, c.cvBody = h.cvBody.ptr, c.tri = h.tri.ptr, c.triArea = h.triArea.ptr;
+    c.triS = h.triS.ptr, c.triId = h.triId.ptr;
     c.xva = h.xva.ptr, c.fc = h.fc.ptr, c.active = h.cvActive.ptr, c.inAv = h.cvInAv.ptr;
     c.dupper = h.dupper.ptr, c.triLo = h.triLo.ptr, c.triHi = h.triHi.ptr;
     c.chunkLo = h.chunkLo.ptr, c.chunkHi = h.chunkHi.ptr, c.superLo = h.superLo.ptr, c.superHi = h.superHi.ptr;
