@@ -20,8 +20,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace pbat_vbd {
@@ -258,6 +260,57 @@ int BsrLanes(pbat_vbd_integrator const& h);
 // Longest block row the row-major assembly takes (4 blocks per lane of 32).
 constexpr int kRowAssembleMaxRow = 128;
 
+// Host threads for the pattern build over n independent items: up to 16, one
+// per 8,192 items.
+int HostThreads(int n)
+{
+    int const hw = static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+    return std::max(1, std::min({16, hw, n / 8192}));
+}
+
+// f(begin, end, part) over [0, n) cut into nParts contiguous ranges, the same
+// ranges for the same (n, nParts), run on nParts host threads; the first
+// exception a part throws is rethrown after every thread has joined.
+template <class F>
+void HostParallel(int n, int nParts, F&& f)
+{
+    auto range = [&](int t) { return static_cast<int>(int64_t(n) * t / nParts); };
+    if (nParts <= 1)
+    {
+        f(0, n, 0);
+        return;
+    }
+    std::vector<std::exception_ptr> err(static_cast<size_t>(nParts));
+    std::vector<std::thread> pool;
+    pool.reserve(static_cast<size_t>(nParts));
+    auto run = [&](int t) {
+        try
+        {
+            f(range(t), range(t + 1), t);
+        }
+        catch (...)
+        {
+            err[t] = std::current_exception();
+        }
+    };
+    try
+    {
+        for (int t = 0; t < nParts; ++t)
+            pool.emplace_back(run, t);
+    }
+    catch (...)
+    {
+        for (auto& th : pool)
+            th.join();
+        throw;
+    }
+    for (auto& th : pool)
+        th.join();
+    for (auto& e : err)
+        if (e)
+            std::rethrow_exception(e);
+}
+
 // Block pattern of the step Hessian over the updated vertices and each block's
 // assembly sources, from the element table; built once per handle and row
 // order. Row s stands for updated vertex rowVertex[s]: the vertices in their
@@ -319,49 +372,67 @@ void BuildBsrPattern(pbat_vbd_integrator& h, bool aggregateOrder)
     for (int s = 0; s < nUpd; ++s)
         rowOf[rowVertex[s]] = s;
     // rows, and per incidence of a row (in row order) the row positions of its
-    // tet's vertices (-1: not updated) through a dense row -> position marker
-    // reset after each row
-    std::vector<int> rowPtr(static_cast<size_t>(nUpd) + 1, 0), col, blockRow, cols;
-    std::vector<int> incPtr(static_cast<size_t>(nUpd) + 1, 0), incCode(vinc.size());
+    // tet's vertices (-1: not updated). Rows are independent: each range of rows
+    // is built on its own thread with its own dense row -> position marker
+    // (reset after each row), then the ranges' columns are concatenated.
+    std::vector<int> rowPtr(static_cast<size_t>(nUpd) + 1, 0), incPtr(static_cast<size_t>(nUpd) + 1, 0);
+    std::vector<int> incCode(vinc.size());
     std::vector<int4> incSlot(vinc.size());
-    std::vector<int> slotOf(static_cast<size_t>(nUpd), -1);
-    col.reserve(16 * static_cast<size_t>(nUpd)), blockRow.reserve(16 * static_cast<size_t>(nUpd));
-    int maxRow = 0;
     for (int s = 0; s < nUpd; ++s)
-    {
-        int const v = rowVertex[s];
-        cols.clear();
-        for (int k = vp[v]; k < vp[v + 1]; ++k)
-            for (int l = 0; l < 4; ++l)
-            {
-                int const j = vid(vinc[k] >> 2, l);
-                if (j < nUpd)
-                    cols.push_back(rowOf[j]);
-            }
-        std::sort(cols.begin(), cols.end());
-        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-        for (size_t c = 0; c < cols.size(); ++c)
-            slotOf[cols[c]] = static_cast<int>(c);
-        int t = incPtr[s];
-        for (int k = vp[v]; k < vp[v + 1]; ++k, ++t)
+        incPtr[s + 1] = incPtr[s] + vp[rowVertex[s] + 1] - vp[rowVertex[s]];
+    int const nParts = HostThreads(nUpd);
+    std::vector<std::vector<int>> colPart(static_cast<size_t>(nParts));
+    std::vector<int> maxPart(static_cast<size_t>(nParts), 0);
+    HostParallel(nUpd, nParts, [&](int r0, int r1, int part) {
+        std::vector<int> slotOf(static_cast<size_t>(nUpd), -1), cols;
+        std::vector<int>& out = colPart[part];
+        out.reserve(16 * static_cast<size_t>(r1 - r0));
+        for (int s = r0; s < r1; ++s)
         {
-            int s4[4];
-            for (int l = 0; l < 4; ++l)
+            int const v = rowVertex[s];
+            // the row's distinct columns (slotOf marks those seen), then sorted
+            cols.clear();
+            for (int k = vp[v]; k < vp[v + 1]; ++k)
+                for (int l = 0; l < 4; ++l)
+                {
+                    int const j = vid(vinc[k] >> 2, l);
+                    if (j < nUpd && slotOf[rowOf[j]] < 0)
+                    {
+                        slotOf[rowOf[j]] = 0;
+                        cols.push_back(rowOf[j]);
+                    }
+                }
+            std::sort(cols.begin(), cols.end());
+            for (size_t c = 0; c < cols.size(); ++c)
+                slotOf[cols[c]] = static_cast<int>(c);
+            for (int k = vp[v], t = incPtr[s]; k < vp[v + 1]; ++k, ++t)
             {
-                int const j = vid(vinc[k] >> 2, l);
-                s4[l] = j < nUpd ? slotOf[rowOf[j]] : -1;
+                int s4[4];
+                for (int l = 0; l < 4; ++l)
+                {
+                    int const j = vid(vinc[k] >> 2, l);
+                    s4[l] = j < nUpd ? slotOf[rowOf[j]] : -1;
+                }
+                incCode[t] = vinc[k];
+                incSlot[t] = make_int4(s4[0], s4[1], s4[2], s4[3]);
             }
-            incCode[t] = vinc[k];
-            incSlot[t] = make_int4(s4[0], s4[1], s4[2], s4[3]);
+            for (int j : cols)
+                slotOf[j] = -1;
+            out.insert(out.end(), cols.begin(), cols.end());
+            rowPtr[s + 1] = static_cast<int>(cols.size()); // the row's length until the scan below
+            maxPart[part] = std::max(maxPart[part], static_cast<int>(cols.size()));
         }
-        incPtr[s + 1] = t;
-        for (int j : cols)
-            slotOf[j] = -1;
-        col.insert(col.end(), cols.begin(), cols.end());
-        blockRow.insert(blockRow.end(), cols.size(), s);
-        rowPtr[s + 1] = rowPtr[s] + static_cast<int>(cols.size());
-        maxRow = std::max(maxRow, static_cast<int>(cols.size()));
-    }
+    });
+    for (int s = 0; s < nUpd; ++s)
+        rowPtr[s + 1] += rowPtr[s];
+    int const maxRow = *std::max_element(maxPart.begin(), maxPart.end());
+    std::vector<int> col(static_cast<size_t>(rowPtr[nUpd])), blockRow(col.size());
+    HostParallel(nUpd, nParts, [&](int r0, int r1, int part) {
+        std::copy(colPart[part].begin(), colPart[part].end(), col.begin() + rowPtr[r0]);
+        for (int s = r0; s < r1; ++s)
+            std::fill(blockRow.begin() + rowPtr[s], blockRow.begin() + rowPtr[s + 1], s);
+    });
+    colPart.clear();
     size_t const nBlocks = col.size();
     auto forEachSource = [&](auto&& f) {
         for (int s = 0; s < nUpd; ++s)
@@ -476,43 +547,88 @@ void BuildBsrPattern(pbat_vbd_integrator& h, bool aggregateOrder)
     }
     R.bsrChunkRows = chunkRows;
     R.pqPart.allocate(std::max<size_t>(chunks.size(), 1));
-    size_t const nAgg = static_cast<size_t>(std::max(h.nAggregates, 1));
-    R.rzPart.allocate(nAgg), R.rrPart.allocate(nAgg), R.bbPart.allocate(nAgg), R.rczcPart.allocate(3 * nAgg);
+    size_t const nAggAlloc = static_cast<size_t>(std::max(h.nAggregates, 1));
+    R.rzPart.allocate(nAggAlloc), R.rrPart.allocate(nAggAlloc), R.bbPart.allocate(nAggAlloc);
+    R.rczcPart.allocate(3 * nAggAlloc);
 
     // coarse blocks: the fine blocks grouped by (aggregate of the row, aggregate
-    // of the column), ascending within each group
-    // (two stable counting passes: by column aggregate, then by row aggregate)
-    int const nAggAll = std::max(h.nAggregates, 1);
-    auto countingPass = [&](std::vector<int> const& in, auto&& keyOf) {
-        std::vector<int> start(static_cast<size_t>(nAggAll) + 1, 0), out(in.size());
-        for (int k : in)
-            ++start[keyOf(k) + 1];
-        for (int a = 0; a < nAggAll; ++a)
-            start[a + 1] += start[a];
-        for (int k : in)
-            out[start[keyOf(k)]++] = k;
-        return out;
-    };
-    std::vector<int> blockOrder(nBlocks);
-    for (size_t k = 0; k < nBlocks; ++k)
-        blockOrder[k] = static_cast<int>(k);
-    blockOrder = countingPass(blockOrder, [&](int k) { return aggOfRow[col[k]]; });
-    blockOrder = countingPass(blockOrder, [&](int k) { return aggOfRow[blockRow[k]]; });
-    std::vector<int> cbA, cbB, cbPtr{0}, cbSrc(nBlocks);
-    for (size_t t = 0; t < nBlocks; ++t)
+    // of the column), ascending within each group. Built per row aggregate, in
+    // parallel ranges of aggregates: its rows ascending and their blocks, binned
+    // by the column's aggregate (ascending) in that order.
+    int const nAgg = h.nAggregates;
+    std::vector<int> aggRowPtr(static_cast<size_t>(nAgg) + 1, 0), aggRowList(static_cast<size_t>(nUpd));
+    for (int s = 0; s < nUpd; ++s)
+        ++aggRowPtr[aggOfRow[s] + 1];
+    for (int a = 0; a < nAgg; ++a)
+        aggRowPtr[a + 1] += aggRowPtr[a];
     {
-        int const k = blockOrder[t], a = aggOfRow[blockRow[k]], b = aggOfRow[col[k]];
-        if (t == 0 || a != cbA.back() || b != cbB.back())
-        {
-            if (t > 0)
-                cbPtr.push_back(static_cast<int>(t));
-            cbA.push_back(a);
-            cbB.push_back(b);
-        }
-        cbSrc[t] = k;
+        std::vector<int> fill(aggRowPtr.begin(), aggRowPtr.end() - 1);
+        for (int s = 0; s < nUpd; ++s)
+            aggRowList[fill[aggOfRow[s]]++] = s;
     }
-    if (nBlocks > 0)
-        cbPtr.push_back(static_cast<int>(nBlocks));
+    struct CoarsePart
+    {
+        std::vector<int> a, b, len, src;
+    };
+    int const nAggParts = std::max(1, std::min(nParts, nAgg / 4));
+    std::vector<CoarsePart> cparts(static_cast<size_t>(nAggParts));
+    HostParallel(nAgg, nAggParts, [&](int a0, int a1, int part) {
+        CoarsePart& P = cparts[part];
+        std::vector<int> groupOf(static_cast<size_t>(nAgg), -1), keys, start;
+        for (int a = a0; a < a1; ++a)
+        {
+            // distinct column aggregates, ascending
+            keys.clear();
+            size_t total = 0;
+            for (int q = aggRowPtr[a]; q < aggRowPtr[a + 1]; ++q)
+            {
+                int const s = aggRowList[q];
+                total += static_cast<size_t>(rowPtr[s + 1] - rowPtr[s]);
+                for (int k = rowPtr[s]; k < rowPtr[s + 1]; ++k)
+                    if (groupOf[colAgg[k]] < 0)
+                    {
+                        groupOf[colAgg[k]] = 0;
+                        keys.push_back(colAgg[k]);
+                    }
+            }
+            std::sort(keys.begin(), keys.end());
+            for (size_t g = 0; g < keys.size(); ++g)
+                groupOf[keys[g]] = static_cast<int>(g);
+            // group sizes, then the blocks placed in order
+            start.assign(keys.size() + 1, 0);
+            for (int q = aggRowPtr[a]; q < aggRowPtr[a + 1]; ++q)
+            {
+                int const s = aggRowList[q];
+                for (int k = rowPtr[s]; k < rowPtr[s + 1]; ++k)
+                    ++start[groupOf[colAgg[k]] + 1];
+            }
+            size_t const base = P.src.size();
+            for (size_t g = 0; g < keys.size(); ++g)
+            {
+                P.a.push_back(a), P.b.push_back(keys[g]), P.len.push_back(start[g + 1]);
+                start[g + 1] += start[g];
+            }
+            P.src.resize(base + total);
+            for (int q = aggRowPtr[a]; q < aggRowPtr[a + 1]; ++q)
+            {
+                int const s = aggRowList[q];
+                for (int k = rowPtr[s]; k < rowPtr[s + 1]; ++k)
+                    P.src[base + static_cast<size_t>(start[groupOf[colAgg[k]]]++)] = k;
+            }
+            for (int key : keys)
+                groupOf[key] = -1;
+        }
+    });
+    std::vector<int> cbA, cbB, cbPtr{0}, cbSrc;
+    cbSrc.reserve(nBlocks);
+    for (CoarsePart const& P : cparts)
+    {
+        cbA.insert(cbA.end(), P.a.begin(), P.a.end());
+        cbB.insert(cbB.end(), P.b.begin(), P.b.end());
+        for (int len : P.len)
+            cbPtr.push_back(cbPtr.back() + len);
+        cbSrc.insert(cbSrc.end(), P.src.begin(), P.src.end());
+    }
     R.cbA.upload(cbA, h.stream);
     R.cbB.upload(cbB, h.stream);
     R.cbSrcPtr.upload(cbPtr, h.stream);
