@@ -159,6 +159,8 @@ struct RolloutState
     DeviceArray<double> coarse[2];                            // coarse operator / its inverse (ping-pong)
     DeviceArray<int> coarseFinal;                             // per mesh: buffer holding its inverse
     DeviceArray<float> coarseF;                               // the inverse, symmetrised, FP32
+    DeviceArray<float4> coarseD;                              // per solver row: rotation offset of the coarse space
+    DeviceArray<double> pivotRows;                            // pivot rows of a Gauss-Jordan step, per mesh
     DeviceArray<double> rc, zc;                               // coarse residual and correction
     DeviceArray<double> scal;                                 // per-mesh solver scalars (kScalarCount x nMeshes)
     DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count
@@ -177,7 +179,6 @@ struct RolloutState
     DeviceArray<int> rowVertex, meshOfRow, aggOfRow, aggRows;
     DeviceArray<int> incPtr, incCode;
     DeviceArray<int4> incSlot;
-    DeviceArray<int> bsrColAgg; // aggregate of each block's column
     DeviceArray<float4> bsrA, bsrB;
     DeviceArray<float> bsrC;
     int bsrMaxRow = 0;

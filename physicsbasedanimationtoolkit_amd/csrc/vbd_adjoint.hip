@@ -181,10 +181,11 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
         R.coarse[0].allocate(total);
         R.coarse[1].allocate(total);
     }
-    if (R.rc.n != 3 * static_cast<size_t>(h.nAggregates))
+    size_t const nCoarse = static_cast<size_t>(device::kCoarseDofs) * h.nAggregates;
+    if (R.rc.n != nCoarse)
     {
-        R.rc.allocate(3 * static_cast<size_t>(h.nAggregates));
-        R.zc.allocate(3 * static_cast<size_t>(h.nAggregates));
+        R.rc.allocate(nCoarse);
+        R.zc.allocate(nCoarse);
     }
     if (total == 0 || h.nAggregates == 0)
         return;
@@ -194,26 +195,46 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
     hipLaunchKernelGGL(device::ZeroDoublesLong, dim3(64), dim3(256), 0, h.stream, R.rc.ptr,
                        static_cast<long long>(R.rc.n));
     AssembleBsr(h, xs, dt, true); // the two-level solver is a fused one: aggregate-major rows
+    // the rotation offsets d of the coarse space at xs, per solver row
+    if (R.coarseD.n != static_cast<size_t>(h.nV))
+        R.coarseD.allocate(static_cast<size_t>(h.nV));
+    hipLaunchKernelGGL(device::CoarseBasis, dim3(h.nAggregates), dim3(128), 0, h.stream,
+                       static_cast<int const*>(h.aggVertPtr.ptr), static_cast<int const*>(R.aggRows.ptr),
+                       static_cast<int const*>(R.rowVertex.ptr), xs, R.coarseD.ptr);
     int const nCB = static_cast<int>(R.cbA.n);
     if (nCB > 0)
-        hipLaunchKernelGGL(device::CoarseFromBsr, dim3(Blocks(nCB)), dim3(256), 0, h.stream,
-                           static_cast<int const*>(R.cbA.ptr), static_cast<int const*>(R.cbB.ptr),
-                           static_cast<int const*>(R.cbSrcPtr.ptr), static_cast<int const*>(R.cbSrc.ptr), nCB,
-                           static_cast<float4 const*>(R.bsrA.ptr), static_cast<float4 const*>(R.bsrB.ptr),
-                           static_cast<float const*>(R.bsrC.ptr), c, R.coarse[0].ptr);
+    {
+        device::CoarseFromBsrArgs g{};
+        g.cbA = R.cbA.ptr, g.cbB = R.cbB.ptr, g.cbSrcPtr = R.cbSrcPtr.ptr, g.cbSrc = R.cbSrc.ptr;
+        g.nCoarseBlocks = nCB;
+        g.va = R.bsrA.ptr, g.vb = R.bsrB.ptr, g.vc = R.bsrC.ptr;
+        g.blockRow = R.bsrBlockRow.ptr, g.col = R.bsrCol.ptr, g.coarseD = R.coarseD.ptr;
+        g.c = c, g.Ac = R.coarse[0].ptr;
+        hipLaunchKernelGGL(device::CoarseFromBsr, dim3((nCB + 3) / 4), dim3(256), 0, h.stream, g);
+    }
     int maxN = 0;
     std::vector<int> finalBuf(static_cast<size_t>(nM));
     for (int m = 0; m < nM; ++m)
     {
-        int const n = 3 * (h.aggBegin[m + 1] - h.aggBegin[m]);
+        int const n = device::kCoarseDofs * (h.aggBegin[m + 1] - h.aggBegin[m]);
         maxN = std::max(maxN, n);
         finalBuf[m] = ((n + device::kPivot - 1) / device::kPivot) % 2; // parity of its step count
     }
     int64_t const maxEntries = int64_t(maxN) * maxN;
     dim3 const grid(static_cast<unsigned>((maxEntries + 255) / 256), static_cast<unsigned>(nM));
+    // block Gauss-Jordan: per step the pivot rows, then the rest in 64 x 64 tiles
+    int const tilesPerRow = (maxN + 63) / 64;
+    if (R.pivotRows.n != static_cast<size_t>(nM) * device::kPivot * maxN)
+        R.pivotRows.allocate(static_cast<size_t>(nM) * device::kPivot * maxN);
     for (int s = 0, k = 0; k < maxN; ++s, k += device::kPivot)
-        hipLaunchKernelGGL(device::CoarseInvertStep, grid, dim3(256), 0, h.stream,
-                           static_cast<double const*>(R.coarse[s & 1].ptr), R.coarse[(s + 1) & 1].ptr, k, c);
+    {
+        double const* Ak = R.coarse[s & 1].ptr;
+        hipLaunchKernelGGL(device::CoarsePivotRows, dim3(tilesPerRow, nM), dim3(256), 0, h.stream, Ak,
+                           R.pivotRows.ptr, k, maxN, c);
+        hipLaunchKernelGGL(device::CoarsePivotUpdate, dim3(tilesPerRow * tilesPerRow, nM), dim3(256), 0, h.stream,
+                           Ak, R.coarse[(s + 1) & 1].ptr, static_cast<double const*>(R.pivotRows.ptr), k, maxN,
+                           tilesPerRow, c);
+    }
     // allocated once per size: the captured CG chunk graph holds this pointer
     if (R.coarseFinal.n != finalBuf.size())
         R.coarseFinal.allocate(finalBuf.size());
@@ -485,11 +506,10 @@ void BuildBsrPattern(pbat_vbd_integrator& h, bool aggregateOrder)
     R.meshOfRow.upload(meshOfRow, h.stream);
     R.aggOfRow.upload(aggOfRow, h.stream);
     R.aggRows.upload(aggRows, h.stream);
-    // aggregate of each block's column (the direction product's coarse term)
+    // aggregate of each block's column (the coarse pattern below)
     std::vector<int> colAgg(nBlocks);
     for (size_t k = 0; k < nBlocks; ++k)
         colAgg[k] = aggOfRow[col[k]];
-    R.bsrColAgg.upload(colAgg, h.stream);
     // row chunks: every run of rows of one mesh (and, in the vertices' own order,
     // one colour) cut into chunks of at most 256 / L rows, then ordered
     // mesh-major (stable: colour, then row order)
@@ -549,7 +569,7 @@ void BuildBsrPattern(pbat_vbd_integrator& h, bool aggregateOrder)
     R.pqPart.allocate(std::max<size_t>(chunks.size(), 1));
     size_t const nAggAlloc = static_cast<size_t>(std::max(h.nAggregates, 1));
     R.rzPart.allocate(nAggAlloc), R.rrPart.allocate(nAggAlloc), R.bbPart.allocate(nAggAlloc);
-    R.rczcPart.allocate(3 * nAggAlloc);
+    R.rczcPart.allocate(static_cast<size_t>(device::kCoarseDofs) * nAggAlloc);
 
     // coarse blocks: the fine blocks grouped by (aggregate of the row, aggregate
     // of the column), ascending within each group. Built per row aggregate, in
@@ -732,7 +752,7 @@ void BsrPass(pbat_vbd_integrator& h, device::BsrArgs a, int vbeg, int vend)
     RolloutState& R = h.rollout;
     a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
     a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
-    a.meshOf = R.meshOfRow.ptr, a.aggOf = R.aggOfRow.ptr, a.dinv = R.dinv.ptr;
+    a.meshOf = R.meshOfRow.ptr, a.dinv = R.dinv.ptr;
     a.vbeg = vbeg, a.vend = vend;
     switch (BsrLanes(h))
     {
@@ -775,7 +795,7 @@ void LaunchDirectionL(pbat_vbd_integrator& h, device::BsrArgs const& a)
 }
 
 // The fused iteration's direction product over the row chunks (BsrDirection):
-// a.y = p (written), a.out = H p, a.z, a.pOld, a.zc as BsrArgs describes them.
+// a.y = p (written), a.out = H p, a.z, a.pOld as BsrArgs describes them.
 void DirectionPass(pbat_vbd_integrator& h, device::BsrArgs a)
 {
     RolloutState& R = h.rollout;
@@ -785,7 +805,7 @@ void DirectionPass(pbat_vbd_integrator& h, device::BsrArgs a)
         Fail(PBAT_VBD_ERROR_INVALID_ARGUMENT, "row chunks built for another lane count");
     a.rowPtr = R.bsrRowPtr.ptr, a.col = R.bsrCol.ptr;
     a.va = R.bsrA.ptr, a.vb = R.bsrB.ptr, a.vc = R.bsrC.ptr;
-    a.meshOf = R.meshOfRow.ptr, a.aggOf = R.aggOfRow.ptr, a.colAgg = R.bsrColAgg.ptr;
+    a.meshOf = R.meshOfRow.ptr;
     switch (BsrLanes(h))
     {
         case 1: LaunchDirectionL<1>(h, a); break;
@@ -1011,8 +1031,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             zeroScalars();
             product(y, R.q.ptr, nullptr);
             hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr,
-                               R.z.ptr, static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, nM, cg.A,
-                               static_cast<int const*>(h.aggOf.ptr), static_cast<double*>(nullptr));
+                               R.z.ptr, static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, nM, cg.A);
             hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
             if (ReadLive(h) == 0)
                 break;
@@ -1023,17 +1042,12 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         return out;
     }
     device::CoarseArgs const cargs = MakeCoarseArgs(h);
-    // two-level: rows of the coarse inverse are read by 1, 2 or 4 waves, by the
-    // largest mesh's row length
-    int maxCoarse = 0;
-    for (int m = 0; m < nM && twoLevel; ++m)
-        maxCoarse = std::max(maxCoarse, 3 * (h.aggBegin[m + 1] - h.aggBegin[m]));
-    int const coarseWavesPerRow = maxCoarse > 512 ? 4 : maxCoarse > 256 ? 2 : 1;
     auto aggUpdate = [&](int init, float4 const* p) {
         device::AggUpdateArgs u{};
         u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = R.aggRows.ptr, u.aggMesh = h.aggMesh.ptr;
         u.y = y, u.r = R.r.ptr, u.z = R.z.ptr, u.p = p, u.q = R.q.ptr, u.b = b, u.dinv = R.dinv.ptr;
         u.rc = twoLevel ? R.rc.ptr : nullptr;
+        u.coarseD = R.coarseD.ptr;
         u.pqPart = R.pqPart.ptr, u.aggChunks = R.aggChunks.ptr;
         u.aggFirst = R.aggFirst.ptr, u.aggDone = R.aggDone.ptr;
         u.rzPart = R.rzPart.ptr, u.rrPart = R.rrPart.ptr, u.bbPart = R.bbPart.ptr;
@@ -1042,10 +1056,11 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             hipLaunchKernelGGL(device::AggUpdate, dim3(h.nAggregates), dim3(128), 0, h.stream, u);
     };
     auto coarseRows = [&](int first) {
-        int const rows = 3 * h.nAggregates, rowsPerBlock = 4 / coarseWavesPerRow;
-        hipLaunchKernelGGL(device::CoarseApplyRows, dim3((rows + rowsPerBlock - 1) / rowsPerBlock), dim3(256), 0,
-                           h.stream, static_cast<float const*>(R.coarseF.ptr), static_cast<double const*>(R.rc.ptr),
-                           R.zc.ptr, cargs, coarseWavesPerRow, cg, R.rczcPart.ptr, first);
+        hipLaunchKernelGGL(device::CoarseApplyRows, dim3(h.nAggregates), dim3(64 * device::kCoarseWaves), 0, h.stream,
+                           static_cast<float const*>(R.coarseF.ptr), static_cast<double const*>(R.rc.ptr), R.zc.ptr,
+                           cargs, cg, R.rczcPart.ptr, static_cast<int const*>(h.aggVertPtr.ptr),
+                           static_cast<int const*>(R.aggRows.ptr), static_cast<float4 const*>(R.coarseD.ptr), R.z.ptr,
+                           first);
     };
     auto finish = [&](int first) {
         device::CgFinishArgs f{};
@@ -1070,7 +1085,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             float4* const pOld = (k & 1) ? R.p2.ptr : R.p.ptr;
             float4* const pNew = (k & 1) ? R.p.ptr : R.p2.ptr;
             device::BsrArgs a{};
-            a.y = pNew, a.out = R.q.ptr, a.z = R.z.ptr, a.pOld = pOld, a.zc = twoLevel ? R.zc.ptr : nullptr;
+            a.y = pNew, a.out = R.q.ptr, a.z = R.z.ptr, a.pOld = pOld;
             a.cg = cg;
             DirectionPass(h, a);
             aggUpdate(0, pNew);
@@ -1083,8 +1098,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     {
         product(y, R.q.ptr, nullptr);
         hipLaunchKernelGGL(device::CgInitResidual, dim3(gridN), dim3(256), 0, h.stream, b, R.q.ptr, R.r.ptr, R.z.ptr,
-                           static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, nM, cg.A,
-                           static_cast<int const*>(h.aggOf.ptr), static_cast<double*>(nullptr));
+                           static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, nM, cg.A);
         GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, false);
         GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, true, cg.A);
         hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 1);
@@ -1093,8 +1107,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             product(R.p.ptr, R.q.ptr, cg.A);
             hipLaunchKernelGGL(device::CgUpdate, dim3(gridN), dim3(256), 0, h.stream, y, R.r.ptr, R.z.ptr,
                                static_cast<float4 const*>(R.p.ptr), static_cast<float4 const*>(R.q.ptr),
-                               static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, cg,
-                               static_cast<int const*>(h.aggOf.ptr), static_cast<double*>(nullptr));
+                               static_cast<float4 const*>(nullptr), h.meshOf.ptr, n, cg);
             GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, false);
             GaussSeidelPass(h, cg, R.z.ptr, R.r.ptr, true, cg.A);
             hipLaunchKernelGGL(device::CgScalars, dim3(1), dim3(256), 0, h.stream, cg, 0);
@@ -1106,8 +1119,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     if (graph)
     {
         std::vector<double> const key{double(dt),        double(cfg.solver), double(check),       cg.tol2,
-                                      double(BsrLanes(h)), double(BsrPerLane(h)), double(h.xcdRemap),
-                                      double(coarseWavesPerRow)};
+                                      double(BsrLanes(h)), double(BsrPerLane(h)), double(h.xcdRemap)};
         // every device address the captured kernels take: a buffer reallocated at
         // another address forces a new capture
         std::vector<void const*> const ptrs{
@@ -1117,8 +1129,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             R.bsrA.ptr,        R.bsrB.ptr,      R.bsrC.ptr,    h.meshOf.ptr,     h.aggOf.ptr,     h.aggVertPtr.ptr,
             h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.aggChunks.ptr,
             R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
-            R.bsrColAgg.ptr,   R.doneDev,       R.aggFirst.ptr, R.aggDone.ptr,    R.yS.ptr,        R.bS.ptr,
-            R.aggRows.ptr,     R.aggOfRow.ptr,  R.meshOfRow.ptr};
+            R.doneDev,       R.aggFirst.ptr, R.aggDone.ptr,    R.yS.ptr,        R.bS.ptr,
+            R.aggRows.ptr,     R.aggOfRow.ptr,  R.meshOfRow.ptr, R.coarseD.ptr};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();

@@ -537,12 +537,10 @@ __device__ __forceinline__ float4 ApplyInverse(float4 const* __restrict__ dinv, 
 }
 
 // r = b - q (q = H y); z = D^-1 r when dinv; accumulates b.b, r.r (and r.z)
-// With rc (two-level): r.z is left to CoarseProlong.
 __global__ __launch_bounds__(256) void CgInitResidual(float4 const* __restrict__ b, float4 const* __restrict__ q,
                                                       float4* __restrict__ r, float4* __restrict__ z,
                                                       float4 const* __restrict__ dinv, int const* __restrict__ meshOf,
-                                                      int n, int nM, double* __restrict__ A,
-                                                      int const* __restrict__ aggOf, double* __restrict__ rc)
+                                                      int n, int nM, double* __restrict__ A)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     double bb = 0., rr = 0., rz = 0.;
@@ -564,19 +562,17 @@ __global__ __launch_bounds__(256) void CgInitResidual(float4 const* __restrict__
     }
     BlockAccumulate(bb, m, A, kAccBb, nM);
     BlockAccumulate(rr, m, A, kAccRr, nM);
-    if (dinv && !rc)
+    if (dinv)
         BlockAccumulate(rz, m, A, kAccRzNew, nM);
 }
 
 // y += alpha p, r -= alpha q with alpha = rz / pq of the vertex's mesh (0 when
 // it is done); r.r accumulated; with dinv: z = D^-1 r and r.z accumulated;
-// without: z = 0 (a Gauss-Seidel preconditioner pass follows). With rc
-// (two-level): r.z left to CoarseProlong.
+// without: z = 0 (a Gauss-Seidel preconditioner pass follows).
 __global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* __restrict__ r,
                                                 float4* __restrict__ z, float4 const* __restrict__ p,
                                                 float4 const* __restrict__ q, float4 const* __restrict__ dinv,
-                                                int const* __restrict__ meshOf, int n, CgState c,
-                                                int const* __restrict__ aggOf, double* __restrict__ rc)
+                                                int const* __restrict__ meshOf, int n, CgState c)
 {
     int const nM = c.nM;
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
@@ -607,7 +603,7 @@ __global__ __launch_bounds__(256) void CgUpdate(float4* __restrict__ y, float4* 
         }
     }
     BlockAccumulate(rr, m, c.A, kAccRr, nM);
-    if (dinv && !rc)
+    if (dinv)
         BlockAccumulate(rz, m, c.A, kAccRzNew, nM);
 }
 
@@ -633,12 +629,16 @@ __global__ __launch_bounds__(256) void CgDirection(float4* __restrict__ p, float
     p[i] = make_float4(zi.x + beta * pi.x, zi.y + beta * pi.y, zi.z + beta * pi.z, 0.f);
 }
 
-// ---- two-level preconditioner: translations of vertex aggregates -----------
-// Coarse space Z: one 3-column block per aggregate (unit translations of its
-// vertices). The coarse operator Z^T H Z is formed per mesh as a dense block
-// from the assembled step Hessian (CoarseFromBsr), inverted in place by
-// Gauss-Jordan steps (SPD: no pivoting), and the preconditioner adds
-// Z (Z^T H Z)^-1 Z^T r to the block-Jacobi one.
+// ---- two-level preconditioner: rigid motions of vertex aggregates ----------
+// Coarse space Z: one 6-column block per aggregate, the unit translations of
+// its vertices and the infinitesimal rotations omega x d_i about its centroid
+// (d_i = (x_i - c) / s, s the aggregate's RMS radius; d = 0 for an aggregate
+// whose vertices are (nearly) collinear, whose rotation columns are then zero
+// and carry an identity diagonal in the coarse operator). Z_i = [I | -[d_i]_x].
+// The coarse operator Z^T H Z is formed per mesh as a dense block from the
+// assembled step Hessian (CoarseFromBsr), inverted in place by Gauss-Jordan
+// steps (SPD: no pivoting), and the preconditioner adds Z (Z^T H Z)^-1 Z^T r
+// to the block-Jacobi one.
 
 struct CoarseArgs
 {
@@ -655,84 +655,162 @@ struct CoarseArgs
 //   B[K, K] = P^-1,   B[K, j] = P^-1 A[K, j],   B[i, K] = -A[i, K] P^-1,
 //   B[i, j] = A[i, j] - A[i, K] P^-1 A[K, j]     (i, j outside K),
 // the same as b scalar steps; after ceil(n / kPivot) steps an n x n SPD block
-// holds its inverse. Every workgroup inverts P in LDS (SPD, no pivoting), one
-// thread per entry.
-constexpr int kPivot = 8;
+// holds its inverse. A step is two launches: CoarsePivotRows forms the b rows
+// W = B[K, :] (P^-1 in the K columns), CoarsePivotUpdate every other row from
+// Ak and W: B[i, j] = (j in K ? 0 : A[i, j]) - A[i, K] W[:, j].
+constexpr int kPivot = 32;
 
-__global__ __launch_bounds__(256) void CoarseInvertStep(double const* __restrict__ Ak, double* __restrict__ Bk,
-                                                        int k, CoarseArgs c)
+// P inverted in LDS (SPD, no pivoting, Gauss-Jordan one entry per thread step)
+// by every workgroup, then W[:, j] for its 64 columns, 4 threads per column
+// each forming 8 of its b entries. W holds kPivot rows of wStride doubles per
+// mesh.
+__global__ __launch_bounds__(256) void CoarsePivotRows(double const* __restrict__ Ak, double* __restrict__ W,
+                                                       int k, int wStride, CoarseArgs c)
 {
     int const m = static_cast<int>(blockIdx.y);
-    int const n = 3 * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
-    if (k >= n)
+    int const n = kCoarseDofs * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
+    int const j0 = static_cast<int>(blockIdx.x) * 64;
+    if (k >= n || j0 >= n)
         return;
     int const b = min(kPivot, n - k);
     double const* A = Ak + c.meshOffset[m];
-    __shared__ double P[kPivot][kPivot];
+    __shared__ double P[kPivot][kPivot + 1];
     int const t = static_cast<int>(threadIdx.x);
-    int const pr = t / kPivot, pc = t % kPivot;
-    bool const inP = t < kPivot * kPivot && pr < b && pc < b;
-    if (inP)
-        P[pr][pc] = A[static_cast<size_t>(k + pr) * n + k + pc];
+    for (int e = t; e < kPivot * kPivot; e += 256)
+    {
+        int const r = e / kPivot, q = e % kPivot;
+        P[r][q] = (r < b && q < b) ? A[static_cast<size_t>(k + r) * n + k + q] : 0.;
+    }
     __syncthreads();
-    // Gauss-Jordan inverse of the b x b pivot block in place, one thread per entry
     for (int q = 0; q < b; ++q)
     {
-        double v = 0.;
-        if (inP)
+        double v[kPivot * kPivot / 256];
+#pragma unroll
+        for (int u = 0; u < kPivot * kPivot / 256; ++u)
         {
+            int const e = t + 256 * u, r = e / kPivot, cc = e % kPivot;
             double const inv = 1. / P[q][q];
-            if (pr == q && pc == q)
-                v = inv;
-            else if (pr == q)
-                v = P[q][pc] * inv;
-            else if (pc == q)
-                v = -P[pr][q] * inv;
+            if (r >= b || cc >= b)
+                v[u] = 0.;
+            else if (r == q && cc == q)
+                v[u] = inv;
+            else if (r == q)
+                v[u] = P[q][cc] * inv;
+            else if (cc == q)
+                v[u] = -P[r][q] * inv;
             else
-                v = P[pr][pc] - P[pr][q] * P[q][pc] * inv;
+                v[u] = P[r][cc] - P[r][q] * P[q][cc] * inv;
         }
         __syncthreads();
-        if (inP)
-            P[pr][pc] = v;
-        __syncthreads();
-    }
-    int64_t const idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (idx >= static_cast<int64_t>(n) * n)
-        return;
-    int const i = static_cast<int>(idx / n), j = static_cast<int>(idx % n);
-    bool const iK = i >= k && i < k + b, jK = j >= k && j < k + b;
-    double v;
-    if (iK && jK)
-    {
-        v = P[i - k][j - k];
-    }
-    else if (iK)
-    {
-        v = 0.;
-        for (int l = 0; l < b; ++l)
-            v += P[i - k][l] * A[static_cast<size_t>(k + l) * n + j];
-    }
-    else if (jK)
-    {
-        v = 0.;
-        for (int l = 0; l < b; ++l)
-            v -= A[static_cast<size_t>(i) * n + k + l] * P[l][j - k];
-    }
-    else
-    {
-        double w[kPivot];
-        for (int l = 0; l < b; ++l)
-            w[l] = A[static_cast<size_t>(k + l) * n + j];
-        v = A[idx];
-        for (int q = 0; q < b; ++q)
+#pragma unroll
+        for (int u = 0; u < kPivot * kPivot / 256; ++u)
         {
-            double t2 = 0.;
-            for (int l = 0; l < b; ++l)
-                t2 += P[q][l] * w[l];
-            v -= A[static_cast<size_t>(i) * n + k + q] * t2;
+            int const e = t + 256 * u;
+            P[e / kPivot][e % kPivot] = v[u];
+        }
+        __syncthreads();
+    }
+    int const j = j0 + (t & 63), l0 = (t >> 6) * (kPivot / 4);
+    if (j >= n)
+        return;
+    double* Wm = W + static_cast<size_t>(m) * kPivot * wStride;
+    if (j >= k && j < k + b)
+    {
+        for (int l = l0; l < l0 + kPivot / 4 && l < b; ++l)
+            Wm[static_cast<size_t>(l) * wStride + j] = P[l][j - k];
+        return;
+    }
+    double a[kPivot];
+#pragma unroll
+    for (int q = 0; q < kPivot; ++q)
+        a[q] = q < b ? A[static_cast<size_t>(k + q) * n + j] : 0.;
+    for (int l = l0; l < l0 + kPivot / 4 && l < b; ++l)
+    {
+        double w = 0.;
+#pragma unroll
+        for (int q = 0; q < kPivot; ++q)
+            w += P[l][q] * a[q];
+        Wm[static_cast<size_t>(l) * wStride + j] = w;
+    }
+}
+
+// The other rows of step k, one 64 x 64 tile of (i, j) per workgroup
+// (blockIdx.x over the tiles of the largest block, blockIdx.y = mesh): A[i, K]
+// and W[:, j] staged in LDS, a 4 x 4 group of entries per thread. Rows in K
+// copy W.
+__global__ __launch_bounds__(256) void CoarsePivotUpdate(double const* __restrict__ Ak, double* __restrict__ Bk,
+                                                         double const* __restrict__ W, int k, int wStride,
+                                                         int tilesPerRow, CoarseArgs c)
+{
+    int const m = static_cast<int>(blockIdx.y);
+    int const n = kCoarseDofs * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
+    int const i0 = (static_cast<int>(blockIdx.x) / tilesPerRow) * 64, j0 = (static_cast<int>(blockIdx.x) % tilesPerRow) * 64;
+    if (k >= n || i0 >= n || j0 >= n)
+        return;
+    int const b = min(kPivot, n - k);
+    double const* A = Ak + c.meshOffset[m];
+    double* B = Bk + c.meshOffset[m];
+    double const* Wm = W + static_cast<size_t>(m) * kPivot * wStride;
+    __shared__ double Ai[64][kPivot + 1];
+    __shared__ double Wj[kPivot][64 + 1];
+    int const t = static_cast<int>(threadIdx.x);
+    for (int e = t; e < 64 * kPivot; e += 256)
+    {
+        int const r = e / kPivot, q = e % kPivot, i = i0 + r;
+        Ai[r][q] = (i < n && q < b) ? A[static_cast<size_t>(i) * n + k + q] : 0.;
+    }
+    for (int e = t; e < kPivot * 64; e += 256)
+    {
+        int const q = e / 64, cc = e % 64, j = j0 + cc;
+        Wj[q][cc] = (j < n && q < b) ? Wm[static_cast<size_t>(q) * wStride + j] : 0.;
+    }
+    __syncthreads();
+    // the 4 x 4 entries (r0 + u, c0 + v) of this thread, A[i, K] W[K, j] summed
+    // over K from registers
+    int const r0 = (t >> 4) * 4, c0 = (t & 15) * 4;
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            acc[u][v] = 0.;
+#pragma unroll 8
+    for (int q = 0; q < kPivot; ++q)
+    {
+        double ai[4], wj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            ai[u] = Ai[r0 + u][q];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            wj[v] = Wj[q][c0 + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                acc[u][v] += ai[u] * wj[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+    {
+        int const i = i0 + r0 + u;
+        if (i >= n)
+            break;
+        bool const iK = i >= k && i < k + b;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+        {
+            int const j = j0 + c0 + v;
+            if (j >= n)
+                break;
+            double out;
+            if (iK)
+                out = Wj[i - k][c0 + v];
+            else
+                out = ((j >= k && j < k + b) ? 0. : A[static_cast<size_t>(i) * n + j]) - acc[u][v];
+            B[static_cast<size_t>(i) * n + j] = out;
         }
     }
-    Bk[c.meshOffset[m] + idx] = v;
 }
 
 // Guard after inversion: a mesh block whose inverse has a non-positive or
@@ -743,7 +821,7 @@ __global__ __launch_bounds__(256) void CoarseGuard(double* __restrict__ buf0, do
 {
     __shared__ int bad;
     int const m = static_cast<int>(blockIdx.x);
-    int const n = 3 * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
+    int const n = kCoarseDofs * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
     double* A = (finalBuf[m] ? buf1 : buf0) + c.meshOffset[m];
     if (threadIdx.x == 0)
         bad = 0;
@@ -769,7 +847,7 @@ __global__ __launch_bounds__(256) void CoarseToFloat(double const* __restrict__ 
                                                      float* __restrict__ out)
 {
     int const m = static_cast<int>(blockIdx.y);
-    int const n = 3 * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
+    int const n = kCoarseDofs * (c.meshAggBegin[m + 1] - c.meshAggBegin[m]);
     int64_t const idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= static_cast<int64_t>(n) * n)
         return;

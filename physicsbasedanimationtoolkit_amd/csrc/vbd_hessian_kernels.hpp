@@ -367,7 +367,7 @@ enum class BsrMode : int
     kProductDot = 1,     // out = H y, acc[mesh] += y . H y
     kGaussSeidel = 2,    // rows of one colour: y_i += D_i^-1 (b_i - (H y)_i)
     kGaussSeidelDot = 3, // same, then acc[mesh] += b_i . y_i
-    kDirection = 4       // p = z + Z zc + beta p_old formed on the fly (written to y);
+    kDirection = 4       // p = z + beta p_old formed on the fly (written to y);
                          // out = H p, p . H p per row chunk (BsrDirection)
 };
 
@@ -383,12 +383,10 @@ struct BsrArgs
     float4* __restrict__ out;        // product
     float4 const* __restrict__ dinv; // inverse diagonal blocks (Gauss-Seidel)
     int const* __restrict__ meshOf;
-    // kDirection: p_j = z_j + zc[aggOf j] + beta p_old_j of mesh m (p_old_j when m is done)
+    // kDirection: p_j = z_j + beta p_old_j of mesh m (p_old_j when m is done); z
+    // holds the whole preconditioned residual, its coarse part included
     float4 const* __restrict__ z;
     float4 const* __restrict__ pOld;
-    double const* __restrict__ zc; // nullable (block Jacobi): no coarse term
-    int const* __restrict__ aggOf;
-    int const* __restrict__ colAgg; // aggregate of each block's column (kDirection)
     CgState cg;
     double* __restrict__ acc; // accumulators (dot modes), array accArray
     int accArray;
@@ -396,22 +394,14 @@ struct BsrArgs
     int xcdChunk;
 };
 
-// p_j = z_j + zc[g] + beta p_old_j with g the aggregate of vertex j (p_old_j when
-// the mesh has converged)
-__device__ __forceinline__ float4 DirectionVector(BsrArgs const& a, int j, int g, float beta, bool frozen)
+// p_j = z_j + beta p_old_j (p_old_j when the mesh has converged)
+__device__ __forceinline__ float4 DirectionVector(BsrArgs const& a, int j, float beta, bool frozen)
 {
     float4 const po = a.pOld[j];
     if (frozen)
         return po;
     float4 const zj = a.z[j];
-    float4 r = make_float4(zj.x + beta * po.x, zj.y + beta * po.y, zj.z + beta * po.z, 0.f);
-    if (a.zc)
-    {
-        r.x += static_cast<float>(a.zc[3 * g]);
-        r.y += static_cast<float>(a.zc[3 * g + 1]);
-        r.z += static_cast<float>(a.zc[3 * g + 2]);
-    }
-    return r;
+    return make_float4(zj.x + beta * po.x, zj.y + beta * po.y, zj.z + beta * po.z, 0.f);
 }
 
 // (H v)_row over blocks [kb, ke) of one row: lane l of the row's L lanes takes
@@ -422,7 +412,7 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
 {
     constexpr bool kDir = M == BsrMode::kDirection;
     float hx = 0.f, hy = 0.f, hz = 0.f;
-    int cj[K], cg[K];
+    int cj[K];
     float4 A[K], B[K];
     float C[K];
 #pragma unroll
@@ -430,12 +420,9 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
     {
         int const k = kb + lane + u * L;
         cj[u] = -1;
-        cg[u] = 0;
         if (k < ke)
         {
             cj[u] = a.col[k];
-            if constexpr (kDir)
-                cg[u] = a.colAgg[k];
             A[u] = a.va[k];
             B[u] = a.vb[k];
             C[u] = a.vc[k];
@@ -447,7 +434,7 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
         if (cj[u] >= 0)
         {
             if constexpr (kDir)
-                P[u] = DirectionVector(a, cj[u], cg[u], beta, frozen);
+                P[u] = DirectionVector(a, cj[u], beta, frozen);
             else
                 P[u] = a.y[cj[u]];
         }
@@ -465,7 +452,7 @@ __device__ __forceinline__ float3 RowProduct(BsrArgs const& a, int kb, int ke, i
         float const Ck = a.vc[k];
         float4 Pk;
         if constexpr (kDir)
-            Pk = DirectionVector(a, a.col[k], a.colAgg[k], beta, frozen);
+            Pk = DirectionVector(a, a.col[k], beta, frozen);
         else
             Pk = a.y[a.col[k]];
         hx += Ak.x * Pk.x + Ak.y * Pk.y + Ak.z * Pk.z;
@@ -532,8 +519,8 @@ __global__ __launch_bounds__(256) void BsrKernel(BsrArgs a)
 // the same partial sums whether it is stepped alone or in a batch.
 //
 // One workgroup per chunk (XCD-aware order when xcdChunk > 0): the direction
-// p = z + Z zc + beta p_old of each of its rows is formed on the fly for every
-// block, written to y, q = H p to out, and the chunk's p . q, summed over its
+// p = z + beta p_old of each of its rows is formed on the fly for every block
+// (z includes the coarse correction), written to y, q = H p to out, and the chunk's p . q, summed over its
 // rows in order, to pqPart[chunk]. Chunks of a converged mesh do nothing.
 template <int L, int K>
 __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int4 const* __restrict__ chunks, int nChunks,
@@ -547,7 +534,7 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int4 const* __res
     if (w >= nChunks)
         return; // padding of the XCD order
     // the chunk (rows and mesh), then its mesh's done flag and beta together
-    // with the row's extent and aggregate
+    // with the row's extent
     int4 const ch = chunks[w];
     int const m = ch.z;
     constexpr bool frozen = false;
@@ -556,19 +543,17 @@ __global__ __launch_bounds__(256) void BsrDirection(BsrArgs a, int4 const* __res
     bool const active = row < ch.y;
     int const done = a.cg.F[kDone * nM + m];
     float const beta = static_cast<float>(a.cg.P[kBeta * nM + m]);
-    int kb = 0, ke = 0, rowAgg = 0;
+    int kb = 0, ke = 0;
     if (active)
     {
         kb = a.rowPtr[row];
         ke = a.rowPtr[row + 1];
-        if (lane == 0 && a.zc)
-            rowAgg = a.aggOf[row];
     }
     if (done != 0)
         return; // the chunk's mesh has converged: its vectors no longer change
     float4 pr = make_float4(0.f, 0.f, 0.f, 0.f);
     if (active && lane == 0) // the row's own direction, in flight during the product
-        pr = DirectionVector(a, row, rowAgg, beta, frozen);
+        pr = DirectionVector(a, row, beta, frozen);
     float3 const h = RowProduct<L, K, BsrMode::kDirection>(a, kb, ke, lane, beta, frozen);
     __shared__ double sdot[R];
     if (lane == 0)
@@ -642,8 +627,8 @@ __device__ __forceinline__ void Sum128(double (&v)[V], double (*lds)[V])
 //   init:   r = b - q               (q = H y0)
 //   update: y += alpha p, r -= alpha q
 // then z = D^-1 r, and the aggregate's r . z, r . r (and b . b at init) and,
-// when rc is given, its restriction rc = sum r, summed in a fixed order and
-// written to the per-aggregate partials. Every updated vertex belongs to
+// when rc is given, its restriction rc = Z^T r = (sum r_i, sum d_i x r_i),
+// summed in a fixed order and written to the per-aggregate partials. Every updated vertex belongs to
 // exactly one aggregate and every aggregate to one mesh.
 struct AggUpdateArgs
 {
@@ -658,6 +643,7 @@ struct AggUpdateArgs
     float4 const* __restrict__ b; // init only
     float4 const* __restrict__ dinv;
     double* __restrict__ rc;      // nullable
+    float4 const* __restrict__ coarseD; // per row: d_i of the coarse rotations (with rc)
     double const* __restrict__ pqPart;
     int2 const* __restrict__ aggChunks; // chunks of the aggregate's mesh
     int const* __restrict__ aggFirst;   // kAggFirst vertices per aggregate (-1: none)
@@ -675,7 +661,7 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 {
     int const nM = u.cg.nM;
     __shared__ double lds1[2][1];
-    __shared__ double lds6[2][6];
+    __shared__ double lds9[2][9];
     int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
     // independent loads first: the mesh, its chunk range, the aggregate's vertex
     // range, its mesh's done flag and the thread's first vertex. A converged
@@ -692,9 +678,11 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
     double const rzOld = u.init ? 0. : u.cg.P[kRz * nM + m];
     // the thread's first vertex: loads issued before alpha is known
     float4 const zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 q0 = zero4, a0 = zero4, y0 = zero4, r0 = zero4, d0 = zero4, d1 = zero4;
+    float4 q0 = zero4, a0 = zero4, y0 = zero4, r0 = zero4, d0 = zero4, d1 = zero4, o0 = zero4;
     if (i0 >= 0)
     {
+        if (u.rc)
+            o0 = u.coarseD[i0];
         q0 = u.q[i0];
         a0 = u.init ? u.b[i0] : u.p[i0];
         if (!u.init)
@@ -711,10 +699,10 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
         Sum128<1>(pq, lds1);
         alpha = pq[0] != 0. ? static_cast<float>(rzOld / pq[0]) : 0.f;
     }
-    double sx = 0., sy = 0., sz = 0., rz = 0., rr = 0., bb = 0.;
+    double sx = 0., sy = 0., sz = 0., ox = 0., oy = 0., oz = 0., rz = 0., rr = 0., bb = 0.;
     // init: a = b, r = b - q; update: a = p, y += alpha p, r -= alpha q
     auto update = [&](int i, float4 const& qi, float4 const& ai, float4 yi, float4 ri, float4 const& D0,
-                      float4 const& D1) {
+                      float4 const& D1, float4 const& d) {
         if (u.init)
         {
             ri = make_float4(ai.x - qi.x, ai.y - qi.y, ai.z - qi.z, 0.f);
@@ -732,11 +720,12 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
                                       D0.z * ri.x + D1.x * ri.y + D1.y * ri.z, 0.f);
         u.z[i] = zi;
         sx += ri.x, sy += ri.y, sz += ri.z;
+        ox += d.y * ri.z - d.z * ri.y, oy += d.z * ri.x - d.x * ri.z, oz += d.x * ri.y - d.y * ri.x;
         rz += Dot3(ri, zi);
         rr += Dot3(ri, ri);
     };
     if (i0 >= 0)
-        update(i0, q0, a0, y0, r0, d0, d1);
+        update(i0, q0, a0, y0, r0, d0, d1, o0);
     for (int k = kBeg + 128; k < kEnd; k += 128)
     {
         int const i = u.aggVerts[k];
@@ -747,18 +736,15 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
             yi = u.y[i];
             ri = u.r[i];
         }
-        update(i, qi, ai, yi, ri, u.dinv[2 * i], u.dinv[2 * i + 1]);
+        update(i, qi, ai, yi, ri, u.dinv[2 * i], u.dinv[2 * i + 1], u.rc ? u.coarseD[i] : zero4);
     }
-    double sums[6] = {rz, rr, bb, sx, sy, sz};
-    Sum128<6>(sums, lds6);
+    double sums[9] = {rz, rr, bb, sx, sy, sz, ox, oy, oz};
+    Sum128<9>(sums, lds9);
     if (t == 0)
     {
         if (u.rc)
-        {
-            u.rc[3 * g] = sums[3];
-            u.rc[3 * g + 1] = sums[4];
-            u.rc[3 * g + 2] = sums[5];
-        }
+            for (int c = 0; c < kCoarseDofs; ++c)
+                u.rc[kCoarseDofs * g + c] = sums[3 + c];
         u.rzPart[g] = sums[0];
         u.rrPart[g] = sums[1];
         if (u.init)
@@ -766,41 +752,98 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
     }
 }
 
-// zc = (Z^T H Z)^-1 rc per mesh (wpr waves per coarse row) and each coarse
-// row's share of rc . zc, the coarse part of the mesh's r . z, into rczcPart.
-__global__ __launch_bounds__(256) void CoarseApplyRows(float const* __restrict__ coarseF,
-                                                       double const* __restrict__ rc, double* __restrict__ zc,
-                                                       CoarseArgs c, int wpr, CgState cg,
-                                                       double* __restrict__ rczcPart, int first)
+// zc = (Z^T H Z)^-1 rc per mesh and each coarse row's share of rc . zc, the
+// coarse part of the mesh's r . z, into rczcPart; then the coarse correction
+// Z_i zc_g = t_g + w_g x d_i (zc_g = (t_g, w_g)) added to z_i for the
+// aggregate's rows, so that z = D^-1 r + Z zc. One workgroup of kCoarseWaves
+// waves per aggregate g computes its kCoarseDofs rows: wave w takes the
+// 64-column chunks w, w + kCoarseWaves, ... of its mesh's n columns, loads rc
+// and the rows' entries of up to kCoarseChunksPerPass of them at once, then
+// accumulates; the waves' sums are added in order. The order depends on n
+// only, so a mesh gets the same zc alone or in a batch.
+constexpr int kCoarseWaves = 8;
+constexpr int kCoarseChunksPerPass = 6; // n <= 3,072 in one pass
+
+__global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(float const* __restrict__ coarseF,
+                                                                     double const* __restrict__ rc,
+                                                                     double* __restrict__ zc, CoarseArgs c,
+                                                                     CgState cg, double* __restrict__ rczcPart,
+                                                                     int const* __restrict__ aggPtr,
+                                                                     int const* __restrict__ aggRows,
+                                                                     float4 const* __restrict__ coarseD,
+                                                                     float4* __restrict__ z, int first)
 {
-    __shared__ double part[4];
+    constexpr int kR = kCoarseDofs, kU = kCoarseChunksPerPass, kW = kCoarseWaves;
+    __shared__ double part[kW][kR];
+    __shared__ float zcS[kR];
+    int const g = static_cast<int>(blockIdx.x);
     int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
-    int const row = static_cast<int>(blockIdx.x) * (4 / wpr) + wave / wpr;
-    int const sub = wave % wpr;
-    double sum = 0.;
-    // rows of converged meshes do nothing
-    bool const live = row < 3 * c.nAgg && (first || cg.F[kDone * cg.nM + c.aggMesh[row / 3]] == 0);
-    if (live)
+    int const m = c.aggMesh[g];
+    if (!first && cg.F[kDone * cg.nM + m] != 0)
+        return; // the mesh has converged
+    int const b0 = kR * c.meshAggBegin[m];
+    int const n = kR * c.meshAggBegin[m + 1] - b0;
+    int const lr0 = kR * g - b0; // the aggregate's first row within the mesh
+    float const* A = coarseF + c.meshOffset[m] + static_cast<size_t>(lr0) * n;
+    double s[kR];
+#pragma unroll
+    for (int i = 0; i < kR; ++i)
+        s[i] = 0.;
+    for (int c0 = wave; c0 * 64 < n; c0 += kW * kU)
     {
-        int const m = c.aggMesh[row / 3];
-        int const b0 = 3 * c.meshAggBegin[m];
-        int const n = 3 * c.meshAggBegin[m + 1] - b0;
-        float const* A = coarseF + c.meshOffset[m] + static_cast<size_t>(row - b0) * n;
-        for (int j = sub * 64 + lane; j < n; j += 64 * wpr)
-            sum += double(A[j]) * rc[b0 + j];
+        double x[kU];
+        float a[kR][kU];
+        // branch-free: out-of-range lanes load column n - 1 and drop it, so
+        // every load of the pass is in flight together
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+        {
+            int const j = min((c0 + kW * u) * 64 + lane, n - 1);
+            x[u] = rc[b0 + j];
+#pragma unroll
+            for (int i = 0; i < kR; ++i)
+                a[i][u] = A[static_cast<size_t>(i) * n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if ((c0 + kW * u) * 64 + lane >= n)
+                x[u] = 0.;
+#pragma unroll
+        for (int i = 0; i < kR; ++i)
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                s[i] += double(a[i][u]) * x[u];
     }
-    for (int off = 32; off > 0; off >>= 1)
-        sum += __shfl_xor(sum, off);
+#pragma unroll
+    for (int i = 0; i < kR; ++i)
+        for (int off = 32; off > 0; off >>= 1)
+            s[i] += __shfl_xor(s[i], off);
     if (lane == 0)
-        part[wave] = sum;
+#pragma unroll
+        for (int i = 0; i < kR; ++i)
+            part[wave][i] = s[i];
     __syncthreads();
-    if (lane == 0 && sub == 0 && live)
+    if (threadIdx.x < kR)
     {
-        double t = part[wave];
-        for (int k = 1; k < wpr; ++k)
-            t += part[wave + k];
+        int const i = static_cast<int>(threadIdx.x), row = b0 + lr0 + i;
+        double t = part[0][i];
+        for (int w = 1; w < kW; ++w)
+            t += part[w][i];
         zc[row] = t;
         rczcPart[row] = t * rc[row];
+        zcS[i] = static_cast<float>(t);
+    }
+    __syncthreads();
+    float const tx = zcS[0], ty = zcS[1], tz = zcS[2], wx = zcS[3], wy = zcS[4], wz = zcS[5];
+    for (int k = aggPtr[g] + static_cast<int>(threadIdx.x); k < aggPtr[g + 1]; k += 64 * kW)
+    {
+        int const i = aggRows[k];
+        float4 const d = coarseD[i];
+        float4 zi = z[i];
+        zi.x += tx + (wy * d.z - wz * d.y);
+        zi.y += ty + (wz * d.x - wx * d.z);
+        zi.z += tz + (wx * d.y - wy * d.x);
+        z[i] = zi;
     }
 }
 
@@ -1110,7 +1153,7 @@ __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
     else if (wave == 1)
         v = StridedSum<64>(f.rrPart, a0, a1, lane);
     else if (wave == 2 && f.rczcPart)
-        v = StridedSum<64, 32>(f.rczcPart, 3 * a0, 3 * a1, lane);
+        v = StridedSum<64, 32>(f.rczcPart, kCoarseDofs * a0, kCoarseDofs * a1, lane);
     else if (wave == 3 && first)
         v = StridedSum<64>(f.bbPart, a0, a1, lane);
     for (int off = 32; off > 0; off >>= 1)
@@ -1131,39 +1174,149 @@ __global__ __launch_bounds__(256) void CgFinish(CgFinishArgs f, int first)
         f.aggDone[g] = doneS;
 }
 
-// Galerkin coarse operator Z^T H Z from the assembled blocks: coarse block k
-// (aggregates cbA[k], cbB[k] of one mesh) is the sum, in ascending order, of
-// the fine blocks H_ij with i in cbA[k] and j in cbB[k]; one thread per coarse
-// block writes its 3x3 entries into the mesh's dense row-major matrix (zeroed
-// beforehand), so the operator is the same on every run.
-__global__ __launch_bounds__(256) void CoarseFromBsr(int const* __restrict__ cbA, int const* __restrict__ cbB,
-                                                     int const* __restrict__ cbSrcPtr,
-                                                     int const* __restrict__ cbSrc, int nCoarseBlocks,
-                                                     float4 const* __restrict__ va, float4 const* __restrict__ vb,
-                                                     float const* __restrict__ vc, CoarseArgs c,
-                                                     double* __restrict__ Ac)
+// Rotation offsets of the coarse space, one workgroup of 128 per aggregate g
+// (rows aggRows[aggPtr[g] .. aggPtr[g + 1]), positions xs of their vertices):
+// c the centroid, s the RMS radius, coarseD[row] = ((x - c) / s, 1), or 0 when
+// the aggregate has no extent or its vertices are nearly collinear (the sum of
+// the 2x2 principal minors of sum (x - c)(x - c)^T below 1e-3 of its trace
+// squared). All sums in a fixed order.
+__global__ __launch_bounds__(128) void CoarseBasis(int const* __restrict__ aggPtr, int const* __restrict__ aggRows,
+                                                   int const* __restrict__ rowVertex,
+                                                   float4 const* __restrict__ xs, float4* __restrict__ coarseD)
 {
-    int const k = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (k >= nCoarseBlocks)
-        return;
-    double s[9] = {0., 0., 0., 0., 0., 0., 0., 0., 0.};
-    for (int q = cbSrcPtr[k]; q < cbSrcPtr[k + 1]; ++q)
+    __shared__ double lds3[2][3];
+    __shared__ double lds6[2][6];
+    int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
+    int const kb = aggPtr[g], ke = aggPtr[g + 1];
+    double c[3] = {0., 0., 0.};
+    for (int k = kb + t; k < ke; k += 128)
     {
-        int const f = cbSrc[q];
-        float4 const A = va[f], B = vb[f];
-        s[0] += A.x, s[1] += A.y, s[2] += A.z;
-        s[3] += A.w, s[4] += B.x, s[5] += B.y;
-        s[6] += B.z, s[7] += B.w, s[8] += vc[f];
+        float4 const x = xs[rowVertex[aggRows[k]]];
+        c[0] += x.x, c[1] += x.y, c[2] += x.z;
     }
-    int const ga = cbA[k], gb = cbB[k];
-    int const m = c.aggMesh[ga];
-    int const a0 = c.meshAggBegin[m];
-    int const n = 3 * (c.meshAggBegin[m + 1] - a0);
-    double* const A = Ac + c.meshOffset[m];
-    int const ra = 3 * (ga - a0), cb = 3 * (gb - a0);
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j)
-            A[static_cast<size_t>(ra + i) * n + cb + j] = s[3 * i + j];
+    Sum128<3>(c, lds3);
+    double const n = static_cast<double>(ke - kb);
+    for (double& v : c)
+        v /= n;
+    double S[6] = {0., 0., 0., 0., 0., 0.}; // xx, yy, zz, xy, xz, yz
+    for (int k = kb + t; k < ke; k += 128)
+    {
+        float4 const x = xs[rowVertex[aggRows[k]]];
+        double const dx = x.x - c[0], dy = x.y - c[1], dz = x.z - c[2];
+        S[0] += dx * dx, S[1] += dy * dy, S[2] += dz * dz;
+        S[3] += dx * dy, S[4] += dx * dz, S[5] += dy * dz;
+    }
+    Sum128<6>(S, lds6);
+    double const tr = S[0] + S[1] + S[2];
+    double const minors = S[0] * S[1] - S[3] * S[3] + S[0] * S[2] - S[4] * S[4] + S[1] * S[2] - S[5] * S[5];
+    bool const use = tr > 0. && minors >= 1e-3 * tr * tr;
+    double const inv = use ? 1. / sqrt(tr / n) : 0.;
+    for (int k = kb + t; k < ke; k += 128)
+    {
+        int const row = aggRows[k];
+        float4 const x = xs[rowVertex[row]];
+        coarseD[row] = use ? make_float4(static_cast<float>((x.x - c[0]) * inv), static_cast<float>((x.y - c[1]) * inv),
+                                         static_cast<float>((x.z - c[2]) * inv), 1.f)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+}
+
+// Galerkin coarse operator Z^T H Z from the assembled blocks: coarse block k
+// (aggregates cbA[k], cbB[k] of one mesh) is the sum over the fine blocks H_ij
+// with i in cbA[k] and j in cbB[k] of Z_i^T H_ij Z_j, with D = [d]_x:
+//   [ H        -H D_j      ]
+//   [ D_i H    -D_i H D_j  ]
+// One wave per coarse block: lane l takes the group's fine blocks l, l + 64, ...
+// in ascending order, then a fixed butterfly; the wave writes the 6x6 entries
+// into the mesh's dense row-major matrix (zeroed beforehand), so the operator
+// is the same on every run. An aggregate without rotations (d = 0) gets an
+// identity rotation diagonal, decoupled from everything else.
+struct CoarseFromBsrArgs
+{
+    int const* __restrict__ cbA;
+    int const* __restrict__ cbB;
+    int const* __restrict__ cbSrcPtr;
+    int const* __restrict__ cbSrc;
+    int nCoarseBlocks;
+    float4 const* __restrict__ va;
+    float4 const* __restrict__ vb;
+    float const* __restrict__ vc;
+    int const* __restrict__ blockRow;
+    int const* __restrict__ col;
+    float4 const* __restrict__ coarseD;
+    CoarseArgs c;
+    double* __restrict__ Ac;
+};
+
+__device__ __forceinline__ void Skew(float4 const& d, double (&D)[3][3])
+{
+    D[0][0] = 0., D[0][1] = -d.z, D[0][2] = d.y;
+    D[1][0] = d.z, D[1][1] = 0., D[1][2] = -d.x;
+    D[2][0] = -d.y, D[2][1] = d.x, D[2][2] = 0.;
+}
+
+__global__ __launch_bounds__(256) void CoarseFromBsr(CoarseFromBsrArgs a)
+{
+    constexpr int kN = kCoarseDofs;
+    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
+    int const k = static_cast<int>(blockIdx.x) * 4 + wave;
+    if (k >= a.nCoarseBlocks)
+        return;
+    double s[kN * kN];
+#pragma unroll
+    for (int e = 0; e < kN * kN; ++e)
+        s[e] = 0.;
+    int const q0 = a.cbSrcPtr[k], q1 = a.cbSrcPtr[k + 1];
+    for (int q = q0 + lane; q < q1; q += 64)
+    {
+        int const f = a.cbSrc[q];
+        float4 const A = a.va[f], B = a.vb[f];
+        float const C = a.vc[f];
+        float4 const di = a.coarseD[a.blockRow[f]], dj = a.coarseD[a.col[f]];
+        double const H[3][3] = {{A.x, A.y, A.z}, {A.w, B.x, B.y}, {B.z, B.w, C}};
+        double Di[3][3], Dj[3][3], HD[3][3], DH[3][3];
+        Skew(di, Di);
+        Skew(dj, Dj);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+            {
+                HD[r][cc] = H[r][0] * Dj[0][cc] + H[r][1] * Dj[1][cc] + H[r][2] * Dj[2][cc];
+                DH[r][cc] = Di[r][0] * H[0][cc] + Di[r][1] * H[1][cc] + Di[r][2] * H[2][cc];
+            }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc)
+            {
+                s[r * kN + cc] += H[r][cc];
+                s[r * kN + 3 + cc] -= HD[r][cc];
+                s[(3 + r) * kN + cc] += DH[r][cc];
+                s[(3 + r) * kN + 3 + cc] -= Di[r][0] * HD[0][cc] + Di[r][1] * HD[1][cc] + Di[r][2] * HD[2][cc];
+            }
+    }
+#pragma unroll
+    for (int e = 0; e < kN * kN; ++e)
+        for (int off = 32; off > 0; off >>= 1)
+            s[e] += __shfl_xor(s[e], off);
+    if (lane != 0)
+        return;
+    int const ga = a.cbA[k], gb = a.cbB[k];
+    if (ga == gb && a.coarseD[a.blockRow[a.cbSrc[q0]]].w == 0.f)
+#pragma unroll
+        for (int r = 3; r < kN; ++r)
+#pragma unroll
+            for (int cc = 3; cc < kN; ++cc)
+                s[r * kN + cc] = r == cc ? 1. : 0.;
+    int const m = a.c.aggMesh[ga];
+    int const a0 = a.c.meshAggBegin[m];
+    int const n = kN * (a.c.meshAggBegin[m + 1] - a0);
+    double* const Am = a.Ac + a.c.meshOffset[m];
+    int const ra = kN * (ga - a0), cb = kN * (gb - a0);
+    for (int r = 0; r < kN; ++r)
+        for (int cc = 0; cc < kN; ++cc)
+            Am[static_cast<size_t>(ra + r) * n + cb + cc] = s[r * kN + cc];
 }
 
 } // namespace pbat_vbd::device

@@ -553,8 +553,9 @@ void DownloadVec(pbat_vbd_integrator& h, float4 const* src, float* dst)
 
 // Aggregates of the adjoint's two-level preconditioner: each mesh's updated
 // vertices binned by rest position into cubes of four mean edge lengths
-// (enlarged so that a mesh has at most kMaxAggregatesPerMesh of them).
-constexpr int kMaxAggregatesPerMesh = 1024;
+// (enlarged so that the cubes spanning a mesh's box number about
+// kMaxAggregatesPerMesh: its dense coarse inverse has 6 rows per aggregate).
+constexpr int kMaxAggregatesPerMesh = 512;
 constexpr double kAggregateEdges = 4.;
 
 void BuildAggregates(pbat_vbd_integrator& h, pbat_vbd_data const* d, std::vector<int> const& meshOfOrig)
@@ -623,7 +624,7 @@ void BuildAggregates(pbat_vbd_integrator& h, pbat_vbd_data const* d, std::vector
             }
             aggOf[km[j].second] = next - 1;
         }
-        int64_t const nc = 3 * int64_t(next - h.aggBegin[m]);
+        int64_t const nc = device::kCoarseDofs * int64_t(next - h.aggBegin[m]);
         h.coarseOffset[m + 1] = h.coarseOffset[m] + nc * nc;
     }
     h.aggBegin[nM] = next;
