@@ -161,6 +161,9 @@ struct RolloutState
     DeviceArray<float> coarseF;                               // the inverse, symmetrised, FP32
     DeviceArray<float4> coarseD;                              // per solver row: rotation offset of the coarse space
     DeviceArray<double> pivotRows;                            // pivot rows of a Gauss-Jordan step, per mesh
+    DeviceArray<int4> coarseInfo;                             // per aggregate: (b0, n, rows begin, rows end)
+    DeviceArray<long long> coarseRowOffset;                   // per aggregate: its first row in coarseF
+    int coarseMaxN = 0;                                       // rows of the largest mesh's coarse operator
     DeviceArray<double> rc, zc;                               // coarse residual and correction
     DeviceArray<double> scal;                                 // per-mesh solver scalars (kScalarCount x nMeshes)
     DeviceArray<int> flags;                                   // per-mesh done flags, iteration counts, live count

@@ -220,6 +220,7 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
         maxN = std::max(maxN, n);
         finalBuf[m] = ((n + device::kPivot - 1) / device::kPivot) % 2; // parity of its step count
     }
+    R.coarseMaxN = maxN;
     int64_t const maxEntries = int64_t(maxN) * maxN;
     dim3 const grid(static_cast<unsigned>((maxEntries + 255) / 256), static_cast<unsigned>(nM));
     // block Gauss-Jordan: per step the pivot rows, then the rest in 64 x 64 tiles
@@ -240,6 +241,36 @@ void BuildCoarse(pbat_vbd_integrator& h, float4 const* xs, float dt)
         R.coarseFinal.allocate(finalBuf.size());
     PBAT_HIP(hipMemcpyAsync(R.coarseFinal.ptr, finalBuf.data(), finalBuf.size() * sizeof(int),
                             hipMemcpyHostToDevice, h.stream));
+    // per aggregate: its mesh's coarse rows (b0, n), its fine rows' range and
+    // the offset of its first row of the inverse (allocated once per size: the
+    // captured CG chunk graph holds these pointers)
+    {
+        std::vector<int4> info(static_cast<size_t>(h.nAggregates));
+        std::vector<long long> rowOff(static_cast<size_t>(h.nAggregates));
+        std::vector<int> aggPtr(static_cast<size_t>(h.nAggregates) + 1);
+        PBAT_HIP(hipMemcpy(aggPtr.data(), h.aggVertPtr.ptr, aggPtr.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (int m = 0; m < nM; ++m)
+        {
+            int const b0 = device::kCoarseDofs * h.aggBegin[m];
+            int const n = device::kCoarseDofs * (h.aggBegin[m + 1] - h.aggBegin[m]);
+            for (int g = h.aggBegin[m]; g < h.aggBegin[m + 1]; ++g)
+            {
+                info[g] = make_int4(b0, n, aggPtr[g], aggPtr[g + 1]);
+                rowOff[g] = static_cast<long long>(h.coarseOffset[m]) +
+                            static_cast<long long>(device::kCoarseDofs * g - b0) * n;
+            }
+        }
+        if (R.coarseInfo.n != info.size())
+        {
+            R.coarseInfo.allocate(info.size());
+            R.coarseRowOffset.allocate(rowOff.size());
+        }
+        PBAT_HIP(hipMemcpyAsync(R.coarseInfo.ptr, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice,
+                                h.stream));
+        PBAT_HIP(hipMemcpyAsync(R.coarseRowOffset.ptr, rowOff.data(), rowOff.size() * sizeof(long long),
+                                hipMemcpyHostToDevice, h.stream));
+        PBAT_HIP(hipStreamSynchronize(h.stream)); // host temporaries
+    }
     PBAT_HIP(hipStreamSynchronize(h.stream)); // finalBuf is a host temporary
     hipLaunchKernelGGL(device::CoarseGuard, dim3(nM), dim3(256), 0, h.stream, R.coarse[0].ptr, R.coarse[1].ptr,
                        static_cast<int const*>(R.coarseFinal.ptr), c);
@@ -1041,7 +1072,6 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         PBAT_HIP(hipGetLastError());
         return out;
     }
-    device::CoarseArgs const cargs = MakeCoarseArgs(h);
     auto aggUpdate = [&](int init, float4 const* p) {
         device::AggUpdateArgs u{};
         u.aggPtr = h.aggVertPtr.ptr, u.aggVerts = R.aggRows.ptr, u.aggMesh = h.aggMesh.ptr;
@@ -1056,11 +1086,27 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             hipLaunchKernelGGL(device::AggUpdate, dim3(h.nAggregates), dim3(128), 0, h.stream, u);
     };
     auto coarseRows = [&](int first) {
-        hipLaunchKernelGGL(device::CoarseApplyRows, dim3(h.nAggregates), dim3(64 * device::kCoarseWaves), 0, h.stream,
-                           static_cast<float const*>(R.coarseF.ptr), static_cast<double const*>(R.rc.ptr), R.zc.ptr,
-                           cargs, cg, R.rczcPart.ptr, static_cast<int const*>(h.aggVertPtr.ptr),
-                           static_cast<int const*>(R.aggRows.ptr), static_cast<float4 const*>(R.coarseD.ptr), R.z.ptr,
-                           first);
+        device::CoarseApplyArgs ca{};
+        ca.coarseF = R.coarseF.ptr, ca.rc = R.rc.ptr, ca.zc = R.zc.ptr, ca.rczcPart = R.rczcPart.ptr;
+        ca.info = R.coarseInfo.ptr, ca.rowOffset = R.coarseRowOffset.ptr, ca.aggDone = R.aggDone.ptr;
+        ca.aggRows = R.aggRows.ptr, ca.coarseD = R.coarseD.ptr, ca.z = R.z.ptr;
+        // 64-column chunks per wave of the largest mesh, rounded up to a kernel's
+        int const perWave = ((R.coarseMaxN + 63) / 64 + device::kCoarseWaves - 1) / device::kCoarseWaves;
+        dim3 const grid(h.nAggregates), block(64 * device::kCoarseWaves);
+        if (perWave <= 1)
+            hipLaunchKernelGGL(device::CoarseApplyRows<1>, grid, block, 0, h.stream, ca, first);
+        else if (perWave <= 2)
+            hipLaunchKernelGGL(device::CoarseApplyRows<2>, grid, block, 0, h.stream, ca, first);
+        else if (perWave <= 3)
+            hipLaunchKernelGGL(device::CoarseApplyRows<3>, grid, block, 0, h.stream, ca, first);
+        else if (perWave <= 4)
+            hipLaunchKernelGGL(device::CoarseApplyRows<4>, grid, block, 0, h.stream, ca, first);
+        else if (perWave <= 6)
+            hipLaunchKernelGGL(device::CoarseApplyRows<6>, grid, block, 0, h.stream, ca, first);
+        else if (perWave <= 9)
+            hipLaunchKernelGGL(device::CoarseApplyRows<9>, grid, block, 0, h.stream, ca, first);
+        else // several passes beyond 3,072 rows
+            hipLaunchKernelGGL(device::CoarseApplyRows<12>, grid, block, 0, h.stream, ca, first);
     };
     auto finish = [&](int first) {
         device::CgFinishArgs f{};
@@ -1119,7 +1165,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
     if (graph)
     {
         std::vector<double> const key{double(dt),        double(cfg.solver), double(check),       cg.tol2,
-                                      double(BsrLanes(h)), double(BsrPerLane(h)), double(h.xcdRemap)};
+                                      double(BsrLanes(h)), double(BsrPerLane(h)), double(h.xcdRemap),
+                                      double(R.coarseMaxN)};
         // every device address the captured kernels take: a buffer reallocated at
         // another address forces a new capture
         std::vector<void const*> const ptrs{
@@ -1130,7 +1177,8 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
             h.aggVerts.ptr,    h.aggMesh.ptr,   h.meshAggBegin.ptr, R.bsrChunks.ptr, R.aggChunks.ptr,
             R.pqPart.ptr,      R.rzPart.ptr,    R.rrPart.ptr,  R.bbPart.ptr,     R.rczcPart.ptr,
             R.doneDev,       R.aggFirst.ptr, R.aggDone.ptr,    R.yS.ptr,        R.bS.ptr,
-            R.aggRows.ptr,     R.aggOfRow.ptr,  R.meshOfRow.ptr, R.coarseD.ptr};
+            R.aggRows.ptr,     R.aggOfRow.ptr,  R.meshOfRow.ptr, R.coarseD.ptr, R.coarseInfo.ptr,
+            R.coarseRowOffset.ptr};
         if (!R.cgExec || R.cgKey != key || R.cgPtrs != ptrs)
         {
             R.ResetGraph();

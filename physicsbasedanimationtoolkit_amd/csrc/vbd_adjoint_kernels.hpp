@@ -662,8 +662,8 @@ constexpr int kPivot = 32;
 
 // P inverted in LDS (SPD, no pivoting, Gauss-Jordan one entry per thread step)
 // by every workgroup, then W[:, j] for its 64 columns, 4 threads per column
-// each forming 8 of its b entries. W holds kPivot rows of wStride doubles per
-// mesh.
+// each forming kPivot / 4 of its b entries. W holds kPivot rows of wStride
+// doubles per mesh.
 __global__ __launch_bounds__(256) void CoarsePivotRows(double const* __restrict__ Ak, double* __restrict__ W,
                                                        int k, int wStride, CoarseArgs c)
 {
@@ -720,18 +720,21 @@ __global__ __launch_bounds__(256) void CoarsePivotRows(double const* __restrict_
             Wm[static_cast<size_t>(l) * wStride + j] = P[l][j - k];
         return;
     }
-    double a[kPivot];
+    double w[kPivot / 4];
 #pragma unroll
-    for (int q = 0; q < kPivot; ++q)
-        a[q] = q < b ? A[static_cast<size_t>(k + q) * n + j] : 0.;
-    for (int l = l0; l < l0 + kPivot / 4 && l < b; ++l)
+    for (int l = 0; l < kPivot / 4; ++l)
+        w[l] = 0.;
+    for (int q = 0; q < b; ++q)
     {
-        double w = 0.;
+        double const aq = A[static_cast<size_t>(k + q) * n + j];
 #pragma unroll
-        for (int q = 0; q < kPivot; ++q)
-            w += P[l][q] * a[q];
-        Wm[static_cast<size_t>(l) * wStride + j] = w;
+        for (int l = 0; l < kPivot / 4; ++l)
+            w[l] += P[l0 + l][q] * aq;
     }
+#pragma unroll
+    for (int l = 0; l < kPivot / 4; ++l)
+        if (l0 + l < b)
+            Wm[static_cast<size_t>(l0 + l) * wStride + j] = w[l];
 }
 
 // The other rows of step k, one 64 x 64 tile of (i, j) per workgroup

@@ -759,32 +759,54 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 // waves per aggregate g computes its kCoarseDofs rows: wave w takes the
 // 64-column chunks w, w + kCoarseWaves, ... of its mesh's n columns, loads rc
 // and the rows' entries of up to kCoarseChunksPerPass of them at once, then
-// accumulates; the waves' sums are added in order. The order depends on n
-// only, so a mesh gets the same zc alone or in a batch.
-constexpr int kCoarseWaves = 8;
-constexpr int kCoarseChunksPerPass = 6; // n <= 3,072 in one pass
+// accumulates (kU chunks per wave per pass, chosen for the largest mesh; the
+// order of the terms does not depend on it); the waves' sums are added in
+// order. The order depends on n only, so a mesh gets the same zc alone or in
+// a batch. The aggregate's
+// descriptor (its mesh's first coarse row b0 and size n, its rows' range) and
+// the offset of its first inverse row are precomputed, and the rows' z and d
+// are loaded before the product, so the kernel waits on memory about twice.
+constexpr int kCoarseWaves = 4;
 
-__global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(float const* __restrict__ coarseF,
-                                                                     double const* __restrict__ rc,
-                                                                     double* __restrict__ zc, CoarseArgs c,
-                                                                     CgState cg, double* __restrict__ rczcPart,
-                                                                     int const* __restrict__ aggPtr,
-                                                                     int const* __restrict__ aggRows,
-                                                                     float4 const* __restrict__ coarseD,
-                                                                     float4* __restrict__ z, int first)
+struct CoarseApplyArgs
 {
-    constexpr int kR = kCoarseDofs, kU = kCoarseChunksPerPass, kW = kCoarseWaves;
+    float const* __restrict__ coarseF;
+    double const* __restrict__ rc;
+    double* __restrict__ zc;
+    double* __restrict__ rczcPart;
+    int4 const* __restrict__ info;            // per aggregate: (b0, n, first row, end row)
+    long long const* __restrict__ rowOffset;  // per aggregate: its first row in coarseF
+    int const* __restrict__ aggDone;          // per aggregate: its mesh's done flag
+    int const* __restrict__ aggRows;
+    float4 const* __restrict__ coarseD;
+    float4* __restrict__ z;
+};
+
+template <int kU>
+__global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(CoarseApplyArgs a, int first)
+{
+    constexpr int kR = kCoarseDofs, kW = kCoarseWaves;
     __shared__ double part[kW][kR];
     __shared__ float zcS[kR];
-    int const g = static_cast<int>(blockIdx.x);
-    int const wave = static_cast<int>(threadIdx.x >> 6), lane = static_cast<int>(threadIdx.x & 63);
-    int const m = c.aggMesh[g];
-    if (!first && cg.F[kDone * cg.nM + m] != 0)
+    int const g = static_cast<int>(blockIdx.x), t = static_cast<int>(threadIdx.x);
+    int const wave = t >> 6, lane = t & 63;
+    int4 const inf = a.info[g];
+    long long const off = a.rowOffset[g];
+    int const done = first ? 0 : a.aggDone[g];
+    if (done != 0)
         return; // the mesh has converged
-    int const b0 = kR * c.meshAggBegin[m];
-    int const n = kR * c.meshAggBegin[m + 1] - b0;
-    int const lr0 = kR * g - b0; // the aggregate's first row within the mesh
-    float const* A = coarseF + c.meshOffset[m] + static_cast<size_t>(lr0) * n;
+    int const b0 = inf.x, n = inf.y;
+    // the thread's first row of the aggregate, its d and z, in flight during the product
+    int const k0 = inf.z + t;
+    int i0 = -1;
+    float4 d0 = make_float4(0.f, 0.f, 0.f, 0.f), z0 = d0;
+    if (k0 < inf.w)
+    {
+        i0 = a.aggRows[k0];
+        d0 = a.coarseD[i0];
+        z0 = a.z[i0];
+    }
+    float const* A = a.coarseF + off;
     double s[kR];
 #pragma unroll
     for (int i = 0; i < kR; ++i)
@@ -792,17 +814,17 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(float const
     for (int c0 = wave; c0 * 64 < n; c0 += kW * kU)
     {
         double x[kU];
-        float a[kR][kU];
+        float v[kR][kU];
         // branch-free: out-of-range lanes load column n - 1 and drop it, so
         // every load of the pass is in flight together
 #pragma unroll
         for (int u = 0; u < kU; ++u)
         {
             int const j = min((c0 + kW * u) * 64 + lane, n - 1);
-            x[u] = rc[b0 + j];
+            x[u] = a.rc[b0 + j];
 #pragma unroll
             for (int i = 0; i < kR; ++i)
-                a[i][u] = A[static_cast<size_t>(i) * n + j];
+                v[i][u] = A[static_cast<size_t>(i) * n + j];
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u)
@@ -812,38 +834,41 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(float const
         for (int i = 0; i < kR; ++i)
 #pragma unroll
             for (int u = 0; u < kU; ++u)
-                s[i] += double(a[i][u]) * x[u];
+                s[i] += double(v[i][u]) * x[u];
     }
 #pragma unroll
     for (int i = 0; i < kR; ++i)
-        for (int off = 32; off > 0; off >>= 1)
-            s[i] += __shfl_xor(s[i], off);
+        for (int o = 32; o > 0; o >>= 1)
+            s[i] += __shfl_xor(s[i], o);
     if (lane == 0)
 #pragma unroll
         for (int i = 0; i < kR; ++i)
             part[wave][i] = s[i];
     __syncthreads();
-    if (threadIdx.x < kR)
+    if (t < kR)
     {
-        int const i = static_cast<int>(threadIdx.x), row = b0 + lr0 + i;
-        double t = part[0][i];
+        int const row = kR * g + t;
+        double tot = part[0][t];
         for (int w = 1; w < kW; ++w)
-            t += part[w][i];
-        zc[row] = t;
-        rczcPart[row] = t * rc[row];
-        zcS[i] = static_cast<float>(t);
+            tot += part[w][t];
+        a.zc[row] = tot;
+        a.rczcPart[row] = tot * a.rc[row];
+        zcS[t] = static_cast<float>(tot);
     }
     __syncthreads();
     float const tx = zcS[0], ty = zcS[1], tz = zcS[2], wx = zcS[3], wy = zcS[4], wz = zcS[5];
-    for (int k = aggPtr[g] + static_cast<int>(threadIdx.x); k < aggPtr[g + 1]; k += 64 * kW)
-    {
-        int const i = aggRows[k];
-        float4 const d = coarseD[i];
-        float4 zi = z[i];
+    auto correct = [&](int i, float4 const& d, float4 zi) {
         zi.x += tx + (wy * d.z - wz * d.y);
         zi.y += ty + (wz * d.x - wx * d.z);
         zi.z += tz + (wx * d.y - wy * d.x);
-        z[i] = zi;
+        a.z[i] = zi;
+    };
+    if (i0 >= 0)
+        correct(i0, d0, z0);
+    for (int k = k0 + 64 * kW; k < inf.w; k += 64 * kW)
+    {
+        int const i = a.aggRows[k];
+        correct(i, a.coarseD[i], a.z[i]);
     }
 }
 
