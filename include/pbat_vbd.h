@@ -293,9 +293,10 @@ enum pbat_vbd_adjoint_solver {
     PBAT_VBD_ADJOINT_PCG = 0,          /* CG preconditioned by symmetric coloured block Gauss-Seidel */
     PBAT_VBD_ADJOINT_GAUSS_SEIDEL = 1, /* coloured block Gauss-Seidel sweeps */
     PBAT_VBD_ADJOINT_PCG_JACOBI = 2,   /* CG preconditioned by the inverse 3x3 diagonal blocks */
-    PBAT_VBD_ADJOINT_PCG_TWO_LEVEL = 3 /* block Jacobi plus a coarse correction over translations of
-                                          vertex aggregates (coarse operator from the rollout's last
-                                          state, inverted once per backward pass) */
+    PBAT_VBD_ADJOINT_PCG_TWO_LEVEL = 3 /* block Jacobi plus a coarse correction over the rigid motions
+                                          (translations, rotations) of vertex aggregates (coarse
+                                          operator from the rollout's last state, inverted once per
+                                          backward pass) */
 };
 
 /* Each mesh of a batch is solved with its own CG scalars and stopping test
