@@ -4,6 +4,7 @@
 // both use. Not part of the C ABI.
 #pragma once
 
+#include "coarse_space.hpp"
 #include "pbat_vbd.h"
 #include "status.hpp"
 #include "vbd_kernels.hpp"

@@ -6,6 +6,7 @@
 // arrays in internal vertex order with zeros on Dirichlet vertices.
 #pragma once
 
+#include "coarse_space.hpp"
 #include "vbd_kernels.hpp"
 
 namespace pbat_vbd::device {

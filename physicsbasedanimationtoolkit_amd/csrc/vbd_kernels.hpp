@@ -24,10 +24,6 @@
 
 namespace pbat_vbd::device {
 
-// Coarse degrees of freedom per vertex aggregate in the adjoint's two-level
-// preconditioner: three translations and three infinitesimal rotations.
-constexpr int kCoarseDofs = 6;
-
 // ---- cross-lane sums over groups of L consecutive lanes (L | 64) ----------
 template <int kCtrl>
 __device__ __forceinline__ float DppRead(float x)
