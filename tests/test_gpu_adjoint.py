@@ -175,6 +175,90 @@ def test_adjoint_solve_long_rows_matches_fp64(gpu_device, n_surface, solver):
     assert e <= 1e-4
 
 
+def cube_with_isolated_vertices():
+    """A jittered 6^3 cube (bottom fixed) plus two small tet groups far from it:
+    one free vertex alone (a one-vertex aggregate) and two free vertices 0.06 apart
+    (one aggregate of two, i.e. collinear, or two of one). Such aggregates have no
+    rotation modes: their coarse rows keep translations only."""
+    X, E = meshes.cube_tet_grid(6, jitter=0.1, seed=5)
+    dbc = meshes.bottom_vertices(X)
+    extra = np.array([[10.0, 0.0, 0.0], [10.5, 0.0, 0.0], [10.0, 0.5, 0.0],   # fixed
+                      [10.1, 0.1, 0.4], [10.15, 0.12, 0.42],                 # free pair
+                      [20.0, 0.0, 0.0], [20.5, 0.0, 0.0], [20.0, 0.5, 0.0],   # fixed
+                      [20.1, 0.1, 0.4]]).T                                   # free, alone
+    n0 = X.shape[1]
+    tets = np.array([[0, 1, 2, 3], [0, 1, 3, 4], [5, 6, 7, 8]]) + n0
+    X = np.concatenate([X, extra], axis=1)
+    for t in tets:
+        a, b, c, d = X[:, t].T
+        if np.dot(np.cross(b - a, c - a), d - a) < 0:
+            t[2], t[3] = t[3], t[2]
+    E = np.concatenate([E, tets.T], axis=1)
+    dbc = np.concatenate([dbc, n0 + np.array([0, 1, 2, 5, 6, 7])]).astype(np.int64)
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(meshes.log_uniform(np.random.default_rng(6), 1e5, 1e7, nT), 0.45)
+    return X, E, mu, lam, dbc
+
+
+def test_adjoint_solve_with_rotation_free_aggregates_matches_fp64(gpu_device):
+    """The two-level solve on a mesh whose far-away free vertices form aggregates
+    without rotation modes (identity rotation diagonal in the coarse operator)
+    converges to the FP64 solution."""
+    X, E, mu, lam, dbc = cube_with_isolated_vertices()
+    nT = E.shape[1]
+    v0 = np.zeros_like(X)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(np.full(nT, 1e3), mu, lam)
+            .with_acceleration(meshes.gravity(X.shape[1])).with_dirichlet_vertices(dbc)
+            .with_velocity(v0).construct())
+    g = pbat.gpu.vbd.Integrator(data)
+    o = O.OracleVbd(X, E, v=v0, aext=meshes.gravity(X.shape[1]), rhoe=np.full(nT, 1e3), mue=mu, lambdae=lam,
+                    dbc=dbc, nthreads=8)
+    x = deformed(X, amp=0.002)
+    b = np.random.default_rng(3).normal(size=X.shape)
+    y, st = g.adjoint_solve(x, DT, b, solver="pcg_two_level", max_iterations=600, tolerance=1e-6,
+                            check_every=10)
+    H, idx = A.hessian(o, x.T, DT)
+    yr = np.zeros(3 * o.nV)
+    yr[idx] = A.spla.spsolve(H, b.T.reshape(-1)[idx])
+    yr = yr.reshape(o.nV, 3).T
+    e = rel(y, yr)
+    print(f"cube + isolated vertices: {st} rel err {e:.3e}")
+    assert st["relative_residual"] <= 1e-5
+    assert e <= 1e-4
+
+
+def test_adjoint_solve_with_more_than_512_aggregates(gpu_device):
+    """A 48^3 cube: the cubes of 1/8 its side leave a ninth layer of face, edge
+    and corner vertices on the upper faces, 7 x 7 x 8 + those aggregates (more
+    than 512, so the coarse product takes several passes over its 64-column
+    chunks), the thin ones planar, collinear (no rotation modes) or a single
+    vertex. The two-level solution satisfies H y = b with H applied by the
+    matrix-free product and agrees with the block-Jacobi solution."""
+    X, E = meshes.cube_tet_grid(48, jitter=0.1, seed=7)
+    nT = E.shape[1]
+    mu, lam = meshes.lame_coefficients(meshes.log_uniform(np.random.default_rng(8), 1e5, 1e7, nT), 0.45)
+    dbc = meshes.bottom_vertices(X)
+    v0 = np.zeros_like(X)
+    data = (pbat.sim.vbd.Data().with_volume_mesh(X, E).with_material(np.full(nT, 1e3), mu, lam)
+            .with_acceleration(meshes.gravity(X.shape[1])).with_dirichlet_vertices(dbc)
+            .with_velocity(v0).construct())
+    g = pbat.gpu.vbd.Integrator(data)
+    # a perturbation small against the cell (1/48), where H stays positive definite
+    x = deformed(X, amp=0.0002)
+    b = np.random.default_rng(9).normal(size=X.shape)
+    b[:, dbc] = 0.0
+    y2, st2 = g.adjoint_solve(x, DT, b, solver="pcg_two_level", max_iterations=2000, tolerance=1e-6,
+                              check_every=10)
+    yj, stj = g.adjoint_solve(x, DT, b, solver="pcg_jacobi", max_iterations=4000, tolerance=1e-6,
+                              check_every=10)
+    res = rel(g.hessian_vector(x, DT, y2), b)
+    print(f"48^3: two-level {st2}, block Jacobi {stj}, |Hy - b|/|b| {res:.3e}, vs Jacobi {rel(y2, yj):.3e}")
+    assert st2["relative_residual"] <= 1e-6 and stj["relative_residual"] <= 1e-6
+    assert st2["iterations"] < stj["iterations"]
+    assert res <= 1e-4
+    assert rel(y2, yj) <= 1e-4
+
+
 def gpu_trajectory(g, x0, v0, steps, iterations):
     """Positions x_0..x_T of stepping g one step at a time (device forward)."""
     g.x = x0
