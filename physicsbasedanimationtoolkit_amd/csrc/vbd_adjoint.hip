@@ -1089,7 +1089,7 @@ SolveResult Solve(pbat_vbd_integrator& h, float4 const* xs, float dt, float4* y,
         device::CoarseApplyArgs ca{};
         ca.coarseF = R.coarseF.ptr, ca.rc = R.rc.ptr, ca.zc = R.zc.ptr, ca.rczcPart = R.rczcPart.ptr;
         ca.info = R.coarseInfo.ptr, ca.rowOffset = R.coarseRowOffset.ptr, ca.aggDone = R.aggDone.ptr;
-        ca.aggRows = R.aggRows.ptr, ca.coarseD = R.coarseD.ptr, ca.z = R.z.ptr;
+        ca.coarseD = R.coarseD.ptr, ca.z = R.z.ptr;
         // 64-column chunks per wave of the largest mesh, rounded up to a kernel's
         int const perWave = ((R.coarseMaxN + 63) / 64 + device::kCoarseWaves - 1) / device::kCoarseWaves;
         dim3 const grid(h.nAggregates), block(64 * device::kCoarseWaves);

@@ -766,6 +766,8 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 // descriptor (its mesh's first coarse row b0 and size n, its rows' range) and
 // the offset of its first inverse row are precomputed, and the rows' z and d
 // are loaded before the product, so the kernel waits on memory about twice.
+// The fused solvers number the rows aggregate-major: aggregate g's rows are
+// the range info.z .. info.w itself.
 constexpr int kCoarseWaves = 4;
 
 struct CoarseApplyArgs
@@ -777,7 +779,6 @@ struct CoarseApplyArgs
     int4 const* __restrict__ info;            // per aggregate: (b0, n, first row, end row)
     long long const* __restrict__ rowOffset;  // per aggregate: its first row in coarseF
     int const* __restrict__ aggDone;          // per aggregate: its mesh's done flag
-    int const* __restrict__ aggRows;
     float4 const* __restrict__ coarseD;
     float4* __restrict__ z;
 };
@@ -802,7 +803,7 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(CoarseApply
     float4 d0 = make_float4(0.f, 0.f, 0.f, 0.f), z0 = d0;
     if (k0 < inf.w)
     {
-        i0 = a.aggRows[k0];
+        i0 = k0;
         d0 = a.coarseD[i0];
         z0 = a.z[i0];
     }
@@ -865,11 +866,8 @@ __global__ __launch_bounds__(64 * kCoarseWaves) void CoarseApplyRows(CoarseApply
     };
     if (i0 >= 0)
         correct(i0, d0, z0);
-    for (int k = k0 + 64 * kW; k < inf.w; k += 64 * kW)
-    {
-        int const i = a.aggRows[k];
+    for (int i = k0 + 64 * kW; i < inf.w; i += 64 * kW)
         correct(i, a.coarseD[i], a.z[i]);
-    }
 }
 
 // A solve's statistics without a host round trip: out = (largest per-mesh
