@@ -952,12 +952,13 @@ SolveResult ReadResult(pbat_vbd_integrator& h)
 // mesh has converged.
 //
 // Block Jacobi and two-level preconditioners (the fused iteration), per iteration:
-//   (1) per row chunk: p = z + Z zc + beta p_old formed on the fly, q = H p,
-//       the chunk's p . q;
+//   (1) per row chunk: p = z + beta p_old formed on the fly, q = H p, the
+//       chunk's p . q;
 //   (2) one workgroup per aggregate: alpha = rz / pq from its mesh's chunk sums,
 //       y += alpha p, r -= alpha q, z = D^-1 r, the aggregate's r . z, r . r and
 //       restriction rc = Z^T r;
-//   (3) two-level: zc = (Z^T H Z)^-1 rc and each coarse row's rc . zc;
+//   (3) two-level, one workgroup per aggregate: its rows of zc = (Z^T H Z)^-1 rc,
+//       each coarse row's rc . zc, and z += Z zc on its rows;
 //   (4) one workgroup: per mesh, r . z = r . D^-1 r + rc . zc and r . r summed
 //       over its partials, stopping test, beta.
 // No atomics: every sum is taken in a fixed order, so a solve gives the same
