@@ -993,12 +993,10 @@ void ProjectInitialGuess(pbat_vbd_integrator& h, float4* y, float4 const* b, int
     a.part = R.projPart.ptr, a.coef = R.projCoef.ptr;
     a.k = nHist, a.n = n, a.nM = nM, a.nAgg = h.nAggregates;
     hipLaunchKernelGGL(device::ProjBasis, dim3(Blocks(n)), dim3(256), 0, h.stream, a);
-    for (int j = 0; j < nHist; ++j)
-    {
-        device::BsrArgs p{};
-        p.y = a.basis[j], p.out = a.hb[j];
-        BsrPass<BsrMode::kProduct>(h, p, 0, n);
-    }
+    hipLaunchKernelGGL(device::ProjProducts, dim3(Blocks(static_cast<int64_t>(n) * 16)), dim3(256), 0, h.stream, a,
+                       static_cast<int const*>(R.bsrRowPtr.ptr), static_cast<int const*>(R.bsrCol.ptr),
+                       static_cast<float4 const*>(R.bsrA.ptr), static_cast<float4 const*>(R.bsrB.ptr),
+                       static_cast<float const*>(R.bsrC.ptr));
     hipLaunchKernelGGL(device::ProjGram, dim3(h.nAggregates), dim3(128), 0, h.stream, a);
     hipLaunchKernelGGL(device::ProjSolve, dim3(nM), dim3(256), 0, h.stream, a);
     hipLaunchKernelGGL(device::ProjCombine, dim3(Blocks(n)), dim3(256), 0, h.stream, a);

@@ -941,6 +941,50 @@ struct ProjArgs
     int k, n, nM, nAgg;
 };
 
+// H b_j for the k basis vectors at once, 16 lanes per row: every block of the
+// row is read once and applied to the k vectors (one pass over the Hessian
+// instead of k); lane sums in block order, then the 16-lane butterfly.
+__global__ __launch_bounds__(256) void ProjProducts(ProjArgs a, int const* __restrict__ rowPtr,
+                                                    int const* __restrict__ col, float4 const* __restrict__ va,
+                                                    float4 const* __restrict__ vb, float const* __restrict__ vc)
+{
+    constexpr int L = 16;
+    int const tid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    int const row = tid / L, lane = tid & (L - 1);
+    int kb = 0, ke = 0;
+    if (row < a.n)
+        kb = rowPtr[row], ke = rowPtr[row + 1];
+    float hx[kProjMax], hy[kProjMax], hz[kProjMax];
+#pragma unroll
+    for (int v = 0; v < kProjMax; ++v)
+        hx[v] = hy[v] = hz[v] = 0.f;
+    for (int k = kb + lane; k < ke; k += L)
+    {
+        float4 const A = va[k], B = vb[k];
+        float const C = vc[k];
+        int const j = col[k];
+#pragma unroll
+        for (int v = 0; v < kProjMax; ++v)
+        {
+            if (v >= a.k)
+                break;
+            float4 const P = a.basis[v][j];
+            hx[v] += A.x * P.x + A.y * P.y + A.z * P.z;
+            hy[v] += A.w * P.x + B.x * P.y + B.y * P.z;
+            hz[v] += B.z * P.x + B.w * P.y + C * P.z;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < kProjMax; ++v)
+    {
+        if (v >= a.k)
+            break;
+        float const x = GroupSum<L>(hx[v]), y = GroupSum<L>(hy[v]), z = GroupSum<L>(hz[v]);
+        if (lane == 0 && row < a.n)
+            a.hb[v][row] = make_float4(x, y, z, 0.f);
+    }
+}
+
 __global__ __launch_bounds__(256) void ProjBasis(ProjArgs a)
 {
     int const i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
