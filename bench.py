@@ -198,7 +198,7 @@ def bench_adjoint(args, torch):
 
     s = meshes.differentiable_scene(args.adjoint_grid, seed=0)
     g = pbat.gpu.vbd.Integrator(_diff_data(pbat, s))
-    g.autotune(s.dt, sweeps=5)
+    g.autotune(s.dt, sweeps=40)  # 8 launches of ~4 us per sweep: a short sample picks noisily
     v0 = g.v.copy()
     dt, it, T, every = s.dt, s.iterations, s.steps, 1
     solver = _adjoint_solver(args)
@@ -258,7 +258,7 @@ def bench_batched_adjoint(args, torch):
     scenes = meshes.batched_differentiable_scenes(args.batch_adjoint_meshes, 8, 20, seed=0)
     datas = [_diff_data(pbat, s) for s in scenes]
     b = pbat.gpu.vbd.BatchIntegrator(datas)
-    b.autotune(scenes[0].dt, sweeps=3)
+    b.autotune(scenes[0].dt, sweeps=10)
     shape = b.sweep_config()
     dt, it, T, every = scenes[0].dt, scenes[0].iterations, args.batch_adjoint_steps, 10
     solver = _adjoint_solver(args)
