@@ -758,16 +758,15 @@ __global__ __launch_bounds__(128) void AggUpdate(AggUpdateArgs u)
 // aggregate's rows, so that z = D^-1 r + Z zc. One workgroup of kCoarseWaves
 // waves per aggregate g computes its kCoarseDofs rows: wave w takes the
 // 64-column chunks w, w + kCoarseWaves, ... of its mesh's n columns, loads rc
-// and the rows' entries of up to kCoarseChunksPerPass of them at once, then
-// accumulates (kU chunks per wave per pass, chosen for the largest mesh; the
-// order of the terms does not depend on it); the waves' sums are added in
-// order. The order depends on n only, so a mesh gets the same zc alone or in
-// a batch. The aggregate's
-// descriptor (its mesh's first coarse row b0 and size n, its rows' range) and
-// the offset of its first inverse row are precomputed, and the rows' z and d
-// are loaded before the product, so the kernel waits on memory about twice.
-// The fused solvers number the rows aggregate-major: aggregate g's rows are
-// the range info.z .. info.w itself.
+// and the rows' entries of kU of them at once, then accumulates (kU chosen for
+// the largest mesh; the order of the terms does not depend on it); the waves'
+// sums are added in order. The order depends on n only, so a mesh gets the
+// same zc alone or in a batch. The aggregate's descriptor (its mesh's first
+// coarse row b0 and size n, its rows' range) and the offset of its first
+// inverse row are precomputed, and the rows' z and d are loaded before the
+// product, so the kernel waits on memory about twice. The fused solvers number
+// the rows aggregate-major: aggregate g's rows are the range info.z .. info.w
+// itself.
 constexpr int kCoarseWaves = 4;
 
 struct CoarseApplyArgs
